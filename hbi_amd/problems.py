@@ -1,0 +1,346 @@
+"""Synthetic fault problems and their geometry, in the reference's conventions.
+
+Host-side logic only (numpy): mesh generators that follow the reference's own
+generators, friction-parameter fields for the benchmark configurations, and
+writers for the reference's ``.in`` / parameter-file / STL input formats.
+
+Reference conventions followed here:
+  * ``coordinate2dp``   main_LH.f90:1701-1718  (2dp / 2dvs)
+  * ``coordinate3ddip`` main_LH.f90:1781-1835  (3dph)
+  * STL facet reading   main_LH.f90:485-510 and ``evcalc`` main_LH.f90:1940-1990 (3dnt)
+  * ``rake = crake/180*pi`` main_LH.f90:518-576
+  * ``psi = a ln(2 V0/V sinh(tau/(a sigma)))`` main_LH.f90:1004
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, Optional
+
+import numpy as np
+
+PI = 4.0 * math.atan(1.0)
+RIGID = 32.04      # m_const.f90:6 (GPa)
+VS = 3.464         # m_const.f90:7 (km/s)
+POIS = 0.25        # m_const.f90:3
+# default vref of main_LH.f90:136 is a single-precision literal stored in real(8)
+VREF_DEFAULT = float(np.float32(1e-6))
+
+KIND_2DVS = 1
+KIND_OKADA_HS = 2     # 3dph / 3dhr
+KIND_OKADA_FS = 3     # 3dnr
+KIND_TRI_FS = 4       # 3dnt
+
+_GEO_FIELDS = ("xcol", "ycol", "zcol", "xel", "xer", "ang", "angd", "dsl", "dsd", "rake",
+               "xs1", "xs2", "xs3", "ys1", "ys2", "ys3", "zs1", "zs2", "zs3",
+               "ev11", "ev12", "ev13", "ev21", "ev22", "ev23", "ev31", "ev32", "ev33")
+
+
+@dataclass
+class Geometry:
+    """Element geometry in mesh order (the reference's ``st_bemv``)."""
+    kind: int
+    n: int
+    w: float = 0.0            # okada normalisation length (st_bemv%w = ds0)
+    vcomp: int = 0            # 0 shear ('s'), 1 normal ('n')
+    md: int = 0               # TD slip mode: 0 's', 1 'st', 2 'dp', 3 'open'
+    ds: Optional[np.ndarray] = None    # element areas (or lengths in 2-D)
+    ds0: float = 0.0
+    arrays: Dict[str, np.ndarray] = field(default_factory=dict)
+
+    def __getattr__(self, name):
+        arrays = self.__dict__.get("arrays", {})
+        if name in _GEO_FIELDS:
+            return arrays.get(name)
+        raise AttributeError(name)
+
+    def field_names(self):
+        return _GEO_FIELDS
+
+    def geo_matrix(self) -> np.ndarray:
+        """(n, 28) column-major table in the field order of ``_GEO_FIELDS``."""
+        g = np.zeros((28, self.n), dtype=np.float64)
+        for k, f in enumerate(_GEO_FIELDS):
+            a = self.arrays.get(f)
+            if a is not None:
+                g[k] = a
+        return np.asfortranarray(g.T)
+
+
+def _c(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def mesh_2dvs(n: int, ds0: float) -> Geometry:
+    """2-D antiplane fault in a half-space (coordinate2dp)."""
+    i = np.arange(1, n + 1, dtype=np.float64)
+    xel = (i - 1) * ds0
+    xer = i * ds0
+    xcol = 0.5 * (xel + xer)
+    z = np.zeros(n)
+    g = Geometry(kind=KIND_2DVS, n=n, ds=np.full(n, ds0), ds0=ds0,
+                 arrays=dict(xcol=_c(xcol), ycol=_c(z), zcol=_c(z.copy()), xel=_c(xel), xer=_c(xer)))
+    return g
+
+
+def mesh_3dph(imax: int, jmax: int, ds0: float, dipangle: float = 90.0, crake: float = 0.0,
+              fullspace: bool = False) -> Geometry:
+    """Planar rectangular fault in a half-space (coordinate3ddip)."""
+    cdip = math.cos(dipangle * PI / 180.0)
+    sdip = math.sin(dipangle * PI / 180.0)
+    yr = [0.0]
+    zr = [0.0]
+    for j in range(1, jmax + 1):
+        yr.append(yr[j - 1] - ds0 * cdip)
+        zr.append(zr[j - 1] - ds0 * sdip)
+    n = imax * jmax
+    xcol = np.empty(n)
+    ycol = np.empty(n)
+    zcol = np.empty(n)
+    angd = np.empty(n)
+    ycol_j = [0.5 * (yr[j - 1] + yr[j]) for j in range(1, jmax + 1)]
+    zcol_j = [0.5 * (zr[j - 1] + zr[j]) for j in range(1, jmax + 1)]
+    angd_j = [math.atan2(zr[j - 1] - zr[j], yr[j - 1] - yr[j]) for j in range(1, jmax + 1)]
+    k = 0
+    for i in range(1, imax + 1):
+        xc = (float(i - imax // 2) - 0.5) * ds0
+        xcol[k:k + jmax] = xc
+        ycol[k:k + jmax] = ycol_j
+        zcol[k:k + jmax] = zcol_j
+        angd[k:k + jmax] = angd_j
+        k += jmax
+    rake = np.full(n, crake / 180.0 * PI)
+    g = Geometry(kind=KIND_OKADA_FS if fullspace else KIND_OKADA_HS, n=n, w=ds0, ds=np.full(n, ds0 * ds0),
+                 ds0=ds0,
+                 arrays=dict(xcol=_c(xcol), ycol=_c(ycol), zcol=_c(zcol), ang=np.zeros(n), angd=_c(angd),
+                             dsl=np.full(n, ds0), dsd=np.full(n, ds0), rake=_c(rake)))
+    return g
+
+
+def evcalc(xs1, xs2, xs3, ys1, ys2, ys3, zs1, zs2, zs3):
+    """Local frames and areas of triangles (evcalc, main_LH.f90:1940-1990)."""
+    vba0, vba1, vba2 = xs2 - xs1, ys2 - ys1, zs2 - zs1
+    vca0, vca1, vca2 = xs3 - xs1, ys3 - ys1, zs3 - zs1
+    ev31 = vba1 * vca2 - vba2 * vca1
+    ev32 = vba2 * vca0 - vba0 * vca2
+    ev33 = vba0 * vca1 - vba1 * vca0
+    rr = np.sqrt(ev31 * ev31 + ev32 * ev32 + ev33 * ev33)
+    ev31, ev32, ev33 = ev31 / rr, ev32 / rr, ev33 / rr
+    horiz = ~(np.abs(ev33) < 1.0)
+    ev11 = -ev32
+    ev12 = ev31.copy()
+    ev13 = np.zeros_like(ev31)
+    rr2 = np.sqrt(ev11 * ev11 + ev12 * ev12)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ev11 = np.where(horiz, 1.0, ev11 / rr2)
+        ev12 = np.where(horiz, 0.0, ev12 / rr2)
+    ev21 = ev32 * ev13 - ev33 * ev12
+    ev22 = ev33 * ev11 - ev31 * ev13
+    ev23 = ev31 * ev12 - ev32 * ev11
+    t1 = vba0 * vba0 + vba1 * vba1 + vba2 * vba2
+    t2 = vca0 * vca0 + vca1 * vca1 + vca2 * vca2
+    t3 = vba0 * vca0 + vba1 * vca1 + vba2 * vca2
+    ds = 0.5 * np.sqrt(t1 * t2 - t3 * t3)
+    return dict(ev11=ev11, ev12=ev12, ev13=ev13, ev21=ev21, ev22=ev22, ev23=ev23,
+                ev31=ev31, ev32=ev32, ev33=ev33), ds
+
+
+def mesh_triangles(tri: np.ndarray, crake: float = 0.0) -> Geometry:
+    """Triangle mesh (3dnt) from an (n, 3, 3) vertex array [element, vertex, xyz]."""
+    tri = np.asarray(tri, dtype=np.float64)
+    n = tri.shape[0]
+    xs1, xs2, xs3 = tri[:, 0, 0].copy(), tri[:, 1, 0].copy(), tri[:, 2, 0].copy()
+    ys1, ys2, ys3 = tri[:, 0, 1].copy(), tri[:, 1, 1].copy(), tri[:, 2, 1].copy()
+    zs1, zs2, zs3 = tri[:, 0, 2].copy(), tri[:, 1, 2].copy(), tri[:, 2, 2].copy()
+    xcol = (xs1 + xs2 + xs3) / 3
+    ycol = (ys1 + ys2 + ys3) / 3
+    zcol = (zs1 + zs2 + zs3) / 3
+    ev, ds = evcalc(xs1, xs2, xs3, ys1, ys2, ys3, zs1, zs2, zs3)
+    arr = dict(xcol=xcol, ycol=ycol, zcol=zcol, xs1=xs1, xs2=xs2, xs3=xs3, ys1=ys1, ys2=ys2, ys3=ys3,
+               zs1=zs1, zs2=zs2, zs3=zs3, rake=np.full(n, crake / 180.0 * PI), **ev)
+    arr = {k: _c(v) for k, v in arr.items()}
+    return Geometry(kind=KIND_TRI_FS, n=n, ds=ds, ds0=float(np.sqrt(2 * ds).min()), arrays=arr)
+
+
+def rough_surface_triangles(length: float, width: float, edge: float, hurst: float = 0.8,
+                            alpha: float = 1e-2, seed: int = 0) -> np.ndarray:
+    """Self-similar rough fault surface y = h(x, z) from a seeded 2-D FFT,
+    meshed as right triangles on a regular (x, depth) grid of spacing ``edge``.
+    Amplitude-to-wavelength ratio ``alpha`` is imposed on the rms height over
+    the longest dimension.  Returns (n, 3, 3) vertices; the surface spans
+    x in [-length/2, length/2], depth z in [-width, 0]."""
+    nx = max(2, int(round(length / edge)))
+    nz = max(2, int(round(width / edge)))
+    rng = np.random.default_rng(seed)
+    kx = np.fft.fftfreq(nx + 1, d=edge)
+    kz = np.fft.fftfreq(nz + 1, d=edge)
+    KX, KZ = np.meshgrid(kx, kz, indexing="ij")
+    k = np.sqrt(KX ** 2 + KZ ** 2)
+    k[0, 0] = np.inf
+    amp = k ** (-(hurst + 1.0))
+    phase = np.exp(2j * PI * rng.random(k.shape))
+    noise = rng.standard_normal(k.shape)
+    h = np.real(np.fft.ifft2(amp * noise * phase))
+    h -= h.mean()
+    lmax = max(length, width)
+    rms = float(np.sqrt(np.mean(h ** 2)))
+    if rms > 0:
+        h *= alpha * lmax / rms
+    xs = np.linspace(-0.5 * length, 0.5 * length, nx + 1)
+    zs = np.linspace(0.0, -width, nz + 1)
+    tris = []
+    for i in range(nx):
+        for j in range(nz):
+            p00 = (xs[i], h[i, j], zs[j])
+            p10 = (xs[i + 1], h[i + 1, j], zs[j])
+            p01 = (xs[i], h[i, j + 1], zs[j + 1])
+            p11 = (xs[i + 1], h[i + 1, j + 1], zs[j + 1])
+            tris.append((p00, p01, p10))
+            tris.append((p10, p01, p11))
+    return np.asarray(tris, dtype=np.float64)
+
+
+def write_stl(path: str, tri: np.ndarray) -> None:
+    """ASCII STL in the 7-lines-per-facet layout the reference reads
+    (main_LH.f90:491-509)."""
+    with open(path, "w") as f:
+        f.write("solid fault\n")
+        for t in tri:
+            f.write("facet normal 0 0 0\n outer loop\n")
+            for v in t:
+                f.write("  vertex %.17e %.17e %.17e\n" % (v[0], v[1], v[2]))
+            f.write(" endloop\nendfacet\n")
+        f.write("endsolid fault\n")
+
+
+def read_stl(path: str) -> np.ndarray:
+    """Read facets written in the 7-lines-per-facet ASCII layout."""
+    verts = []
+    with open(path) as f:
+        for line in f:
+            s = line.split()
+            if s and s[0] == "vertex":
+                verts.append([float(s[1]), float(s[2]), float(s[3])])
+    return np.asarray(verts, dtype=np.float64).reshape(-1, 3, 3)
+
+
+# ---------------------------------------------------------------------------
+# friction fields for the benchmark configurations (mesh order)
+# ---------------------------------------------------------------------------
+@dataclass
+class Friction:
+    a: np.ndarray
+    b: np.ndarray
+    dc: np.ndarray
+    f0: np.ndarray
+    vpl: np.ndarray
+    tau: np.ndarray
+    sigma: np.ndarray
+    vel: np.ndarray
+    taudot: np.ndarray
+    sigdot: np.ndarray
+    law: int = 0           # 0 aging, 1 slip
+    vref: float = 1e-6
+
+
+def steady_tau(a, b, f0, sigma, v, vref):
+    """tau such that psi = f0 - b ln(V/V0) at V: inverts main_LH.f90:1004."""
+    psi_ss = f0 - b * np.log(v / vref)
+    return a * sigma * np.arcsinh(v / (2 * vref) * np.exp(psi_ss / a))
+
+
+def friction_c1(g: Geometry) -> Friction:
+    """2-D antiplane: a = 0.010 above 15 km, ramp to a-b = +0.004 at 18 km;
+    b = 0.015, Dc = 8 mm, sigma 50 MPa, Vpl 1e-9, aging law, V0 = 1e-6."""
+    depth = g.xcol
+    b = np.full(g.n, 0.015)
+    a = np.where(depth < 15.0, 0.010,
+                 np.where(depth < 18.0, 0.010 + (depth - 15.0) / 3.0 * 0.009, 0.019))
+    return _friction(g, a, b, dc=np.full(g.n, 0.008), sigma=50.0, vpl=1e-9, vref=1e-6, law=0)
+
+
+def friction_c2(g: Geometry, imax: int, jmax: int, ds0: float, seed: int = 1,
+                patch=(60.0, 30.0), sigma_log: float = 0.1, dc0: float = 0.02, law: int = 0) -> Friction:
+    """3-D planar fault: velocity-weakening patch (a-b = -0.004) centred along
+    strike and in the upper part of the fault, a-b = +0.004 elsewhere, a = 0.01,
+    Dc = 20 mm, sigma 50 MPa; a and Dc perturbed lognormally from a seed."""
+    x = g.xcol
+    depth = -g.zcol
+    w_total = jmax * ds0
+    top = 0.5 * (w_total - patch[1])
+    in_patch = (np.abs(x) <= 0.5 * patch[0]) & (depth >= top) & (depth <= top + patch[1])
+    a = np.full(g.n, 0.01)
+    amb = np.where(in_patch, -0.004, 0.004)
+    rng = np.random.default_rng(seed)
+    za = rng.standard_normal(g.n)
+    zd = rng.standard_normal(g.n)
+    a = a * np.exp(sigma_log * za)
+    b = a - amb
+    dc = dc0 * np.exp(sigma_log * zd)
+    return _friction(g, a, b, dc=dc, sigma=50.0, vpl=1e-9, vref=1e-6, law=law)
+
+
+def friction_ensemble(g: Geometry, imax: int, jmax: int, ds0: float, nreal: int, seed: int = 7,
+                      patch=(40.0, 20.0)) -> list:
+    """Ensemble realisations for the 128x128 fault: a-b ~ N(-0.003, 0.001) in
+    the patch, Dc ~ lognormal(median 15 mm, sigma_log 0.3)."""
+    rng = np.random.default_rng(seed)
+    x = g.xcol
+    depth = -g.zcol
+    w_total = jmax * ds0
+    top = 0.5 * (w_total - patch[1])
+    in_patch = (np.abs(x) <= 0.5 * patch[0]) & (depth >= top) & (depth <= top + patch[1])
+    out = []
+    for _ in range(nreal):
+        amb_p = rng.normal(-0.003, 0.001)
+        dc = 0.015 * np.exp(0.3 * rng.standard_normal(g.n))
+        a = np.full(g.n, 0.01)
+        b = a - np.where(in_patch, amb_p, 0.004)
+        out.append(_friction(g, a, b, dc=dc, sigma=50.0, vpl=1e-9, vref=1e-6, law=0))
+    return out
+
+
+def _friction(g, a, b, dc, sigma, vpl, vref, law):
+    n = g.n
+    f0 = np.full(n, 0.6)
+    sig = np.full(n, float(sigma))
+    vel = np.full(n, float(vpl))
+    tau = steady_tau(a, b, f0, sig, vel, vref)
+    return Friction(a=_c(a), b=_c(b), dc=_c(dc), f0=f0, vpl=np.full(n, float(vpl)), tau=_c(tau), sigma=sig,
+                    vel=vel, taudot=np.zeros(n), sigdot=np.zeros(n), law=law, vref=vref)
+
+
+def init_state(fr: Friction, lod: np.ndarray) -> np.ndarray:
+    """Interleaved (psi, tau, sigma) state in cluster order (main_LH.f90:1004, 1208-1215)."""
+    a = fr.a[lod]
+    tau = fr.tau[lod]
+    sig = fr.sigma[lod]
+    vel = fr.vel[lod]
+    psi = a * np.log(2 * fr.vref / vel * np.sinh(tau / sig / a))
+    y = np.empty(3 * len(lod))
+    y[0::3] = psi
+    y[1::3] = tau
+    y[2::3] = sig
+    return y
+
+
+# ---------------------------------------------------------------------------
+# the reference's input formats
+# ---------------------------------------------------------------------------
+def write_param_file(path: str, fr: Friction, columns=("a", "b", "dc", "f0", "tau", "sigma", "vel", "vpl")) -> None:
+    """Column-named parameter file read by main_LH.f90:530-581 (header row,
+    then one row per element in mesh order)."""
+    data = np.stack([getattr(fr, c) for c in columns], axis=1)
+    with open(path, "w") as f:
+        f.write(" ".join(columns) + "\n")
+        for row in data:
+            f.write(" ".join("%.17e" % v for v in row) + "\n")
+
+
+def write_in_file(path: str, keys: Dict[str, object]) -> None:
+    """``key value`` input file parsed by main_LH.f90:182-328."""
+    with open(path, "w") as f:
+        for k, v in keys.items():
+            if isinstance(v, bool):
+                v = "T" if v else "F"
+            f.write("%s %s\n" % (k, v))
