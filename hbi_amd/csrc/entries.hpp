@@ -1,0 +1,513 @@
+// entries.hpp -- FP64 matrix-entry kernels evaluated on the GPU.
+//
+// K(i,j): traction at receiver i from unit slip on source element j, as in
+// HACApK_entry_ij (m_HACApK_calc_entry_ij.f90:24-90):
+//   2dvs   antiplane with free-surface image   m_HACApK_calc_entry_ij.f90:55-57, 685-695
+//   3dph   Okada (1992) rectangle, half-space  m_HACApK_calc_entry_ij.f90:1065-1145, okada.f90
+//   3dnr   the same without the image part
+//   3dnt   Nikkhoo & Walter (2015) triangle     m_HACApK_calc_entry_ij.f90:802-851, TDstressFS.f90
+// The closed-form expressions are those of the published solutions written
+// term by term in the reference's evaluation order; FP contraction is off in
+// every function here so that the device result tracks the FP64 CPU result
+// (remaining differences come from the device libm).  The reference has no
+// singular-case guards (okada.f90:73-92 are commented out); neither do we,
+// so IEEE +-Inf / NaN propagate exactly as there.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace hbi {
+
+constexpr double kRigid = 32.04;   // m_const.f90:6
+constexpr double kVs = 3.464;      // m_const.f90:7
+constexpr double kPois = 0.25;     // m_const.f90:3
+
+struct DevGeom {
+    int kind, n, vcomp, md;
+    double w;
+    const double *xcol, *ycol, *zcol, *xel, *xer, *ang, *angd, *dsl, *dsd, *rake;
+    const double *xs1, *xs2, *xs3, *ys1, *ys2, *ys3, *zs1, *zs2, *zs3;
+    const double *ev11, *ev12, *ev13, *ev21, *ev22, *ev23, *ev31, *ev32, *ev33;
+};
+
+__device__ __forceinline__ double dpi() { return 4.0 * atan(1.0); }
+
+// ---------------------------------------------------------------- Okada ----
+struct OkGeo {
+    double xi2, et2, q2, r, r2, r3, r5, yp, d, tt, alx, ale, x11, y11, x32, y32, ey, ez, fy, fz, gy, gz;
+};
+
+__device__ __noinline__ void ok_geo(OkGeo &c, double xi, double et, double q, double sd, double cd)
+{
+#pragma clang fp contract(off)
+    c.xi2 = xi * xi;
+    c.et2 = et * et;
+    c.q2 = q * q;
+    c.r2 = c.xi2 + c.et2 + c.q2;
+    c.r = sqrt(c.r2);
+    c.r3 = c.r * c.r2;
+    c.r5 = c.r3 * c.r2;
+    c.yp = et * cd + q * sd;
+    c.d = et * sd - q * cd;
+    c.tt = atan(xi * et / (q * c.r));
+    const double rxi = c.r + xi;
+    c.alx = log(rxi);
+    c.x11 = 1.0 / (c.r * rxi);
+    c.x32 = (c.r + rxi) * c.x11 * c.x11 / c.r;
+    const double ret = c.r + et;
+    c.ale = log(ret);
+    c.y11 = 1.0 / (c.r * ret);
+    c.y32 = (c.r + ret) * c.y11 * c.y11 / c.r;
+    c.ey = sd / c.r - c.yp * q / c.r3;
+    c.ez = cd / c.r + c.d * q / c.r3;
+    c.fy = c.d / c.r3 + c.xi2 * c.y32 * sd;
+    c.fz = c.yp / c.r3 + c.xi2 * c.y32 * cd;
+    c.gy = 2.0 * c.x11 * sd - c.yp * q * c.x32;
+    c.gz = 2.0 * c.x11 * cd + c.d * q * c.x32;
+}
+
+// strike (s1) and dip (s2) terms are combined as out = s1/2pi * du_strike + s2/2pi * du_dip
+__device__ __noinline__ void ok_part_a(const OkGeo &c, double xi, double et, double q, double sd, double cd,
+                                       double s1, double s2, double alpha, double *o)
+{
+#pragma clang fp contract(off)
+    const double pi2 = 8.0 * atan(1.0);
+    const double alp1 = (1.0 - alpha) / 2.0, alp2 = alpha / 2.0;
+    const double r = c.r, r3 = c.r3, q2 = c.q2, xi2 = c.xi2, d = c.d, yp = c.yp;
+    const double xy = xi * c.y11, qx = q * c.x11, qy = q * c.y11;
+    const double f1 = s1 / pi2, f2 = s2 / pi2;
+    o[0] = (0.0 + f1 * (c.tt / 2.0 + alp2 * xi * qy)) + f2 * (alp2 * q / r);
+    o[1] = (0.0 + f1 * (alp2 * q / r)) + f2 * (c.tt / 2.0 + alp2 * et * qx);
+    o[2] = (0.0 + f1 * (alp1 * c.ale - alp2 * q * qy)) + f2 * (alp1 * c.alx - alp2 * q * qx);
+    o[3] = (0.0 + f1 * (-alp1 * qy - alp2 * xi2 * q * c.y32)) + f2 * (-alp2 * xi * q / r3);
+    o[4] = (0.0 + f1 * (-alp2 * xi * q / r3)) + f2 * (-qy / 2.0 - alp2 * et * q / r3);
+    o[5] = (0.0 + f1 * (alp1 * xy + alp2 * xi * q2 * c.y32)) + f2 * (alp1 / r + alp2 * q2 / r3);
+    o[6] = (0.0 + f1 * (alp1 * xy * sd + alp2 * xi * c.fy + d / 2.0 * c.x11)) + f2 * (alp2 * c.ey);
+    o[7] = (0.0 + f1 * (alp2 * c.ey)) + f2 * (alp1 * d * c.x11 + xy / 2.0 * sd + alp2 * et * c.gy);
+    o[8] = (0.0 + f1 * (alp1 * (cd / r + qy * sd) - alp2 * q * c.fy)) + f2 * (alp1 * yp * c.x11 - alp2 * q * c.gy);
+    o[9] = (0.0 + f1 * (alp1 * xy * cd + alp2 * xi * c.fz + yp / 2.0 * c.x11)) + f2 * (alp2 * c.ez);
+    o[10] = (0.0 + f1 * (alp2 * c.ez)) + f2 * (alp1 * yp * c.x11 + xy / 2.0 * cd + alp2 * et * c.gz);
+    o[11] = (0.0 + f1 * (-alp1 * (sd / r - qy * cd) - alp2 * q * c.fz)) + f2 * (-alp1 * d * c.x11 - alp2 * q * c.gz);
+}
+
+__device__ __noinline__ void ok_part_b(const OkGeo &c, double xi, double et, double q, double sd, double cd,
+                                       double s1, double s2, double alpha, double *o)
+{
+#pragma clang fp contract(off)
+    const double pi2 = 8.0 * atan(1.0);
+    const double alp3 = (1.0 - alpha) / alpha;
+    const double r = c.r, r3 = c.r3, q2 = c.q2, xi2 = c.xi2, d = c.d, yp = c.yp;
+    const double rd = r + d;
+    const double d11 = 1.0 / (r * rd);
+    const double aj2 = xi * yp / rd * d11;
+    const double aj5 = -(d + yp * yp / rd) * d11;
+    double ai3, ai4, ak1, ak3, aj3, aj6;
+    if (cd != 0.0) {
+        if (xi == 0.0) {
+            ai4 = 0.0;
+        } else {
+            const double xs = sqrt(xi2 + q2);
+            ai4 = 1.0 / cd / cd * (xi / rd * sd * cd + 2.0 * atan((et * (xs + q * cd) + xs * (r + xs) * sd) / (xi * (r + xs) * cd)));
+        }
+        ai3 = (yp * cd / rd - c.ale + sd * log(rd)) / cd / cd;
+        ak1 = xi * (d11 - c.y11 * sd) / cd;
+        ak3 = (q * c.y11 - yp * d11) / cd;
+        aj3 = (ak1 - aj2 * sd) / cd;
+        aj6 = (ak3 - aj5 * sd) / cd;
+    } else {
+        const double rd2 = rd * rd;
+        ai3 = (et / rd + yp * q / rd2 - c.ale) / 2.0;
+        ai4 = xi * yp / rd2 / 2.0;
+        ak1 = xi * q / rd * d11;
+        ak3 = sd / rd * (xi2 * d11 - 1.0);
+        aj3 = -xi / rd2 * (q2 * d11 - 0.5);
+        aj6 = -yp / rd2 * (xi2 * d11 - 0.5);
+    }
+    const double xy = xi * c.y11;
+    const double ai1 = -xi / rd * cd - ai4 * sd;
+    const double ai2 = log(rd) + ai3 * sd;
+    const double ak2 = 1.0 / r + ak3 * sd;
+    const double ak4 = xy * cd - ak1 * sd;
+    const double aj1 = aj5 * cd - aj6 * sd;
+    const double aj4 = -xy - aj2 * cd + aj3 * sd;
+    const double qx = q * c.x11, qy = q * c.y11;
+    const double f1 = s1 / pi2, f2 = s2 / pi2;
+    o[0] = (0.0 + f1 * (-xi * qy - c.tt - alp3 * ai1 * sd)) + f2 * (-q / r + alp3 * ai3 * sd * cd);
+    o[1] = (0.0 + f1 * (-q / r + alp3 * yp / rd * sd)) + f2 * (-et * qx - c.tt - alp3 * xi / rd * sd * cd);
+    o[2] = (0.0 + f1 * (q * qy - alp3 * ai2 * sd)) + f2 * (q * qx + alp3 * ai4 * sd * cd);
+    o[3] = (0.0 + f1 * (xi2 * q * c.y32 - alp3 * aj1 * sd)) + f2 * (xi * q / r3 + alp3 * aj4 * sd * cd);
+    o[4] = (0.0 + f1 * (xi * q / r3 - alp3 * aj2 * sd)) + f2 * (et * q / r3 + qy + alp3 * aj5 * sd * cd);
+    o[5] = (0.0 + f1 * (-xi * q2 * c.y32 - alp3 * aj3 * sd)) + f2 * (-q2 / r3 + alp3 * aj6 * sd * cd);
+    o[6] = (0.0 + f1 * (-xi * c.fy - d * c.x11 + alp3 * (xy + aj4) * sd)) + f2 * (-c.ey + alp3 * aj1 * sd * cd);
+    o[7] = (0.0 + f1 * (-c.ey + alp3 * (1.0 / r + aj5) * sd)) + f2 * (-et * c.gy - xy * sd + alp3 * aj2 * sd * cd);
+    o[8] = (0.0 + f1 * (q * c.fy - alp3 * (qy - aj6) * sd)) + f2 * (q * c.gy + alp3 * aj3 * sd * cd);
+    o[9] = (0.0 + f1 * (-xi * c.fz - yp * c.x11 + alp3 * ak1 * sd)) + f2 * (-c.ez - alp3 * ak3 * sd * cd);
+    o[10] = (0.0 + f1 * (-c.ez + alp3 * yp * d11 * sd)) + f2 * (-et * c.gz - xy * cd - alp3 * xi * d11 * sd * cd);
+    o[11] = (0.0 + f1 * (q * c.fz + alp3 * ak2 * sd)) + f2 * (q * c.gz - alp3 * ak4 * sd * cd);
+}
+
+__device__ __noinline__ void ok_part_c(const OkGeo &c, double xi, double et, double q, double z, double sd, double cd,
+                                       double s1, double s2, double alpha, double *o)
+{
+#pragma clang fp contract(off)
+    const double pi2 = 8.0 * atan(1.0);
+    const double alp4 = 1.0 - alpha, alp5 = alpha;
+    const double r = c.r, r2 = c.r2, r3 = c.r3, r5 = c.r5, q2 = c.q2, xi2 = c.xi2, et2 = c.et2, d = c.d, yp = c.yp;
+    const double x11 = c.x11, y11 = c.y11, x32 = c.x32, y32 = c.y32;
+    const double cc = d + z;
+    const double x53 = (8.0 * r2 + 9.0 * r * xi + 3.0 * xi2) * x11 * x11 * x11 / r2;
+    const double y53 = (8.0 * r2 + 9.0 * r * et + 3.0 * et2) * y11 * y11 * y11 / r2;
+    const double h = q * cd - z;
+    const double z32 = sd / r3 - h * y32;
+    const double z53 = 3.0 * sd / r5 - h * y53;
+    const double y0 = y11 - xi2 * y32;
+    const double z0 = z32 - xi2 * z53;
+    const double ppy = cd / r3 + q * y32 * sd;
+    const double ppz = sd / r3 - q * y32 * cd;
+    const double qq = z * y32 + z32 + z0;
+    const double qqy = 3.0 * cc * d / r5 - qq * sd;
+    const double qqz = 3.0 * cc * yp / r5 - qq * cd + q * y32;
+    const double xy = xi * y11, qy = q * y11;
+    const double qr = 3.0 * q / r5;
+    const double cdr = (cc + d) / r3;
+    const double yy0 = yp / r3 - y0 * cd;
+    const double f1 = s1 / pi2, f2 = s2 / pi2;
+    o[0] = (0.0 + f1 * (alp4 * xy * cd - alp5 * xi * q * z32)) + f2 * (alp4 * cd / r - qy * sd - alp5 * cc * q / r3);
+    o[1] = (0.0 + f1 * (alp4 * (cd / r + 2.0 * qy * sd) - alp5 * cc * q / r3)) + f2 * (alp4 * yp * x11 - alp5 * cc * et * q * x32);
+    o[2] = (0.0 + f1 * (alp4 * qy * cd - alp5 * (cc * et / r3 - z * y11 + xi2 * z32))) +
+           f2 * (-d * x11 - xy * sd - alp5 * cc * (x11 - q2 * x32));
+    o[3] = (0.0 + f1 * (alp4 * y0 * cd - alp5 * q * z0)) + f2 * (-alp4 * xi / r3 * cd + alp5 * cc * xi * qr + xi * q * y32 * sd);
+    o[4] = (0.0 + f1 * (-alp4 * xi * (cd / r3 + 2.0 * q * y32 * sd) + alp5 * cc * xi * qr)) + f2 * (-alp4 * yp / r3 + alp5 * cc * et * qr);
+    o[5] = (0.0 + f1 * (-alp4 * xi * q * y32 * cd + alp5 * xi * (3.0 * cc * et / r5 - qq))) +
+           f2 * (d / r3 - y0 * sd + alp5 * cc / r3 * (1.0 - 3.0 * q2 / r2));
+    o[6] = (0.0 + f1 * (-alp4 * xi * ppy * cd - alp5 * xi * qqy)) + f2 * (-alp4 * et / r3 + y0 * sd * sd - alp5 * (cdr * sd - cc * yp * qr));
+    o[7] = (0.0 + f1 * (alp4 * 2.0 * (d / r3 - y0 * sd) * sd - yp / r3 * cd - alp5 * (cdr * sd - et / r3 - cc * yp * qr))) +
+           f2 * (alp4 * (x11 - yp * yp * x32) - alp5 * cc * ((d + 2.0 * q * cd) * x32 - yp * et * q * x53));
+    o[8] = (0.0 + f1 * (-alp4 * q / r3 + yy0 * sd + alp5 * (cdr * cd + cc * d * qr - (y0 * cd + q * z0) * sd))) +
+           f2 * (xi * ppy * sd + yp * d * x32 + alp5 * cc * ((yp + 2.0 * q * sd) * x32 - yp * q2 * x53));
+    o[9] = (0.0 + f1 * (alp4 * xi * ppz * cd - alp5 * xi * qqz)) + f2 * (-q / r3 + y0 * sd * cd - alp5 * (cdr * cd + cc * d * qr));
+    o[10] = (0.0 + f1 * (alp4 * 2.0 * (yp / r3 - y0 * cd) * sd + d / r3 * cd - alp5 * (cdr * cd + cc * d * qr))) +
+            f2 * (alp4 * yp * d * x32 - alp5 * cc * ((yp - 2.0 * q * sd) * x32 + d * et * q * x53));
+    o[11] = (0.0 + f1 * (yy0 * cd - alp5 * (cdr * sd - cc * yp * qr - y0 * sd * sd + q * z0 * cd))) +
+            f2 * (-xi * ppz * sd + x11 - d * d * x32 - alp5 * cc * ((d - 2.0 * q * cd) * x32 - d * q2 * x53));
+}
+
+// displacement gradients u[3..11] = (uxx,uyx,uzx,uxy,uyy,uzy,uxz,uyz,uzz) of a
+// finite rectangle with strike slip s1 and dip slip s2 (okada.f90:10-226)
+__device__ __noinline__ void ok_dc3d(double alpha, double x, double y, double z, double depth, double dip,
+                                     double al1, double al2, double aw1, double aw2, double s1, double s2,
+                                     bool fullspace, double *u)
+{
+#pragma clang fp contract(off)
+    const double pi2 = 8.0 * atan(1.0), epso = 1e-6;
+    const double p18 = pi2 / 360.0;
+    double sd = sin(dip * p18), cd = cos(dip * p18);
+    if (fabs(cd) < epso) {
+        cd = 0.0;
+        if (sd > 0.0) sd = 1.0;
+        if (sd < 0.0) sd = -1.0;
+    }
+    const double xi[2] = {x - al1, x - al2};
+    double d = depth + z;
+    double p = y * cd + d * sd;
+    double q = y * sd - d * cd;
+    double et[2] = {p - aw1, p - aw2};
+    double da[12], db[12], dcv[12], du[12];
+    for (int i = 0; i < 12; i++) u[i] = 0.0;
+    OkGeo c;
+    for (int k = 0; k < 2; k++)
+        for (int j = 0; j < 2; j++) {
+            ok_geo(c, xi[j], et[k], q, sd, cd);
+            ok_part_a(c, xi[j], et[k], q, sd, cd, s1, s2, alpha, da);
+            for (int b = 0; b < 12; b += 3) {
+                du[b] = -da[b];
+                du[b + 1] = -da[b + 1] * cd + da[b + 2] * sd;
+                du[b + 2] = -da[b + 1] * sd - da[b + 2] * cd;
+            }
+            du[9] = da[9];
+            du[10] = da[10] * cd - da[11] * sd;
+            du[11] = da[10] * sd + da[11] * cd;
+            const bool add = (j + k) != 1;
+            for (int i = 0; i < 12; i++) u[i] = add ? u[i] + du[i] : u[i] - du[i];
+        }
+    if (fullspace) return;
+    d = depth - z;
+    p = y * cd + d * sd;
+    q = y * sd - d * cd;
+    et[0] = p - aw1;
+    et[1] = p - aw2;
+    for (int k = 0; k < 2; k++)
+        for (int j = 0; j < 2; j++) {
+            ok_geo(c, xi[j], et[k], q, sd, cd);
+            ok_part_a(c, xi[j], et[k], q, sd, cd, s1, s2, alpha, da);
+            ok_part_b(c, xi[j], et[k], q, sd, cd, s1, s2, alpha, db);
+            ok_part_c(c, xi[j], et[k], q, z, sd, cd, s1, s2, alpha, dcv);
+            for (int b = 0; b < 12; b += 3) {
+                du[b] = da[b] + db[b] + z * dcv[b];
+                du[b + 1] = (da[b + 1] + db[b + 1] + z * dcv[b + 1]) * cd - (da[b + 2] + db[b + 2] + z * dcv[b + 2]) * sd;
+                du[b + 2] = (da[b + 1] + db[b + 1] - z * dcv[b + 1]) * sd + (da[b + 2] + db[b + 2] - z * dcv[b + 2]) * cd;
+            }
+            du[9] = du[9] + dcv[0];
+            du[10] = du[10] + dcv[1] * cd - dcv[2] * sd;
+            du[11] = du[11] - dcv[1] * sd - dcv[2] * cd;
+            const bool add = (j + k) != 1;
+            for (int i = 0; i < 12; i++) u[i] = add ? u[i] + du[i] : u[i] - du[i];
+        }
+}
+
+// tensor rotation T2 = A^T T1 A of TensTrans (TDstressFS.f90:295-314); A[r][c] = A(r+1,c+1)
+__device__ __forceinline__ void tens_trans(const double *t, const double A[3][3], double *o)
+{
+#pragma clang fp contract(off)
+    const double Txx = t[0], Tyy = t[1], Tzz = t[2], Txy = t[3], Txz = t[4], Tyz = t[5];
+    const double a11 = A[0][0], a12 = A[0][1], a13 = A[0][2];
+    const double a21 = A[1][0], a22 = A[1][1], a23 = A[1][2];
+    const double a31 = A[2][0], a32 = A[2][1], a33 = A[2][2];
+    o[0] = a11 * a11 * Txx + 2 * a11 * a21 * Txy + 2 * a11 * a31 * Txz + 2 * a21 * a31 * Tyz + a21 * a21 * Tyy + a31 * a31 * Tzz;
+    o[1] = a12 * a12 * Txx + 2 * a12 * a22 * Txy + 2 * a12 * a32 * Txz + 2 * a22 * a32 * Tyz + a22 * a22 * Tyy + a32 * a32 * Tzz;
+    o[2] = a13 * a13 * Txx + 2 * a13 * a23 * Txy + 2 * a13 * a33 * Txz + 2 * a23 * a33 * Tyz + a23 * a23 * Tyy + a33 * a33 * Tzz;
+    o[3] = a11 * a12 * Txx + (a11 * a22 + a21 * a12) * Txy + (a11 * a32 + a12 * a31) * Txz + (a32 * a21 + a31 * a22) * Tyz + a22 * a21 * Tyy + a31 * a32 * Tzz;
+    o[4] = a11 * a13 * Txx + (a11 * a23 + a13 * a21) * Txy + (a11 * a33 + a31 * a13) * Txz + (a33 * a21 + a31 * a23) * Tyz + a23 * a21 * Tyy + a31 * a33 * Tzz;
+    o[5] = a12 * a13 * Txx + (a13 * a22 + a12 * a23) * Txy + (a13 * a32 + a12 * a33) * Txz + (a23 * a32 + a33 * a22) * Tyz + a22 * a23 * Tyy + a32 * a33 * Tzz;
+}
+
+__device__ __noinline__ double okada_entry(const DevGeom &g, int i, int j, bool fullspace)
+{
+#pragma clang fp contract(off)
+    const double pi = dpi();
+    const double alpha = (1.0 + (0.5 / kPois - 1.0)) / (1.0 + 2.0 * (0.5 / kPois - 1.0));
+    const double angj = g.ang[j];
+    const double dxg = g.xcol[i] - g.xcol[j], dyg = g.ycol[i] - g.ycol[j];
+    const double dx = cos(angj) * dxg + sin(angj) * dyg;
+    const double dy = -sin(angj) * dxg + cos(angj) * dyg;
+    const double w = g.w;
+    const double z = g.zcol[i];
+    const double depth = -g.zcol[j];
+    const double dip = g.angd[j] * 180.0 / pi;
+    const double rs = g.dsl[j] / w, rd = g.dsd[j] / w;
+    double u[12];
+    ok_dc3d(alpha, dx / w, dy / w, z / w, depth / w, dip, -0.5 * rs, 0.5 * rs, -0.5 * rd, 0.5 * rd,
+            -cos(g.rake[j]), -sin(g.rake[j]), fullspace, u);
+    const double exx = u[3], eyy = u[7], ezz = u[11];
+    const double exy = 0.5 * (u[6] + u[4]), eyz = 0.5 * (u[10] + u[8]), ezx = 0.5 * (u[5] + u[9]);
+    const double R = kRigid;
+    double s[6];
+    s[0] = R * (exx + eyy + ezz) + 2 * R * exx;
+    s[1] = R * (exx + eyy + ezz) + 2 * R * eyy;
+    s[2] = R * (exx + eyy + ezz) + 2 * R * ezz;
+    s[3] = 2 * R * exy;
+    s[4] = 2 * R * ezx;
+    s[5] = 2 * R * eyz;
+    const double rot = angj - g.ang[i];
+    const double dp = g.angd[i];
+    const double A[3][3] = {{cos(rot), sin(rot) * cos(dp), sin(rot) * sin(dp)},
+                            {-sin(rot), cos(rot) * cos(dp), cos(rot) * sin(dp)},
+                            {0.0, sin(dp), -cos(dp)}};
+    double p[6];
+    tens_trans(s, A, p);
+    if (g.vcomp == 1) return p[2];
+    return p[4] * cos(g.rake[i]) + p[5] * sin(g.rake[i]);
+}
+
+// ------------------------------------------------------------- triangles ----
+__device__ __forceinline__ double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - b[1] * a[2];
+    c[1] = a[2] * b[0] - b[2] * a[0];
+    c[2] = a[0] * b[1] - b[0] * a[1];
+}
+
+// strain of an angular dislocation (Nikkhoo & Walter 2015), TDstressFS.f90:560-627
+__device__ __noinline__ void ang_dis_strain(double x, double y, double z, double alpha, double bx, double by, double bz,
+                                            double nu, double *e)
+{
+#pragma clang fp contract(off)
+    const double pi = dpi();
+    const double sinA = sin(alpha), cosA = cos(alpha);
+    const double eta = y * cosA - z * sinA;
+    const double zeta = y * sinA + z * cosA;
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double r2 = x2 + y2 + z2;
+    const double r = sqrt(r2);
+    const double r3 = r * r2;
+    const double rz = r * (r - z);
+    const double r2z2 = r2 * ((r - z) * (r - z));
+    const double r3z = r3 * (r - z);
+    const double W = zeta - r;
+    const double W2 = W * W;
+    const double Wr = W * r;
+    const double W2r = W2 * r;
+    const double Wr3 = W * r3;
+    const double W2r2 = W2 * r2;
+    const double C = (r * cosA - z) / Wr;
+    const double S = (r * sinA - y) / Wr;
+    const double rFi_rx = (eta / r / (r - zeta) - y / r / (r - z)) / 4 / pi;
+    const double rFi_ry = (x / r / (r - z) - cosA * x / r / (r - zeta)) / 4 / pi;
+    const double rFi_rz = (sinA * x / r / (r - zeta)) / 4 / pi;
+    e[0] = bx * (rFi_rx) + bx / 8 / pi / (1 - nu) * (eta / Wr + eta * x2 / W2r2 - eta * x2 / Wr3 + y / rz -
+        x2 * y / r2z2 - x2 * y / r3z) - by * x / 8 / pi / (1 - nu) * (((2 * nu + 1) / Wr + x2 / W2r2 - x2 / Wr3) * cosA +
+        (2 * nu + 1) / rz - x2 / r2z2 - x2 / r3z) + bz * x * sinA / 8 / pi / (1 - nu) * ((2 * nu + 1) / Wr + x2 / W2r2 - x2 / Wr3);
+    e[1] = by * (rFi_ry) + bx / 8 / pi / (1 - nu) * ((1 / Wr + S * S - y2 / Wr3) * eta + (2 * nu + 1) * y / rz - y * y * y / r2z2 -
+        y * y * y / r3z - 2 * nu * cosA * S) - by * x / 8 / pi / (1 - nu) * (1 / rz - y2 / r2z2 - y2 / r3z +
+        (1 / Wr + S * S - y2 / Wr3) * cosA) + bz * x * sinA / 8 / pi / (1 - nu) * (1 / Wr + S * S - y2 / Wr3);
+    e[2] = bz * (rFi_rz) + bx / 8 / pi / (1 - nu) * (eta / W / r + eta * C * C - eta * z2 / Wr3 + y * z / r3 +
+        2 * nu * sinA * C) - by * x / 8 / pi / (1 - nu) * ((1 / Wr + C * C - z2 / Wr3) * cosA + z / r3) +
+        bz * x * sinA / 8 / pi / (1 - nu) * (1 / Wr + C * C - z2 / Wr3);
+    e[3] = bx * (rFi_ry) / 2 + by * (rFi_rx) / 2 - bx / 8 / pi / (1 - nu) * (x * y2 / r2z2 - nu * x / rz +
+        x * y2 / r3z - nu * x * cosA / Wr + eta * x * S / Wr + eta * x * y / Wr3) +
+        by / 8 / pi / (1 - nu) * (x2 * y / r2z2 - nu * y / rz + x2 * y / r3z + nu * cosA * S +
+        x2 * y * cosA / Wr3 + x2 * cosA * S / Wr) - bz * sinA / 8 / pi / (1 - nu) * (nu * S + x2 * S / Wr + x2 * y / Wr3);
+    e[4] = bx * (rFi_rz) / 2 + bz * (rFi_rx) / 2 - bx / 8 / pi / (1 - nu) * (-x * y / r3 + nu * x * sinA / Wr + eta * x * C / Wr +
+        eta * x * z / Wr3) + by / 8 / pi / (1 - nu) * (-x2 / r3 + nu / r + nu * cosA * C + x2 * z * cosA / Wr3 +
+        x2 * cosA * C / Wr) - bz * sinA / 8 / pi / (1 - nu) * (nu * C + x2 * C / Wr + x2 * z / Wr3);
+    e[5] = by * (rFi_rz) / 2 + bz * (rFi_ry) / 2 + bx / 8 / pi / (1 - nu) * (y2 / r3 - nu / r - nu * cosA * C +
+        nu * sinA * S + eta * sinA * cosA / W2 - eta * (y * cosA + z * sinA) / W2r + eta * y * z / W2r2 - eta * y * z / Wr3) -
+        by * x / 8 / pi / (1 - nu) * (y / r3 + sinA * (cosA * cosA) / W2 - cosA * (y * cosA + z * sinA) /
+        W2r + y * z * cosA / W2r2 - y * z * cosA / Wr3) - bz * x * sinA / 8 / pi / (1 - nu) *
+        (y * z / Wr3 - sinA * cosA / W2 + (y * cosA + z * sinA) / W2r - y * z / W2r2);
+}
+
+// TDSetupD, TDstressFS.f90:352-403
+__device__ __noinline__ void td_setup_d(double x, double y, double z, double alpha, double bx, double by, double bz,
+                                        double nu, const double *tv, const double *sv, double *e)
+{
+#pragma clang fp contract(off)
+    const double pi = dpi();
+    const double A11 = sv[2], A12 = -sv[1], A21 = sv[1], A22 = sv[2];
+    const double y1 = A11 * (y - tv[1]) + A12 * (z - tv[2]);
+    const double z1 = A21 * (y - tv[1]) + A22 * (z - tv[2]);
+    const double by1 = A11 * by + A12 * bz;
+    const double bz1 = A21 * by + A22 * bz;
+    double ea[6];
+    ang_dis_strain(x, y1, z1, -pi + alpha, bx, by1, bz1, nu, ea);
+    const double B[3][3] = {{1.0, 0.0, 0.0}, {0.0, A11, A12}, {0.0, A21, A22}};
+    tens_trans(ea, B, e);
+}
+
+__device__ __forceinline__ int tri_mode(double x, double y, double z, const double *p1, const double *p2, const double *p3)
+{
+#pragma clang fp contract(off)
+    const double den = (p2[1] - p3[1]) * (p1[0] - p3[0]) + (p3[0] - p2[0]) * (p1[1] - p3[1]);
+    const double a = ((p2[1] - p3[1]) * (x - p3[0]) + (p3[0] - p2[0]) * (y - p3[1])) / den;
+    const double b = ((p3[1] - p1[1]) * (x - p3[0]) + (p1[0] - p3[0]) * (y - p3[1])) / den;
+    const double c = 1.0 - a - b;
+    int m = 1;
+    if ((a <= 0.0) && (b > c) && (c > a)) m = -1;
+    if ((b <= 0.0) && (c > a) && (a > b)) m = -1;
+    if ((c <= 0.0) && (a > b) && (b > c)) m = -1;
+    if ((a == 0.0) && (b >= 0.0) && (c >= 0.0)) m = 0;
+    if ((a >= 0.0) && (b == 0.0) && (c >= 0.0)) m = 0;
+    if ((a >= 0.0) && (b >= 0.0) && (c == 0.0)) m = 0;
+    if ((m == 0) && (z != 0.0)) m = 1;
+    return m;
+}
+
+// full-space stress of a triangular dislocation, TDstressFS.f90:45-235
+__device__ __noinline__ void td_stress_fs(double X, double Y, double Z, const double *P1, const double *P2, const double *P3,
+                                          double Ss, double Ds, double Ts, double mu, double lambda, double *Sout)
+{
+#pragma clang fp contract(off)
+    const double nu = 1 / (1 + lambda / mu) / 2;
+    const double bx = Ts, by = Ss, bz = Ds;
+    double a[3], b[3], vn[3], vs[3], vd[3];
+    for (int k = 0; k < 3; k++) { a[k] = P2[k] - P1[k]; b[k] = P3[k] - P1[k]; }
+    cross3(a, b, vn);
+    double nn = sqrt(dot3(vn, vn));
+    for (int k = 0; k < 3; k++) vn[k] = vn[k] / nn;
+    const double eY[3] = {0.0, 1.0, 0.0}, eZ[3] = {0.0, 0.0, 1.0};
+    cross3(eZ, vn, vs);
+    nn = sqrt(dot3(vs, vs));
+    for (int k = 0; k < 3; k++) vs[k] = vs[k] / nn;
+    if (dot3(vs, vs) == 0.0)
+        for (int k = 0; k < 3; k++) vs[k] = eY[k] * vn[2];
+    nn = sqrt(dot3(vs, vs));
+    for (int k = 0; k < 3; k++) vs[k] = vs[k] / nn;
+    cross3(vn, vs, vd);
+    double At[3][3];
+    for (int k = 0; k < 3; k++) { At[0][k] = vn[k]; At[1][k] = vs[k]; At[2][k] = vd[k]; }
+    auto ct = [&](double a0, double a1, double a2, double *o) {
+        o[0] = At[0][0] * a0 + At[0][1] * a1 + At[0][2] * a2;
+        o[1] = At[1][0] * a0 + At[1][1] * a1 + At[1][2] * a2;
+        o[2] = At[2][0] * a0 + At[2][1] * a1 + At[2][2] * a2;
+    };
+    double xyz[3], p1[3], p2[3] = {0.0, 0.0, 0.0}, p3[3];
+    ct(X - P2[0], Y - P2[1], Z - P2[2], xyz);
+    ct(P1[0] - P2[0], P1[1] - P2[1], P1[2] - P2[2], p1);
+    ct(P3[0] - P2[0], P3[1] - P2[1], P3[2] - P2[2], p3);
+    double e12[3], e13[3], e23[3];
+    for (int k = 0; k < 3; k++) { e12[k] = p2[k] - p1[k]; e13[k] = p3[k] - p1[k]; e23[k] = p3[k] - p2[k]; }
+    nn = sqrt(dot3(e12, e12)); for (int k = 0; k < 3; k++) e12[k] = e12[k] / nn;
+    nn = sqrt(dot3(e13, e13)); for (int k = 0; k < 3; k++) e13[k] = e13[k] / nn;
+    nn = sqrt(dot3(e23, e23)); for (int k = 0; k < 3; k++) e23[k] = e23[k] / nn;
+    const double m12[3] = {-e12[0], -e12[1], -e12[2]}, m13[3] = {-e13[0], -e13[1], -e13[2]}, m23[3] = {-e23[0], -e23[1], -e23[2]};
+    const double A = acos(dot3(e12, e13));
+    const double B = acos(dot3(m12, e23));
+    const double C = acos(dot3(e23, e13));
+    const int mode = tri_mode(xyz[1], xyz[2], xyz[0], p1 + 1, p2 + 1, p3 + 1);
+    double e1[6], e2[6], e3[6], e[6];
+    if (mode == 1) {
+        td_setup_d(xyz[0], xyz[1], xyz[2], A, bx, by, bz, nu, p1, m13, e1);
+        td_setup_d(xyz[0], xyz[1], xyz[2], B, bx, by, bz, nu, p2, e12, e2);
+        td_setup_d(xyz[0], xyz[1], xyz[2], C, bx, by, bz, nu, p3, e23, e3);
+    } else {
+        td_setup_d(xyz[0], xyz[1], xyz[2], A, bx, by, bz, nu, p1, e13, e1);
+        td_setup_d(xyz[0], xyz[1], xyz[2], B, bx, by, bz, nu, p2, m12, e2);
+        td_setup_d(xyz[0], xyz[1], xyz[2], C, bx, by, bz, nu, p3, m23, e3);
+    }
+    for (int k = 0; k < 6; k++) e[k] = e1[k] + e2[k] + e3[k];
+    if (mode == 0)
+        for (int k = 0; k < 6; k++) e[k] = __builtin_nan("");
+    double E[6];
+    tens_trans(e, At, E);
+    Sout[0] = 2 * mu * E[0] + lambda * (E[0] + E[1] + E[2]);
+    Sout[1] = 2 * mu * E[1] + lambda * (E[0] + E[1] + E[2]);
+    Sout[2] = 2 * mu * E[2] + lambda * (E[0] + E[1] + E[2]);
+    Sout[3] = 2 * mu * E[3];
+    Sout[4] = 2 * mu * E[4];
+    Sout[5] = 2 * mu * E[5];
+}
+
+__device__ __noinline__ double tri_entry(const DevGeom &g, int i, int j)
+{
+#pragma clang fp contract(off)
+    const double lambda = 2 * kPois / (1 - 2 * kPois) * kRigid;
+    const double P1[3] = {g.xs1[j], g.ys1[j], g.zs1[j]};
+    const double P2[3] = {g.xs2[j], g.ys2[j], g.zs2[j]};
+    const double P3[3] = {g.xs3[j], g.ys3[j], g.zs3[j]};
+    double ss = 0.0, dd = 0.0, tt = 0.0;
+    if (g.md == 1) ss = 1.0;
+    else if (g.md == 2) dd = 1.0;
+    else if (g.md == 3) tt = 1.0;
+    else { ss = cos(g.rake[i]); dd = sin(g.rake[i]); }
+    double S[6];
+    td_stress_fs(g.xcol[i], g.ycol[i], g.zcol[i], P1, P2, P3, ss, dd, tt, kRigid, lambda, S);
+    const double A[3][3] = {{g.ev11[i], g.ev21[i], g.ev31[i]},
+                            {g.ev12[i], g.ev22[i], g.ev32[i]},
+                            {g.ev13[i], g.ev23[i], g.ev33[i]}};
+    double p[6];
+    tens_trans(S, A, p);
+    if (g.vcomp == 1) return -p[2];
+    return p[4] * cos(g.rake[i]) + p[5] * sin(g.rake[i]);
+}
+
+__device__ __forceinline__ double antiplane(double xc, double xl, double xr)
+{
+#pragma clang fp contract(off)
+    const double factor = kRigid / (2.0 * dpi());
+    return factor * (1.0 / (xc - xr) - 1.0 / (xc - xl));
+}
+
+// HACApK_entry_ij: receiver i, source j (mesh order, 0-based)
+__device__ __forceinline__ double entry(const DevGeom &g, int i, int j)
+{
+    switch (g.kind) {
+    case 1: return antiplane(g.xcol[i], g.xel[j], g.xer[j]) - antiplane(g.xcol[i], -g.xel[j], -g.xer[j]);
+    case 2: return okada_entry(g, i, j, false);
+    case 3: return okada_entry(g, i, j, true);
+    case 4: return tri_entry(g, i, j);
+    }
+    return __builtin_nan("");
+}
+
+}  // namespace hbi

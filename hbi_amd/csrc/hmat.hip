@@ -1,0 +1,1144 @@
+// hmat.hip -- GPU assembly and products of the lattice H-matrix.
+//
+// Build (hbi_hmat_build, replaces HACApK_generate + HACApK_construct_LH):
+//   1. cluster tree + leaf partition on the host (partition.cpp)
+//   2. dense leaves: one thread per FP64 entry (fill_leafmtx_hyp's dense
+//      branch, m_HACApK_base.f90:2020-2031)
+//   3. low-rank leaves: ACA+ with one 256-thread workgroup per block
+//      (HACApK_acaplus, m_HACApK_base.f90:1748-1946): pivot searches, norms
+//      and residual rows/columns are block-wide operations, the sequential
+//      pivot logic runs uniformly in every thread
+//   4. packing into per-precision HBM layouts for the products
+//
+// Product y = H x (replaces HACApK_adot_lattice_hyp at np=1) in two phases:
+//   phase 1  z_b = a1_b^T x for every low-rank block, rows of a1^T stored
+//            contiguously and read with 16-byte loads
+//   phase 2  row tiles of TR rows: every dense block and every a2 factor
+//            touching the tile is stored as TR x ncol column-major values, a
+//            lane owns a fixed set of rows and streams columns; work items
+//            are bounded runs of one tile's segments writing partial sums that
+//            are added in a fixed order -> deterministic, no atomics
+#include "hmat.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+namespace hbi {
+
+// ------------------------------------------------------------ entry eval --
+__global__ void entries_kernel(DevGeom g, int np, const int *__restrict__ ii, const int *__restrict__ jj,
+                               double *__restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) out[p] = entry(g, ii[p], jj[p]);
+}
+
+// dense leaves: leaf f occupies dense[off_f + il*nc + it] = K(lod[r0+il], lod[c0+it])
+struct DenseJob {
+    long long off, start;
+    int r0, nr, c0, nc;
+};
+__global__ void dense_fill_kernel(DevGeom g, const int *__restrict__ lod, const DenseJob *__restrict__ jobs, int njobs,
+                                  long long total, double *__restrict__ dense)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        int lo = 0, hi = njobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (jobs[mid].start <= e) lo = mid; else hi = mid - 1;
+        }
+        const DenseJob J = jobs[lo];
+        const long long loc = e - J.start;
+        const int il = (int)(loc / J.nc), it = (int)(loc % J.nc);
+        dense[J.off + loc] = entry(g, lod[J.r0 + il], lod[J.c0 + it]);
+    }
+}
+
+// ------------------------------------------------------------------ ACA+ --
+struct AcaJob {
+    int r0, nr, c0, nc, kcap, leaf;
+    long long oa, ob, ora, orb, om;
+    double znrm, aca_eps;
+};
+
+constexpr int kAcaThreads = 256;
+
+struct AcaBlk {
+    const DevGeom &g;
+    const int *lr, *lc;
+    int nr, nc;
+    double *zaa, *zab;
+    uint8_t *mr, *mc;
+    double *sv;
+    int *si;
+
+    // residual column j of the block (HACApK_calc_vec, ld=1)
+    __device__ void col(int j, int k, double *vec) const
+    {
+#pragma clang fp contract(off)
+        const int gj = lc[j];
+        for (int ii = threadIdx.x; ii < nr; ii += kAcaThreads) {
+            if (!mr[ii]) {
+                double e = entry(g, lr[ii], gj);
+                if (k > 0) {
+                    double zau = 0.0;
+                    for (int m = 0; m < k; m++) zau = zau + zaa[(size_t)m * nr + ii] * zab[(size_t)m * nc + j];
+                    e = e - zau;
+                }
+                vec[ii] = e;
+            } else if (k > 0) {
+                vec[ii] = 0.0;
+            }
+        }
+        __syncthreads();
+    }
+    // residual row i (HACApK_calc_vec, ld=0)
+    __device__ void row(int i, int k, double *vec) const
+    {
+#pragma clang fp contract(off)
+        const int gi = lr[i];
+        for (int jj = threadIdx.x; jj < nc; jj += kAcaThreads) {
+            if (!mc[jj]) {
+                double e = entry(g, gi, lc[jj]);
+                if (k > 0) {
+                    double zau = 0.0;
+                    for (int m = 0; m < k; m++) zau = zau + zab[(size_t)m * nc + jj] * zaa[(size_t)m * nr + i];
+                    e = e - zau;
+                }
+                vec[jj] = e;
+            } else if (k > 0) {
+                vec[jj] = 0.0;
+            }
+        }
+        __syncthreads();
+    }
+    // scaled 2-norm of HACApK_unrm_d (entries below 1e-30 ignored)
+    __device__ double unrm(const double *v, int n) const
+    {
+#pragma clang fp contract(off)
+        double m = 0.0;
+        for (int i = threadIdx.x; i < n; i += kAcaThreads) {
+            const double a = fabs(v[i]);
+            if (a >= 1.0e-30) m = fmax(m, a);
+        }
+        m = block_max<kAcaThreads>(m, sv);
+        if (m == 0.0) return 0.0;
+        double s = 0.0;
+        for (int i = threadIdx.x; i < n; i += kAcaThreads) {
+            const double a = fabs(v[i]);
+            if (!(a < 1.0e-30)) s = s + (a / m) * (a / m);
+        }
+        s = block_sum<kAcaThreads>(s, sv);
+        return m * sqrt(s);
+    }
+    // first index of max |v| (HACApK_maxabsvalloc_d) or min |v| (minabsvalloc)
+    template <bool kMax>
+    __device__ int argext(const double *v, int n, double &val) const
+    {
+        double best = kMax ? -1.0 : INFINITY;
+        int bi = -1;
+        for (int i = threadIdx.x; i < n; i += kAcaThreads) {
+            const double a = fabs(v[i]);
+            if (kMax ? (a > best) : (a < best)) { best = a; bi = i; }
+        }
+        block_argext<kAcaThreads, kMax>(best, bi, sv, si);
+        const double a0 = fabs(v[0]);
+        __syncthreads();
+        if (a0 != a0 || bi < 0) { val = a0; return 0; }
+        val = best;
+        return bi;
+    }
+    __device__ void copy(double *dst, const double *src, int n) const
+    {
+        for (int i = threadIdx.x; i < n; i += kAcaThreads) dst[i] = src[i];
+        __syncthreads();
+    }
+    __device__ void scale(double *v, int n, double s) const
+    {
+#pragma clang fp contract(off)
+        for (int i = threadIdx.x; i < n; i += kAcaThreads) v[i] = v[i] * s;
+        __syncthreads();
+    }
+    __device__ void axpy(double *y, const double *x, double a, int n) const
+    {
+#pragma clang fp contract(off)
+        for (int i = threadIdx.x; i < n; i += kAcaThreads) y[i] = y[i] + x[i] * a;
+        __syncthreads();
+    }
+    __device__ void set_mask(uint8_t *m, int i) const
+    {
+        __syncthreads();
+        if (threadIdx.x == 0) m[i] = 1;
+        __syncthreads();
+    }
+};
+
+__global__ __launch_bounds__(kAcaThreads) void aca_plus_kernel(DevGeom g, const int *__restrict__ lod,
+                                                               const AcaJob *__restrict__ jobs, double *ws,
+                                                               uint8_t *mws, int *kt_out, int *capped,
+                                                               double eps, int kmax)
+{
+#pragma clang fp contract(off)
+    __shared__ double sv[kAcaThreads / kWave];
+    __shared__ int si[kAcaThreads / kWave];
+    const AcaJob J = jobs[blockIdx.x];
+    const int nr = J.nr, nc = J.nc;
+    AcaBlk B{g, lod + J.r0, lod + J.c0, nr, nc, ws + J.oa, ws + J.ob, mws + J.om, mws + J.om + nr, sv, si};
+    double *pa = ws + J.ora, *pb = ws + J.orb;
+    const double ACA_EPS = J.aca_eps, znrm = J.znrm;
+    const double za_eps = (double)1.0e-30f, piv_tiny = (double)1.0e-20f;
+    const int klim = min(kmax, J.kcap);
+    int ntries = max(nr, nc) + 1, ntr = 6, ntc = 6;
+    int k = 0, jref = 0;
+    B.col(jref, 0, pa);
+    double colnorm = B.unrm(pa, nr);
+    double rownorm;
+    int iref = B.argext<false>(pa, nr, rownorm);
+    B.row(iref, 0, pb);
+    rownorm = B.unrm(pb, nc);
+    double apx = 0.0;
+    int lstop = 0, i = 0, j = 0;
+    bool broke = false;
+    while (k < klim && (ntr > 0 || ntc > 0) && ntries > 0) {
+        ntries--;
+        double *pcol = B.zaa + (size_t)k * nr, *prow = B.zab + (size_t)k * nc;
+        double cmax, rmax;
+        i = B.argext<true>(pa, nr, cmax);
+        j = B.argext<true>(pb, nc, rmax);
+        if (rmax > cmax) {
+            if (j != jref) B.col(j, k, pcol); else B.copy(pcol, pa, nr);
+            i = B.argext<true>(pcol, nr, cmax);
+            if (cmax < ACA_EPS && k >= 1) {
+                lstop = 1;
+            } else {
+                B.row(i, k, prow);
+                const double piv = pcol[i];
+                if (fabs(piv) > piv_tiny) {
+                    B.scale(pcol, nr, 1.0 / piv);
+                } else {
+                    k = max(k - 1, 0);
+                    broke = true;
+                    break;
+                }
+            }
+        } else {
+            if (i != iref) B.row(i, k, prow); else B.copy(prow, pb, nc);
+            j = B.argext<true>(prow, nc, rmax);
+            if (rmax < ACA_EPS && k >= 1) {
+                lstop = 1;
+            } else {
+                B.col(j, k, pcol);
+                const double piv = prow[j];
+                if (fabs(piv) > piv_tiny) {
+                    B.scale(prow, nc, 1.0 / piv);
+                } else {
+                    k = max(k - 1, 0);
+                    broke = true;
+                    break;
+                }
+            }
+        }
+        B.set_mask(B.mr, i);
+        B.set_mask(B.mc, j);
+        if (i != iref) {
+            const double zinv = -pcol[iref];
+            B.axpy(pb, prow, zinv, nc);
+            rownorm = B.unrm(pb, nc);
+        }
+        if (i == iref || rownorm < ACA_EPS) {
+            if (i == iref) ntr++;
+            if (ntr > 0) {
+                rownorm = 0.0;
+                i = iref;
+                const int stop = (iref - 1 + nr) % nr;
+                while (i != stop && rownorm < za_eps && ntr > 0) {
+                    if (B.mr[i] == 0) {
+                        B.row(i, k + 1, pb);
+                        rownorm = B.unrm(pb, nc);
+                        if (rownorm < ACA_EPS) B.set_mask(B.mr, i);
+                        ntr--;
+                    } else {
+                        rownorm = 0.0;
+                    }
+                    i = (i + 1) % nr;
+                }
+                iref = (i - 1 + nr) % nr;
+            }
+        }
+        if (j != jref) {
+            const double zinv = -prow[jref];
+            B.axpy(pa, pcol, zinv, nr);
+            colnorm = B.unrm(pa, nr);
+        }
+        if (j == jref || colnorm < ACA_EPS) {
+            if (j == jref) ntc++;
+            if (ntc > 0) {
+                colnorm = 0.0;
+                j = jref;
+                const int stop = (jref - 1 + nc) % nc;
+                while (j != stop && colnorm < za_eps && ntc > 0) {
+                    if (B.mc[j] == 0) {
+                        B.col(j, k + 1, pa);
+                        colnorm = B.unrm(pa, nr);
+                        if (colnorm < ACA_EPS) B.set_mask(B.mc, j);
+                        ntc--;
+                    } else {
+                        colnorm = 0.0;
+                    }
+                    j = (j + 1) % nc;
+                }
+                jref = (j - 1 + nc) % nc;
+            }
+        }
+        if (colnorm < ACA_EPS && rownorm < ACA_EPS && k >= 1) {
+            lstop = 1;
+            k = k + 1;
+        }
+        if (lstop == 0) {
+            const double bn = B.unrm(pcol, nr) * B.unrm(prow, nc);
+            if (k == 0) {
+                apx = znrm;
+            } else if (bn < apx * eps && rownorm < apx * eps && colnorm < apx * eps && k >= 1) {
+                lstop = 1;
+            }
+        }
+        if (lstop == 1 && k >= 1) {
+            broke = true;
+            break;
+        }
+        k = k + 1;
+    }
+    if (threadIdx.x == 0) {
+        kt_out[blockIdx.x] = k;
+        capped[blockIdx.x] = (!broke && k >= J.kcap && J.kcap < kmax) ? 1 : 0;
+    }
+}
+
+// ------------------------------------------------------------------ pack --
+__device__ __forceinline__ void store_as(uint8_t *p, int dtype, double v)
+{
+    if (dtype == kF64) *(double *)p = v;
+    else if (dtype == kF32) *(float *)p = (float)v;
+    else *(uint16_t *)p = f32_to_bf16_rne((float)v);
+}
+
+__global__ void pack_kernel(const CopyJob *__restrict__ jobs, int njobs, long long total, const double *__restrict__ s0,
+                            const double *__restrict__ s1, uint8_t *__restrict__ dst)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        int lo = 0, hi = njobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (jobs[mid].start <= e) lo = mid; else hi = mid - 1;
+        }
+        const CopyJob J = jobs[lo];
+        const long long loc = e - J.start;
+        const int r = (int)(loc % J.rows), c = (int)(loc / J.rows);
+        double v = 0.0;
+        if (r >= J.r_lo && r < J.r_hi && c < J.c_hi) {
+            const double *s = J.which ? s1 : s0;
+            v = s[J.src + (long long)(r - J.r_lo) * J.src_rs + (long long)c * J.src_cs];
+        }
+        const int sz = J.dtype == kF64 ? 8 : (J.dtype == kF32 ? 4 : 2);
+        store_as(dst + J.dst + ((long long)r * J.dst_rs + (long long)c * J.dst_cs) * sz, J.dtype, v);
+    }
+}
+
+// Frobenius norms of both factors of every low-rank leaf (for the BF16 rule)
+__global__ void lr_norms_kernel(const double *__restrict__ lr, const long long *__restrict__ off,
+                                const int *__restrict__ len1, const int *__restrict__ len2, double *__restrict__ out)
+{
+    __shared__ double sv[4];
+    const int b = blockIdx.x;
+    const double *a1 = lr + off[b], *a2 = a1 + len1[b];
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = threadIdx.x; i < len1[b]; i += 256) s1 += a1[i] * a1[i];
+    for (int i = threadIdx.x; i < len2[b]; i += 256) s2 += a2[i] * a2[i];
+    s1 = block_sum<256>(s1, sv);
+    s2 = block_sum<256>(s2, sv);
+    if (threadIdx.x == 0) { out[2 * b] = sqrt(s1); out[2 * b + 1] = sqrt(s2); }
+}
+
+// --------------------------------------------------------------- matvec --
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef uint32_t uvec4 __attribute__((ext_vector_type(4)));
+
+// one 16-byte chunk of packed factors, widened to FP64 (streamed once per product)
+template <typename T> struct Chunk;
+template <> struct Chunk<double> {
+    static constexpr int V = 2;
+    __device__ static void load(const uint8_t *p, double *o)
+    {
+        const dvec2 v = __builtin_nontemporal_load((const dvec2 *)p);
+        o[0] = v.x; o[1] = v.y;
+    }
+};
+template <> struct Chunk<float> {
+    static constexpr int V = 4;
+    __device__ static void load(const uint8_t *p, double *o)
+    {
+        const fvec4 v = __builtin_nontemporal_load((const fvec4 *)p);
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    }
+};
+template <> struct Chunk<uint16_t> {
+    static constexpr int V = 8;
+    __device__ static void load(const uint8_t *p, double *o)
+    {
+        const uvec4 v = __builtin_nontemporal_load((const uvec4 *)p);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            o[2 * q] = __uint_as_float(w[q] << 16);
+            o[2 * q + 1] = __uint_as_float(w[q] & 0xffff0000u);
+        }
+    }
+};
+
+constexpr int kMvThreads = 256;
+constexpr int kUnroll = 4;
+
+// phase 1: z[zoff + k] = sum_t a1T[k][t] x[xoff + t]
+template <typename T>
+__device__ __forceinline__ void lrvt_item(const Item1 &it, const uint8_t *__restrict__ buf, const double *__restrict__ x,
+                                          double *__restrict__ z, double *lds)
+{
+    constexpr int V = Chunk<T>::V;
+    constexpr int KG = 4;
+    const int tid = threadIdx.x;
+    for (int k0 = 0; k0 < it.nk; k0 += KG) {
+        double acc[KG];
+#pragma unroll
+        for (int q = 0; q < KG; q++) acc[q] = 0.0;
+        for (int t = tid * V; t < it.ldt; t += kMvThreads * V) {
+            double xv[V];
+#pragma unroll
+            for (int e = 0; e < V; e++) xv[e] = (t + e < it.nc) ? x[it.xoff + t + e] : 0.0;
+#pragma unroll
+            for (int q = 0; q < KG; q++) {
+                if (k0 + q < it.nk) {
+                    double a[V];
+                    Chunk<T>::load(buf + it.off + ((size_t)(k0 + q) * it.ldt + t) * sizeof(T), a);
+#pragma unroll
+                    for (int e = 0; e < V; e++) acc[q] = fma(a[e], xv[e], acc[q]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < KG; q++) acc[q] = wave_sum(acc[q]);
+        const int w = tid / kWave, l = tid % kWave;
+        __syncthreads();
+        if (l == 0)
+#pragma unroll
+            for (int q = 0; q < KG; q++) lds[q * 4 + w] = acc[q];
+        __syncthreads();
+        if (tid < KG && k0 + tid < it.nk) {
+            double s = 0.0;
+            for (int ww = 0; ww < kMvThreads / kWave; ww++) s += lds[tid * 4 + ww];
+            z[it.zoff + k0 + tid] = s;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kMvThreads) void lrvt_kernel(const Item1 *__restrict__ items, const uint8_t *__restrict__ buf,
+                                                          const double *__restrict__ x, double *__restrict__ z,
+                                                          const int *__restrict__ gate)
+{
+    __shared__ double lds[16];
+    if (gate && *gate) return;
+    const Item1 it = items[blockIdx.x];
+    if (it.dtype == kF64) lrvt_item<double>(it, buf, x, z, lds);
+    else if (it.dtype == kF32) lrvt_item<float>(it, buf, x, z, lds);
+    else lrvt_item<uint16_t>(it, buf, x, z, lds);
+}
+
+// phase 2: partial y of one work item
+template <typename T, int TR>
+__device__ __forceinline__ void tile_item(const Item2 &it, const Seg *__restrict__ segs, const uint8_t *__restrict__ buf,
+                                          const double *__restrict__ x, const double *__restrict__ z,
+                                          double *__restrict__ ypart, double *lds)
+{
+    constexpr int V = Chunk<T>::V;
+    constexpr int LPC = TR / V;        // lanes per column
+    constexpr int CPI = kWave / LPC;   // columns per wave load
+    constexpr int NW = kMvThreads / kWave;
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    const int rsub = lane % LPC, csub = lane / LPC;
+    double acc[V];
+#pragma unroll
+    for (int e = 0; e < V; e++) acc[e] = 0.0;
+    for (int s = it.seg0; s < it.seg1; s++) {
+        const Seg sg = segs[s];
+        const uint8_t *base = buf + sg.off + (size_t)rsub * V * sizeof(T);
+        const double *xin = (sg.src ? z : x) + sg.xoff;
+        const int ngroups = (sg.ncol + CPI - 1) / CPI;
+        for (int g0 = w; g0 < ngroups; g0 += NW * kUnroll) {
+            double a[kUnroll][V];
+            double xv[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int g = g0 + u * NW;
+                const int c = g * CPI + csub;
+                if (g < ngroups && c < sg.ncol) {
+                    Chunk<T>::load(base + (size_t)c * TR * sizeof(T), a[u]);
+                    xv[u] = xin[c];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; e++) a[u][e] = 0.0;
+                    xv[u] = 0.0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+#pragma unroll
+                for (int e = 0; e < V; e++) acc[e] = fma(a[u][e], xv[u], acc[e]);
+        }
+    }
+    // lanes with the same rsub hold the same rows: add them in a fixed order
+#pragma unroll
+    for (int e = 0; e < V; e++) lds[(w * kWave + lane) * V + e] = acc[e];
+    __syncthreads();
+    for (int r = threadIdx.x; r < TR; r += kMvThreads) {
+        const int rs = r / V, e = r % V;
+        double s = 0.0;
+        for (int ww = 0; ww < NW; ww++)
+            for (int cs = 0; cs < CPI; cs++) s += lds[(ww * kWave + cs * LPC + rs) * V + e];
+        ypart[(size_t)it.part * TR + r] = s;
+    }
+}
+
+template <int TR>
+__global__ __launch_bounds__(kMvThreads) void tile_kernel(const Item2 *__restrict__ items, const Seg *__restrict__ segs,
+                                                          const uint8_t *__restrict__ buf, const double *__restrict__ x,
+                                                          const double *__restrict__ z, double *__restrict__ ypart,
+                                                          const int *__restrict__ gate)
+{
+    __shared__ double lds[kMvThreads * 8];
+    if (gate && *gate) return;
+    const Item2 it = items[blockIdx.x];
+    if (it.dtype == kF64) tile_item<double, TR>(it, segs, buf, x, z, ypart, lds);
+    else if (it.dtype == kF32) tile_item<float, TR>(it, segs, buf, x, z, ypart, lds);
+    else tile_item<uint16_t, TR>(it, segs, buf, x, z, ypart, lds);
+}
+
+__global__ void tile_reduce_kernel(int n, int tr, const int *__restrict__ part0, const double *__restrict__ ypart,
+                                   double *__restrict__ y)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int t = r / tr, rr = r % tr;
+    double s = 0.0;
+    for (int p = part0[t]; p < part0[t + 1]; p++) s += ypart[(size_t)p * tr + rr];
+    y[r] = s;
+}
+
+// ------------------------------------------------------------ host side --
+static int dtype_size(int d) { return d == kF64 ? 8 : (d == kF32 ? 4 : 2); }
+static inline size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
+
+static void build_plan(hbi_hmat *h, int prec)
+{
+    Plan &P = h->plan[prec];
+    P = Plan();
+    const int TR = h->tr;
+    const int nt = h->ntiles;
+    const auto &L = h->part.leaves;
+    std::vector<CopyJob> jobs;
+    long long elems = 0;
+    size_t bytes = 0;
+    auto leaf_dtype = [&](int ip) -> int {
+        if (prec == HBI_PREC_F64) return kF64;
+        if (prec == HBI_PREC_MIXED && L[ip].ltmtx == 1 && h->bf16_ok[ip]) return kBF16;
+        return kF32;
+    };
+    // ---- phase 1 items
+    for (int ip = 0; ip < (int)L.size(); ip++) {
+        const Leaf &f = L[ip];
+        if (f.ltmtx != 1 || f.kt == 0) continue;
+        const int dt = leaf_dtype(ip), sz = dtype_size(dt);
+        const int ldt = (f.nc + 7) / 8 * 8;
+        const int rows_per = std::max(1, std::min(f.kt, (int)(h->chunk_bytes / ((size_t)ldt * sz))));
+        const size_t off0 = bytes;
+        CopyJob J{};
+        J.src = h->fac_off[ip]; J.dst = (long long)off0; J.start = elems; J.which = 1;
+        J.rows = f.kt; J.cols = ldt; J.src_rs = f.nc; J.src_cs = 1; J.r_lo = 0; J.r_hi = f.kt; J.c_hi = f.nc;
+        J.dst_rs = ldt; J.dst_cs = 1; J.dtype = dt;
+        jobs.push_back(J);
+        elems += (long long)f.kt * ldt;
+        bytes = align16(bytes + (size_t)f.kt * ldt * sz);
+        for (int k0 = 0; k0 < f.kt; k0 += rows_per) {
+            Item1 it{};
+            it.off = off0 + (size_t)k0 * ldt * sz;
+            it.ldt = ldt; it.nc = f.nc; it.nk = std::min(rows_per, f.kt - k0);
+            it.xoff = f.c0; it.zoff = h->zoff[ip] + k0; it.dtype = dt;
+            P.it1.push_back(it);
+        }
+        P.alg_bytes += (long long)f.nc * f.kt * sz;
+    }
+    // ---- phase 2 segments per tile
+    std::vector<std::vector<int>> tl(nt);
+    for (int ip = 0; ip < (int)L.size(); ip++) {
+        const Leaf &f = L[ip];
+        if (f.ltmtx == 1 && f.kt == 0) continue;
+        for (int t = f.r0 / TR; t <= (f.r0 + f.nr - 1) / TR; t++) tl[t].push_back(ip);
+    }
+    P.tile_part0.assign(nt + 1, 0);
+    for (int t = 0; t < nt; t++) {
+        P.tile_part0[t] = P.nparts;
+        const int T0 = t * TR;
+        for (int dt = 0; dt < 3; dt++) {
+            size_t cur = 0;
+            int seg_begin = (int)P.segs.size();
+            bool any = false;
+            for (int ip : tl[t]) {
+                if (leaf_dtype(ip) != dt) continue;
+                const Leaf &f = L[ip];
+                const int sz = dtype_size(dt);
+                const int lo = std::max(f.r0, T0), hi = std::min(f.r0 + f.nr, T0 + TR);
+                const int ncol = f.ltmtx == 1 ? f.kt : f.nc;
+                const size_t segbytes = (size_t)TR * ncol * sz;
+                if (any && cur + segbytes > (size_t)h->chunk_bytes) {
+                    P.it2.push_back({t, seg_begin, (int)P.segs.size(), P.nparts++, dt, 0});
+                    seg_begin = (int)P.segs.size();
+                    cur = 0;
+                }
+                Seg sg{};
+                sg.off = bytes; sg.ncol = ncol; sg.dtype = dt;
+                CopyJob J{};
+                J.dst = (long long)bytes; J.start = elems; J.rows = TR; J.cols = ncol;
+                J.r_lo = lo - T0; J.r_hi = hi - T0; J.c_hi = ncol; J.dst_rs = 1; J.dst_cs = TR; J.dtype = dt;
+                if (f.ltmtx == 1) {
+                    sg.src = 1; sg.xoff = h->zoff[ip];
+                    J.which = 1; J.src = h->fac_off[ip] + (long long)f.nc * f.kt + (lo - f.r0);
+                    J.src_rs = 1; J.src_cs = f.nr;
+                    P.alg_bytes += (long long)(hi - lo) * f.kt * sz;
+                } else {
+                    sg.src = 0; sg.xoff = f.c0;
+                    J.which = 0; J.src = h->fac_off[ip] + (long long)(lo - f.r0) * f.nc;
+                    J.src_rs = f.nc; J.src_cs = 1;
+                    P.alg_bytes += (long long)(hi - lo) * f.nc * sz;
+                }
+                jobs.push_back(J);
+                elems += (long long)TR * ncol;
+                bytes = align16(bytes + segbytes);
+                P.segs.push_back(sg);
+                cur += segbytes;
+                any = true;
+            }
+            if (any) P.it2.push_back({t, seg_begin, (int)P.segs.size(), P.nparts++, dt, 0});
+        }
+    }
+    P.tile_part0[nt] = P.nparts;
+    P.alg_bytes += 2LL * h->n * 8;   // x read, y written (FP64)
+    for (int ip = 0; ip < (int)L.size(); ip++)
+        if (leaf_dtype(ip) == kBF16) P.nbf16++;
+    // largest items first for load balance
+    std::stable_sort(P.it1.begin(), P.it1.end(), [](const Item1 &a, const Item1 &b) {
+        return (long long)a.nk * a.ldt * dtype_size(a.dtype) > (long long)b.nk * b.ldt * dtype_size(b.dtype);
+    });
+    // pack on the device
+    P.bytes = bytes;
+    P.buf.alloc(std::max<size_t>(bytes, 16));
+    if (!jobs.empty()) {
+        DevBuf<CopyJob> dj;
+        dj.upload(jobs, h->stream);
+        const int grid = (int)std::min<long long>((elems + 255) / 256, 65536);
+        pack_kernel<<<grid, 256, 0, h->stream>>>(dj.p, (int)jobs.size(), elems, h->d_dense.p, h->d_lr.p, P.buf.p);
+        HBI_HIP(hipGetLastError());
+        HBI_HIP(hipStreamSynchronize(h->stream));
+    }
+    P.d_segs.upload(P.segs, h->stream);
+    P.d_it1.upload(P.it1, h->stream);
+    P.d_it2.upload(P.it2, h->stream);
+    P.d_tile_part0.upload(P.tile_part0, h->stream);
+    HBI_HIP(hipStreamSynchronize(h->stream));
+    P.built = true;
+}
+
+const Plan &hmat_plan(hbi_hmat *h, int prec)
+{
+    HBI_REQUIRE(prec >= 0 && prec <= 2, "precision must be HBI_PREC_F64, _F32 or _MIXED");
+    if (!h->plan[prec].built) build_plan(h, prec);
+    return h->plan[prec];
+}
+
+void hmat_phase1(hbi_hmat *h, int prec, const double *x, double *z, hipStream_t s, const int *gate)
+{
+    const Plan &P = hmat_plan(h, prec);
+    if (!P.it1.empty())
+        lrvt_kernel<<<(int)P.it1.size(), kMvThreads, 0, s>>>(P.d_it1.p, P.buf.p, x, z, gate);
+}
+
+void hmat_phase2(hbi_hmat *h, int prec, const double *x, const double *z, double *ypart, hipStream_t s,
+                 const int *gate)
+{
+    const Plan &P = hmat_plan(h, prec);
+    if (P.it2.empty()) return;
+    if (h->tr == 64)
+        tile_kernel<64><<<(int)P.it2.size(), kMvThreads, 0, s>>>(P.d_it2.p, P.d_segs.p, P.buf.p, x, z, ypart, gate);
+    else
+        tile_kernel<32><<<(int)P.it2.size(), kMvThreads, 0, s>>>(P.d_it2.p, P.d_segs.p, P.buf.p, x, z, ypart, gate);
+}
+
+void hmat_apply(hbi_hmat *h, int prec, const double *x, double *y)
+{
+    const Plan &P = hmat_plan(h, prec);
+    if ((int)h->d_ypart.n < std::max(P.nparts, 1) * h->tr) h->d_ypart.alloc((size_t)std::max(P.nparts, 1) * h->tr);
+    hmat_phase1(h, prec, x, h->d_z.p, h->stream, nullptr);
+    hmat_phase2(h, prec, x, h->d_z.p, h->d_ypart.p, h->stream, nullptr);
+    tile_reduce_kernel<<<(h->n + 255) / 256, 256, 0, h->stream>>>(h->n, h->tr, P.d_tile_part0.p, h->d_ypart.p, y);
+}
+
+// ------------------------------------------------------------- geometry --
+static void upload_geom(hbi_hmat *h, const hbi_geom *g)
+{
+    const double *src[28] = {g->xcol, g->ycol, g->zcol, g->xel, g->xer, g->ang, g->angd, g->dsl, g->dsd, g->rake,
+                             g->xs1, g->xs2, g->xs3, g->ys1, g->ys2, g->ys3, g->zs1, g->zs2, g->zs3,
+                             g->ev11, g->ev12, g->ev13, g->ev21, g->ev22, g->ev23, g->ev31, g->ev32, g->ev33};
+    const double **dst[28] = {&h->dgeo.xcol, &h->dgeo.ycol, &h->dgeo.zcol, &h->dgeo.xel, &h->dgeo.xer, &h->dgeo.ang,
+                              &h->dgeo.angd, &h->dgeo.dsl, &h->dgeo.dsd, &h->dgeo.rake, &h->dgeo.xs1, &h->dgeo.xs2,
+                              &h->dgeo.xs3, &h->dgeo.ys1, &h->dgeo.ys2, &h->dgeo.ys3, &h->dgeo.zs1, &h->dgeo.zs2,
+                              &h->dgeo.zs3, &h->dgeo.ev11, &h->dgeo.ev12, &h->dgeo.ev13, &h->dgeo.ev21,
+                              &h->dgeo.ev22, &h->dgeo.ev23, &h->dgeo.ev31, &h->dgeo.ev32, &h->dgeo.ev33};
+    h->geo.clear();
+    h->geo.reserve(28);
+    for (int k = 0; k < 28; k++) {
+        *dst[k] = nullptr;
+        if (!src[k]) continue;
+        h->geo.emplace_back(g->n);
+        h->geo.back().upload(src[k], g->n, h->stream);
+        *dst[k] = h->geo.back().p;
+    }
+    h->dgeo.kind = g->kind;
+    h->dgeo.n = g->n;
+    h->dgeo.vcomp = g->vcomp;
+    h->dgeo.md = g->md;
+    h->dgeo.w = g->w;
+}
+
+static void check_geom(const hbi_geom *g)
+{
+    HBI_REQUIRE(g && g->n > 0, "geometry with n > 0 required");
+    HBI_REQUIRE(g->kind >= 1 && g->kind <= 4, "unknown geometry kind");
+    HBI_REQUIRE(g->xcol, "xcol required");
+    if (g->kind == HBI_KIND_2DVS) HBI_REQUIRE(g->xel && g->xer, "2dvs needs xel, xer");
+    if (g->kind == HBI_KIND_OKADA_HS || g->kind == HBI_KIND_OKADA_FS)
+        HBI_REQUIRE(g->ycol && g->zcol && g->ang && g->angd && g->dsl && g->dsd && g->rake && g->w > 0,
+                    "okada kernels need ycol, zcol, ang, angd, dsl, dsd, rake and w > 0");
+    if (g->kind == HBI_KIND_TRI_FS)
+        HBI_REQUIRE(g->ycol && g->zcol && g->xs1 && g->xs2 && g->xs3 && g->ys1 && g->ys2 && g->ys3 && g->zs1 &&
+                        g->zs2 && g->zs3 && g->ev11 && g->ev12 && g->ev13 && g->ev21 && g->ev22 && g->ev23 &&
+                        g->ev31 && g->ev32 && g->ev33 && g->rake,
+                    "triangle kernel needs vertices, frames and rake");
+}
+
+static double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+static void build(hbi_hmat *h, const hbi_geom *g, const hbi_hmat_opts *o)
+{
+    check_geom(g);
+    h->opts = *o;
+    if (h->opts.leaf <= 0) h->opts.leaf = 15;
+    if (h->opts.eta <= 0) h->opts.eta = 2.0;
+    if (h->opts.kmax <= 0) h->opts.kmax = 1000;
+    HBI_REQUIRE(h->opts.eps_aca > 0, "eps_aca must be > 0");
+    h->n = g->n;
+    h->tr = h->opts.tile_rows > 0 ? h->opts.tile_rows : (h->opts.leaf <= 32 ? 32 : 64);
+    HBI_REQUIRE(h->tr == 32 || h->tr == 64, "tile_rows must be 32 or 64");
+    h->chunk_bytes = (h->opts.chunk_kb > 0 ? h->opts.chunk_kb : 64) * 1024;
+    h->ntiles = (h->n + h->tr - 1) / h->tr;
+    HBI_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    for (auto &e : h->ev) HBI_HIP(hipEventCreate(&e));
+    double t0 = now();
+    h->part = build_partition(g->n, g->xcol, g->ycol, g->zcol, h->opts.leaf, h->opts.eta);
+    h->st.t_partition = now() - t0;
+    upload_geom(h, g);
+    h->d_lod.upload(h->part.lod, h->stream);
+    auto &L = h->part.leaves;
+    const int nl = (int)L.size();
+    h->fac_off.assign(nl, -1);
+    h->zoff.assign(nl, -1);
+    h->bf16_ok.assign(nl, 0);
+
+    // ---- dense leaves
+    t0 = now();
+    std::vector<DenseJob> dj;
+    long long dtot = 0;
+    for (int ip = 0; ip < nl; ip++) {
+        if (L[ip].ltmtx != 2) continue;
+        h->fac_off[ip] = dtot;
+        dj.push_back({dtot, dtot, L[ip].r0, L[ip].nr, L[ip].c0, L[ip].nc});
+        dtot += (long long)L[ip].nr * L[ip].nc;
+    }
+    h->d_dense.alloc(std::max<long long>(dtot, 1));
+    if (!dj.empty()) {
+        DevBuf<DenseJob> ddj;
+        ddj.upload(dj, h->stream);
+        const int grid = (int)std::min<long long>((dtot + 255) / 256, 1 << 20);
+        dense_fill_kernel<<<grid, 256, 0, h->stream>>>(h->dgeo, h->d_lod.p, ddj.p, (int)dj.size(), dtot, h->d_dense.p);
+        HBI_HIP(hipGetLastError());
+        HBI_HIP(hipStreamSynchronize(h->stream));
+    }
+    h->st.t_dense = now() - t0;
+
+    // ---- low-rank leaves: ACA+, growing the rank capacity where a block reaches it
+    t0 = now();
+    std::vector<int> lr_ids;
+    for (int ip = 0; ip < nl; ip++)
+        if (L[ip].ltmtx == 1) lr_ids.push_back(ip);
+    const double znrmmat = (double)(1.0f / (float)h->n);
+    const double paca = (double)1.0e-6f * h->opts.eps_aca;
+    const int kmax = h->opts.kmax;
+    std::vector<int> kt_all(nl, 0);
+    // per pass: leaves still to do and their capacity
+    std::vector<int> todo = lr_ids;
+    int cap0 = 64;
+    struct Keep { DevBuf<double> ws; std::vector<AcaJob> jobs; std::vector<int> kt; std::vector<int> capped; };
+    std::vector<Keep> passes;
+    std::vector<int> pass_of(nl, -1), job_of(nl, -1);
+    while (!todo.empty()) {
+        // biggest blocks first
+        std::sort(todo.begin(), todo.end(), [&](int a, int b) {
+            return (long long)L[a].nr * L[a].nc > (long long)L[b].nr * L[b].nc;
+        });
+        Keep K;
+        long long wtot = 0, mtot = 0;
+        for (int ip : todo) {
+            const Leaf &f = L[ip];
+            AcaJob J{};
+            J.r0 = f.r0; J.nr = f.nr; J.c0 = f.c0; J.nc = f.nc; J.leaf = ip;
+            J.kcap = std::min({cap0, kmax, std::min(f.nr, f.nc) + 1});
+            J.oa = wtot; wtot += (long long)f.nr * J.kcap;
+            J.ob = wtot; wtot += (long long)f.nc * J.kcap;
+            J.ora = wtot; wtot += f.nr;
+            J.orb = wtot; wtot += f.nc;
+            J.om = mtot; mtot += f.nr + f.nc;
+            J.znrm = znrmmat * (double)sqrtf((float)f.nr * (float)f.nc);
+            J.aca_eps = paca * J.znrm;
+            K.jobs.push_back(J);
+        }
+        K.ws.alloc(std::max<long long>(wtot, 1));
+        DevBuf<uint8_t> masks(std::max<long long>(mtot, 1));
+        HBI_HIP(hipMemsetAsync(K.ws.p, 0, K.ws.n * sizeof(double), h->stream));
+        HBI_HIP(hipMemsetAsync(masks.p, 0, masks.n, h->stream));
+        DevBuf<AcaJob> djobs;
+        djobs.upload(K.jobs, h->stream);
+        DevBuf<int> dkt(K.jobs.size()), dcap(K.jobs.size());
+        aca_plus_kernel<<<(int)K.jobs.size(), kAcaThreads, 0, h->stream>>>(h->dgeo, h->d_lod.p, djobs.p, K.ws.p, masks.p,
+                                                                            dkt.p, dcap.p, h->opts.eps_aca, kmax);
+        HBI_HIP(hipGetLastError());
+        K.kt.resize(K.jobs.size());
+        K.capped.resize(K.jobs.size());
+        dkt.download(K.kt.data(), K.kt.size(), h->stream);
+        dcap.download(K.capped.data(), K.capped.size(), h->stream);
+        HBI_HIP(hipStreamSynchronize(h->stream));
+        std::vector<int> next;
+        const int pidx = (int)passes.size();
+        for (size_t q = 0; q < K.jobs.size(); q++) {
+            const int ip = K.jobs[q].leaf;
+            const int bound = std::min(kmax, std::min(L[ip].nr, L[ip].nc) + 1);
+            if (K.capped[q] && std::min(cap0 * 4, bound) > K.jobs[q].kcap) { next.push_back(ip); continue; }
+            kt_all[ip] = K.kt[q];
+            pass_of[ip] = pidx;
+            job_of[ip] = (int)q;
+        }
+        passes.push_back(std::move(K));
+        todo.swap(next);
+        cap0 *= 4;
+    }
+    // compact FP64 factors: a1 (nc x kt) then a2 (nr x kt) per leaf
+    long long ltot = 0;
+    int zt = 0;
+    for (int ip : lr_ids) {
+        L[ip].kt = kt_all[ip];
+        h->fac_off[ip] = ltot;
+        ltot += (long long)(L[ip].nr + L[ip].nc) * L[ip].kt;
+        h->zoff[ip] = zt;
+        zt += L[ip].kt;
+    }
+    h->nz = zt;
+    h->d_lr.alloc(std::max<long long>(ltot, 1));
+    for (size_t pi = 0; pi < passes.size(); pi++) {
+        std::vector<CopyJob> cj;
+        long long el = 0;
+        for (int ip : lr_ids) {
+            if (pass_of[ip] != (int)pi || L[ip].kt == 0) continue;
+            const AcaJob &J = passes[pi].jobs[job_of[ip]];
+            const int kt = L[ip].kt;
+            CopyJob a1{}, a2{};
+            a1.src = J.ob; a1.dst = h->fac_off[ip] * 8; a1.start = el; a1.which = 0; a1.rows = J.nc * kt; a1.cols = 1;
+            a1.src_rs = 1; a1.src_cs = 0; a1.r_lo = 0; a1.r_hi = a1.rows; a1.c_hi = 1; a1.dst_rs = 1; a1.dst_cs = 0;
+            a1.dtype = kF64;
+            el += a1.rows;
+            a2 = a1;
+            a2.src = J.oa; a2.dst = (h->fac_off[ip] + (long long)J.nc * kt) * 8; a2.start = el; a2.rows = J.nr * kt;
+            a2.r_hi = a2.rows;
+            el += a2.rows;
+            cj.push_back(a1);
+            cj.push_back(a2);
+        }
+        if (cj.empty()) continue;
+        DevBuf<CopyJob> dcj;
+        dcj.upload(cj, h->stream);
+        const int grid = (int)std::min<long long>((el + 255) / 256, 65536);
+        pack_kernel<<<grid, 256, 0, h->stream>>>(dcj.p, (int)cj.size(), el, passes[pi].ws.p, passes[pi].ws.p,
+                                                  (uint8_t *)h->d_lr.p);
+        HBI_HIP(hipGetLastError());
+        HBI_HIP(hipStreamSynchronize(h->stream));
+    }
+    for (auto &p : passes) {
+        for (size_t q = 0; q < p.jobs.size(); q++)
+            if (p.kt[q] >= kmax) h->st.ncapped++;
+    }
+    passes.clear();
+    h->st.t_aca = now() - t0;
+
+    // ---- BF16 eligibility of low-rank blocks: 2 u ||a2||_F ||a1||_F <= margin * eps * znrm
+    if (h->opts.bf16_margin > 0 && !lr_ids.empty()) {
+        std::vector<long long> off;
+        std::vector<int> l1, l2;
+        for (int ip : lr_ids) {
+            off.push_back(h->fac_off[ip]);
+            l1.push_back(L[ip].nc * L[ip].kt);
+            l2.push_back(L[ip].nr * L[ip].kt);
+        }
+        DevBuf<long long> doff;
+        DevBuf<int> dl1, dl2;
+        doff.upload(off, h->stream);
+        dl1.upload(l1, h->stream);
+        dl2.upload(l2, h->stream);
+        DevBuf<double> dn(2 * lr_ids.size());
+        lr_norms_kernel<<<(int)lr_ids.size(), 256, 0, h->stream>>>(h->d_lr.p, doff.p, dl1.p, dl2.p, dn.p);
+        HBI_HIP(hipGetLastError());
+        std::vector<double> nn(2 * lr_ids.size());
+        dn.download(nn.data(), nn.size(), h->stream);
+        HBI_HIP(hipStreamSynchronize(h->stream));
+        const double u = std::ldexp(1.0, -8);
+        for (size_t q = 0; q < lr_ids.size(); q++) {
+            const Leaf &f = L[lr_ids[q]];
+            const double znrm = znrmmat * (double)sqrtf((float)f.nr * (float)f.nc);
+            if (2.0 * u * nn[2 * q] * nn[2 * q + 1] <= h->opts.bf16_margin * h->opts.eps_aca * znrm)
+                h->bf16_ok[lr_ids[q]] = 1;
+        }
+    }
+
+    // ---- vectors and packed plans
+    h->d_x.alloc(h->n + 8);
+    h->d_y.alloc(h->n + 8);
+    h->d_z.alloc(std::max(h->nz, 1) + 8);
+    HBI_HIP(hipMemsetAsync(h->d_x.p, 0, h->d_x.n * 8, h->stream));
+    HBI_HIP(hipMemsetAsync(h->d_z.p, 0, h->d_z.n * 8, h->stream));
+    t0 = now();
+    for (int p = 0; p < 3; p++) hmat_plan(h, p);
+    h->st.t_pack = now() - t0;
+
+    // ---- statistics
+    hbi_hmat_stats &S = h->st;
+    S.n = h->n;
+    S.nleaf = nl;
+    S.tile_rows = h->tr;
+    long long ktsum = 0;
+    for (const Leaf &f : L) {
+        if (f.ltmtx == 1) {
+            S.nlr++;
+            S.lr_entries += (long long)(f.nr + f.nc) * f.kt;
+            S.lr_dims += f.nr + f.nc;
+            ktsum += f.kt;
+            S.kt_max = std::max(S.kt_max, f.kt);
+        } else {
+            S.ndense++;
+            S.dense_entries += (long long)f.nr * f.nc;
+        }
+    }
+    S.kt_mean = S.nlr ? (double)ktsum / S.nlr : 0.0;
+    for (int p = 0; p < 3; p++) {
+        S.stored_bytes[p] = (long long)h->plan[p].bytes;
+        S.alg_bytes[p] = h->plan[p].alg_bytes;
+    }
+    S.nbf16_blocks = h->plan[HBI_PREC_MIXED].nbf16;
+}
+
+}  // namespace hbi
+
+// ===================================================================== ABI ==
+namespace hbi {
+static thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+}  // namespace hbi
+
+
+extern "C" {
+
+int hbi_version(void) { return 1; }
+const char *hbi_last_error(void) { return hbi::g_err.c_str(); }
+
+int hbi_device_count(int *count)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(count, "count is NULL");
+        HBI_HIP(hipGetDeviceCount(count));
+    });
+}
+
+int hbi_entries(const hbi_geom *g, int npairs, const int *i, const int *j, double *out)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(npairs >= 0 && (npairs == 0 || (i && j && out)), "bad entry arguments");
+        hbi::check_geom(g);
+        for (int p = 0; p < npairs; p++)
+            HBI_REQUIRE(i[p] >= 0 && i[p] < g->n && j[p] >= 0 && j[p] < g->n, "entry index out of range");
+        if (npairs == 0) return;
+        hbi_hmat tmp;
+        HBI_HIP(hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking));
+        struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{tmp.stream};
+        hbi::upload_geom(&tmp, g);
+        hbi::DevBuf<int> di, dj;
+        hbi::DevBuf<double> dout(npairs);
+        di.upload(i, npairs, tmp.stream);
+        dj.upload(j, npairs, tmp.stream);
+        hbi::entries_kernel<<<(npairs + 127) / 128, 128, 0, tmp.stream>>>(tmp.dgeo, npairs, di.p, dj.p, dout.p);
+        HBI_HIP(hipGetLastError());
+        dout.download(out, npairs, tmp.stream);
+        HBI_HIP(hipStreamSynchronize(tmp.stream));
+        tmp.stream = nullptr;
+    });
+}
+
+int hbi_hmat_build(const hbi_geom *g, const hbi_hmat_opts *opts, hbi_hmat **out)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(out && opts, "NULL argument");
+        *out = nullptr;
+        hbi_hmat *h = new hbi_hmat();
+        try {
+            hbi::build(h, g, opts);
+        } catch (...) {
+            hbi_hmat_destroy(h);
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int hbi_hmat_destroy(hbi_hmat *h)
+{
+    if (!h) return HBI_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto &e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    hipStream_t s = h->stream;
+    delete h;
+    if (s) (void)hipStreamDestroy(s);
+    return HBI_OK;
+}
+
+int hbi_hmat_stats_get(const hbi_hmat *h, hbi_hmat_stats *st)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(h && st, "NULL argument");
+        *st = h->st;
+    });
+}
+
+int hbi_hmat_lod(const hbi_hmat *h, int *lod)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(h && lod, "NULL argument");
+        std::copy(h->part.lod.begin(), h->part.lod.end(), lod);
+    });
+}
+
+int hbi_hmat_leaves(const hbi_hmat *h, int *out6)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(h && out6, "NULL argument");
+        for (size_t k = 0; k < h->part.leaves.size(); k++) {
+            const hbi::Leaf &f = h->part.leaves[k];
+            int *o = out6 + 6 * k;
+            o[0] = f.ltmtx; o[1] = f.r0; o[2] = f.nr; o[3] = f.c0; o[4] = f.nc; o[5] = f.kt;
+        }
+    });
+}
+
+int hbi_hmat_leaf_data(const hbi_hmat *h, int ip, double *out)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(h && out && ip >= 0 && ip < (int)h->part.leaves.size(), "bad leaf index");
+        const hbi::Leaf &f = h->part.leaves[ip];
+        const size_t cnt = f.ltmtx == 1 ? (size_t)(f.nr + f.nc) * f.kt : (size_t)f.nr * f.nc;
+        if (cnt == 0) return;
+        const double *src = (f.ltmtx == 1 ? h->d_lr.p : h->d_dense.p) + h->fac_off[ip];
+        HBI_HIP(hipMemcpyAsync(out, src, cnt * 8, hipMemcpyDeviceToHost, h->stream));
+        HBI_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+int hbi_hmat_matvec(hbi_hmat *h, const double *x, double *y, int prec, int on_device)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(h && x && y, "NULL argument");
+        if (on_device) {
+            hbi::hmat_apply(h, prec, x, y);
+            HBI_HIP(hipGetLastError());
+            HBI_HIP(hipStreamSynchronize(h->stream));
+        } else {
+            h->d_x.upload(x, h->n, h->stream);
+            hbi::hmat_apply(h, prec, h->d_x.p, h->d_y.p);
+            HBI_HIP(hipGetLastError());
+            h->d_y.download(y, h->n, h->stream);
+            HBI_HIP(hipStreamSynchronize(h->stream));
+        }
+    });
+}
+
+int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs, int prec)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(h && X && Y && nrhs > 0, "bad arguments");
+        for (int r = 0; r < nrhs; r++) {
+            h->d_x.upload(X + (size_t)r * h->n, h->n, h->stream);
+            hbi::hmat_apply(h, prec, h->d_x.p, h->d_y.p);
+            HBI_HIP(hipGetLastError());
+            h->d_y.download(Y + (size_t)r * h->n, h->n, h->stream);
+        }
+        HBI_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+int hbi_hmat_time_matvec(hbi_hmat *h, int prec, int iters, double *ms)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(h && ms && iters > 0, "bad arguments");
+        const hbi::Plan &P = hbi::hmat_plan(h, prec);
+        if ((int)h->d_ypart.n < std::max(P.nparts, 1) * h->tr) h->d_ypart.alloc((size_t)std::max(P.nparts, 1) * h->tr);
+        for (int w = 0; w < 3; w++) hbi::hmat_apply(h, prec, h->d_x.p, h->d_y.p);
+        HBI_HIP(hipStreamSynchronize(h->stream));
+        float t1 = 0, t2 = 0, t3 = 0;
+        for (int it = 0; it < iters; it++) {
+            HBI_HIP(hipEventRecord(h->ev[0], h->stream));
+            hbi::hmat_phase1(h, prec, h->d_x.p, h->d_z.p, h->stream, nullptr);
+            HBI_HIP(hipEventRecord(h->ev[1], h->stream));
+            hbi::hmat_phase2(h, prec, h->d_x.p, h->d_z.p, h->d_ypart.p, h->stream, nullptr);
+            HBI_HIP(hipEventRecord(h->ev[2], h->stream));
+            hbi::tile_reduce_kernel<<<(h->n + 255) / 256, 256, 0, h->stream>>>(h->n, h->tr, P.d_tile_part0.p,
+                                                                                h->d_ypart.p, h->d_y.p);
+            HBI_HIP(hipEventRecord(h->ev[3], h->stream));
+            HBI_HIP(hipEventSynchronize(h->ev[3]));
+            float a, b, c;
+            HBI_HIP(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
+            HBI_HIP(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
+            HBI_HIP(hipEventElapsedTime(&c, h->ev[0], h->ev[3]));
+            t1 += a; t2 += b; t3 += c;
+        }
+        ms[0] = t1 / iters;
+        ms[1] = t2 / iters;
+        ms[2] = t3 / iters;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,644 @@
+// rk.hip -- rate-and-state friction and Cash-Karp RK45 on the device.
+//
+// One RK attempt (rkqs -> rkck, main_LH.f90:2270-2370, 2472-2554) is a fixed
+// sequence of kernels, captured once as a HIP graph and replayed:
+//   stage0                 V, matvec input from y            derivs 2079-2109
+//   6 x [ phase1, phase2 (H-matvec)  ->  stage kernel ]
+//       stage kernel s: tile partials -> H x (/ds0) + taudot -> deriv ->
+//       k_s, then either the next stage's input (y + h sum b_sj k_j, V, V-Vpl)
+//       or, after the 6th stage, yout / yerr and the ordered error partials
+//   decide                 errmax with the reference's guard, accept / shrink,
+//                          hnext, underflow stop                      2309-2357
+//   commit                 y <- yout, trapezoidal slip, V         1266-1287
+//   monitor                max V, mean |slip|, history rows, stop criteria
+// Every kernel first reads a device flag and returns when the batch target
+// is reached or the run stopped, so surplus graph launches cost ~nothing and
+// no host synchronisation happens inside a batch.  h, t and the counters
+// live in device memory.
+#include "hmat.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace hbi {
+
+struct Tab {   // Cash-Karp tableau of main_LH.f90:2490-2496 (see rk_tableau)
+    double a2, a3, a4, a5, a6, b21, b31, b32, b41, b42, b43, b51, b52, b53, b54,
+        b61, b62, b63, b64, b65, c1, c3, c4, c6, dc1, dc3, dc4, dc5, dc6;
+};
+
+// The reference declares these real(8) parameters but writes them as
+// default-real (single precision) literals, so b_ij, c_i, dc_i carry
+// single-precision rounding; A2..A6 and B21 are double literals.
+static Tab rk_tableau()
+{
+    Tab t;
+    t.a2 = .2; t.a3 = .3; t.a4 = .6; t.a5 = 1.0; t.a6 = .875; t.b21 = .2;
+    t.b31 = (double)(3.f / 40.f); t.b32 = (double)(9.f / 40.f); t.b41 = (double).3f; t.b42 = (double)-.9f;
+    t.b43 = (double)1.2f; t.b51 = (double)(-11.f / 54.f); t.b52 = (double)2.5f; t.b53 = (double)(-70.f / 27.f);
+    t.b54 = (double)(35.f / 27.f); t.b61 = (double)(1631.f / 55296.f); t.b62 = (double)(175.f / 512.f);
+    t.b63 = (double)(575.f / 13824.f); t.b64 = (double)(44275.f / 110592.f); t.b65 = (double)(253.f / 4096.f);
+    t.c1 = (double)(37.f / 378.f); t.c3 = (double)(250.f / 621.f); t.c4 = (double)(125.f / 594.f);
+    t.c6 = (double)(512.f / 1771.f);
+    t.dc1 = t.c1 - (double)(2825.f / 27648.f); t.dc3 = t.c3 - (double)(18575.f / 48384.f);
+    t.dc4 = t.c4 - (double)(13525.f / 55296.f); t.dc5 = (double)(-277.f / 14336.f); t.dc6 = t.c6 - (double).25f;
+    return t;
+}
+
+struct RkPar {
+    int n, law, backslip, limitsigma, has_hn, tr, tr_n;
+    double vref, minsig, maxsig, dtmax, hdiv, eps, velmin, velmax, sg;
+    Tab T;
+};
+
+struct RkCtl {
+    double h, t, hdid, errmax, vmax, meanslip;
+    long long attempts, accepted, rejected, target, hist_base, hist_cap;
+    int nrjct, done, status, acc;
+};
+
+struct RkArrays {
+    const double *a, *b, *dc, *f0, *vpl, *taudot, *sigdot, *area;
+    double *y, *ytmp, *yout, *yerr, *k, *vel, *x, *slip, *velc;
+    const double *ypart, *ypart_n;
+    const int *part0, *part0_n;
+    double *errpart, *monpart;
+    double *hist, *vhist;
+    const int *rec;
+    int nrec;
+};
+
+constexpr int kRkThreads = 256;
+
+// V = 2 V0 exp(-psi/a) sinh(tau/(sigma a))   main_LH.f90:2084
+__device__ __forceinline__ double slip_rate(double psi, double tau, double sig, double a, double vref)
+{
+#pragma clang fp contract(off)
+    return 2 * vref * exp(-psi / a) * sinh(tau / sig / a);
+}
+
+// deriv, main_LH.f90:2165-2244 (aging and slip laws)
+__device__ __forceinline__ void deriv(const RkPar &P, double gs, double gn, double psi, double tau, double sig, double vel,
+                                      double a, double b, double dc, double f0, double *o)
+{
+#pragma clang fp contract(off)
+    const double rigid = kRigid, vs = kVs, vref = P.vref;
+    double dsigdt = gn;
+    if (P.limitsigma) {
+        if (sig < P.minsig) dsigdt = 0;
+        if (sig > P.maxsig) dsigdt = 0;
+    }
+    double dpsidt;
+    if (P.law == 0) {
+        if (b == 0) dpsidt = 0.0;
+        else dpsidt = b / dc * vref * exp((f0 - psi) / b) - b * fabs(vel) / dc;
+    } else {
+        const double fss = f0 + (a - b) * log(fabs(vel) / vref);
+        dpsidt = -fabs(vel) / dc * (fabs(tau) / sig - fss);
+    }
+    const double e = exp(-psi / a), ch = cosh(tau / sig / a);
+    const double dvdtau = 2 * vref * e * ch / (a * sig);
+    const double dvdsig = -2 * vref * e * ch * tau / (a * (sig * sig));
+    const double dvdpsi = -vel / a;
+    double dtaudt = gs - 0.5 * rigid / vs * (dvdpsi * dpsidt + dvdsig * dsigdt);
+    dtaudt = dtaudt / (1.0 + 0.5 * rigid / vs * dvdtau);
+    o[0] = dpsidt;
+    o[1] = dtaudt;
+    o[2] = dsigdt;
+}
+
+__global__ void rk_arm_kernel(RkCtl *c, long long nsteps, double h_override, int set_h)
+{
+    if (set_h) c->h = h_override;
+    if (c->status == 0) c->done = 0;
+    c->target = c->accepted + nsteps;
+    c->hist_base = c->accepted;
+    c->hist_cap = nsteps;
+    c->acc = 0;
+}
+
+__global__ void rk_stage0_kernel(RkPar P, const RkCtl *__restrict__ c, RkArrays A, int from_tmp)
+{
+#pragma clang fp contract(off)
+    if (c->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const double *y = from_tmp ? A.ytmp : A.y;
+    const double v = slip_rate(y[3 * i], y[3 * i + 1], y[3 * i + 2], A.a[i], P.vref);
+    A.vel[i] = v;
+    A.x[i] = P.backslip ? v - A.vpl[i] : v;
+}
+
+// ordered (guard, value) pair of the reference's errmax loop: a guard element
+// (ytemp - 1 == ytemp) overwrites errmax with 10, otherwise errmax = max(errmax, e)
+struct ErrPair {
+    int g;
+    double v;
+};
+__device__ __forceinline__ ErrPair err_combine(ErrPair l, ErrPair r)
+{
+    if (r.g) return r;
+    return ErrPair{l.g, fmax(l.v, r.v)};
+}
+
+// stage s in 0..5; mode 0: full attempt, 1: derivs only (writes k_0)
+__global__ __launch_bounds__(kRkThreads) void rk_stage_kernel(RkPar P, const RkCtl *__restrict__ c, RkArrays A, int s,
+                                                               int mode)
+{
+#pragma clang fp contract(off)
+    __shared__ int sg[kRkThreads / kWave];
+    __shared__ double sv[kRkThreads / kWave];
+    if (c->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = P.n;
+    const Tab &T = P.T;
+    ErrPair ep{0, 0.0};
+    if (i < n) {
+        // H x: tile partial sums in a fixed order
+        const int tr = P.tr;
+        const int t = i / tr, rr = i % tr;
+        double hx = 0.0;
+        for (int p = A.part0[t]; p < A.part0[t + 1]; p++) hx += A.ypart[(size_t)p * tr + rr];
+        double gs = P.hdiv != 0.0 ? hx / P.hdiv : hx;
+        double gn = 0.0;
+        if (P.has_hn) {
+            const int tn = i / P.tr_n, rn = i % P.tr_n;
+            double hy = 0.0;
+            for (int p = A.part0_n[tn]; p < A.part0_n[tn + 1]; p++) hy += A.ypart_n[(size_t)p * P.tr_n + rn];
+            gn = P.hdiv != 0.0 ? hy / P.hdiv : hy;
+        }
+        gn = gn + A.sigdot[i];
+        gs = gs + A.taudot[i];
+        const double *ys = (s == 0) ? A.y : A.ytmp;
+        double kk[3];
+        deriv(P, gs, gn, ys[3 * i], ys[3 * i + 1], ys[3 * i + 2], A.vel[i], A.a[i], A.b[i], A.dc[i], A.f0[i], kk);
+        const size_t m3 = 3 * (size_t)n;
+        double *ks = A.k + s * m3;
+        ks[3 * i] = kk[0];
+        ks[3 * i + 1] = kk[1];
+        ks[3 * i + 2] = kk[2];
+        if (mode == 0) {
+            const double h = c->h;
+            const double *k1 = A.k, *k2 = A.k + m3, *k3 = A.k + 2 * m3, *k4 = A.k + 3 * m3, *k5 = A.k + 4 * m3,
+                         *k6 = A.k + 5 * m3;
+            if (s < 5) {
+                double yt[3];
+                for (int q = 0; q < 3; q++) {
+                    const size_t e = 3 * (size_t)i + q;
+                    const double y = A.y[e];
+                    switch (s) {
+                    case 0: yt[q] = y + T.b21 * h * k1[e]; break;
+                    case 1: yt[q] = y + h * (T.b31 * k1[e] + T.b32 * k2[e]); break;
+                    case 2: yt[q] = y + h * (T.b41 * k1[e] + T.b42 * k2[e] + T.b43 * k3[e]); break;
+                    case 3: yt[q] = y + h * (T.b51 * k1[e] + T.b52 * k2[e] + T.b53 * k3[e] + T.b54 * k4[e]); break;
+                    default:
+                        yt[q] = y + h * (T.b61 * k1[e] + T.b62 * k2[e] + T.b63 * k3[e] + T.b64 * k4[e] + T.b65 * k5[e]);
+                        break;
+                    }
+                    A.ytmp[e] = yt[q];
+                }
+                const double v = slip_rate(yt[0], yt[1], yt[2], A.a[i], P.vref);
+                A.vel[i] = v;
+                A.x[i] = P.backslip ? v - A.vpl[i] : v;
+            } else {
+                for (int q = 0; q < 3; q++) {
+                    const size_t e = 3 * (size_t)i + q;
+                    const double yo = A.y[e] + h * (T.c1 * k1[e] + T.c3 * k3[e] + T.c4 * k4[e] + T.c6 * k6[e]);
+                    const double ye = h * (T.dc1 * k1[e] + T.dc3 * k3[e] + T.dc4 * k4[e] + T.dc5 * k5[e] + T.dc6 * k6[e]);
+                    A.yout[e] = yo;
+                    A.yerr[e] = ye;
+                    const double er = fabs(ye / yo) / P.eps;
+                    ErrPair cur{0, er > 0.0 ? er : 0.0};
+                    if (er != er) cur.v = 0.0;
+                    if (yo - 1 == yo) cur = ErrPair{1, 10.0};
+                    ep = err_combine(ep, cur);
+                }
+            }
+        }
+    }
+    if (mode != 0 || s != 5) return;
+    // ordered block combine: lane l absorbs lane l+o (higher indices)
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int g2 = __shfl_down(ep.g, o, kWave);
+        const double v2 = __shfl_down(ep.v, o, kWave);
+        const int lane = threadIdx.x % kWave;
+        if ((lane & (2 * o - 1)) == 0 && lane + o < kWave) ep = err_combine(ep, ErrPair{g2, v2});
+    }
+    const int w = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) { sg[w] = ep.g; sv[w] = ep.v; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ErrPair b{sg[0], sv[0]};
+        for (int q = 1; q < kRkThreads / kWave; q++) b = err_combine(b, ErrPair{sg[q], sv[q]});
+        A.errpart[2 * blockIdx.x] = (double)b.g;
+        A.errpart[2 * blockIdx.x + 1] = b.v;
+    }
+}
+
+__global__ void rk_decide_kernel(RkPar P, RkCtl *c, const double *__restrict__ errpart, int nblk)
+{
+#pragma clang fp contract(off)
+    if (c->done || threadIdx.x != 0) return;
+    const double SAFETY = (double)0.9f, PGROW = (double)-0.2f, PSHRNK = (double)-0.25f, tiny = (double)1e-20f;
+    ErrPair e{0, 0.0};
+    for (int b = 0; b < nblk; b++) e = err_combine(e, ErrPair{(int)errpart[2 * b], errpart[2 * b + 1]});
+    const double errmax = e.v;
+    const double h = c->h;
+    c->attempts++;
+    c->errmax = errmax;
+    if (errmax < 1.0 && errmax > tiny) {
+        double hn = fmin(2 * h, SAFETY * h * pow(errmax, PGROW));
+        hn = fmin(hn, P.dtmax);
+        c->hdid = h;
+        c->t = c->t + h;
+        c->h = hn;
+        c->acc = 1;
+        c->accepted++;
+    } else {
+        double hnew;
+        if (errmax > tiny) hnew = fmax(0.5 * h, SAFETY * h * pow(errmax, PSHRNK));
+        else hnew = 0.5 * h;
+        c->acc = 0;
+        c->nrjct++;
+        c->rejected++;
+        c->h = hnew;
+        const double xnew = c->t + hnew;
+        if (xnew - c->t < 1.e-15) {
+            c->status = HBI_E_STEP;
+            c->done = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const RkCtl *__restrict__ c, RkArrays A)
+{
+#pragma clang fp contract(off)
+    __shared__ double sv[kRkThreads / kWave], sm[kRkThreads / kWave];
+    if (c->done || !c->acc) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double dt = c->hdid;
+    double vmax = 0.0, ms = 0.0;
+    if (i < P.n) {
+        for (int q = 0; q < 3; q++) A.y[3 * (size_t)i + q] = A.yout[3 * (size_t)i + q];
+        double sl = A.slip[i];
+        sl = sl + A.velc[i] * dt * 0.5;
+        const double v = slip_rate(A.y[3 * i], A.y[3 * i + 1], A.y[3 * i + 2], A.a[i], P.vref);
+        sl = sl + v * dt * 0.5;
+        A.slip[i] = sl;
+        A.velc[i] = v;
+        vmax = fabs(v);
+        ms = fabs(sl) * A.area[i];
+    }
+    // per-block max and (ordered) sum
+    for (int o = 1; o < kWave; o <<= 1) {
+        const double v2 = __shfl_down(vmax, o, kWave), m2 = __shfl_down(ms, o, kWave);
+        const int lane = threadIdx.x % kWave;
+        if ((lane & (2 * o - 1)) == 0 && lane + o < kWave) { vmax = fmax(vmax, v2); ms = ms + m2; }
+    }
+    const int w = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) { sv[w] = vmax; sm[w] = ms; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = sv[0], b = sm[0];
+        for (int q = 1; q < kRkThreads / kWave; q++) { a = fmax(a, sv[q]); b = b + sm[q]; }
+        A.monpart[2 * blockIdx.x] = a;
+        A.monpart[2 * blockIdx.x + 1] = b;
+    }
+}
+
+__global__ void rk_monitor_kernel(RkPar P, RkCtl *c, RkArrays A, int nblk)
+{
+#pragma clang fp contract(off)
+    if (c->done || !c->acc) return;
+    __shared__ double vm, msum;
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < nblk; q++) { a = fmax(a, A.monpart[2 * q]); b = b + A.monpart[2 * q + 1]; }
+        vm = a;
+        msum = b / P.sg;
+        c->vmax = a;
+        c->meanslip = msum;
+    }
+    __syncthreads();
+    const long long row = c->accepted - c->hist_base - 1;
+    if (row >= 0 && row < c->hist_cap) {
+        if (A.hist && threadIdx.x == 0) {
+            double *hr = A.hist + 6 * row;
+            hr[0] = c->t; hr[1] = c->hdid; hr[2] = c->errmax; hr[3] = (double)c->nrjct; hr[4] = log10(vm);
+            hr[5] = msum;
+        }
+        if (A.vhist)
+            for (int r = threadIdx.x; r < A.nrec; r += blockDim.x) A.vhist[row * A.nrec + r] = A.velc[A.rec[r]];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c->nrjct = 0;
+        if (vm > P.velmax || vm < P.velmin) { c->status = 1; c->done = 1; }
+        if (c->accepted >= c->target) c->done = 1;
+    }
+}
+
+}  // namespace hbi
+
+// ------------------------------------------------------------------ object --
+struct hbi_rk {
+    hipStream_t stream = nullptr;
+    hbi_hmat *hs = nullptr, *hn = nullptr;
+    hbi_rk_opts opts{};
+    hbi::RkPar P{};
+    int n = 0, nblk = 0;
+    hbi::DevBuf<double> a, b, dc, f0, vpl, taudot, sigdot, area;
+    hbi::DevBuf<double> y, ytmp, yout, yerr, k, vel, x, slip, velc, z, zn, ypart, ypart_n, errpart, monpart;
+    hbi::DevBuf<double> hist, vhist;
+    hbi::DevBuf<int> rec;
+    hbi::DevBuf<hbi::RkCtl> ctl;
+    hipGraphExec_t graph = nullptr;
+    int graph_nrec = -1;
+    double *graph_hist = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double last_ms = 0.0;
+};
+
+namespace hbi {
+
+static RkArrays arrays(hbi_rk *r)
+{
+    RkArrays A{};
+    A.a = r->a.p; A.b = r->b.p; A.dc = r->dc.p; A.f0 = r->f0.p; A.vpl = r->vpl.p; A.taudot = r->taudot.p;
+    A.sigdot = r->sigdot.p; A.area = r->area.p;
+    A.y = r->y.p; A.ytmp = r->ytmp.p; A.yout = r->yout.p; A.yerr = r->yerr.p; A.k = r->k.p; A.vel = r->vel.p;
+    A.x = r->x.p; A.slip = r->slip.p; A.velc = r->velc.p;
+    A.ypart = r->ypart.p; A.ypart_n = r->ypart_n.p;
+    A.part0 = hmat_plan(r->hs, r->opts.prec).d_tile_part0.p;
+    A.part0_n = r->hn ? hmat_plan(r->hn, r->opts.prec).d_tile_part0.p : nullptr;
+    A.errpart = r->errpart.p; A.monpart = r->monpart.p;
+    A.hist = r->hist.n ? r->hist.p : nullptr;
+    A.vhist = r->vhist.n ? r->vhist.p : nullptr;
+    A.rec = r->rec.p;
+    A.nrec = (int)r->rec.n;
+    return A;
+}
+
+// the six derivs evaluations of one attempt (or one, for derivs only)
+static void launch_stages(hbi_rk *r, int nstages, int mode)
+{
+    const int g = (r->n + kRkThreads - 1) / kRkThreads;
+    RkArrays A = arrays(r);
+    const int *gate = &r->ctl.p->done;
+    rk_stage0_kernel<<<g, kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A, 0);
+    for (int s = 0; s < nstages; s++) {
+        hmat_phase1(r->hs, r->opts.prec, r->x.p, r->z.p, r->stream, gate);
+        hmat_phase2(r->hs, r->opts.prec, r->x.p, r->z.p, r->ypart.p, r->stream, gate);
+        if (r->hn) {
+            hmat_phase1(r->hn, r->opts.prec, r->x.p, r->zn.p, r->stream, gate);
+            hmat_phase2(r->hn, r->opts.prec, r->x.p, r->zn.p, r->ypart_n.p, r->stream, gate);
+        }
+        rk_stage_kernel<<<g, kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A, s, mode);
+    }
+}
+
+static void launch_attempt(hbi_rk *r)
+{
+    const int g = r->nblk;
+    launch_stages(r, 6, 0);
+    rk_decide_kernel<<<1, 64, 0, r->stream>>>(r->P, r->ctl.p, r->errpart.p, g);
+    RkArrays A = arrays(r);
+    rk_commit_kernel<<<g, kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A);
+    rk_monitor_kernel<<<1, 256, 0, r->stream>>>(r->P, r->ctl.p, A, g);
+}
+
+static void capture(hbi_rk *r)
+{
+    if (r->graph) {
+        (void)hipGraphExecDestroy(r->graph);
+        r->graph = nullptr;
+    }
+    hipGraph_t gr;
+    HBI_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+    launch_attempt(r);
+    HBI_HIP(hipStreamEndCapture(r->stream, &gr));
+    HBI_HIP(hipGraphInstantiate(&r->graph, gr, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(gr);
+    r->graph_nrec = (int)r->rec.n;
+    r->graph_hist = r->hist.p;
+}
+
+}  // namespace hbi
+
+extern "C" {
+
+int hbi_rk_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_rk_opts *opts, const double *a, const double *b,
+                  const double *dc, const double *f0, const double *vpl, const double *taudot, const double *sigdot,
+                  const double *area, hbi_rk **out)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(hs && opts && out && a && b && dc && f0 && vpl && taudot && sigdot && area, "NULL argument");
+        HBI_REQUIRE(!hn || hn->n == hs->n, "normal-stress matrix size differs");
+        HBI_REQUIRE(opts->prec >= 0 && opts->prec <= 2, "bad precision");
+        HBI_REQUIRE(opts->eps_r > 0, "eps_r must be > 0");
+        *out = nullptr;
+        hbi_rk *r = new hbi_rk();
+        try {
+            r->hs = hs;
+            r->hn = hn;
+            r->opts = *opts;
+            const int n = hs->n;
+            r->n = n;
+            r->nblk = (n + hbi::kRkThreads - 1) / hbi::kRkThreads;
+            HBI_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+            HBI_HIP(hipEventCreate(&r->e0));
+            HBI_HIP(hipEventCreate(&r->e1));
+            hbi::RkPar &P = r->P;
+            P.n = n; P.law = opts->law; P.backslip = opts->backslip; P.limitsigma = opts->limitsigma;
+            P.has_hn = hn ? 1 : 0; P.tr = hs->tr; P.tr_n = hn ? hn->tr : 0;
+            P.vref = opts->vref; P.minsig = opts->minsig; P.maxsig = opts->maxsig; P.dtmax = opts->dtmax;
+            P.hdiv = opts->hdiv; P.eps = opts->eps_r; P.velmin = opts->velmin; P.velmax = opts->velmax;
+            P.T = hbi::rk_tableau();
+            double sgsum = 0.0;
+            for (int i = 0; i < n; i++) sgsum = sgsum + area[i];
+            P.sg = sgsum;
+            r->a.upload(a, n, r->stream); r->b.upload(b, n, r->stream); r->dc.upload(dc, n, r->stream);
+            r->f0.upload(f0, n, r->stream); r->vpl.upload(vpl, n, r->stream); r->taudot.upload(taudot, n, r->stream);
+            r->sigdot.upload(sigdot, n, r->stream); r->area.upload(area, n, r->stream);
+            const size_t m3 = 3 * (size_t)n;
+            r->y.alloc(m3); r->ytmp.alloc(m3); r->yout.alloc(m3); r->yerr.alloc(m3); r->k.alloc(6 * m3);
+            r->vel.alloc(n); r->x.alloc(n + 8); r->slip.alloc(n); r->velc.alloc(n);
+            HBI_HIP(hipMemsetAsync(r->x.p, 0, (n + 8) * 8, r->stream));
+            HBI_HIP(hipMemsetAsync(r->slip.p, 0, n * 8, r->stream));
+            HBI_HIP(hipMemsetAsync(r->velc.p, 0, n * 8, r->stream));
+            HBI_HIP(hipMemsetAsync(r->k.p, 0, 6 * m3 * 8, r->stream));
+            const hbi::Plan &Ps = hbi::hmat_plan(hs, opts->prec);
+            r->z.alloc(std::max(hs->nz, 1) + 8);
+            r->ypart.alloc((size_t)std::max(Ps.nparts, 1) * hs->tr);
+            if (hn) {
+                const hbi::Plan &Pn = hbi::hmat_plan(hn, opts->prec);
+                r->zn.alloc(std::max(hn->nz, 1) + 8);
+                r->ypart_n.alloc((size_t)std::max(Pn.nparts, 1) * hn->tr);
+            }
+            r->errpart.alloc(2 * (size_t)r->nblk);
+            r->monpart.alloc(2 * (size_t)r->nblk);
+            r->ctl.alloc(1);
+            hbi::RkCtl c{};
+            c.h = 1.0;
+            HBI_HIP(hipMemcpyAsync(r->ctl.p, &c, sizeof(c), hipMemcpyHostToDevice, r->stream));
+            HBI_HIP(hipStreamSynchronize(r->stream));
+        } catch (...) {
+            hbi_rk_destroy(r);
+            throw;
+        }
+        *out = r;
+    });
+}
+
+int hbi_rk_destroy(hbi_rk *r)
+{
+    if (!r) return HBI_OK;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    if (r->graph) (void)hipGraphExecDestroy(r->graph);
+    if (r->e0) (void)hipEventDestroy(r->e0);
+    if (r->e1) (void)hipEventDestroy(r->e1);
+    hipStream_t s = r->stream;
+    delete r;
+    if (s) (void)hipStreamDestroy(s);
+    return HBI_OK;
+}
+
+int hbi_rk_set_state(hbi_rk *r, const double *y, const double *slip, const double *vel, double t, double dt_next)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && y, "NULL argument");
+        const int n = r->n;
+        r->y.upload(y, 3 * (size_t)n, r->stream);
+        if (slip) r->slip.upload(slip, n, r->stream);
+        else HBI_HIP(hipMemsetAsync(r->slip.p, 0, n * 8, r->stream));
+        if (vel) {
+            r->velc.upload(vel, n, r->stream);
+        } else {
+            hbi::RkArrays A = hbi::arrays(r);
+            // V of the state, written into velc through the stage-0 kernel
+            hbi::RkCtl c{};
+            HBI_HIP(hipMemcpyAsync(r->ctl.p, &c, sizeof(c), hipMemcpyHostToDevice, r->stream));
+            hbi::rk_stage0_kernel<<<r->nblk, hbi::kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A, 0);
+            HBI_HIP(hipMemcpyAsync(r->velc.p, r->vel.p, n * 8, hipMemcpyDeviceToDevice, r->stream));
+        }
+        hbi::RkCtl c{};
+        c.h = dt_next;
+        c.t = t;
+        HBI_HIP(hipMemcpyAsync(r->ctl.p, &c, sizeof(c), hipMemcpyHostToDevice, r->stream));
+        HBI_HIP(hipStreamSynchronize(r->stream));
+    });
+}
+
+int hbi_rk_get_state(hbi_rk *r, double *y, double *slip, double *vel, hbi_rk_info *info)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r, "NULL argument");
+        const int n = r->n;
+        if (y) r->y.download(y, 3 * (size_t)n, r->stream);
+        if (slip) r->slip.download(slip, n, r->stream);
+        if (vel) r->velc.download(vel, n, r->stream);
+        hbi::RkCtl c;
+        HBI_HIP(hipMemcpyAsync(&c, r->ctl.p, sizeof(c), hipMemcpyDeviceToHost, r->stream));
+        HBI_HIP(hipStreamSynchronize(r->stream));
+        if (info) {
+            info->t = c.t; info->dt_next = c.h; info->dt_last = c.hdid; info->errmax_last = c.errmax;
+            info->attempts = c.attempts; info->accepted = c.accepted; info->rejected = c.rejected;
+            info->vmax = c.vmax; info->status = c.status;
+        }
+    });
+}
+
+int hbi_rk_derivs(hbi_rk *r, const double *y, double *dydx)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && y && dydx, "NULL argument");
+        const int n = r->n;
+        HBI_HIP(hipMemcpyAsync(r->y.p, y, 3 * (size_t)n * 8, hipMemcpyHostToDevice, r->stream));
+        hbi::rk_arm_kernel<<<1, 1, 0, r->stream>>>(r->ctl.p, 0, 0.0, 0);
+        hbi::launch_stages(r, 1, 1);
+        HBI_HIP(hipGetLastError());
+        HBI_HIP(hipMemcpyAsync(dydx, r->k.p, 3 * (size_t)n * 8, hipMemcpyDeviceToHost, r->stream));
+        HBI_HIP(hipStreamSynchronize(r->stream));
+    });
+}
+
+int hbi_rk_rkck(hbi_rk *r, const double *y, double h, double *yout, double *yerr)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && y && yout && yerr, "NULL argument");
+        const int n = r->n;
+        HBI_HIP(hipMemcpyAsync(r->y.p, y, 3 * (size_t)n * 8, hipMemcpyHostToDevice, r->stream));
+        hbi::rk_arm_kernel<<<1, 1, 0, r->stream>>>(r->ctl.p, 0, h, 1);
+        hbi::launch_stages(r, 6, 0);
+        HBI_HIP(hipGetLastError());
+        HBI_HIP(hipMemcpyAsync(yout, r->yout.p, 3 * (size_t)n * 8, hipMemcpyDeviceToHost, r->stream));
+        HBI_HIP(hipMemcpyAsync(yerr, r->yerr.p, 3 * (size_t)n * 8, hipMemcpyDeviceToHost, r->stream));
+        HBI_HIP(hipStreamSynchronize(r->stream));
+    });
+}
+
+int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, double *vhist, hbi_rk_info *info)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && nsteps >= 0, "bad arguments");
+        HBI_REQUIRE(nrec == 0 || (rec && vhist), "rec and vhist required when nrec > 0");
+        for (int q = 0; q < nrec; q++) HBI_REQUIRE(rec[q] >= 0 && rec[q] < r->n, "record index out of range");
+        // history buffers (the graph records their addresses)
+        const size_t hneed = hist ? 6 * (size_t)nsteps : 0;
+        const size_t vneed = nrec > 0 ? (size_t)nsteps * nrec : 0;
+        bool recapture = !r->graph;
+        if (hneed > r->hist.n || (hneed == 0 && r->hist.n)) {
+            if (hneed) r->hist.alloc(hneed); else r->hist.release();
+            recapture = true;
+        }
+        if (vneed > r->vhist.n || (vneed == 0 && r->vhist.n)) {
+            if (vneed) r->vhist.alloc(vneed); else r->vhist.release();
+            recapture = true;
+        }
+        if (nrec != (int)r->rec.n) {
+            if (nrec) r->rec.alloc(nrec); else r->rec.release();
+            recapture = true;
+        }
+        if (nrec) r->rec.upload(rec, nrec, r->stream);
+        if (recapture) hbi::capture(r);
+        hbi::rk_arm_kernel<<<1, 1, 0, r->stream>>>(r->ctl.p, nsteps, 0.0, 0);
+        HBI_HIP(hipEventRecord(r->e0, r->stream));
+        hbi::RkCtl c;
+        long long launched = 0, budget = 50LL * nsteps + 100;
+        int batch = nsteps;
+        for (;;) {
+            for (int q = 0; q < batch; q++) HBI_HIP(hipGraphLaunch(r->graph, r->stream));
+            launched += batch;
+            HBI_HIP(hipMemcpyAsync(&c, r->ctl.p, sizeof(c), hipMemcpyDeviceToHost, r->stream));
+            HBI_HIP(hipStreamSynchronize(r->stream));
+            if (c.done || launched >= budget || nsteps == 0) break;
+            batch = (int)std::max<long long>(1, std::min<long long>(c.target - c.accepted + 2, budget - launched));
+        }
+        HBI_HIP(hipEventRecord(r->e1, r->stream));
+        HBI_HIP(hipEventSynchronize(r->e1));
+        float ms = 0;
+        HBI_HIP(hipEventElapsedTime(&ms, r->e0, r->e1));
+        r->last_ms = ms;
+        const long long done = c.accepted - c.hist_base;
+        if (hist && done > 0) r->hist.download(hist, 6 * (size_t)std::min<long long>(done, nsteps), r->stream);
+        if (nrec && done > 0)
+            r->vhist.download(vhist, (size_t)std::min<long long>(done, nsteps) * nrec, r->stream);
+        HBI_HIP(hipStreamSynchronize(r->stream));
+        if (info) {
+            info->t = c.t; info->dt_next = c.h; info->dt_last = c.hdid; info->errmax_last = c.errmax;
+            info->attempts = c.attempts; info->accepted = c.accepted; info->rejected = c.rejected;
+            info->vmax = c.vmax; info->status = c.status;
+        }
+        if (c.status == HBI_E_STEP) throw hbi::Error{HBI_E_STEP, "Runge-Kutta step size underflow"};
+    });
+}
+
+int hbi_rk_last_ms(const hbi_rk *r, double *ms)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && ms, "NULL argument");
+        *ms = r->last_ms;
+    });
+}
+
+}  // extern "C"

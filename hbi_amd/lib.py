@@ -1,0 +1,288 @@
+"""ctypes binding of libhbi.so (include/hbi.h).
+
+The HIP library is the product path: there is no CPU fallback.  Loading
+raises if hbi_amd/libhbi.so is missing (build it with ``python -m hbi_amd.build``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libhbi.so")
+
+PREC_F64, PREC_F32, PREC_MIXED = 0, 1, 2
+E_STEP = -4
+
+_FIELDS = ("xcol", "ycol", "zcol", "xel", "xer", "ang", "angd", "dsl", "dsd", "rake",
+           "xs1", "xs2", "xs3", "ys1", "ys2", "ys3", "zs1", "zs2", "zs3",
+           "ev11", "ev12", "ev13", "ev21", "ev22", "ev23", "ev31", "ev32", "ev33")
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Geom(C.Structure):
+    _fields_ = [("kind", C.c_int), ("n", C.c_int), ("vcomp", C.c_int), ("md", C.c_int), ("w", C.c_double)] + \
+               [(f, _dp) for f in _FIELDS]
+
+
+class HmatOpts(C.Structure):
+    _fields_ = [("leaf", C.c_int), ("eta", C.c_double), ("eps_aca", C.c_double), ("kmax", C.c_int),
+                ("bf16_margin", C.c_double), ("tile_rows", C.c_int), ("chunk_kb", C.c_int)]
+
+
+class HmatStats(C.Structure):
+    _fields_ = [("n", C.c_int), ("nleaf", C.c_int), ("nlr", C.c_int), ("ndense", C.c_int), ("kt_max", C.c_int),
+                ("tile_rows", C.c_int), ("kt_mean", C.c_double), ("dense_entries", C.c_longlong),
+                ("lr_entries", C.c_longlong), ("lr_dims", C.c_longlong), ("stored_bytes", C.c_longlong * 3),
+                ("alg_bytes", C.c_longlong * 3), ("nbf16_blocks", C.c_int), ("ncapped", C.c_int),
+                ("t_partition", C.c_double), ("t_dense", C.c_double), ("t_aca", C.c_double), ("t_pack", C.c_double)]
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
+class RkOpts(C.Structure):
+    _fields_ = [("law", C.c_int), ("backslip", C.c_int), ("limitsigma", C.c_int), ("prec", C.c_int),
+                ("vref", C.c_double), ("minsig", C.c_double), ("maxsig", C.c_double), ("dtmax", C.c_double),
+                ("hdiv", C.c_double), ("eps_r", C.c_double), ("velmin", C.c_double), ("velmax", C.c_double)]
+
+
+class RkInfo(C.Structure):
+    _fields_ = [("t", C.c_double), ("dt_next", C.c_double), ("dt_last", C.c_double), ("errmax_last", C.c_double),
+                ("attempts", C.c_longlong), ("accepted", C.c_longlong), ("rejected", C.c_longlong),
+                ("vmax", C.c_double), ("status", C.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class HbiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("hbi error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "hbi_hmat_build", "hbi_hmat_destroy",
+           "hbi_hmat_stats_get", "hbi_hmat_lod", "hbi_hmat_leaves", "hbi_hmat_leaf_data", "hbi_hmat_matvec",
+           "hbi_hmat_matmul", "hbi_hmat_time_matvec", "hbi_rk_create", "hbi_rk_destroy", "hbi_rk_set_state",
+           "hbi_rk_get_state", "hbi_rk_derivs", "hbi_rk_rkck", "hbi_rk_step", "hbi_rk_last_ms")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("libhbi.so not built: run `python -m hbi_amd.build` (no CPU fallback exists)")
+        L = C.CDLL(LIBPATH)
+        L.hbi_last_error.restype = C.c_char_p
+        L.hbi_entries.argtypes = [C.POINTER(Geom), C.c_int, _ip, _ip, _dp]
+        L.hbi_hmat_build.argtypes = [C.POINTER(Geom), C.POINTER(HmatOpts), C.POINTER(C.c_void_p)]
+        L.hbi_hmat_destroy.argtypes = [C.c_void_p]
+        L.hbi_hmat_stats_get.argtypes = [C.c_void_p, C.POINTER(HmatStats)]
+        L.hbi_hmat_lod.argtypes = [C.c_void_p, _ip]
+        L.hbi_hmat_leaves.argtypes = [C.c_void_p, _ip]
+        L.hbi_hmat_leaf_data.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.hbi_hmat_matvec.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_int]
+        L.hbi_hmat_matmul.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_int]
+        L.hbi_hmat_time_matvec.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.hbi_rk_create.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(RkOpts)] + [_dp] * 8 + [C.POINTER(C.c_void_p)]
+        L.hbi_rk_destroy.argtypes = [C.c_void_p]
+        L.hbi_rk_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double]
+        L.hbi_rk_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, C.POINTER(RkInfo)]
+        L.hbi_rk_derivs.argtypes = [C.c_void_p, _dp, _dp]
+        L.hbi_rk_rkck.argtypes = [C.c_void_p, _dp, C.c_double, _dp, _dp]
+        L.hbi_rk_step.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip, _dp, C.POINTER(RkInfo)]
+        L.hbi_rk_last_ms.argtypes = [C.c_void_p, _dp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise HbiError(rc, lib().hbi_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _ipp(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class _GeomHold:
+    def __init__(self, g):
+        self.keep = []
+        s = Geom()
+        s.kind, s.n, s.vcomp, s.md, s.w = int(g.kind), int(g.n), int(g.vcomp), int(g.md), float(g.w)
+        for f in _FIELDS:
+            a = g.arrays.get(f)
+            if a is not None:
+                a = _f64(a)
+                self.keep.append(a)
+                setattr(s, f, _p(a))
+        self.s = s
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    _check(lib().hbi_device_count(C.byref(c)))
+    return c.value
+
+
+def entries(g, ii, jj) -> np.ndarray:
+    """K(i, j) on the GPU (HACApK_entry_ij)."""
+    G = _GeomHold(g)
+    ii = np.ascontiguousarray(ii, dtype=np.int32)
+    jj = np.ascontiguousarray(jj, dtype=np.int32)
+    out = np.empty(len(ii))
+    _check(lib().hbi_entries(C.byref(G.s), len(ii), _ipp(ii), _ipp(jj), _p(out)))
+    return out
+
+
+class HMatrix:
+    """Device lattice H-matrix (HACApK_generate + HACApK_construct_LH)."""
+
+    def __init__(self, g, leaf=15, eta=2.0, eps=1e-4, kmax=1000, bf16_margin=0.0, tile_rows=0, chunk_kb=0):
+        G = _GeomHold(g)
+        o = HmatOpts(leaf=leaf, eta=eta, eps_aca=eps, kmax=kmax, bf16_margin=bf16_margin, tile_rows=tile_rows,
+                     chunk_kb=chunk_kb)
+        h = C.c_void_p()
+        _check(lib().hbi_hmat_build(C.byref(G.s), C.byref(o), C.byref(h)))
+        self.h = h
+        self.n = int(g.n)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hbi_hmat_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def stats(self) -> dict:
+        s = HmatStats()
+        _check(lib().hbi_hmat_stats_get(self.h, C.byref(s)))
+        return s.as_dict()
+
+    @property
+    def lod(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.int32)
+        _check(lib().hbi_hmat_lod(self.h, _ipp(out)))
+        return out
+
+    @property
+    def leaves(self) -> np.ndarray:
+        nl = self.stats["nleaf"]
+        out = np.empty((nl, 6), dtype=np.int32)
+        _check(lib().hbi_hmat_leaves(self.h, _ipp(out)))
+        return out
+
+    def leaf_data(self, ip: int) -> np.ndarray:
+        lt, _, ndl, _, ndt, kt = self.leaves[ip]
+        size = (ndl + ndt) * kt if lt == 1 else ndl * ndt
+        out = np.empty(max(size, 1))
+        _check(lib().hbi_hmat_leaf_data(self.h, int(ip), _p(out)))
+        return out[:size]
+
+    def matvec(self, x, prec=PREC_F64) -> np.ndarray:
+        x = _f64(x)
+        y = np.empty(self.n)
+        _check(lib().hbi_hmat_matvec(self.h, _p(x), _p(y), int(prec), 0))
+        return y
+
+    def matmul(self, X, prec=PREC_F64) -> np.ndarray:
+        X = np.asfortranarray(X, dtype=np.float64)
+        nrhs = X.shape[1]
+        Y = np.empty((self.n, nrhs), order="F")
+        _check(lib().hbi_hmat_matmul(self.h, _p(X), _p(Y), int(nrhs), int(prec)))
+        return Y
+
+    def time_matvec(self, prec=PREC_F64, iters=20):
+        ms = np.zeros(3)
+        _check(lib().hbi_hmat_time_matvec(self.h, int(prec), int(iters), _p(ms)))
+        return {"lrvt_ms": ms[0], "tile_ms": ms[1], "total_ms": ms[2]}
+
+
+class RK:
+    """Device friction + RK45 integrator over one (or two) H-matrices."""
+
+    def __init__(self, hs: HMatrix, fr, lod, area, hdiv=0.0, eps_r=1e-6, prec=PREC_F64, hn: Optional[HMatrix] = None,
+                 backslip=True, limitsigma=True, minsig=1.0, maxsig=300.0, dtmax=1e10, velmin=1e-20, velmax=1e7):
+        self.hs, self.hn = hs, hn
+        lod = np.asarray(lod)
+        self.keep = [_f64(getattr(fr, k)[lod]) for k in ("a", "b", "dc", "f0", "vpl", "taudot", "sigdot")]
+        self.keep.append(_f64(np.asarray(area)[lod]))
+        o = RkOpts(law=int(fr.law), backslip=int(backslip), limitsigma=int(limitsigma), prec=int(prec),
+                   vref=float(fr.vref), minsig=minsig, maxsig=maxsig, dtmax=dtmax, hdiv=float(hdiv), eps_r=eps_r,
+                   velmin=velmin, velmax=velmax)
+        r = C.c_void_p()
+        _check(lib().hbi_rk_create(hs.h, hn.h if hn is not None else None, C.byref(o),
+                                   *[_p(a) for a in self.keep], C.byref(r)))
+        self.r = r
+        self.n = hs.n
+
+    def close(self):
+        if getattr(self, "r", None):
+            lib().hbi_rk_destroy(self.r)
+            self.r = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, y, t=0.0, dt=1.0, slip=None, vel=None):
+        y = _f64(y)
+        slip = _f64(slip) if slip is not None else None
+        vel = _f64(vel) if vel is not None else None
+        _check(lib().hbi_rk_set_state(self.r, _p(y), _p(slip), _p(vel), float(t), float(dt)))
+
+    def get_state(self):
+        y = np.empty(3 * self.n)
+        slip = np.empty(self.n)
+        vel = np.empty(self.n)
+        info = RkInfo()
+        _check(lib().hbi_rk_get_state(self.r, _p(y), _p(slip), _p(vel), C.byref(info)))
+        return y, slip, vel, info.as_dict()
+
+    def derivs(self, y):
+        y = _f64(y)
+        d = np.empty_like(y)
+        _check(lib().hbi_rk_derivs(self.r, _p(y), _p(d)))
+        return d
+
+    def rkck(self, y, h):
+        y = _f64(y)
+        yo = np.empty_like(y)
+        ye = np.empty_like(y)
+        _check(lib().hbi_rk_rkck(self.r, _p(y), float(h), _p(yo), _p(ye)))
+        return yo, ye
+
+    def step(self, nsteps, history=False, rec=None):
+        hist = np.zeros((nsteps, 6)) if history else None
+        rec = np.ascontiguousarray(rec, dtype=np.int32) if rec is not None and len(rec) else None
+        nrec = 0 if rec is None else len(rec)
+        vh = np.zeros((nsteps, nrec)) if nrec else None
+        info = RkInfo()
+        rc = lib().hbi_rk_step(self.r, int(nsteps), _p(hist), nrec, _ipp(rec), _p(vh), C.byref(info))
+        _check(rc)
+        d = info.as_dict()
+        return d, hist, vh
+
+    def last_ms(self) -> float:
+        ms = C.c_double(0)
+        _check(lib().hbi_rk_last_ms(self.r, C.byref(ms)))
+        return ms.value

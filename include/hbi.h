@@ -1,0 +1,169 @@
+/*
+ * hbi.h -- C ABI of libhbi: the MI355X (gfx950) hot path of the HBI
+ * earthquake-cycle simulator (lattice H-matrix assembly, H-matrix-vector
+ * product, rate-and-state friction with adaptive Cash-Karp RK45).
+ *
+ * Plain C types only.  Every entry point returns an int status (HBI_OK or a
+ * negative HBI_E* code) and never aborts; hbi_last_error() gives the text.
+ * Vectors are in cluster order (the HACApK permutation lod), exactly as the
+ * reference's st_vel%vs / st_sum%vs.  All device memory is owned by the
+ * library behind opaque handles; host pointers are accepted unless a
+ * function says otherwise.  One HIP stream per handle, single host thread.
+ *
+ * Reference interfaces replaced (amtseismo/hbi):
+ *   hbi_hmat_build        HACApK_init + HACApK_generate + HACApK_construct_LH
+ *                         (m_HACApK_base.f90:246, m_HACApK_use.f90:133, :263;
+ *                          call sites main_LH.f90:590, 691, 693)
+ *   hbi_hmat_lod          HACApK_gen_lattice_vector's lodc (m_HACApK_base.f90:3449;
+ *                         call sites main_LH.f90:695-696)
+ *   hbi_hmat_matvec       HACApK_adot_lattice_hyp (m_HACApK_solve.f90:220;
+ *                         call sites main_LH.f90:2111, 2122, 2643)
+ *   hbi_hmat_destroy      HACApK_free_leafmtxp / HACApK_finalize (main_LH.f90:1479-1485)
+ *   hbi_entries           HACApK_entry_ij (m_HACApK_calc_entry_ij.f90:24)
+ *   hbi_rk_derivs         derivs (main_LH.f90:2053)
+ *   hbi_rk_rkck           rkck (main_LH.f90:2472)
+ *   hbi_rk_step           rkqs called in the time loop (main_LH.f90:1219-1287)
+ */
+#ifndef HBI_H
+#define HBI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HBI_OK 0
+#define HBI_E_ARG (-1)      /* invalid argument / configuration */
+#define HBI_E_HIP (-2)      /* HIP runtime error */
+#define HBI_E_NOMEM (-3)    /* device allocation failed */
+#define HBI_E_STEP (-4)     /* RK step size underflow (main_LH.f90:2348-2352) */
+#define HBI_E_STATE (-5)    /* object in the wrong state for this call */
+
+/* problem kinds (st_bemv%problem) */
+#define HBI_KIND_2DVS 1       /* 2-D antiplane, half-space image: '2dvs' */
+#define HBI_KIND_OKADA_HS 2   /* rectangles, half-space: '3dph', '3dhr' */
+#define HBI_KIND_OKADA_FS 3   /* rectangles, full space: '3dnr' */
+#define HBI_KIND_TRI_FS 4     /* triangles, full space: '3dnt' */
+
+/* storage precision of the H-matrix factors used by a product */
+#define HBI_PREC_F64 0        /* FP64 factors (the reference's arithmetic) */
+#define HBI_PREC_F32 1        /* FP32 factors, FP64 accumulation */
+#define HBI_PREC_MIXED 2      /* FP32 factors, BF16 where a block's ACA tolerance allows */
+
+/* Element geometry in mesh order (st_HACApK_calc_entry,
+   m_HACApK_calc_entry_ij.f90:7-19).  Unused arrays may be NULL. */
+typedef struct hbi_geom {
+    int kind, n;
+    int vcomp;   /* 0: shear traction ('s'), 1: normal traction ('n') */
+    int md;      /* triangle slip mode: 0 's' (receiver rake), 1 'st', 2 'dp', 3 'open' */
+    double w;    /* okada normalisation length (st_bemv%w) */
+    const double *xcol, *ycol, *zcol;
+    const double *xel, *xer;
+    const double *ang, *angd, *dsl, *dsd, *rake;
+    const double *xs1, *xs2, *xs3, *ys1, *ys2, *ys3, *zs1, *zs2, *zs3;
+    const double *ev11, *ev12, *ev13, *ev21, *ev22, *ev23, *ev31, *ev32, *ev33;
+} hbi_geom;
+
+/* H-matrix construction options (the st_ctl%param entries that matter) */
+typedef struct hbi_hmat_opts {
+    int leaf;            /* param(21) leaf size; reference default 15 */
+    double eta;          /* param(51) admissibility; default 2 */
+    double eps_aca;      /* eps_h = param(71) */
+    int kmax;            /* param(63) ACA rank cap; default 1000 */
+    double bf16_margin;  /* HBI_PREC_MIXED: BF16 factors for a low-rank block when its
+                            BF16 rounding error bound <= bf16_margin x its ACA tolerance;
+                            0 disables BF16 */
+    int tile_rows;       /* matvec row tile (32 or 64); 0 = from leaf size */
+    int chunk_kb;        /* matvec work-item size target in KiB; 0 = default */
+} hbi_hmat_opts;
+
+typedef struct hbi_hmat_stats {
+    int n, nleaf, nlr, ndense, kt_max, tile_rows;
+    double kt_mean;
+    long long dense_entries;     /* sum ndl*ndt over dense leaves */
+    long long lr_entries;        /* sum (ndl+ndt)*kt over low-rank leaves */
+    long long lr_dims;           /* sum (ndl+ndt) over low-rank leaves */
+    long long stored_bytes[3];   /* packed bytes per precision (incl. padding) */
+    long long alg_bytes[3];      /* algorithmic bytes per single-vector product per precision */
+    int nbf16_blocks;            /* low-rank blocks stored as BF16 in HBM_PREC_MIXED */
+    int ncapped;                 /* low-rank blocks whose ACA reached kmax */
+    double t_partition, t_dense, t_aca, t_pack;   /* build phase times, seconds */
+} hbi_hmat_stats;
+
+typedef struct hbi_hmat hbi_hmat;
+typedef struct hbi_rk hbi_rk;
+
+int hbi_version(void);
+const char *hbi_last_error(void);
+int hbi_device_count(int *count);
+
+/* Matrix entries K(i,j) (receiver i, source j, mesh order, 0-based) evaluated
+   on the GPU in FP64.  Replaces HACApK_entry_ij. */
+int hbi_entries(const hbi_geom *g, int npairs, const int *i, const int *j, double *out);
+
+/* Build the lattice H-matrix on the GPU: cluster tree and partition (host),
+   dense leaves and ACA+ low-rank leaves (device, FP64), then packed
+   FP64 / FP32 / mixed copies for the products. */
+int hbi_hmat_build(const hbi_geom *g, const hbi_hmat_opts *opts, hbi_hmat **out);
+int hbi_hmat_destroy(hbi_hmat *h);
+int hbi_hmat_stats_get(const hbi_hmat *h, hbi_hmat_stats *st);
+/* lod[k] = mesh element at cluster position k (0-based) */
+int hbi_hmat_lod(const hbi_hmat *h, int *lod);
+/* per leaf in the reference's (row, col) sorted order: ltmtx, nstrtl, ndl, nstrtt, ndt, kt */
+int hbi_hmat_leaves(const hbi_hmat *h, int *out6);
+/* FP64 factors of one leaf: low rank -> a1(ndt,kt) then a2(ndl,kt); dense -> a1(ndt,ndl) */
+int hbi_hmat_leaf_data(const hbi_hmat *h, int ip, double *out);
+
+/* y = H x in cluster order, with the factors of precision prec.
+   on_device = 0: x, y are host pointers; 1: device pointers. */
+int hbi_hmat_matvec(hbi_hmat *h, const double *x, double *y, int prec, int on_device);
+/* Y = H X for nrhs right-hand sides, X and Y column-major N x nrhs (host). */
+int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs, int prec);
+/* Average device time per call of each matvec kernel over iters products on
+   device-resident vectors, from HIP events on the library stream:
+   ms[0] = low-rank V^T x kernel, ms[1] = row-tile kernel, ms[2] = whole product. */
+int hbi_hmat_time_matvec(hbi_hmat *h, int prec, int iters, double *ms);
+
+/* Friction / RK45 options (main_LH.f90 variables) */
+typedef struct hbi_rk_opts {
+    int law;          /* 0 aging, 1 slip */
+    int backslip;     /* matvec input V - Vpl (1) or V (0) */
+    int limitsigma;
+    int prec;         /* HBI_PREC_* of the shear matvec */
+    double vref, minsig, maxsig, dtmax;
+    double hdiv;      /* matvec result divided by this (ds0 for okada problems), 0 = none */
+    double eps_r;     /* RK relative tolerance */
+    double velmin, velmax;   /* stop criteria on max V (main_LH.f90:1438-1445) */
+} hbi_rk_opts;
+
+typedef struct hbi_rk_info {
+    double t, dt_next, dt_last, errmax_last;
+    long long attempts, accepted, rejected;
+    double vmax;       /* max |V| after the last accepted step */
+    int status;        /* 0 running, 1 stopped by velmax/velmin, HBI_E_STEP on underflow */
+} hbi_rk_info;
+
+/* Per-element arrays in cluster order (host pointers). hn may be NULL (sigmaconst). */
+int hbi_rk_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_rk_opts *opts,
+                  const double *a, const double *b, const double *dc, const double *f0,
+                  const double *vpl, const double *taudot, const double *sigdot,
+                  const double *area, hbi_rk **out);
+int hbi_rk_destroy(hbi_rk *r);
+/* y: interleaved (psi, tau, sigma) x N; slip, vel: N (may be NULL -> 0 / from y) */
+int hbi_rk_set_state(hbi_rk *r, const double *y, const double *slip, const double *vel,
+                     double t, double dt_next);
+int hbi_rk_get_state(hbi_rk *r, double *y, double *slip, double *vel, hbi_rk_info *info);
+int hbi_rk_derivs(hbi_rk *r, const double *y, double *dydx);
+int hbi_rk_rkck(hbi_rk *r, const double *y, double h, double *yout, double *yerr);
+/* Advance nsteps accepted steps (rkqs + post-step update) through a captured
+   HIP graph of one attempt; no host synchronisation inside.  When hist is
+   not NULL it receives nsteps rows of (t, dt, errmax, nrjct, log10 Vmax,
+   mean |slip|), and vhist nsteps x nrec slip rates at cluster indices rec. */
+int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, double *vhist,
+                hbi_rk_info *info);
+/* Device time of the last hbi_rk_step call, milliseconds (HIP events) */
+int hbi_rk_last_ms(const hbi_rk *r, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HBI_H */

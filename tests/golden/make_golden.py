@@ -1,0 +1,87 @@
+"""Write golden kernel vectors from the reference's own entry function.
+
+Runs HACApK_entry_ij of the reference, compiled unmodified from its sources
+into oracle/_ref/libhbiref.so by oracle/Makefile (``make -C oracle ref``), on
+sampled (receiver, source) pairs of the three benchmark geometries and stores
+geometry + pairs + values as compressed .npz fixtures next to this script.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from hbi_amd import problems as P  # noqa: E402
+from oracle import oracle_lib as O  # noqa: E402
+
+
+def sample_pairs(n, rng, nrand, near=None):
+    ii = list(rng.integers(0, n, nrand))
+    jj = list(rng.integers(0, n, nrand))
+    if near is not None:
+        for i, j in near:
+            ii.append(i)
+            jj.append(j)
+    return np.asarray(ii, dtype=np.int32), np.asarray(jj, dtype=np.int32)
+
+
+def save(name, g, ii, jj, vals, **extra):
+    arrays = {("geo_" + k): v for k, v in g.arrays.items()}
+    np.savez_compressed(os.path.join(HERE, name), kind=g.kind, n=g.n, w=g.w, vcomp=g.vcomp, md=g.md, ii=ii, jj=jj,
+                        vals=vals, **arrays, **extra)
+    print(name, g.n, len(ii), "nan:", int(np.isnan(vals).sum()))
+
+
+def main():
+    O.build()
+    assert O.ref_available(), "oracle/_ref/libhbiref.so missing (needs /root/reference)"
+    rng = np.random.default_rng(20251203)
+    # C1: 2dvs, N=2048 over 40 km: all self terms, neighbours and random pairs
+    g = P.mesh_2dvs(2048, 40.0 / 2048)
+    near = [(i, i) for i in range(0, 2048, 7)] + [(i, i + 1) for i in range(0, 2047, 13)] + \
+           [(i + 1, i) for i in range(0, 2047, 17)]
+    ii, jj = sample_pairs(g.n, rng, 3000, near)
+    save("kernel_entries_2dvs.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    # C2: 3dph 256 x 128, ds 0.390625 km, vertical: self, in-plane neighbours, surface row, random
+    imax, jmax, ds0 = 256, 128, 0.390625
+    g = P.mesh_3dph(imax, jmax, ds0, 90.0)
+    near = []
+    for i in rng.integers(0, g.n, 300):
+        i = int(i)
+        for dj in (0, 1, -1, jmax, -jmax, jmax + 1, 2, 2 * jmax):
+            j = i + dj
+            if 0 <= j < g.n:
+                near.append((i, j))
+    near += [(k * jmax, m * jmax) for k in range(0, imax, 17) for m in range(0, imax, 23)]   # surface row
+    ii, jj = sample_pairs(g.n, rng, 4000, near)
+    save("kernel_entries_3dph.npz", g, ii, jj, O.ref_entries(g, ii, jj), imax=imax, jmax=jmax, ds0=ds0)
+    # the same mesh, dip 60 degrees, and normal traction, small
+    g = P.mesh_3dph(32, 16, 1.0, 60.0, crake=30.0)
+    g.vcomp = 1
+    ii, jj = sample_pairs(g.n, rng, 1500, [(i, i) for i in range(0, g.n, 5)])
+    save("kernel_entries_3dph_dip60_n.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    # C3: rough triangular surface (Hurst 0.8, 1e-2), small patch
+    tri = P.rough_surface_triangles(6.0, 3.0, 0.25, hurst=0.8, alpha=1e-2, seed=11)
+    g = P.mesh_triangles(tri, crake=0.0)
+    ii, jj = sample_pairs(g.n, rng, 3000, [(i, i) for i in range(0, g.n, 3)] + [(i, i + 1) for i in range(0, g.n - 1, 5)])
+    save("kernel_entries_3dnt.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    g.vcomp = 1
+    ii, jj = sample_pairs(g.n, rng, 1000)
+    save("kernel_entries_3dnt_n.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    # TD slip modes 'st', 'dp', 'open' on the same patch
+    for md, name in ((1, "st"), (2, "dp"), (3, "open")):
+        g.vcomp = 0
+        g.md = md
+        ii, jj = sample_pairs(g.n, rng, 500)
+        save("kernel_entries_3dnt_md%s.npz" % name, g, ii, jj, O.ref_entries(g, ii, jj))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,276 @@
+"""CPU tests of the oracle (oracle/hbi_oracle.c) and of the host-side problem
+generators.  Every check here is chosen so that a plausible mistake in the
+restatement (dropped term, wrong sign or index, transposed operand) fails it:
+
+* kernels: against golden vectors written by the reference's own compiled
+  HACApK_entry_ij (tests/golden), and TD (two triangles) against Okada
+  full space (one rectangle): two independent closed forms;
+* partition: leaf counts recounted independently for the benchmark meshes,
+  cover / disjointness / admissibility invariants;
+* ACA+: H x against the dense product;
+* RK45: tableau identities, convergence order, the errmax guard;
+* friction: steady state and finite-difference derivatives.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_FILES, load_golden
+from hbi_amd import problems as P
+from oracle import oracle_lib as O
+
+
+@pytest.mark.parametrize("name", GOLDEN_FILES)
+def test_oracle_kernels_match_reference_golden(name):
+    g, ii, jj, vals = load_golden(name)
+    got = O.entries(g, ii, jj)
+    if "_md" not in name:
+        # the slip modes of the benchmark problems: bit-identical to the reference
+        same = (got == vals) | (np.isnan(got) & np.isnan(vals))
+        assert same.all(), "%d of %d differ, max |d| %g" % ((~same).sum(), len(vals), np.nanmax(np.abs(got - vals)))
+    else:
+        # 'st', 'dp', 'open' modes: last-bit differences from the Fortran evaluation order
+        assert np.allclose(got, vals, rtol=1e-13, atol=1e-13 * np.nanmax(np.abs(vals)), equal_nan=True)
+
+
+def test_reference_harness_matches_oracle_when_available():
+    if not O.ref_available():
+        pytest.skip("reference sources not present in this environment")
+    g = P.mesh_3dph(16, 8, 0.5, 75.0, crake=10.0)
+    rng = np.random.default_rng(3)
+    ii, jj = rng.integers(0, g.n, 500), rng.integers(0, g.n, 500)
+    assert np.array_equal(O.ref_entries(g, ii, jj), O.entries(g, ii, jj))
+
+
+def test_td_two_triangles_equal_okada_fullspace_rectangle():
+    ds = 1.0
+    g = P.mesh_3dph(8, 8, ds, 90.0, fullspace=True)
+    g.arrays["zcol"] = g.arrays["zcol"] - 50.0
+    n = g.n
+    tri = []
+    for k in range(n):
+        x0, x1 = g.xcol[k] - ds / 2, g.xcol[k] + ds / 2
+        z0, z1 = g.zcol[k] - ds / 2, g.zcol[k] + ds / 2
+        a, b, c, d = (x0, 0.0, z0), (x1, 0.0, z0), (x1, 0.0, z1), (x0, 0.0, z1)
+        tri += [(a, b, c), (a, c, d)]
+    gt = P.mesh_triangles(np.array(tri))
+    for f, src in (("xcol", g.xcol), ("ycol", np.zeros(n)), ("zcol", g.zcol)):
+        a = gt.arrays[f].copy()
+        a[0::2] = src
+        a[1::2] = src
+        gt.arrays[f] = a
+    rng = np.random.default_rng(1)
+    ii, jj = rng.integers(0, n, 400), rng.integers(0, n, 400)
+    keep = ii != jj     # a square's centre lies on its triangles' shared edge (TD gives NaN there)
+    ii, jj = ii[keep], jj[keep]
+    ok = O.entries(g, ii, jj)
+    td = O.entries(gt, 2 * ii, 2 * jj) + O.entries(gt, 2 * ii, 2 * jj + 1)
+    assert np.allclose(td, ok, rtol=1e-9, atol=1e-12 * np.abs(ok).max())
+    # the image (half-space) part is what distinguishes 3dph from 3dnr near the surface
+    g.kind = P.KIND_OKADA_HS
+    g.arrays["zcol"] = g.arrays["zcol"] + 50.0 - 0.6
+    assert not np.allclose(O.entries(g, ii, jj), O.entries(P.Geometry(kind=P.KIND_OKADA_FS, n=n, w=g.w,
+                                                                       arrays=g.arrays), ii, jj))
+
+
+def test_antiplane_closed_form_and_image():
+    g = P.mesh_2dvs(64, 0.5)
+    rng = np.random.default_rng(2)
+    ii, jj = rng.integers(0, 64, 300), rng.integers(0, 64, 300)
+    xc, xl, xr = g.xcol[ii], g.xel[jj], g.xer[jj]
+    f = 32.04 / (2 * math.pi)
+    expect = f * (1 / (xc - xr) - 1 / (xc - xl)) - f * (1 / (xc + xr) - 1 / (xc + xl))
+    assert np.allclose(O.entries(g, ii, jj), expect, rtol=1e-13)
+    # self term is negative (restoring), the image weakens it near the surface
+    d = O.entries(g, np.arange(64), np.arange(64))
+    assert (d < 0).all() and d[0] > d[-1]
+
+
+def test_mesh_generators_match_oracle():
+    imax, jmax, ds0 = 12, 7, 0.390625
+    g = P.mesh_3dph(imax, jmax, ds0, 90.0)
+    n = imax * jmax
+    bufs = [np.empty(n) for _ in range(5)]
+    O.lib().og_mesh_3ddip(imax, jmax, ds0, 90.0, *[O._ptr(b) for b in bufs])
+    for k, f in enumerate(("xcol", "ycol", "zcol", "ang", "angd")):
+        assert np.array_equal(getattr(g, f), bufs[k]), f
+    g2 = P.mesh_2dvs(100, 0.4)
+    b2 = [np.empty(100) for _ in range(3)]
+    O.lib().og_mesh_2dp(100, 0.4, *[O._ptr(b) for b in b2])
+    assert np.array_equal(g2.xel, b2[0]) and np.array_equal(g2.xer, b2[1]) and np.array_equal(g2.xcol, b2[2])
+
+
+# ---------------------------------------------------------------- partition --
+def _counts(L):
+    lr, de = L[L[:, 0] == 1], L[L[:, 0] == 2]
+    return len(lr), int((lr[:, 2] + lr[:, 4]).sum()), len(de), int((de[:, 2] * de[:, 4]).sum())
+
+
+@pytest.mark.parametrize("leaf,expect", [(15, (720, 61632, 382, 97792)), (32, (156, 37632, 376, 385024))])
+def test_partition_counts_2dvs_2048(leaf, expect):
+    H = O.HMatrix(P.mesh_2dvs(2048, 40 / 2048), leaf=leaf, partition_only=True)
+    assert _counts(H.leaves) == expect
+
+
+def test_partition_counts_3dph_32768_leaf64():
+    H = O.HMatrix(P.mesh_3dph(256, 128, 0.390625), leaf=64, partition_only=True)
+    assert _counts(H.leaves) == (4522, 2599424, 8520, 34897920)
+
+
+@pytest.mark.parametrize("make", [lambda: P.mesh_2dvs(300, 0.1), lambda: P.mesh_3dph(20, 13, 1.0, 70.0),
+                                  lambda: P.mesh_triangles(P.rough_surface_triangles(4.0, 2.0, 0.25, seed=5))])
+def test_partition_invariants(make):
+    g = make()
+    leaf = 8
+    H = O.HMatrix(g, leaf=leaf, partition_only=True)
+    L = H.leaves
+    lod = H.lod
+    assert sorted(lod) == list(range(g.n))
+    cov = np.zeros((g.n, g.n), dtype=np.int32)
+    for lt, r0, nr, c0, nc, _ in L:
+        cov[r0:r0 + nr, c0:c0 + nc] += 1
+        if lt == 1:   # low rank: both sizes > leaf, not index-adjacent, admissible
+            assert nr > leaf and nc > leaf and r0 + nr != c0 and c0 + nc != r0
+            xyz = np.stack([g.xcol, g.ycol, g.zcol], 1)[lod]
+            a, b = xyz[r0:r0 + nr], xyz[c0:c0 + nc]
+            da = np.linalg.norm(a.max(0) - a.min(0))
+            db = np.linalg.norm(b.max(0) - b.min(0))
+            gap = np.maximum(0, np.maximum(b.min(0) - a.max(0), a.min(0) - b.max(0)))
+            assert min(da, db) <= 2.0 * np.linalg.norm(gap) + 1e-12
+    assert (cov == 1).all()
+    key = L[:, 1].astype(np.int64) * g.n + L[:, 3]
+    assert (np.diff(key) > 0).all()
+
+
+# --------------------------------------------------------------------- ACA+ --
+@pytest.mark.parametrize("make,leaf,eps,tol", [
+    (lambda: P.mesh_2dvs(512, 40 / 512), 16, 1e-6, 1e-9),
+    (lambda: P.mesh_3dph(32, 16, 1.5), 32, 1e-5, 1e-6),
+    (lambda: P.mesh_triangles(P.rough_surface_triangles(8.0, 4.0, 0.25, seed=9)), 32, 1e-4, 1e-4)])
+def test_oracle_hmatrix_against_dense(make, leaf, eps, tol):
+    g = make()
+    H = O.HMatrix(g, leaf=leaf, eps=eps, nthreads=4)
+    A = H.dense()
+    rng = np.random.default_rng(0)
+    for _ in range(2):
+        x = rng.standard_normal(g.n)
+        y = H.matvec(x)
+        assert np.linalg.norm(y - A @ x) <= tol * np.linalg.norm(A @ x)
+    # the threaded product adds the same leaf contributions in another grouping
+    x = rng.standard_normal(g.n)
+    assert np.allclose(H.matvec(x, 4), H.matvec(x, 1), rtol=1e-12, atol=1e-12 * np.abs(H.matvec(x)).max())
+    L = H.leaves
+    assert (L[L[:, 0] == 1][:, 5] >= 1).all()
+
+
+# ---------------------------------------------------------------------- RK --
+def test_tableau_identities():
+    t = O.tableau()
+    a2, a3, a4, a5, a6, b21, b31, b32, b41, b42, b43, b51, b52, b53, b54, b61, b62, b63, b64, b65, \
+        c1, c3, c4, c6, dc1, dc3, dc4, dc5, dc6 = t
+    # row sums equal the nodes to single precision (the reference's literals are default real)
+    for row, a in (((b21,), a2), ((b31, b32), a3), ((b41, b42, b43), a4), ((b51, b52, b53, b54), a5),
+                   ((b61, b62, b63, b64, b65), a6)):
+        assert abs(sum(row) - a) < 5e-7
+    assert abs(c1 + c3 + c4 + c6 - 1) < 5e-7
+    assert abs(dc1 + dc3 + dc4 + dc5 + dc6) < 5e-7
+    # the coefficients carry single-precision rounding: B31 is not 3/40 in double
+    assert b31 == float(np.float32(3) / np.float32(40)) and b31 != 3 / 40
+
+
+def _slider(law=0, n=1):
+    """Spring-slider: N=1 antiplane element, H = K (1x1)."""
+    g = P.mesh_2dvs(n, 1.0)
+    H = O.HMatrix(g, leaf=15, eps=1e-8)
+    fr = P.friction_c1(g)
+    fr.a[:] = 0.01
+    fr.b[:] = 0.015
+    fr.law = law
+    lod = H.lod
+    sim = O.Sim(H, fr, lod, backslip=True)
+    y = P.init_state(fr, lod)
+    y[0::3] += 0.02   # off steady state
+    return H, fr, sim, y
+
+
+def test_rkck_convergence_order():
+    """One step of h against two of h/2: the difference shrinks ~16x per
+    halving in this regime (h lambda ~ 0.1-0.5, 5th-order solution); a missing
+    or mis-indexed stage term drops the ratio to 2-8."""
+    _, _, sim, y = _slider()
+    y[0::3] -= 0.12     # V ~ 2e-5 m/s: dynamics on a ~400 s scale
+    errs = []
+    for hh in (200.0, 100.0, 50.0):
+        y1, _ = sim.rkck(y, hh)
+        y2, _ = sim.rkck(y, hh / 2)
+        y2, _ = sim.rkck(y2, hh / 2)
+        errs.append(np.abs(y1 - y2).max())
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert r1 > 12 and r2 > 12, (errs, r1, r2)
+
+
+def test_embedded_error_estimate_decreases_with_h():
+    _, _, sim, y = _slider()
+    y[0::3] -= 0.12
+    e = [np.abs(sim.rkck(y, h)[1]).max() for h in (100.0, 50.0, 25.0)]
+    assert e[0] / e[1] > 10 and e[1] / e[2] > 10, e
+
+
+def test_errmax_guard_semantics():
+    L = O.lib()
+    eps = 1e-6
+    y = np.array([1.0, 2.0, np.inf, 4.0, 5.0])
+    ye = np.array([1e-6, 5e-5, 0.0, 3e-6, 1e-7]) * y
+    ye[2] = 0.0
+    # element 2 is the guard (inf - 1 == inf): errmax is overwritten with 10 and
+    # only later elements can raise it again
+    em = L.og_errmax(5, O._ptr(np.ascontiguousarray(y)), O._ptr(np.ascontiguousarray(ye)), eps)
+    assert em == 10.0
+    ye2 = ye.copy()
+    ye2[4] = 2e-5 * y[4]
+    em = L.og_errmax(5, O._ptr(np.ascontiguousarray(y)), O._ptr(np.ascontiguousarray(ye2)), eps)
+    assert abs(em - 20.0) < 1e-9
+
+
+def test_deriv_steady_state_and_derivatives():
+    L = O.lib()
+    vref, a, b, dc, f0, sig, V = 1e-6, 0.01, 0.015, 0.008, 0.6, 50.0, 1e-9
+    psi = f0 - b * math.log(V / vref)
+    tau = a * sig * math.asinh(V / (2 * vref) * math.exp(psi / a))
+    out = np.zeros(3)
+    p = [O._ptr(out[i:i + 1]) for i in range(3)]
+    vel = 2 * vref * math.exp(-psi / a) * math.sinh(tau / sig / a)
+    L.og_deriv(0, 1, vref, 1.0, 300.0, 0.0, 0.0, psi, tau, sig, vel, a, b, dc, f0, *p)
+    assert abs(out[0]) < 1e-12 * (b / dc * V)      # aging law at steady state
+    L.og_deriv(1, 1, vref, 1.0, 300.0, 0.0, 0.0, psi, tau, sig, vel, a, b, dc, f0, *p)
+    assert abs(out[0]) < 1e-10 * (V / dc)          # slip law at steady state
+    # quasi-dynamic balance: dtau (1 + eta dV/dtau) + eta (dV/dpsi dpsi + dV/dsig dsig) = gs
+    gs, gn = 3e-3, 2e-3
+    psi2, tau2 = psi + 0.01, tau * 1.01
+    V2 = 2 * vref * math.exp(-psi2 / a) * math.sinh(tau2 / sig / a)
+    L.og_deriv(0, 1, vref, 1.0, 300.0, gs, gn, psi2, tau2, sig, V2, a, b, dc, f0, *p)
+    Vf = lambda ps, ta, si: 2 * vref * math.exp(-ps / a) * math.sinh(ta / si / a)
+    hh = 1e-7
+    dvdtau = (Vf(psi2, tau2 * (1 + hh), sig) - Vf(psi2, tau2 * (1 - hh), sig)) / (2 * tau2 * hh)
+    dvdpsi = (Vf(psi2 + hh, tau2, sig) - Vf(psi2 - hh, tau2, sig)) / (2 * hh)
+    dvdsig = (Vf(psi2, tau2, sig * (1 + hh)) - Vf(psi2, tau2, sig * (1 - hh))) / (2 * sig * hh)
+    eta = 0.5 * 32.04 / 3.464
+    lhs = out[1] * (1 + eta * dvdtau) + eta * (dvdpsi * out[0] + dvdsig * out[2])
+    assert out[2] == gn
+    assert abs(lhs - gs) < 1e-6 * abs(gs)
+
+
+def test_oracle_spring_slider_cycle_runs_and_rejects_bad_state():
+    H, fr, sim, y = _slider()
+    n = 1
+    slip = np.zeros(n)
+    vel = np.array([2 * fr.vref * math.exp(-y[0] / fr.a[0]) * math.sinh(y[1] / y[2] / fr.a[0])])
+    done, x, dt, hist, _ = sim.run(y, 0.0, 1.0, 1e-6, 300, slip, vel, np.ones(n))
+    assert done == 300 and x > 0 and (np.diff(hist[:, 0]) > 0).all()
+    # a NaN state is never accepted and ends in step-size underflow (the
+    # reference stops there, main_LH.f90:2348-2352)
+    ybad = y.copy()
+    ybad[1] = np.nan
+    st, *_ = sim.rkqs(ybad, 0.0, 1.0, 1e-6)
+    assert st == -1
