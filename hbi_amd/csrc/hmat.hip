@@ -579,6 +579,7 @@ static void build_plan(hbi_hmat *h, int prec)
             P.it1.push_back(it);
         }
         P.alg_bytes += (long long)f.nc * f.kt * sz;
+        P.alg_bytes_p1 += (long long)f.nc * f.kt * sz;
     }
     // ---- phase 2 segments per tile
     std::vector<std::vector<int>> tl(nt);
@@ -617,11 +618,13 @@ static void build_plan(hbi_hmat *h, int prec)
                     J.which = 1; J.src = h->fac_off[ip] + (long long)f.nc * f.kt + (lo - f.r0);
                     J.src_rs = 1; J.src_cs = f.nr;
                     P.alg_bytes += (long long)(hi - lo) * f.kt * sz;
+                    P.alg_bytes_p2 += (long long)(hi - lo) * f.kt * sz;
                 } else {
                     sg.src = 0; sg.xoff = f.c0;
                     J.which = 0; J.src = h->fac_off[ip] + (long long)(lo - f.r0) * f.nc;
                     J.src_rs = f.nc; J.src_cs = 1;
                     P.alg_bytes += (long long)(hi - lo) * f.nc * sz;
+                    P.alg_bytes_p2 += (long long)(hi - lo) * f.nc * sz;
                 }
                 jobs.push_back(J);
                 elems += (long long)TR * ncol;
@@ -635,6 +638,8 @@ static void build_plan(hbi_hmat *h, int prec)
     }
     P.tile_part0[nt] = P.nparts;
     P.alg_bytes += 2LL * h->n * 8;   // x read, y written (FP64)
+    P.alg_bytes_p1 += (long long)h->n * 8 + (long long)h->nz * 8;
+    P.alg_bytes_p2 += 2LL * h->n * 8;
     for (int ip = 0; ip < (int)L.size(); ip++)
         if (leaf_dtype(ip) == kBF16) P.nbf16++;
     // largest items first for load balance
@@ -959,6 +964,8 @@ static void build(hbi_hmat *h, const hbi_geom *g, const hbi_hmat_opts *o)
     for (int p = 0; p < 3; p++) {
         S.stored_bytes[p] = (long long)h->plan[p].bytes;
         S.alg_bytes[p] = h->plan[p].alg_bytes;
+        S.alg_bytes_p1[p] = h->plan[p].alg_bytes_p1;
+        S.alg_bytes_p2[p] = h->plan[p].alg_bytes_p2;
     }
     S.nbf16_blocks = h->plan[HBI_PREC_MIXED].nbf16;
 }

@@ -45,7 +45,9 @@ struct Plan {
     DevBuf<Item2> d_it2;
     DevBuf<int> d_tile_part0;
     int nparts = 0;
-    long long alg_bytes = 0;
+    long long alg_bytes = 0;      // whole product: all factor bytes + x read + y written
+    long long alg_bytes_p1 = 0;   // phase-1 kernel: a1 factors + x + z
+    long long alg_bytes_p2 = 0;   // phase-2 kernel: dense + a2 factors + x + y
     int nbf16 = 0;
 };
 

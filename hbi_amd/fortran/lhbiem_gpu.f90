@@ -1,0 +1,406 @@
+! lhbiem_gpu.f90 -- Fortran host of the GPU hot path: `lhbiem_gpu <file.in>`.
+!
+! Reads the reference's input formats (the `key value` .in file of
+! main_LH.f90:182-328 with its defaults 133-180, the column-named parameter
+! file 530-581, ASCII STL facets 485-510), generates the meshes of the
+! reference (2dp/2dvs 1701-1718, 3dph 1781-1835, STL frames 1940-1990), builds
+! the H-matrix through the HACApK call sites (m_HACApK_gpu) and advances the
+! quasi-dynamic earthquake cycle with the device RK45 (hbi_rk_step), writing
+! output/monitor<n>.dat (main_LH.f90:1489-1493 columns) and the final
+! `time(s)` line.  `forward T` writes H*1 (main_LH.f90:2586-2664) and stops.
+! Keys beyond the reference's: leaf (param(21)), precision (f64|f32|mixed),
+! bf16margin, outbatch (steps per device batch).
+program lhbiem_gpu
+  use iso_c_binding
+  use m_hbi_abi
+  use m_HACApK_gpu
+  implicit none
+  real(8), parameter :: pi = 4 * datan(1.d0), rigid = 32.04d0
+  character(len=256) :: input_file, param, pvalue, fname, dum, precision, evlaw, problem, geofile, parameter_file
+  character(len=128) :: param2(20)
+  integer :: ios, number, ncellg, imax, jmax, nstep, i, k, ncol, leaf, outbatch, nb, nrec_dummy(1)
+  real(8) :: ds0, eps_r, eps_h, vref, velmax, velmin, tmax, dtinit, dtmax, crake, dipangle, maxsig, minsig
+  real(8) :: a0, b0, dc0, mu0, vpl, velinit, tauinit, sigmainit, sr, bf16margin, t1, t2, sg
+  logical :: backslip, sigmaconst, forward, parameterfromfile, limitsigma
+  type(st_HACApK_calc_entry) :: st_bemv
+  type(st_HACApK_lcontrol) :: st_ctl
+  type(st_HACApK_leafmtxp) :: st_leafmtxp_s
+  type(st_HACApK_LHp) :: st_LHp_s
+  type(st_HACApK_latticevec) :: st_vel, st_sum
+  real(8), allocatable :: coord(:, :), wws(:), values(:, :), y(:), slip(:), vel(:), hist(:), vhist(:)
+  real(8), allocatable, target :: xcol(:), ycol(:), zcol(:), xel(:), xer(:), ang(:), angd(:), dsl(:), dsd(:), rake(:)
+  real(8), allocatable, target :: xs(:, :), ev(:, :)
+  real(8), allocatable :: ds(:), ag(:), bg(:), dcg(:), f0g(:), taug(:), sigmag(:), velg(:), vplg(:), taudotg(:), sigdotg(:)
+  real(8), allocatable :: a(:), b(:), dc(:), f0(:), vplv(:), taudot(:), sigdot(:), area(:)
+  type(hbi_rk_opts) :: ro
+  type(hbi_rk_info) :: info
+  type(c_ptr) :: rk
+  integer(c_int) :: rc, nrec
+  integer :: lrtrn, i_, kdone
+
+  call get_command_argument(1, input_file)
+  open(33, file=input_file, status='old', iostat=ios)
+  if (ios /= 0) then
+    write(*, *) 'ERROR: Failed to open input file'
+    stop 1
+  end if
+  number = 0
+  ! defaults (main_LH.f90:133-180); vref, velth are single-precision literals there
+  eps_r = 1d-4; eps_h = 1d-4; vref = 1e-6; velmax = 1d7; velmin = 1d-20; tmax = 1d4; dtinit = 1d0; dtmax = 1e10
+  maxsig = 300d0; minsig = 1d0; sigmaconst = .false.; forward = .false.; parameterfromfile = .false.
+  limitsigma = .true.; backslip = .true.; evlaw = 'aging'; crake = 0d0; dipangle = 90d0; sr = 0d0
+  geofile = 'default'; parameter_file = 'default'; ncol = 0; nstep = 1000; leaf = 15
+  precision = 'f64'; bf16margin = 1d0; outbatch = 200
+  a0 = 0.01d0; b0 = 0.015d0; dc0 = 0.01d0; mu0 = 0.6d0; vpl = 1d-9; velinit = 1d-9; tauinit = 0d0; sigmainit = 50d0
+  imax = 0; jmax = 0; ncellg = 0; ds0 = 0d0; problem = ' '
+  do
+    read(33, *, iostat=ios) param, pvalue
+    if (ios /= 0) exit
+    select case (trim(param))
+    case ('problem'); read(pvalue, *) problem
+    case ('ncellg'); read(pvalue, *) ncellg
+    case ('imax'); read(pvalue, *) imax
+    case ('jmax'); read(pvalue, *) jmax
+    case ('nstep'); read(pvalue, *) nstep
+    case ('filenumber'); read(pvalue, *) number
+    case ('ds'); read(pvalue, *) ds0
+    case ('velmax'); read(pvalue, *) velmax
+    case ('velmin'); read(pvalue, *) velmin
+    case ('a'); read(pvalue, *) a0
+    case ('b'); read(pvalue, *) b0
+    case ('dc'); read(pvalue, *) dc0
+    case ('f0'); read(pvalue, *) mu0
+    case ('vref'); read(pvalue, *) vref
+    case ('vpl'); read(pvalue, *) vpl
+    case ('sr'); read(pvalue, *) sr
+    case ('velinit'); read(pvalue, *) velinit
+    case ('tauinit'); read(pvalue, *) tauinit
+    case ('sigmainit'); read(pvalue, *) sigmainit
+    case ('dtinit'); read(pvalue, *) dtinit
+    case ('dtmax'); read(pvalue, *) dtmax
+    case ('tmax'); read(pvalue, *) tmax
+    case ('eps_r'); read(pvalue, *) eps_r
+    case ('eps_h'); read(pvalue, *) eps_h
+    case ('backslip'); read(pvalue, *) backslip
+    case ('limitsigma'); read(pvalue, *) limitsigma
+    case ('sigmaconst'); read(pvalue, *) sigmaconst
+    case ('forward'); read(pvalue, *) forward
+    case ('maxsig'); read(pvalue, *) maxsig
+    case ('minsig'); read(pvalue, *) minsig
+    case ('crake'); read(pvalue, *) crake
+    case ('dipangle'); read(pvalue, *) dipangle
+    case ('geometry_file'); read(pvalue, '(a)') geofile
+    case ('parameterfromfile'); read(pvalue, *) parameterfromfile
+    case ('parameter_file'); read(pvalue, '(a)') parameter_file
+    case ('parameter_file_ncol'); read(pvalue, *) ncol
+    case ('evlaw'); read(pvalue, '(a)') evlaw
+    case ('leaf'); read(pvalue, *) leaf
+    case ('precision'); read(pvalue, '(a)') precision
+    case ('bf16margin'); read(pvalue, *) bf16margin
+    case ('outbatch'); read(pvalue, *) outbatch
+    case default
+      write(*, *) 'WARNING: ', trim(param), ' is an unknown parameter'
+    end select
+  end do
+  close(33)
+  tmax = tmax * 365 * 24 * 3600
+  if (parameter_file == 'default') write(parameter_file, '("examples/param",i0,".dat")') number
+
+  ! ---------------------------------------------------------------- mesh --
+  select case (trim(problem))
+  case ('3dph')
+    ncellg = imax * jmax
+  case ('3dnt')
+    ncellg = count_stl_facets(geofile)
+  end select
+  if (ncellg == 0) then
+    write(*, *) 'ERROR: Ncellg is zero'
+    stop 1
+  end if
+  allocate(xcol(ncellg), ycol(ncellg), zcol(ncellg), ds(ncellg), rake(ncellg))
+  xcol = 0d0; ycol = 0d0; zcol = 0d0
+  select case (trim(problem))
+  case ('2dp', '2dvs')
+    allocate(xel(ncellg), xer(ncellg))
+    do i = 1, ncellg
+      ds(i) = ds0
+      xel(i) = (i - 1) * ds0
+      xer(i) = i * ds0
+      xcol(i) = 0.5d0 * (xel(i) + xer(i))
+    end do
+  case ('3dph')
+    allocate(ang(ncellg), angd(ncellg), dsl(ncellg), dsd(ncellg))
+    call mesh_3ddip(imax, jmax, ds0, dipangle, xcol, ycol, zcol, ang, angd)
+    ds = ds0 * ds0
+    dsl = ds0
+    dsd = ds0
+  case ('3dnt')
+    allocate(xs(ncellg, 9), ev(ncellg, 9))
+    call read_stl(geofile, ncellg, xs, xcol, ycol, zcol)
+    call evcalc(ncellg, xs, ev, ds)
+  case default
+    write(*, *) 'ERROR: unsupported problem ', trim(problem)
+    stop 1
+  end select
+  rake = crake
+  allocate(ag(ncellg), bg(ncellg), dcg(ncellg), f0g(ncellg), taug(ncellg), sigmag(ncellg), velg(ncellg), vplg(ncellg))
+  allocate(taudotg(ncellg), sigdotg(ncellg))
+  ag = a0; bg = b0; dcg = dc0; f0g = mu0; vplg = vpl; velg = velinit; sigmag = sigmainit; taug = tauinit
+  taudotg = 0d0; sigdotg = 0d0
+  if (.not. backslip) taudotg = sr
+  if (parameterfromfile) then
+    open(99, file=parameter_file, status='old', iostat=ios)
+    if (ios /= 0) then
+      write(*, *) 'ERROR: Failed to open parameter file'
+      stop 1
+    end if
+    read(99, *) param2(1:ncol)
+    allocate(values(ncellg, ncol))
+    do i = 1, ncellg
+      read(99, *) values(i, 1:ncol)
+    end do
+    close(99)
+    do k = 1, ncol
+      select case (trim(param2(k)))
+      case ('rake'); rake = values(:, k)
+      case ('a'); ag = values(:, k)
+      case ('b'); bg = values(:, k)
+      case ('dc'); dcg = values(:, k)
+      case ('f0'); f0g = values(:, k)
+      case ('tau'); taug = values(:, k)
+      case ('sigma'); sigmag = values(:, k)
+      case ('vel'); velg = values(:, k)
+      case ('vpl'); vplg = values(:, k)
+      case ('taudot'); taudotg = values(:, k)
+      case ('sigmadot'); sigdotg = values(:, k)
+      end select
+    end do
+  end if
+  rake = rake / 180d0 * pi
+
+  ! ------------------------------------------------------------ H-matrix --
+  lrtrn = HACApK_init(ncellg, st_ctl, st_bemv)
+  st_ctl%param(21) = leaf
+  st_ctl%param(P_BF16) = bf16margin
+  select case (trim(precision))
+  case ('f32')
+    st_ctl%param(P_PREC) = HBI_PREC_F32
+  case ('mixed')
+    st_ctl%param(P_PREC) = HBI_PREC_MIXED
+  case default
+    st_ctl%param(P_PREC) = HBI_PREC_F64
+  end select
+  st_bemv%problem = problem
+  st_bemv%xcol => xcol; st_bemv%ycol => ycol; st_bemv%zcol => zcol; st_bemv%rake => rake
+  select case (trim(problem))
+  case ('2dp', '2dvs')
+    st_bemv%problem = '2dvs'
+    st_bemv%xel => xel; st_bemv%xer => xer
+  case ('3dph')
+    st_bemv%ang => ang; st_bemv%angd => angd; st_bemv%dsl => dsl; st_bemv%dsd => dsd
+    st_bemv%w = ds0
+  case ('3dnt')
+    st_bemv%xs1 => xs(:, 1); st_bemv%xs2 => xs(:, 2); st_bemv%xs3 => xs(:, 3)
+    st_bemv%ys1 => xs(:, 4); st_bemv%ys2 => xs(:, 5); st_bemv%ys3 => xs(:, 6)
+    st_bemv%zs1 => xs(:, 7); st_bemv%zs2 => xs(:, 8); st_bemv%zs3 => xs(:, 9)
+    st_bemv%ev11 => ev(:, 1); st_bemv%ev12 => ev(:, 2); st_bemv%ev13 => ev(:, 3)
+    st_bemv%ev21 => ev(:, 4); st_bemv%ev22 => ev(:, 5); st_bemv%ev23 => ev(:, 6)
+    st_bemv%ev31 => ev(:, 7); st_bemv%ev32 => ev(:, 8); st_bemv%ev33 => ev(:, 9)
+  end select
+  allocate(coord(ncellg, 3))
+  coord(:, 1) = xcol; coord(:, 2) = ycol; coord(:, 3) = zcol
+  st_bemv%md = 's'
+  st_bemv%v = 's'
+  call cpu_time(t1)
+  lrtrn = HACApK_generate(st_leafmtxp_s, st_bemv, st_ctl, coord, eps_h)
+  if (lrtrn /= 0) stop 2
+  lrtrn = HACApK_construct_LH(st_LHp_s, st_leafmtxp_s, st_bemv, st_ctl, coord, eps_h)
+  allocate(wws(ncellg))
+  lrtrn = HACApK_gen_lattice_vector(st_vel, st_leafmtxp_s, st_ctl)
+  lrtrn = HACApK_gen_lattice_vector(st_sum, st_leafmtxp_s, st_ctl)
+  write(*, *) 'NCELLg', ncellg, ' H-matrix built on the GPU'
+  call system('mkdir -p output')
+
+  if (forward) then
+    ! H*1 with the okada scaling, main_LH.f90:2641-2664
+    st_vel%vs = 1d0
+    call HACApK_adot_lattice_hyp(st_sum, st_LHp_s, st_ctl, wws, st_vel)
+    if (trim(problem) == '3dph') st_sum%vs = st_sum%vs / ds0
+    write(fname, '("output/stress",i0)') number
+    open(29, file=fname)
+    do i = 1, ncellg
+      i_ = st_sum%lodc(i)
+      write(29, '(6e16.4)') xcol(i_), ycol(i_), zcol(i_), 1d0, st_sum%vs(i), 0d0
+    end do
+    close(29)
+    lrtrn = HACApK_free_leafmtxp(st_leafmtxp_s)
+    stop
+  end if
+
+  ! ----------------------------------------------------- ODE in cluster order --
+  allocate(a(ncellg), b(ncellg), dc(ncellg), f0(ncellg), vplv(ncellg), taudot(ncellg), sigdot(ncellg), area(ncellg))
+  allocate(y(3 * ncellg), slip(ncellg), vel(ncellg))
+  do i = 1, ncellg
+    i_ = st_sum%lodc(i)
+    a(i) = ag(i_); b(i) = bg(i_); dc(i) = dcg(i_); f0(i) = f0g(i_); vplv(i) = vplg(i_)
+    taudot(i) = taudotg(i_); sigdot(i) = sigdotg(i_); area(i) = ds(i_)
+    vel(i) = velg(i_)
+    y(3 * i - 2) = a(i) * dlog(2 * vref / vel(i) * sinh(taug(i_) / sigmag(i_) / a(i)))
+    y(3 * i - 1) = taug(i_)
+    y(3 * i) = sigmag(i_)
+  end do
+  slip = 0d0
+  ro%law = 0
+  if (trim(evlaw) == 'slip') ro%law = 1
+  ro%backslip = merge(1, 0, backslip)
+  ro%limitsigma = merge(1, 0, limitsigma)
+  ro%prec = int(st_ctl%param(P_PREC))
+  ro%vref = vref; ro%minsig = minsig; ro%maxsig = maxsig; ro%dtmax = dtmax
+  ro%hdiv = 0d0
+  if (trim(problem) == '3dph') ro%hdiv = ds0
+  ro%eps_r = eps_r; ro%velmin = velmin; ro%velmax = velmax
+  rc = hbi_rk_create(st_LHp_s%h, c_null_ptr, ro, a, b, dc, f0, vplv, taudot, sigdot, area, rk)
+  if (rc /= HBI_OK) then
+    write(*, *) 'hbi_rk_create: ', trim(hbi_error_text())
+    stop 2
+  end if
+  rc = hbi_rk_set_state(rk, y, slip, vel, 0d0, dtinit)
+  sg = sum(area)
+  write(fname, '("output/monitor",i0,".dat")') number
+  open(52, file=fname)
+  allocate(hist(6 * outbatch), vhist(1))
+  nrec = 0
+  nrec_dummy = 0
+  kdone = 0
+  call cpu_time(t1)
+  do while (kdone < nstep)
+    nb = min(outbatch, nstep - kdone)
+    rc = hbi_rk_step(rk, int(nb, c_int), hist, nrec, nrec_dummy, vhist, info)
+    do k = 1, int(info%accepted) - kdone
+      write(52, '(i7,f19.4,3e16.5,2e16.5,i4)') kdone + k, hist(6 * k - 5), hist(6 * k - 1), hist(6 * k), &
+        hist(6 * k - 3), hist(6 * k - 4), int(hist(6 * k - 2))
+    end do
+    kdone = int(info%accepted)
+    if (rc == HBI_E_STEP) then
+      write(*, *) 'ERROR: Runge-Kutta method did not converge'
+      exit
+    end if
+    if (info%status == 1) then
+      write(*, *) 'slip rate outside [velmin, velmax] at time step=', kdone
+      exit
+    end if
+    if (info%t > tmax) then
+      write(*, *) 'time exceeds tmax at time step=', kdone
+      exit
+    end if
+  end do
+  call cpu_time(t2)
+  close(52)
+  rc = hbi_rk_get_state(rk, y, slip, vel, info)
+  write(*, *) 'time(s)', t2 - t1, ' steps', kdone, ' rejected', info%rejected, ' t_end', info%t
+  rc = hbi_rk_destroy(rk)
+  lrtrn = HACApK_free_leafmtxp(st_leafmtxp_s)
+  lrtrn = HACApK_finalize(st_ctl)
+
+contains
+
+  subroutine mesh_3ddip(imax, jmax, ds0, dipangle, xcol, ycol, zcol, ang, angd)
+    integer, intent(in) :: imax, jmax
+    real(8), intent(in) :: ds0, dipangle
+    real(8), intent(out) :: xcol(:), ycol(:), zcol(:), ang(:), angd(:)
+    real(8) :: yr(0:jmax), zr(0:jmax)
+    integer :: i, j, k
+    yr(0) = 0d0; zr(0) = 0d0
+    do j = 1, jmax
+      yr(j) = yr(j - 1) - ds0 * cos(dipangle * pi / 180d0)
+      zr(j) = zr(j - 1) - ds0 * sin(dipangle * pi / 180d0)
+    end do
+    k = 0
+    do i = 1, imax
+      do j = 1, jmax
+        k = k + 1
+        xcol(k) = (i - imax / 2 - 0.5d0) * ds0
+        ycol(k) = 0.5d0 * (yr(j - 1) + yr(j))
+        zcol(k) = 0.5d0 * (zr(j - 1) + zr(j))
+        angd(k) = datan2(zr(j - 1) - zr(j), yr(j - 1) - yr(j))
+        ang(k) = 0d0
+      end do
+    end do
+  end subroutine mesh_3ddip
+
+  integer function count_stl_facets(path)
+    character(*), intent(in) :: path
+    integer :: u, ios, nl
+    open(newunit=u, file=path, status='old', iostat=ios)
+    if (ios /= 0) then
+      write(*, *) 'ERROR: Failed to open geometry file'
+      stop 1
+    end if
+    nl = 0
+    do
+      read(u, '(a)', iostat=ios)
+      if (ios /= 0) exit
+      nl = nl + 1
+    end do
+    close(u)
+    count_stl_facets = nl / 7
+  end function count_stl_facets
+
+  subroutine read_stl(path, n, xs, xcol, ycol, zcol)
+    character(*), intent(in) :: path
+    integer, intent(in) :: n
+    real(8), intent(out) :: xs(n, 9), xcol(n), ycol(n), zcol(n)
+    character(len=64) :: w
+    integer :: u, k
+    open(newunit=u, file=path, status='old')
+    do
+      read(u, *) w
+      if (w == 'facet') exit
+    end do
+    do k = 1, n
+      read(u, *)
+      read(u, *) w, xs(k, 1), xs(k, 4), xs(k, 7)
+      read(u, *) w, xs(k, 2), xs(k, 5), xs(k, 8)
+      read(u, *) w, xs(k, 3), xs(k, 6), xs(k, 9)
+      read(u, *)
+      read(u, *)
+      if (k < n) read(u, *)
+      xcol(k) = (xs(k, 1) + xs(k, 2) + xs(k, 3)) / 3
+      ycol(k) = (xs(k, 4) + xs(k, 5) + xs(k, 6)) / 3
+      zcol(k) = (xs(k, 7) + xs(k, 8) + xs(k, 9)) / 3
+    end do
+    close(u)
+  end subroutine read_stl
+
+  subroutine evcalc(n, xs, ev, ds)
+    integer, intent(in) :: n
+    real(8), intent(in) :: xs(n, 9)
+    real(8), intent(out) :: ev(n, 9), ds(n)
+    real(8) :: vba(3), vca(3), rr, t1, t2, t3
+    integer :: k
+    do k = 1, n
+      vba = (/ xs(k, 2) - xs(k, 1), xs(k, 5) - xs(k, 4), xs(k, 8) - xs(k, 7) /)
+      vca = (/ xs(k, 3) - xs(k, 1), xs(k, 6) - xs(k, 4), xs(k, 9) - xs(k, 7) /)
+      ev(k, 7) = vba(2) * vca(3) - vba(3) * vca(2)
+      ev(k, 8) = vba(3) * vca(1) - vba(1) * vca(3)
+      ev(k, 9) = vba(1) * vca(2) - vba(2) * vca(1)
+      rr = sqrt(ev(k, 7) * ev(k, 7) + ev(k, 8) * ev(k, 8) + ev(k, 9) * ev(k, 9))
+      ev(k, 7:9) = ev(k, 7:9) / rr
+      if (abs(ev(k, 9)) < 1.0d0) then
+        ev(k, 1) = -ev(k, 8); ev(k, 2) = ev(k, 7); ev(k, 3) = 0d0
+        rr = sqrt(ev(k, 1) * ev(k, 1) + ev(k, 2) * ev(k, 2))
+        ev(k, 1) = ev(k, 1) / rr; ev(k, 2) = ev(k, 2) / rr
+      else
+        ev(k, 1) = 1d0; ev(k, 2) = 0d0; ev(k, 3) = 0d0
+      end if
+      ev(k, 4) = ev(k, 8) * ev(k, 3) - ev(k, 9) * ev(k, 2)
+      ev(k, 5) = ev(k, 9) * ev(k, 1) - ev(k, 7) * ev(k, 3)
+      ev(k, 6) = ev(k, 7) * ev(k, 2) - ev(k, 8) * ev(k, 1)
+      t1 = vba(1) * vba(1) + vba(2) * vba(2) + vba(3) * vba(3)
+      t2 = vca(1) * vca(1) + vca(2) * vca(2) + vca(3) * vca(3)
+      t3 = vba(1) * vca(1) + vba(2) * vca(2) + vba(3) * vca(3)
+      ds(k) = 0.5d0 * sqrt(t1 * t2 - t3 * t3)
+    end do
+  end subroutine evcalc
+
+end program lhbiem_gpu

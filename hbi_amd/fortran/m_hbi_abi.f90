@@ -1,0 +1,126 @@
+! m_hbi_abi.f90 -- ISO_C_BINDING interfaces to libhbi (include/hbi.h).
+module m_hbi_abi
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: HBI_OK = 0, HBI_E_STEP = -4
+  integer(c_int), parameter :: HBI_KIND_2DVS = 1, HBI_KIND_OKADA_HS = 2, HBI_KIND_OKADA_FS = 3, HBI_KIND_TRI_FS = 4
+  integer(c_int), parameter :: HBI_PREC_F64 = 0, HBI_PREC_F32 = 1, HBI_PREC_MIXED = 2
+
+  type, bind(C) :: hbi_geom
+    integer(c_int) :: kind = 0, n = 0, vcomp = 0, md = 0
+    real(c_double) :: w = 0d0
+    type(c_ptr) :: xcol = c_null_ptr, ycol = c_null_ptr, zcol = c_null_ptr
+    type(c_ptr) :: xel = c_null_ptr, xer = c_null_ptr
+    type(c_ptr) :: ang = c_null_ptr, angd = c_null_ptr, dsl = c_null_ptr, dsd = c_null_ptr, rake = c_null_ptr
+    type(c_ptr) :: xs1 = c_null_ptr, xs2 = c_null_ptr, xs3 = c_null_ptr
+    type(c_ptr) :: ys1 = c_null_ptr, ys2 = c_null_ptr, ys3 = c_null_ptr
+    type(c_ptr) :: zs1 = c_null_ptr, zs2 = c_null_ptr, zs3 = c_null_ptr
+    type(c_ptr) :: ev11 = c_null_ptr, ev12 = c_null_ptr, ev13 = c_null_ptr
+    type(c_ptr) :: ev21 = c_null_ptr, ev22 = c_null_ptr, ev23 = c_null_ptr
+    type(c_ptr) :: ev31 = c_null_ptr, ev32 = c_null_ptr, ev33 = c_null_ptr
+  end type hbi_geom
+
+  type, bind(C) :: hbi_hmat_opts
+    integer(c_int) :: leaf = 15
+    real(c_double) :: eta = 2d0
+    real(c_double) :: eps_aca = 1d-4
+    integer(c_int) :: kmax = 1000
+    real(c_double) :: bf16_margin = 0d0
+    integer(c_int) :: tile_rows = 0
+    integer(c_int) :: chunk_kb = 0
+  end type hbi_hmat_opts
+
+  type, bind(C) :: hbi_rk_opts
+    integer(c_int) :: law = 0, backslip = 1, limitsigma = 1, prec = 0
+    real(c_double) :: vref = 1d-6, minsig = 1d0, maxsig = 300d0, dtmax = 1d10
+    real(c_double) :: hdiv = 0d0, eps_r = 1d-4, velmin = 1d-20, velmax = 1d7
+  end type hbi_rk_opts
+
+  type, bind(C) :: hbi_rk_info
+    real(c_double) :: t, dt_next, dt_last, errmax_last
+    integer(c_long_long) :: attempts, accepted, rejected
+    real(c_double) :: vmax
+    integer(c_int) :: status
+  end type hbi_rk_info
+
+  interface
+    integer(c_int) function hbi_version() bind(C, name="hbi_version")
+      import :: c_int
+    end function
+    type(c_ptr) function hbi_last_error() bind(C, name="hbi_last_error")
+      import :: c_ptr
+    end function
+    integer(c_int) function hbi_hmat_build(g, opts, h) bind(C, name="hbi_hmat_build")
+      import :: c_int, c_ptr, hbi_geom, hbi_hmat_opts
+      type(hbi_geom), intent(in) :: g
+      type(hbi_hmat_opts), intent(in) :: opts
+      type(c_ptr), intent(out) :: h
+    end function
+    integer(c_int) function hbi_hmat_destroy(h) bind(C, name="hbi_hmat_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function hbi_hmat_lod(h, lod) bind(C, name="hbi_hmat_lod")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: lod(*)
+    end function
+    integer(c_int) function hbi_hmat_matvec(h, x, y, prec, on_device) bind(C, name="hbi_hmat_matvec")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(out) :: y(*)
+      integer(c_int), value :: prec, on_device
+    end function
+    integer(c_int) function hbi_rk_create(hs, hn, opts, a, b, dc, f0, vpl, taudot, sigdot, area, r) &
+        bind(C, name="hbi_rk_create")
+      import :: c_int, c_ptr, c_double, hbi_rk_opts
+      type(c_ptr), value :: hs, hn
+      type(hbi_rk_opts), intent(in) :: opts
+      real(c_double), intent(in) :: a(*), b(*), dc(*), f0(*), vpl(*), taudot(*), sigdot(*), area(*)
+      type(c_ptr), intent(out) :: r
+    end function
+    integer(c_int) function hbi_rk_destroy(r) bind(C, name="hbi_rk_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: r
+    end function
+    integer(c_int) function hbi_rk_set_state(r, y, slip, vel, t, dt) bind(C, name="hbi_rk_set_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: r
+      real(c_double), intent(in) :: y(*), slip(*), vel(*)
+      real(c_double), value :: t, dt
+    end function
+    integer(c_int) function hbi_rk_get_state(r, y, slip, vel, info) bind(C, name="hbi_rk_get_state")
+      import :: c_int, c_ptr, c_double, hbi_rk_info
+      type(c_ptr), value :: r
+      real(c_double), intent(out) :: y(*), slip(*), vel(*)
+      type(hbi_rk_info), intent(out) :: info
+    end function
+    integer(c_int) function hbi_rk_step(r, nsteps, hist, nrec, rec, vhist, info) bind(C, name="hbi_rk_step")
+      import :: c_int, c_ptr, c_double, hbi_rk_info
+      type(c_ptr), value :: r
+      integer(c_int), value :: nsteps, nrec
+      real(c_double), intent(out) :: hist(*)
+      integer(c_int), intent(in) :: rec(*)
+      real(c_double), intent(out) :: vhist(*)
+      type(hbi_rk_info), intent(out) :: info
+    end function
+  end interface
+
+contains
+
+  ! text of hbi_last_error() as a Fortran string
+  function hbi_error_text() result(s)
+    character(len=512) :: s
+    character(kind=c_char), pointer :: p(:)
+    integer :: i
+    s = ' '
+    call c_f_pointer(hbi_last_error(), p, [512])
+    do i = 1, 512
+      if (p(i) == c_null_char) exit
+      s(i:i) = p(i)
+    end do
+  end function hbi_error_text
+
+end module m_hbi_abi

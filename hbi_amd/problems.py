@@ -310,6 +310,15 @@ def _friction(g, a, b, dc, sigma, vpl, vref, law):
                     vel=vel, taudot=np.zeros(n), sigdot=np.zeros(n), law=law, vref=vref)
 
 
+def perturb_state(y: np.ndarray, g: Geometry, lod: np.ndarray, amp: float = 0.02, depth: float = 8.0,
+                  width: float = 2.0) -> np.ndarray:
+    """Lower psi by a smooth bump (nucleation seed) so the first event does not
+    grow from rounding noise; in place, cluster order."""
+    d = g.xcol[lod] if g.kind == KIND_2DVS else -g.zcol[lod]
+    y[0::3] -= amp * np.exp(-((d - depth) / width) ** 2)
+    return y
+
+
 def init_state(fr: Friction, lod: np.ndarray) -> np.ndarray:
     """Interleaved (psi, tau, sigma) state in cluster order (main_LH.f90:1004, 1208-1215)."""
     a = fr.a[lod]
