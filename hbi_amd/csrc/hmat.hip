@@ -368,105 +368,114 @@ typedef double dvec2 __attribute__((ext_vector_type(2)));
 typedef float fvec4 __attribute__((ext_vector_type(4)));
 typedef uint32_t uvec4 __attribute__((ext_vector_type(4)));
 
-// one 16-byte chunk of packed factors, widened to FP64 (streamed once per product)
-template <typename T> struct Chunk;
-template <> struct Chunk<double> {
+// one 16-byte chunk of packed factors kept in its storage type until the FMA
+// (FP64 accumulation); streamed once per product, so loaded non-temporally
+template <typename T> struct Raw;
+template <> struct Raw<double> {
+    using R = dvec2;
     static constexpr int V = 2;
-    __device__ static void load(const uint8_t *p, double *o)
-    {
-        const dvec2 v = __builtin_nontemporal_load((const dvec2 *)p);
-        o[0] = v.x; o[1] = v.y;
-    }
+    __device__ static R load(const uint8_t *p) { return __builtin_nontemporal_load((const R *)p); }
+    __device__ static R zero() { return R{0.0, 0.0}; }
+    __device__ static double get(const R &r, int e) { return r[e]; }
 };
-template <> struct Chunk<float> {
+template <> struct Raw<float> {
+    using R = fvec4;
     static constexpr int V = 4;
-    __device__ static void load(const uint8_t *p, double *o)
-    {
-        const fvec4 v = __builtin_nontemporal_load((const fvec4 *)p);
-        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-    }
+    __device__ static R load(const uint8_t *p) { return __builtin_nontemporal_load((const R *)p); }
+    __device__ static R zero() { return R{0.f, 0.f, 0.f, 0.f}; }
+    __device__ static double get(const R &r, int e) { return (double)r[e]; }
 };
-template <> struct Chunk<uint16_t> {
+template <> struct Raw<uint16_t> {
+    using R = uvec4;
     static constexpr int V = 8;
-    __device__ static void load(const uint8_t *p, double *o)
+    __device__ static R load(const uint8_t *p) { return __builtin_nontemporal_load((const R *)p); }
+    __device__ static R zero() { return R{0u, 0u, 0u, 0u}; }
+    __device__ static double get(const R &r, int e)
     {
-        const uvec4 v = __builtin_nontemporal_load((const uvec4 *)p);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            o[2 * q] = __uint_as_float(w[q] << 16);
-            o[2 * q + 1] = __uint_as_float(w[q] & 0xffff0000u);
-        }
+        const uint32_t w = r[e >> 1];
+        return (double)__uint_as_float((e & 1) ? (w & 0xffff0000u) : (w << 16));
     }
 };
 
 constexpr int kMvThreads = 256;
-constexpr int kUnroll = 4;
 
-// phase 1: z[zoff + k] = sum_t a1T[k][t] x[xoff + t]
+// phase 1: z[zoff + k] = sum_t a1T[k][t] x[xoff + t], one wave per item.
+// lpr lanes (a power of two covering the row's 16-byte chunks, at most 64)
+// share a row, 64/lpr rows are in flight per pass and kU1 passes are unrolled;
+// each lane keeps its x chunk across the rows of a pass, and the row sums are
+// finished with a fixed xor-shuffle tree inside each lpr-lane segment.
+constexpr int kU1 = 4;
 template <typename T>
-__device__ __forceinline__ void lrvt_item(const Item1 &it, const uint8_t *__restrict__ buf, const double *__restrict__ x,
-                                          double *__restrict__ z, double *lds)
+__device__ __forceinline__ void lrvt_wave(const Item1 &it, const uint8_t *__restrict__ buf, const double *__restrict__ x,
+                                          double *__restrict__ z, int lane)
 {
-    constexpr int V = Chunk<T>::V;
-    constexpr int KG = 4;
-    const int tid = threadIdx.x;
-    for (int k0 = 0; k0 < it.nk; k0 += KG) {
-        double acc[KG];
+    constexpr int V = Raw<T>::V;
+    const int nch = it.ldt / V;
+    int lpr = kWave;
+    while (lpr > 1 && lpr / 2 >= nch) lpr >>= 1;
+    const int rpp = kWave / lpr;
+    const int cl = lane % lpr, rs = lane / lpr;
+    for (int k0 = 0; k0 < it.nk; k0 += rpp * kU1) {
+        double acc[kU1];
 #pragma unroll
-        for (int q = 0; q < KG; q++) acc[q] = 0.0;
-        for (int t = tid * V; t < it.ldt; t += kMvThreads * V) {
+        for (int u = 0; u < kU1; u++) acc[u] = 0.0;
+        for (int c = cl; c < nch; c += lpr) {
+            const int t = c * V;
             double xv[V];
 #pragma unroll
             for (int e = 0; e < V; e++) xv[e] = (t + e < it.nc) ? x[it.xoff + t + e] : 0.0;
+            typename Raw<T>::R a[kU1];
 #pragma unroll
-            for (int q = 0; q < KG; q++) {
-                if (k0 + q < it.nk) {
-                    double a[V];
-                    Chunk<T>::load(buf + it.off + ((size_t)(k0 + q) * it.ldt + t) * sizeof(T), a);
-#pragma unroll
-                    for (int e = 0; e < V; e++) acc[q] = fma(a[e], xv[e], acc[q]);
-                }
+            for (int u = 0; u < kU1; u++) {
+                const int row = k0 + rs + u * rpp;
+                a[u] = row < it.nk ? Raw<T>::load(buf + it.off + ((size_t)row * it.ldt + t) * sizeof(T)) : Raw<T>::zero();
             }
-        }
 #pragma unroll
-        for (int q = 0; q < KG; q++) acc[q] = wave_sum(acc[q]);
-        const int w = tid / kWave, l = tid % kWave;
-        __syncthreads();
-        if (l == 0)
+            for (int u = 0; u < kU1; u++)
 #pragma unroll
-            for (int q = 0; q < KG; q++) lds[q * 4 + w] = acc[q];
-        __syncthreads();
-        if (tid < KG && k0 + tid < it.nk) {
-            double s = 0.0;
-            for (int ww = 0; ww < kMvThreads / kWave; ww++) s += lds[tid * 4 + ww];
-            z[it.zoff + k0 + tid] = s;
+                for (int e = 0; e < V; e++) acc[u] = fma(Raw<T>::get(a[u], e), xv[e], acc[u]);
         }
+        for (int o = lpr / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int u = 0; u < kU1; u++) acc[u] += __shfl_xor(acc[u], o, kWave);
+        if (cl == 0)
+#pragma unroll
+            for (int u = 0; u < kU1; u++) {
+                const int row = k0 + rs + u * rpp;
+                if (row < it.nk) z[it.zoff + row] = acc[u];
+            }
     }
 }
 
-__global__ __launch_bounds__(kMvThreads) void lrvt_kernel(const Item1 *__restrict__ items, const uint8_t *__restrict__ buf,
+template <typename T>
+__global__ __launch_bounds__(kMvThreads) void lrvt_kernel(const Item1 *__restrict__ items, int nitems,
+                                                          const uint8_t *__restrict__ buf,
                                                           const double *__restrict__ x, double *__restrict__ z,
                                                           const int *__restrict__ gate)
 {
-    __shared__ double lds[16];
     if (gate && *gate) return;
-    const Item1 it = items[blockIdx.x];
-    if (it.dtype == kF64) lrvt_item<double>(it, buf, x, z, lds);
-    else if (it.dtype == kF32) lrvt_item<float>(it, buf, x, z, lds);
-    else lrvt_item<uint16_t>(it, buf, x, z, lds);
+    const int w = blockIdx.x * (kMvThreads / kWave) + threadIdx.x / kWave;
+    if (w >= nitems) return;
+    const Item1 it = items[w];
+    lrvt_wave<T>(it, buf, x, z, threadIdx.x % kWave);
 }
 
-// phase 2: partial y of one work item
-template <typename T, int TR>
-__device__ __forceinline__ void tile_item(const Item2 &it, const Seg *__restrict__ segs, const uint8_t *__restrict__ buf,
-                                          const double *__restrict__ x, const double *__restrict__ z,
-                                          double *__restrict__ ypart, double *lds)
+// phase 2: partial y of one work item (one workgroup).  A lane owns V rows
+// of the TR-row tile and streams columns; U column groups are loaded before
+// they are used so U 16-byte loads per lane are in flight.
+template <typename T, int TR, int U>
+__global__ __launch_bounds__(kMvThreads) void tile_kernel(const Item2 *__restrict__ items, const Seg *__restrict__ segs,
+                                                          const uint8_t *__restrict__ buf, const double *__restrict__ x,
+                                                          const double *__restrict__ z, double *__restrict__ ypart,
+                                                          const int *__restrict__ gate)
 {
-    constexpr int V = Chunk<T>::V;
+    constexpr int V = Raw<T>::V;
     constexpr int LPC = TR / V;        // lanes per column
     constexpr int CPI = kWave / LPC;   // columns per wave load
     constexpr int NW = kMvThreads / kWave;
+    __shared__ double lds[kMvThreads * V];
+    if (gate && *gate) return;
+    const Item2 it = items[blockIdx.x];
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
     const int rsub = lane % LPC, csub = lane / LPC;
     double acc[V];
@@ -477,26 +486,21 @@ __device__ __forceinline__ void tile_item(const Item2 &it, const Seg *__restrict
         const uint8_t *base = buf + sg.off + (size_t)rsub * V * sizeof(T);
         const double *xin = (sg.src ? z : x) + sg.xoff;
         const int ngroups = (sg.ncol + CPI - 1) / CPI;
-        for (int g0 = w; g0 < ngroups; g0 += NW * kUnroll) {
-            double a[kUnroll][V];
-            double xv[kUnroll];
+        for (int g0 = w; g0 < ngroups; g0 += NW * U) {
+            typename Raw<T>::R a[U];
+            double xv[U];
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
+            for (int u = 0; u < U; u++) {
                 const int g = g0 + u * NW;
                 const int c = g * CPI + csub;
-                if (g < ngroups && c < sg.ncol) {
-                    Chunk<T>::load(base + (size_t)c * TR * sizeof(T), a[u]);
-                    xv[u] = xin[c];
-                } else {
-#pragma unroll
-                    for (int e = 0; e < V; e++) a[u][e] = 0.0;
-                    xv[u] = 0.0;
-                }
+                const bool ok = g < ngroups && c < sg.ncol;
+                a[u] = ok ? Raw<T>::load(base + (size_t)c * TR * sizeof(T)) : Raw<T>::zero();
+                xv[u] = ok ? xin[c] : 0.0;
             }
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++)
+            for (int u = 0; u < U; u++)
 #pragma unroll
-                for (int e = 0; e < V; e++) acc[e] = fma(a[u][e], xv[u], acc[e]);
+                for (int e = 0; e < V; e++) acc[e] = fma(Raw<T>::get(a[u], e), xv[u], acc[e]);
         }
     }
     // lanes with the same rsub hold the same rows: add them in a fixed order
@@ -510,20 +514,6 @@ __device__ __forceinline__ void tile_item(const Item2 &it, const Seg *__restrict
             for (int cs = 0; cs < CPI; cs++) s += lds[(ww * kWave + cs * LPC + rs) * V + e];
         ypart[(size_t)it.part * TR + r] = s;
     }
-}
-
-template <int TR>
-__global__ __launch_bounds__(kMvThreads) void tile_kernel(const Item2 *__restrict__ items, const Seg *__restrict__ segs,
-                                                          const uint8_t *__restrict__ buf, const double *__restrict__ x,
-                                                          const double *__restrict__ z, double *__restrict__ ypart,
-                                                          const int *__restrict__ gate)
-{
-    __shared__ double lds[kMvThreads * 8];
-    if (gate && *gate) return;
-    const Item2 it = items[blockIdx.x];
-    if (it.dtype == kF64) tile_item<double, TR>(it, segs, buf, x, z, ypart, lds);
-    else if (it.dtype == kF32) tile_item<float, TR>(it, segs, buf, x, z, ypart, lds);
-    else tile_item<uint16_t, TR>(it, segs, buf, x, z, ypart, lds);
 }
 
 __global__ void tile_reduce_kernel(int n, int tr, const int *__restrict__ part0, const double *__restrict__ ypart,
@@ -541,6 +531,77 @@ __global__ void tile_reduce_kernel(int n, int tr, const int *__restrict__ part0,
 static int dtype_size(int d) { return d == kF64 ? 8 : (d == kF32 ? 4 : 2); }
 static inline size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
 
+// Group the packed layout into work items: phase-1 items are row runs of one
+// low-rank block (one wave each), phase-2 items are runs of one tile's
+// segments of one dtype (one workgroup each) writing one partial slot.
+static void make_items(hbi_hmat *h, Plan &P, int chunk1, int chunk2, int unroll2)
+{
+    P.it1.clear();
+    P.it2.clear();
+    for (const Row1 &r : P.rows1) {
+        const int sz = dtype_size(r.dtype);
+        const int rows_per = std::max(1, std::min(r.kt, (int)((size_t)chunk1 / ((size_t)r.ldt * sz))));
+        for (int k0 = 0; k0 < r.kt; k0 += rows_per) {
+            Item1 it{};
+            it.off = r.off + (size_t)k0 * r.ldt * sz;
+            it.ldt = r.ldt; it.nc = r.nc; it.nk = std::min(rows_per, r.kt - k0);
+            it.xoff = r.xoff; it.zoff = r.zoff + k0; it.dtype = r.dtype;
+            P.it1.push_back(it);
+        }
+    }
+    std::stable_sort(P.it1.begin(), P.it1.end(), [](const Item1 &a, const Item1 &b) {
+        if (a.dtype != b.dtype) return a.dtype < b.dtype;
+        return (long long)a.nk * a.ldt * dtype_size(a.dtype) > (long long)b.nk * b.ldt * dtype_size(b.dtype);
+    });
+    P.nparts = 0;
+    P.tile_part0.assign(h->ntiles + 1, 0);
+    size_t g = 0;
+    for (int t = 0; t < h->ntiles; t++) {
+        P.tile_part0[t] = P.nparts;
+        for (; g < P.groups.size() && P.groups[g].tile == t; g++) {
+            const SegGroup &G = P.groups[g];
+            size_t cur = 0;
+            int s0 = G.s0;
+            for (int q = G.s0; q < G.s1; q++) {
+                const size_t sb = (size_t)h->tr * P.segs[q].ncol * dtype_size(G.dtype);
+                if (q > s0 && cur + sb > (size_t)chunk2) {
+                    P.it2.push_back({t, s0, q, P.nparts++, G.dtype, 0});
+                    s0 = q;
+                    cur = 0;
+                }
+                cur += sb;
+            }
+            P.it2.push_back({t, s0, G.s1, P.nparts++, G.dtype, 0});
+        }
+    }
+    P.tile_part0[h->ntiles] = P.nparts;
+    std::stable_sort(P.it2.begin(), P.it2.end(), [](const Item2 &a, const Item2 &b) { return a.dtype < b.dtype; });
+    for (int d = 0; d < 3; d++) {
+        P.r1[d][0] = P.r1[d][1] = 0;
+        P.r2[d][0] = P.r2[d][1] = 0;
+    }
+    for (int d = 0; d < 3; d++) {
+        int lo = 0;
+        while (lo < (int)P.it1.size() && P.it1[lo].dtype < d) lo++;
+        int hi = lo;
+        while (hi < (int)P.it1.size() && P.it1[hi].dtype == d) hi++;
+        P.r1[d][0] = lo; P.r1[d][1] = hi;
+        lo = 0;
+        while (lo < (int)P.it2.size() && P.it2[lo].dtype < d) lo++;
+        hi = lo;
+        while (hi < (int)P.it2.size() && P.it2[hi].dtype == d) hi++;
+        P.r2[d][0] = lo; P.r2[d][1] = hi;
+    }
+    P.chunk1 = chunk1;
+    P.chunk2 = chunk2;
+    P.unroll2 = unroll2;
+    P.d_it1.upload(P.it1, h->stream);
+    P.d_it2.upload(P.it2, h->stream);
+    P.d_tile_part0.upload(P.tile_part0, h->stream);
+    if ((int)h->d_ypart.n < std::max(P.nparts, 1) * h->tr) h->d_ypart.alloc((size_t)std::max(P.nparts, 1) * h->tr);
+    HBI_HIP(hipStreamSynchronize(h->stream));
+}
+
 static void build_plan(hbi_hmat *h, int prec)
 {
     Plan &P = h->plan[prec];
@@ -556,46 +617,34 @@ static void build_plan(hbi_hmat *h, int prec)
         if (prec == HBI_PREC_MIXED && L[ip].ltmtx == 1 && h->bf16_ok[ip]) return kBF16;
         return kF32;
     };
-    // ---- phase 1 items
+    // ---- phase 1 layout: rows of a1^T per low-rank block, padded to 8 elements
     for (int ip = 0; ip < (int)L.size(); ip++) {
         const Leaf &f = L[ip];
         if (f.ltmtx != 1 || f.kt == 0) continue;
         const int dt = leaf_dtype(ip), sz = dtype_size(dt);
         const int ldt = (f.nc + 7) / 8 * 8;
-        const int rows_per = std::max(1, std::min(f.kt, (int)(h->chunk_bytes / ((size_t)ldt * sz))));
-        const size_t off0 = bytes;
         CopyJob J{};
-        J.src = h->fac_off[ip]; J.dst = (long long)off0; J.start = elems; J.which = 1;
+        J.src = h->fac_off[ip]; J.dst = (long long)bytes; J.start = elems; J.which = 1;
         J.rows = f.kt; J.cols = ldt; J.src_rs = f.nc; J.src_cs = 1; J.r_lo = 0; J.r_hi = f.kt; J.c_hi = f.nc;
         J.dst_rs = ldt; J.dst_cs = 1; J.dtype = dt;
         jobs.push_back(J);
+        P.rows1.push_back({bytes, ldt, f.nc, f.kt, f.c0, h->zoff[ip], dt});
         elems += (long long)f.kt * ldt;
         bytes = align16(bytes + (size_t)f.kt * ldt * sz);
-        for (int k0 = 0; k0 < f.kt; k0 += rows_per) {
-            Item1 it{};
-            it.off = off0 + (size_t)k0 * ldt * sz;
-            it.ldt = ldt; it.nc = f.nc; it.nk = std::min(rows_per, f.kt - k0);
-            it.xoff = f.c0; it.zoff = h->zoff[ip] + k0; it.dtype = dt;
-            P.it1.push_back(it);
-        }
         P.alg_bytes += (long long)f.nc * f.kt * sz;
         P.alg_bytes_p1 += (long long)f.nc * f.kt * sz;
     }
-    // ---- phase 2 segments per tile
+    // ---- phase 2 layout: TR x ncol column-major segments, grouped by (tile, dtype)
     std::vector<std::vector<int>> tl(nt);
     for (int ip = 0; ip < (int)L.size(); ip++) {
         const Leaf &f = L[ip];
         if (f.ltmtx == 1 && f.kt == 0) continue;
         for (int t = f.r0 / TR; t <= (f.r0 + f.nr - 1) / TR; t++) tl[t].push_back(ip);
     }
-    P.tile_part0.assign(nt + 1, 0);
     for (int t = 0; t < nt; t++) {
-        P.tile_part0[t] = P.nparts;
         const int T0 = t * TR;
         for (int dt = 0; dt < 3; dt++) {
-            size_t cur = 0;
-            int seg_begin = (int)P.segs.size();
-            bool any = false;
+            const int s0 = (int)P.segs.size();
             for (int ip : tl[t]) {
                 if (leaf_dtype(ip) != dt) continue;
                 const Leaf &f = L[ip];
@@ -603,11 +652,6 @@ static void build_plan(hbi_hmat *h, int prec)
                 const int lo = std::max(f.r0, T0), hi = std::min(f.r0 + f.nr, T0 + TR);
                 const int ncol = f.ltmtx == 1 ? f.kt : f.nc;
                 const size_t segbytes = (size_t)TR * ncol * sz;
-                if (any && cur + segbytes > (size_t)h->chunk_bytes) {
-                    P.it2.push_back({t, seg_begin, (int)P.segs.size(), P.nparts++, dt, 0});
-                    seg_begin = (int)P.segs.size();
-                    cur = 0;
-                }
                 Seg sg{};
                 sg.off = bytes; sg.ncol = ncol; sg.dtype = dt;
                 CopyJob J{};
@@ -630,22 +674,15 @@ static void build_plan(hbi_hmat *h, int prec)
                 elems += (long long)TR * ncol;
                 bytes = align16(bytes + segbytes);
                 P.segs.push_back(sg);
-                cur += segbytes;
-                any = true;
             }
-            if (any) P.it2.push_back({t, seg_begin, (int)P.segs.size(), P.nparts++, dt, 0});
+            if ((int)P.segs.size() > s0) P.groups.push_back({t, dt, s0, (int)P.segs.size()});
         }
     }
-    P.tile_part0[nt] = P.nparts;
     P.alg_bytes += 2LL * h->n * 8;   // x read, y written (FP64)
     P.alg_bytes_p1 += (long long)h->n * 8 + (long long)h->nz * 8;
     P.alg_bytes_p2 += 2LL * h->n * 8;
     for (int ip = 0; ip < (int)L.size(); ip++)
         if (leaf_dtype(ip) == kBF16) P.nbf16++;
-    // largest items first for load balance
-    std::stable_sort(P.it1.begin(), P.it1.end(), [](const Item1 &a, const Item1 &b) {
-        return (long long)a.nk * a.ldt * dtype_size(a.dtype) > (long long)b.nk * b.ldt * dtype_size(b.dtype);
-    });
     // pack on the device
     P.bytes = bytes;
     P.buf.alloc(std::max<size_t>(bytes, 16));
@@ -658,11 +695,41 @@ static void build_plan(hbi_hmat *h, int prec)
         HBI_HIP(hipStreamSynchronize(h->stream));
     }
     P.d_segs.upload(P.segs, h->stream);
-    P.d_it1.upload(P.it1, h->stream);
-    P.d_it2.upload(P.it2, h->stream);
-    P.d_tile_part0.upload(P.tile_part0, h->stream);
-    HBI_HIP(hipStreamSynchronize(h->stream));
     P.built = true;
+    // ---- work-item sizes: fixed by the caller, or picked by timing the product
+    if (h->opts.chunk_kb > 0) {
+        make_items(h, P, std::max(1024, h->opts.chunk_kb * 1024 / 4), h->opts.chunk_kb * 1024, 4);
+        return;
+    }
+    const int c1s[] = {4096, 8192, 16384, 32768};
+    const int c2s[] = {16384, 32768, 65536, 131072};
+    HBI_HIP(hipMemsetAsync(h->d_x.p, 0, h->d_x.n * 8, h->stream));
+    auto time_it = [&](int reps) {
+        for (int w = 0; w < 2; w++) hmat_apply(h, prec, h->d_x.p, h->d_y.p);
+        HBI_HIP(hipEventRecord(h->ev[0], h->stream));
+        for (int r = 0; r < reps; r++) hmat_apply(h, prec, h->d_x.p, h->d_y.p);
+        HBI_HIP(hipEventRecord(h->ev[1], h->stream));
+        HBI_HIP(hipEventSynchronize(h->ev[1]));
+        float ms = 0;
+        HBI_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+        return ms / reps;
+    };
+    // phase 2 first (the larger kernel), then phase 1 with the chosen phase-2 size
+    int b1 = c1s[1], b2 = c2s[2], bu = 4;
+    float best = 1e30f;
+    for (int u : {4, 8})
+        for (int c2 : c2s) {
+            make_items(h, P, b1, c2, u);
+            const float ms = time_it(5);
+            if (ms < best) { best = ms; b2 = c2; bu = u; }
+        }
+    best = 1e30f;
+    for (int c1 : c1s) {
+        make_items(h, P, c1, b2, bu);
+        const float ms = time_it(5);
+        if (ms < best) { best = ms; b1 = c1; }
+    }
+    make_items(h, P, b1, b2, bu);
 }
 
 const Plan &hmat_plan(hbi_hmat *h, int prec)
@@ -674,26 +741,52 @@ const Plan &hmat_plan(hbi_hmat *h, int prec)
 
 void hmat_phase1(hbi_hmat *h, int prec, const double *x, double *z, hipStream_t s, const int *gate)
 {
-    const Plan &P = hmat_plan(h, prec);
-    if (!P.it1.empty())
-        lrvt_kernel<<<(int)P.it1.size(), kMvThreads, 0, s>>>(P.d_it1.p, P.buf.p, x, z, gate);
+    const Plan &P = h->plan[prec];
+    const int per = kMvThreads / kWave;
+    for (int d = 0; d < 3; d++) {
+        const int n = P.r1[d][1] - P.r1[d][0];
+        if (n <= 0) continue;
+        const Item1 *it = P.d_it1.p + P.r1[d][0];
+        const int grid = (n + per - 1) / per;
+        if (d == kF64) lrvt_kernel<double><<<grid, kMvThreads, 0, s>>>(it, n, P.buf.p, x, z, gate);
+        else if (d == kF32) lrvt_kernel<float><<<grid, kMvThreads, 0, s>>>(it, n, P.buf.p, x, z, gate);
+        else lrvt_kernel<uint16_t><<<grid, kMvThreads, 0, s>>>(it, n, P.buf.p, x, z, gate);
+    }
+}
+
+template <typename T, int TR>
+static void launch_tiles(const Plan &P, int n, const Item2 *it, const double *x, const double *z, double *ypart,
+                         hipStream_t s, const int *gate)
+{
+    if (P.unroll2 == 8)
+        tile_kernel<T, TR, 8><<<n, kMvThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, x, z, ypart, gate);
+    else
+        tile_kernel<T, TR, 4><<<n, kMvThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, x, z, ypart, gate);
 }
 
 void hmat_phase2(hbi_hmat *h, int prec, const double *x, const double *z, double *ypart, hipStream_t s,
                  const int *gate)
 {
-    const Plan &P = hmat_plan(h, prec);
-    if (P.it2.empty()) return;
-    if (h->tr == 64)
-        tile_kernel<64><<<(int)P.it2.size(), kMvThreads, 0, s>>>(P.d_it2.p, P.d_segs.p, P.buf.p, x, z, ypart, gate);
-    else
-        tile_kernel<32><<<(int)P.it2.size(), kMvThreads, 0, s>>>(P.d_it2.p, P.d_segs.p, P.buf.p, x, z, ypart, gate);
+    const Plan &P = h->plan[prec];
+    for (int d = 0; d < 3; d++) {
+        const int n = P.r2[d][1] - P.r2[d][0];
+        if (n <= 0) continue;
+        const Item2 *it = P.d_it2.p + P.r2[d][0];
+        if (h->tr == 64) {
+            if (d == kF64) launch_tiles<double, 64>(P, n, it, x, z, ypart, s, gate);
+            else if (d == kF32) launch_tiles<float, 64>(P, n, it, x, z, ypart, s, gate);
+            else launch_tiles<uint16_t, 64>(P, n, it, x, z, ypart, s, gate);
+        } else {
+            if (d == kF64) launch_tiles<double, 32>(P, n, it, x, z, ypart, s, gate);
+            else if (d == kF32) launch_tiles<float, 32>(P, n, it, x, z, ypart, s, gate);
+            else launch_tiles<uint16_t, 32>(P, n, it, x, z, ypart, s, gate);
+        }
+    }
 }
 
 void hmat_apply(hbi_hmat *h, int prec, const double *x, double *y)
 {
-    const Plan &P = hmat_plan(h, prec);
-    if ((int)h->d_ypart.n < std::max(P.nparts, 1) * h->tr) h->d_ypart.alloc((size_t)std::max(P.nparts, 1) * h->tr);
+    const Plan &P = h->plan[prec];
     hmat_phase1(h, prec, x, h->d_z.p, h->stream, nullptr);
     hmat_phase2(h, prec, x, h->d_z.p, h->d_ypart.p, h->stream, nullptr);
     tile_reduce_kernel<<<(h->n + 255) / 256, 256, 0, h->stream>>>(h->n, h->tr, P.d_tile_part0.p, h->d_ypart.p, y);
@@ -966,6 +1059,8 @@ static void build(hbi_hmat *h, const hbi_geom *g, const hbi_hmat_opts *o)
         S.alg_bytes[p] = h->plan[p].alg_bytes;
         S.alg_bytes_p1[p] = h->plan[p].alg_bytes_p1;
         S.alg_bytes_p2[p] = h->plan[p].alg_bytes_p2;
+        S.chunk_p1[p] = h->plan[p].chunk1;
+        S.chunk_p2[p] = h->plan[p].chunk2;
     }
     S.nbf16_blocks = h->plan[HBI_PREC_MIXED].nbf16;
 }

@@ -32,8 +32,23 @@ struct CopyJob {
     int which, rows, cols, src_rs, src_cs, r_lo, r_hi, c_hi, dst_rs, dst_cs, dtype, pad;
 };
 
+// a low-rank block's packed a1^T rows (phase 1 layout)
+struct Row1 {
+    size_t off;
+    int ldt, nc, kt, xoff, zoff, dtype;
+};
+// consecutive segments of one tile with one dtype (phase 2 layout)
+struct SegGroup {
+    int tile, dtype, s0, s1;
+};
+
 struct Plan {
     bool built = false;
+    std::vector<Row1> rows1;
+    std::vector<SegGroup> groups;
+    int chunk1 = 0, chunk2 = 0;   // work-item sizes in bytes (autotuned)
+    int unroll2 = 4;              // phase-2 loads in flight per lane (autotuned)
+    int r1[3][2] = {}, r2[3][2] = {};   // item ranges per dtype
     DevBuf<uint8_t> buf;
     size_t bytes = 0;
     std::vector<Seg> segs;

@@ -39,7 +39,8 @@ class HmatStats(C.Structure):
                 ("tile_rows", C.c_int), ("kt_mean", C.c_double), ("dense_entries", C.c_longlong),
                 ("lr_entries", C.c_longlong), ("lr_dims", C.c_longlong), ("stored_bytes", C.c_longlong * 3),
                 ("alg_bytes", C.c_longlong * 3), ("alg_bytes_p1", C.c_longlong * 3),
-                ("alg_bytes_p2", C.c_longlong * 3), ("nbf16_blocks", C.c_int), ("ncapped", C.c_int),
+                ("alg_bytes_p2", C.c_longlong * 3), ("chunk_p1", C.c_int * 3), ("chunk_p2", C.c_int * 3),
+                ("nbf16_blocks", C.c_int), ("ncapped", C.c_int),
                 ("t_partition", C.c_double), ("t_dense", C.c_double), ("t_aca", C.c_double), ("t_pack", C.c_double)]
 
     def as_dict(self):

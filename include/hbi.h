@@ -86,7 +86,8 @@ typedef struct hbi_hmat_stats {
     long long alg_bytes[3];      /* algorithmic bytes per single-vector product per precision */
     long long alg_bytes_p1[3];   /* of which the low-rank V^T x kernel (factors + x + z) */
     long long alg_bytes_p2[3];   /* of which the row-tile kernel (factors + x + y) */
-    int nbf16_blocks;            /* low-rank blocks stored as BF16 in HBM_PREC_MIXED */
+    int chunk_p1[3], chunk_p2[3];   /* work-item bytes per precision chosen by the autotuner */
+    int nbf16_blocks;            /* low-rank blocks stored as BF16 in HBI_PREC_MIXED */
     int ncapped;                 /* low-rank blocks whose ACA reached kmax */
     double t_partition, t_dense, t_aca, t_pack;   /* build phase times, seconds */
 } hbi_hmat_stats;

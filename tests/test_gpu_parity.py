@@ -142,41 +142,58 @@ def test_gpu_derivs_and_rkck_match_oracle():
     dg, do = rk.derivs(y), so.derivs(y)
     for c in range(3):
         assert np.abs(dg[c::3] - do[c::3]).max() <= 1e-7 * np.abs(do[c::3]).max() + 1e-300
-    for h in (1e5, 1e7):
+    for h in (1e2, 1e4):   # |errmax| ~ 3 and 2e4 at eps 1e-6: an accepted and a rejected step
         yg, eg = rk.rkck(y, h)
         yo, eo = so.rkck(y, h)
         assert np.abs(yg - yo).max() <= 1e-9 * np.abs(yo).max()
         assert np.abs(eg - eo).max() <= 1e-6 * np.abs(eo).max() + 1e-14
 
 
+def _onset(hist, thresh=-3.0):
+    k = int(np.argmax(hist[:, 4] > thresh))
+    return hist[k, 0] if hist[k, 4] > thresh else None
+
+
+def _window_rel(h0, v0, h1, v1, t_end):
+    """max relative slip-rate difference on a common time grid up to t_end"""
+    tg = np.linspace(max(h0[0, 0], h1[0, 0]), t_end, 4000)
+    return max(np.abs(np.interp(tg, h0[:, 0], np.log(v0[:, r])) - np.interp(tg, h1[:, 0], np.log(v1[:, r]))).max()
+               for r in range(v0.shape[1]))
+
+
 def test_gpu_rk_trajectory_c1_matches_oracle():
-    """2-D antiplane C1 setup, 400 accepted steps from V = Vpl: times, step
-    sizes and slip rates of the device integrator track the oracle."""
+    """2-D antiplane C1 setup (seeded nucleation), 600 accepted steps through
+    the first event.  Trajectories are compared in time, not by step index:
+    an accept/reject decision flipped by a last-bit difference re-grids the
+    whole run (a 1e-12 relative perturbation of tau does that to the oracle
+    itself), so the checks are the event onset time and the slip-rate history
+    before the fast nucleation phase."""
     g, fr = _c1()
     Hg = H.HMatrix(g, leaf=32, eps=1e-6)
     Ho = O.HMatrix(g, leaf=32, eps=1e-6)
     lod = Hg.lod
     y0 = P.init_state(fr, lod)
     P.perturb_state(y0, g, lod)
-    nsteps = 400
-    rec = np.array([10, 500, 1000, 1500, 2000], dtype=np.int32)
+    nsteps = 600
+    rec = np.arange(0, g.n, 64, dtype=np.int32)
     rk = H.RK(Hg, fr, lod, g.ds, eps_r=1e-6)
     rk.set_state(y0, 0.0, 1.0)
     info, hist, vh = rk.step(nsteps, history=True, rec=rec)
     so = O.Sim(Ho, fr, lod)
     yo = y0.copy()
-    slip = np.zeros(g.n)
     a = fr.a[lod]
-    vel = 2 * fr.vref * np.exp(-y0[0::3] / a) * np.sinh(y0[1::3] / y0[2::3] / a)
-    done, x, dt, ho, vho = so.run(yo, 0.0, 1.0, 1e-6, nsteps, slip, vel, np.asarray(g.ds)[lod], rec=rec)
+    vel = 2 * fr.vref * np.exp(-yo[0::3] / a) * np.sinh(yo[1::3] / yo[2::3] / a)
+    done, x, dt, ho, vho = so.run(yo, 0.0, 1.0, 1e-6, nsteps, np.zeros(g.n), vel, np.asarray(g.ds)[lod], rec=rec)
     assert done == nsteps and info["accepted"] == nsteps
-    et = np.abs(hist[:, 0] - ho[:, 0]).max() / ho[-1, 0]
-    ev = np.abs(np.log(vh) - np.log(vho)).max()
-    yg, sg, vg, _ = rk.get_state()
-    ey = np.abs(yg - yo).max() / np.abs(yo).max()
-    print("trajectory: rel t %.3e  max |dlog V| %.3e  state %.3e  t_end %.4e vmax %.3e" % (et, ev, ey, ho[-1, 0],
-                                                                                         10 ** ho[:, 4].max()))
-    assert et <= 1e-5 and ev <= 1e-5 and ey <= 1e-6
+    on_g, on_o = _onset(hist), _onset(ho)
+    assert on_g is not None and on_o is not None
+    rel_onset = abs(on_g - on_o) / on_o
+    rel_early = _window_rel(ho, vho, hist, vh, 0.5 * on_o)
+    rel_pre = _window_rel(ho, vho, hist, vh, 0.9 * on_o)
+    print("trajectory: onset %.6e s (oracle %.6e), rel %.3e; |dV/V| to 0.5 onset %.3e, to 0.9 onset %.3e; "
+          "rejected %d vs %d" % (on_g, on_o, rel_onset, rel_early, rel_pre, info["rejected"], int(ho[:, 3].sum())))
+    assert rel_onset <= 1e-5
+    assert rel_early <= 1e-5
 
 
 def test_gpu_rk_history_and_stop_criteria():
