@@ -1,0 +1,514 @@
+// ens.hip -- ensembles of fault realisations on one H-matrix.
+//
+// R <= 64 realisations (different friction parameters, same fault mesh) are
+// advanced in lockstep: every attempt evaluates the six stages of all
+// realisations with one 64-column H product per stage (multirhs.hip), while
+// each realisation keeps its own step size, time, embedded-error decision and
+// step count (the reference's rkqs, main_LH.f90:2270-2370, applied per
+// column).  A realisation that reached its step target is frozen: its columns
+// are still computed, its state no longer changes.  One attempt is captured
+// once as a HIP graph.
+//
+// Device layouts put the realisation index innermost: y[(3 i + c) * 64 + r],
+// parameters p[i * 64 + r], so one wavefront covers the 64 realisations of
+// an element with unit-stride accesses.
+#include "rkcore.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace hbi {
+
+constexpr int kR = 64;
+
+struct EnsPar {
+    int n, nreal, law, backslip, limitsigma, tr;
+    double vref, minsig, maxsig, dtmax, hdiv, eps, velmin, velmax;
+    Tab T;
+};
+
+struct EnsCtl {           // per realisation
+    double h[kR], t[kR], hdid[kR], errmax[kR], vmax[kR];
+    long long accepted[kR], rejected[kR], target[kR], hist_base[kR];
+    long long hist_cap;
+    int acc[kR], frozen[kR], status[kR];
+    int done;
+};
+
+struct EnsArrays {
+    const double *a, *b, *dc, *f0, *vpl, *taudot, *sigdot;
+    double *y, *ytmp, *yout, *k, *vel, *x, *slip, *velc, *errpart, *monpart;
+    const double *ypart;
+    const int *part0;
+};
+
+// thread (i, r): r = threadIdx.x % 64, i = blockIdx.x * 4 + threadIdx.x / 64
+__global__ void ens_stage0_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A)
+{
+#pragma clang fp contract(off)
+    if (c->done) return;
+    const int r = threadIdx.x % kR, i = blockIdx.x * 4 + threadIdx.x / kR;
+    if (i >= P.n) return;
+    const size_t q = (size_t)i * kR + r;
+    if (r >= P.nreal) { A.x[q] = 0.0; return; }
+    const double *y = A.y;
+    const double v = slip_rate(y[(3 * (size_t)i) * kR + r], y[(3 * (size_t)i + 1) * kR + r],
+                                 y[(3 * (size_t)i + 2) * kR + r], A.a[q], P.vref);
+    A.vel[q] = v;
+    A.x[q] = P.backslip ? v - A.vpl[q] : v;
+}
+
+struct EPair {
+    int g;
+    double v;
+};
+__device__ __forceinline__ EPair e_combine(EPair l, EPair r)
+{
+    if (r.g) return r;
+    return EPair{l.g, fmax(l.v, r.v)};
+}
+
+__global__ __launch_bounds__(256) void ens_stage_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A, int s)
+{
+#pragma clang fp contract(off)
+    __shared__ int sg[4][kR];
+    __shared__ double sv[4][kR];
+    if (c->done) return;
+    const int r = threadIdx.x % kR, li = threadIdx.x / kR, i = blockIdx.x * 4 + li;
+    EPair ep{0, 0.0};
+    if (i < P.n && r < P.nreal) {
+        const size_t q = (size_t)i * kR + r;
+        const int tr = P.tr, t = i / tr, ro = i % tr;
+        double hx = 0.0;
+        for (int p = A.part0[t]; p < A.part0[t + 1]; p++) hx += A.ypart[((size_t)p * tr + ro) * kR + r];
+        double gs = P.hdiv != 0.0 ? hx / P.hdiv : hx;
+        double gn = 0.0;
+        gn = gn + A.sigdot[q];
+        gs = gs + A.taudot[q];
+        const double *ys = (s == 0) ? A.y : A.ytmp;
+        const size_t e0 = (3 * (size_t)i) * kR + r;
+        double kk[3];
+        deriv(P, gs, gn, ys[e0], ys[e0 + kR], ys[e0 + 2 * kR], A.vel[q], A.a[q], A.b[q], A.dc[q], A.f0[q], kk);
+        const size_t m3 = 3 * (size_t)P.n * kR;
+        double *ks = A.k + s * m3;
+        ks[e0] = kk[0];
+        ks[e0 + kR] = kk[1];
+        ks[e0 + 2 * kR] = kk[2];
+        const double h = c->h[r];
+        const double *k1 = A.k, *k2 = A.k + m3, *k3 = A.k + 2 * m3, *k4 = A.k + 3 * m3, *k5 = A.k + 4 * m3,
+                     *k6 = A.k + 5 * m3;
+        if (s < 5) {
+            double yt[3];
+            for (int cc = 0; cc < 3; cc++) {
+                const size_t e = e0 + cc * kR;
+                const double y = A.y[e];
+                switch (s) {
+                case 0: yt[cc] = y + P.T.b21 * h * k1[e]; break;
+                case 1: yt[cc] = y + h * (P.T.b31 * k1[e] + P.T.b32 * k2[e]); break;
+                case 2: yt[cc] = y + h * (P.T.b41 * k1[e] + P.T.b42 * k2[e] + P.T.b43 * k3[e]); break;
+                case 3: yt[cc] = y + h * (P.T.b51 * k1[e] + P.T.b52 * k2[e] + P.T.b53 * k3[e] + P.T.b54 * k4[e]); break;
+                default:
+                    yt[cc] = y + h * (P.T.b61 * k1[e] + P.T.b62 * k2[e] + P.T.b63 * k3[e] + P.T.b64 * k4[e] + P.T.b65 * k5[e]);
+                    break;
+                }
+                A.ytmp[e] = yt[cc];
+            }
+            const double v = slip_rate(yt[0], yt[1], yt[2], A.a[q], P.vref);
+            A.vel[q] = v;
+            A.x[q] = P.backslip ? v - A.vpl[q] : v;
+        } else {
+            for (int cc = 0; cc < 3; cc++) {
+                const size_t e = e0 + cc * kR;
+                const double yo = A.y[e] + h * (P.T.c1 * k1[e] + P.T.c3 * k3[e] + P.T.c4 * k4[e] + P.T.c6 * k6[e]);
+                const double ye = h * (P.T.dc1 * k1[e] + P.T.dc3 * k3[e] + P.T.dc4 * k4[e] + P.T.dc5 * k5[e] + P.T.dc6 * k6[e]);
+                A.yout[e] = yo;
+                const double er = fabs(ye / yo) / P.eps;
+                EPair cur{0, er > 0.0 ? er : 0.0};
+                if (yo - 1 == yo) cur = EPair{1, 10.0};
+                ep = e_combine(ep, cur);
+            }
+        }
+    }
+    if (s != 5) return;
+    sg[li][r] = ep.g;
+    sv[li][r] = ep.v;
+    __syncthreads();
+    if (li == 0) {
+        EPair b{sg[0][r], sv[0][r]};
+        for (int q = 1; q < 4; q++) b = e_combine(b, EPair{sg[q][r], sv[q][r]});
+        A.errpart[(2 * (size_t)blockIdx.x) * kR + r] = (double)b.g;
+        A.errpart[(2 * (size_t)blockIdx.x + 1) * kR + r] = b.v;
+    }
+}
+
+// 1024 threads: lane = realisation, wave w combines a contiguous range of the
+// stage blocks' partials, then wave 0 combines the 16 ranges in order
+constexpr int kDecThreads = 1024, kDecWaves = kDecThreads / kR;
+
+__global__ __launch_bounds__(kDecThreads) void ens_decide_kernel(EnsPar P, EnsCtl *c, const double *__restrict__ errpart,
+                                                                 int nblk)
+{
+#pragma clang fp contract(off)
+    __shared__ int sg[kDecWaves][kR];
+    __shared__ double sv[kDecWaves][kR];
+    if (c->done) return;
+    const int r = threadIdx.x % kR, w = threadIdx.x / kR;
+    const int per = (nblk + kDecWaves - 1) / kDecWaves;
+    const int b0 = min(nblk, w * per), b1 = min(nblk, b0 + per);
+    EPair e{0, 0.0};
+    for (int b = b0; b < b1; b++)
+        e = e_combine(e, EPair{(int)errpart[(2 * (size_t)b) * kR + r], errpart[(2 * (size_t)b + 1) * kR + r]});
+    sg[w][r] = e.g;
+    sv[w][r] = e.v;
+    __syncthreads();
+    if (w != 0) return;
+    if (r < P.nreal && !c->frozen[r]) {
+        const double SAFETY = (double)0.9f, PGROW = (double)-0.2f, PSHRNK = (double)-0.25f, tiny = (double)1e-20f;
+        EPair t{sg[0][r], sv[0][r]};
+        for (int q = 1; q < kDecWaves; q++) t = e_combine(t, EPair{sg[q][r], sv[q][r]});
+        const double errmax = t.v, h = c->h[r];
+        c->errmax[r] = errmax;
+        if (errmax < 1.0 && errmax > tiny) {
+            double hn = fmin(2 * h, SAFETY * h * pow(errmax, PGROW));
+            hn = fmin(hn, P.dtmax);
+            c->hdid[r] = h;
+            c->t[r] = c->t[r] + h;
+            c->h[r] = hn;
+            c->acc[r] = 1;
+            c->accepted[r]++;
+        } else {
+            const double hnew = errmax > tiny ? fmax(0.5 * h, SAFETY * h * pow(errmax, PSHRNK)) : 0.5 * h;
+            c->acc[r] = 0;
+            c->rejected[r]++;
+            c->h[r] = hnew;
+            if (c->t[r] + hnew - c->t[r] < 1.e-15) {
+                c->status[r] = HBI_E_STEP;
+                c->frozen[r] = 1;
+            }
+        }
+    } else {
+        c->acc[r] = 0;
+    }
+}
+
+__global__ void ens_commit_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A)
+{
+#pragma clang fp contract(off)
+    __shared__ double vm[4][kR];
+    if (c->done) return;
+    const int r = threadIdx.x % kR, li = threadIdx.x / kR, i = blockIdx.x * 4 + li;
+    double v = 0.0;
+    if (i < P.n && r < P.nreal && c->acc[r]) {
+        const size_t q = (size_t)i * kR + r, e0 = (3 * (size_t)i) * kR + r;
+        for (int cc = 0; cc < 3; cc++) A.y[e0 + cc * kR] = A.yout[e0 + cc * kR];
+        const double dt = c->hdid[r];
+        double sl = A.slip[q];
+        sl = sl + A.velc[q] * dt * 0.5;
+        v = slip_rate(A.y[e0], A.y[e0 + kR], A.y[e0 + 2 * kR], A.a[q], P.vref);
+        sl = sl + v * dt * 0.5;
+        A.slip[q] = sl;
+        A.velc[q] = v;
+    }
+    vm[li][r] = fabs(v);
+    __syncthreads();
+    if (li == 0) A.monpart[(size_t)blockIdx.x * kR + r] = fmax(fmax(vm[0][r], vm[1][r]), fmax(vm[2][r], vm[3][r]));
+}
+
+// history row per accepted step of realisation r: (t, dt, errmax, log10 Vmax)
+__global__ __launch_bounds__(kDecThreads) void ens_monitor_kernel(EnsPar P, EnsCtl *c, const double *__restrict__ monpart,
+                                                                  int nblk, double *__restrict__ hist)
+{
+    __shared__ double sm[kDecWaves][kR];
+    if (c->done) return;
+    const int r = threadIdx.x % kR, w = threadIdx.x / kR;
+    const int per = (nblk + kDecWaves - 1) / kDecWaves;
+    const int b0 = min(nblk, w * per), b1 = min(nblk, b0 + per);
+    double m = 0.0;
+    for (int b = b0; b < b1; b++) m = fmax(m, monpart[(size_t)b * kR + r]);
+    sm[w][r] = m;
+    __syncthreads();
+    if (w != 0) return;
+    if (r < P.nreal) {
+        if (c->acc[r]) {
+            double v = sm[0][r];
+            for (int q = 1; q < kDecWaves; q++) v = fmax(v, sm[q][r]);
+            c->vmax[r] = v;
+            const long long row = c->accepted[r] - 1 - c->hist_base[r];
+            if (hist && row >= 0 && row < c->hist_cap) {
+                double *o = hist + ((size_t)r * c->hist_cap + row) * 4;
+                o[0] = c->t[r];
+                o[1] = c->hdid[r];
+                o[2] = c->errmax[r];
+                o[3] = log10(v);
+            }
+            if (v > P.velmax || v < P.velmin) { c->status[r] = 1; c->frozen[r] = 1; }
+            if (c->accepted[r] >= c->target[r]) c->frozen[r] = 1;
+        }
+    }
+    const unsigned long long fz = __ballot(r < P.nreal && c->frozen[r]);
+    if (r == 0 && __popcll(fz) >= P.nreal) c->done = 1;
+}
+
+__global__ void ens_arm_kernel(EnsCtl *c, int nreal, long long nsteps, long long hist_cap)
+{
+    const int r = threadIdx.x;
+    if (r == 0) c->hist_cap = hist_cap;
+    if (r < nreal) {
+        c->hist_base[r] = c->accepted[r];
+        c->target[r] = c->accepted[r] + nsteps;
+        c->frozen[r] = c->status[r] != 0 ? 1 : 0;
+        c->acc[r] = 0;
+    }
+    if (r == 0) c->done = nsteps > 0 ? 0 : 1;
+}
+
+}  // namespace hbi
+
+struct hbi_ens {
+    hipStream_t stream = nullptr;
+    hbi_hmat *hs = nullptr;
+    hbi_rk_opts opts{};
+    hbi::EnsPar P{};
+    int n = 0, nreal = 0, nblk = 0;
+    hbi::DevBuf<double> a, b, dc, f0, vpl, taudot, sigdot;
+    hbi::DevBuf<double> y, ytmp, yout, k, vel, x, slip, velc, errpart, monpart, z, zpart, ypart;
+    hbi::DevBuf<double> hist;   // nreal-major rows of 4, capacity hist_rows per realisation (graph argument)
+    long long hist_rows = 0;
+    hbi::DevBuf<hbi::EnsCtl> ctl;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double last_ms = 0.0;
+};
+
+namespace hbi {
+
+static EnsArrays e_arrays(hbi_ens *E)
+{
+    EnsArrays A{};
+    A.a = E->a.p; A.b = E->b.p; A.dc = E->dc.p; A.f0 = E->f0.p; A.vpl = E->vpl.p; A.taudot = E->taudot.p;
+    A.sigdot = E->sigdot.p; A.y = E->y.p; A.ytmp = E->ytmp.p; A.yout = E->yout.p; A.k = E->k.p; A.vel = E->vel.p;
+    A.x = E->x.p; A.slip = E->slip.p; A.velc = E->velc.p; A.errpart = E->errpart.p; A.monpart = E->monpart.p;
+    A.ypart = E->ypart.p;
+    A.part0 = hmat_plan(E->hs, E->opts.prec).d_tile_part0.p;
+    return A;
+}
+
+static void e_attempt(hbi_ens *E)
+{
+    const int g = E->nblk;
+    EnsArrays A = e_arrays(E);
+    const int *gate = &E->ctl.p->done;
+    ens_stage0_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
+    for (int s = 0; s < 6; s++) {
+        mr_apply_parts(E->hs, E->opts.prec, E->x.p, E->zpart.p, E->z.p, E->ypart.p, E->stream, gate);
+        ens_stage_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A, s);
+    }
+    ens_decide_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->errpart.p, g);
+    ens_commit_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
+    ens_monitor_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->monpart.p, g, E->hist.p);
+}
+
+// host (realisation-major, nreal x n) -> device (element-major, n x 64)
+static void upload_rm(DevBuf<double> &d, const double *src, int nreal, int n, int comps, hipStream_t s)
+{
+    std::vector<double> t((size_t)n * comps * kR, 0.0);
+    for (int r = 0; r < nreal; r++)
+        for (size_t e = 0; e < (size_t)n * comps; e++) t[e * kR + r] = src[(size_t)r * n * comps + e];
+    d.upload(t, s);
+    HBI_HIP(hipStreamSynchronize(s));
+}
+static void download_rm(const DevBuf<double> &d, double *dst, int nreal, int n, int comps, hipStream_t s)
+{
+    std::vector<double> t((size_t)n * comps * kR);
+    d.download(t.data(), t.size(), s);
+    HBI_HIP(hipStreamSynchronize(s));
+    for (int r = 0; r < nreal; r++)
+        for (size_t e = 0; e < (size_t)n * comps; e++) dst[(size_t)r * n * comps + e] = t[e * kR + r];
+}
+
+}  // namespace hbi
+
+extern "C" {
+
+int hbi_ens_create(hbi_hmat *hs, const hbi_rk_opts *opts, int nreal, const double *a, const double *b,
+                   const double *dc, const double *f0, const double *vpl, const double *taudot,
+                   const double *sigdot, hbi_ens **out)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(hs && opts && out && a && b && dc && f0 && vpl && taudot && sigdot, "NULL argument");
+        HBI_REQUIRE(nreal >= 1 && nreal <= hbi::kR, "1 <= nreal <= 64");
+        HBI_REQUIRE(opts->prec >= 0 && opts->prec <= 2 && opts->eps_r > 0, "bad options");
+        *out = nullptr;
+        hbi_ens *E = new hbi_ens();
+        try {
+            E->hs = hs;
+            E->opts = *opts;
+            const int n = hs->n;
+            E->n = n;
+            E->nreal = nreal;
+            E->nblk = (n + 3) / 4;
+            HBI_HIP(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+            HBI_HIP(hipEventCreate(&E->e0));
+            HBI_HIP(hipEventCreate(&E->e1));
+            hbi::EnsPar &P = E->P;
+            P.n = n; P.nreal = nreal; P.law = opts->law; P.backslip = opts->backslip; P.limitsigma = opts->limitsigma;
+            P.tr = hs->tr; P.vref = opts->vref; P.minsig = opts->minsig; P.maxsig = opts->maxsig;
+            P.dtmax = opts->dtmax; P.hdiv = opts->hdiv; P.eps = opts->eps_r; P.velmin = opts->velmin;
+            P.velmax = opts->velmax;
+            P.T = hbi::rk_tableau();
+            hbi::upload_rm(E->a, a, nreal, n, 1, E->stream);
+            hbi::upload_rm(E->b, b, nreal, n, 1, E->stream);
+            hbi::upload_rm(E->dc, dc, nreal, n, 1, E->stream);
+            hbi::upload_rm(E->f0, f0, nreal, n, 1, E->stream);
+            hbi::upload_rm(E->vpl, vpl, nreal, n, 1, E->stream);
+            hbi::upload_rm(E->taudot, taudot, nreal, n, 1, E->stream);
+            hbi::upload_rm(E->sigdot, sigdot, nreal, n, 1, E->stream);
+            // padding realisations get a = 1 so their (unused) slip rates stay finite
+            {
+                std::vector<double> pa((size_t)n * hbi::kR);
+                HBI_HIP(hipMemcpy(pa.data(), E->a.p, pa.size() * 8, hipMemcpyDeviceToHost));
+                for (int i = 0; i < n; i++)
+                    for (int r = nreal; r < hbi::kR; r++) pa[(size_t)i * hbi::kR + r] = 1.0;
+                E->a.upload(pa, E->stream);
+            }
+            const size_t m = (size_t)n * hbi::kR;
+            E->y.alloc(3 * m); E->ytmp.alloc(3 * m); E->yout.alloc(3 * m); E->k.alloc(18 * m);
+            E->vel.alloc(m); E->x.alloc(m); E->slip.alloc(m); E->velc.alloc(m);
+            E->errpart.alloc(2 * (size_t)E->nblk * hbi::kR);
+            E->monpart.alloc((size_t)E->nblk * hbi::kR);
+            E->z.alloc((size_t)std::max(hs->nz, 1) * hbi::kR);
+            E->zpart.alloc(hbi::mr_zpart_size(hs, opts->prec));
+            E->ypart.alloc(hbi::mr_ypart_size(hs, opts->prec));
+            for (auto *d : {&E->y, &E->ytmp, &E->yout, &E->k, &E->vel, &E->x, &E->slip, &E->velc})
+                HBI_HIP(hipMemsetAsync(d->p, 0, d->n * 8, E->stream));
+            E->ctl.alloc(1);
+            hbi::EnsCtl c{};
+            for (int r = 0; r < hbi::kR; r++) c.h[r] = 1.0;
+            HBI_HIP(hipMemcpyAsync(E->ctl.p, &c, sizeof(c), hipMemcpyHostToDevice, E->stream));
+            HBI_HIP(hipStreamSynchronize(E->stream));
+        } catch (...) {
+            hbi_ens_destroy(E);
+            throw;
+        }
+        *out = E;
+    });
+}
+
+int hbi_ens_destroy(hbi_ens *E)
+{
+    if (!E) return HBI_OK;
+    if (E->stream) (void)hipStreamSynchronize(E->stream);
+    if (E->graph) (void)hipGraphExecDestroy(E->graph);
+    if (E->e0) (void)hipEventDestroy(E->e0);
+    if (E->e1) (void)hipEventDestroy(E->e1);
+    hipStream_t s = E->stream;
+    delete E;
+    if (s) (void)hipStreamDestroy(s);
+    return HBI_OK;
+}
+
+int hbi_ens_set_state(hbi_ens *E, const double *y, double t, double dt_next)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(E && y, "NULL argument");
+        const int n = E->n;
+        hbi::upload_rm(E->y, y, E->nreal, n, 3, E->stream);
+        // padding realisations: a finite state (psi 0, tau 0, sigma 1)
+        std::vector<double> t3((size_t)3 * n * hbi::kR);
+        HBI_HIP(hipMemcpy(t3.data(), E->y.p, t3.size() * 8, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; i++)
+            for (int r = E->nreal; r < hbi::kR; r++) t3[(3 * (size_t)i + 2) * hbi::kR + r] = 1.0;
+        E->y.upload(t3, E->stream);
+        HBI_HIP(hipMemsetAsync(E->slip.p, 0, E->slip.n * 8, E->stream));
+        hbi::EnsCtl c{};
+        for (int r = 0; r < hbi::kR; r++) { c.h[r] = dt_next; c.t[r] = t; }
+        HBI_HIP(hipMemcpyAsync(E->ctl.p, &c, sizeof(c), hipMemcpyHostToDevice, E->stream));
+        // committed slip rate of the initial state
+        hbi::EnsArrays A = hbi::e_arrays(E);
+        hbi::ens_stage0_kernel<<<E->nblk, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
+        HBI_HIP(hipMemcpyAsync(E->velc.p, E->vel.p, E->vel.n * 8, hipMemcpyDeviceToDevice, E->stream));
+        HBI_HIP(hipStreamSynchronize(E->stream));
+    });
+}
+
+int hbi_ens_get_state(hbi_ens *E, double *y, double *slip, double *vel, double *t, long long *accepted,
+                      long long *rejected, int *status)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(E, "NULL argument");
+        const int n = E->n;
+        if (y) hbi::download_rm(E->y, y, E->nreal, n, 3, E->stream);
+        if (slip) hbi::download_rm(E->slip, slip, E->nreal, n, 1, E->stream);
+        if (vel) hbi::download_rm(E->velc, vel, E->nreal, n, 1, E->stream);
+        hbi::EnsCtl c;
+        HBI_HIP(hipMemcpy(&c, E->ctl.p, sizeof(c), hipMemcpyDeviceToHost));
+        for (int r = 0; r < E->nreal; r++) {
+            if (t) t[r] = c.t[r];
+            if (accepted) accepted[r] = c.accepted[r];
+            if (rejected) rejected[r] = c.rejected[r];
+            if (status) status[r] = c.status[r];
+        }
+    });
+}
+
+int hbi_ens_step(hbi_ens *E, int nsteps, double *hist, long long *attempts)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(E && nsteps >= 0, "bad arguments");
+        const size_t hrows = (size_t)E->nreal * nsteps;
+        if (hist && (long long)hrows > E->hist_rows) {
+            // the history buffer is a graph argument: a larger one needs a new capture
+            E->hist.alloc(hrows * 4);
+            E->hist_rows = (long long)hrows;
+            if (E->graph) {
+                HBI_HIP(hipGraphExecDestroy(E->graph));
+                E->graph = nullptr;
+            }
+        }
+        if (hist) HBI_HIP(hipMemsetAsync(E->hist.p, 0, hrows * 4 * sizeof(double), E->stream));
+        if (!E->graph) {
+            hipGraph_t gr;
+            HBI_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+            hbi::e_attempt(E);
+            HBI_HIP(hipStreamEndCapture(E->stream, &gr));
+            HBI_HIP(hipGraphInstantiate(&E->graph, gr, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(gr);
+        }
+        hbi::ens_arm_kernel<<<1, hbi::kR, 0, E->stream>>>(E->ctl.p, E->nreal, nsteps, hist ? nsteps : 0);
+        HBI_HIP(hipEventRecord(E->e0, E->stream));
+        hbi::EnsCtl c;
+        long long launched = 0, budget = 50LL * nsteps + 100;
+        int batch = nsteps;
+        for (;;) {
+            for (int q = 0; q < batch; q++) HBI_HIP(hipGraphLaunch(E->graph, E->stream));
+            launched += batch;
+            HBI_HIP(hipMemcpyAsync(&c, E->ctl.p, sizeof(c), hipMemcpyDeviceToHost, E->stream));
+            HBI_HIP(hipStreamSynchronize(E->stream));
+            if (c.done || launched >= budget || nsteps == 0) break;
+            long long need = 0;
+            for (int r = 0; r < E->nreal; r++)
+                if (!c.frozen[r]) need = std::max(need, c.target[r] - c.accepted[r]);
+            batch = (int)std::max<long long>(1, std::min<long long>(need + 2, budget - launched));
+        }
+        HBI_HIP(hipEventRecord(E->e1, E->stream));
+        HBI_HIP(hipEventSynchronize(E->e1));
+        float ms = 0;
+        HBI_HIP(hipEventElapsedTime(&ms, E->e0, E->e1));
+        E->last_ms = ms;
+        if (attempts) *attempts = launched;
+        if (hist) {
+            E->hist.download(hist, hrows * 4, E->stream);
+            HBI_HIP(hipStreamSynchronize(E->stream));
+        }
+    });
+}
+
+int hbi_ens_last_ms(const hbi_ens *E, double *ms)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(E && ms, "NULL argument");
+        *ms = E->last_ms;
+    });
+}
+
+}  // extern "C"

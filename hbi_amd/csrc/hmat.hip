@@ -528,7 +528,6 @@ __global__ void tile_reduce_kernel(int n, int tr, const int *__restrict__ part0,
 }
 
 // ------------------------------------------------------------ host side --
-static int dtype_size(int d) { return d == kF64 ? 8 : (d == kF32 ? 4 : 2); }
 static inline size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
 
 // Group the packed layout into work items: phase-1 items are row runs of one
@@ -1130,6 +1129,7 @@ int hbi_hmat_build(const hbi_geom *g, const hbi_hmat_opts *opts, hbi_hmat **out)
 int hbi_hmat_destroy(hbi_hmat *h)
 {
     if (!h) return HBI_OK;
+    hbi::mr_forget(h);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1195,20 +1195,6 @@ int hbi_hmat_matvec(hbi_hmat *h, const double *x, double *y, int prec, int on_de
             h->d_y.download(y, h->n, h->stream);
             HBI_HIP(hipStreamSynchronize(h->stream));
         }
-    });
-}
-
-int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs, int prec)
-{
-    return hbi::guarded([&] {
-        HBI_REQUIRE(h && X && Y && nrhs > 0, "bad arguments");
-        for (int r = 0; r < nrhs; r++) {
-            h->d_x.upload(X + (size_t)r * h->n, h->n, h->stream);
-            hbi::hmat_apply(h, prec, h->d_x.p, h->d_y.p);
-            HBI_HIP(hipGetLastError());
-            h->d_y.download(Y + (size_t)r * h->n, h->n, h->stream);
-        }
-        HBI_HIP(hipStreamSynchronize(h->stream));
     });
 }
 

@@ -8,6 +8,7 @@
 namespace hbi {
 
 enum DType : int { kF64 = 0, kF32 = 1, kBF16 = 2 };
+inline int dtype_size(int d) { return d == kF64 ? 8 : (d == kF32 ? 4 : 2); }
 
 // A run of TR x ncol column-major values inside a row tile (phase 2).
 struct Seg {
@@ -94,4 +95,11 @@ void hmat_phase1(hbi_hmat *h, int prec, const double *x, double *z, hipStream_t 
 void hmat_phase2(hbi_hmat *h, int prec, const double *x, const double *z, double *ypart, hipStream_t s,
                  const int *gate);
 const Plan &hmat_plan(hbi_hmat *h, int prec);
+// multi-RHS product (RR = 64 columns, row-major N x 64)
+void mr_apply_parts(hbi_hmat *h, int prec, const double *X, double *zpart, double *Z, double *ypart, hipStream_t s,
+                    const int *gate);
+void mr_reduce(hbi_hmat *h, int prec, const double *ypart, double *Y, hipStream_t s);
+size_t mr_zpart_size(hbi_hmat *h, int prec);
+size_t mr_ypart_size(hbi_hmat *h, int prec);
+void mr_forget(const hbi_hmat *h);
 }  // namespace hbi

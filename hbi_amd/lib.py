@@ -76,7 +76,9 @@ _lib = None
 SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "hbi_hmat_build", "hbi_hmat_destroy",
            "hbi_hmat_stats_get", "hbi_hmat_lod", "hbi_hmat_leaves", "hbi_hmat_leaf_data", "hbi_hmat_matvec",
            "hbi_hmat_matmul", "hbi_hmat_time_matvec", "hbi_rk_create", "hbi_rk_destroy", "hbi_rk_set_state",
-           "hbi_rk_get_state", "hbi_rk_derivs", "hbi_rk_rkck", "hbi_rk_step", "hbi_rk_last_ms")
+           "hbi_rk_get_state", "hbi_rk_derivs", "hbi_rk_rkck", "hbi_rk_step", "hbi_rk_last_ms",
+           "hbi_hmat_time_matmul", "hbi_ens_create", "hbi_ens_destroy", "hbi_ens_set_state", "hbi_ens_get_state",
+           "hbi_ens_step", "hbi_ens_last_ms")
 
 
 def lib():
@@ -104,6 +106,14 @@ def lib():
         L.hbi_rk_rkck.argtypes = [C.c_void_p, _dp, C.c_double, _dp, _dp]
         L.hbi_rk_step.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip, _dp, C.POINTER(RkInfo)]
         L.hbi_rk_last_ms.argtypes = [C.c_void_p, _dp]
+        L.hbi_hmat_time_matmul.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        _lp = C.POINTER(C.c_longlong)
+        L.hbi_ens_create.argtypes = [C.c_void_p, C.POINTER(RkOpts), C.c_int] + [_dp] * 7 + [C.POINTER(C.c_void_p)]
+        L.hbi_ens_destroy.argtypes = [C.c_void_p]
+        L.hbi_ens_set_state.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double]
+        L.hbi_ens_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _lp, _lp, _ip]
+        L.hbi_ens_step.argtypes = [C.c_void_p, C.c_int, _dp, _lp]
+        L.hbi_ens_last_ms.argtypes = [C.c_void_p, _dp]
         _lib = L
     return _lib
 
@@ -219,6 +229,12 @@ class HMatrix:
         _check(lib().hbi_hmat_time_matvec(self.h, int(prec), int(iters), _p(ms)))
         return {"lrvt_ms": ms[0], "tile_ms": ms[1], "total_ms": ms[2]}
 
+    def time_matmul(self, prec=PREC_F64, iters=20) -> float:
+        """Device milliseconds of one 64-column product."""
+        ms = np.zeros(1)
+        _check(lib().hbi_hmat_time_matmul(self.h, int(prec), int(iters), _p(ms)))
+        return float(ms[0])
+
 
 class RK:
     """Device friction + RK45 integrator over one (or two) H-matrices."""
@@ -287,4 +303,72 @@ class RK:
     def last_ms(self) -> float:
         ms = C.c_double(0)
         _check(lib().hbi_rk_last_ms(self.r, C.byref(ms)))
+        return ms.value
+
+
+class Ensemble:
+    """nreal <= 64 friction realisations on one fault, advanced in lockstep
+    (one 64-column H product per RK stage, per-realisation step control)."""
+
+    def __init__(self, hs: HMatrix, frs, lod, hdiv=0.0, eps_r=1e-6, prec=PREC_F64, backslip=True,
+                 limitsigma=True, minsig=1.0, maxsig=300.0, dtmax=1e10, velmin=1e-20, velmax=1e7):
+        frs = list(frs)
+        if not 1 <= len(frs) <= 64:
+            raise ValueError("1 <= number of realisations <= 64")
+        law, vref = int(frs[0].law), float(frs[0].vref)
+        if any(int(f.law) != law or float(f.vref) != vref for f in frs):
+            raise ValueError("realisations must share the friction law and vref")
+        lod = np.asarray(lod)
+        self.hs = hs
+        self.nreal = len(frs)
+        self.n = hs.n
+        self.keep = [np.ascontiguousarray(np.stack([np.asarray(getattr(f, k), dtype=np.float64)[lod] for f in frs]))
+                     for k in ("a", "b", "dc", "f0", "vpl", "taudot", "sigdot")]
+        o = RkOpts(law=law, backslip=int(backslip), limitsigma=int(limitsigma), prec=int(prec), vref=vref,
+                   minsig=minsig, maxsig=maxsig, dtmax=dtmax, hdiv=float(hdiv), eps_r=eps_r, velmin=velmin,
+                   velmax=velmax)
+        e = C.c_void_p()
+        _check(lib().hbi_ens_create(hs.h, C.byref(o), self.nreal, *[_p(a) for a in self.keep], C.byref(e)))
+        self.e = e
+
+    def close(self):
+        if getattr(self, "e", None):
+            lib().hbi_ens_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, y, t=0.0, dt=1.0):
+        """y: (nreal, 3N) interleaved (psi, tau, sigma) per realisation, cluster order."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if y.shape != (self.nreal, 3 * self.n):
+            raise ValueError("y must be (nreal, 3N)")
+        _check(lib().hbi_ens_set_state(self.e, _p(y), float(t), float(dt)))
+
+    def get_state(self):
+        y = np.empty((self.nreal, 3 * self.n))
+        slip = np.empty((self.nreal, self.n))
+        vel = np.empty((self.nreal, self.n))
+        t = np.empty(self.nreal)
+        acc = np.empty(self.nreal, dtype=np.int64)
+        rej = np.empty(self.nreal, dtype=np.int64)
+        st = np.empty(self.nreal, dtype=np.int32)
+        _check(lib().hbi_ens_get_state(self.e, _p(y), _p(slip), _p(vel), _p(t),
+                                       acc.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                       rej.ctypes.data_as(C.POINTER(C.c_longlong)), _ipp(st)))
+        return {"y": y, "slip": slip, "vel": vel, "t": t, "accepted": acc, "rejected": rej, "status": st}
+
+    def step(self, nsteps, history=False):
+        """Advance every running realisation nsteps accepted steps.  Returns the
+        lockstep attempts, and with history=True also (nreal, nsteps, 4) rows of
+        (t, dt, errmax, log10 Vmax)."""
+        att = C.c_longlong(0)
+        hist = np.zeros((self.nreal, nsteps, 4)) if history else None
+        _check(lib().hbi_ens_step(self.e, int(nsteps), _p(hist), C.byref(att)))
+        return (att.value, hist) if history else att.value
+
+    def last_ms(self) -> float:
+        ms = C.c_double(0)
+        _check(lib().hbi_ens_last_ms(self.e, C.byref(ms)))
         return ms.value

@@ -119,8 +119,13 @@ int hbi_hmat_leaf_data(const hbi_hmat *h, int ip, double *out);
 /* y = H x in cluster order, with the factors of precision prec.
    on_device = 0: x, y are host pointers; 1: device pointers. */
 int hbi_hmat_matvec(hbi_hmat *h, const double *x, double *y, int prec, int on_device);
-/* Y = H X for nrhs right-hand sides, X and Y column-major N x nrhs (host). */
+/* Y = H X for nrhs right-hand sides, X and Y column-major N x nrhs (host).
+   Computed 64 columns at a time by the multi-RHS kernels (factors read once
+   per 64 columns, FP64 accumulation); no reference counterpart (the
+   reference applies H to one vector at a time). */
 int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs, int prec);
+/* Average device time (ms) of one 64-column product on device-resident data */
+int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms);
 /* Average device time per call of each matvec kernel over iters products on
    device-resident vectors, from HIP events on the library stream:
    ms[0] = low-rank V^T x kernel, ms[1] = row-tile kernel, ms[2] = whole product. */
@@ -165,6 +170,32 @@ int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, d
                 hbi_rk_info *info);
 /* Device time of the last hbi_rk_step call, milliseconds (HIP events) */
 int hbi_rk_last_ms(const hbi_rk *r, double *ms);
+
+/* Ensembles: nreal <= 64 realisations of the friction parameters on one fault
+   (one shear H-matrix, sigma constant or prescribed through sigdot), advanced
+   in lockstep with one 64-column H product per RK stage.  Each realisation
+   keeps its own step size, time, accept/reject decision (rkqs,
+   main_LH.f90:2270-2370) and step count; the reference runs them as separate
+   jobs.  Per-realisation arrays are realisation-major: p[r * N + i] for
+   parameters, y[(r * N + i) * 3 + c] for the state; all host pointers. */
+typedef struct hbi_ens hbi_ens;
+int hbi_ens_create(hbi_hmat *hs, const hbi_rk_opts *opts, int nreal,
+                   const double *a, const double *b, const double *dc, const double *f0,
+                   const double *vpl, const double *taudot, const double *sigdot, hbi_ens **out);
+int hbi_ens_destroy(hbi_ens *e);
+/* Same initial time and first trial step for every realisation; slip reset to 0 */
+int hbi_ens_set_state(hbi_ens *e, const double *y, double t, double dt_next);
+/* Any output pointer may be NULL; t, accepted, rejected, status have nreal
+   entries (status as hbi_rk_info.status) */
+int hbi_ens_get_state(hbi_ens *e, double *y, double *slip, double *vel, double *t,
+                      long long *accepted, long long *rejected, int *status);
+/* Advance every running realisation by nsteps accepted steps (one captured
+   HIP graph per lockstep attempt); *attempts = lockstep attempts launched.
+   A realisation that underflows (HBI_E_STEP) or leaves [velmin, velmax]
+   stops there; the others go on.  hist (may be NULL): nreal x nsteps rows of
+   (t, dt, errmax, log10 Vmax) per accepted step, zero after a stop. */
+int hbi_ens_step(hbi_ens *e, int nsteps, double *hist, long long *attempts);
+int hbi_ens_last_ms(const hbi_ens *e, double *ms);
 
 #ifdef __cplusplus
 }

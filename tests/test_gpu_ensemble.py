@@ -1,0 +1,151 @@
+"""GPU: the 64-column H product and the lockstep ensemble integrator.
+
+* hbi_hmat_matmul against hbi_hmat_matvec column by column, same factors:
+  both accumulate in FP64 in different orders -> |dY| <= 1e-12 max|Y|
+* ensemble realisation r against a single-run integration of the same
+  friction parameters and initial state: the products differ in summation
+  order only, so the step grids coincide over the first steps (states to
+  1e-9 relative) and the slip-rate histories agree in time to 1e-5
+"""
+import numpy as np
+import pytest
+
+from hbi_amd import lib as H
+from hbi_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("label,make,leaf,eps", [
+    ("2dvs", lambda: P.mesh_2dvs(2048, 40 / 2048), 32, 1e-6),
+    ("3dph", lambda: P.mesh_3dph(48, 24, 1.0), 32, 1e-5),
+    ("3dph-highrank", lambda: P.mesh_3dph(64, 32, 1.0), 64, 1e-10),
+])
+def test_gpu_matmul_matches_matvec_columns(label, make, leaf, eps):
+    g = make()
+    Hg = H.HMatrix(g, leaf=leaf, eps=eps, bf16_margin=1.0)
+    st = Hg.stats
+    print(label, g.n, "kt_max", st["kt_max"], flush=True)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((g.n, 70))      # one full 64-column pass and a partial one
+    for prec in (H.PREC_F64, H.PREC_F32, H.PREC_MIXED):
+        Y = Hg.matmul(X, prec)
+        for c in range(X.shape[1]):
+            y = Hg.matvec(X[:, c], prec)
+            assert np.abs(Y[:, c] - y).max() <= 1e-12 * np.abs(y).max(), (prec, c)
+    if label == "3dph-highrank":
+        assert st["kt_max"] > 64      # exercises the split of ranks above 64
+
+
+def _c1_realisations(n, nreal):
+    g = P.mesh_2dvs(n, 40.0 / n)
+    base = P.friction_c1(g)
+    frs = []
+    for r in range(nreal):
+        f = P.friction_c1(g)
+        f.dc = base.dc * (1.0 + 0.02 * r)
+        f.b = base.b * (1.0 + 0.01 * (r % 5))
+        frs.append(f)
+    return g, frs
+
+
+def _onset(t, lv, thresh=-3.0):
+    k = int(np.argmax(lv > thresh))
+    return t[k] if lv[k] > thresh else None
+
+
+@pytest.mark.parametrize("nreal,prec", [(3, H.PREC_F64), (64, H.PREC_F64), (5, H.PREC_F32)])
+def test_gpu_ensemble_matches_single_runs(nreal, prec):
+    """Realisation r of the ensemble against a single-run integration of the
+    same parameters: identical step grids over the first 12 steps (states to
+    1e-9), then, through the first event, the event onset time to 1e-5.
+    Once steps start being rejected, the error estimate of this problem moves
+    by ~1% under 1e-10 relative changes of t (the single-run GPU integrator
+    and the oracle part there too), so step grids are compared only before;
+    after that, ln Vmax interpolated between two different step grids differs
+    by the interpolation error (up to ~4e-4 here), checked only as a guard
+    against gross errors (2e-3)."""
+    g, frs = _c1_realisations(1024, nreal)
+    Hg = H.HMatrix(g, leaf=32, eps=1e-6)
+    lod = Hg.lod
+    ys = []
+    for r, f in enumerate(frs):
+        y = P.init_state(f, lod)
+        P.perturb_state(y, g, lod, amp=0.02 * (1 + 0.005 * r))
+        ys.append(y)
+    ys = np.array(ys)
+    n1, n2 = 12, 400
+    ens = H.Ensemble(Hg, frs, lod, hdiv=0.0, eps_r=1e-6, prec=prec)
+    ens.set_state(ys, 0.0, 1.0)
+    attempts = ens.step(n1)
+    es = ens.get_state()
+    assert (es["accepted"] == n1).all() and (es["status"] == 0).all()
+    assert attempts >= n1 + es["rejected"].max()
+    _, eh = ens.step(n2, history=True)
+    es2 = ens.get_state()
+    print("nreal %d: rejected %s" % (nreal, es2["rejected"][:8]), flush=True)
+    assert (es2["accepted"] == n1 + n2).all()
+    for r in sorted({0, nreal // 2, nreal - 1}):
+        rk = H.RK(Hg, frs[r], lod, g.ds, eps_r=1e-6, prec=prec)
+        rk.set_state(ys[r], 0.0, 1.0)
+        rk.step(n1)
+        y1, slip1, vel1, inf = rk.get_state()
+        assert inf["accepted"] == n1 and inf["rejected"] == es["rejected"][r]
+        assert abs(es["t"][r] - inf["t"]) <= 1e-9 * inf["t"]
+        for c in range(3):
+            ref = y1[c::3]
+            assert np.abs(es["y"][r, c::3] - ref).max() <= 1e-9 * np.abs(ref).max(), (r, c)
+        assert np.abs(es["slip"][r] - slip1).max() <= 1e-9 * np.abs(slip1).max() + 1e-15
+        assert np.abs(es["vel"][r] - vel1).max() <= 1e-8 * np.abs(vel1).max()
+        _, sh, _ = rk.step(n2, history=True)
+        ts, lvs = sh[:, 0], sh[:, 4]
+        te, lve = eh[r, :, 0], eh[r, :, 3]
+        on_s, on_e = _onset(ts, lvs), _onset(te, lve)
+        assert (on_s is None) == (on_e is None)
+        t_end = 0.5 * on_s if on_s is not None else min(ts[-1], te[-1])
+        tg = np.linspace(max(ts[0], te[0]), t_end, 2000)
+        dl = np.log(10.0) * np.abs(np.interp(tg, ts, lvs) - np.interp(tg, te, lve)).max()
+        if on_s is not None:
+            tg9 = np.linspace(max(ts[0], te[0]), 0.9 * on_s, 2000)
+            dl9 = np.log(10.0) * np.abs(np.interp(tg9, ts, lvs) - np.interp(tg9, te, lve)).max()
+        else:
+            dl9 = float("nan")
+        print("  r %d: onset %s vs %s, max |d ln Vmax| to 0.5 onset %.2e, to 0.9 onset %.2e"
+              % (r, on_e, on_s, dl, dl9), flush=True)
+        if on_s is not None:
+            assert abs(on_e - on_s) <= 1e-5 * on_s
+        assert dl <= 2e-3
+
+
+def test_gpu_ensemble_stop_is_per_realisation():
+    """Stops are per realisation: one leaves [velmin, velmax]
+    (main_LH.f90:1438-1445), one underflows the step size (an exact steady
+    state has errmax <= tiny, which rkqs treats as a failed step and halves h
+    until t + h == t, main_LH.f90:2309-2352; the oracle does the same), and
+    the others complete their steps, also on a second call."""
+    g, frs = _c1_realisations(512, 4)
+    frs[2] = frs[0]                # unmodified parameters: init_state is their exact steady state
+    Hg = H.HMatrix(g, leaf=32, eps=1e-6)
+    lod = Hg.lod
+    ys = np.array([P.init_state(f, lod) for f in frs])
+    P.perturb_state(ys[0], g, lod)
+    ys[1, 1::3] *= 0.8             # max V ~ 2e-12 < velmin
+    P.perturb_state(ys[3], g, lod)
+    ens = H.Ensemble(Hg, frs, lod, eps_r=1e-6, velmin=1e-11)
+    ens.set_state(ys, 0.0, 1.0)
+    ens.step(200)
+    es = ens.get_state()
+    assert list(es["status"]) == [0, 1, H.E_STEP, 0]
+    assert list(es["accepted"]) == [200, 1, 0, 200]
+    ens.step(10)
+    es2 = ens.get_state()
+    assert list(es2["accepted"]) == [210, 1, 0, 210]
+    # the oracle underflows on the same steady state
+    from oracle import oracle_lib as O
+    Ho = O.HMatrix(g, leaf=32, eps=1e-6)
+    so = O.Sim(Ho, frs[2], lod)
+    y = ys[2].copy()
+    a = frs[2].a[lod]
+    vel = 2 * frs[2].vref * np.exp(-y[0::3] / a) * np.sinh(y[1::3] / y[2::3] / a)
+    done, *_ = so.run(y, 0.0, 1.0, 1e-6, 10, np.zeros(g.n), vel, np.asarray(g.ds)[lod])
+    assert done == 0
