@@ -82,10 +82,16 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // Waits for the copy: a pageable host source may be read after
+    // hipMemcpyAsync returns, so it must outlive the copy; callers often pass
+    // temporaries.
     void upload(const T *h, size_t count, hipStream_t s)
     {
         if (count > n) alloc(count);
-        if (count) HBI_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+        if (count) {
+            HBI_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+            HBI_HIP(hipStreamSynchronize(s));
+        }
     }
     void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
     void download(T *h, size_t count, hipStream_t s) const
@@ -93,6 +99,14 @@ struct DevBuf {
         if (count) HBI_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
     }
 };
+
+// Host -> device copy of a small host object (e.g. a control block on the
+// stack), complete on return.
+inline void to_device_sync(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    HBI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    HBI_HIP(hipStreamSynchronize(s));
+}
 
 // ------------------------------------------------------------------ blocks --
 constexpr int kWave = 64;

@@ -141,6 +141,44 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(EnsPar P, const EnsCtl *
     }
 }
 
+// First level of the per-realisation reductions: workgroup g folds a
+// contiguous range of the stage blocks' partials into partial g, in order
+// (the guarded error combine is associative, so the fold order only has to
+// keep the element order).  mode 0: (guard, errmax) pairs, mode 1: max.
+constexpr int kFold = 64;
+
+__global__ __launch_bounds__(1024) void ens_fold_kernel(const EnsCtl *__restrict__ c, const double *__restrict__ in,
+                                                        int nblk, double *__restrict__ out, int mode)
+{
+    __shared__ int sg[16][kR];
+    __shared__ double sv[16][kR];
+    if (c->done) return;
+    const int r = threadIdx.x % kR, w = threadIdx.x / kR;
+    const int per = (nblk + kFold - 1) / kFold;
+    const int g0 = min(nblk, (int)blockIdx.x * per), g1 = min(nblk, g0 + per);
+    const int pw = (g1 - g0 + 15) / 16;
+    const int b0 = min(g1, g0 + w * pw), b1 = min(g1, b0 + pw);
+    EPair e{0, 0.0};
+    if (mode == 0) {
+        for (int b = b0; b < b1; b++)
+            e = e_combine(e, EPair{(int)in[(2 * (size_t)b) * kR + r], in[(2 * (size_t)b + 1) * kR + r]});
+    } else {
+        for (int b = b0; b < b1; b++) e.v = fmax(e.v, in[(size_t)b * kR + r]);
+    }
+    sg[w][r] = e.g;
+    sv[w][r] = e.v;
+    __syncthreads();
+    if (w != 0) return;
+    EPair t{sg[0][r], sv[0][r]};
+    for (int q = 1; q < 16; q++) t = mode == 0 ? e_combine(t, EPair{sg[q][r], sv[q][r]}) : EPair{0, fmax(t.v, sv[q][r])};
+    if (mode == 0) {
+        out[(2 * (size_t)blockIdx.x) * kR + r] = (double)t.g;
+        out[(2 * (size_t)blockIdx.x + 1) * kR + r] = t.v;
+    } else {
+        out[(size_t)blockIdx.x * kR + r] = t.v;
+    }
+}
+
 // 1024 threads: lane = realisation, wave w combines a contiguous range of the
 // stage blocks' partials, then wave 0 combines the 16 ranges in order
 constexpr int kDecThreads = 1024, kDecWaves = kDecThreads / kR;
@@ -272,6 +310,7 @@ struct hbi_ens {
     int n = 0, nreal = 0, nblk = 0;
     hbi::DevBuf<double> a, b, dc, f0, vpl, taudot, sigdot;
     hbi::DevBuf<double> y, ytmp, yout, k, vel, x, slip, velc, errpart, monpart, z, zpart, ypart;
+    hbi::DevBuf<double> fold;   // first-level reduction partials (kFold x 2 x 64)
     hbi::DevBuf<double> hist;   // nreal-major rows of 4, capacity hist_rows per realisation (graph argument)
     long long hist_rows = 0;
     hbi::DevBuf<hbi::EnsCtl> ctl;
@@ -289,7 +328,7 @@ static EnsArrays e_arrays(hbi_ens *E)
     A.sigdot = E->sigdot.p; A.y = E->y.p; A.ytmp = E->ytmp.p; A.yout = E->yout.p; A.k = E->k.p; A.vel = E->vel.p;
     A.x = E->x.p; A.slip = E->slip.p; A.velc = E->velc.p; A.errpart = E->errpart.p; A.monpart = E->monpart.p;
     A.ypart = E->ypart.p;
-    A.part0 = hmat_plan(E->hs, E->opts.prec).d_tile_part0.p;
+    A.part0 = mr_part0(E->hs, E->opts.prec);
     return A;
 }
 
@@ -303,19 +342,24 @@ static void e_attempt(hbi_ens *E)
         mr_apply_parts(E->hs, E->opts.prec, E->x.p, E->zpart.p, E->z.p, E->ypart.p, E->stream, gate);
         ens_stage_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A, s);
     }
-    ens_decide_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->errpart.p, g);
+    ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->errpart.p, g, E->fold.p, 0);
+    ens_decide_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold);
     ens_commit_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
-    ens_monitor_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->monpart.p, g, E->hist.p);
+    ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->monpart.p, g, E->fold.p, 1);
+    ens_monitor_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold, E->hist.p);
 }
 
-// host (realisation-major, nreal x n) -> device (element-major, n x 64)
-static void upload_rm(DevBuf<double> &d, const double *src, int nreal, int n, int comps, hipStream_t s)
+// host (realisation-major, nreal x n x comps) -> device (element-major,
+// n x comps x 64); the unused realisation slots r >= nreal get pad[c]
+static void upload_rm(DevBuf<double> &d, const double *src, int nreal, int n, int comps, const double *pad,
+                      hipStream_t s)
 {
-    std::vector<double> t((size_t)n * comps * kR, 0.0);
+    std::vector<double> t((size_t)n * comps * kR);
+    for (size_t e = 0; e < (size_t)n * comps; e++)
+        for (int r = nreal; r < kR; r++) t[e * kR + r] = pad[e % comps];
     for (int r = 0; r < nreal; r++)
         for (size_t e = 0; e < (size_t)n * comps; e++) t[e * kR + r] = src[(size_t)r * n * comps + e];
     d.upload(t, s);
-    HBI_HIP(hipStreamSynchronize(s));
 }
 static void download_rm(const DevBuf<double> &d, double *dst, int nreal, int n, int comps, hipStream_t s)
 {
@@ -356,26 +400,21 @@ int hbi_ens_create(hbi_hmat *hs, const hbi_rk_opts *opts, int nreal, const doubl
             P.dtmax = opts->dtmax; P.hdiv = opts->hdiv; P.eps = opts->eps_r; P.velmin = opts->velmin;
             P.velmax = opts->velmax;
             P.T = hbi::rk_tableau();
-            hbi::upload_rm(E->a, a, nreal, n, 1, E->stream);
-            hbi::upload_rm(E->b, b, nreal, n, 1, E->stream);
-            hbi::upload_rm(E->dc, dc, nreal, n, 1, E->stream);
-            hbi::upload_rm(E->f0, f0, nreal, n, 1, E->stream);
-            hbi::upload_rm(E->vpl, vpl, nreal, n, 1, E->stream);
-            hbi::upload_rm(E->taudot, taudot, nreal, n, 1, E->stream);
-            hbi::upload_rm(E->sigdot, sigdot, nreal, n, 1, E->stream);
-            // padding realisations get a = 1 so their (unused) slip rates stay finite
-            {
-                std::vector<double> pa((size_t)n * hbi::kR);
-                HBI_HIP(hipMemcpy(pa.data(), E->a.p, pa.size() * 8, hipMemcpyDeviceToHost));
-                for (int i = 0; i < n; i++)
-                    for (int r = nreal; r < hbi::kR; r++) pa[(size_t)i * hbi::kR + r] = 1.0;
-                E->a.upload(pa, E->stream);
-            }
+            // unused realisation slots: a = 1 keeps their (never used) slip rates finite
+            const double one = 1.0, zero = 0.0;
+            hbi::upload_rm(E->a, a, nreal, n, 1, &one, E->stream);
+            hbi::upload_rm(E->b, b, nreal, n, 1, &zero, E->stream);
+            hbi::upload_rm(E->dc, dc, nreal, n, 1, &one, E->stream);
+            hbi::upload_rm(E->f0, f0, nreal, n, 1, &zero, E->stream);
+            hbi::upload_rm(E->vpl, vpl, nreal, n, 1, &zero, E->stream);
+            hbi::upload_rm(E->taudot, taudot, nreal, n, 1, &zero, E->stream);
+            hbi::upload_rm(E->sigdot, sigdot, nreal, n, 1, &zero, E->stream);
             const size_t m = (size_t)n * hbi::kR;
             E->y.alloc(3 * m); E->ytmp.alloc(3 * m); E->yout.alloc(3 * m); E->k.alloc(18 * m);
             E->vel.alloc(m); E->x.alloc(m); E->slip.alloc(m); E->velc.alloc(m);
             E->errpart.alloc(2 * (size_t)E->nblk * hbi::kR);
             E->monpart.alloc((size_t)E->nblk * hbi::kR);
+            E->fold.alloc(2 * (size_t)hbi::kFold * hbi::kR);
             E->z.alloc((size_t)std::max(hs->nz, 1) * hbi::kR);
             E->zpart.alloc(hbi::mr_zpart_size(hs, opts->prec));
             E->ypart.alloc(hbi::mr_ypart_size(hs, opts->prec));
@@ -384,7 +423,7 @@ int hbi_ens_create(hbi_hmat *hs, const hbi_rk_opts *opts, int nreal, const doubl
             E->ctl.alloc(1);
             hbi::EnsCtl c{};
             for (int r = 0; r < hbi::kR; r++) c.h[r] = 1.0;
-            HBI_HIP(hipMemcpyAsync(E->ctl.p, &c, sizeof(c), hipMemcpyHostToDevice, E->stream));
+            hbi::to_device_sync(E->ctl.p, &c, sizeof(c), E->stream);
             HBI_HIP(hipStreamSynchronize(E->stream));
         } catch (...) {
             hbi_ens_destroy(E);
@@ -412,17 +451,13 @@ int hbi_ens_set_state(hbi_ens *E, const double *y, double t, double dt_next)
     return hbi::guarded([&] {
         HBI_REQUIRE(E && y, "NULL argument");
         const int n = E->n;
-        hbi::upload_rm(E->y, y, E->nreal, n, 3, E->stream);
-        // padding realisations: a finite state (psi 0, tau 0, sigma 1)
-        std::vector<double> t3((size_t)3 * n * hbi::kR);
-        HBI_HIP(hipMemcpy(t3.data(), E->y.p, t3.size() * 8, hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; i++)
-            for (int r = E->nreal; r < hbi::kR; r++) t3[(3 * (size_t)i + 2) * hbi::kR + r] = 1.0;
-        E->y.upload(t3, E->stream);
+        // unused realisation slots: a finite state (psi 0, tau 0, sigma 1)
+        const double pad[3] = {0.0, 0.0, 1.0};
+        hbi::upload_rm(E->y, y, E->nreal, n, 3, pad, E->stream);
         HBI_HIP(hipMemsetAsync(E->slip.p, 0, E->slip.n * 8, E->stream));
         hbi::EnsCtl c{};
         for (int r = 0; r < hbi::kR; r++) { c.h[r] = dt_next; c.t[r] = t; }
-        HBI_HIP(hipMemcpyAsync(E->ctl.p, &c, sizeof(c), hipMemcpyHostToDevice, E->stream));
+        hbi::to_device_sync(E->ctl.p, &c, sizeof(c), E->stream);
         // committed slip rate of the initial state
         hbi::EnsArrays A = hbi::e_arrays(E);
         hbi::ens_stage0_kernel<<<E->nblk, 256, 0, E->stream>>>(E->P, E->ctl.p, A);

@@ -616,10 +616,19 @@ static void build_plan(hbi_hmat *h, int prec)
         if (prec == HBI_PREC_MIXED && L[ip].ltmtx == 1 && h->bf16_ok[ip]) return kBF16;
         return kF32;
     };
-    // ---- phase 1 layout: rows of a1^T per low-rank block, padded to 8 elements
-    for (int ip = 0; ip < (int)L.size(); ip++) {
+    // ---- phase 1 layout: rows of a1^T per low-rank block, padded to 8 elements;
+    // blocks of one column cluster (and dtype) are adjacent, so their rows
+    // form one contiguous matrix against the same x rows
+    std::vector<int> lr_order;
+    for (int ip = 0; ip < (int)L.size(); ip++)
+        if (L[ip].ltmtx == 1 && L[ip].kt > 0) lr_order.push_back(ip);
+    std::stable_sort(lr_order.begin(), lr_order.end(), [&](int x, int y) {
+        if (L[x].c0 != L[y].c0) return L[x].c0 < L[y].c0;
+        if (L[x].nc != L[y].nc) return L[x].nc < L[y].nc;
+        return leaf_dtype(x) < leaf_dtype(y);
+    });
+    for (int ip : lr_order) {
         const Leaf &f = L[ip];
-        if (f.ltmtx != 1 || f.kt == 0) continue;
         const int dt = leaf_dtype(ip), sz = dtype_size(dt);
         const int ldt = (f.nc + 7) / 8 * 8;
         CopyJob J{};

@@ -99,7 +99,10 @@ const Plan &hmat_plan(hbi_hmat *h, int prec);
 void mr_apply_parts(hbi_hmat *h, int prec, const double *X, double *zpart, double *Z, double *ypart, hipStream_t s,
                     const int *gate);
 void mr_reduce(hbi_hmat *h, int prec, const double *ypart, double *Y, hipStream_t s);
+void mr_phase1(hbi_hmat *h, int prec, const double *X, double *zpart, double *Z, hipStream_t s, const int *gate);
+void mr_phase2(hbi_hmat *h, int prec, const double *X, const double *Z, double *ypart, hipStream_t s, const int *gate);
 size_t mr_zpart_size(hbi_hmat *h, int prec);
 size_t mr_ypart_size(hbi_hmat *h, int prec);
+const int *mr_part0(hbi_hmat *h, int prec);
 void mr_forget(const hbi_hmat *h);
 }  // namespace hbi

@@ -421,7 +421,7 @@ int hbi_rk_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_rk_opts *opts, const dou
             r->ctl.alloc(1);
             hbi::RkCtl c{};
             c.h = 1.0;
-            HBI_HIP(hipMemcpyAsync(r->ctl.p, &c, sizeof(c), hipMemcpyHostToDevice, r->stream));
+            hbi::to_device_sync(r->ctl.p, &c, sizeof(c), r->stream);
             HBI_HIP(hipStreamSynchronize(r->stream));
         } catch (...) {
             hbi_rk_destroy(r);
@@ -458,14 +458,14 @@ int hbi_rk_set_state(hbi_rk *r, const double *y, const double *slip, const doubl
             hbi::RkArrays A = hbi::arrays(r);
             // V of the state, written into velc through the stage-0 kernel
             hbi::RkCtl c{};
-            HBI_HIP(hipMemcpyAsync(r->ctl.p, &c, sizeof(c), hipMemcpyHostToDevice, r->stream));
+            hbi::to_device_sync(r->ctl.p, &c, sizeof(c), r->stream);
             hbi::rk_stage0_kernel<<<r->nblk, hbi::kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A, 0);
             HBI_HIP(hipMemcpyAsync(r->velc.p, r->vel.p, n * 8, hipMemcpyDeviceToDevice, r->stream));
         }
         hbi::RkCtl c{};
         c.h = dt_next;
         c.t = t;
-        HBI_HIP(hipMemcpyAsync(r->ctl.p, &c, sizeof(c), hipMemcpyHostToDevice, r->stream));
+        hbi::to_device_sync(r->ctl.p, &c, sizeof(c), r->stream);
         HBI_HIP(hipStreamSynchronize(r->stream));
     });
 }
@@ -494,7 +494,7 @@ int hbi_rk_derivs(hbi_rk *r, const double *y, double *dydx)
     return hbi::guarded([&] {
         HBI_REQUIRE(r && y && dydx, "NULL argument");
         const int n = r->n;
-        HBI_HIP(hipMemcpyAsync(r->y.p, y, 3 * (size_t)n * 8, hipMemcpyHostToDevice, r->stream));
+        hbi::to_device_sync(r->y.p, y, 3 * (size_t)n * 8, r->stream);
         hbi::rk_arm_kernel<<<1, 1, 0, r->stream>>>(r->ctl.p, 0, 0.0, 0);
         hbi::launch_stages(r, 1, 1);
         HBI_HIP(hipGetLastError());
@@ -508,7 +508,7 @@ int hbi_rk_rkck(hbi_rk *r, const double *y, double h, double *yout, double *yerr
     return hbi::guarded([&] {
         HBI_REQUIRE(r && y && yout && yerr, "NULL argument");
         const int n = r->n;
-        HBI_HIP(hipMemcpyAsync(r->y.p, y, 3 * (size_t)n * 8, hipMemcpyHostToDevice, r->stream));
+        hbi::to_device_sync(r->y.p, y, 3 * (size_t)n * 8, r->stream);
         hbi::rk_arm_kernel<<<1, 1, 0, r->stream>>>(r->ctl.p, 0, h, 1);
         hbi::launch_stages(r, 6, 0);
         HBI_HIP(hipGetLastError());

@@ -229,11 +229,11 @@ class HMatrix:
         _check(lib().hbi_hmat_time_matvec(self.h, int(prec), int(iters), _p(ms)))
         return {"lrvt_ms": ms[0], "tile_ms": ms[1], "total_ms": ms[2]}
 
-    def time_matmul(self, prec=PREC_F64, iters=20) -> float:
-        """Device milliseconds of one 64-column product."""
-        ms = np.zeros(1)
+    def time_matmul(self, prec=PREC_F64, iters=20):
+        """Device milliseconds of one 64-column product and of its two phases."""
+        ms = np.zeros(3)
         _check(lib().hbi_hmat_time_matmul(self.h, int(prec), int(iters), _p(ms)))
-        return float(ms[0])
+        return {"p1_ms": ms[0], "p2_ms": ms[1], "total_ms": ms[2]}
 
 
 class RK:

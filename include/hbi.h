@@ -124,7 +124,9 @@ int hbi_hmat_matvec(hbi_hmat *h, const double *x, double *y, int prec, int on_de
    per 64 columns, FP64 accumulation); no reference counterpart (the
    reference applies H to one vector at a time). */
 int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs, int prec);
-/* Average device time (ms) of one 64-column product on device-resident data */
+/* Average device time per call over iters 64-column products on device-resident
+   data, HIP events on the library stream: ms[0] = phase 1 (a1^T X of the
+   low-rank blocks), ms[1] = phase 2 (row tiles), ms[2] = whole product. */
 int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms);
 /* Average device time per call of each matvec kernel over iters products on
    device-resident vectors, from HIP events on the library stream:
