@@ -16,6 +16,7 @@
 // no host synchronisation happens inside a batch.  h, t and the counters
 // live in device memory.
 #include "rkcore.hpp"
+#include "learn.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -26,12 +27,17 @@ struct RkPar {
     int n, law, backslip, limitsigma, has_hn, tr, tr_n;
     double vref, minsig, maxsig, dtmax, hdiv, eps, velmin, velmax, sg;
     Tab T;
+    int learn, use_mlp;                   // step features on; MLP proposals on
+    double safety, ratio_lo, ratio_hi;    // learned proposal: factor and bounds relative to rkqs
 };
 
 struct RkCtl {
     double h, t, hdid, errmax, vmax, meanslip;
     long long attempts, accepted, rejected, target, hist_base, hist_cap;
     int nrjct, done, status, acc;
+    // learned step size: previous accepted step's slip-rate quantiles, h, errmax; trace rows written
+    double prev_q[3], prev_h, prev_e;
+    int have_prev, trace_n;
 };
 
 struct RkArrays {
@@ -43,7 +49,15 @@ struct RkArrays {
     double *hist, *vhist;
     const int *rec;
     int nrec;
+    int *bins;            // histogram of log10 |V| over the elements after an accepted step
+    double *trace;        // rows of 12 features + (h, errmax)
+    int trace_cap;
+    MlpDev mlp;
 };
+
+// log10 |V| histogram for the step features: kBins bins over [kBinLo, kBinHi)
+constexpr int kBins = 1024;
+constexpr double kBinLo = -20.0, kBinHi = 4.0;
 
 constexpr int kRkThreads = 256;
 
@@ -229,6 +243,11 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
         A.velc[i] = v;
         vmax = fabs(v);
         ms = fabs(sl) * A.area[i];
+        if (P.learn) {
+            const double lv = vmax > 0.0 ? log10(vmax) : kBinLo;
+            const int b = min(kBins - 1, max(0, (int)((lv - kBinLo) * (kBins / (kBinHi - kBinLo)))));
+            atomicAdd(&A.bins[b], 1);
+        }
     }
     // per-block max and (ordered) sum
     for (int o = 1; o < kWave; o <<= 1) {
@@ -245,6 +264,82 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
         A.monpart[2 * blockIdx.x] = a;
         A.monpart[2 * blockIdx.x + 1] = b;
     }
+}
+
+// After an accepted step: the 12 step features (slip-rate quantiles from the
+// histogram, their changes, h and errmax of this and the previous step, the
+// rejection flag, the error-optimal step h e^(-1/5)); optional trace row;
+// with the MLP on, the next h replaces the rkqs proposal (within bounds).
+// Runs on one workgroup of kHid threads; clears the histogram.
+__global__ __launch_bounds__(kHid) void rk_learn_kernel(RkPar P, RkCtl *c, RkArrays A)
+{
+    __shared__ int part[kHid];
+    __shared__ float x[kIn], h1[kHid], red[kHid];
+    __shared__ double q[3];
+    if (c->done || !c->acc || !P.learn) return;
+    const int j = threadIdx.x;
+    constexpr int per = kBins / kHid;
+    int cnt = 0;
+    for (int b = 0; b < per; b++) cnt += A.bins[j * per + b];
+    part[j] = cnt;
+    __syncthreads();
+    if (j == 0) {
+        const double w = (kBinHi - kBinLo) / kBins;
+        const long long n50 = (P.n + 1) / 2, n99 = (long long)ceil(0.99 * P.n);
+        long long cum = 0;
+        int b50 = -1, b99 = -1, bmax = 0;
+        for (int t = 0; t < kHid; t++) {
+            if (part[t] == 0) continue;
+            for (int b = t * per; b < (t + 1) * per; b++) {
+                const int k = A.bins[b];
+                if (k == 0) continue;
+                cum += k;
+                if (b50 < 0 && cum >= n50) b50 = b;
+                if (b99 < 0 && cum >= n99) b99 = b;
+                bmax = b;
+            }
+        }
+        q[0] = kBinLo + (bmax + 0.5) * w;
+        q[1] = kBinLo + (max(b50, 0) + 0.5) * w;
+        q[2] = kBinLo + (max(b99, 0) + 0.5) * w;
+        const double lh = log10(c->hdid), le = log10(c->errmax);
+        const double ph = c->have_prev ? log10(c->prev_h) : lh, pe = c->have_prev ? log10(c->prev_e) : le;
+        for (int k = 0; k < 3; k++) {
+            x[k] = (float)q[k];
+            x[3 + k] = c->have_prev ? (float)(q[k] - c->prev_q[k]) : 0.0f;
+        }
+        x[6] = (float)lh;
+        x[7] = (float)le;
+        x[8] = c->nrjct > 0 ? 1.0f : 0.0f;
+        x[9] = (float)ph;
+        x[10] = (float)pe;
+        x[11] = (float)(lh - 0.2 * le);
+        if (A.trace && c->trace_n < A.trace_cap) {
+            double *row = A.trace + 14 * (size_t)c->trace_n;
+            for (int k = 0; k < kIn; k++) row[k] = (double)x[k];
+            row[12] = c->hdid;
+            row[13] = c->errmax;
+            c->trace_n++;
+        }
+    }
+    __syncthreads();
+    if (P.use_mlp) {
+        const float out = mlp_infer_block(A.mlp, x, h1, red);
+        if (j == 0) {
+            const double h_pi = c->h;
+            double hn = P.safety * pow(10.0, (double)x[11] + (double)out);
+            if (!(hn > 0.0) || !isfinite(hn)) hn = h_pi;
+            hn = fmin(fmax(hn, P.ratio_lo * h_pi), P.ratio_hi * h_pi);
+            c->h = fmin(hn, P.dtmax);
+        }
+    }
+    if (j == 0) {
+        for (int k = 0; k < 3; k++) c->prev_q[k] = q[k];
+        c->prev_h = c->hdid;
+        c->prev_e = c->errmax;
+        c->have_prev = 1;
+    }
+    for (int b = 0; b < per; b++) A.bins[j * per + b] = 0;
 }
 
 __global__ void rk_monitor_kernel(RkPar P, RkCtl *c, RkArrays A, int nblk)
@@ -293,6 +388,12 @@ struct hbi_rk {
     hbi::DevBuf<double> hist, vhist;
     hbi::DevBuf<int> rec;
     hbi::DevBuf<hbi::RkCtl> ctl;
+    hbi::DevBuf<int> bins;
+    hbi::DevBuf<double> trace;
+    int trace_cap = 0;
+    hbi::DevBuf<float> mlp_p, mlp_mu, mlp_sd;   // copy of the step-size MLP
+    hbi::DevBuf<uint16_t> mlp_w;
+    bool dirty = false;                        // graph arguments changed: recapture
     hipGraphExec_t graph = nullptr;
     int graph_nrec = -1;
     double *graph_hist = nullptr;
@@ -313,6 +414,10 @@ static RkArrays arrays(hbi_rk *r)
     A.part0 = hmat_plan(r->hs, r->opts.prec).d_tile_part0.p;
     A.part0_n = r->hn ? hmat_plan(r->hn, r->opts.prec).d_tile_part0.p : nullptr;
     A.errpart = r->errpart.p; A.monpart = r->monpart.p;
+    A.bins = r->bins.p;
+    A.trace = r->trace.p;
+    A.trace_cap = r->trace_cap;
+    A.mlp = MlpDev{r->mlp_w.p, r->mlp_p.p, r->mlp_mu.p, r->mlp_sd.p};
     A.hist = r->hist.n ? r->hist.p : nullptr;
     A.vhist = r->vhist.n ? r->vhist.p : nullptr;
     A.rec = r->rec.p;
@@ -345,6 +450,7 @@ static void launch_attempt(hbi_rk *r)
     rk_decide_kernel<<<1, 64, 0, r->stream>>>(r->P, r->ctl.p, r->errpart.p, g);
     RkArrays A = arrays(r);
     rk_commit_kernel<<<g, kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A);
+    if (r->P.learn) rk_learn_kernel<<<1, kHid, 0, r->stream>>>(r->P, r->ctl.p, A);
     rk_monitor_kernel<<<1, 256, 0, r->stream>>>(r->P, r->ctl.p, A, g);
 }
 
@@ -419,6 +525,11 @@ int hbi_rk_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_rk_opts *opts, const dou
             r->errpart.alloc(2 * (size_t)r->nblk);
             r->monpart.alloc(2 * (size_t)r->nblk);
             r->ctl.alloc(1);
+            r->bins.alloc(hbi::kBins);
+            HBI_HIP(hipMemsetAsync(r->bins.p, 0, hbi::kBins * sizeof(int), r->stream));
+            r->P.safety = 0.9;
+            r->P.ratio_lo = 0.125;
+            r->P.ratio_hi = 8.0;
             hbi::RkCtl c{};
             c.h = 1.0;
             hbi::to_device_sync(r->ctl.p, &c, sizeof(c), r->stream);
@@ -541,7 +652,9 @@ int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, d
             recapture = true;
         }
         if (nrec) r->rec.upload(rec, nrec, r->stream);
+        if (r->dirty) recapture = true;
         if (recapture) hbi::capture(r);
+        r->dirty = false;
         hbi::rk_arm_kernel<<<1, 1, 0, r->stream>>>(r->ctl.p, nsteps, 0.0, 0);
         HBI_HIP(hipEventRecord(r->e0, r->stream));
         hbi::RkCtl c;
@@ -579,6 +692,78 @@ int hbi_rk_last_ms(const hbi_rk *r, double *ms)
     return hbi::guarded([&] {
         HBI_REQUIRE(r && ms, "NULL argument");
         *ms = r->last_ms;
+    });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int hbi_rk_set_controller(hbi_rk *r, int mode, const hbi_mlp *m, double safety, double ratio_lo, double ratio_hi)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && (mode == 0 || mode == 1), "mode must be 0 (rkqs) or 1 (learned)");
+        HBI_REQUIRE(mode == 0 || m, "learned mode needs an MLP");
+        HBI_REQUIRE(mode == 0 || (safety > 0 && ratio_lo > 0 && ratio_hi >= ratio_lo), "bad proposal bounds");
+        if (mode == 1) {
+            r->mlp_p.alloc(hbi::kNParam);
+            r->mlp_w.alloc(hbi::kNParam);
+            r->mlp_mu.alloc(hbi::kIn);
+            r->mlp_sd.alloc(hbi::kIn);
+            HBI_HIP(hipStreamSynchronize(m->stream));
+            HBI_HIP(hipMemcpyAsync(r->mlp_p.p, m->p.p, hbi::kNParam * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
+            HBI_HIP(hipMemcpyAsync(r->mlp_w.p, m->w.p, hbi::kNParam * sizeof(uint16_t), hipMemcpyDeviceToDevice, r->stream));
+            HBI_HIP(hipMemcpyAsync(r->mlp_mu.p, m->mu.p, hbi::kIn * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
+            HBI_HIP(hipMemcpyAsync(r->mlp_sd.p, m->sd.p, hbi::kIn * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
+            HBI_HIP(hipStreamSynchronize(r->stream));
+            r->P.safety = safety;
+            r->P.ratio_lo = ratio_lo;
+            r->P.ratio_hi = ratio_hi;
+        }
+        r->P.use_mlp = mode;
+        r->P.learn = (mode == 1 || r->trace_cap > 0) ? 1 : 0;
+        r->dirty = true;
+    });
+}
+
+int hbi_rk_set_trace(hbi_rk *r, int cap)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && cap >= 0, "bad arguments");
+        if (cap > 0) r->trace.alloc((size_t)cap * 14);
+        else r->trace.release();
+        r->trace_cap = cap;
+        r->P.learn = (r->P.use_mlp || cap > 0) ? 1 : 0;
+        // restart the row counter
+        hbi::RkCtl c;
+        HBI_HIP(hipMemcpy(&c, r->ctl.p, sizeof(c), hipMemcpyDeviceToHost));
+        c.trace_n = 0;
+        hbi::to_device_sync(r->ctl.p, &c, sizeof(c), r->stream);
+        r->dirty = true;
+    });
+}
+
+int hbi_rk_get_trace(hbi_rk *r, double *feat, double *raw, int *count)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && count, "NULL argument");
+        hbi::RkCtl c;
+        HBI_HIP(hipStreamSynchronize(r->stream));
+        HBI_HIP(hipMemcpy(&c, r->ctl.p, sizeof(c), hipMemcpyDeviceToHost));
+        const int n = std::min(c.trace_n, r->trace_cap);
+        *count = n;
+        if (n == 0) return;
+        std::vector<double> t((size_t)n * 14);
+        r->trace.download(t.data(), t.size(), r->stream);
+        HBI_HIP(hipStreamSynchronize(r->stream));
+        for (int i = 0; i < n; i++) {
+            if (feat)
+                for (int k = 0; k < hbi::kIn; k++) feat[(size_t)i * hbi::kIn + k] = t[(size_t)i * 14 + k];
+            if (raw) {
+                raw[2 * (size_t)i] = t[(size_t)i * 14 + 12];
+                raw[2 * (size_t)i + 1] = t[(size_t)i * 14 + 13];
+            }
+        }
     });
 }
 

@@ -78,7 +78,9 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_hmat_matmul", "hbi_hmat_time_matvec", "hbi_rk_create", "hbi_rk_destroy", "hbi_rk_set_state",
            "hbi_rk_get_state", "hbi_rk_derivs", "hbi_rk_rkck", "hbi_rk_step", "hbi_rk_last_ms",
            "hbi_hmat_time_matmul", "hbi_ens_create", "hbi_ens_destroy", "hbi_ens_set_state", "hbi_ens_get_state",
-           "hbi_ens_step", "hbi_ens_last_ms")
+           "hbi_ens_step", "hbi_ens_last_ms", "hbi_mlp_create", "hbi_mlp_destroy", "hbi_mlp_nparams",
+           "hbi_mlp_set_params", "hbi_mlp_get_params", "hbi_mlp_forward", "hbi_mlp_loss_grad", "hbi_mlp_train",
+           "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace")
 
 
 def lib():
@@ -114,6 +116,18 @@ def lib():
         L.hbi_ens_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _lp, _lp, _ip]
         L.hbi_ens_step.argtypes = [C.c_void_p, C.c_int, _dp, _lp]
         L.hbi_ens_last_ms.argtypes = [C.c_void_p, _dp]
+        _fp = C.POINTER(C.c_float)
+        L.hbi_mlp_create.argtypes = [C.c_ulonglong, C.POINTER(C.c_void_p)]
+        L.hbi_mlp_destroy.argtypes = [C.c_void_p]
+        L.hbi_mlp_nparams.argtypes = []
+        L.hbi_mlp_set_params.argtypes = [C.c_void_p, _fp, _fp, _fp]
+        L.hbi_mlp_get_params.argtypes = [C.c_void_p, _fp, _fp, _fp]
+        L.hbi_mlp_forward.argtypes = [C.c_void_p, _fp, C.c_int, _fp]
+        L.hbi_mlp_loss_grad.argtypes = [C.c_void_p, _fp, _fp, C.c_int, _dp, _fp]
+        L.hbi_mlp_train.argtypes = [C.c_void_p, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _dp]
+        L.hbi_rk_set_controller.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.hbi_rk_set_trace.argtypes = [C.c_void_p, C.c_int]
+        L.hbi_rk_get_trace.argtypes = [C.c_void_p, _dp, _dp, _ip]
         _lib = L
     return _lib
 
@@ -304,6 +318,82 @@ class RK:
         ms = C.c_double(0)
         _check(lib().hbi_rk_last_ms(self.r, C.byref(ms)))
         return ms.value
+
+    def set_controller(self, mode, mlp=None, safety=0.9, ratio_lo=0.125, ratio_hi=8.0):
+        """mode 0: rkqs (the reference's controller); 1: learned proposals from mlp."""
+        _check(lib().hbi_rk_set_controller(self.r, int(mode), mlp.m if mlp is not None else None, float(safety),
+                                           float(ratio_lo), float(ratio_hi)))
+
+    def set_trace(self, cap):
+        _check(lib().hbi_rk_set_trace(self.r, int(cap)))
+        self._trace_cap = int(cap)
+
+    def get_trace(self):
+        """(features (n, 12), raw (n, 2) = (h, errmax)) of the recorded accepted steps."""
+        cap = getattr(self, "_trace_cap", 0)
+        feat = np.zeros((max(cap, 1), 12))
+        raw = np.zeros((max(cap, 1), 2))
+        cnt = C.c_int(0)
+        _check(lib().hbi_rk_get_trace(self.r, _p(feat), _p(raw), C.byref(cnt)))
+        return feat[:cnt.value], raw[:cnt.value]
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+class MLP:
+    """Step-size MLP (12 -> 128 -> 128 -> 1, GELU, BF16 weights) on the device."""
+
+    N_IN = 12
+
+    def __init__(self, seed=0):
+        m = C.c_void_p()
+        _check(lib().hbi_mlp_create(int(seed), C.byref(m)))
+        self.m = m
+        self.nparams = int(lib().hbi_mlp_nparams())
+
+    def close(self):
+        if getattr(self, "m", None):
+            lib().hbi_mlp_destroy(self.m)
+            self.m = None
+
+    def __del__(self):
+        self.close()
+
+    def get_params(self):
+        p = np.empty(self.nparams, dtype=np.float32)
+        mu = np.empty(12, dtype=np.float32)
+        sd = np.empty(12, dtype=np.float32)
+        _check(lib().hbi_mlp_get_params(self.m, _fptr(p), _fptr(mu), _fptr(sd)))
+        return p, mu, sd
+
+    def set_params(self, params=None, mu=None, sd=None):
+        cv = lambda a: np.ascontiguousarray(a, dtype=np.float32) if a is not None else None
+        params, mu, sd = cv(params), cv(mu), cv(sd)
+        _check(lib().hbi_mlp_set_params(self.m, _fptr(params), _fptr(mu), _fptr(sd)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty(X.shape[0], dtype=np.float32)
+        _check(lib().hbi_mlp_forward(self.m, _fptr(X), X.shape[0], _fptr(Y)))
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(self.nparams, dtype=np.float32)
+        L = C.c_double(0)
+        _check(lib().hbi_mlp_loss_grad(self.m, _fptr(X), _fptr(T), X.shape[0], C.byref(L), _fptr(g)))
+        return L.value, g
+
+    def train(self, X, T, epochs=20, batch=1024, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        hist = np.zeros(max(epochs, 1))
+        _check(lib().hbi_mlp_train(self.m, _fptr(X), _fptr(T), X.shape[0], int(epochs), int(batch), float(lr),
+                                   int(seed), _p(hist)))
+        return hist[:epochs]
 
 
 class Ensemble:

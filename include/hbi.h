@@ -173,6 +173,41 @@ int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, d
 /* Device time of the last hbi_rk_step call, milliseconds (HIP events) */
 int hbi_rk_last_ms(const hbi_rk *r, double *ms);
 
+/* Learned step size.  No reference counterpart: the reference's only step
+   controller is rkqs (main_LH.f90:2309-2357), which stays the judge of every
+   step and the fallback on rejection.  A small MLP (12 -> 128 -> 128 -> 1,
+   GELU; BF16 weights, FP32 biases, activations and accumulation) maps
+   features of the last accepted step to a log10 correction of the step size
+   h e^(-1/5) that would have put that step's error ratio e at 1.
+   Parameter vector (FP32): W1 (128 x 12), b1, W2 (128 x 128), b2, W3 (128),
+   b3; features are normalised as (x - mu) / sd. */
+typedef struct hbi_mlp hbi_mlp;
+int hbi_mlp_create(unsigned long long seed, hbi_mlp **out);
+int hbi_mlp_destroy(hbi_mlp *m);
+int hbi_mlp_nparams(void);
+/* any pointer may be NULL (left unchanged); setting params resets Adam */
+int hbi_mlp_set_params(hbi_mlp *m, const float *params, const float *mu, const float *sd);
+int hbi_mlp_get_params(const hbi_mlp *m, float *params, float *mu, float *sd);
+/* X: n x 12 raw features (row-major), Y: n outputs */
+int hbi_mlp_forward(hbi_mlp *m, const float *X, int n, float *Y);
+/* mean squared error over the n samples and its gradient (NULL: not computed) */
+int hbi_mlp_loss_grad(hbi_mlp *m, const float *X, const float *T, int n, double *loss, float *grad);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) on shuffled minibatches; epoch_loss[e]
+   = mean squared error over epoch e (may be NULL) */
+int hbi_mlp_train(hbi_mlp *m, const float *X, const float *T, int n, int epochs, int batch, float lr,
+                  unsigned long long seed, double *epoch_loss);
+
+/* Step-size control of an hbi_rk: mode 0 = rkqs (the reference's), 1 =
+   learned proposals after accepted steps (h = safety 10^(log10(h e^(-1/5)) +
+   mlp), kept within [ratio_lo, ratio_hi] x the rkqs proposal and dtmax;
+   a rejected attempt shrinks h by the rkqs rule).  The MLP's weights are
+   copied; the graph is recaptured. */
+int hbi_rk_set_controller(hbi_rk *r, int mode, const hbi_mlp *m, double safety, double ratio_lo, double ratio_hi);
+/* Record the 12 features and (h, errmax) of up to cap accepted steps from the
+   next hbi_rk_step on (cap 0: off); read them back with hbi_rk_get_trace. */
+int hbi_rk_set_trace(hbi_rk *r, int cap);
+int hbi_rk_get_trace(hbi_rk *r, double *feat, double *raw, int *count);
+
 /* Ensembles: nreal <= 64 realisations of the friction parameters on one fault
    (one shear H-matrix, sigma constant or prescribed through sigdot), advanced
    in lockstep with one 64-column H product per RK stage.  Each realisation
