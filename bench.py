@@ -1,18 +1,23 @@
 """Benchmark: accepted RK45 steps/s of the earthquake-cycle hot path on one GPU.
 
-Workload: the 32768-element 3-D planar strike-slip fault (256 x 128 elements,
+Fault: the 32768-element 3-D planar strike-slip fault (256 x 128 elements,
 ds 0.390625 km, vertical, half-space Okada kernel), lattice H-matrix with
-leaf 64 and ACA eps 1e-5, velocity-weakening patch 60 x 30 km, lognormal a, Dc
-from a seed, aging law, sigma 50 MPa, V0 = Vpl = 1e-9 m/s initial state.  A
-step is one accepted rkqs step (Cash-Karp, eps_r 1e-6, 6 H-matvecs per
-attempt) plus the post-step update, all on the device.
+leaf 64 and ACA eps 1e-5; velocity-weakening patch 60 x 30 km, lognormal a
+and Dc from a seed, aging law, sigma 50 MPa, V0 = Vpl = 1e-9 m/s initial
+state with a nucleation seed.  A step is one accepted rkqs step (Cash-Karp,
+eps_r 1e-6, 6 H products per attempt) plus the post-step update, all on the
+device.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Prints one JSON line.  ``value`` = accepted steps/s of the mixed-precision
-form (FP32 factors, BF16 where the ACA tolerance allows, FP64 ODE state);
-the plain FP64 form, the H-matvec bandwidth of every precision, the
-roofline of the dominant kernel and the CPU oracle baseline ride along.
+Headline (``value``): the batched form -- 64 realisations of the fault's
+friction (seeds 1..64) advanced in lockstep with one 64-column H product per
+RK stage, FP32 factors (mixed precision, FP64 state) -- as accepted steps/s
+summed over the realisations, over K lockstep steps after W warm-up steps.
+Alongside: single-run FP64 (the plain GPU form) and mixed, the single-vector
+H-matvec bandwidth of every precision, the 64-column product on the matrix
+cores, trajectory agreement mixed vs FP64, the roofline of the dominant
+kernel and the CPU oracle baseline.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -28,27 +33,36 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFS = 78.6
 IMAX, JMAX, DS0, LEAF, EPS_H, EPS_R = 256, 128, 0.390625, 64, 1e-5, 1e-6
+NREAL = 64
 
 
-def gpu_part(steps: int, warmup: int, cpu_steps: int):
+def _onset(t, lv, thr=-3.0):
+    k = int(np.argmax(lv > thr))
+    return float(t[k]) if lv[k] > thr else None
+
+
+def gpu_part(steps: int, warmup: int, single_steps: int):
     from hbi_amd import lib as H
     from hbi_amd import problems as P
 
+    print("bench: building the 32768-element H-matrix", file=sys.stderr, flush=True)
     g = P.mesh_3dph(IMAX, JMAX, DS0, 90.0)
-    fr = P.friction_c2(g, IMAX, JMAX, DS0, seed=1)
     t0 = time.perf_counter()
     Hm = H.HMatrix(g, leaf=LEAF, eps=EPS_H, bf16_margin=1.0)
     t_build = time.perf_counter() - t0
     st = Hm.stats
     lod = Hm.lod
-    y0 = P.init_state(fr, lod)
-    P.perturb_state(y0, g, lod, amp=0.02, depth=15.0, width=3.0)
+    lv = Hm.leaves
+    dense, lr = lv[:, 0] != 1, lv[:, 0] == 1
+    p1_entries = int((lv[lr, 4] * lv[lr, 5]).sum())
+    p2_entries = int((lv[dense, 2] * lv[dense, 4]).sum() + (lv[lr, 2] * lv[lr, 5]).sum())
+    out = {"build_s": round(t_build, 3),
+           "hmat": {k: st[k] for k in ("nlr", "ndense", "kt_mean", "kt_max", "dense_entries", "lr_entries",
+                                       "nbf16_blocks")}}
 
-    out = {"build_s": round(t_build, 3), "hmat": {k: st[k] for k in ("nlr", "ndense", "kt_mean", "kt_max",
-                                                                        "dense_entries", "lr_entries",
-                                                                        "nbf16_blocks")}}
-    # matvec kernels (HIP events on the library stream)
+    # single-vector products (HIP events on the library stream)
     mv = {}
     for prec, name in ((H.PREC_F64, "f64"), (H.PREC_F32, "f32"), (H.PREC_MIXED, "mixed")):
         t = Hm.time_matvec(prec, 50)
@@ -60,37 +74,78 @@ def gpu_part(steps: int, warmup: int, cpu_steps: int):
         mv[name]["pct_hbm"] = 100.0 * mv[name]["GBps"] / HBM_PEAK_GBS
     out["matvec"] = mv
 
-    res = {}
-    finals = {}
+    # 64-column products on the matrix cores
+    mm = {}
     for prec, name in ((H.PREC_F64, "f64"), (H.PREC_MIXED, "mixed")):
+        t = Hm.time_matmul(prec, 20)
+        mm[name] = {"ms": t["total_ms"], "p1_ms": t["p1_ms"], "p2_ms": t["p2_ms"],
+                    "p1_TFs": p1_entries * 64 * 2 / (t["p1_ms"] * 1e-3) / 1e12,
+                    "p2_TFs": p2_entries * 64 * 2 / (t["p2_ms"] * 1e-3) / 1e12,
+                    "per_rhs_vs_matvec": mv[name]["ms"] * 64 / t["total_ms"]}
+    out["matmul64"] = mm
+
+    # single runs: FP64 (the plain GPU form) and mixed, with slip-rate histories
+    fr = P.friction_c2(g, IMAX, JMAX, DS0, seed=1)
+    y0 = P.init_state(fr, lod)
+    P.perturb_state(y0, g, lod, amp=0.02, depth=15.0, width=3.0)
+    rec = np.arange(0, g.n, 997, dtype=np.int32)
+    res, traj = {}, {}
+    for prec, name in ((H.PREC_F64, "f64"), (H.PREC_MIXED, "mixed")):
+        print("bench: single run %s, %d steps" % (name, single_steps), file=sys.stderr, flush=True)
         rk = H.RK(Hm, fr, lod, g.ds, hdiv=DS0, eps_r=EPS_R, prec=prec)
         rk.set_state(y0, 0.0, 1.0)
-        if warmup:
-            rk.step(warmup)
+        rk.step(20)
+        rk.set_state(y0, 0.0, 1.0)
         t0 = time.perf_counter()
-        info, hist, _ = rk.step(steps, history=True)
+        info, hist, vh = rk.step(single_steps, history=True, rec=rec)
         wall = time.perf_counter() - t0
-        dev_ms = rk.last_ms()
-        y, slip, vel, _ = rk.get_state()
-        finals[name] = (hist, vel)
-        res[name] = {"steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps, "device_ms": dev_ms,
-                     "attempts": info["attempts"], "rejected": info["rejected"], "t_end_s": info["t"],
-                     "vmax": info["vmax"]}
+        traj[name] = (hist, vh)
+        res[name] = {"steps_per_s": single_steps / wall, "ms_per_step": 1e3 * wall / single_steps,
+                     "device_ms": rk.last_ms(), "attempts": info["attempts"], "rejected": info["rejected"],
+                     "t_end_s": info["t"], "onset_s": _onset(hist[:, 0], hist[:, 4])}
         rk.close()
-    h64, v64 = finals["f64"]
-    hmx, vmx = finals["mixed"]
-    res["mixed_vs_f64"] = {
-        "max_rel_slip_rate": float(np.max(np.abs(vmx - v64) / np.abs(v64))),
-        "max_rel_time": float(np.max(np.abs(hmx[:, 0] - h64[:, 0]) / h64[:, 0])),
-        "speedup": res["mixed"]["steps_per_s"] / res["f64"]["steps_per_s"]}
-    out["rk"] = res
+    h64, v64 = traj["f64"]
+    hmx, vmx = traj["mixed"]
+    on64, onmx = res["f64"]["onset_s"], res["mixed"]["onset_s"]
+    t_end = 0.5 * on64 if on64 else min(h64[-1, 0], hmx[-1, 0])
+    tg = np.linspace(max(h64[0, 0], hmx[0, 0]), t_end, 4000)
+    dlnv = max(float(np.abs(np.interp(tg, h64[:, 0], np.log(np.abs(v64[:, r])))
+                            - np.interp(tg, hmx[:, 0], np.log(np.abs(vmx[:, r])))).max()) for r in range(len(rec)))
+    out["single"] = res
+    out["accuracy_mixed_vs_f64"] = {
+        "onset_rel": abs(onmx - on64) / on64 if (on64 and onmx) else None,
+        "max_abs_dlnV_to_half_onset": dlnv,
+        "note": "slip-rate histories at %d elements compared on a common time grid (step grids differ)" % len(rec)}
+
+    # batched form: 64 realisations in lockstep, mixed precision
+    print("bench: ensemble of %d realisations, %d + %d steps" % (NREAL, warmup, steps), file=sys.stderr, flush=True)
+    frs = [P.friction_c2(g, IMAX, JMAX, DS0, seed=1 + r) for r in range(NREAL)]
+    ys = []
+    for f in frs:
+        y = P.init_state(f, lod)
+        P.perturb_state(y, g, lod, amp=0.02, depth=15.0, width=3.0)
+        ys.append(y)
+    E = H.Ensemble(Hm, frs, lod, hdiv=DS0, eps_r=EPS_R, prec=H.PREC_MIXED)
+    E.set_state(np.array(ys), 0.0, 1.0)
+    if warmup:
+        E.step(warmup)
+    t0 = time.perf_counter()
+    att = E.step(steps)
+    wall = time.perf_counter() - t0
+    es = E.get_state()
+    out["ensemble"] = {"realisations": NREAL, "steps": steps, "wall_s": wall, "device_ms": E.last_ms(),
+                       "lockstep_attempts": att, "ms_per_attempt": 1e3 * wall / max(att, 1),
+                       "steps_per_s": NREAL * steps / wall, "max_rejected": int(es["rejected"].max()),
+                       "status_ok": bool((es["status"] == 0).all())}
+    E.close()
+    out["p2_entries"], out["p1_entries"] = p2_entries, p1_entries
     Hm.close()
     return out
 
 
 def cpu_part(nsteps: int):
     """CPU oracle (FP64, OpenMP over all host cores) on a bounded sample:
-    the same fault and matrix settings, nsteps accepted steps."""
+    the same fault and matrix settings, nsteps accepted single-run steps."""
     from hbi_amd import problems as P
     from oracle import oracle_lib as O
 
@@ -115,16 +170,17 @@ def cpu_part(nsteps: int):
     t0 = time.perf_counter()
     done, *_ = sim.run(y, 0.0, 1.0, EPS_R, nsteps, slip, vel, np.asarray(g.ds)[lod])
     wall = time.perf_counter() - t0
-    return {"value": done / wall, "unit": "RK steps/s", "cores": ncores, "kind": "port",
-            "sample": "%d accepted RK45 steps (FP64 oracle, OpenMP) of the 32768-element fault after a %.1f s "
-                      "H-matrix build" % (done, t_build)}
+    return {"value": done / wall, "unit": "accepted RK steps/s (one realisation)", "cores": ncores, "kind": "port",
+            "sample": "%d accepted RK45 steps (FP64 oracle, OpenMP) of one realisation of the 32768-element fault "
+                      "after a %.1f s H-matrix build" % (done, t_build)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=300, help="timed lockstep steps of the ensemble")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--single-steps", type=int, default=2000)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -132,35 +188,43 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (one process, one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    r = gpu_part(args.steps, args.warmup, args.cpu_steps)
-    mv = r["matvec"]["mixed"]
-    tile_bytes = None
-    from hbi_amd import lib as H  # noqa: F401
+    r = gpu_part(args.steps, args.warmup, args.single_steps)
+    ens = r["ensemble"]
+    mmx = r["matmul64"]["mixed"]
+    mvx = r["matvec"]["mixed"]
+    f64 = r["single"]["f64"]["steps_per_s"]
     line = {
         "metric": "RK steps/s & H-matvec GB/s (%HBM peak), mixed+learned vs FP64, 1e-5 rel err",
-        "value": round(r["rk"]["mixed"]["steps_per_s"], 3),
-        "unit": "accepted RK steps/s",
+        "value": round(ens["steps_per_s"], 1),
+        "unit": "accepted RK steps/s (sum over %d realisations)" % NREAL,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r["rk"]["mixed"]["ms_per_step"], 4),
+        "ms_per_step": round(1e3 * ens["wall_s"] / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 state / f32+bf16 H factors",
+        "dtype": "f64 state / f32 H factors (FP64 MFMA accumulation)",
         "data": "synthetic",
-        "config": {"workload": "3dph 256x128 planar fault, N=32768, leaf 64, ACA eps 1e-5, RK45 eps_r 1e-6, "
-                               "mixed-precision H (FP32/BF16 factors), hipGraph RK attempts",
-                   "n_elements": IMAX * JMAX},
-        "roofline": {"bound": "hbm", "kernel": "tile_kernel (row-tile H-matvec, mixed)",
-                     "achieved": round(mv["tile_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(mv["tile_GBps"] / HBM_PEAK_GBS, 4), "traffic": tile_bytes},
-        "fp64_steps_per_s": round(r["rk"]["f64"]["steps_per_s"], 3),
-        "speedup_vs_fp64": round(r["rk"]["mixed_vs_f64"]["speedup"], 3),
-        "mixed_vs_fp64_max_rel_slip_rate": r["rk"]["mixed_vs_f64"]["max_rel_slip_rate"],
+        "config": {"workload": "3dph 256x128 planar fault, N=32768, leaf 64, ACA eps 1e-5, RK45 eps_r 1e-6; "
+                               "64 friction realisations in lockstep, one 64-column H product per RK stage, "
+                               "FP32 factors, hipGraph attempts",
+                   "n_elements": IMAX * JMAX, "realisations": NREAL},
+        "roofline": {"bound": "mfma", "kernel": "mr_tile_kernel<float,64> (64-RHS row tiles, FP64 MFMA)",
+                     "achieved": round(mmx["p2_TFs"], 2), "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                     "frac": round(mmx["p2_TFs"] / FP64_MFMA_PEAK_TFS, 4), "traffic": None},
+        "roofline_matvec": {"bound": "hbm", "kernel": "tile_kernel<float,64> (single-vector row tiles, mixed)",
+                            "achieved": round(mvx["tile_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(mvx["tile_GBps"] / HBM_PEAK_GBS, 4), "traffic": None},
+        "fp64_single_steps_per_s": round(f64, 2),
+        "mixed_single_steps_per_s": round(r["single"]["mixed"]["steps_per_s"], 2),
+        "speedup_vs_fp64_single": round(ens["steps_per_s"] / f64, 2),
+        "accuracy_mixed_vs_f64": r["accuracy_mixed_vs_f64"],
+        "ensemble": ens,
         "matvec": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in r["matvec"].items()},
-        "rk": r["rk"],
+        "matmul64": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in r["matmul64"].items()},
+        "single": r["single"],
         "build_s": r["build_s"],
         "hmat": r["hmat"],
     }

@@ -23,13 +23,10 @@ constexpr int kRR = 64;      // right-hand sides per product
 constexpr int kKS = 32;      // k-slice staged in LDS
 constexpr int kMrThreads = 256;
 
-template <typename T> __device__ __forceinline__ double ld_as_double(const T *p, size_t i);
-template <> __device__ __forceinline__ double ld_as_double<double>(const double *p, size_t i) { return p[i]; }
-template <> __device__ __forceinline__ double ld_as_double<float>(const float *p, size_t i) { return (double)p[i]; }
-template <> __device__ __forceinline__ double ld_as_double<uint16_t>(const uint16_t *p, size_t i)
-{
-    return (double)__uint_as_float(((uint32_t)p[i]) << 16);
-}
+template <typename T> __device__ __forceinline__ double widen(T v);
+template <> __device__ __forceinline__ double widen<double>(double v) { return v; }
+template <> __device__ __forceinline__ double widen<float>(float v) { return (double)v; }
+template <> __device__ __forceinline__ double widen<uint16_t>(uint16_t v) { return (double)__uint_as_float(((uint32_t)v) << 16); }
 
 // ------------------------------------------------------------- phase 1 --
 struct MItem1 {
@@ -70,13 +67,16 @@ __global__ __launch_bounds__(kMrThreads) void mr_lrvt_kernel(const MItem1 *__res
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
     const int li = lane % 16, lk = lane / 16;
     constexpr int na = 16 * NRB * KS;
-    double ra[NA], rx[NX];
+    // raw factor values stay in registers until the LDS store, so their loads
+    // overlap the MFMAs of the previous slice
+    T ra[NA];
+    double rx[NX];
     auto load = [&](int ts) {
         const int kn = min(KS, it.tn - ts);
 #pragma unroll
         for (int u = 0; u < NA; u++) {
             const int q = tid + u * kMrThreads, k = q / KS, tt = q % KS;
-            ra[u] = (q < na && k < it.kt && tt < kn) ? ld_as_double<T>(a, (size_t)k * it.ldt + it.t0 + ts + tt) : 0.0;
+            ra[u] = (q < na && k < it.kt && tt < kn) ? a[(size_t)k * it.ldt + it.t0 + ts + tt] : (T)0;
         }
 #pragma unroll
         for (int u = 0; u < NX; u++) {
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kMrThreads) void mr_lrvt_kernel(const MItem1 *__res
 #pragma unroll
         for (int u = 0; u < NA; u++) {
             const int q = tid + u * kMrThreads;
-            if (q < na) as[q % KS][q / KS] = ra[u];
+            if (q < na) as[q % KS][q / KS] = widen<T>(ra[u]);
         }
 #pragma unroll
         for (int u = 0; u < NX; u++) {
@@ -162,7 +162,8 @@ __global__ __launch_bounds__(kMrThreads) void mr_tile_kernel(const Item2 *__rest
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
     const int wr = w / 2, wc = w % 2;
     const int li = lane % 16, lk = lane / 16;
-    double ra[NA], rx[NX];
+    T ra[NA];            // raw factor values: widened at the LDS store (see phase 1)
+    double rx[NX];
     // slice c of segment g: columns c .. c+KS of the column-major segment are contiguous
     auto load = [&](const Seg &g, int c) {
         const T *a = (const T *)(buf + g.off);
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(kMrThreads) void mr_tile_kernel(const Item2 *__rest
 #pragma unroll
         for (int u = 0; u < NA; u++) {
             const int q = tid + u * kMrThreads;
-            ra[u] = q / TR < kn ? ld_as_double<T>(a, (size_t)c * TR + q) : 0.0;
+            ra[u] = q / TR < kn ? a[(size_t)c * TR + q] : (T)0;
         }
 #pragma unroll
         for (int u = 0; u < NX; u++) {
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(kMrThreads) void mr_tile_kernel(const Item2 *__rest
 #pragma unroll
         for (int u = 0; u < NA; u++) {
             const int q = tid + u * kMrThreads;
-            as[q / TR][q % TR] = ra[u];
+            as[q / TR][q % TR] = widen<T>(ra[u]);
         }
 #pragma unroll
         for (int u = 0; u < NX; u++) {
