@@ -106,6 +106,70 @@ module m_hbi_abi
       real(c_double), intent(out) :: vhist(*)
       type(hbi_rk_info), intent(out) :: info
     end function
+    integer(c_int) function hbi_hmat_matmul(h, x, y, nrhs, prec) bind(C, name="hbi_hmat_matmul")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(out) :: y(*)
+      integer(c_int), value :: nrhs, prec
+    end function
+    ! ensembles
+    integer(c_int) function hbi_ens_create(hs, opts, nreal, a, b, dc, f0, vpl, taudot, sigdot, e) &
+        bind(C, name="hbi_ens_create")
+      import :: c_int, c_ptr, c_double, hbi_rk_opts
+      type(c_ptr), value :: hs
+      type(hbi_rk_opts), intent(in) :: opts
+      integer(c_int), value :: nreal
+      real(c_double), intent(in) :: a(*), b(*), dc(*), f0(*), vpl(*), taudot(*), sigdot(*)
+      type(c_ptr), intent(out) :: e
+    end function
+    integer(c_int) function hbi_ens_destroy(e) bind(C, name="hbi_ens_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function hbi_ens_set_state(e, y, t, dt) bind(C, name="hbi_ens_set_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double), intent(in) :: y(*)
+      real(c_double), value :: t, dt
+    end function
+    integer(c_int) function hbi_ens_get_state(e, y, slip, vel, t, accepted, rejected, status) &
+        bind(C, name="hbi_ens_get_state")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: e
+      real(c_double), intent(out) :: y(*), slip(*), vel(*), t(*)
+      integer(c_long_long), intent(out) :: accepted(*), rejected(*)
+      integer(c_int), intent(out) :: status(*)
+    end function
+    integer(c_int) function hbi_ens_step(e, nsteps, hist, attempts) bind(C, name="hbi_ens_step")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: e
+      integer(c_int), value :: nsteps
+      real(c_double), intent(out) :: hist(*)
+      integer(c_long_long), intent(out) :: attempts
+    end function
+    ! learned step size
+    integer(c_int) function hbi_mlp_create(seed, m) bind(C, name="hbi_mlp_create")
+      import :: c_int, c_ptr, c_long_long
+      integer(c_long_long), value :: seed
+      type(c_ptr), intent(out) :: m
+    end function
+    integer(c_int) function hbi_mlp_destroy(m) bind(C, name="hbi_mlp_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function hbi_mlp_set_params(m, params, mu, sd) bind(C, name="hbi_mlp_set_params")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      real(c_float), intent(in) :: params(*), mu(*), sd(*)
+    end function
+    integer(c_int) function hbi_rk_set_controller(r, mode, m, safety, ratio_lo, ratio_hi) &
+        bind(C, name="hbi_rk_set_controller")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: r, m
+      integer(c_int), value :: mode
+      real(c_double), value :: safety, ratio_lo, ratio_hi
+    end function
   end interface
 
 contains
