@@ -46,3 +46,30 @@ def test_no_cpu_fallback_when_library_missing(monkeypatch, tmp_path):
     monkeypatch.setattr(H, "_lib", None)
     with pytest.raises(RuntimeError, match="not built"):
         H.lib()
+
+
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    """Size and every field offset of the ctypes mirrors equal the C
+    structs of include/hbi.h (a short mirror would let the library write past
+    the Python-owned buffer)."""
+    import subprocess
+    structs = {"hbi_geom": H.Geom, "hbi_hmat_opts": H.HmatOpts, "hbi_hmat_stats": H.HmatStats,
+               "hbi_rk_opts": H.RkOpts, "hbi_rk_info": H.RkInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hbi.h"', 'int main(void) {']
+    for cname, py in structs.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines += ['return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        a, b, c = ln.split()
+        got[(a, b)] = int(c)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
