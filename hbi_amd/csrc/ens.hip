@@ -13,6 +13,7 @@
 // parameters p[i * 64 + r], so one wavefront covers the 64 realisations of
 // an element with unit-stride accesses.
 #include "rkcore.hpp"
+#include "learn.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -20,11 +21,14 @@
 namespace hbi {
 
 constexpr int kR = 64;
+constexpr int kHistCols = 5;
 
 struct EnsPar {
     int n, nreal, law, backslip, limitsigma, tr;
     double vref, minsig, maxsig, dtmax, hdiv, eps, velmin, velmax;
     Tab T;
+    int learn, use_mlp;                   // step features on; MLP proposals on
+    double safety, ratio_lo, ratio_hi;    // learned proposal: factor and bounds relative to rkqs
 };
 
 struct EnsCtl {           // per realisation
@@ -33,6 +37,9 @@ struct EnsCtl {           // per realisation
     long long hist_cap;
     int acc[kR], frozen[kR], status[kR];
     int done;
+    // learned step size: previous accepted step, rejections since it, trace rows written
+    StepMem mem[kR];
+    int nrjct[kR], trace_n[kR];
 };
 
 struct EnsArrays {
@@ -40,6 +47,10 @@ struct EnsArrays {
     double *y, *ytmp, *yout, *k, *vel, *x, *slip, *velc, *errpart, *monpart;
     const double *ypart;
     const int *part0;
+    int *bins;            // kR x kBins histograms of log10 |V| after an accepted step
+    double *trace;        // kR x trace_cap rows of 12 features + (h, errmax)
+    int trace_cap;
+    MlpDev mlp;
 };
 
 // thread (i, r): r = threadIdx.x % 64, i = blockIdx.x * 4 + threadIdx.x / 64
@@ -218,6 +229,7 @@ __global__ __launch_bounds__(kDecThreads) void ens_decide_kernel(EnsPar P, EnsCt
             const double hnew = errmax > tiny ? fmax(0.5 * h, SAFETY * h * pow(errmax, PSHRNK)) : 0.5 * h;
             c->acc[r] = 0;
             c->rejected[r]++;
+            c->nrjct[r]++;
             c->h[r] = hnew;
             if (c->t[r] + hnew - c->t[r] < 1.e-15) {
                 c->status[r] = HBI_E_STEP;
@@ -246,13 +258,14 @@ __global__ void ens_commit_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArr
         sl = sl + v * dt * 0.5;
         A.slip[q] = sl;
         A.velc[q] = v;
+        if (P.learn) atomicAdd(&A.bins[(size_t)r * kBins + vbin(fabs(v))], 1);
     }
     vm[li][r] = fabs(v);
     __syncthreads();
     if (li == 0) A.monpart[(size_t)blockIdx.x * kR + r] = fmax(fmax(vm[0][r], vm[1][r]), fmax(vm[2][r], vm[3][r]));
 }
 
-// history row per accepted step of realisation r: (t, dt, errmax, log10 Vmax)
+// history row per accepted step of realisation r: (t, dt, errmax, rejections before it, log10 Vmax)
 __global__ __launch_bounds__(kDecThreads) void ens_monitor_kernel(EnsPar P, EnsCtl *c, const double *__restrict__ monpart,
                                                                   int nblk, double *__restrict__ hist)
 {
@@ -273,18 +286,53 @@ __global__ __launch_bounds__(kDecThreads) void ens_monitor_kernel(EnsPar P, EnsC
             c->vmax[r] = v;
             const long long row = c->accepted[r] - 1 - c->hist_base[r];
             if (hist && row >= 0 && row < c->hist_cap) {
-                double *o = hist + ((size_t)r * c->hist_cap + row) * 4;
+                double *o = hist + ((size_t)r * c->hist_cap + row) * kHistCols;
                 o[0] = c->t[r];
                 o[1] = c->hdid[r];
                 o[2] = c->errmax[r];
-                o[3] = log10(v);
+                o[3] = (double)c->nrjct[r];
+                o[4] = log10(v);
             }
+            c->nrjct[r] = 0;
             if (v > P.velmax || v < P.velmin) { c->status[r] = 1; c->frozen[r] = 1; }
             if (c->accepted[r] >= c->target[r]) c->frozen[r] = 1;
         }
     }
     const unsigned long long fz = __ballot(r < P.nreal && c->frozen[r]);
     if (r == 0 && __popcll(fz) >= P.nreal) c->done = 1;
+}
+
+// One workgroup per realisation that accepted a step: its step features,
+// optional trace row and, with the MLP on, the learned next h (rkqs stays
+// the judge; a rejection shrinks h by the rkqs rule).  Clears its histogram.
+__global__ __launch_bounds__(kHid) void ens_learn_kernel(EnsPar P, EnsCtl *c, EnsArrays A)
+{
+    __shared__ int part[kHid];
+    __shared__ float x[kIn], h1[kHid], red[kHid];
+    __shared__ double q[3];
+    const int r = blockIdx.x, j = threadIdx.x;
+    if (c->done || !P.learn || !c->acc[r]) return;
+    int *bins = A.bins + (size_t)r * kBins;
+    step_features(bins, P.n, c->hdid[r], c->errmax[r], c->nrjct[r] > 0, c->mem[r], part, x, q);
+    if (j == 0 && A.trace && c->trace_n[r] < A.trace_cap) {
+        double *row = A.trace + 14 * ((size_t)r * A.trace_cap + c->trace_n[r]);
+        for (int k = 0; k < kIn; k++) row[k] = (double)x[k];
+        row[12] = c->hdid[r];
+        row[13] = c->errmax[r];
+        c->trace_n[r]++;
+    }
+    if (P.use_mlp) {
+        const float out = mlp_infer_block(A.mlp, x, h1, red);
+        if (j == 0) c->h[r] = learned_h(x[11], out, c->h[r], P.safety, P.ratio_lo, P.ratio_hi, P.dtmax);
+    }
+    if (j == 0) {
+        StepMem &m = c->mem[r];
+        for (int k = 0; k < 3; k++) m.prev_q[k] = q[k];
+        m.prev_h = c->hdid[r];
+        m.prev_e = c->errmax[r];
+        m.have_prev = 1;
+    }
+    for (int b = 0; b < kBins / kHid; b++) bins[j * (kBins / kHid) + b] = 0;
 }
 
 __global__ void ens_arm_kernel(EnsCtl *c, int nreal, long long nsteps, long long hist_cap)
@@ -311,9 +359,14 @@ struct hbi_ens {
     hbi::DevBuf<double> a, b, dc, f0, vpl, taudot, sigdot;
     hbi::DevBuf<double> y, ytmp, yout, k, vel, x, slip, velc, errpart, monpart, z, zpart, ypart;
     hbi::DevBuf<double> fold;   // first-level reduction partials (kFold x 2 x 64)
-    hbi::DevBuf<double> hist;   // nreal-major rows of 4, capacity hist_rows per realisation (graph argument)
+    hbi::DevBuf<double> hist;   // nreal-major rows of kHistCols (graph argument)
     long long hist_rows = 0;
     hbi::DevBuf<hbi::EnsCtl> ctl;
+    hbi::DevBuf<int> bins;
+    hbi::DevBuf<double> trace;
+    int trace_cap = 0;
+    hbi::DevBuf<float> mlp_p, mlp_mu, mlp_sd;   // copy of the step-size MLP
+    hbi::DevBuf<uint16_t> mlp_w;
     hipGraphExec_t graph = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0.0;
@@ -329,6 +382,10 @@ static EnsArrays e_arrays(hbi_ens *E)
     A.x = E->x.p; A.slip = E->slip.p; A.velc = E->velc.p; A.errpart = E->errpart.p; A.monpart = E->monpart.p;
     A.ypart = E->ypart.p;
     A.part0 = mr_part0(E->hs, E->opts.prec);
+    A.bins = E->bins.p;
+    A.trace = E->trace.p;
+    A.trace_cap = E->trace_cap;
+    A.mlp = MlpDev{E->mlp_w.p, E->mlp_p.p, E->mlp_mu.p, E->mlp_sd.p};
     return A;
 }
 
@@ -345,6 +402,7 @@ static void e_attempt(hbi_ens *E)
     ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->errpart.p, g, E->fold.p, 0);
     ens_decide_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold);
     ens_commit_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
+    if (E->P.learn) ens_learn_kernel<<<E->nreal, kHid, 0, E->stream>>>(E->P, E->ctl.p, A);
     ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->monpart.p, g, E->fold.p, 1);
     ens_monitor_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold, E->hist.p);
 }
@@ -415,6 +473,11 @@ int hbi_ens_create(hbi_hmat *hs, const hbi_rk_opts *opts, int nreal, const doubl
             E->errpart.alloc(2 * (size_t)E->nblk * hbi::kR);
             E->monpart.alloc((size_t)E->nblk * hbi::kR);
             E->fold.alloc(2 * (size_t)hbi::kFold * hbi::kR);
+            E->bins.alloc((size_t)hbi::kR * hbi::kBins);
+            HBI_HIP(hipMemsetAsync(E->bins.p, 0, E->bins.n * sizeof(int), E->stream));
+            P.safety = 0.9;
+            P.ratio_lo = 0.125;
+            P.ratio_hi = 8.0;
             E->z.alloc((size_t)std::max(hs->nz, 1) * hbi::kR);
             E->zpart.alloc(hbi::mr_zpart_size(hs, opts->prec));
             E->ypart.alloc(hbi::mr_ypart_size(hs, opts->prec));
@@ -493,14 +556,14 @@ int hbi_ens_step(hbi_ens *E, int nsteps, double *hist, long long *attempts)
         const size_t hrows = (size_t)E->nreal * nsteps;
         if (hist && (long long)hrows > E->hist_rows) {
             // the history buffer is a graph argument: a larger one needs a new capture
-            E->hist.alloc(hrows * 4);
+            E->hist.alloc(hrows * hbi::kHistCols);
             E->hist_rows = (long long)hrows;
             if (E->graph) {
                 HBI_HIP(hipGraphExecDestroy(E->graph));
                 E->graph = nullptr;
             }
         }
-        if (hist) HBI_HIP(hipMemsetAsync(E->hist.p, 0, hrows * 4 * sizeof(double), E->stream));
+        if (hist) HBI_HIP(hipMemsetAsync(E->hist.p, 0, hrows * hbi::kHistCols * sizeof(double), E->stream));
         if (!E->graph) {
             hipGraph_t gr;
             HBI_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
@@ -532,7 +595,7 @@ int hbi_ens_step(hbi_ens *E, int nsteps, double *hist, long long *attempts)
         E->last_ms = ms;
         if (attempts) *attempts = launched;
         if (hist) {
-            E->hist.download(hist, hrows * 4, E->stream);
+            E->hist.download(hist, hrows * hbi::kHistCols, E->stream);
             HBI_HIP(hipStreamSynchronize(E->stream));
         }
     });
@@ -543,6 +606,93 @@ int hbi_ens_last_ms(const hbi_ens *E, double *ms)
     return hbi::guarded([&] {
         HBI_REQUIRE(E && ms, "NULL argument");
         *ms = E->last_ms;
+    });
+}
+
+}  // extern "C"
+
+namespace {
+void drop_graph(hbi_ens *E)
+{
+    if (E->graph) {
+        (void)hipGraphExecDestroy(E->graph);
+        E->graph = nullptr;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int hbi_ens_set_controller(hbi_ens *E, int mode, const hbi_mlp *m, double safety, double ratio_lo, double ratio_hi)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(E && (mode == 0 || mode == 1), "mode must be 0 (rkqs) or 1 (learned)");
+        HBI_REQUIRE(mode == 0 || m, "learned mode needs an MLP");
+        HBI_REQUIRE(mode == 0 || (safety > 0 && ratio_lo > 0 && ratio_hi >= ratio_lo), "bad proposal bounds");
+        if (mode == 1) {
+            E->mlp_p.alloc(hbi::kNParam);
+            E->mlp_w.alloc(hbi::kNParam);
+            E->mlp_mu.alloc(hbi::kIn);
+            E->mlp_sd.alloc(hbi::kIn);
+            HBI_HIP(hipStreamSynchronize(m->stream));
+            HBI_HIP(hipMemcpyAsync(E->mlp_p.p, m->p.p, hbi::kNParam * sizeof(float), hipMemcpyDeviceToDevice, E->stream));
+            HBI_HIP(hipMemcpyAsync(E->mlp_w.p, m->w.p, hbi::kNParam * sizeof(uint16_t), hipMemcpyDeviceToDevice, E->stream));
+            HBI_HIP(hipMemcpyAsync(E->mlp_mu.p, m->mu.p, hbi::kIn * sizeof(float), hipMemcpyDeviceToDevice, E->stream));
+            HBI_HIP(hipMemcpyAsync(E->mlp_sd.p, m->sd.p, hbi::kIn * sizeof(float), hipMemcpyDeviceToDevice, E->stream));
+            HBI_HIP(hipStreamSynchronize(E->stream));
+            E->P.safety = safety;
+            E->P.ratio_lo = ratio_lo;
+            E->P.ratio_hi = ratio_hi;
+        }
+        E->P.use_mlp = mode;
+        E->P.learn = (mode == 1 || E->trace_cap > 0) ? 1 : 0;
+        drop_graph(E);
+    });
+}
+
+int hbi_ens_set_trace(hbi_ens *E, int cap)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(E && cap >= 0, "bad arguments");
+        if (cap > 0) E->trace.alloc((size_t)hbi::kR * cap * 14);
+        else E->trace.release();
+        E->trace_cap = cap;
+        E->P.learn = (E->P.use_mlp || cap > 0) ? 1 : 0;
+        hbi::EnsCtl c;
+        HBI_HIP(hipMemcpy(&c, E->ctl.p, sizeof(c), hipMemcpyDeviceToHost));
+        for (int r = 0; r < hbi::kR; r++) c.trace_n[r] = 0;
+        hbi::to_device_sync(E->ctl.p, &c, sizeof(c), E->stream);
+        drop_graph(E);
+    });
+}
+
+int hbi_ens_get_trace(hbi_ens *E, double *feat, double *raw, int *count)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(E && count, "NULL argument");
+        HBI_HIP(hipStreamSynchronize(E->stream));
+        hbi::EnsCtl c;
+        HBI_HIP(hipMemcpy(&c, E->ctl.p, sizeof(c), hipMemcpyDeviceToHost));
+        const int cap = E->trace_cap;
+        std::vector<double> t;
+        if (cap > 0) {
+            t.resize((size_t)hbi::kR * cap * 14);
+            E->trace.download(t.data(), t.size(), E->stream);
+            HBI_HIP(hipStreamSynchronize(E->stream));
+        }
+        for (int r = 0; r < E->nreal; r++) {
+            const int n = std::min(c.trace_n[r], cap);
+            count[r] = n;
+            for (int i = 0; i < n; i++) {
+                const double *row = t.data() + 14 * ((size_t)r * cap + i);
+                if (feat)
+                    for (int k = 0; k < hbi::kIn; k++) feat[((size_t)r * cap + i) * hbi::kIn + k] = row[k];
+                if (raw) {
+                    raw[((size_t)r * cap + i) * 2] = row[12];
+                    raw[((size_t)r * cap + i) * 2 + 1] = row[13];
+                }
+            }
+        }
     });
 }
 

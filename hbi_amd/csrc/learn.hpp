@@ -46,6 +46,82 @@ __device__ __forceinline__ float mlp_infer_block(const MlpDev &M, const float *x
     return red[0] + M.p[kOffB3];
 }
 
+// ---- step features (shared by the single-run and the ensemble RK graphs)
+// histogram of log10 |V| over the elements after an accepted step
+constexpr int kBins = 1024;
+constexpr double kBinLo = -20.0, kBinHi = 4.0;
+
+__device__ __forceinline__ int vbin(double absv)
+{
+    const double lv = absv > 0.0 ? log10(absv) : kBinLo;
+    return min(kBins - 1, max(0, (int)((lv - kBinLo) * (kBins / (kBinHi - kBinLo)))));
+}
+
+// memory of the previous accepted step
+struct StepMem {
+    double prev_q[3], prev_h, prev_e;
+    int have_prev;
+};
+
+// Block of kHid threads; bins: the histogram of n elements.  Thread 0 fills
+// q (max, median, p99 of log10 |V|, bin centres) and the 12 features x:
+// q, their changes since the previous step, log10 h, log10 errmax, the
+// rejection flag, log10 h and errmax of the previous step, and the
+// error-optimal step log10(h errmax^(-1/5)).  part: kHid ints of LDS.
+__device__ __forceinline__ void step_features(const int *bins, int n, double hdid, double errmax, bool rejected,
+                                              const StepMem &m, int *part, float *x, double *q)
+{
+    const int j = threadIdx.x;
+    constexpr int per = kBins / kHid;
+    int cnt = 0;
+    for (int b = 0; b < per; b++) cnt += bins[j * per + b];
+    part[j] = cnt;
+    __syncthreads();
+    if (j == 0) {
+        const double w = (kBinHi - kBinLo) / kBins;
+        const long long n50 = (n + 1) / 2, n99 = (long long)ceil(0.99 * n);
+        long long cum = 0;
+        int b50 = -1, b99 = -1, bmax = 0;
+        for (int t = 0; t < kHid; t++) {
+            if (part[t] == 0) continue;
+            for (int b = t * per; b < (t + 1) * per; b++) {
+                const int k = bins[b];
+                if (k == 0) continue;
+                cum += k;
+                if (b50 < 0 && cum >= n50) b50 = b;
+                if (b99 < 0 && cum >= n99) b99 = b;
+                bmax = b;
+            }
+        }
+        q[0] = kBinLo + (bmax + 0.5) * w;
+        q[1] = kBinLo + (max(b50, 0) + 0.5) * w;
+        q[2] = kBinLo + (max(b99, 0) + 0.5) * w;
+        const double lh = log10(hdid), le = log10(errmax);
+        const double ph = m.have_prev ? log10(m.prev_h) : lh, pe = m.have_prev ? log10(m.prev_e) : le;
+        for (int k = 0; k < 3; k++) {
+            x[k] = (float)q[k];
+            x[3 + k] = m.have_prev ? (float)(q[k] - m.prev_q[k]) : 0.0f;
+        }
+        x[6] = (float)lh;
+        x[7] = (float)le;
+        x[8] = rejected ? 1.0f : 0.0f;
+        x[9] = (float)ph;
+        x[10] = (float)pe;
+        x[11] = (float)(lh - 0.2 * le);
+    }
+    __syncthreads();
+}
+
+// the learned proposal, within [lo, hi] x the rkqs proposal h_pi and dtmax
+__device__ __forceinline__ double learned_h(float x11, float out, double h_pi, double safety, double lo, double hi,
+                                            double dtmax)
+{
+    double hn = safety * pow(10.0, (double)x11 + (double)out);
+    if (!(hn > 0.0) || !isfinite(hn)) hn = h_pi;
+    hn = fmin(fmax(hn, lo * h_pi), hi * h_pi);
+    return fmin(hn, dtmax);
+}
+
 }  // namespace hbi
 
 struct hbi_mlp {

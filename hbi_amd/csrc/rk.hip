@@ -35,9 +35,9 @@ struct RkCtl {
     double h, t, hdid, errmax, vmax, meanslip;
     long long attempts, accepted, rejected, target, hist_base, hist_cap;
     int nrjct, done, status, acc;
-    // learned step size: previous accepted step's slip-rate quantiles, h, errmax; trace rows written
-    double prev_q[3], prev_h, prev_e;
-    int have_prev, trace_n;
+    // learned step size: the previous accepted step; trace rows written
+    StepMem mem;
+    int trace_n;
 };
 
 struct RkArrays {
@@ -55,9 +55,6 @@ struct RkArrays {
     MlpDev mlp;
 };
 
-// log10 |V| histogram for the step features: kBins bins over [kBinLo, kBinHi)
-constexpr int kBins = 1024;
-constexpr double kBinLo = -20.0, kBinHi = 4.0;
 
 constexpr int kRkThreads = 256;
 
@@ -243,11 +240,7 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
         A.velc[i] = v;
         vmax = fabs(v);
         ms = fabs(sl) * A.area[i];
-        if (P.learn) {
-            const double lv = vmax > 0.0 ? log10(vmax) : kBinLo;
-            const int b = min(kBins - 1, max(0, (int)((lv - kBinLo) * (kBins / (kBinHi - kBinLo)))));
-            atomicAdd(&A.bins[b], 1);
-        }
+        if (P.learn) atomicAdd(&A.bins[vbin(vmax)], 1);
     }
     // per-block max and (ordered) sum
     for (int o = 1; o < kWave; o <<= 1) {
@@ -278,68 +271,25 @@ __global__ __launch_bounds__(kHid) void rk_learn_kernel(RkPar P, RkCtl *c, RkArr
     __shared__ double q[3];
     if (c->done || !c->acc || !P.learn) return;
     const int j = threadIdx.x;
-    constexpr int per = kBins / kHid;
-    int cnt = 0;
-    for (int b = 0; b < per; b++) cnt += A.bins[j * per + b];
-    part[j] = cnt;
-    __syncthreads();
-    if (j == 0) {
-        const double w = (kBinHi - kBinLo) / kBins;
-        const long long n50 = (P.n + 1) / 2, n99 = (long long)ceil(0.99 * P.n);
-        long long cum = 0;
-        int b50 = -1, b99 = -1, bmax = 0;
-        for (int t = 0; t < kHid; t++) {
-            if (part[t] == 0) continue;
-            for (int b = t * per; b < (t + 1) * per; b++) {
-                const int k = A.bins[b];
-                if (k == 0) continue;
-                cum += k;
-                if (b50 < 0 && cum >= n50) b50 = b;
-                if (b99 < 0 && cum >= n99) b99 = b;
-                bmax = b;
-            }
-        }
-        q[0] = kBinLo + (bmax + 0.5) * w;
-        q[1] = kBinLo + (max(b50, 0) + 0.5) * w;
-        q[2] = kBinLo + (max(b99, 0) + 0.5) * w;
-        const double lh = log10(c->hdid), le = log10(c->errmax);
-        const double ph = c->have_prev ? log10(c->prev_h) : lh, pe = c->have_prev ? log10(c->prev_e) : le;
-        for (int k = 0; k < 3; k++) {
-            x[k] = (float)q[k];
-            x[3 + k] = c->have_prev ? (float)(q[k] - c->prev_q[k]) : 0.0f;
-        }
-        x[6] = (float)lh;
-        x[7] = (float)le;
-        x[8] = c->nrjct > 0 ? 1.0f : 0.0f;
-        x[9] = (float)ph;
-        x[10] = (float)pe;
-        x[11] = (float)(lh - 0.2 * le);
-        if (A.trace && c->trace_n < A.trace_cap) {
-            double *row = A.trace + 14 * (size_t)c->trace_n;
-            for (int k = 0; k < kIn; k++) row[k] = (double)x[k];
-            row[12] = c->hdid;
-            row[13] = c->errmax;
-            c->trace_n++;
-        }
+    step_features(A.bins, P.n, c->hdid, c->errmax, c->nrjct > 0, c->mem, part, x, q);
+    if (j == 0 && A.trace && c->trace_n < A.trace_cap) {
+        double *row = A.trace + 14 * (size_t)c->trace_n;
+        for (int k = 0; k < kIn; k++) row[k] = (double)x[k];
+        row[12] = c->hdid;
+        row[13] = c->errmax;
+        c->trace_n++;
     }
-    __syncthreads();
     if (P.use_mlp) {
         const float out = mlp_infer_block(A.mlp, x, h1, red);
-        if (j == 0) {
-            const double h_pi = c->h;
-            double hn = P.safety * pow(10.0, (double)x[11] + (double)out);
-            if (!(hn > 0.0) || !isfinite(hn)) hn = h_pi;
-            hn = fmin(fmax(hn, P.ratio_lo * h_pi), P.ratio_hi * h_pi);
-            c->h = fmin(hn, P.dtmax);
-        }
+        if (j == 0) c->h = learned_h(x[11], out, c->h, P.safety, P.ratio_lo, P.ratio_hi, P.dtmax);
     }
     if (j == 0) {
-        for (int k = 0; k < 3; k++) c->prev_q[k] = q[k];
-        c->prev_h = c->hdid;
-        c->prev_e = c->errmax;
-        c->have_prev = 1;
+        for (int k = 0; k < 3; k++) c->mem.prev_q[k] = q[k];
+        c->mem.prev_h = c->hdid;
+        c->mem.prev_e = c->errmax;
+        c->mem.have_prev = 1;
     }
-    for (int b = 0; b < per; b++) A.bins[j * per + b] = 0;
+    for (int b = 0; b < kBins / kHid; b++) A.bins[j * (kBins / kHid) + b] = 0;
 }
 
 __global__ void rk_monitor_kernel(RkPar P, RkCtl *c, RkArrays A, int nblk)

@@ -148,6 +148,13 @@ module m_hbi_abi
       real(c_double), intent(out) :: hist(*)
       integer(c_long_long), intent(out) :: attempts
     end function
+    integer(c_int) function hbi_ens_set_controller(e, mode, m, safety, ratio_lo, ratio_hi) &
+        bind(C, name="hbi_ens_set_controller")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e, m
+      integer(c_int), value :: mode
+      real(c_double), value :: safety, ratio_lo, ratio_hi
+    end function
     ! learned step size
     integer(c_int) function hbi_mlp_create(seed, m) bind(C, name="hbi_mlp_create")
       import :: c_int, c_ptr, c_long_long

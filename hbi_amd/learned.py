@@ -55,6 +55,22 @@ def trace_run(Hm, g, fr, lod, y0, nsteps: int, eps_r=1e-6, prec=H.PREC_F64, mlp=
     return feat, raw
 
 
+def ensemble_trace_run(Hm, g, frs, lod, ys, nsteps: int, hdiv=0.0, eps_r=1e-6, prec=H.PREC_F64, mlp=None,
+                       safety=0.9):
+    """Traces of nsteps accepted steps of up to 64 realisations advanced in
+    lockstep (one 64-column H product per stage), under rkqs or the learned
+    controller; one (features, raw) pair per realisation."""
+    E = H.Ensemble(Hm, frs, lod, hdiv=hdiv, eps_r=eps_r, prec=prec)
+    if mlp is not None:
+        E.set_controller(1, mlp, safety)
+    E.set_trace(nsteps)
+    E.set_state(np.asarray(ys), 0.0, 1.0)
+    E.step(nsteps)
+    out = E.get_trace()
+    E.close()
+    return out
+
+
 def dataset(traces):
     """(X, T) from a list of (features, raw) traces: X = features of step n,
     T = f11[n+1] - f11[n]."""

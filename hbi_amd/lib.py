@@ -80,7 +80,8 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_hmat_time_matmul", "hbi_ens_create", "hbi_ens_destroy", "hbi_ens_set_state", "hbi_ens_get_state",
            "hbi_ens_step", "hbi_ens_last_ms", "hbi_mlp_create", "hbi_mlp_destroy", "hbi_mlp_nparams",
            "hbi_mlp_set_params", "hbi_mlp_get_params", "hbi_mlp_forward", "hbi_mlp_loss_grad", "hbi_mlp_train",
-           "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace")
+           "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace", "hbi_ens_set_controller",
+           "hbi_ens_set_trace", "hbi_ens_get_trace")
 
 
 def lib():
@@ -128,6 +129,9 @@ def lib():
         L.hbi_rk_set_controller.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.hbi_rk_set_trace.argtypes = [C.c_void_p, C.c_int]
         L.hbi_rk_get_trace.argtypes = [C.c_void_p, _dp, _dp, _ip]
+        L.hbi_ens_set_controller.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.hbi_ens_set_trace.argtypes = [C.c_void_p, C.c_int]
+        L.hbi_ens_get_trace.argtypes = [C.c_void_p, _dp, _dp, _ip]
         _lib = L
     return _lib
 
@@ -451,10 +455,10 @@ class Ensemble:
 
     def step(self, nsteps, history=False):
         """Advance every running realisation nsteps accepted steps.  Returns the
-        lockstep attempts, and with history=True also (nreal, nsteps, 4) rows of
-        (t, dt, errmax, log10 Vmax)."""
+        lockstep attempts, and with history=True also (nreal, nsteps, 5) rows of
+        (t, dt, errmax, rejections before the step, log10 Vmax)."""
         att = C.c_longlong(0)
-        hist = np.zeros((self.nreal, nsteps, 4)) if history else None
+        hist = np.zeros((self.nreal, nsteps, 5)) if history else None
         _check(lib().hbi_ens_step(self.e, int(nsteps), _p(hist), C.byref(att)))
         return (att.value, hist) if history else att.value
 
@@ -462,3 +466,21 @@ class Ensemble:
         ms = C.c_double(0)
         _check(lib().hbi_ens_last_ms(self.e, C.byref(ms)))
         return ms.value
+
+    def set_controller(self, mode, mlp=None, safety=0.9, ratio_lo=0.125, ratio_hi=8.0):
+        """mode 0: rkqs for every realisation; 1: learned proposals from mlp."""
+        _check(lib().hbi_ens_set_controller(self.e, int(mode), mlp.m if mlp is not None else None, float(safety),
+                                            float(ratio_lo), float(ratio_hi)))
+
+    def set_trace(self, cap):
+        _check(lib().hbi_ens_set_trace(self.e, int(cap)))
+        self._trace_cap = int(cap)
+
+    def get_trace(self):
+        """list over realisations of (features (n, 12), raw (n, 2) = (h, errmax))"""
+        cap = max(getattr(self, "_trace_cap", 0), 1)
+        feat = np.zeros((self.nreal, cap, 12))
+        raw = np.zeros((self.nreal, cap, 2))
+        cnt = np.zeros(self.nreal, dtype=np.int32)
+        _check(lib().hbi_ens_get_trace(self.e, _p(feat), _p(raw), _ipp(cnt)))
+        return [(feat[r, :cnt[r]], raw[r, :cnt[r]]) for r in range(self.nreal)]

@@ -230,8 +230,16 @@ int hbi_ens_get_state(hbi_ens *e, double *y, double *slip, double *vel, double *
    HIP graph per lockstep attempt); *attempts = lockstep attempts launched.
    A realisation that underflows (HBI_E_STEP) or leaves [velmin, velmax]
    stops there; the others go on.  hist (may be NULL): nreal x nsteps rows of
-   (t, dt, errmax, log10 Vmax) per accepted step, zero after a stop. */
+   (t, dt, errmax, rejected attempts before the step, log10 Vmax) per accepted
+   step, zero after a stop. */
 int hbi_ens_step(hbi_ens *e, int nsteps, double *hist, long long *attempts);
+/* Step-size control of every realisation, as hbi_rk_set_controller (the
+   features and proposals are per realisation; the MLP is shared). */
+int hbi_ens_set_controller(hbi_ens *e, int mode, const hbi_mlp *m, double safety, double ratio_lo, double ratio_hi);
+/* Per-realisation traces of up to cap accepted steps: feat nreal x cap x 12,
+   raw nreal x cap x 2 (h, errmax), count nreal. */
+int hbi_ens_set_trace(hbi_ens *e, int cap);
+int hbi_ens_get_trace(hbi_ens *e, double *feat, double *raw, int *count);
 int hbi_ens_last_ms(const hbi_ens *e, double *ms);
 
 #ifdef __cplusplus

@@ -85,6 +85,7 @@ def test_gpu_ensemble_matches_single_runs(nreal, prec):
     es2 = ens.get_state()
     print("nreal %d: rejected %s" % (nreal, es2["rejected"][:8]), flush=True)
     assert (es2["accepted"] == n1 + n2).all()
+    assert np.array_equal(eh[:, :, 3].sum(axis=1), es2["rejected"] - es["rejected"])
     for r in sorted({0, nreal // 2, nreal - 1}):
         rk = H.RK(Hg, frs[r], lod, g.ds, eps_r=1e-6, prec=prec)
         rk.set_state(ys[r], 0.0, 1.0)
@@ -99,7 +100,7 @@ def test_gpu_ensemble_matches_single_runs(nreal, prec):
         assert np.abs(es["vel"][r] - vel1).max() <= 1e-8 * np.abs(vel1).max()
         _, sh, _ = rk.step(n2, history=True)
         ts, lvs = sh[:, 0], sh[:, 4]
-        te, lve = eh[r, :, 0], eh[r, :, 3]
+        te, lve = eh[r, :, 0], eh[r, :, 4]
         on_s, on_e = _onset(ts, lvs), _onset(te, lve)
         assert (on_s is None) == (on_e is None)
         t_end = 0.5 * on_s if on_s is not None else min(ts[-1], te[-1])

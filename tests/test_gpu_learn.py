@@ -182,3 +182,60 @@ def test_gpu_learned_step_gain_on_held_out_seeds():
     print("rejections: rkqs %d, learned %d" % tuple(rej), flush=True)
     assert np.mean(ratios) >= 1.05 and min(ratios) >= 1.0
     assert rej[1] <= 0.6 * rej[0]
+
+
+def test_gpu_ensemble_traces_and_learned_steps_match_single_runs():
+    """Features recorded by the ensemble graph equal those of single runs of
+    the same realisations while the step grids coincide (first 10 steps), and
+    the learned controller takes the same steps in both."""
+    g = P.mesh_2dvs(1024, 40.0 / 1024)
+    Hm = H.HMatrix(g, leaf=32, eps=1e-6)
+    lod = Hm.lod
+    frs = [L.c1_realisation(g, s) for s in (11, 12, 13)]
+    ys = np.array([L.initial_state(g, f, lod, s) for f, s in zip(frs, (11, 12, 13))])
+    from hbi_amd import train_step_mlp as TS
+    m = L.load(TS.DEFAULT_OUT)
+    w = 24.0 / 1024
+    for mlp in (None, m):
+        et = L.ensemble_trace_run(Hm, g, frs, lod, ys, 10, mlp=mlp)
+        for r, f in enumerate(frs):
+            st = L.trace_run(Hm, g, f, lod, ys[r], 10, mlp=mlp)
+            fe, re_ = et[r]
+            fs, rs = st
+            assert fe.shape == fs.shape == (10, 12)
+            assert np.allclose(re_[:, 0], rs[:, 0], rtol=1e-9), (mlp is None, r)
+            assert np.abs(fe[:, :6] - fs[:, :6]).max() <= 1.01 * w
+            assert np.abs(fe[:, 6:] - fs[:, 6:]).max() <= 1e-5
+
+
+def test_gpu_learned_step_gain_on_the_3d_fault_ensemble():
+    """The committed 3-D model (python -m hbi_amd.train_step_mlp --problem c2:
+    64 realisations x 1500 lockstep steps, 3 rounds) on 64 realisations it
+    was not trained on (seeds 101-164), mixed precision, in one ensemble: RK
+    attempts of each realisation to reach the simulated time of 1500 rkqs
+    steps, and the lockstep attempts until every realisation got there."""
+    from hbi_amd import train_step_mlp as TS
+    m = L.load(TS.C2_OUT)
+    g, Hm, lod, frs, ys = TS.c2_setup(64, first_seed=101)
+    K = 1500
+    E = H.Ensemble(Hm, frs, lod, hdiv=TS.C2["ds0"], eps_r=1e-6, prec=H.PREC_MIXED)
+    E.set_state(ys, 0.0, 1.0)
+    _, h0 = E.step(K, history=True)
+    E.close()
+    t_end = h0[:, -1, 0]
+    a0 = np.array([L.attempts_to_time(h0[r], t_end[r]) for r in range(64)])
+    E = H.Ensemble(Hm, frs, lod, hdiv=TS.C2["ds0"], eps_r=1e-6, prec=H.PREC_MIXED)
+    E.set_controller(1, m, 0.9)
+    E.set_state(ys, 0.0, 1.0)
+    hs = []
+    while not hs or (np.any(np.concatenate(hs, axis=1)[:, -1, 0] < t_end) and len(hs) < 8):
+        hs.append(E.step(500, history=True)[1])
+    E.close()
+    hl = np.concatenate(hs, axis=1)
+    assert (hl[:, -1, 0] >= t_end).all()
+    a1 = np.array([L.attempts_to_time(hl[r], t_end[r]) for r in range(64)])
+    ratio = a0 / a1
+    print("per-realisation attempt ratio: mean %.3f min %.3f max %.3f; lockstep attempts %d vs %d"
+          % (ratio.mean(), ratio.min(), ratio.max(), a1.max(), a0.max()), flush=True)
+    assert ratio.mean() >= 1.05 and ratio.min() >= 1.0
+    assert a1.max() < a0.max()
