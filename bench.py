@@ -10,10 +10,14 @@ device.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Headline (``value``): the batched form -- 64 realisations of the fault's
-friction (seeds 1..64) advanced in lockstep with one 64-column H product per
-RK stage, FP32 factors (mixed precision, FP64 state) -- as accepted steps/s
-summed over the realisations, over K lockstep steps after W warm-up steps.
+Headline (``value``): the mixed-precision, batched, learned-step form -- 64
+realisations of the fault's friction (seeds 1..64) advanced in lockstep with
+one 64-column H product per RK stage, FP32 factors (FP64 state), each
+realisation's next step proposed by the step-size MLP trained on this fault
+(hbi_amd/data/step_mlp_c2.npz; the embedded error check judges every step)
+-- as accepted steps/s summed over the realisations, over K lockstep steps
+after W warm-up steps.  The same ensemble under the reference's controller
+(rkqs) is measured beside it.
 Alongside: single-run FP64 (the plain GPU form) and mixed, the single-vector
 H-matvec bandwidth of every precision, the 64-column product on the matrix
 cores, trajectory agreement mixed vs FP64, the roofline of the dominant
@@ -117,27 +121,48 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
         "max_abs_dlnV_to_half_onset": dlnv,
         "note": "slip-rate histories at %d elements compared on a common time grid (step grids differ)" % len(rec)}
 
-    # batched form: 64 realisations in lockstep, mixed precision
-    print("bench: ensemble of %d realisations, %d + %d steps" % (NREAL, warmup, steps), file=sys.stderr, flush=True)
+    # batched form: 64 realisations in lockstep, mixed precision, rkqs and learned step size
+    from hbi_amd import learned as L
+    mlp = L.load(os.path.join(ROOT, "hbi_amd", "data", "step_mlp_c2.npz"))
     frs = [P.friction_c2(g, IMAX, JMAX, DS0, seed=1 + r) for r in range(NREAL)]
     ys = []
     for f in frs:
         y = P.init_state(f, lod)
         P.perturb_state(y, g, lod, amp=0.02, depth=15.0, width=3.0)
         ys.append(y)
-    E = H.Ensemble(Hm, frs, lod, hdiv=DS0, eps_r=EPS_R, prec=H.PREC_MIXED)
-    E.set_state(np.array(ys), 0.0, 1.0)
-    if warmup:
-        E.step(warmup)
-    t0 = time.perf_counter()
-    att = E.step(steps)
-    wall = time.perf_counter() - t0
-    es = E.get_state()
-    out["ensemble"] = {"realisations": NREAL, "steps": steps, "wall_s": wall, "device_ms": E.last_ms(),
-                       "lockstep_attempts": att, "ms_per_attempt": 1e3 * wall / max(att, 1),
-                       "steps_per_s": NREAL * steps / wall, "max_rejected": int(es["rejected"].max()),
-                       "status_ok": bool((es["status"] == 0).all())}
-    E.close()
+    ys = np.array(ys)
+    hists = {}
+    for name, learned in (("ensemble_rkqs", False), ("ensemble", True)):
+        print("bench: ensemble of %d realisations (%s), %d + %d steps" % (NREAL, name, warmup, steps),
+              file=sys.stderr, flush=True)
+        E = H.Ensemble(Hm, frs, lod, hdiv=DS0, eps_r=EPS_R, prec=H.PREC_MIXED)
+        if learned:
+            E.set_controller(1, mlp, 0.9)
+        E.set_state(ys, 0.0, 1.0)
+        hw = E.step(warmup, history=True)[1] if warmup else np.zeros((NREAL, 0, 5))
+        t0 = time.perf_counter()
+        att, ht = E.step(steps, history=True)
+        wall = time.perf_counter() - t0
+        es = E.get_state()
+        hists[name] = np.concatenate([hw, ht], axis=1)
+        out[name] = {"realisations": NREAL, "learned_step": learned, "steps": steps, "wall_s": wall,
+                     "device_ms": E.last_ms(), "lockstep_attempts": att, "ms_per_attempt": 1e3 * wall / max(att, 1),
+                     "steps_per_s": NREAL * steps / wall, "max_rejected": int(es["rejected"].max()),
+                     "status_ok": bool((es["status"] == 0).all())}
+        E.close()
+    # the two controllers from the same start to a common simulated time per realisation
+    h0, h1 = hists["ensemble_rkqs"], hists["ensemble"]
+    common = np.minimum(h0[:, -1, 0], h1[:, -1, 0])
+    a0 = np.array([L.attempts_to_time(h0[r], common[r]) for r in range(NREAL)])
+    a1 = np.array([L.attempts_to_time(h1[r], common[r]) for r in range(NREAL)])
+    w0 = a0.max() * out["ensemble_rkqs"]["ms_per_attempt"]
+    w1 = a1.max() * out["ensemble"]["ms_per_attempt"]
+    out["learned_vs_rkqs"] = {"attempt_ratio_median": float(np.median(a0 / a1)),
+                              "attempt_ratio_min": float((a0 / a1).min()),
+                              "lockstep_attempts": [int(a0.max()), int(a1.max())],
+                              "time_to_common_sim_time_ms": [w0, w1], "speedup": w0 / w1,
+                              "note": "attempts (accepted + rejected) of each realisation from t = 0 to the smaller "
+                                      "of the two controllers' final times; lockstep wall = max attempts x ms/attempt"}
     out["p2_entries"], out["p1_entries"] = p2_entries, p1_entries
     Hm.close()
     return out
@@ -178,7 +203,7 @@ def cpu_part(nsteps: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300, help="timed lockstep steps of the ensemble")
+    ap.add_argument("--steps", type=int, default=1500, help="timed lockstep steps of the ensemble")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--single-steps", type=int, default=2000)
     ap.add_argument("--cpu-steps", type=int, default=8)
@@ -190,6 +215,7 @@ def main():
         sys.exit(2)
     r = gpu_part(args.steps, args.warmup, args.single_steps)
     ens = r["ensemble"]
+    ens0 = r["ensemble_rkqs"]
     mmx = r["matmul64"]["mixed"]
     mvx = r["matvec"]["mixed"]
     f64 = r["single"]["f64"]["steps_per_s"]
@@ -208,7 +234,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "3dph 256x128 planar fault, N=32768, leaf 64, ACA eps 1e-5, RK45 eps_r 1e-6; "
                                "64 friction realisations in lockstep, one 64-column H product per RK stage, "
-                               "FP32 factors, hipGraph attempts",
+                               "FP32 factors, learned step-size proposals (rkqs check), hipGraph attempts",
                    "n_elements": IMAX * JMAX, "realisations": NREAL},
         "roofline": {"bound": "mfma", "kernel": "mr_tile_kernel<float,64> (64-RHS row tiles, FP64 MFMA)",
                      "achieved": round(mmx["p2_TFs"], 2), "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
@@ -219,6 +245,9 @@ def main():
         "fp64_single_steps_per_s": round(f64, 2),
         "mixed_single_steps_per_s": round(r["single"]["mixed"]["steps_per_s"], 2),
         "speedup_vs_fp64_single": round(ens["steps_per_s"] / f64, 2),
+        "batched_rkqs_steps_per_s": round(ens0["steps_per_s"], 1),
+        "learned_vs_rkqs": r["learned_vs_rkqs"],
+        "ensemble_rkqs": ens0,
         "accuracy_mixed_vs_f64": r["accuracy_mixed_vs_f64"],
         "ensemble": ens,
         "matvec": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}

@@ -241,28 +241,51 @@ __global__ __launch_bounds__(kDecThreads) void ens_decide_kernel(EnsPar P, EnsCt
     }
 }
 
-__global__ void ens_commit_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A)
+// Each wave commits kCommitRun consecutive elements for its 64 realisations
+// (lane = realisation); the slip-rate histogram of the learned step size is
+// filled with run-length aggregated atomics (neighbouring elements mostly
+// share a bin).
+constexpr int kCommitRun = 16, kCommitPerBlock = 4 * kCommitRun;
+
+__global__ __launch_bounds__(256) void ens_commit_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A)
 {
 #pragma clang fp contract(off)
     __shared__ double vm[4][kR];
     if (c->done) return;
-    const int r = threadIdx.x % kR, li = threadIdx.x / kR, i = blockIdx.x * 4 + li;
-    double v = 0.0;
-    if (i < P.n && r < P.nreal && c->acc[r]) {
-        const size_t q = (size_t)i * kR + r, e0 = (3 * (size_t)i) * kR + r;
-        for (int cc = 0; cc < 3; cc++) A.y[e0 + cc * kR] = A.yout[e0 + cc * kR];
+    const int r = threadIdx.x % kR, w = threadIdx.x / kR;
+    const int i0 = blockIdx.x * kCommitPerBlock + w * kCommitRun;
+    double vmax = 0.0;
+    if (r < P.nreal && c->acc[r]) {
         const double dt = c->hdid[r];
-        double sl = A.slip[q];
-        sl = sl + A.velc[q] * dt * 0.5;
-        v = slip_rate(A.y[e0], A.y[e0 + kR], A.y[e0 + 2 * kR], A.a[q], P.vref);
-        sl = sl + v * dt * 0.5;
-        A.slip[q] = sl;
-        A.velc[q] = v;
-        if (P.learn) atomicAdd(&A.bins[(size_t)r * kBins + vbin(fabs(v))], 1);
+        int cur_b = -1, cur_n = 0;
+        for (int k = 0; k < kCommitRun; k++) {
+            const int i = i0 + k;
+            if (i >= P.n) break;
+            const size_t q = (size_t)i * kR + r, e0 = (3 * (size_t)i) * kR + r;
+            for (int cc = 0; cc < 3; cc++) A.y[e0 + cc * kR] = A.yout[e0 + cc * kR];
+            double sl = A.slip[q];
+            sl = sl + A.velc[q] * dt * 0.5;
+            const double v = slip_rate(A.y[e0], A.y[e0 + kR], A.y[e0 + 2 * kR], A.a[q], P.vref);
+            sl = sl + v * dt * 0.5;
+            A.slip[q] = sl;
+            A.velc[q] = v;
+            vmax = fmax(vmax, fabs(v));
+            if (P.learn) {
+                const int bb = vbin(fabs(v));
+                if (bb == cur_b) {
+                    cur_n++;
+                } else {
+                    if (cur_n) atomicAdd(&A.bins[(size_t)r * kBins + cur_b], cur_n);
+                    cur_b = bb;
+                    cur_n = 1;
+                }
+            }
+        }
+        if (cur_n) atomicAdd(&A.bins[(size_t)r * kBins + cur_b], cur_n);
     }
-    vm[li][r] = fabs(v);
+    vm[w][r] = vmax;
     __syncthreads();
-    if (li == 0) A.monpart[(size_t)blockIdx.x * kR + r] = fmax(fmax(vm[0][r], vm[1][r]), fmax(vm[2][r], vm[3][r]));
+    if (w == 0) A.monpart[(size_t)blockIdx.x * kR + r] = fmax(fmax(vm[0][r], vm[1][r]), fmax(vm[2][r], vm[3][r]));
 }
 
 // history row per accepted step of realisation r: (t, dt, errmax, rejections before it, log10 Vmax)
@@ -307,13 +330,13 @@ __global__ __launch_bounds__(kDecThreads) void ens_monitor_kernel(EnsPar P, EnsC
 // the judge; a rejection shrinks h by the rkqs rule).  Clears its histogram.
 __global__ __launch_bounds__(kHid) void ens_learn_kernel(EnsPar P, EnsCtl *c, EnsArrays A)
 {
-    __shared__ int part[kHid];
+    __shared__ int part[kHid + 1], hb[kBins];
     __shared__ float x[kIn], h1[kHid], red[kHid];
     __shared__ double q[3];
     const int r = blockIdx.x, j = threadIdx.x;
     if (c->done || !P.learn || !c->acc[r]) return;
     int *bins = A.bins + (size_t)r * kBins;
-    step_features(bins, P.n, c->hdid[r], c->errmax[r], c->nrjct[r] > 0, c->mem[r], part, x, q);
+    step_features(bins, P.n, c->hdid[r], c->errmax[r], c->nrjct[r] > 0, c->mem[r], part, hb, x, q);
     if (j == 0 && A.trace && c->trace_n[r] < A.trace_cap) {
         double *row = A.trace + 14 * ((size_t)r * A.trace_cap + c->trace_n[r]);
         for (int k = 0; k < kIn; k++) row[k] = (double)x[k];
@@ -401,9 +424,10 @@ static void e_attempt(hbi_ens *E)
     }
     ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->errpart.p, g, E->fold.p, 0);
     ens_decide_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold);
-    ens_commit_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
+    const int gc = (E->n + kCommitPerBlock - 1) / kCommitPerBlock;
+    ens_commit_kernel<<<gc, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
     if (E->P.learn) ens_learn_kernel<<<E->nreal, kHid, 0, E->stream>>>(E->P, E->ctl.p, A);
-    ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->monpart.p, g, E->fold.p, 1);
+    ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->monpart.p, gc, E->fold.p, 1);
     ens_monitor_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold, E->hist.p);
 }
 

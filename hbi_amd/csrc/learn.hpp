@@ -67,35 +67,42 @@ struct StepMem {
 // q (max, median, p99 of log10 |V|, bin centres) and the 12 features x:
 // q, their changes since the previous step, log10 h, log10 errmax, the
 // rejection flag, log10 h and errmax of the previous step, and the
-// error-optimal step log10(h errmax^(-1/5)).  part: kHid ints of LDS.
+// error-optimal step log10(h errmax^(-1/5)).  part: kHid + 1 ints of LDS,
+// hb: kBins ints of LDS.
 __device__ __forceinline__ void step_features(const int *bins, int n, double hdid, double errmax, bool rejected,
-                                              const StepMem &m, int *part, float *x, double *q)
+                                              const StepMem &m, int *part, int *hb, float *x, double *q)
 {
     const int j = threadIdx.x;
     constexpr int per = kBins / kHid;
     int cnt = 0;
-    for (int b = 0; b < per; b++) cnt += bins[j * per + b];
-    part[j] = cnt;
+    for (int b = 0; b < per; b++) {
+        const int k = bins[b * kHid + j];      // coalesced; hb in bin order
+        hb[b * kHid + j] = k;
+    }
+    __syncthreads();
+    for (int b = 0; b < per; b++) cnt += hb[j * per + b];
+    part[j + 1] = cnt;
     __syncthreads();
     if (j == 0) {
+        part[0] = 0;
+        for (int t = 0; t < kHid; t++) part[t + 1] += part[t];   // part[t] = elements in chunks < t
         const double w = (kBinHi - kBinLo) / kBins;
-        const long long n50 = (n + 1) / 2, n99 = (long long)ceil(0.99 * n);
-        long long cum = 0;
-        int b50 = -1, b99 = -1, bmax = 0;
-        for (int t = 0; t < kHid; t++) {
-            if (part[t] == 0) continue;
+        // smallest bin whose cumulative count reaches target (1-based rank)
+        auto rank_bin = [&](long long target) {
+            int t = 0;
+            while (t < kHid - 1 && part[t + 1] < target) t++;
+            long long cum = part[t];
             for (int b = t * per; b < (t + 1) * per; b++) {
-                const int k = bins[b];
-                if (k == 0) continue;
-                cum += k;
-                if (b50 < 0 && cum >= n50) b50 = b;
-                if (b99 < 0 && cum >= n99) b99 = b;
-                bmax = b;
+                cum += hb[b];
+                if (cum >= target) return b;
             }
-        }
+            return (t + 1) * per - 1;
+        };
+        const long long n50 = (n + 1) / 2, n99 = (long long)ceil(0.99 * n);
+        const int b50 = rank_bin(n50), b99 = rank_bin(n99), bmax = rank_bin(n);
         q[0] = kBinLo + (bmax + 0.5) * w;
-        q[1] = kBinLo + (max(b50, 0) + 0.5) * w;
-        q[2] = kBinLo + (max(b99, 0) + 0.5) * w;
+        q[1] = kBinLo + (b50 + 0.5) * w;
+        q[2] = kBinLo + (b99 + 0.5) * w;
         const double lh = log10(hdid), le = log10(errmax);
         const double ph = m.have_prev ? log10(m.prev_h) : lh, pe = m.have_prev ? log10(m.prev_e) : le;
         for (int k = 0; k < 3; k++) {

@@ -266,12 +266,12 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
 // Runs on one workgroup of kHid threads; clears the histogram.
 __global__ __launch_bounds__(kHid) void rk_learn_kernel(RkPar P, RkCtl *c, RkArrays A)
 {
-    __shared__ int part[kHid];
+    __shared__ int part[kHid + 1], hb[kBins];
     __shared__ float x[kIn], h1[kHid], red[kHid];
     __shared__ double q[3];
     if (c->done || !c->acc || !P.learn) return;
     const int j = threadIdx.x;
-    step_features(A.bins, P.n, c->hdid, c->errmax, c->nrjct > 0, c->mem, part, x, q);
+    step_features(A.bins, P.n, c->hdid, c->errmax, c->nrjct > 0, c->mem, part, hb, x, q);
     if (j == 0 && A.trace && c->trace_n < A.trace_cap) {
         double *row = A.trace + 14 * (size_t)c->trace_n;
         for (int k = 0; k < kIn; k++) row[k] = (double)x[k];
