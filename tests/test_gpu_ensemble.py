@@ -30,9 +30,10 @@ def test_gpu_matmul_matches_matvec_columns(label, make, leaf, eps):
     X = rng.standard_normal((g.n, 70))      # one full 64-column pass and a partial one
     for prec in (H.PREC_F64, H.PREC_F32, H.PREC_MIXED):
         Y = Hg.matmul(X, prec)
+        tol = 1e-12
         for c in range(X.shape[1]):
             y = Hg.matvec(X[:, c], prec)
-            assert np.abs(Y[:, c] - y).max() <= 1e-12 * np.abs(y).max(), (prec, c)
+            assert np.abs(Y[:, c] - y).max() <= tol * np.abs(y).max(), (prec, c)
     if label == "3dph-highrank":
         assert st["kt_max"] > 64      # exercises the split of ranks above 64
 
@@ -49,6 +50,15 @@ def _c1_realisations(n, nreal):
     return g, frs
 
 
+def _crossings(t, lv, levels=(-7.5, -7.0, -6.0, -5.0, -4.0, -3.0, -2.0, -1.0)):
+    out = {}
+    for L in levels:
+        k = int(np.argmax(lv > L))
+        if lv[k] > L and k > 0:
+            out[L] = t[k - 1] + (L - lv[k - 1]) / (lv[k] - lv[k - 1]) * (t[k] - t[k - 1])
+    return out
+
+
 def _onset(t, lv, thresh=-3.0):
     k = int(np.argmax(lv > thresh))
     return t[k] if lv[k] > thresh else None
@@ -62,9 +72,10 @@ def test_gpu_ensemble_matches_single_runs(nreal, prec):
     Once steps start being rejected, the error estimate of this problem moves
     by ~1% under 1e-10 relative changes of t (the single-run GPU integrator
     and the oracle part there too), so step grids are compared only before;
-    after that, ln Vmax interpolated between two different step grids differs
-    by the interpolation error (up to ~4e-4 here), checked only as a guard
-    against gross errors (2e-3)."""
+    after that, the times at which log10 Vmax crosses levels -7.5 .. -1
+    (interpolated within each history, so independent of the other grid)
+    agree to 2e-4 as a guard against gross errors (the coarse early steps,
+    ~2e5 s, limit the interpolation to ~5e-5)."""
     g, frs = _c1_realisations(1024, nreal)
     Hg = H.HMatrix(g, leaf=32, eps=1e-6)
     lod = Hg.lod
@@ -103,19 +114,16 @@ def test_gpu_ensemble_matches_single_runs(nreal, prec):
         te, lve = eh[r, :, 0], eh[r, :, 4]
         on_s, on_e = _onset(ts, lvs), _onset(te, lve)
         assert (on_s is None) == (on_e is None)
-        t_end = 0.5 * on_s if on_s is not None else min(ts[-1], te[-1])
-        tg = np.linspace(max(ts[0], te[0]), t_end, 2000)
-        dl = np.log(10.0) * np.abs(np.interp(tg, ts, lvs) - np.interp(tg, te, lve)).max()
-        if on_s is not None:
-            tg9 = np.linspace(max(ts[0], te[0]), 0.9 * on_s, 2000)
-            dl9 = np.log(10.0) * np.abs(np.interp(tg9, ts, lvs) - np.interp(tg9, te, lve)).max()
-        else:
-            dl9 = float("nan")
-        print("  r %d: onset %s vs %s, max |d ln Vmax| to 0.5 onset %.2e, to 0.9 onset %.2e"
-              % (r, on_e, on_s, dl, dl9), flush=True)
+        # times at which log10 Vmax first crosses a set of levels (each
+        # interpolated within its own history): grid-independent
+        cs, ce = _crossings(ts, lvs), _crossings(te, lve)
+        both = [k for k in cs if k in ce]
+        rel = max(abs(ce[k] - cs[k]) / cs[k] for k in both)
+        print("  r %d: onset %s vs %s, %d level crossings, max relative time difference %.2e"
+              % (r, on_e, on_s, len(both), rel), flush=True)
         if on_s is not None:
             assert abs(on_e - on_s) <= 1e-5 * on_s
-        assert dl <= 2e-3
+        assert len(both) >= 3 and rel <= 2e-4
 
 
 def test_gpu_ensemble_stop_is_per_realisation():
