@@ -57,6 +57,12 @@ struct RkArrays {
 
 
 constexpr int kRkThreads = 256;
+// per-block monitor partials: max |V|, sum |slip| area, sum (tau / sigma) area, max sigma, min sigma
+constexpr int kMon = 5;
+// history row per accepted step (the quantities of main_LH.f90:1491):
+// t, dt, errmax, rejected attempts before the step, log10 max |V|, mean |slip|,
+// mean tau / sigma (area weighted), max sigma, min sigma
+constexpr int kRkHist = 9;
 
 
 __global__ void rk_arm_kernel(RkCtl *c, long long nsteps, double h_override, int set_h)
@@ -225,11 +231,12 @@ __global__ void rk_decide_kernel(RkPar P, RkCtl *c, const double *__restrict__ e
 __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const RkCtl *__restrict__ c, RkArrays A)
 {
 #pragma clang fp contract(off)
-    __shared__ double sv[kRkThreads / kWave], sm[kRkThreads / kWave];
+    __shared__ double sp[kRkThreads / kWave][kMon];
     if (c->done || !c->acc) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const double dt = c->hdid;
-    double vmax = 0.0, ms = 0.0;
+    // max |V|, sum |slip| area, sum mu area, max sigma, min sigma (identities for i >= n)
+    double m[kMon] = {0.0, 0.0, 0.0, -HUGE_VAL, HUGE_VAL};
     if (i < P.n) {
         for (int q = 0; q < 3; q++) A.y[3 * (size_t)i + q] = A.yout[3 * (size_t)i + q];
         double sl = A.slip[i];
@@ -238,24 +245,42 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
         sl = sl + v * dt * 0.5;
         A.slip[i] = sl;
         A.velc[i] = v;
-        vmax = fabs(v);
-        ms = fabs(sl) * A.area[i];
-        if (P.learn) atomicAdd(&A.bins[vbin(vmax)], 1);
+        const double sig = A.y[3 * i + 2];
+        m[0] = fabs(v);
+        m[1] = fabs(sl) * A.area[i];
+        m[2] = A.y[3 * i + 1] / sig * A.area[i];
+        m[3] = sig;
+        m[4] = sig;
+        if (P.learn) atomicAdd(&A.bins[vbin(m[0])], 1);
     }
-    // per-block max and (ordered) sum
+    // per-block combine in a fixed order (sums ordered by lane, then by wave)
     for (int o = 1; o < kWave; o <<= 1) {
-        const double v2 = __shfl_down(vmax, o, kWave), m2 = __shfl_down(ms, o, kWave);
+        double o2[kMon];
+        for (int k = 0; k < kMon; k++) o2[k] = __shfl_down(m[k], o, kWave);
         const int lane = threadIdx.x % kWave;
-        if ((lane & (2 * o - 1)) == 0 && lane + o < kWave) { vmax = fmax(vmax, v2); ms = ms + m2; }
+        if ((lane & (2 * o - 1)) == 0 && lane + o < kWave) {
+            m[0] = fmax(m[0], o2[0]);
+            m[1] = m[1] + o2[1];
+            m[2] = m[2] + o2[2];
+            m[3] = fmax(m[3], o2[3]);
+            m[4] = fmin(m[4], o2[4]);
+        }
     }
     const int w = threadIdx.x / kWave;
-    if (threadIdx.x % kWave == 0) { sv[w] = vmax; sm[w] = ms; }
+    if (threadIdx.x % kWave == 0)
+        for (int k = 0; k < kMon; k++) sp[w][k] = m[k];
     __syncthreads();
     if (threadIdx.x == 0) {
-        double a = sv[0], b = sm[0];
-        for (int q = 1; q < kRkThreads / kWave; q++) { a = fmax(a, sv[q]); b = b + sm[q]; }
-        A.monpart[2 * blockIdx.x] = a;
-        A.monpart[2 * blockIdx.x + 1] = b;
+        double t[kMon];
+        for (int k = 0; k < kMon; k++) t[k] = sp[0][k];
+        for (int q = 1; q < kRkThreads / kWave; q++) {
+            t[0] = fmax(t[0], sp[q][0]);
+            t[1] = t[1] + sp[q][1];
+            t[2] = t[2] + sp[q][2];
+            t[3] = fmax(t[3], sp[q][3]);
+            t[4] = fmin(t[4], sp[q][4]);
+        }
+        for (int k = 0; k < kMon; k++) A.monpart[kMon * (size_t)blockIdx.x + k] = t[k];
     }
 }
 
@@ -296,12 +321,22 @@ __global__ void rk_monitor_kernel(RkPar P, RkCtl *c, RkArrays A, int nblk)
 {
 #pragma clang fp contract(off)
     if (c->done || !c->acc) return;
-    __shared__ double vm, msum;
+    __shared__ double vm, msum, mmu, smax, smin;
     if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (int q = 0; q < nblk; q++) { a = fmax(a, A.monpart[2 * q]); b = b + A.monpart[2 * q + 1]; }
+        double a = 0.0, b = 0.0, u = 0.0, hi = -HUGE_VAL, lo = HUGE_VAL;
+        for (int q = 0; q < nblk; q++) {
+            const double *mp = A.monpart + kMon * (size_t)q;
+            a = fmax(a, mp[0]);
+            b = b + mp[1];
+            u = u + mp[2];
+            hi = fmax(hi, mp[3]);
+            lo = fmin(lo, mp[4]);
+        }
         vm = a;
         msum = b / P.sg;
+        mmu = u / P.sg;
+        smax = hi;
+        smin = lo;
         c->vmax = a;
         c->meanslip = msum;
     }
@@ -309,9 +344,9 @@ __global__ void rk_monitor_kernel(RkPar P, RkCtl *c, RkArrays A, int nblk)
     const long long row = c->accepted - c->hist_base - 1;
     if (row >= 0 && row < c->hist_cap) {
         if (A.hist && threadIdx.x == 0) {
-            double *hr = A.hist + 6 * row;
+            double *hr = A.hist + kRkHist * row;
             hr[0] = c->t; hr[1] = c->hdid; hr[2] = c->errmax; hr[3] = (double)c->nrjct; hr[4] = log10(vm);
-            hr[5] = msum;
+            hr[5] = msum; hr[6] = mmu; hr[7] = smax; hr[8] = smin;
         }
         if (A.vhist)
             for (int r = threadIdx.x; r < A.nrec; r += blockDim.x) A.vhist[row * A.nrec + r] = A.velc[A.rec[r]];
@@ -473,7 +508,7 @@ int hbi_rk_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_rk_opts *opts, const dou
                 r->ypart_n.alloc((size_t)std::max(Pn.nparts, 1) * hn->tr);
             }
             r->errpart.alloc(2 * (size_t)r->nblk);
-            r->monpart.alloc(2 * (size_t)r->nblk);
+            r->monpart.alloc(hbi::kMon * (size_t)r->nblk);
             r->ctl.alloc(1);
             r->bins.alloc(hbi::kBins);
             HBI_HIP(hipMemsetAsync(r->bins.p, 0, hbi::kBins * sizeof(int), r->stream));
@@ -586,7 +621,7 @@ int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, d
         HBI_REQUIRE(nrec == 0 || (rec && vhist), "rec and vhist required when nrec > 0");
         for (int q = 0; q < nrec; q++) HBI_REQUIRE(rec[q] >= 0 && rec[q] < r->n, "record index out of range");
         // history buffers (the graph records their addresses)
-        const size_t hneed = hist ? 6 * (size_t)nsteps : 0;
+        const size_t hneed = hist ? hbi::kRkHist * (size_t)nsteps : 0;
         const size_t vneed = nrec > 0 ? (size_t)nsteps * nrec : 0;
         bool recapture = !r->graph;
         if (hneed > r->hist.n || (hneed == 0 && r->hist.n)) {
@@ -624,7 +659,8 @@ int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, d
         HBI_HIP(hipEventElapsedTime(&ms, r->e0, r->e1));
         r->last_ms = ms;
         const long long done = c.accepted - c.hist_base;
-        if (hist && done > 0) r->hist.download(hist, 6 * (size_t)std::min<long long>(done, nsteps), r->stream);
+        if (hist && done > 0)
+            r->hist.download(hist, hbi::kRkHist * (size_t)std::min<long long>(done, nsteps), r->stream);
         if (nrec && done > 0)
             r->vhist.download(vhist, (size_t)std::min<long long>(done, nsteps) * nrec, r->stream);
         HBI_HIP(hipStreamSynchronize(r->stream));

@@ -37,6 +37,8 @@ program lhbiem_gpu
   type(c_ptr) :: rk
   integer(c_int) :: rc, nrec
   integer :: lrtrn, i_, kdone
+  integer(8) :: c0, c1, crate
+  real(8) :: mu_avg
 
   call get_command_argument(1, input_file)
   open(33, file=input_file, status='old', iostat=ios)
@@ -268,17 +270,28 @@ program lhbiem_gpu
   sg = sum(area)
   write(fname, '("output/monitor",i0,".dat")') number
   open(52, file=fname)
-  allocate(hist(6 * outbatch), vhist(1))
+  allocate(hist(9 * outbatch), vhist(1))
   nrec = 0
   nrec_dummy = 0
   kdone = 0
-  call cpu_time(t1)
+  call system_clock(c0, crate)
+  ! initial row (main_LH.f90:1175-1181): k = 0, errmax = dt = 0
+  mu_avg = 0d0
+  do i = 1, ncellg
+    mu_avg = mu_avg + y(3 * i - 1) / y(3 * i) * area(i)
+  end do
+  write(52, '(i7,f19.4,7e16.5,i4,f16.4)') 0, 0d0, log10(maxval(abs(vel))), 0d0, mu_avg / sg, &
+    maxval(y(3::3)), minval(y(3::3)), 0d0, 0d0, 0, 0d0
   do while (kdone < nstep)
     nb = min(outbatch, nstep - kdone)
     rc = hbi_rk_step(rk, int(nb, c_int), hist, nrec, nrec_dummy, vhist, info)
+    call system_clock(c1)
+    ! rows as output_monitor (main_LH.f90:1489-1493); the wall time is the
+    ! elapsed time at the end of the device batch the step belongs to
     do k = 1, int(info%accepted) - kdone
-      write(52, '(i7,f19.4,3e16.5,2e16.5,i4)') kdone + k, hist(6 * k - 5), hist(6 * k - 1), hist(6 * k), &
-        hist(6 * k - 3), hist(6 * k - 4), int(hist(6 * k - 2))
+      write(52, '(i7,f19.4,7e16.5,i4,f16.4)') kdone + k, hist(9 * k - 8), hist(9 * k - 4), hist(9 * k - 3), &
+        hist(9 * k - 2), hist(9 * k - 1), hist(9 * k), hist(9 * k - 6), hist(9 * k - 7), int(hist(9 * k - 5)), &
+        real(c1 - c0, 8) / real(crate, 8)
     end do
     kdone = int(info%accepted)
     if (rc == HBI_E_STEP) then

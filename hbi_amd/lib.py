@@ -308,7 +308,7 @@ class RK:
         return yo, ye
 
     def step(self, nsteps, history=False, rec=None):
-        hist = np.zeros((nsteps, 6)) if history else None
+        hist = np.zeros((nsteps, 9)) if history else None
         rec = np.ascontiguousarray(rec, dtype=np.int32) if rec is not None and len(rec) else None
         nrec = 0 if rec is None else len(rec)
         vh = np.zeros((nsteps, nrec)) if nrec else None

@@ -166,8 +166,11 @@ int hbi_rk_derivs(hbi_rk *r, const double *y, double *dydx);
 int hbi_rk_rkck(hbi_rk *r, const double *y, double h, double *yout, double *yerr);
 /* Advance nsteps accepted steps (rkqs + post-step update) through a captured
    HIP graph of one attempt; no host synchronisation inside.  When hist is
-   not NULL it receives nsteps rows of (t, dt, errmax, nrjct, log10 Vmax,
-   mean |slip|), and vhist nsteps x nrec slip rates at cluster indices rec. */
+   not NULL it receives nsteps rows of 9: t, dt, errmax, rejected attempts
+   before the step, log10 max|V|, mean |slip|, mean tau/sigma (both area
+   weighted), max sigma, min sigma (the monitor quantities of
+   main_LH.f90:1289-1336, 1491), and vhist nsteps x nrec slip rates at
+   cluster indices rec. */
 int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, double *vhist,
                 hbi_rk_info *info);
 /* Device time of the last hbi_rk_step call, milliseconds (HIP events) */
