@@ -234,3 +234,47 @@ def test_gpu_c2_full_size_structure_and_linearity():
     for r in rows:
         kr = H.entries(g, np.full(g.n, lod[r], dtype=np.int32), lod.astype(np.int32))
         assert abs(y[r] - kr @ x) <= 1e-4 * np.abs(kr).sum() * np.abs(x).max() / math.sqrt(g.n) + 1e-9 * abs(kr @ x)
+
+
+def test_gpu_normal_stress_matrix_against_oracle():
+    """sigmaconst F: a second H-matrix of normal tractions drives dsigma/dt
+    (main_LH.f90:2100-2125, 2165-2180 with limitsigma).  60-degree dipping
+    thrust fault (normal tractions change with slip), 48 x 24 elements:
+    both matrices against the oracle's, derivs, an RK step, and the first
+    steps of a run (identical step grids) against the oracle."""
+    import dataclasses
+    imax, jmax, ds0 = 48, 24, 1.0
+    gs = P.mesh_3dph(imax, jmax, ds0, dipangle=60.0, crake=90.0)
+    gn = dataclasses.replace(gs, vcomp=1)
+    Hs, Hn = H.HMatrix(gs, leaf=32, eps=1e-6), H.HMatrix(gn, leaf=32, eps=1e-6)
+    Os, On = O.HMatrix(gs, leaf=32, eps=1e-6, nthreads=8), O.HMatrix(gn, leaf=32, eps=1e-6, nthreads=8)
+    assert np.array_equal(Hs.lod, Hn.lod) and np.array_equal(Hs.lod, Os.lod)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(gs.n)
+    yn, yno = Hn.matvec(x, H.PREC_F64), On.matvec(x)
+    assert _relerr(yn, yno) <= 1e-6 and np.abs(yn).max() > 1e-3 * np.abs(Hs.matvec(x, H.PREC_F64)).max()
+    fr = P.friction_c2(gs, imax, jmax, ds0, seed=3, patch=(20.0, 10.0))
+    lod = Hs.lod
+    y0 = P.init_state(fr, lod)
+    P.perturb_state(y0, gs, lod, amp=0.02, depth=8.0, width=2.0)
+    rk = H.RK(Hs, fr, lod, gs.ds, hdiv=ds0, eps_r=1e-6, hn=Hn)
+    so = O.Sim(Os, fr, lod, hdiv=ds0, hn=On)
+    dg, do = rk.derivs(y0), so.derivs(y0)
+    for c in range(3):
+        assert np.abs(dg[c::3] - do[c::3]).max() <= 1e-7 * np.abs(do[c::3]).max() + 1e-300, c
+    assert np.abs(do[2::3]).max() > 0.0           # sigma moves
+    yg, eg = rk.rkck(y0, 1e3)
+    yo, eo = so.rkck(y0, 1e3)
+    assert np.abs(yg - yo).max() <= 1e-9 * np.abs(yo).max()
+    # first steps of a run: the step grids coincide
+    rk.set_state(y0, 0.0, 1.0)
+    info, hist, _ = rk.step(15, history=True)
+    yr, _, _, _ = rk.get_state()
+    yo = y0.copy()
+    a = fr.a[lod]
+    vel = 2 * fr.vref * np.exp(-yo[0::3] / a) * np.sinh(yo[1::3] / yo[2::3] / a)
+    done, t_o, _, ho, _ = so.run(yo, 0.0, 1.0, 1e-6, 15, np.zeros(gs.n), vel, np.asarray(gs.ds)[lod])
+    assert done == 15 and info["accepted"] == 15
+    assert np.allclose(hist[:, 0], ho[:, 0], rtol=1e-9)
+    for c in range(3):
+        assert np.abs(yr[c::3] - yo[c::3]).max() <= 1e-9 * np.abs(yo[c::3]).max(), c
