@@ -259,6 +259,18 @@ def friction_c1(g: Geometry) -> Friction:
     return _friction(g, a, b, dc=np.full(g.n, 0.008), sigma=50.0, vpl=1e-9, vref=1e-6, law=0)
 
 
+def friction_c3(g: Geometry) -> Friction:
+    """Rough triangular fault: the a-b depth profile of the 2-D antiplane
+    setup mapped onto the surface (a = 0.010 above 15 km, ramp to a - b =
+    +0.004 at 18 km and below), b = 0.015, Dc = 8 mm, sigma 50 MPa, Vpl 1e-9,
+    slip law, V0 = 1e-6."""
+    depth = -g.zcol
+    b = np.full(g.n, 0.015)
+    a = np.where(depth < 15.0, 0.010,
+                 np.where(depth < 18.0, 0.010 + (depth - 15.0) / 3.0 * 0.009, 0.019))
+    return _friction(g, a, b, dc=np.full(g.n, 0.008), sigma=50.0, vpl=1e-9, vref=1e-6, law=1)
+
+
 def friction_c2(g: Geometry, imax: int, jmax: int, ds0: float, seed: int = 1,
                 patch=(60.0, 30.0), sigma_log: float = 0.1, dc0: float = 0.02, law: int = 0) -> Friction:
     """3-D planar fault: velocity-weakening patch (a-b = -0.004) centred along

@@ -278,3 +278,15 @@ def test_gpu_normal_stress_matrix_against_oracle():
     assert np.allclose(hist[:, 0], ho[:, 0], rtol=1e-9)
     for c in range(3):
         assert np.abs(yr[c::3] - yo[c::3]).max() <= 1e-9 * np.abs(yo[c::3]).max(), c
+
+
+def test_gpu_rough_triangle_workload_small():
+    """The rough-surface triangular workload (slip law, TD kernel) end to end
+    at a small size: structure, no capped ranks, and the mixed single run
+    tracking the FP64 one over the first steps."""
+    from hbi_amd import workloads as W
+    r = W.c3(length=12.0, width=6.0, edge=0.25, leaf=32, eps=1e-5, steps=40)
+    assert r["n"] == 2 * 48 * 24 and r["hmat"]["ncapped"] == 0
+    f64, mx = r["rk"]["f64"], r["rk"]["mixed"]
+    assert f64["status"] == 0 and mx["status"] == 0
+    assert abs(mx["t_end_s"] - f64["t_end_s"]) <= 1e-6 * f64["t_end_s"]
