@@ -614,34 +614,63 @@ int hbi_rk_rkck(hbi_rk *r, const double *y, double h, double *yout, double *yerr
     });
 }
 
+}  // extern "C"
+
+namespace hbi {
+
+// history buffers (the graph records their addresses), recapture when any
+// graph argument changed, arm the step target, start the timing event
+static void step_prepare(hbi_rk *r, int nsteps, bool hist, int nrec, const int *rec)
+{
+    HBI_REQUIRE(nsteps >= 0, "bad arguments");
+    for (int q = 0; q < nrec; q++) HBI_REQUIRE(rec[q] >= 0 && rec[q] < r->n, "record index out of range");
+    const size_t hneed = hist ? kRkHist * (size_t)nsteps : 0;
+    const size_t vneed = nrec > 0 ? (size_t)nsteps * nrec : 0;
+    bool recapture = !r->graph;
+    if (hneed > r->hist.n || (hneed == 0 && r->hist.n)) {
+        if (hneed) r->hist.alloc(hneed); else r->hist.release();
+        recapture = true;
+    }
+    if (vneed > r->vhist.n || (vneed == 0 && r->vhist.n)) {
+        if (vneed) r->vhist.alloc(vneed); else r->vhist.release();
+        recapture = true;
+    }
+    if (nrec != (int)r->rec.n) {
+        if (nrec) r->rec.alloc(nrec); else r->rec.release();
+        recapture = true;
+    }
+    if (nrec) r->rec.upload(rec, nrec, r->stream);
+    if (r->dirty) recapture = true;
+    if (recapture) capture(r);
+    r->dirty = false;
+    rk_arm_kernel<<<1, 1, 0, r->stream>>>(r->ctl.p, nsteps, 0.0, 0);
+    HBI_HIP(hipEventRecord(r->e0, r->stream));
+}
+
+static void step_finish(hbi_rk *r, const RkCtl &c, hbi_rk_info *info)
+{
+    HBI_HIP(hipEventRecord(r->e1, r->stream));
+    HBI_HIP(hipEventSynchronize(r->e1));
+    float ms = 0;
+    HBI_HIP(hipEventElapsedTime(&ms, r->e0, r->e1));
+    r->last_ms = ms;
+    if (info) {
+        info->t = c.t; info->dt_next = c.h; info->dt_last = c.hdid; info->errmax_last = c.errmax;
+        info->attempts = c.attempts; info->accepted = c.accepted; info->rejected = c.rejected;
+        info->vmax = c.vmax; info->status = c.status;
+    }
+}
+
+}  // namespace hbi
+
+extern "C" {
+
 int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, double *vhist, hbi_rk_info *info)
 {
     return hbi::guarded([&] {
         HBI_REQUIRE(r && nsteps >= 0, "bad arguments");
         HBI_REQUIRE(nrec == 0 || (rec && vhist), "rec and vhist required when nrec > 0");
-        for (int q = 0; q < nrec; q++) HBI_REQUIRE(rec[q] >= 0 && rec[q] < r->n, "record index out of range");
-        // history buffers (the graph records their addresses)
-        const size_t hneed = hist ? hbi::kRkHist * (size_t)nsteps : 0;
-        const size_t vneed = nrec > 0 ? (size_t)nsteps * nrec : 0;
-        bool recapture = !r->graph;
-        if (hneed > r->hist.n || (hneed == 0 && r->hist.n)) {
-            if (hneed) r->hist.alloc(hneed); else r->hist.release();
-            recapture = true;
-        }
-        if (vneed > r->vhist.n || (vneed == 0 && r->vhist.n)) {
-            if (vneed) r->vhist.alloc(vneed); else r->vhist.release();
-            recapture = true;
-        }
-        if (nrec != (int)r->rec.n) {
-            if (nrec) r->rec.alloc(nrec); else r->rec.release();
-            recapture = true;
-        }
-        if (nrec) r->rec.upload(rec, nrec, r->stream);
-        if (r->dirty) recapture = true;
-        if (recapture) hbi::capture(r);
-        r->dirty = false;
-        hbi::rk_arm_kernel<<<1, 1, 0, r->stream>>>(r->ctl.p, nsteps, 0.0, 0);
-        HBI_HIP(hipEventRecord(r->e0, r->stream));
+        hbi::step_prepare(r, nsteps, hist != nullptr, nrec, rec);
         hbi::RkCtl c;
         long long launched = 0, budget = 50LL * nsteps + 100;
         int batch = nsteps;
@@ -653,23 +682,53 @@ int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, d
             if (c.done || launched >= budget || nsteps == 0) break;
             batch = (int)std::max<long long>(1, std::min<long long>(c.target - c.accepted + 2, budget - launched));
         }
-        HBI_HIP(hipEventRecord(r->e1, r->stream));
-        HBI_HIP(hipEventSynchronize(r->e1));
-        float ms = 0;
-        HBI_HIP(hipEventElapsedTime(&ms, r->e0, r->e1));
-        r->last_ms = ms;
         const long long done = c.accepted - c.hist_base;
         if (hist && done > 0)
             r->hist.download(hist, hbi::kRkHist * (size_t)std::min<long long>(done, nsteps), r->stream);
         if (nrec && done > 0)
             r->vhist.download(vhist, (size_t)std::min<long long>(done, nsteps) * nrec, r->stream);
         HBI_HIP(hipStreamSynchronize(r->stream));
-        if (info) {
-            info->t = c.t; info->dt_next = c.h; info->dt_last = c.hdid; info->errmax_last = c.errmax;
-            info->attempts = c.attempts; info->accepted = c.accepted; info->rejected = c.rejected;
-            info->vmax = c.vmax; info->status = c.status;
-        }
+        hbi::step_finish(r, c, info);
         if (c.status == HBI_E_STEP) throw hbi::Error{HBI_E_STEP, "Runge-Kutta step size underflow"};
+    });
+}
+
+int hbi_rk_step_many(hbi_rk *const *rs, int m, int nsteps, hbi_rk_info *infos)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(rs && m > 0 && nsteps >= 0, "bad arguments");
+        for (int f = 0; f < m; f++) HBI_REQUIRE(rs[f], "NULL run");
+        for (int f = 0; f < m; f++)
+            for (int g = f + 1; g < m; g++) HBI_REQUIRE(rs[f] != rs[g], "a run appears twice");
+        for (int f = 0; f < m; f++) hbi::step_prepare(rs[f], nsteps, false, 0, nullptr);
+        std::vector<hbi::RkCtl> c(m);
+        std::vector<char> fin(m, 0);
+        std::vector<long long> launched(m, 0);
+        const long long budget = 50LL * nsteps + 100;
+        int batch = nsteps;
+        for (;;) {
+            // interleaved over the runs' streams: the device overlaps the runs
+            for (int q = 0; q < batch; q++)
+                for (int f = 0; f < m; f++)
+                    if (!fin[f] && launched[f] + q < budget) HBI_HIP(hipGraphLaunch(rs[f]->graph, rs[f]->stream));
+            for (int f = 0; f < m; f++) {
+                if (fin[f]) continue;
+                launched[f] += batch;
+                HBI_HIP(hipMemcpyAsync(&c[f], rs[f]->ctl.p, sizeof(hbi::RkCtl), hipMemcpyDeviceToHost, rs[f]->stream));
+            }
+            long long need = 0;
+            for (int f = 0; f < m; f++) {
+                if (fin[f]) continue;
+                HBI_HIP(hipStreamSynchronize(rs[f]->stream));
+                if (c[f].done || launched[f] >= budget || nsteps == 0) fin[f] = 1;
+                else need = std::max(need, std::min<long long>(c[f].target - c[f].accepted + 2, budget - launched[f]));
+            }
+            if (need == 0) break;
+            batch = (int)std::max<long long>(1, need);
+        }
+        for (int f = 0; f < m; f++) hbi::step_finish(rs[f], c[f], infos ? &infos[f] : nullptr);
+        for (int f = 0; f < m; f++)
+            if (c[f].status == HBI_E_STEP) throw hbi::Error{HBI_E_STEP, "Runge-Kutta step size underflow"};
     });
 }
 

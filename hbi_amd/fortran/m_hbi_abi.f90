@@ -113,6 +113,12 @@ module m_hbi_abi
       real(c_double), intent(out) :: y(*)
       integer(c_int), value :: nrhs, prec
     end function
+    integer(c_int) function hbi_rk_step_many(rs, m, nsteps, infos) bind(C, name="hbi_rk_step_many")
+      import :: c_int, c_ptr, hbi_rk_info
+      type(c_ptr), intent(in) :: rs(*)
+      integer(c_int), value :: m, nsteps
+      type(hbi_rk_info), intent(out) :: infos(*)
+    end function
     ! ensembles
     integer(c_int) function hbi_ens_create(hs, opts, nreal, a, b, dc, f0, vpl, taudot, sigdot, e) &
         bind(C, name="hbi_ens_create")

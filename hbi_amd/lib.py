@@ -81,7 +81,7 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_ens_step", "hbi_ens_last_ms", "hbi_mlp_create", "hbi_mlp_destroy", "hbi_mlp_nparams",
            "hbi_mlp_set_params", "hbi_mlp_get_params", "hbi_mlp_forward", "hbi_mlp_loss_grad", "hbi_mlp_train",
            "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace", "hbi_ens_set_controller",
-           "hbi_ens_set_trace", "hbi_ens_get_trace")
+           "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_rk_step_many")
 
 
 def lib():
@@ -131,6 +131,7 @@ def lib():
         L.hbi_rk_get_trace.argtypes = [C.c_void_p, _dp, _dp, _ip]
         L.hbi_ens_set_controller.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.hbi_ens_set_trace.argtypes = [C.c_void_p, C.c_int]
+        L.hbi_rk_step_many.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(RkInfo)]
         L.hbi_ens_get_trace.argtypes = [C.c_void_p, _dp, _dp, _ip]
         _lib = L
     return _lib
@@ -322,6 +323,15 @@ class RK:
         ms = C.c_double(0)
         _check(lib().hbi_rk_last_ms(self.r, C.byref(ms)))
         return ms.value
+
+    @staticmethod
+    def step_many(runs, nsteps):
+        """Advance several runs (own matrices, own step control) nsteps accepted
+        steps each, concurrently on the device; returns their info dicts."""
+        arr = (C.c_void_p * len(runs))(*[r.r for r in runs])
+        infos = (RkInfo * len(runs))()
+        _check(lib().hbi_rk_step_many(arr, len(runs), int(nsteps), infos))
+        return [i.as_dict() for i in infos]
 
     def set_controller(self, mode, mlp=None, safety=0.9, ratio_lo=0.125, ratio_hi=8.0):
         """mode 0: rkqs (the reference's controller); 1: learned proposals from mlp."""

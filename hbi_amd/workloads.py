@@ -2,6 +2,7 @@
 
     python -m hbi_amd.workloads c3 [--length 60] [--width 30] [--edge 0.134]
         [--leaf 64] [--eps 1e-5] [--steps 200]
+    python -m hbi_amd.workloads c4b [--faults 16] [--steps 200]
 
 c3: a 60 x 30 km self-similar rough fault surface (Hurst 0.8, amplitude to
 wavelength 1e-2, seed 0) meshed with right triangles of edge `edge` km
@@ -65,9 +66,65 @@ def c3(length=60.0, width=30.0, edge=0.134, leaf=64, eps=1e-5, steps=200):
     return out
 
 
+def c4b(nfaults=16, steps=200, seed=4, nmin=2000, nmax=30000):
+    """Distinct planar faults (100 x 50 km, imax x imax/2 elements, sizes
+    drawn uniformly from [nmin, nmax] with a seed), each with its own
+    H-matrix (leaf 64, ACA eps 1e-5), friction realisation and step control,
+    mixed precision: nsteps accepted steps of every fault, one after another
+    and concurrently (hbi_rk_step_many)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, nfaults)
+    runs_data = []
+    t0 = time.perf_counter()
+    for f, n in enumerate(sizes):
+        imax = max(4, int(round(np.sqrt(2.0 * n) / 2.0)) * 2)
+        jmax = imax // 2
+        ds0 = 100.0 / imax
+        g = P.mesh_3dph(imax, jmax, ds0)
+        Hm = H.HMatrix(g, leaf=64, eps=1e-5, bf16_margin=1.0)
+        fr = P.friction_c2(g, imax, jmax, ds0, seed=100 + f)
+        lod = Hm.lod
+        y0 = P.init_state(fr, lod)
+        P.perturb_state(y0, g, lod, amp=0.02, depth=15.0, width=3.0)
+        runs_data.append((g, Hm, fr, lod, y0, ds0))
+    t_build = time.perf_counter() - t0
+    ns = [d[0].n for d in runs_data]
+    _log("c4b: %d faults, %d..%d elements (%d total), built in %.1f s" % (nfaults, min(ns), max(ns), sum(ns), t_build))
+
+    def make_runs():
+        rs = []
+        for g, Hm, fr, lod, y0, ds0 in runs_data:
+            rk = H.RK(Hm, fr, lod, g.ds, hdiv=ds0, eps_r=1e-6, prec=H.PREC_MIXED)
+            rk.set_state(y0, 0.0, 1.0)
+            rk.step(5)                   # graph capture outside the timing
+            rs.append(rk)
+        return rs
+
+    rs = make_runs()
+    t0 = time.perf_counter()
+    for rk in rs:
+        rk.step(steps)
+    wall_seq = time.perf_counter() - t0
+    for rk in rs:
+        rk.close()
+    rs = make_runs()
+    t0 = time.perf_counter()
+    infos = H.RK.step_many(rs, steps)
+    wall_many = time.perf_counter() - t0
+    for rk in rs:
+        rk.close()
+    total = nfaults * steps
+    return {"workload": "c4b", "faults": nfaults, "elements": ns, "steps_per_fault": steps, "build_s": t_build,
+            "sequential": {"wall_s": wall_seq, "steps_per_s": total / wall_seq},
+            "concurrent": {"wall_s": wall_many, "steps_per_s": total / wall_many,
+                           "all_ok": all(i["status"] == 0 and i["accepted"] >= steps + 5 for i in infos)},
+            "speedup": wall_seq / wall_many}
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=("c3",))
+    ap.add_argument("workload", choices=("c3", "c4b"))
+    ap.add_argument("--faults", type=int, default=16)
     ap.add_argument("--length", type=float, default=60.0)
     ap.add_argument("--width", type=float, default=30.0)
     ap.add_argument("--edge", type=float, default=0.134)
@@ -75,7 +132,10 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-5)
     ap.add_argument("--steps", type=int, default=200)
     a = ap.parse_args()
-    print(json.dumps(c3(a.length, a.width, a.edge, a.leaf, a.eps, a.steps)), flush=True)
+    if a.workload == "c3":
+        print(json.dumps(c3(a.length, a.width, a.edge, a.leaf, a.eps, a.steps)), flush=True)
+    else:
+        print(json.dumps(c4b(a.faults, a.steps)), flush=True)
 
 
 if __name__ == "__main__":
