@@ -399,203 +399,306 @@ template <> struct Raw<uint16_t> {
 
 constexpr int kMvThreads = 256;
 
-// phase 1: z[zoff + k] = sum_t a1T[k][t] x[xoff + t], one wave per item.
-// lpr lanes (a power of two covering the row's 16-byte chunks, at most 64)
-// share a row, 64/lpr rows are in flight per pass and kU1 passes are unrolled;
-// each lane keeps its x chunk across the rows of a pass, and the row sums are
-// finished with a fixed xor-shuffle tree inside each lpr-lane segment.
-constexpr int kU1 = 4;
-template <typename T>
-__device__ __forceinline__ void lrvt_wave(const Item1 &it, const uint8_t *__restrict__ buf, const double *__restrict__ x,
-                                          double *__restrict__ z, int lane)
+// phase 1: z rows of one stacked row tile = (tile, tr x ncol, column-major)
+// times x[xoff ..].  One workgroup per item; lane l owns V rows (16 bytes) of
+// the column csub = l / (tr / V) of every column group, so a wave load covers
+// 64 / (tr / V) consecutive columns (1 KB, contiguous) and U loads per lane
+// are in flight.  Row sums: a fixed xor-shuffle tree over the lanes sharing
+// rows, then the four waves in order.  Thread r < tr returns row r's sum.
+template <typename T, int U>
+__device__ __forceinline__ double stack_rows(const Item1 &it, const uint8_t *__restrict__ buf,
+                                             const double *__restrict__ x, double (*red)[64])
 {
     constexpr int V = Raw<T>::V;
-    const int nch = it.ldt / V;
-    int lpr = kWave;
-    while (lpr > 1 && lpr / 2 >= nch) lpr >>= 1;
-    const int rpp = kWave / lpr;
-    const int cl = lane % lpr, rs = lane / lpr;
-    for (int k0 = 0; k0 < it.nk; k0 += rpp * kU1) {
-        double acc[kU1];
+    constexpr int NW = kMvThreads / kWave;
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    const int lpc = it.tr / V, cpi = kWave / lpc;
+    const int rsub = lane % lpc, csub = lane / lpc;
+    const uint8_t *base = buf + it.off + (size_t)rsub * 16;
+    const size_t cstride = (size_t)it.tr * sizeof(T);
+    const double *xin = x + it.xoff;
+    double acc[V];
 #pragma unroll
-        for (int u = 0; u < kU1; u++) acc[u] = 0.0;
-        for (int c = cl; c < nch; c += lpr) {
-            const int t = c * V;
-            double xv[V];
+    for (int e = 0; e < V; e++) acc[e] = 0.0;
+    const int ngroups = (it.ncol + cpi - 1) / cpi;
+    for (int g0 = w; g0 < ngroups; g0 += NW * U) {
+        typename Raw<T>::R a[U];
+        double xv[U];
 #pragma unroll
-            for (int e = 0; e < V; e++) xv[e] = (t + e < it.nc) ? x[it.xoff + t + e] : 0.0;
-            typename Raw<T>::R a[kU1];
-#pragma unroll
-            for (int u = 0; u < kU1; u++) {
-                const int row = k0 + rs + u * rpp;
-                a[u] = row < it.nk ? Raw<T>::load(buf + it.off + ((size_t)row * it.ldt + t) * sizeof(T)) : Raw<T>::zero();
-            }
-#pragma unroll
-            for (int u = 0; u < kU1; u++)
-#pragma unroll
-                for (int e = 0; e < V; e++) acc[u] = fma(Raw<T>::get(a[u], e), xv[e], acc[u]);
+        for (int u = 0; u < U; u++) {
+            const int c = (g0 + u * NW) * cpi + csub;
+            const bool ok = c < it.ncol;
+            a[u] = ok ? Raw<T>::load(base + (size_t)c * cstride) : Raw<T>::zero();
+            xv[u] = ok ? xin[c] : 0.0;
         }
-        for (int o = lpr / 2; o > 0; o >>= 1)
 #pragma unroll
-            for (int u = 0; u < kU1; u++) acc[u] += __shfl_xor(acc[u], o, kWave);
-        if (cl == 0)
+        for (int u = 0; u < U; u++)
 #pragma unroll
-            for (int u = 0; u < kU1; u++) {
-                const int row = k0 + rs + u * rpp;
-                if (row < it.nk) z[it.zoff + row] = acc[u];
-            }
+            for (int e = 0; e < V; e++) acc[e] = fma(Raw<T>::get(a[u], e), xv[u], acc[e]);
     }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kMvThreads) void lrvt_kernel(const Item1 *__restrict__ items, int nitems,
-                                                          const uint8_t *__restrict__ buf,
-                                                          const double *__restrict__ x, double *__restrict__ z,
-                                                          const int *__restrict__ gate)
-{
-    if (gate && *gate) return;
-    const int w = blockIdx.x * (kMvThreads / kWave) + threadIdx.x / kWave;
-    if (w >= nitems) return;
-    const Item1 it = items[w];
-    lrvt_wave<T>(it, buf, x, z, threadIdx.x % kWave);
+    for (int o = lpc; o < kWave; o <<= 1)
+#pragma unroll
+        for (int e = 0; e < V; e++) acc[e] += __shfl_xor(acc[e], o, kWave);
+    if (csub == 0)
+#pragma unroll
+        for (int e = 0; e < V; e++) red[w][rsub * V + e] = acc[e];
+    __syncthreads();
+    const int r = threadIdx.x;
+    return r < it.tr ? ((red[0][r] + red[1][r]) + red[2][r]) + red[3][r] : 0.0;
 }
 
 // phase 2: partial y of one work item (one workgroup).  A lane owns V rows
 // of the TR-row tile and streams columns; U column groups are loaded before
 // they are used so U 16-byte loads per lane are in flight.
 template <typename T, int TR, int U>
-__global__ __launch_bounds__(kMvThreads) void tile_kernel(const Item2 *__restrict__ items, const Seg *__restrict__ segs,
-                                                          const uint8_t *__restrict__ buf, const double *__restrict__ x,
-                                                          const double *__restrict__ z, double *__restrict__ ypart,
-                                                          const int *__restrict__ gate)
+__device__ __forceinline__ void tile_seg(const Seg &sg, const uint8_t *__restrict__ buf, const double *__restrict__ xin,
+                                         int rsub, int csub, int w, double (&acc)[Raw<T>::V])
 {
     constexpr int V = Raw<T>::V;
     constexpr int LPC = TR / V;        // lanes per column
     constexpr int CPI = kWave / LPC;   // columns per wave load
     constexpr int NW = kMvThreads / kWave;
-    __shared__ double lds[kMvThreads * V];
-    if (gate && *gate) return;
-    const Item2 it = items[blockIdx.x];
+    const uint8_t *base = buf + sg.off + (size_t)rsub * V * sizeof(T);
+    const int ngroups = (sg.ncol + CPI - 1) / CPI;
+    for (int g0 = w; g0 < ngroups; g0 += NW * U) {
+        typename Raw<T>::R a[U];
+        double xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int g = g0 + u * NW;
+            const int c = g * CPI + csub;
+            const bool ok = g < ngroups && c < sg.ncol;
+            a[u] = ok ? Raw<T>::load(base + (size_t)c * TR * sizeof(T)) : Raw<T>::zero();
+            xv[u] = ok ? xin[c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int e = 0; e < V; e++) acc[e] = fma(Raw<T>::get(a[u], e), xv[u], acc[e]);
+    }
+}
+
+// Segments [s0, s1) of one tile (one dtype; dense ones read x, low-rank
+// ones z), then the lanes that share rows are added in a fixed order through
+// red.  Thread r < TR returns row r's sum; red is free again on return.
+template <typename T, int TR, int U>
+__device__ __forceinline__ double tile_rows(int s0, int s1, const Seg *__restrict__ segs,
+                                            const uint8_t *__restrict__ buf, const double *__restrict__ x,
+                                            const double *__restrict__ z, double *red)
+{
+    constexpr int V = Raw<T>::V;
+    constexpr int LPC = TR / V;
+    constexpr int CPI = kWave / LPC;
+    constexpr int NW = kMvThreads / kWave;
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
     const int rsub = lane % LPC, csub = lane / LPC;
     double acc[V];
 #pragma unroll
     for (int e = 0; e < V; e++) acc[e] = 0.0;
-    for (int s = it.seg0; s < it.seg1; s++) {
+    for (int s = s0; s < s1; s++) {
         const Seg sg = segs[s];
-        const uint8_t *base = buf + sg.off + (size_t)rsub * V * sizeof(T);
-        const double *xin = (sg.src ? z : x) + sg.xoff;
-        const int ngroups = (sg.ncol + CPI - 1) / CPI;
-        for (int g0 = w; g0 < ngroups; g0 += NW * U) {
-            typename Raw<T>::R a[U];
-            double xv[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int g = g0 + u * NW;
-                const int c = g * CPI + csub;
-                const bool ok = g < ngroups && c < sg.ncol;
-                a[u] = ok ? Raw<T>::load(base + (size_t)c * TR * sizeof(T)) : Raw<T>::zero();
-                xv[u] = ok ? xin[c] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++)
-#pragma unroll
-                for (int e = 0; e < V; e++) acc[e] = fma(Raw<T>::get(a[u], e), xv[u], acc[e]);
-        }
+        tile_seg<T, TR, U>(sg, buf, (sg.src ? z : x) + sg.xoff, rsub, csub, w, acc);
     }
-    // lanes with the same rsub hold the same rows: add them in a fixed order
 #pragma unroll
-    for (int e = 0; e < V; e++) lds[(w * kWave + lane) * V + e] = acc[e];
+    for (int e = 0; e < V; e++) red[(w * kWave + lane) * V + e] = acc[e];
     __syncthreads();
-    for (int r = threadIdx.x; r < TR; r += kMvThreads) {
+    double sum = 0.0;
+    const int r = threadIdx.x;
+    if (r < TR) {
         const int rs = r / V, e = r % V;
-        double s = 0.0;
         for (int ww = 0; ww < NW; ww++)
-            for (int cs = 0; cs < CPI; cs++) s += lds[(ww * kWave + cs * LPC + rs) * V + e];
-        ypart[(size_t)it.part * TR + r] = s;
+            for (int cs = 0; cs < CPI; cs++) sum += red[(ww * kWave + cs * LPC + rs) * V + e];
     }
+    __syncthreads();
+    return sum;
 }
 
-__global__ void tile_reduce_kernel(int n, int tr, const int *__restrict__ part0, const double *__restrict__ ypart,
-                                   double *__restrict__ y)
+template <int TR, int U>
+__device__ __forceinline__ double tile_rows_any(int dtype, int s0, int s1, const Seg *__restrict__ segs,
+                                                const uint8_t *__restrict__ buf, const double *__restrict__ x,
+                                                const double *__restrict__ z, double *red)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int t = r / tr, rr = r % tr;
-    double s = 0.0;
-    for (int p = part0[t]; p < part0[t + 1]; p++) s += ypart[(size_t)p * tr + rr];
-    y[r] = s;
+    if (dtype == kF64) return tile_rows<double, TR, U>(s0, s1, segs, buf, x, z, red);
+    if (dtype == kF32) return tile_rows<float, TR, U>(s0, s1, segs, buf, x, z, red);
+    return tile_rows<uint16_t, TR, U>(s0, s1, segs, buf, x, z, red);
+}
+
+// Launch A: blocks [0, n1) are phase-1 items, one row tile each (its rows'
+// sums go to z through zrow); the rest are tile items of dense blocks (x
+// only), each writing its partial slot.  Nothing here reads z.
+template <int TR, int U1, int U2>
+__global__ __launch_bounds__(kMvThreads) void stream_kernel(const Item1 *__restrict__ it1, int n1,
+                                                            const Item2 *__restrict__ it2,
+                                                            const Seg *__restrict__ segs,
+                                                            const int *__restrict__ zrow,
+                                                            const uint8_t *__restrict__ buf,
+                                                            const double *__restrict__ x, double *__restrict__ z,
+                                                            double *__restrict__ ypart, const int *__restrict__ gate)
+{
+    __shared__ double sm[kMvThreads * 8];
+    if (gate && *gate) return;
+    if ((int)blockIdx.x < n1) {
+        const Item1 it = it1[blockIdx.x];
+        double (*red)[64] = reinterpret_cast<double (*)[64]>(sm);
+        double s;
+        if (it.dtype == kF64) s = stack_rows<double, U1>(it, buf, x, red);
+        else if (it.dtype == kF32) s = stack_rows<float, U1>(it, buf, x, red);
+        else s = stack_rows<uint16_t, U1>(it, buf, x, red);
+        if ((int)threadIdx.x < it.nrows) z[zrow[it.row0 + threadIdx.x]] = s;
+        return;
+    }
+    const Item2 it = it2[blockIdx.x - n1];
+    const double s = tile_rows_any<TR, U2>(it.dtype, it.seg0, it.seg1, segs, buf, x, nullptr, sm);
+    if (threadIdx.x < TR) ypart[(size_t)it.part * TR + threadIdx.x] = s;
+}
+
+// The low-rank columns of one tile and dtype (contiguous in the layout,
+// ncol of them from byte offset off; column j multiplies z[zc[j]]): the z
+// values are gathered into LDS, kZL at a time, and the columns streamed as
+// one run.  Thread r < TR returns row r's sum; red and zl are free again on
+// return.
+constexpr int kZL = 2048;
+template <typename T, int TR, int U>
+__device__ __forceinline__ double lowrank_rows(uint64_t off, int ncol, const int *__restrict__ zc,
+                                               const uint8_t *__restrict__ buf, const double *__restrict__ z,
+                                               double *zl, double *red)
+{
+    constexpr int V = Raw<T>::V;
+    constexpr int LPC = TR / V;
+    constexpr int CPI = kWave / LPC;
+    constexpr int NW = kMvThreads / kWave;
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    const int rsub = lane % LPC, csub = lane / LPC;
+    double acc[V];
+#pragma unroll
+    for (int e = 0; e < V; e++) acc[e] = 0.0;
+    for (int c0 = 0; c0 < ncol; c0 += kZL) {
+        const int nb = min(kZL, ncol - c0);
+        for (int j = threadIdx.x; j < nb; j += kMvThreads) zl[j] = z[zc[c0 + j]];
+        __syncthreads();
+        Seg sg{};
+        sg.off = off + (uint64_t)c0 * TR * sizeof(T);
+        sg.ncol = nb;
+        tile_seg<T, TR, U>(sg, buf, zl, rsub, csub, w, acc);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int e = 0; e < V; e++) red[(w * kWave + lane) * V + e] = acc[e];
+    __syncthreads();
+    double sum = 0.0;
+    const int r = threadIdx.x;
+    if (r < TR) {
+        const int rs = r / V, e = r % V;
+        for (int ww = 0; ww < NW; ww++)
+            for (int cs = 0; cs < CPI; cs++) sum += red[(ww * kWave + cs * LPC + rs) * V + e];
+    }
+    __syncthreads();
+    return sum;
+}
+
+// Launch B: one item per row tile.  Its low-rank columns (a2 times z; one
+// run per dtype, in dtype order) plus the tile's dense slots from launch A
+// (in slot order) give the tile's rows of y.
+template <int TR, int U>
+__global__ __launch_bounds__(kMvThreads) void lowrank_kernel(const ItemB *__restrict__ items,
+                                                             const int *__restrict__ zcol,
+                                                             const uint8_t *__restrict__ buf,
+                                                             const double *__restrict__ z,
+                                                             const double *__restrict__ ypart,
+                                                             const int *__restrict__ part0, double *__restrict__ y,
+                                                             int n, const int *__restrict__ gate)
+{
+    __shared__ double zl[kZL];
+    __shared__ double red[kMvThreads * 8];
+    if (gate && *gate) return;
+    const ItemB it = items[blockIdx.x];
+    double lr = 0.0;
+    int zc = it.zc0;
+    if (it.ncol[kF64] > 0) lr += lowrank_rows<double, TR, U>(it.off[kF64], it.ncol[kF64], zcol + zc, buf, z, zl, red);
+    zc += it.ncol[kF64];
+    if (it.ncol[kF32] > 0) lr += lowrank_rows<float, TR, U>(it.off[kF32], it.ncol[kF32], zcol + zc, buf, z, zl, red);
+    zc += it.ncol[kF32];
+    if (it.ncol[kBF16] > 0)
+        lr += lowrank_rows<uint16_t, TR, U>(it.off[kBF16], it.ncol[kBF16], zcol + zc, buf, z, zl, red);
+    const int r = threadIdx.x, row = it.tile * TR + r;
+    if (r < TR && row < n) {
+        double s = 0.0;
+        for (int p = part0[it.tile]; p < part0[it.tile + 1]; p++) s += ypart[(size_t)p * TR + r];
+        y[row] = s + lr;
+    }
 }
 
 // ------------------------------------------------------------ host side --
 static inline size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
 
-// Group the packed layout into work items: phase-1 items are row runs of one
-// low-rank block (one wave each), phase-2 items are runs of one tile's
-// segments of one dtype (one workgroup each) writing one partial slot.
-static void make_items(hbi_hmat *h, Plan &P, int chunk1, int chunk2, int unroll2)
+// z values of the single-vector product
+static size_t z_size(const hbi_hmat *h) { return (size_t)std::max(h->nz, 1) + 8; }
+
+// Group the phase-2 layout into work items: runs of one tile's dense
+// segments of one dtype (one workgroup each) writing one partial slot, a
+// tile's slots consecutive; and one item per tile with its low-rank segments.
+// (Phase-1 items are the row tiles.)
+static void make_items(hbi_hmat *h, Plan &P, int chunk2, int unroll1, int unroll2, int unroll3)
 {
-    P.it1.clear();
     P.it2.clear();
-    for (const Row1 &r : P.rows1) {
-        const int sz = dtype_size(r.dtype);
-        const int rows_per = std::max(1, std::min(r.kt, (int)((size_t)chunk1 / ((size_t)r.ldt * sz))));
-        for (int k0 = 0; k0 < r.kt; k0 += rows_per) {
-            Item1 it{};
-            it.off = r.off + (size_t)k0 * r.ldt * sz;
-            it.ldt = r.ldt; it.nc = r.nc; it.nk = std::min(rows_per, r.kt - k0);
-            it.xoff = r.xoff; it.zoff = r.zoff + k0; it.dtype = r.dtype;
-            P.it1.push_back(it);
-        }
-    }
-    std::stable_sort(P.it1.begin(), P.it1.end(), [](const Item1 &a, const Item1 &b) {
-        if (a.dtype != b.dtype) return a.dtype < b.dtype;
-        return (long long)a.nk * a.ldt * dtype_size(a.dtype) > (long long)b.nk * b.ldt * dtype_size(b.dtype);
-    });
+    P.itb.clear();
+    P.zcol.clear();
     P.nparts = 0;
     P.tile_part0.assign(h->ntiles + 1, 0);
     size_t g = 0;
     for (int t = 0; t < h->ntiles; t++) {
         P.tile_part0[t] = P.nparts;
+        ItemB ib{};
+        ib.tile = t;
+        ib.zc0 = (int)P.zcol.size();
+        const size_t g_first = g;
         for (; g < P.groups.size() && P.groups[g].tile == t; g++) {
+            // dense segments come first in a group; the low-rank ones that
+            // follow are contiguous in the layout
             const SegGroup &G = P.groups[g];
+            const int sz = dtype_size(G.dtype);
+            int sb = G.s0;
+            while (sb < G.s1 && !P.segs[sb].src) sb++;
+            if (sb < G.s1) ib.off[G.dtype] = P.segs[sb].off;
+            for (int q = sb; q < G.s1; q++) {
+                const Seg &sg = P.segs[q];
+                HBI_REQUIRE(sg.off == ib.off[G.dtype] + (uint64_t)ib.ncol[G.dtype] * h->tr * sz,
+                            "low-rank segments of a tile are not contiguous");
+                ib.ncol[G.dtype] += sg.ncol;
+            }
+            if (sb == G.s0) continue;
             size_t cur = 0;
             int s0 = G.s0;
-            for (int q = G.s0; q < G.s1; q++) {
-                const size_t sb = (size_t)h->tr * P.segs[q].ncol * dtype_size(G.dtype);
-                if (q > s0 && cur + sb > (size_t)chunk2) {
+            for (int q = G.s0; q < sb; q++) {
+                const size_t bq = (size_t)h->tr * P.segs[q].ncol * dtype_size(G.dtype);
+                if (q > s0 && cur + bq > (size_t)chunk2) {
                     P.it2.push_back({t, s0, q, P.nparts++, G.dtype, 0});
                     s0 = q;
                     cur = 0;
                 }
-                cur += sb;
+                cur += bq;
             }
-            P.it2.push_back({t, s0, G.s1, P.nparts++, G.dtype, 0});
+            P.it2.push_back({t, s0, sb, P.nparts++, G.dtype, 0});
         }
+        // z index per low-rank column, dtype by dtype
+        for (int d = 0; d < 3; d++)
+            for (size_t q = g_first; q < g; q++) {
+                const SegGroup &G = P.groups[q];
+                if (G.dtype != d) continue;
+                for (int k = G.s0; k < G.s1; k++)
+                    if (P.segs[k].src)
+                        for (int j = 0; j < P.segs[k].ncol; j++) P.zcol.push_back(P.segs[k].xoff + j);
+            }
+        P.itb.push_back(ib);
     }
     P.tile_part0[h->ntiles] = P.nparts;
-    std::stable_sort(P.it2.begin(), P.it2.end(), [](const Item2 &a, const Item2 &b) { return a.dtype < b.dtype; });
-    for (int d = 0; d < 3; d++) {
-        P.r1[d][0] = P.r1[d][1] = 0;
-        P.r2[d][0] = P.r2[d][1] = 0;
-    }
-    for (int d = 0; d < 3; d++) {
-        int lo = 0;
-        while (lo < (int)P.it1.size() && P.it1[lo].dtype < d) lo++;
-        int hi = lo;
-        while (hi < (int)P.it1.size() && P.it1[hi].dtype == d) hi++;
-        P.r1[d][0] = lo; P.r1[d][1] = hi;
-        lo = 0;
-        while (lo < (int)P.it2.size() && P.it2[lo].dtype < d) lo++;
-        hi = lo;
-        while (hi < (int)P.it2.size() && P.it2[hi].dtype == d) hi++;
-        P.r2[d][0] = lo; P.r2[d][1] = hi;
-    }
-    P.chunk1 = chunk1;
     P.chunk2 = chunk2;
+    P.unroll1 = unroll1;
     P.unroll2 = unroll2;
-    P.d_it1.upload(P.it1, h->stream);
+    P.unroll3 = unroll3;
     P.d_it2.upload(P.it2, h->stream);
+    P.d_itb.upload(P.itb, h->stream);
+    P.d_zcol.upload(P.zcol, h->stream);
     P.d_tile_part0.upload(P.tile_part0, h->stream);
     if ((int)h->d_ypart.n < std::max(P.nparts, 1) * h->tr) h->d_ypart.alloc((size_t)std::max(P.nparts, 1) * h->tr);
     HBI_HIP(hipStreamSynchronize(h->stream));
@@ -642,6 +745,66 @@ static void build_plan(hbi_hmat *h, int prec)
         P.alg_bytes += (long long)f.nc * f.kt * sz;
         P.alg_bytes_p1 += (long long)f.nc * f.kt * sz;
     }
+    // ---- single-vector phase-1 layout: the blocks of one column cluster and
+    // dtype stacked (rows in lr_order) and cut into column-major row tiles:
+    // trm rows (the largest power of two from 64 down to 16 bytes of rows
+    // with trm x nc values in kTile1Bytes), then the rest rounded up to 16
+    // bytes of rows as tiles of decreasing powers of two
+    for (size_t q0 = 0; q0 < lr_order.size();) {
+        const Leaf &g = L[lr_order[q0]];
+        const int dt = leaf_dtype(lr_order[q0]), sz = dtype_size(dt), V = 16 / sz;
+        size_t q1 = q0;
+        int K = 0;
+        while (q1 < lr_order.size() && L[lr_order[q1]].c0 == g.c0 && L[lr_order[q1]].nc == g.nc &&
+               leaf_dtype(lr_order[q1]) == dt)
+            K += L[lr_order[q1++]].kt;
+        const int row0 = (int)P.zrow.size();
+        for (size_t q = q0; q < q1; q++)
+            for (int k = 0; k < L[lr_order[q]].kt; k++) P.zrow.push_back(h->zoff[lr_order[q]] + k);
+        int trm = 64;
+        while (trm > V && (size_t)trm * g.nc * sz > (size_t)kTile1Bytes) trm >>= 1;
+        std::vector<int> trs(K / trm, trm);
+        for (int rest = (K % trm + V - 1) / V * V, b = trm; rest > 0; b >>= 1)   // rest: a multiple of V <= trm
+            if (rest >= b) { trs.push_back(b); rest -= b; }
+        int T0 = 0;
+        for (int tr : trs) {
+            const int nrows = std::min(tr, K - T0);
+            Item1 it{};
+            it.off = bytes; it.ncol = g.nc; it.xoff = g.c0; it.row0 = row0 + T0; it.nrows = nrows;
+            it.tr = tr; it.dtype = dt;
+            P.it1.push_back(it);
+            // the pieces of the stacked blocks that fall in rows [T0, T0 + tr)
+            int kofs = 0;
+            for (size_t q = q0; q < q1; q++) {
+                const int ip = lr_order[q], kt = L[ip].kt;
+                const int lo = std::max(kofs, T0), hi = std::min(kofs + kt, T0 + tr);
+                if (lo < hi) {
+                    CopyJob J{};
+                    J.which = 1; J.src = h->fac_off[ip] + (long long)(lo - kofs) * g.nc;
+                    J.dst = (long long)(bytes + (size_t)(lo - T0) * sz); J.start = elems;
+                    J.rows = hi - lo; J.cols = g.nc; J.src_rs = g.nc; J.src_cs = 1;
+                    J.r_lo = 0; J.r_hi = hi - lo; J.c_hi = g.nc; J.dst_rs = 1; J.dst_cs = tr; J.dtype = dt;
+                    jobs.push_back(J);
+                    elems += (long long)(hi - lo) * g.nc;
+                }
+                kofs += kt;
+            }
+            if (nrows < tr) {   // zero rows below the stack
+                CopyJob J{};
+                J.which = 1; J.dst = (long long)(bytes + (size_t)nrows * sz); J.start = elems;
+                J.rows = tr - nrows; J.cols = g.nc; J.src_rs = 0; J.src_cs = 0;
+                J.r_lo = 0; J.r_hi = 0; J.c_hi = 0; J.dst_rs = 1; J.dst_cs = tr; J.dtype = dt;
+                jobs.push_back(J);
+                elems += (long long)(tr - nrows) * g.nc;
+            }
+            bytes = align16(bytes + (size_t)tr * g.nc * sz);
+            T0 += tr;
+        }
+        q0 = q1;
+    }
+    std::stable_sort(P.it1.begin(), P.it1.end(), [](const Item1 &a, const Item1 &b) {
+        return (long long)a.ncol * a.tr * dtype_size(a.dtype) > (long long)b.ncol * b.tr * dtype_size(b.dtype);
+    });
     // ---- phase 2 layout: TR x ncol column-major segments, grouped by (tile, dtype)
     std::vector<std::vector<int>> tl(nt);
     for (int ip = 0; ip < (int)L.size(); ip++) {
@@ -649,6 +812,8 @@ static void build_plan(hbi_hmat *h, int prec)
         if (f.ltmtx == 1 && f.kt == 0) continue;
         for (int t = f.r0 / TR; t <= (f.r0 + f.nr - 1) / TR; t++) tl[t].push_back(ip);
     }
+    // dense segments first: they form the work items that need no z
+    for (auto &v : tl) std::stable_partition(v.begin(), v.end(), [&](int ip) { return L[ip].ltmtx != 1; });
     for (int t = 0; t < nt; t++) {
         const int T0 = t * TR;
         for (int dt = 0; dt < 3; dt++) {
@@ -676,7 +841,7 @@ static void build_plan(hbi_hmat *h, int prec)
                     J.which = 0; J.src = h->fac_off[ip] + (long long)(lo - f.r0) * f.nc;
                     J.src_rs = f.nc; J.src_cs = 1;
                     P.alg_bytes += (long long)(hi - lo) * f.nc * sz;
-                    P.alg_bytes_p2 += (long long)(hi - lo) * f.nc * sz;
+                    P.alg_bytes_p1 += (long long)(hi - lo) * f.nc * sz;
                 }
                 jobs.push_back(J);
                 elems += (long long)TR * ncol;
@@ -687,8 +852,8 @@ static void build_plan(hbi_hmat *h, int prec)
         }
     }
     P.alg_bytes += 2LL * h->n * 8;   // x read, y written (FP64)
-    P.alg_bytes_p1 += (long long)h->n * 8 + (long long)h->nz * 8;
-    P.alg_bytes_p2 += 2LL * h->n * 8;
+    P.alg_bytes_p1 += (long long)h->n * 8 + (long long)h->nz * 8;   // launch A: + x read, z written
+    P.alg_bytes_p2 += (long long)h->nz * 8 + (long long)h->n * 8;   // launch B: + z read, y written
     for (int ip = 0; ip < (int)L.size(); ip++)
         if (leaf_dtype(ip) == kBF16) P.nbf16++;
     // pack on the device
@@ -703,14 +868,16 @@ static void build_plan(hbi_hmat *h, int prec)
         HBI_HIP(hipStreamSynchronize(h->stream));
     }
     P.d_segs.upload(P.segs, h->stream);
+    P.d_it1.upload(P.it1, h->stream);
+    P.d_zrow.upload(P.zrow, h->stream);
+    P.chunk1 = kTile1Bytes;
     P.built = true;
     // ---- work-item sizes: fixed by the caller, or picked by timing the product
     if (h->opts.chunk_kb > 0) {
-        make_items(h, P, std::max(1024, h->opts.chunk_kb * 1024 / 4), h->opts.chunk_kb * 1024, 4);
+        make_items(h, P, h->opts.chunk_kb * 1024, 4, 4, 8);
         return;
     }
-    const int c1s[] = {4096, 8192, 16384, 32768};
-    const int c2s[] = {16384, 32768, 65536, 131072};
+    const int c2s[] = {16384, 32768, 65536, 131072, 262144};
     HBI_HIP(hipMemsetAsync(h->d_x.p, 0, h->d_x.n * 8, h->stream));
     auto time_it = [&](int reps) {
         for (int w = 0; w < 2; w++) hmat_apply(h, prec, h->d_x.p, h->d_y.p);
@@ -722,23 +889,25 @@ static void build_plan(hbi_hmat *h, int prec)
         HBI_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
         return ms / reps;
     };
-    // phase 2 first (the larger kernel), then phase 1 with the chosen phase-2 size
-    int b1 = c1s[1], b2 = c2s[2], bu = 4;
+    // dense tile items and their loads in flight, then the loads in flight of
+    // the phase-1 and the low-rank items
+    int b2 = c2s[2], bu1 = 4, bu2 = 4, bu3 = 8;
     float best = 1e30f;
     for (int u : {4, 8})
         for (int c2 : c2s) {
-            make_items(h, P, b1, c2, u);
+            make_items(h, P, c2, bu1, u, bu3);
             const float ms = time_it(5);
-            if (ms < best) { best = ms; b2 = c2; bu = u; }
+            if (ms < best) { best = ms; b2 = c2; bu2 = u; }
         }
-    best = 1e30f;
-    for (int c1 : c1s) {
-        make_items(h, P, c1, b2, bu);
-        const float ms = time_it(5);
-        if (ms < best) { best = ms; b1 = c1; }
-    }
-    make_items(h, P, b1, b2, bu);
+    make_items(h, P, b2, 8, bu2, bu3);
+    float ms = time_it(5);
+    if (ms < best) { best = ms; bu1 = 8; }
+    make_items(h, P, b2, bu1, bu2, 16);
+    if (time_it(5) < best) bu3 = 16;
+    make_items(h, P, b2, bu1, bu2, bu3);
 }
+
+size_t hmat_z_size(hbi_hmat *h, int prec) { return z_size(h); }
 
 const Plan &hmat_plan(hbi_hmat *h, int prec)
 {
@@ -747,57 +916,56 @@ const Plan &hmat_plan(hbi_hmat *h, int prec)
     return h->plan[prec];
 }
 
-void hmat_phase1(hbi_hmat *h, int prec, const double *x, double *z, hipStream_t s, const int *gate)
+template <int TR>
+static void launch_a(const Plan &P, const double *x, double *z, double *ypart, hipStream_t s, const int *gate)
+{
+    const int n1 = (int)P.it1.size(), n = n1 + (int)P.it2.size();
+    if (n == 0) return;
+#define HBI_LAUNCH_A(U1, U2)                                                                                   \
+    stream_kernel<TR, U1, U2><<<n, kMvThreads, 0, s>>>(P.d_it1.p, n1, P.d_it2.p, P.d_segs.p, P.d_zrow.p, P.buf.p, \
+                                                       x, z, ypart, gate)
+    if (P.unroll1 == 8 && P.unroll2 == 8) HBI_LAUNCH_A(8, 8);
+    else if (P.unroll1 == 8) HBI_LAUNCH_A(8, 4);
+    else if (P.unroll2 == 8) HBI_LAUNCH_A(4, 8);
+    else HBI_LAUNCH_A(4, 4);
+#undef HBI_LAUNCH_A
+}
+
+template <int TR>
+static void launch_b(const hbi_hmat *h, const Plan &P, const double *z, const double *ypart, double *y,
+                     hipStream_t s, const int *gate)
+{
+    const int n = (int)P.itb.size();
+    if (n == 0) return;
+    if (P.unroll3 == 16)
+        lowrank_kernel<TR, 16><<<n, kMvThreads, 0, s>>>(P.d_itb.p, P.d_zcol.p, P.buf.p, z, ypart, P.d_tile_part0.p,
+                                                        y, h->n, gate);
+    else
+        lowrank_kernel<TR, 8><<<n, kMvThreads, 0, s>>>(P.d_itb.p, P.d_zcol.p, P.buf.p, z, ypart, P.d_tile_part0.p,
+                                                       y, h->n, gate);
+}
+
+void hmat_phase1(hbi_hmat *h, int prec, const double *x, double *z, double *ypart, hipStream_t s, const int *gate)
 {
     const Plan &P = h->plan[prec];
-    const int per = kMvThreads / kWave;
-    for (int d = 0; d < 3; d++) {
-        const int n = P.r1[d][1] - P.r1[d][0];
-        if (n <= 0) continue;
-        const Item1 *it = P.d_it1.p + P.r1[d][0];
-        const int grid = (n + per - 1) / per;
-        if (d == kF64) lrvt_kernel<double><<<grid, kMvThreads, 0, s>>>(it, n, P.buf.p, x, z, gate);
-        else if (d == kF32) lrvt_kernel<float><<<grid, kMvThreads, 0, s>>>(it, n, P.buf.p, x, z, gate);
-        else lrvt_kernel<uint16_t><<<grid, kMvThreads, 0, s>>>(it, n, P.buf.p, x, z, gate);
-    }
+    if (h->tr == 64) launch_a<64>(P, x, z, ypart, s, gate);
+    else launch_a<32>(P, x, z, ypart, s, gate);
 }
 
-template <typename T, int TR>
-static void launch_tiles(const Plan &P, int n, const Item2 *it, const double *x, const double *z, double *ypart,
-                         hipStream_t s, const int *gate)
-{
-    if (P.unroll2 == 8)
-        tile_kernel<T, TR, 8><<<n, kMvThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, x, z, ypart, gate);
-    else
-        tile_kernel<T, TR, 4><<<n, kMvThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, x, z, ypart, gate);
-}
-
-void hmat_phase2(hbi_hmat *h, int prec, const double *x, const double *z, double *ypart, hipStream_t s,
+void hmat_phase2(hbi_hmat *h, int prec, const double *z, const double *ypart, double *y, hipStream_t s,
                  const int *gate)
 {
     const Plan &P = h->plan[prec];
-    for (int d = 0; d < 3; d++) {
-        const int n = P.r2[d][1] - P.r2[d][0];
-        if (n <= 0) continue;
-        const Item2 *it = P.d_it2.p + P.r2[d][0];
-        if (h->tr == 64) {
-            if (d == kF64) launch_tiles<double, 64>(P, n, it, x, z, ypart, s, gate);
-            else if (d == kF32) launch_tiles<float, 64>(P, n, it, x, z, ypart, s, gate);
-            else launch_tiles<uint16_t, 64>(P, n, it, x, z, ypart, s, gate);
-        } else {
-            if (d == kF64) launch_tiles<double, 32>(P, n, it, x, z, ypart, s, gate);
-            else if (d == kF32) launch_tiles<float, 32>(P, n, it, x, z, ypart, s, gate);
-            else launch_tiles<uint16_t, 32>(P, n, it, x, z, ypart, s, gate);
-        }
-    }
+    if (h->tr == 64) launch_b<64>(h, P, z, ypart, y, s, gate);
+    else launch_b<32>(h, P, z, ypart, y, s, gate);
 }
+
+size_t hmat_ypart_size(hbi_hmat *h, int prec) { return (size_t)std::max(hmat_plan(h, prec).nparts, 1) * h->tr; }
 
 void hmat_apply(hbi_hmat *h, int prec, const double *x, double *y)
 {
-    const Plan &P = h->plan[prec];
-    hmat_phase1(h, prec, x, h->d_z.p, h->stream, nullptr);
-    hmat_phase2(h, prec, x, h->d_z.p, h->d_ypart.p, h->stream, nullptr);
-    tile_reduce_kernel<<<(h->n + 255) / 256, 256, 0, h->stream>>>(h->n, h->tr, P.d_tile_part0.p, h->d_ypart.p, y);
+    hmat_phase1(h, prec, x, h->d_z.p, h->d_ypart.p, h->stream, nullptr);
+    hmat_phase2(h, prec, h->d_z.p, h->d_ypart.p, y, h->stream, nullptr);
 }
 
 // ------------------------------------------------------------- geometry --
@@ -1218,18 +1386,15 @@ int hbi_hmat_time_matvec(hbi_hmat *h, int prec, int iters, double *ms)
         float t1 = 0, t2 = 0, t3 = 0;
         for (int it = 0; it < iters; it++) {
             HBI_HIP(hipEventRecord(h->ev[0], h->stream));
-            hbi::hmat_phase1(h, prec, h->d_x.p, h->d_z.p, h->stream, nullptr);
+            hbi::hmat_phase1(h, prec, h->d_x.p, h->d_z.p, h->d_ypart.p, h->stream, nullptr);
             HBI_HIP(hipEventRecord(h->ev[1], h->stream));
-            hbi::hmat_phase2(h, prec, h->d_x.p, h->d_z.p, h->d_ypart.p, h->stream, nullptr);
+            hbi::hmat_phase2(h, prec, h->d_z.p, h->d_ypart.p, h->d_y.p, h->stream, nullptr);
             HBI_HIP(hipEventRecord(h->ev[2], h->stream));
-            hbi::tile_reduce_kernel<<<(h->n + 255) / 256, 256, 0, h->stream>>>(h->n, h->tr, P.d_tile_part0.p,
-                                                                                h->d_ypart.p, h->d_y.p);
-            HBI_HIP(hipEventRecord(h->ev[3], h->stream));
-            HBI_HIP(hipEventSynchronize(h->ev[3]));
+            HBI_HIP(hipEventSynchronize(h->ev[2]));
             float a, b, c;
             HBI_HIP(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
             HBI_HIP(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
-            HBI_HIP(hipEventElapsedTime(&c, h->ev[0], h->ev[3]));
+            HBI_HIP(hipEventElapsedTime(&c, h->ev[0], h->ev[2]));
             t1 += a; t2 += b; t3 += c;
         }
         ms[0] = t1 / iters;

@@ -22,11 +22,25 @@ struct Seg {
 struct Item2 {
     int tile, seg0, seg1, part, dtype, pad;
 };
-// Phase-1 work item: nk rows (of ldt elements) of a1^T of one low-rank block.
-struct Item1 {
-    uint64_t off;
-    int ldt, nc, nk, xoff, zoff, dtype;
+// Low-rank work item of the single-vector product: the low-rank segments of
+// one row tile, for each dtype d ncol[d] contiguous columns from byte offset
+// off[d]; their z indices are zcol[zc0 ..], dtype by dtype.
+struct ItemB {
+    uint64_t off[3];
+    int ncol[3], tile, zc0, pad;
 };
+// Single-vector phase 1: the a1^T rows of the low-rank blocks that share a
+// column cluster and a dtype are stacked and cut into row tiles of tr rows
+// (a power of two, 16 bytes of rows up to 64 rows, at most about kTile1Bytes
+// per tile where the rows allow), each stored column-major (tr values per
+// column, zero padded).  One work item (one workgroup) per tile.
+struct Item1 {
+    uint64_t off;         // byte offset of the tile
+    int ncol, xoff;       // columns, x index of the first column
+    int row0, nrows;      // stacked rows zrow[row0 .. row0 + nrows) -> z indices
+    int tr, dtype;        // rows per column, factor dtype
+};
+constexpr int kTile1Bytes = 64 * 1024;
 // strided FP64 -> dtype copy used to pack factors
 struct CopyJob {
     long long src, dst, start;
@@ -45,20 +59,25 @@ struct SegGroup {
 
 struct Plan {
     bool built = false;
-    std::vector<Row1> rows1;
+    std::vector<Row1> rows1;      // row-major a1^T per block (64-column product)
+    std::vector<int> zrow;        // stacked row -> z index (single-vector phase 1)
     std::vector<SegGroup> groups;
-    int chunk1 = 0, chunk2 = 0;   // work-item sizes in bytes (autotuned)
-    int unroll2 = 4;              // phase-2 loads in flight per lane (autotuned)
-    int r1[3][2] = {}, r2[3][2] = {};   // item ranges per dtype
+    int chunk1 = 0, chunk2 = 0;   // work-item sizes in bytes (phase 2 autotuned)
+    int unroll1 = 4, unroll2 = 4, unroll3 = 8;   // loads in flight per lane: phase 1, dense, low-rank (autotuned)
     DevBuf<uint8_t> buf;
     size_t bytes = 0;
     std::vector<Seg> segs;
     std::vector<Item1> it1;
-    std::vector<Item2> it2;
-    std::vector<int> tile_part0;   // ntiles + 1
+    std::vector<Item2> it2;        // tile items of dense blocks (launch A)
+    std::vector<ItemB> itb;        // one item per tile: its low-rank blocks (launch B)
+    std::vector<int> zcol;         // z index of every low-rank column of itb
+    std::vector<int> tile_part0;   // ntiles + 1: the dense items' slots per tile
     DevBuf<Seg> d_segs;
     DevBuf<Item1> d_it1;
+    DevBuf<int> d_zrow;
     DevBuf<Item2> d_it2;
+    DevBuf<ItemB> d_itb;
+    DevBuf<int> d_zcol;
     DevBuf<int> d_tile_part0;
     int nparts = 0;
     long long alg_bytes = 0;      // whole product: all factor bytes + x read + y written
@@ -90,9 +109,14 @@ struct hbi_hmat {
 namespace hbi {
 // y = H x with device pointers on h->stream (no synchronisation)
 void hmat_apply(hbi_hmat *h, int prec, const double *x, double *y);
-// the two phases separately; ypart receives tile partials (phase 2)
-void hmat_phase1(hbi_hmat *h, int prec, const double *x, double *z, hipStream_t s, const int *gate);
-void hmat_phase2(hbi_hmat *h, int prec, const double *x, const double *z, double *ypart, hipStream_t s,
+// the product in two launches: phase 1 writes the low-rank a1^T x products
+// into z (hmat_z_size doubles) and the dense blocks' tile partials into
+// ypart (slots [tile_part0[t], tile_part0[t + 1]) of tile t); phase 2 adds
+// the low-rank blocks' a2 z to them and writes y
+size_t hmat_z_size(hbi_hmat *h, int prec);
+size_t hmat_ypart_size(hbi_hmat *h, int prec);
+void hmat_phase1(hbi_hmat *h, int prec, const double *x, double *z, double *ypart, hipStream_t s, const int *gate);
+void hmat_phase2(hbi_hmat *h, int prec, const double *z, const double *ypart, double *y, hipStream_t s,
                  const int *gate);
 const Plan &hmat_plan(hbi_hmat *h, int prec);
 // multi-RHS product (RR = 64 columns, row-major N x 64)

@@ -43,8 +43,7 @@ struct RkCtl {
 struct RkArrays {
     const double *a, *b, *dc, *f0, *vpl, *taudot, *sigdot, *area;
     double *y, *ytmp, *yout, *yerr, *k, *vel, *x, *slip, *velc;
-    const double *ypart, *ypart_n;
-    const int *part0, *part0_n;
+    const double *hx, *hx_n;    // H x of the shear (and normal) stress matrices
     double *errpart, *monpart;
     double *hist, *vhist;
     const int *rec;
@@ -112,17 +111,11 @@ __global__ __launch_bounds__(kRkThreads) void rk_stage_kernel(RkPar P, const RkC
     const Tab &T = P.T;
     ErrPair ep{0, 0.0};
     if (i < n) {
-        // H x: tile partial sums in a fixed order
-        const int tr = P.tr;
-        const int t = i / tr, rr = i % tr;
-        double hx = 0.0;
-        for (int p = A.part0[t]; p < A.part0[t + 1]; p++) hx += A.ypart[(size_t)p * tr + rr];
+        const double hx = A.hx[i];
         double gs = P.hdiv != 0.0 ? hx / P.hdiv : hx;
         double gn = 0.0;
         if (P.has_hn) {
-            const int tn = i / P.tr_n, rn = i % P.tr_n;
-            double hy = 0.0;
-            for (int p = A.part0_n[tn]; p < A.part0_n[tn + 1]; p++) hy += A.ypart_n[(size_t)p * P.tr_n + rn];
+            const double hy = A.hx_n[i];
             gn = P.hdiv != 0.0 ? hy / P.hdiv : hy;
         }
         gn = gn + A.sigdot[i];
@@ -369,7 +362,7 @@ struct hbi_rk {
     hbi::RkPar P{};
     int n = 0, nblk = 0;
     hbi::DevBuf<double> a, b, dc, f0, vpl, taudot, sigdot, area;
-    hbi::DevBuf<double> y, ytmp, yout, yerr, k, vel, x, slip, velc, z, zn, ypart, ypart_n, errpart, monpart;
+    hbi::DevBuf<double> y, ytmp, yout, yerr, k, vel, x, slip, velc, z, zn, ypart, ypart_n, hx, hx_n, errpart, monpart;
     hbi::DevBuf<double> hist, vhist;
     hbi::DevBuf<int> rec;
     hbi::DevBuf<hbi::RkCtl> ctl;
@@ -395,9 +388,7 @@ static RkArrays arrays(hbi_rk *r)
     A.sigdot = r->sigdot.p; A.area = r->area.p;
     A.y = r->y.p; A.ytmp = r->ytmp.p; A.yout = r->yout.p; A.yerr = r->yerr.p; A.k = r->k.p; A.vel = r->vel.p;
     A.x = r->x.p; A.slip = r->slip.p; A.velc = r->velc.p;
-    A.ypart = r->ypart.p; A.ypart_n = r->ypart_n.p;
-    A.part0 = hmat_plan(r->hs, r->opts.prec).d_tile_part0.p;
-    A.part0_n = r->hn ? hmat_plan(r->hn, r->opts.prec).d_tile_part0.p : nullptr;
+    A.hx = r->hx.p; A.hx_n = r->hx_n.p;
     A.errpart = r->errpart.p; A.monpart = r->monpart.p;
     A.bins = r->bins.p;
     A.trace = r->trace.p;
@@ -418,11 +409,11 @@ static void launch_stages(hbi_rk *r, int nstages, int mode)
     const int *gate = &r->ctl.p->done;
     rk_stage0_kernel<<<g, kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A, 0);
     for (int s = 0; s < nstages; s++) {
-        hmat_phase1(r->hs, r->opts.prec, r->x.p, r->z.p, r->stream, gate);
-        hmat_phase2(r->hs, r->opts.prec, r->x.p, r->z.p, r->ypart.p, r->stream, gate);
+        hmat_phase1(r->hs, r->opts.prec, r->x.p, r->z.p, r->ypart.p, r->stream, gate);
+        hmat_phase2(r->hs, r->opts.prec, r->z.p, r->ypart.p, r->hx.p, r->stream, gate);
         if (r->hn) {
-            hmat_phase1(r->hn, r->opts.prec, r->x.p, r->zn.p, r->stream, gate);
-            hmat_phase2(r->hn, r->opts.prec, r->x.p, r->zn.p, r->ypart_n.p, r->stream, gate);
+            hmat_phase1(r->hn, r->opts.prec, r->x.p, r->zn.p, r->ypart_n.p, r->stream, gate);
+            hmat_phase2(r->hn, r->opts.prec, r->zn.p, r->ypart_n.p, r->hx_n.p, r->stream, gate);
         }
         rk_stage_kernel<<<g, kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A, s, mode);
     }
@@ -499,13 +490,13 @@ int hbi_rk_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_rk_opts *opts, const dou
             HBI_HIP(hipMemsetAsync(r->slip.p, 0, n * 8, r->stream));
             HBI_HIP(hipMemsetAsync(r->velc.p, 0, n * 8, r->stream));
             HBI_HIP(hipMemsetAsync(r->k.p, 0, 6 * m3 * 8, r->stream));
-            const hbi::Plan &Ps = hbi::hmat_plan(hs, opts->prec);
-            r->z.alloc(std::max(hs->nz, 1) + 8);
-            r->ypart.alloc((size_t)std::max(Ps.nparts, 1) * hs->tr);
+            r->z.alloc(hbi::hmat_z_size(hs, opts->prec));
+            r->ypart.alloc(hbi::hmat_ypart_size(hs, opts->prec));
+            r->hx.alloc(n);
             if (hn) {
-                const hbi::Plan &Pn = hbi::hmat_plan(hn, opts->prec);
-                r->zn.alloc(std::max(hn->nz, 1) + 8);
-                r->ypart_n.alloc((size_t)std::max(Pn.nparts, 1) * hn->tr);
+                r->zn.alloc(hbi::hmat_z_size(hn, opts->prec));
+                r->ypart_n.alloc(hbi::hmat_ypart_size(hn, opts->prec));
+                r->hx_n.alloc(n);
             }
             r->errpart.alloc(2 * (size_t)r->nblk);
             r->monpart.alloc(hbi::kMon * (size_t)r->nblk);

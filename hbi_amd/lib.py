@@ -246,7 +246,7 @@ class HMatrix:
     def time_matvec(self, prec=PREC_F64, iters=20):
         ms = np.zeros(3)
         _check(lib().hbi_hmat_time_matvec(self.h, int(prec), int(iters), _p(ms)))
-        return {"lrvt_ms": ms[0], "tile_ms": ms[1], "total_ms": ms[2]}
+        return {"a_ms": ms[0], "b_ms": ms[1], "total_ms": ms[2]}
 
     def time_matmul(self, prec=PREC_F64, iters=20):
         """Device milliseconds of one 64-column product and of its two phases."""

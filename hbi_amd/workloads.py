@@ -45,7 +45,8 @@ def c3(length=60.0, width=30.0, edge=0.134, leaf=64, eps=1e-5, steps=200):
     for prec, name in ((H.PREC_F64, "f64"), (H.PREC_F32, "f32"), (H.PREC_MIXED, "mixed")):
         t = Hm.time_matvec(prec, 20)
         out["matvec"][name] = {"ms": t["total_ms"], "GBps": st["alg_bytes"][prec] / (t["total_ms"] * 1e-3) / 1e9,
-                               "tile_GBps": st["alg_bytes_p2"][prec] / (t["tile_ms"] * 1e-3) / 1e9}
+                               "a_GBps": st["alg_bytes_p1"][prec] / (t["a_ms"] * 1e-3) / 1e9,
+                               "b_GBps": st["alg_bytes_p2"][prec] / (t["b_ms"] * 1e-3) / 1e9}
     for prec, name in ((H.PREC_F64, "f64"), (H.PREC_MIXED, "mixed")):
         t = Hm.time_matmul(prec, 5)
         out["matmul64"][name] = {"ms": t["total_ms"], "per_rhs_vs_matvec": out["matvec"][name]["ms"] * 64 / t["total_ms"]}

@@ -84,8 +84,8 @@ typedef struct hbi_hmat_stats {
     long long lr_dims;           /* sum (ndl+ndt) over low-rank leaves */
     long long stored_bytes[3];   /* packed bytes per precision (incl. padding) */
     long long alg_bytes[3];      /* algorithmic bytes per single-vector product per precision */
-    long long alg_bytes_p1[3];   /* of which the low-rank V^T x kernel (factors + x + z) */
-    long long alg_bytes_p2[3];   /* of which the row-tile kernel (factors + x + y) */
+    long long alg_bytes_p1[3];   /* first launch: a1 and dense factors + x read + z written */
+    long long alg_bytes_p2[3];   /* second launch: a2 factors + z read + y written */
     int chunk_p1[3], chunk_p2[3];   /* work-item bytes per precision chosen by the autotuner */
     int nbf16_blocks;            /* low-rank blocks stored as BF16 in HBI_PREC_MIXED */
     int ncapped;                 /* low-rank blocks whose ACA reached kmax */
@@ -128,9 +128,11 @@ int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs, int prec)
    data, HIP events on the library stream: ms[0] = phase 1 (a1^T X of the
    low-rank blocks), ms[1] = phase 2 (row tiles), ms[2] = whole product. */
 int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms);
-/* Average device time per call of each matvec kernel over iters products on
-   device-resident vectors, from HIP events on the library stream:
-   ms[0] = low-rank V^T x kernel, ms[1] = row-tile kernel, ms[2] = whole product. */
+/* Average device time per call of the two matvec launches over iters products
+   on device-resident vectors, from HIP events on the library stream:
+   ms[0] = first launch (a1^T x of the stacked low-rank row tiles, and the
+   dense blocks' row tiles), ms[1] = second launch (the low-rank blocks' a2 z
+   per row tile, adding the tile's dense partials into y), ms[2] = whole product. */
 int hbi_hmat_time_matvec(hbi_hmat *h, int prec, int iters, double *ms);
 
 /* Friction / RK45 options (main_LH.f90 variables) */
