@@ -20,6 +20,7 @@
 namespace hbi {
 
 constexpr int kRR = 64;      // right-hand sides per product
+constexpr int kP1Rows = 64;  // a1^T rows per phase-1 item (one wave)
 constexpr int kKS = 32;      // k-slice staged in LDS
 constexpr int kMrThreads = 256;
 
@@ -31,92 +32,154 @@ template <> __device__ __forceinline__ double widen<uint16_t>(uint16_t v) { retu
 // ------------------------------------------------------------- phase 1 --
 struct MItem1 {
     uint64_t off;      // byte offset of the first a1^T row
-    int ldt, kt;       // row stride (elements), rows (<= 64: whole blocks of one column cluster, stacked)
+    int ldt, kt;       // row stride (elements), rows (<= kP1Rows: whole blocks of one column cluster, stacked)
     int t0, tn;        // column chunk [t0, t0+tn)
     int xoff;          // first x row of the column cluster
-    int part;          // partial slot: zpart[part][64][RR]
-    int dtype;
+    int part;          // partial slot zpart[part][kP1Rows][RR], or -1: the item is the rows' whole
+                       // column range and writes Z rows zrow[zr .. zr + kt) directly
+    int dtype, zr;
 };
 struct MBlk1 {         // rows roff .. roff+kt of partial slots [p0, p1) -> z rows zoff .. zoff+kt
     int p0, p1, kt, zoff, roff;
 };
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef uint32_t uvec4 __attribute__((ext_vector_type(4)));
+typedef uint32_t uvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec2 __attribute__((ext_vector_type(2)));
 
-// One workgroup (4 waves) per item: Zpart (kt x 64) = a1^T[rows, t0..t0+tn]
-// X[t0..t0+tn, :] on the FP64 matrix cores (16x16x4 MFMA, lane l: A[l % 16]
-// [l / 16], B[l / 16][l % 16], result q of lane l at D[l / 16 + 4 q][l % 16]).
-// Wave w owns right-hand sides [16 w, 16 w + 16) and the NRB = ceil(kt / 16)
-// row blocks (items are launched grouped by NRB).  KS-deep slices of a1^T (transposed into LDS) and X pass through
-// registers: the next slice's global loads are in flight while the current
-// slice is multiplied.
+// zeros read in place of masked factor values and X rows (never written)
+static __device__ double g_zero_row[kRR];
+
+// 2 consecutive factor values of one a1^T row (4, 8 or 16 bytes, aligned)
+template <typename T> struct Run2;
+template <> struct Run2<double> {
+    dvec2 v;
+    __device__ void load(const double *p) { v = __builtin_nontemporal_load((const dvec2 *)p); }
+    __device__ double get(int e) const { return v[e]; }
+};
+template <> struct Run2<float> {
+    fvec2 v;
+    __device__ void load(const float *p) { v = __builtin_nontemporal_load((const fvec2 *)p); }
+    __device__ double get(int e) const { return (double)v[e]; }
+};
+template <> struct Run2<uint16_t> {
+    uint32_t v;
+    __device__ void load(const uint16_t *p) { v = __builtin_nontemporal_load((const uint32_t *)p); }
+    __device__ double get(int e) const { return (double)__uint_as_float(e ? (v & 0xffff0000u) : (v << 16)); }
+};
+
+// One wave per item: Zpart (kt x 64) = a1^T[rows, t0 .. t0+tn) X[t0 .. t0+tn, :]
+// on the FP64 matrix cores (16x16x4 MFMA; lane l = li + 16 lk supplies
+// A[li][lk] and B[lk][li] and holds D[lk + 4 q][li]).  The columns pass in
+// 8-column steps k; the reduction index of MFMA s (0, 1) of step k is
+// t = t0 + 8 k + 2 lk + s, so each lane reads 2 consecutive a1^T values of its
+// row per row block, and MFMA column n of right-hand-side block j is
+// right-hand side 4 n + j, so each lane reads 32 contiguous bytes of an X
+// row.  No LDS: three register buffers keep the loads of the next two steps
+// in flight while one step is multiplied; two waves per SIMD overlap too.
 template <typename T, int NRB>
-__global__ __launch_bounds__(kMrThreads) void mr_lrvt_kernel(const MItem1 *__restrict__ items,
-                                                             const uint8_t *__restrict__ buf,
-                                                             const double *__restrict__ X, double *__restrict__ zpart,
-                                                             const int *__restrict__ gate)
+__device__ __forceinline__ void mr_p1_wave(const MItem1 &it, const uint8_t *__restrict__ buf,
+                                           const double *__restrict__ X, double *__restrict__ zpart,
+                                           const int *__restrict__ zrow, double *__restrict__ Z)
 {
-    constexpr int KS = 32;
-    constexpr int AP = 64 + 1, XP = kRR + 16;   // odd AP: the transposing stores hit 32 distinct banks
-    constexpr int NA = 64 * KS / kMrThreads, NX = KS * kRR / kMrThreads;
-    __shared__ double as[KS][AP];     // as[t][k] = a1^T[k][t]
-    __shared__ double xs[KS][XP];
-    if (gate && *gate) return;
-    const MItem1 it = items[blockIdx.x];
+    const int lane = threadIdx.x % 64, li = lane % 16, lk = lane / 16;
     const T *a = (const T *)(buf + it.off);
-    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
-    const int li = lane % 16, lk = lane / 16;
-    constexpr int na = 16 * NRB * KS;
-    // raw factor values stay in registers until the LDS store, so their loads
-    // overlap the MFMAs of the previous slice
-    T ra[NA];
-    double rx[NX];
-    auto load = [&](int ts) {
-        const int kn = min(KS, it.tn - ts);
-#pragma unroll
-        for (int u = 0; u < NA; u++) {
-            const int q = tid + u * kMrThreads, k = q / KS, tt = q % KS;
-            ra[u] = (q < na && k < it.kt && tt < kn) ? a[(size_t)k * it.ldt + it.t0 + ts + tt] : (T)0;
-        }
-#pragma unroll
-        for (int u = 0; u < NX; u++) {
-            const int q = tid + u * kMrThreads;
-            rx[u] = q / kRR < kn ? X[(size_t)(it.xoff + it.t0 + ts) * kRR + q] : 0.0;
-        }
+    const int tend = it.t0 + it.tn;
+    struct Buf {
+        Run2<T> a[NRB];
+        dvec2 x[2][2];
     };
-    v4d acc[NRB];
+    Buf b0, b1, b2;
+    // a 2-run never crosses an item boundary (items start at multiples of 32)
+    // and rows are zero-padded up to ldt (a multiple of 8), so one test covers A
+    // Masked loads read the zero row instead: no branches and no selects on
+    // loaded values, so the waits before a step's products cover that step only.
+    auto load = [&](Buf &B, int k) {
+        const int tb = it.t0 + 8 * k + 2 * lk;
+        const bool ok = tb < tend;
 #pragma unroll
-    for (int ib = 0; ib < NRB; ib++) acc[ib] = v4d{0.0, 0.0, 0.0, 0.0};
-    load(0);
-    for (int ts = 0; ts < it.tn; ts += KS) {
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < NA; u++) {
-            const int q = tid + u * kMrThreads;
-            if (q < na) as[q % KS][q / KS] = widen<T>(ra[u]);
+        for (int rb = 0; rb < NRB; rb++) {
+            const int row = 16 * rb + li;
+            B.a[rb].load(ok && row < it.kt ? a + (size_t)row * it.ldt + tb : (const T *)g_zero_row);
         }
 #pragma unroll
-        for (int u = 0; u < NX; u++) {
-            const int q = tid + u * kMrThreads;
-            xs[q / kRR][q % kRR] = rx[u];
+        for (int s = 0; s < 2; s++) {
+            const dvec2 *px = tb + s < tend ? (const dvec2 *)(X + (size_t)(it.xoff + tb + s) * kRR + 4 * li)
+                                            : (const dvec2 *)g_zero_row;
+            B.x[s][0] = px[0];
+            B.x[s][1] = px[1];
         }
-        __syncthreads();
-        if (ts + KS < it.tn) load(ts + KS);
+        __builtin_amdgcn_sched_barrier(0);   // keep each step's loads where they are issued
+    };
+    v4d acc[NRB][4];
 #pragma unroll
-        for (int kk = 0; kk < KS; kk += 4) {
-            const double bf = xs[kk + lk][16 * w + li];
+    for (int rb = 0; rb < NRB; rb++)
 #pragma unroll
-            for (int ib = 0; ib < NRB; ib++)
-                acc[ib] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[kk + lk][16 * ib + li], bf, acc[ib], 0, 0, 0);
-        }
+        for (int j = 0; j < 4; j++) acc[rb][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    auto mul = [&](const Buf &B) {
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int rb = 0; rb < NRB; rb++) {
+                const double av = B.a[rb].get(s);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[rb][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, B.x[s][j >> 1][j & 1], acc[rb][j], 0, 0, 0);
+            }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    // steps past the item's end load zeros: nothing is guarded
+    const int nk = (it.tn + 7) / 8;
+    load(b0, 0);
+    load(b1, 1);
+    int k = 0;
+    for (; k + 3 <= nk; k += 3) {
+        load(b2, k + 2);
+        mul(b0);
+        load(b0, k + 3);
+        mul(b1);
+        load(b1, k + 4);
+        mul(b2);
     }
+    if (k < nk) mul(b0);
+    if (k + 1 < nk) mul(b1);
 #pragma unroll
-    for (int ib = 0; ib < NRB; ib++)
+    for (int rb = 0; rb < NRB; rb++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int k = 16 * ib + lk + 4 * q;
-            if (k < it.kt) zpart[((size_t)it.part * 64 + k) * kRR + 16 * w + li] = acc[ib][q];
+            const int row = 16 * rb + lk + 4 * q;
+            if (row < it.kt) {
+                double *d = it.part < 0 ? Z + (size_t)zrow[it.zr + row] * kRR
+                                        : zpart + ((size_t)it.part * kP1Rows + row) * kRR;
+                dvec2 *dst = (dvec2 *)(d + 4 * li);
+                dst[0] = dvec2{acc[rb][0][q], acc[rb][1][q]};
+                dst[1] = dvec2{acc[rb][2][q], acc[rb][3][q]};
+            }
         }
+}
+
+__global__ __launch_bounds__(64) void mr_lrvt_kernel(const MItem1 *__restrict__ items,
+                                                     const uint8_t *__restrict__ buf, const double *__restrict__ X,
+                                                     double *__restrict__ zpart, const int *__restrict__ zrow,
+                                                     double *__restrict__ Z, const int *__restrict__ gate)
+{
+    if (gate && *gate) return;
+    const MItem1 it = items[blockIdx.x];
+    const int nrb = (it.kt + 15) / 16;
+#define HBI_P1(TT)                                                  \
+    switch (nrb) {                                                  \
+    case 1: mr_p1_wave<TT, 1>(it, buf, X, zpart, zrow, Z); break;            \
+    case 2: mr_p1_wave<TT, 2>(it, buf, X, zpart, zrow, Z); break;            \
+    case 3: mr_p1_wave<TT, 3>(it, buf, X, zpart, zrow, Z); break;            \
+    default: mr_p1_wave<TT, 4>(it, buf, X, zpart, zrow, Z); break;           \
+    }
+    if (it.dtype == kF64) { HBI_P1(double) }
+    else if (it.dtype == kF32) { HBI_P1(float) }
+    else { HBI_P1(uint16_t) }
+#undef HBI_P1
 }
 
 __global__ void mr_zreduce_kernel(const MBlk1 *__restrict__ blks, int nblk, const double *__restrict__ zpart,
@@ -129,7 +192,7 @@ __global__ void mr_zreduce_kernel(const MBlk1 *__restrict__ blks, int nblk, cons
     for (int q = threadIdx.x; q < B.kt * kRR; q += blockDim.x) {
         const int k = q / kRR, rr = q % kRR;
         double s = 0.0;
-        for (int p = B.p0; p < B.p1; p++) s += zpart[((size_t)p * 64 + B.roff + k) * kRR + rr];
+        for (int p = B.p0; p < B.p1; p++) s += zpart[((size_t)p * kP1Rows + B.roff + k) * kRR + rr];
         Z[(size_t)(B.zoff + k) * kRR + rr] = s;
     }
 }
@@ -251,7 +314,6 @@ __global__ void mr_tile_reduce_kernel(int n, int tr, const int *__restrict__ par
 struct MrPlan {
     bool built = false;
     int kc1 = 512, cols2 = 256;    // phase-1 column chunk, phase-2 columns per item (autotuned)
-    int r1[3][5][2] = {};          // phase-1 item range per (dtype, row blocks 1..4)
     int r2[3][2] = {};             // phase-2 item range per dtype
     std::vector<MItem1> it1;
     std::vector<MBlk1> blk1;
@@ -280,69 +342,67 @@ namespace hbi {
 
 // Phase-1 items.  Runs of blocks with the same column cluster and dtype are
 // adjacent in the packed layout (same ldt): whole blocks are packed into row
-// groups of at most 64 rows, each read against its x rows once per column
-// chunk of kc1.  Ranks above 64 are split into 64-row pieces of their own.
+// groups of at most kP1Rows rows, each read against its x rows once per column
+// chunk of kc1.  Ranks above kP1Rows are split into pieces of their own.
 static void mr_items1(const Plan &P, MrPlan &M, int kc1)
 {
     M.it1.clear();
     M.blk1.clear();
+    M.nzpart = 0;
     const auto &R = P.rows1;
+    // stacked row of every rows1 entry (rows1 and the z map follow one order)
+    std::vector<int> srow(R.size() + 1, 0);
+    for (size_t q = 0; q < R.size(); q++) srow[q + 1] = srow[q] + R[q].kt;
     size_t i = 0;
     while (i < R.size()) {
-        struct Piece { uint64_t off; int kt, zoff; };
+        struct Piece { uint64_t off; int kt, zoff, sr; };
         std::vector<Piece> grp;
         int rows = 0;
         const Row1 &f = R[i];
-        if (f.kt > 64) {
-            for (int k0 = 0; k0 < f.kt; k0 += 64)
-                grp.push_back({f.off + (uint64_t)k0 * f.ldt * dtype_size(f.dtype), std::min(64, f.kt - k0), f.zoff + k0});
+        if (f.kt > kP1Rows) {
+            for (int k0 = 0; k0 < f.kt; k0 += kP1Rows)
+                grp.push_back({f.off + (uint64_t)k0 * f.ldt * dtype_size(f.dtype), std::min(kP1Rows, f.kt - k0),
+                               f.zoff + k0, srow[i] + k0});
             i++;
         } else {
             size_t j = i;
-            while (j < R.size() && R[j].xoff == f.xoff && R[j].nc == f.nc && R[j].dtype == f.dtype && R[j].kt <= 64 &&
-                   rows + R[j].kt <= 64 &&
+            while (j < R.size() && R[j].xoff == f.xoff && R[j].nc == f.nc && R[j].dtype == f.dtype &&
+                   R[j].kt <= kP1Rows && rows + R[j].kt <= kP1Rows &&
                    (j == i || R[j].off == R[j - 1].off + (uint64_t)R[j - 1].kt * R[j - 1].ldt * dtype_size(f.dtype))) {
-                grp.push_back({R[j].off, R[j].kt, R[j].zoff});
+                grp.push_back({R[j].off, R[j].kt, R[j].zoff, srow[j]});
                 rows += R[j].kt;
                 j++;
             }
             i = j;
         }
         // one packed run per group, except split pieces: each piece is its own run
-        const bool split = f.kt > 64;
+        const bool split = f.kt > kP1Rows;
         for (size_t g0 = 0; g0 < grp.size();) {
             const size_t g1 = split ? g0 + 1 : grp.size();
             int kt = 0;
             for (size_t g = g0; g < g1; g++) kt += grp[g].kt;
-            const int p0 = (int)M.it1.size();
+            const bool direct = f.nc <= kc1;
+            const int p0 = M.nzpart;
             for (int t0 = 0; t0 < f.nc; t0 += kc1) {
                 MItem1 it{};
                 it.off = grp[g0].off; it.ldt = f.ldt; it.kt = kt; it.t0 = t0; it.tn = std::min(kc1, f.nc - t0);
-                it.xoff = f.xoff; it.part = (int)M.it1.size(); it.dtype = f.dtype;
+                it.xoff = f.xoff; it.part = direct ? -1 : M.nzpart++; it.dtype = f.dtype; it.zr = grp[g0].sr;
                 M.it1.push_back(it);
             }
-            const int p1 = (int)M.it1.size();
-            int roff = 0;
-            for (size_t g = g0; g < g1; g++) {
-                M.blk1.push_back({p0, p1, grp[g].kt, grp[g].zoff, roff});
-                roff += grp[g].kt;
+            if (!direct) {
+                int roff = 0;
+                for (size_t g = g0; g < g1; g++) {
+                    M.blk1.push_back({p0, M.nzpart, grp[g].kt, grp[g].zoff, roff});
+                    roff += grp[g].kt;
+                }
             }
             g0 = g1;
         }
     }
-    M.nzpart = (int)M.it1.size();
-    // launch order: grouped by (dtype, row blocks); partial slots keep their numbers
+    // launch order: most MFMA work first; partial slots keep their numbers
     std::stable_sort(M.it1.begin(), M.it1.end(), [](const MItem1 &x, const MItem1 &y) {
-        if (x.dtype != y.dtype) return x.dtype < y.dtype;
-        return (x.kt + 15) / 16 < (y.kt + 15) / 16;
+        return (x.kt + 15) / 16 * ((x.tn + 31) / 32) > (y.kt + 15) / 16 * ((y.tn + 31) / 32);
     });
-    for (int d = 0; d < 3; d++)
-        for (int b = 0; b <= 4; b++) M.r1[d][b][0] = M.r1[d][b][1] = 0;
-    for (int q = 0; q < (int)M.it1.size(); q++) {
-        const int d = M.it1[q].dtype, b = (M.it1[q].kt + 15) / 16;
-        if (M.r1[d][b][1] == 0) M.r1[d][b][0] = q;
-        M.r1[d][b][1] = q + 1;
-    }
     M.kc1 = kc1;
 }
 
@@ -427,7 +487,7 @@ void mr_forget(const hbi_hmat *h)
 }
 
 // buffer sizes cover every candidate of the tuner
-static size_t zpart_cap(const MrPlan &M) { return (size_t)std::max(M.nzpart, 1) * 64 * kRR; }
+static size_t zpart_cap(const MrPlan &M) { return (size_t)std::max(M.nzpart, 1) * kP1Rows * kRR; }
 static size_t ypart_cap(const hbi_hmat *h, const MrPlan &M) { return (size_t)std::max(M.nparts, 1) * h->tr * kRR; }
 size_t mr_zpart_size(hbi_hmat *h, int prec) { return zpart_cap(mr_plan(h, prec)); }
 size_t mr_ypart_size(hbi_hmat *h, int prec) { return ypart_cap(h, mr_plan(h, prec)); }
@@ -438,26 +498,9 @@ static void phase1(const Plan &P, const MrPlan &M, const double *X, double *zpar
                    const int *gate)
 {
     if (!M.it1.empty()) {
-        for (int d = 0; d < 3; d++)
-            for (int nb = 1; nb <= 4; nb++) {
-                const int lo = M.r1[d][nb][0], cnt = M.r1[d][nb][1] - lo;
-                if (cnt <= 0) continue;
-                const MItem1 *it = M.d_it1.p + lo;
-#define HBI_MR_LRVT(TT, NB) mr_lrvt_kernel<TT, NB><<<cnt, kMrThreads, 0, s>>>(it, P.buf.p, X, zpart, gate)
-#define HBI_MR_LRVT_NB(TT)                         \
-    switch (nb) {                                  \
-    case 1: HBI_MR_LRVT(TT, 1); break;             \
-    case 2: HBI_MR_LRVT(TT, 2); break;             \
-    case 3: HBI_MR_LRVT(TT, 3); break;             \
-    default: HBI_MR_LRVT(TT, 4); break;            \
-    }
-                if (d == kF64) { HBI_MR_LRVT_NB(double) }
-                else if (d == kF32) { HBI_MR_LRVT_NB(float) }
-                else { HBI_MR_LRVT_NB(uint16_t) }
-#undef HBI_MR_LRVT_NB
-#undef HBI_MR_LRVT
-            }
-        mr_zreduce_kernel<<<(int)M.blk1.size(), 256, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(), zpart, Z, gate);
+        mr_lrvt_kernel<<<(int)M.it1.size(), 64, 0, s>>>(M.d_it1.p, P.buf.p, X, zpart, P.d_zrow.p, Z, gate);
+        if (!M.blk1.empty())
+            mr_zreduce_kernel<<<(int)M.blk1.size(), 256, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(), zpart, Z, gate);
     }
 }
 
@@ -516,7 +559,7 @@ namespace hbi {
 static void mr_tune(hbi_hmat *h, int prec, MrPlan &M)
 {
     const Plan &P = hmat_plan(h, prec);
-    const int kc1s[] = {256, 512, 1024};
+    const int kc1s[] = {64, 128, 256, 512, 1024};
     const int c2s[] = {128, 256, 512, 1024};
     // buffers sized for the smallest chunks (most partial slots)
     mr_items1(P, M, kc1s[0]);
