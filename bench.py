@@ -87,6 +87,8 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
                     "p2_TFs": p2_entries * 64 * 2 / (t["p2_ms"] * 1e-3) / 1e12,
                     "per_rhs_vs_matvec": mv[name]["ms"] * 64 / t["total_ms"]}
     out["matmul64"] = mm
+    # what the FP64 matrix cores sustain here for this instruction (no memory traffic)
+    out["mfma_f64_loop_tfs"] = {str(w): round(H.mfma_f64_rate(w), 2) for w in (1, 2, 4)}
 
     # single runs: FP64 (the plain GPU form) and mixed, with slip-rate histories
     fr = P.friction_c2(g, IMAX, JMAX, DS0, seed=1)
@@ -238,7 +240,9 @@ def main():
                    "n_elements": IMAX * JMAX, "realisations": NREAL},
         "roofline": {"bound": "mfma", "kernel": "mr_tile_kernel<float,64> (64-RHS row tiles, FP64 MFMA)",
                      "achieved": round(mmx["p2_TFs"], 2), "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
-                     "frac": round(mmx["p2_TFs"] / FP64_MFMA_PEAK_TFS, 4), "traffic": None},
+                     "frac": round(mmx["p2_TFs"] / FP64_MFMA_PEAK_TFS, 4), "traffic": None,
+                     "measured_loop_ceiling": max(r["mfma_f64_loop_tfs"].values()),
+                     "frac_of_measured_ceiling": round(mmx["p2_TFs"] / max(r["mfma_f64_loop_tfs"].values()), 4)},
         "roofline_matvec": {"bound": "hbm", "kernel": "stream_kernel<64> (single-vector product, first launch: "
                                                       "stacked a1^T tiles + dense tiles, mixed)",
                             "achieved": round(mvx["a_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace hbi {
 
@@ -42,8 +43,13 @@ struct EnsCtl {           // per realisation
     int nrjct[kR], trace_n[kR];
 };
 
+// friction / loading arrays: per element and realisation (i * 64 + r), or,
+// when all realisations share the array, once per element (i); bit k of
+// `shared` is set for array k in the order a, b, dc, f0, vpl, taudot, sigdot
+enum { kPa, kPb, kPdc, kPf0, kPvpl, kPtaudot, kPsigdot, kNPar };
 struct EnsArrays {
     const double *a, *b, *dc, *f0, *vpl, *taudot, *sigdot;
+    int shared;
     double *y, *ytmp, *yout, *k, *vel, *x, *slip, *velc, *errpart, *monpart;
     const double *ypart;
     const int *part0;
@@ -52,6 +58,11 @@ struct EnsArrays {
     int trace_cap;
     MlpDev mlp;
 };
+
+__device__ __forceinline__ double prm(const EnsArrays &A, const double *p, int k, int i, int r)
+{
+    return (A.shared >> k & 1) ? p[i] : p[(size_t)i * kR + r];
+}
 
 // thread (i, r): r = threadIdx.x % 64, i = blockIdx.x * 4 + threadIdx.x / 64
 __global__ void ens_stage0_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A)
@@ -64,9 +75,9 @@ __global__ void ens_stage0_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArr
     if (r >= P.nreal) { A.x[q] = 0.0; return; }
     const double *y = A.y;
     const double v = slip_rate(y[(3 * (size_t)i) * kR + r], y[(3 * (size_t)i + 1) * kR + r],
-                                 y[(3 * (size_t)i + 2) * kR + r], A.a[q], P.vref);
+                                 y[(3 * (size_t)i + 2) * kR + r], prm(A, A.a, kPa, i, r), P.vref);
     A.vel[q] = v;
-    A.x[q] = P.backslip ? v - A.vpl[q] : v;
+    A.x[q] = P.backslip ? v - prm(A, A.vpl, kPvpl, i, r) : v;
 }
 
 struct EPair {
@@ -94,12 +105,14 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(EnsPar P, const EnsCtl *
         for (int p = A.part0[t]; p < A.part0[t + 1]; p++) hx += A.ypart[((size_t)p * tr + ro) * kR + r];
         double gs = P.hdiv != 0.0 ? hx / P.hdiv : hx;
         double gn = 0.0;
-        gn = gn + A.sigdot[q];
-        gs = gs + A.taudot[q];
+        gn = gn + prm(A, A.sigdot, kPsigdot, i, r);
+        gs = gs + prm(A, A.taudot, kPtaudot, i, r);
         const double *ys = (s == 0) ? A.y : A.ytmp;
         const size_t e0 = (3 * (size_t)i) * kR + r;
         double kk[3];
-        deriv(P, gs, gn, ys[e0], ys[e0 + kR], ys[e0 + 2 * kR], A.vel[q], A.a[q], A.b[q], A.dc[q], A.f0[q], kk);
+        const double ai = prm(A, A.a, kPa, i, r);
+        deriv(P, gs, gn, ys[e0], ys[e0 + kR], ys[e0 + 2 * kR], A.vel[q], ai, prm(A, A.b, kPb, i, r),
+              prm(A, A.dc, kPdc, i, r), prm(A, A.f0, kPf0, i, r), kk);
         const size_t m3 = 3 * (size_t)P.n * kR;
         double *ks = A.k + s * m3;
         ks[e0] = kk[0];
@@ -124,9 +137,9 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(EnsPar P, const EnsCtl *
                 }
                 A.ytmp[e] = yt[cc];
             }
-            const double v = slip_rate(yt[0], yt[1], yt[2], A.a[q], P.vref);
+            const double v = slip_rate(yt[0], yt[1], yt[2], ai, P.vref);
             A.vel[q] = v;
-            A.x[q] = P.backslip ? v - A.vpl[q] : v;
+            A.x[q] = P.backslip ? v - prm(A, A.vpl, kPvpl, i, r) : v;
         } else {
             for (int cc = 0; cc < 3; cc++) {
                 const size_t e = e0 + cc * kR;
@@ -265,7 +278,7 @@ __global__ __launch_bounds__(256) void ens_commit_kernel(EnsPar P, const EnsCtl 
             for (int cc = 0; cc < 3; cc++) A.y[e0 + cc * kR] = A.yout[e0 + cc * kR];
             double sl = A.slip[q];
             sl = sl + A.velc[q] * dt * 0.5;
-            const double v = slip_rate(A.y[e0], A.y[e0 + kR], A.y[e0 + 2 * kR], A.a[q], P.vref);
+            const double v = slip_rate(A.y[e0], A.y[e0 + kR], A.y[e0 + 2 * kR], prm(A, A.a, kPa, i, r), P.vref);
             sl = sl + v * dt * 0.5;
             A.slip[q] = sl;
             A.velc[q] = v;
@@ -379,6 +392,7 @@ struct hbi_ens {
     hbi_rk_opts opts{};
     hbi::EnsPar P{};
     int n = 0, nreal = 0, nblk = 0;
+    int shared = 0;   // arrays kept once per element (bit k: EnsArrays order)
     hbi::DevBuf<double> a, b, dc, f0, vpl, taudot, sigdot;
     hbi::DevBuf<double> y, ytmp, yout, k, vel, x, slip, velc, errpart, monpart, z, zpart, ypart;
     hbi::DevBuf<double> fold;   // first-level reduction partials (kFold x 2 x 64)
@@ -400,6 +414,7 @@ namespace hbi {
 static EnsArrays e_arrays(hbi_ens *E)
 {
     EnsArrays A{};
+    A.shared = E->shared;
     A.a = E->a.p; A.b = E->b.p; A.dc = E->dc.p; A.f0 = E->f0.p; A.vpl = E->vpl.p; A.taudot = E->taudot.p;
     A.sigdot = E->sigdot.p; A.y = E->y.p; A.ytmp = E->ytmp.p; A.yout = E->yout.p; A.k = E->k.p; A.vel = E->vel.p;
     A.x = E->x.p; A.slip = E->slip.p; A.velc = E->velc.p; A.errpart = E->errpart.p; A.monpart = E->monpart.p;
@@ -482,15 +497,24 @@ int hbi_ens_create(hbi_hmat *hs, const hbi_rk_opts *opts, int nreal, const doubl
             P.dtmax = opts->dtmax; P.hdiv = opts->hdiv; P.eps = opts->eps_r; P.velmin = opts->velmin;
             P.velmax = opts->velmax;
             P.T = hbi::rk_tableau();
-            // unused realisation slots: a = 1 keeps their (never used) slip rates finite
+            // an array equal for every realisation is kept once per element;
+            // unused realisation slots are never evaluated (a = 1 pads the rest)
             const double one = 1.0, zero = 0.0;
-            hbi::upload_rm(E->a, a, nreal, n, 1, &one, E->stream);
-            hbi::upload_rm(E->b, b, nreal, n, 1, &zero, E->stream);
-            hbi::upload_rm(E->dc, dc, nreal, n, 1, &one, E->stream);
-            hbi::upload_rm(E->f0, f0, nreal, n, 1, &zero, E->stream);
-            hbi::upload_rm(E->vpl, vpl, nreal, n, 1, &zero, E->stream);
-            hbi::upload_rm(E->taudot, taudot, nreal, n, 1, &zero, E->stream);
-            hbi::upload_rm(E->sigdot, sigdot, nreal, n, 1, &zero, E->stream);
+            const double *src[hbi::kNPar] = {a, b, dc, f0, vpl, taudot, sigdot};
+            hbi::DevBuf<double> *dst[hbi::kNPar] = {&E->a, &E->b, &E->dc, &E->f0, &E->vpl, &E->taudot, &E->sigdot};
+            E->shared = 0;
+            for (int k = 0; k < hbi::kNPar; k++) {
+                bool same = true;
+                for (int r = 1; r < nreal && same; r++)
+                    same = std::memcmp(src[k], src[k] + (size_t)r * n, (size_t)n * sizeof(double)) == 0;
+                if (same) {
+                    dst[k]->upload(src[k], n, E->stream);
+                    E->shared |= 1 << k;
+                } else {
+                    hbi::upload_rm(*dst[k], src[k], nreal, n, 1, (k == hbi::kPa || k == hbi::kPdc) ? &one : &zero,
+                                   E->stream);
+                }
+            }
             const size_t m = (size_t)n * hbi::kR;
             E->y.alloc(3 * m); E->ytmp.alloc(3 * m); E->yout.alloc(3 * m); E->k.alloc(18 * m);
             E->vel.alloc(m); E->x.alloc(m); E->slip.alloc(m); E->velc.alloc(m);

@@ -560,25 +560,31 @@ static void mr_tune(hbi_hmat *h, int prec, MrPlan &M)
 {
     const Plan &P = hmat_plan(h, prec);
     const int kc1s[] = {64, 128, 256, 512, 1024};
-    const int c2s[] = {128, 256, 512, 1024};
+    const int c2s[] = {128, 256, 512, 1024, 2048};
     // buffers sized for the smallest chunks (most partial slots)
     mr_items1(P, M, kc1s[0]);
     mr_items2(h, P, M, c2s[0]);
     DevBuf<double> dX((size_t)h->n * kRR), dZ((size_t)std::max(h->nz, 1) * kRR);
-    DevBuf<double> zp(zpart_cap(M)), yp(ypart_cap(h, M));
+    DevBuf<double> zp(zpart_cap(M)), yp(ypart_cap(h, M)), dY((size_t)h->n * kRR);
     HBI_HIP(hipMemsetAsync(dX.p, 0, dX.n * 8, h->stream));
     HBI_HIP(hipMemsetAsync(dZ.p, 0, dZ.n * 8, h->stream));
     auto time_it = [&](int which) {
         mr_upload(h, M);
-        for (int w = 0; w < 2; w++) {
-            if (which == 1) phase1(P, M, dX.p, zp.p, dZ.p, h->stream, nullptr);
-            else phase2(h, P, M, dX.p, dZ.p, yp.p, h->stream, nullptr);
-        }
+        // phase 2 is timed with the reduction of its partial slots, which every
+        // consumer pays (fewer, larger items: fewer slots to read)
+        auto run = [&] {
+            if (which == 1) {
+                phase1(P, M, dX.p, zp.p, dZ.p, h->stream, nullptr);
+            } else {
+                phase2(h, P, M, dX.p, dZ.p, yp.p, h->stream, nullptr);
+                const size_t tot = (size_t)h->n * kRR;
+                mr_tile_reduce_kernel<<<(int)((tot + 255) / 256), 256, 0, h->stream>>>(h->n, h->tr, M.d_part0.p,
+                                                                                        yp.p, dY.p);
+            }
+        };
+        for (int w = 0; w < 2; w++) run();
         HBI_HIP(hipEventRecord(h->ev[0], h->stream));
-        for (int r = 0; r < 5; r++) {
-            if (which == 1) phase1(P, M, dX.p, zp.p, dZ.p, h->stream, nullptr);
-            else phase2(h, P, M, dX.p, dZ.p, yp.p, h->stream, nullptr);
-        }
+        for (int r = 0; r < 5; r++) run();
         HBI_HIP(hipEventRecord(h->ev[1], h->stream));
         HBI_HIP(hipEventSynchronize(h->ev[1]));
         float ms = 0;
@@ -663,5 +669,56 @@ extern "C" int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms
         ms[0] = timed(0);
         ms[1] = timed(1);
         ms[2] = timed(2);
+    });
+}
+
+// Rate of the FP64 matrix cores on this device for the instruction both
+// 64-column kernels use: every wave runs iters rounds of 8 independent
+// v_mfma_f64_16x16x4f64 chains (no memory traffic), waves_per_simd waves on
+// each SIMD.  The ceiling the products are compared with beside the data-sheet
+// peak.
+namespace {
+__global__ __launch_bounds__(64) void mfma_rate_kernel(double *out, int iters, double a0)
+{
+    using hbi::v4d;
+    v4d acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
+    const double a = a0 + threadIdx.x, b = 1.0 - a0 * threadIdx.x;
+    for (int it = 0; it < iters; it++)
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) s += acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3];
+    out[(size_t)blockIdx.x * 64 + threadIdx.x] = s;
+}
+}  // namespace
+
+extern "C" int hbi_mfma_f64_rate(int waves_per_simd, int iters, double *tflops)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(tflops && waves_per_simd > 0 && waves_per_simd <= 8 && iters > 0, "bad arguments");
+        int dev = 0, ncu = 0;
+        HBI_HIP(hipGetDevice(&dev));
+        HBI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const int nwaves = ncu * 4 * waves_per_simd;
+        hbi::DevBuf<double> out((size_t)nwaves * 64);
+        hipStream_t s = nullptr;
+        HBI_HIP(hipStreamCreate(&s));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HBI_HIP(hipEventCreate(&e0));
+        HBI_HIP(hipEventCreate(&e1));
+        mfma_rate_kernel<<<nwaves, 64, 0, s>>>(out.p, iters, 0.5);
+        HBI_HIP(hipEventRecord(e0, s));
+        mfma_rate_kernel<<<nwaves, 64, 0, s>>>(out.p, iters, 0.5);
+        HBI_HIP(hipEventRecord(e1, s));
+        HBI_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        HBI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+        *tflops = (double)nwaves * iters * 8 * 2048.0 / (ms * 1e-3) / 1e12;
     });
 }

@@ -128,6 +128,12 @@ int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs, int prec)
    data, HIP events on the library stream: ms[0] = phase 1 (a1^T X of the
    low-rank blocks), ms[1] = phase 2 (row tiles), ms[2] = whole product. */
 int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms);
+/* Measured FP64 matrix-core rate (TFLOP/s) of v_mfma_f64_16x16x4f64, the
+   instruction of both 64-column kernels: waves_per_simd (1..8) waves per SIMD
+   each run iters rounds of 8 independent chains, no memory traffic.  The
+   ceiling the 64-column products are set against beside the data-sheet peak;
+   no reference counterpart. */
+int hbi_mfma_f64_rate(int waves_per_simd, int iters, double *tflops);
 /* Average device time per call of the two matvec launches over iters products
    on device-resident vectors, from HIP events on the library stream:
    ms[0] = first launch (a1^T x of the stacked low-rank row tiles, and the
