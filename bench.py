@@ -38,6 +38,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
+# HBM bytes per launch of the two roofline kernels (mixed precision), from
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE passes over
+# hbi_amd/profile_kernels.py, scaled by the counters of a 256 MiB device copy
+# (fetch x 2.00, write x 1.00)
+TRAFFIC_MR_TILE_MIXED = 623.5e6
+TRAFFIC_STREAM_MIXED = 225.7e6
 IMAX, JMAX, DS0, LEAF, EPS_H, EPS_R = 256, 128, 0.390625, 64, 1e-5, 1e-6
 NREAL = 64
 
@@ -240,13 +246,13 @@ def main():
                    "n_elements": IMAX * JMAX, "realisations": NREAL},
         "roofline": {"bound": "mfma", "kernel": "mr_tile_kernel<float,64> (64-RHS row tiles, FP64 MFMA)",
                      "achieved": round(mmx["p2_TFs"], 2), "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
-                     "frac": round(mmx["p2_TFs"] / FP64_MFMA_PEAK_TFS, 4), "traffic": None,
+                     "frac": round(mmx["p2_TFs"] / FP64_MFMA_PEAK_TFS, 4), "traffic": TRAFFIC_MR_TILE_MIXED,
                      "measured_loop_ceiling": max(r["mfma_f64_loop_tfs"].values()),
                      "frac_of_measured_ceiling": round(mmx["p2_TFs"] / max(r["mfma_f64_loop_tfs"].values()), 4)},
         "roofline_matvec": {"bound": "hbm", "kernel": "stream_kernel<64> (single-vector product, first launch: "
                                                       "stacked a1^T tiles + dense tiles, mixed)",
                             "achieved": round(mvx["a_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(mvx["a_GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                            "frac": round(mvx["a_GBps"] / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_STREAM_MIXED,
                             "whole_product_frac": {k: round(v["GBps"] / HBM_PEAK_GBS, 4)
                                                    for k, v in r["matvec"].items()}},
         "fp64_single_steps_per_s": round(f64, 2),

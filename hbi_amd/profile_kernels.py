@@ -7,8 +7,9 @@ Builds the 256 x 128 half-space fault's lattice H-matrix (leaf 64, ACA eps
 1e-5) and times, with HIP events on the library stream, the single-vector
 product of every precision and the 64-column product (FP64 and mixed) over
 `iters` calls each; the same kernels and work items the RK graphs launch.
-With --copy-mb > 0 it also runs a plain device copy of that many MiB (torch)
-`iters` times, the reference for scaling the profiler's byte counters.
+With --copy-mb > 0 it also runs a plain device-to-device copy of that many
+MiB (HIP runtime blit kernel) `iters` times, the reference for scaling the
+profiler's byte counters.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -43,19 +44,39 @@ def main():
     for prec, name in ((H.PREC_F64, "f64"), (H.PREC_MIXED, "mixed")):
         out["matmul64"][name] = Hm.time_matmul(prec, a.iters)
     if a.copy_mb > 0:
-        import torch
-        n = a.copy_mb * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(a.iters):
-            dst.copy_(src)
-        e1.record()
-        torch.cuda.synchronize()
-        out["copy"] = {"bytes_read": n * 4, "bytes_written": n * 4, "ms": e0.elapsed_time(e1) / a.iters}
+        out["copy"] = device_copy(a.copy_mb * (1 << 20), a.iters)
     print(json.dumps(out), flush=True)
+
+
+def device_copy(nbytes: int, iters: int) -> dict:
+    """iters device-to-device copies of nbytes through the HIP runtime (its
+    blit kernel), timed with HIP events: the reference for scaling byte counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    vp = C.c_void_p
+
+    def ok(e):
+        if e != 0:
+            raise RuntimeError("HIP error %d" % e)
+    src, dst, e0, e1 = vp(), vp(), vp(), vp()
+    ok(hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)))
+    ok(hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)))
+    ok(hip.hipMemset(src, 1, C.c_size_t(nbytes)))
+    ok(hip.hipEventCreate(C.byref(e0)))
+    ok(hip.hipEventCreate(C.byref(e1)))
+    kind_d2d = 3
+    ok(hip.hipMemcpy(dst, src, C.c_size_t(nbytes), kind_d2d))
+    ok(hip.hipDeviceSynchronize())
+    ok(hip.hipEventRecord(e0, None))
+    for _ in range(iters):
+        ok(hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), kind_d2d, None))
+    ok(hip.hipEventRecord(e1, None))
+    ok(hip.hipEventSynchronize(e1))
+    ms = C.c_float()
+    ok(hip.hipEventElapsedTime(C.byref(ms), e0, e1))
+    for b in (src, dst):
+        ok(hip.hipFree(b))
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms.value / iters}
 
 
 if __name__ == "__main__":
