@@ -29,6 +29,10 @@ struct RkPar {
     Tab T;
     int learn, use_mlp;                   // step features on; MLP proposals on
     double safety, ratio_lo, ratio_hi;    // learned proposal: factor and bounds relative to rkqs
+    // outputs (snapshots, event list): on, interval, buffer sizes, step offset, thresholds
+    int out_on, interval, snap_cap, ev_cap;
+    long long k0;
+    double velth, dtout_inter, dtout_co;
 };
 
 struct RkCtl {
@@ -38,6 +42,12 @@ struct RkCtl {
     // learned step size: the previous accepted step; trace rows written
     StepMem mem;
     int trace_n;
+    // outputs (main_LH.f90:1339-1407): event detector state, the decisions
+    // of the last accepted step (pending until the next attempt), buffer fill
+    int slipping, hypoloc, out_pending, out_now, onset_now, end_now, snap_slot, ev_slot;
+    int nsnap, nev, out_full;
+    long long eventcount;
+    double moment0, onset_time, tout, dtout;
 };
 
 struct RkArrays {
@@ -52,12 +62,18 @@ struct RkArrays {
     double *trace;        // rows of 12 features + (h, errmax)
     int trace_cap;
     MlpDev mlp;
+    // outputs: snapshots (slot x [vel, slip, sigma, tau] x n), their step and
+    // time; slip at the event onset; finished events and their coseismic slip
+    double *snap, *snap_t, *islip, *eqslip;
+    long long *snap_k;
+    hbi_rk_event *events;
 };
 
 
 constexpr int kRkThreads = 256;
-// per-block monitor partials: max |V|, sum |slip| area, sum (tau / sigma) area, max sigma, min sigma
-constexpr int kMon = 5;
+// per-block monitor partials: max |V|, sum |slip| area, sum (tau / sigma) area, max sigma, min sigma,
+// the index of the first max |V|
+constexpr int kMon = 6;
 // history row per accepted step (the quantities of main_LH.f90:1491):
 // t, dt, errmax, rejected attempts before the step, log10 max |V|, mean |slip|,
 // mean tau / sigma (area weighted), max sigma, min sigma
@@ -190,6 +206,7 @@ __global__ void rk_decide_kernel(RkPar P, RkCtl *c, const double *__restrict__ e
 {
 #pragma clang fp contract(off)
     if (c->done || threadIdx.x != 0) return;
+    c->out_pending = 0;
     const double SAFETY = (double)0.9f, PGROW = (double)-0.2f, PSHRNK = (double)-0.25f, tiny = (double)1e-20f;
     ErrPair e{0, 0.0};
     for (int b = 0; b < nblk; b++) e = err_combine(e, ErrPair{(int)errpart[2 * b], errpart[2 * b + 1]});
@@ -228,8 +245,9 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
     if (c->done || !c->acc) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const double dt = c->hdid;
-    // max |V|, sum |slip| area, sum mu area, max sigma, min sigma (identities for i >= n)
-    double m[kMon] = {0.0, 0.0, 0.0, -HUGE_VAL, HUGE_VAL};
+    // max |V|, sum |slip| area, sum mu area, max sigma, min sigma, index of the
+    // first max |V| (identities for i >= n; -1 < |V| so element 0 wins ties with it)
+    double m[kMon] = {-1.0, 0.0, 0.0, -HUGE_VAL, HUGE_VAL, -1.0};
     if (i < P.n) {
         for (int q = 0; q < 3; q++) A.y[3 * (size_t)i + q] = A.yout[3 * (size_t)i + q];
         double sl = A.slip[i];
@@ -244,6 +262,7 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
         m[2] = A.y[3 * i + 1] / sig * A.area[i];
         m[3] = sig;
         m[4] = sig;
+        m[5] = (double)i;
         if (P.learn) atomicAdd(&A.bins[vbin(m[0])], 1);
     }
     // per-block combine in a fixed order (sums ordered by lane, then by wave)
@@ -252,7 +271,8 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
         for (int k = 0; k < kMon; k++) o2[k] = __shfl_down(m[k], o, kWave);
         const int lane = threadIdx.x % kWave;
         if ((lane & (2 * o - 1)) == 0 && lane + o < kWave) {
-            m[0] = fmax(m[0], o2[0]);
+            // the higher lanes hold higher indices: take theirs only when larger
+            if (o2[0] > m[0]) { m[0] = o2[0]; m[5] = o2[5]; }
             m[1] = m[1] + o2[1];
             m[2] = m[2] + o2[2];
             m[3] = fmax(m[3], o2[3]);
@@ -267,7 +287,7 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
         double t[kMon];
         for (int k = 0; k < kMon; k++) t[k] = sp[0][k];
         for (int q = 1; q < kRkThreads / kWave; q++) {
-            t[0] = fmax(t[0], sp[q][0]);
+            if (sp[q][0] > t[0]) { t[0] = sp[q][0]; t[5] = sp[q][5]; }
             t[1] = t[1] + sp[q][1];
             t[2] = t[2] + sp[q][2];
             t[3] = fmax(t[3], sp[q][3]);
@@ -316,10 +336,10 @@ __global__ void rk_monitor_kernel(RkPar P, RkCtl *c, RkArrays A, int nblk)
     if (c->done || !c->acc) return;
     __shared__ double vm, msum, mmu, smax, smin;
     if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0, u = 0.0, hi = -HUGE_VAL, lo = HUGE_VAL;
+        double a = -1.0, b = 0.0, u = 0.0, hi = -HUGE_VAL, lo = HUGE_VAL, loc = -1.0;
         for (int q = 0; q < nblk; q++) {
             const double *mp = A.monpart + kMon * (size_t)q;
-            a = fmax(a, mp[0]);
+            if (mp[0] > a) { a = mp[0]; loc = mp[5]; }
             b = b + mp[1];
             u = u + mp[2];
             hi = fmax(hi, mp[3]);
@@ -332,6 +352,47 @@ __global__ void rk_monitor_kernel(RkPar P, RkCtl *c, RkArrays A, int nblk)
         smin = lo;
         c->vmax = a;
         c->meanslip = msum;
+        if (P.out_on) {
+            // main_LH.f90:1339-1407: event start / end, snapshot decision
+            const double t = c->t;
+            const long long k = P.k0 + c->accepted;
+            int out = 0, onset = 0, end = 0;
+            if (!c->slipping && a > P.velth) {
+                out = 1; onset = 1;
+                c->slipping = 1;
+                c->dtout = P.dtout_co;
+                c->tout = t + c->dtout;
+                c->eventcount++;
+                c->moment0 = msum;
+                c->hypoloc = (int)loc + 1;
+                c->onset_time = t;
+            }
+            if (c->slipping && a < 0.5 * P.velth) {
+                out = 1; end = 1;
+                c->slipping = 0;
+                c->dtout = P.dtout_inter;
+                c->tout = t + c->dtout;
+                const double moment = msum - c->moment0;
+                hbi_rk_event ev{};
+                ev.count = c->eventcount; ev.k = k; ev.onset_time = c->onset_time;
+                ev.mw = (log10(moment * kRigid * P.sg) + 5.9) / 1.5;
+                ev.moment = moment;
+                ev.hypoloc = c->hypoloc;
+                c->ev_slot = c->nev;
+                A.events[c->nev++] = ev;
+            }
+            if (P.interval > 0 && k % P.interval == 0) out = 1;
+            if (t > c->tout) { out = 1; c->tout = t + c->dtout; }
+            c->out_now = out; c->onset_now = onset; c->end_now = end;
+            c->out_pending = out | onset | end;
+            if (out) {
+                c->snap_slot = c->nsnap;
+                A.snap_k[c->nsnap] = k;
+                A.snap_t[c->nsnap] = t;
+                c->nsnap++;
+            }
+            if (c->nsnap >= P.snap_cap || c->nev >= P.ev_cap) c->out_full = 1;
+        }
     }
     __syncthreads();
     const long long row = c->accepted - c->hist_base - 1;
@@ -348,8 +409,30 @@ __global__ void rk_monitor_kernel(RkPar P, RkCtl *c, RkArrays A, int nblk)
     if (threadIdx.x == 0) {
         c->nrjct = 0;
         if (vm > P.velmax || vm < P.velmin) { c->status = 1; c->done = 1; }
-        if (c->accepted >= c->target) c->done = 1;
+        if (c->accepted >= c->target || c->out_full) c->done = 1;
     }
+}
+
+// The output decisions of the last accepted step: snapshot of vel, slip,
+// sigma, tau into its slot; slip kept at an event onset; coseismic slip
+// |slip - slip at onset| at an event end.  Rewriting the same values is
+// harmless, so the kernel repeats them until the next attempt clears the flag.
+__global__ void rk_output_kernel(RkPar P, const RkCtl *__restrict__ c, RkArrays A)
+{
+    if (!c->out_pending) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const size_t n = P.n;
+    const double sl = A.slip[i];
+    if (c->out_now) {
+        double *s = A.snap + (size_t)c->snap_slot * 4 * n;
+        s[i] = A.velc[i];
+        s[n + i] = sl;
+        s[2 * n + i] = A.y[3 * (size_t)i + 2];
+        s[3 * n + i] = A.y[3 * (size_t)i + 1];
+    }
+    if (c->onset_now) A.islip[i] = sl;
+    if (c->end_now) A.eqslip[(size_t)c->ev_slot * n + i] = fabs(sl - A.islip[i]);
 }
 
 }  // namespace hbi
@@ -371,6 +454,10 @@ struct hbi_rk {
     int trace_cap = 0;
     hbi::DevBuf<float> mlp_p, mlp_mu, mlp_sd;   // copy of the step-size MLP
     hbi::DevBuf<uint16_t> mlp_w;
+    hbi::DevBuf<double> snap, snap_t, islip, eqslip;   // outputs (hbi_rk_set_output)
+    hbi::DevBuf<long long> snap_k;
+    hbi::DevBuf<hbi_rk_event> events;
+    bool out_full = false;                     // output held on the device fills its buffers
     bool dirty = false;                        // graph arguments changed: recapture
     hipGraphExec_t graph = nullptr;
     int graph_nrec = -1;
@@ -394,6 +481,8 @@ static RkArrays arrays(hbi_rk *r)
     A.trace = r->trace.p;
     A.trace_cap = r->trace_cap;
     A.mlp = MlpDev{r->mlp_w.p, r->mlp_p.p, r->mlp_mu.p, r->mlp_sd.p};
+    A.snap = r->snap.p; A.snap_t = r->snap_t.p; A.snap_k = r->snap_k.p; A.islip = r->islip.p;
+    A.eqslip = r->eqslip.p; A.events = r->events.p;
     A.hist = r->hist.n ? r->hist.p : nullptr;
     A.vhist = r->vhist.n ? r->vhist.p : nullptr;
     A.rec = r->rec.p;
@@ -428,6 +517,7 @@ static void launch_attempt(hbi_rk *r)
     rk_commit_kernel<<<g, kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A);
     if (r->P.learn) rk_learn_kernel<<<1, kHid, 0, r->stream>>>(r->P, r->ctl.p, A);
     rk_monitor_kernel<<<1, 256, 0, r->stream>>>(r->P, r->ctl.p, A, g);
+    if (r->P.out_on) rk_output_kernel<<<g, kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A);
 }
 
 static void capture(hbi_rk *r)
@@ -552,6 +642,10 @@ int hbi_rk_set_state(hbi_rk *r, const double *y, const double *slip, const doubl
         hbi::RkCtl c{};
         c.h = dt_next;
         c.t = t;
+        // a fresh event detector (main_LH.f90:1188-1195); undrained output is dropped
+        c.dtout = r->P.dtout_inter;
+        c.tout = r->P.dtout_inter;
+        r->out_full = false;
         hbi::to_device_sync(r->ctl.p, &c, sizeof(c), r->stream);
         HBI_HIP(hipStreamSynchronize(r->stream));
     });
@@ -614,6 +708,7 @@ namespace hbi {
 static void step_prepare(hbi_rk *r, int nsteps, bool hist, int nrec, const int *rec)
 {
     HBI_REQUIRE(nsteps >= 0, "bad arguments");
+    if (r->out_full) throw Error{HBI_E_STATE, "output buffers full: drain them with hbi_rk_get_output"};
     for (int q = 0; q < nrec; q++) HBI_REQUIRE(rec[q] >= 0 && rec[q] < r->n, "record index out of range");
     const size_t hneed = hist ? kRkHist * (size_t)nsteps : 0;
     const size_t vneed = nrec > 0 ? (size_t)nsteps * nrec : 0;
@@ -645,10 +740,11 @@ static void step_finish(hbi_rk *r, const RkCtl &c, hbi_rk_info *info)
     float ms = 0;
     HBI_HIP(hipEventElapsedTime(&ms, r->e0, r->e1));
     r->last_ms = ms;
+    r->out_full = c.out_full != 0;
     if (info) {
         info->t = c.t; info->dt_next = c.h; info->dt_last = c.hdid; info->errmax_last = c.errmax;
         info->attempts = c.attempts; info->accepted = c.accepted; info->rejected = c.rejected;
-        info->vmax = c.vmax; info->status = c.status;
+        info->vmax = c.vmax; info->status = c.status ? c.status : (c.out_full ? HBI_RK_OUTPUT_FULL : 0);
     }
 }
 
@@ -720,6 +816,74 @@ int hbi_rk_step_many(hbi_rk *const *rs, int m, int nsteps, hbi_rk_info *infos)
         for (int f = 0; f < m; f++) hbi::step_finish(rs[f], c[f], infos ? &infos[f] : nullptr);
         for (int f = 0; f < m; f++)
             if (c[f].status == HBI_E_STEP) throw hbi::Error{HBI_E_STEP, "Runge-Kutta step size underflow"};
+    });
+}
+
+int hbi_rk_set_output(hbi_rk *r, const hbi_rk_out_opts *o)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r, "NULL argument");
+        hbi::RkPar &P = r->P;
+        if (!o) {
+            P.out_on = 0;
+            r->snap.release(); r->snap_t.release(); r->snap_k.release(); r->islip.release();
+            r->eqslip.release(); r->events.release();
+        } else {
+            HBI_REQUIRE(o->capacity >= 1 && o->ev_capacity >= 1 && o->velth > 0 && o->dtout_inter > 0 &&
+                            o->dtout_co > 0 && o->k0 >= 0,
+                        "bad output options");
+            const size_t n = r->n;
+            r->snap.alloc((size_t)o->capacity * 4 * n);
+            r->snap_t.alloc(o->capacity);
+            r->snap_k.alloc(o->capacity);
+            r->islip.alloc(n);
+            r->eqslip.alloc((size_t)o->ev_capacity * n);
+            r->events.alloc(o->ev_capacity);
+            HBI_HIP(hipMemsetAsync(r->islip.p, 0, n * 8, r->stream));
+            P.out_on = 1; P.interval = o->interval; P.snap_cap = o->capacity; P.ev_cap = o->ev_capacity;
+            P.k0 = o->k0; P.velth = o->velth; P.dtout_inter = o->dtout_inter; P.dtout_co = o->dtout_co;
+        }
+        // the detector starts afresh (as a new run would) from the current state
+        hbi::RkCtl c;
+        HBI_HIP(hipMemcpyAsync(&c, r->ctl.p, sizeof(c), hipMemcpyDeviceToHost, r->stream));
+        HBI_HIP(hipStreamSynchronize(r->stream));
+        c.slipping = 0; c.eventcount = 0; c.hypoloc = 0; c.moment0 = 0.0; c.onset_time = 0.0;
+        c.dtout = P.dtout_inter; c.tout = P.dtout_inter;
+        c.out_pending = c.out_now = c.onset_now = c.end_now = 0;
+        c.nsnap = c.nev = c.out_full = 0;
+        hbi::to_device_sync(r->ctl.p, &c, sizeof(c), r->stream);
+        r->out_full = false;
+        r->dirty = true;
+    });
+}
+
+int hbi_rk_get_output(hbi_rk *r, int *nsnap, long long *snap_k, double *snap_t, double *fields, int *nev,
+                      hbi_rk_event *events, double *eqslip)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && r->P.out_on, "outputs not set (hbi_rk_set_output)");
+        hbi::RkCtl c;
+        HBI_HIP(hipMemcpyAsync(&c, r->ctl.p, sizeof(c), hipMemcpyDeviceToHost, r->stream));
+        HBI_HIP(hipStreamSynchronize(r->stream));
+        const size_t n = r->n;
+        if (nsnap) *nsnap = c.nsnap;
+        if (nev) *nev = c.nev;
+        if (c.nsnap > 0) {
+            if (snap_k) r->snap_k.download(snap_k, c.nsnap, r->stream);
+            if (snap_t) r->snap_t.download(snap_t, c.nsnap, r->stream);
+            if (fields) r->snap.download(fields, (size_t)c.nsnap * 4 * n, r->stream);
+        }
+        if (c.nev > 0) {
+            if (events) r->events.download(events, c.nev, r->stream);
+            if (eqslip) r->eqslip.download(eqslip, (size_t)c.nev * n, r->stream);
+        }
+        HBI_HIP(hipStreamSynchronize(r->stream));
+        // the copies are taken: clear the buffers (the flags of the last step
+        // stay, so its output kernel still finds its slot if launched again)
+        c.nsnap = c.nev = c.out_full = 0;
+        c.out_pending = 0;
+        hbi::to_device_sync(r->ctl.p, &c, sizeof(c), r->stream);
+        r->out_full = false;
     });
 }
 

@@ -57,6 +57,19 @@ class RkOpts(C.Structure):
                 ("hdiv", C.c_double), ("eps_r", C.c_double), ("velmin", C.c_double), ("velmax", C.c_double)]
 
 
+RK_OUTPUT_FULL = 2   # hbi_rk_info.status: output buffers full, drain with get_output
+
+
+class RkOutOpts(C.Structure):
+    _fields_ = [("interval", C.c_int), ("capacity", C.c_int), ("ev_capacity", C.c_int), ("pad", C.c_int),
+                ("k0", C.c_longlong), ("velth", C.c_double), ("dtout_inter", C.c_double), ("dtout_co", C.c_double)]
+
+
+class RkEvent(C.Structure):
+    _fields_ = [("count", C.c_longlong), ("k", C.c_longlong), ("onset_time", C.c_double), ("mw", C.c_double),
+                ("moment", C.c_double), ("hypoloc", C.c_int), ("pad", C.c_int)]
+
+
 class RkInfo(C.Structure):
     _fields_ = [("t", C.c_double), ("dt_next", C.c_double), ("dt_last", C.c_double), ("errmax_last", C.c_double),
                 ("attempts", C.c_longlong), ("accepted", C.c_longlong), ("rejected", C.c_longlong),
@@ -81,7 +94,7 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_ens_step", "hbi_ens_last_ms", "hbi_mlp_create", "hbi_mlp_destroy", "hbi_mlp_nparams",
            "hbi_mlp_set_params", "hbi_mlp_get_params", "hbi_mlp_forward", "hbi_mlp_loss_grad", "hbi_mlp_train",
            "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace", "hbi_ens_set_controller",
-           "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_rk_step_many")
+           "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_rk_step_many", "hbi_rk_set_output", "hbi_rk_get_output")
 
 
 def lib():
@@ -133,6 +146,9 @@ def lib():
         L.hbi_ens_set_controller.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.hbi_ens_set_trace.argtypes = [C.c_void_p, C.c_int]
         L.hbi_rk_step_many.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(RkInfo)]
+        L.hbi_rk_set_output.argtypes = [C.c_void_p, C.POINTER(RkOutOpts)]
+        L.hbi_rk_get_output.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), _dp, _dp,
+                                        C.POINTER(C.c_int), C.POINTER(RkEvent), _dp]
         L.hbi_ens_get_trace.argtypes = [C.c_void_p, _dp, _dp, _ip]
         _lib = L
     return _lib
@@ -331,6 +347,36 @@ class RK:
         ms = C.c_double(0)
         _check(lib().hbi_rk_last_ms(self.r, C.byref(ms)))
         return ms.value
+
+    def set_output(self, velth=1e-2, dtout_inter=365 * 24 * 3600.0, dtout_co=1000.0, interval=0, capacity=64,
+                   ev_capacity=16, k0=0, off=False):
+        """Field snapshots and event list decided on the device (main_LH.f90:1339-1407)."""
+        if off:
+            _check(lib().hbi_rk_set_output(self.r, None))
+            self.out_caps = None
+            return
+        o = RkOutOpts(interval=int(interval), capacity=int(capacity), ev_capacity=int(ev_capacity), k0=int(k0),
+                      velth=float(velth), dtout_inter=float(dtout_inter), dtout_co=float(dtout_co))
+        _check(lib().hbi_rk_set_output(self.r, C.byref(o)))
+        self.out_caps = (int(capacity), int(ev_capacity))
+
+    def get_output(self) -> dict:
+        """Drain the held output: snapshot steps, times and fields (vel, slip,
+        sigma, tau; cluster order), finished events and their coseismic slip."""
+        cap, ecap = self.out_caps
+        n = self.n
+        ks = np.zeros(cap, dtype=np.int64)
+        ts = np.zeros(cap)
+        f = np.zeros((cap, 4, n))
+        ev = (RkEvent * ecap)()
+        eq = np.zeros((ecap, n))
+        ns, ne = C.c_int(0), C.c_int(0)
+        _check(lib().hbi_rk_get_output(self.r, C.byref(ns), ks.ctypes.data_as(C.POINTER(C.c_longlong)), _p(ts),
+                                       _p(f), C.byref(ne), ev, _p(eq)))
+        ns, ne = ns.value, ne.value
+        events = [{k: getattr(ev[i], k) for k, _ in RkEvent._fields_ if k != "pad"} for i in range(ne)]
+        return {"k": ks[:ns].copy(), "t": ts[:ns].copy(), "vel": f[:ns, 0].copy(), "slip": f[:ns, 1].copy(),
+                "sigma": f[:ns, 2].copy(), "tau": f[:ns, 3].copy(), "events": events, "eqslip": eq[:ne].copy()}
 
     @staticmethod
     def step_many(runs, nsteps):

@@ -157,8 +157,10 @@ typedef struct hbi_rk_info {
     double t, dt_next, dt_last, errmax_last;
     long long attempts, accepted, rejected;
     double vmax;       /* max |V| after the last accepted step */
-    int status;        /* 0 running, 1 stopped by velmax/velmin, HBI_E_STEP on underflow */
+    int status;        /* 0 running, 1 stopped by velmax/velmin, HBI_E_STEP on underflow,
+                          HBI_RK_OUTPUT_FULL: output buffers full (drain with hbi_rk_get_output) */
 } hbi_rk_info;
+#define HBI_RK_OUTPUT_FULL 2
 
 /* Per-element arrays in cluster order (host pointers). hn may be NULL (sigmaconst). */
 int hbi_rk_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_rk_opts *opts,
@@ -190,6 +192,50 @@ int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, d
 int hbi_rk_step_many(hbi_rk *const *rs, int m, int nsteps, hbi_rk_info *infos);
 /* Device time of the last hbi_rk_step call, milliseconds (HIP events) */
 int hbi_rk_last_ms(const hbi_rk *r, double *ms);
+
+/* Field snapshots and the event list, decided on the device after every
+   accepted step k = k0 + accepted (main_LH.f90:1339-1407):
+   - event detector: an event starts when max |V| > velth (onset time, the
+     mean slip and the slip field are kept, hypoloc = 1-based cluster-order
+     index of the first max |V|) and ends when max |V| < velth / 2; its row is
+     (count, k, onset time, Mw = (log10((mean slip - mean slip at onset)
+     rigid area) + 5.9) / 1.5, hypoloc) and its coseismic slip |slip - slip
+     at onset| is kept (main_LH.f90:1376-1379, output_EQslip);
+   - a snapshot of vel, slip, sigma, tau (cluster order) is taken at every
+     event start and end, every interval-th step and when t > tout, tout
+     advancing by dtout_inter outside and dtout_co during events
+     (main_LH.f90:1399-1407, output_field).
+   Snapshots and finished events are held on the device; when either buffer
+   fills, hbi_rk_step stops after that step with status HBI_RK_OUTPUT_FULL
+   and hbi_rk_get_output must drain them before the next step.
+   hbi_rk_set_state restarts the detector (no event, tout = dtout_inter, as
+   main_LH.f90:1188-1195) and discards undrained output. */
+typedef struct hbi_rk_out_opts {
+    int interval;          /* snapshot every interval-th step; <= 0: not by count */
+    int capacity;          /* snapshots held on the device (>= 1) */
+    int ev_capacity;       /* finished events held on the device (>= 1) */
+    int pad;
+    long long k0;          /* step number before the first step of this run (restart) */
+    double velth;          /* event threshold on max |V| (main_LH.f90 velth) */
+    double dtout_inter;    /* snapshot spacing in time outside events (s) */
+    double dtout_co;       /* snapshot spacing in time during events (s) */
+} hbi_rk_out_opts;
+typedef struct hbi_rk_event {
+    long long count, k;    /* event number (from 1), step at which it ended */
+    double onset_time;     /* t at the onset step */
+    double mw;             /* moment magnitude as main_LH.f90:1376 */
+    double moment;         /* mean |slip| increase over the event (area weighted) */
+    int hypoloc, pad;      /* 1-based cluster-order index of max |V| at the onset */
+} hbi_rk_event;
+/* o == NULL switches the outputs off.  No reference counterpart as a call:
+   the reference writes these files from its host loop. */
+int hbi_rk_set_output(hbi_rk *r, const hbi_rk_out_opts *o);
+/* Copies out and clears the held output.  nsnap, nev: counts (outputs).
+   Any array may be NULL (not copied); sizes: snap_k, snap_t: capacity;
+   fields: capacity x 4 x N (vel, slip, sigma, tau of each snapshot); events:
+   ev_capacity; eqslip: ev_capacity x N. */
+int hbi_rk_get_output(hbi_rk *r, int *nsnap, long long *snap_k, double *snap_t, double *fields, int *nev,
+                      hbi_rk_event *events, double *eqslip);
 
 /* Learned step size.  No reference counterpart: the reference's only step
    controller is rkqs (main_LH.f90:2309-2357), which stays the judge of every

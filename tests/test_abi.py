@@ -54,7 +54,8 @@ def test_ctypes_structs_match_the_c_layout(tmp_path):
     the Python-owned buffer)."""
     import subprocess
     structs = {"hbi_geom": H.Geom, "hbi_hmat_opts": H.HmatOpts, "hbi_hmat_stats": H.HmatStats,
-               "hbi_rk_opts": H.RkOpts, "hbi_rk_info": H.RkInfo}
+               "hbi_rk_opts": H.RkOpts, "hbi_rk_info": H.RkInfo, "hbi_rk_out_opts": H.RkOutOpts,
+               "hbi_rk_event": H.RkEvent}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hbi.h"', 'int main(void) {']
     for cname, py in structs.items():
         lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
