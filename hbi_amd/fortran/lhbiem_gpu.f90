@@ -6,10 +6,16 @@
 ! reference (2dp/2dvs 1701-1718, 3dph 1781-1835, STL frames 1940-1990), builds
 ! the H-matrix through the HACApK call sites (m_HACApK_gpu) and advances the
 ! quasi-dynamic earthquake cycle with the device RK45 (hbi_rk_step), writing
-! output/monitor<n>.dat (main_LH.f90:1489-1493 columns) and the final
-! `time(s)` line.  `forward T` writes H*1 (main_LH.f90:2586-2664) and stops.
-! Keys beyond the reference's: leaf (param(21)), precision (f64|f32|mixed),
-! bf16margin, outbatch (steps per device batch).
+! the reference's output files: monitor<n>.dat (main_LH.f90:1489-1493
+! columns), time<n>.dat, event<n>.dat, the vel/slip/sigma/tau<n>.dat FP64
+! streams (cluster order) and EQslip<n>.dat (main_LH.f90:1339-1407,
+! 1518-1631, events and snapshots decided on the device), xyz/ind<n>.dat
+! (output_coord), and the final `time(s)` line.  `restart T` continues a run
+! from the snapshot of the last interval step in its monitor file (found
+! through time<n>.dat, main_LH.f90:836-957).  `forward T` writes H*1
+! (main_LH.f90:2586-2664) and stops.  Keys beyond the reference's: leaf
+! (param(21)), precision (f64|f32|mixed), bf16margin, outbatch (steps per
+! device batch).
 program lhbiem_gpu
   use iso_c_binding
   use m_hbi_abi
@@ -21,7 +27,14 @@ program lhbiem_gpu
   integer :: ios, number, ncellg, imax, jmax, nstep, i, k, ncol, leaf, outbatch, nb, nrec_dummy(1)
   real(8) :: ds0, eps_r, eps_h, vref, velmax, velmin, tmax, dtinit, dtmax, crake, dipangle, maxsig, minsig
   real(8) :: a0, b0, dc0, mu0, vpl, velinit, tauinit, sigmainit, sr, bf16margin, t1, t2, sg
-  logical :: backslip, sigmaconst, forward, parameterfromfile, limitsigma
+  real(8) :: velth, dtout, dtout_co, x0, dtnxt, dummy(9)
+  integer :: interval, kstart, kend, kmon, nsnap_c, ksnap, kt_read
+  logical :: backslip, sigmaconst, forward, parameterfromfile, limitsigma, restart, found
+  type(hbi_rk_out_opts) :: oo
+  integer(c_int) :: nsnap, nev
+  integer(c_long_long), allocatable :: snap_k(:)
+  real(8), allocatable :: snap_t(:), fields(:), eqslip(:)
+  type(hbi_rk_event), allocatable :: events(:)
   type(st_HACApK_calc_entry) :: st_bemv
   type(st_HACApK_lcontrol) :: st_ctl
   type(st_HACApK_leafmtxp) :: st_leafmtxp_s
@@ -53,6 +66,7 @@ program lhbiem_gpu
   limitsigma = .true.; backslip = .true.; evlaw = 'aging'; crake = 0d0; dipangle = 90d0; sr = 0d0
   geofile = 'default'; parameter_file = 'default'; ncol = 0; nstep = 1000; leaf = 15
   precision = 'f64'; bf16margin = 1d0; outbatch = 200
+  interval = 0; velth = 1e-2; dtout = 365 * 24 * 3600; dtout_co = 1000.0; restart = .false.
   a0 = 0.01d0; b0 = 0.015d0; dc0 = 0.01d0; mu0 = 0.6d0; vpl = 1d-9; velinit = 1d-9; tauinit = 0d0; sigmainit = 50d0
   imax = 0; jmax = 0; ncellg = 0; ds0 = 0d0; problem = ' '
   do
@@ -100,12 +114,18 @@ program lhbiem_gpu
     case ('precision'); read(pvalue, '(a)') precision
     case ('bf16margin'); read(pvalue, *) bf16margin
     case ('outbatch'); read(pvalue, *) outbatch
+    case ('interval'); read(pvalue, *) interval
+    case ('velth'); read(pvalue, *) velth
+    case ('dtout'); read(pvalue, *) dtout
+    case ('dtout_co'); read(pvalue, *) dtout_co
+    case ('restart'); read(pvalue, *) restart
     case default
       write(*, *) 'WARNING: ', trim(param), ' is an unknown parameter'
     end select
   end do
   close(33)
   tmax = tmax * 365 * 24 * 3600
+  if (interval == 0) interval = nstep
   if (parameter_file == 'default') write(parameter_file, '("examples/param",i0,".dat")') number
 
   ! ---------------------------------------------------------------- mesh --
@@ -266,34 +286,112 @@ program lhbiem_gpu
     write(*, *) 'hbi_rk_create: ', trim(hbi_error_text())
     stop 2
   end if
-  rc = hbi_rk_set_state(rk, y, slip, vel, 0d0, dtinit)
   sg = sum(area)
+  x0 = 0d0
+  dtnxt = dtinit
+  kstart = 1
+  kend = 0
+  if (restart) then
+    ! main_LH.f90:836-957: the last interval step of the monitor file, its time
+    ! and next step; the state from that step's snapshot (located through the
+    ! time file, whose rows index the snapshot streams)
+    write(fname, '("output/monitor",i0,".dat")') number
+    open(52, file=fname, status='old')
+    do
+      read(52, *, iostat=ios) kmon, dummy(1:9)
+      if (ios /= 0) exit
+      if (mod(kmon, interval) == 0) then
+        kstart = kmon + 1
+        x0 = dummy(1)
+        dtnxt = min(2 * dummy(8), 0.9 * dummy(8) * (dummy(7)**(-0.2)))
+      end if
+      kend = kmon
+    end do
+    close(52)
+    write(fname, '("output/time",i0,".dat")') number
+    open(50, file=fname, status='old')
+    found = .false.
+    nsnap_c = 0
+    do
+      read(50, *, iostat=ios) kt_read
+      if (ios /= 0) exit
+      nsnap_c = nsnap_c + 1
+      if (kt_read == kstart - 1) then
+        ksnap = nsnap_c
+        found = .true.
+      end if
+    end do
+    close(50)
+    if (.not. found) then
+      write(*, *) 'ERROR: no snapshot of step', kstart - 1
+      stop 1
+    end if
+    call read_snapshot('vel', ksnap, vel)
+    call read_snapshot('slip', ksnap, slip)
+    call read_snapshot('sigma', ksnap, y(3::3))
+    call read_snapshot('tau', ksnap, y(2::3))
+    do i = 1, ncellg
+      y(3 * i - 2) = a(i) * dlog(2 * vref / vel(i) * sinh(y(3 * i - 1) / y(3 * i) / a(i)))
+    end do
+    write(*, *) kstart, x0, dtnxt
+  end if
+  rc = hbi_rk_set_state(rk, y, slip, vel, x0, dtnxt)
+  ! device output decisions (main_LH.f90:1188-1195, 1339-1407)
+  oo%interval = interval
+  oo%capacity = min(outbatch, 64)
+  oo%ev_capacity = 16
+  oo%k0 = kstart - 1
+  oo%velth = velth
+  oo%dtout_inter = dtout * 365 * 24 * 3600
+  oo%dtout_co = dtout_co
+  rc = hbi_rk_set_output(rk, oo)
+  allocate(snap_k(oo%capacity), snap_t(oo%capacity), fields(4 * int(oo%capacity) * ncellg))
+  allocate(events(oo%ev_capacity), eqslip(int(oo%ev_capacity) * ncellg))
   write(fname, '("output/monitor",i0,".dat")') number
-  open(52, file=fname)
+  if (restart) then
+    open(52, file=fname, status='old', position='append')
+  else
+    open(52, file=fname)
+  end if
+  write(fname, '("output/time",i0,".dat")') number
+  call open_text(50, fname, restart)
+  write(fname, '("output/event",i0,".dat")') number
+  call open_text(44, fname, restart)
+  call open_stream(80, 'vel', restart)
+  call open_stream(81, 'slip', restart)
+  call open_stream(82, 'sigma', restart)
+  call open_stream(83, 'tau', restart)
+  call open_stream(84, 'EQslip', restart)
+  if (.not. restart) call output_coord()
   allocate(hist(9 * outbatch), vhist(1))
   nrec = 0
   nrec_dummy = 0
-  kdone = 0
+  kdone = kstart - 1
   call system_clock(c0, crate)
-  ! initial row (main_LH.f90:1175-1181): k = 0, errmax = dt = 0
-  mu_avg = 0d0
-  do i = 1, ncellg
-    mu_avg = mu_avg + y(3 * i - 1) / y(3 * i) * area(i)
-  end do
-  write(52, '(i7,f19.4,7e16.5,i4,f16.4)') 0, 0d0, log10(maxval(abs(vel))), 0d0, mu_avg / sg, &
-    maxval(y(3::3)), minval(y(3::3)), 0d0, 0d0, 0, 0d0
+  if (.not. restart) then
+    ! initial row (main_LH.f90:1175-1181): k = 0, errmax = dt = 0
+    mu_avg = 0d0
+    do i = 1, ncellg
+      mu_avg = mu_avg + y(3 * i - 1) / y(3 * i) * area(i)
+    end do
+    write(52, '(i7,f19.4,7e16.5,i4,f16.4)') 0, 0d0, log10(maxval(abs(vel))), 0d0, mu_avg / sg, &
+      maxval(y(3::3)), minval(y(3::3)), 0d0, 0d0, 0, 0d0
+  end if
   do while (kdone < nstep)
     nb = min(outbatch, nstep - kdone)
     rc = hbi_rk_step(rk, int(nb, c_int), hist, nrec, nrec_dummy, vhist, info)
     call system_clock(c1)
     ! rows as output_monitor (main_LH.f90:1489-1493); the wall time is the
-    ! elapsed time at the end of the device batch the step belongs to
-    do k = 1, int(info%accepted) - kdone
-      write(52, '(i7,f19.4,7e16.5,i4,f16.4)') kdone + k, hist(9 * k - 8), hist(9 * k - 4), hist(9 * k - 3), &
-        hist(9 * k - 2), hist(9 * k - 1), hist(9 * k), hist(9 * k - 6), hist(9 * k - 7), int(hist(9 * k - 5)), &
-        real(c1 - c0, 8) / real(crate, 8)
+    ! elapsed time at the end of the device batch the step belongs to.  A
+    ! restarted run writes the steps after the old file's last row only.
+    do k = 1, int(info%accepted) + kstart - 1 - kdone
+      if (kdone + k > kend) &
+        write(52, '(i7,f19.4,7e16.5,i4,f16.4)') kdone + k, hist(9 * k - 8), hist(9 * k - 4), hist(9 * k - 3), &
+          hist(9 * k - 2), hist(9 * k - 1), hist(9 * k), hist(9 * k - 6), hist(9 * k - 7), int(hist(9 * k - 5)), &
+          real(c1 - c0, 8) / real(crate, 8)
     end do
-    kdone = int(info%accepted)
+    kdone = int(info%accepted) + kstart - 1
+    call write_outputs()
     if (rc == HBI_E_STEP) then
       write(*, *) 'ERROR: Runge-Kutta method did not converge'
       exit
@@ -309,6 +407,11 @@ program lhbiem_gpu
   end do
   call cpu_time(t2)
   close(52)
+  close(50)
+  close(44)
+  do i = 80, 84
+    close(i)
+  end do
   rc = hbi_rk_get_state(rk, y, slip, vel, info)
   write(*, *) 'time(s)', t2 - t1, ' steps', kdone, ' rejected', info%rejected, ' t_end', info%t
   rc = hbi_rk_destroy(rk)
@@ -316,6 +419,88 @@ program lhbiem_gpu
   lrtrn = HACApK_finalize(st_ctl)
 
 contains
+
+  ! the decided outputs of the last batch: time rows and field snapshots
+  ! (main_LH.f90:1410-1425, output_field), event rows and coseismic slip
+  ! (1376-1379, output_EQslip); a restarted run skips steps it already wrote
+  subroutine write_outputs()
+    integer :: j, e, n
+    n = ncellg
+    rc = hbi_rk_get_output(rk, nsnap, snap_k, snap_t, fields, nev, events, eqslip)
+    do e = 1, nev
+      if (events(e)%k > kend) then
+        write(44, '(i0,i8,f17.2,f14.4,i8)') events(e)%count, events(e)%k, events(e)%onset_time, events(e)%mw, &
+          events(e)%hypoloc
+        write(84) eqslip((e - 1) * n + 1:e * n)
+      end if
+    end do
+    do j = 1, nsnap
+      if (snap_k(j) > kend) then
+        write(50, '(i7,f19.4)') snap_k(j), snap_t(j)
+        write(80) fields((j - 1) * 4 * n + 1:(j - 1) * 4 * n + n)
+        write(81) fields((j - 1) * 4 * n + n + 1:(j - 1) * 4 * n + 2 * n)
+        write(82) fields((j - 1) * 4 * n + 2 * n + 1:(j - 1) * 4 * n + 3 * n)
+        write(83) fields((j - 1) * 4 * n + 3 * n + 1:j * 4 * n)
+      end if
+    end do
+  end subroutine write_outputs
+
+  subroutine open_text(u, path, append)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: path
+    logical, intent(in) :: append
+    if (append) then
+      open(u, file=path, position='append')
+    else
+      open(u, file=path, status='replace')
+    end if
+  end subroutine open_text
+
+  subroutine open_stream(u, name, append)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: name
+    logical, intent(in) :: append
+    character(len=256) :: path
+    write(path, '("output/",a,i0,".dat")') name, number
+    if (append) then
+      open(u, file=path, form='unformatted', access='stream', status='old', position='append')
+    else
+      open(u, file=path, form='unformatted', access='stream', status='replace')
+    end if
+  end subroutine open_stream
+
+  ! snapshot j (1-based) of output/<name><n>.dat into v (cluster order)
+  subroutine read_snapshot(name, j, v)
+    character(len=*), intent(in) :: name
+    integer, intent(in) :: j
+    real(8), intent(out) :: v(:)
+    character(len=256) :: path
+    integer :: u
+    integer(8) :: pos
+    write(path, '("output/",a,i0,".dat")') name, number
+    open(newunit=u, file=path, form='unformatted', access='stream', status='old')
+    pos = int(j - 1, 8) * int(ncellg, 8) * 8 + 1
+    read(u, pos=pos) v
+    close(u)
+  end subroutine read_snapshot
+
+  ! output_coord (main_LH.f90:1659-1697): collocation points and element
+  ! ids in cluster order
+  subroutine output_coord()
+    integer :: u1, u2, j, jj
+    character(len=256) :: path
+    write(path, '("output/xyz",i0,".dat")') number
+    open(newunit=u1, file=path, status='replace')
+    write(path, '("output/ind",i0,".dat")') number
+    open(newunit=u2, file=path, status='replace')
+    do j = 1, ncellg
+      jj = st_sum%lodc(j)
+      write(u2, *) jj
+      write(u1, '(3e15.6)') xcol(jj), ycol(jj), zcol(jj)
+    end do
+    close(u1)
+    close(u2)
+  end subroutine output_coord
 
   subroutine mesh_3ddip(imax, jmax, ds0, dipangle, xcol, ycol, zcol, ang, angd)
     integer, intent(in) :: imax, jmax

@@ -3,7 +3,7 @@ module m_hbi_abi
   use iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: HBI_OK = 0, HBI_E_STEP = -4
+  integer(c_int), parameter :: HBI_OK = 0, HBI_E_STEP = -4, HBI_RK_OUTPUT_FULL = 2
   integer(c_int), parameter :: HBI_KIND_2DVS = 1, HBI_KIND_OKADA_HS = 2, HBI_KIND_OKADA_FS = 3, HBI_KIND_TRI_FS = 4
   integer(c_int), parameter :: HBI_PREC_F64 = 0, HBI_PREC_F32 = 1, HBI_PREC_MIXED = 2
 
@@ -43,6 +43,18 @@ module m_hbi_abi
     real(c_double) :: vmax
     integer(c_int) :: status
   end type hbi_rk_info
+
+  type, bind(C) :: hbi_rk_out_opts
+    integer(c_int) :: interval = 0, capacity = 64, ev_capacity = 16, pad = 0
+    integer(c_long_long) :: k0 = 0
+    real(c_double) :: velth = 1d-2, dtout_inter = 3.1536d7, dtout_co = 1d3
+  end type hbi_rk_out_opts
+
+  type, bind(C) :: hbi_rk_event
+    integer(c_long_long) :: count, k
+    real(c_double) :: onset_time, mw, moment
+    integer(c_int) :: hypoloc, pad
+  end type hbi_rk_event
 
   interface
     integer(c_int) function hbi_version() bind(C, name="hbi_version")
@@ -96,6 +108,20 @@ module m_hbi_abi
       type(c_ptr), value :: r
       real(c_double), intent(out) :: y(*), slip(*), vel(*)
       type(hbi_rk_info), intent(out) :: info
+    end function
+    integer(c_int) function hbi_rk_set_output(r, o) bind(C, name="hbi_rk_set_output")
+      import :: c_int, c_ptr, hbi_rk_out_opts
+      type(c_ptr), value :: r
+      type(hbi_rk_out_opts), intent(in) :: o
+    end function
+    integer(c_int) function hbi_rk_get_output(r, nsnap, snap_k, snap_t, fields, nev, events, eqslip) &
+        bind(C, name="hbi_rk_get_output")
+      import :: c_int, c_ptr, c_double, c_long_long, hbi_rk_event
+      type(c_ptr), value :: r
+      integer(c_int), intent(out) :: nsnap, nev
+      integer(c_long_long), intent(out) :: snap_k(*)
+      real(c_double), intent(out) :: snap_t(*), fields(*), eqslip(*)
+      type(hbi_rk_event), intent(out) :: events(*)
     end function
     integer(c_int) function hbi_rk_step(r, nsteps, hist, nrec, rec, vhist, info) bind(C, name="hbi_rk_step")
       import :: c_int, c_ptr, c_double, hbi_rk_info

@@ -86,3 +86,109 @@ def test_gpu_driver_cycle_matches_library_run(tmp_path):
     assert np.isclose(hist[-1, 5], np.sum(np.abs(slip) * area) / area.sum(), rtol=1e-12)
     assert np.isclose(hist[-1, 6], np.sum(y[1::3] / y[2::3] * area) / area.sum(), rtol=1e-12)
     assert hist[-1, 7] == y[2::3].max() and hist[-1, 8] == y[2::3].min()
+
+
+def _c1_nucleating(n=2048):
+    """C1 with the seeded nucleation perturbation of the library tests,
+    written through the parameter file's tau / sigma / vel columns (the
+    driver derives psi from them, main_LH.f90:1004)."""
+    ds0 = 40.0 / n
+    g = P.mesh_2dvs(n, ds0)
+    fr = P.friction_c1(g)
+    Hg = H.HMatrix(g, leaf=32, eps=1e-6)
+    lod = Hg.lod
+    y0 = P.init_state(fr, lod)
+    P.perturb_state(y0, g, lod)
+    a = fr.a[lod]
+    v = 2 * fr.vref * np.exp(-y0[0::3] / a) * np.sinh(y0[1::3] / y0[2::3] / a)
+    fr.vel = np.empty(n); fr.vel[lod] = v
+    fr.tau = np.empty(n); fr.tau[lod] = y0[1::3]
+    fr.sigma = np.empty(n); fr.sigma[lod] = y0[2::3]
+    return g, fr, lod, ds0
+
+
+def _keys(n, ds0, nstep, **kw):
+    k = {"problem": "2dvs", "ncellg": n, "ds": ds0, "nstep": nstep, "eps_r": 1e-6, "eps_h": 1e-6, "leaf": 32,
+         "filenumber": 5, "parameterfromfile": True, "parameter_file": "param.dat", "parameter_file_ncol": 8,
+         "outbatch": 400, "vpl": 1e-9, "tmax": 1e4, "interval": 500, "dtout_co": 1.0, "dtout": 1.0}
+    k.update(kw)
+    return k
+
+
+def _stream(path, n):
+    d = np.fromfile(path, dtype=np.float64)
+    assert d.size % n == 0
+    return d.reshape(-1, n)
+
+
+def test_gpu_driver_events_and_snapshots(tmp_path):
+    """output/time, vel, slip, sigma, tau, event, EQslip, xyz, ind of the
+    driver (main_LH.f90:1339-1425, 1518-1697): one snapshot per time row; an
+    event row per finished event whose onset / end snapshots bracket the
+    threshold, whose hypocentre is the first max |V| of the onset snapshot,
+    whose coseismic slip is the difference of the two slip snapshots and whose
+    Mw follows from their area-weighted mean slips."""
+    n = 2048
+    g, fr, lod, ds0 = _c1_nucleating(n)
+    P.write_param_file(str(tmp_path / "param.dat"), fr)
+    _run(tmp_path, _keys(n, ds0, 2500))
+    out = tmp_path / "output"
+    trows = np.loadtxt(out / "time5.dat", ndmin=2)
+    vel, slip = _stream(out / "vel5.dat", n), _stream(out / "slip5.dat", n)
+    sig, tau = _stream(out / "sigma5.dat", n), _stream(out / "tau5.dat", n)
+    assert len(trows) == len(vel) == len(slip) == len(sig) == len(tau) > 0
+    ks = trows[:, 0].astype(int)
+    assert (np.diff(ks) > 0).all()
+    assert all(k in ks for k in range(500, 2501, 500))          # every interval step
+    ev = np.loadtxt(out / "event5.dat", ndmin=2)
+    assert len(ev) >= 1
+    eq = _stream(out / "EQslip5.dat", n)
+    assert len(eq) == len(ev)
+    velth = float(np.float32(1e-2))
+    area = np.full(n, ds0)
+    for e, row in enumerate(ev):
+        count, k_end, onset, mw, hyp = int(row[0]), int(row[1]), row[2], row[3], int(row[4])
+        assert count == e + 1
+        j_end = int(np.where(ks == k_end)[0][0])
+        # the onset snapshot: taken at the onset time (f19.4 vs f17.2 rows;
+        # snapshots during an event are dtout_co = 1 s apart)
+        j_on = int(np.argmin(np.abs(trows[:j_end, 1] - onset)))
+        assert abs(trows[j_on, 1] - onset) <= 0.006 + 1e-9 * abs(onset)
+        assert np.abs(vel[j_on]).max() > velth and np.abs(vel[j_on - 1]).max() <= velth
+        assert np.abs(vel[j_end]).max() < 0.5 * velth
+        assert hyp == int(np.argmax(np.abs(vel[j_on]))) + 1
+        assert np.array_equal(eq[e], np.abs(slip[j_end] - slip[j_on]))
+        m = (np.abs(slip[j_end]) * area).sum() / area.sum() - (np.abs(slip[j_on]) * area).sum() / area.sum()
+        assert abs(mw - (np.log10(m * 32.04 * area.sum()) + 5.9) / 1.5) <= 6e-5
+    # the end snapshot is the state the monitor row of that step describes
+    mon = np.loadtxt(out / "monitor5.dat")
+    for j, k in enumerate(ks[:5]):
+        assert abs(mon[k, 6] - sig[j].min()) <= 6e-5 * abs(sig[j].min())
+    xyz = np.loadtxt(out / "xyz5.dat")
+    ind = np.loadtxt(out / "ind5.dat", dtype=int)
+    assert np.array_equal(ind, lod + 1)
+    assert np.allclose(xyz[:, 0], g.xcol[lod], rtol=5.01e-6, atol=1e-12)     # e15.6: 6 significant digits
+
+
+def test_gpu_driver_restart_continues_the_run(tmp_path):
+    """restart T (main_LH.f90:836-957): the run continues from the snapshot
+    of the last interval step in its monitor file; the monitor then holds
+    every step once, and the restarted state equals that snapshot."""
+    n = 1024
+    g, fr, lod, ds0 = _c1_nucleating(n)
+    P.write_param_file(str(tmp_path / "param.dat"), fr)
+    _run(tmp_path, _keys(n, ds0, 1300, interval=400))
+    out = tmp_path / "output"
+    vel1 = _stream(out / "vel5.dat", n)
+    k1 = np.loadtxt(out / "time5.dat", ndmin=2)[:, 0].astype(int)
+    snap1200 = vel1[int(np.where(k1 == 1200)[0][0])].copy()
+    _run(tmp_path, _keys(n, ds0, 2000, interval=400, restart=True))
+    mon = np.loadtxt(out / "monitor5.dat")
+    assert np.array_equal(mon[:, 0], np.arange(0, 2001))
+    assert (np.diff(mon[:, 1]) >= 0).all()
+    k2 = np.loadtxt(out / "time5.dat", ndmin=2)[:, 0].astype(int)
+    assert (np.diff(k2) > 0).all() and 1600 in k2 and 2000 in k2
+    vel2 = _stream(out / "vel5.dat", n)
+    assert len(vel2) == len(k2)
+    # the first steps of the restarted file are the old file's, unchanged
+    assert np.array_equal(vel2[int(np.where(k2 == 1200)[0][0])], snap1200)
