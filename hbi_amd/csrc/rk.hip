@@ -24,7 +24,7 @@
 namespace hbi {
 
 struct RkPar {
-    int n, law, backslip, limitsigma, has_hn, tr, tr_n;
+    int n, law, backslip, limitsigma, has_hn, tr, tr_n, has_open;
     double vref, minsig, maxsig, dtmax, hdiv, eps, velmin, velmax, sg;
     Tab T;
     int learn, use_mlp;                   // step features on; MLP proposals on
@@ -54,6 +54,8 @@ struct RkArrays {
     const double *a, *b, *dc, *f0, *vpl, *taudot, *sigdot, *area;
     double *y, *ytmp, *yout, *yerr, *k, *vel, *x, *slip, *velc;
     const double *hx, *hx_n;    // H x of the shear (and normal) stress matrices
+    const double *hxo, *hxo_n;  // opening: H_s2 xn, H_n2 xn
+    double *xn, *veln, *slipn;  // opening rate (matvec input), post-step opening velocity and opening
     double *errpart, *monpart;
     double *hist, *vhist;
     const int *rec;
@@ -100,6 +102,8 @@ __global__ void rk_stage0_kernel(RkPar P, const RkCtl *__restrict__ c, RkArrays 
     const double v = slip_rate(y[3 * i], y[3 * i + 1], y[3 * i + 2], A.a[i], P.vref);
     A.vel[i] = v;
     A.x[i] = P.backslip ? v - A.vpl[i] : v;
+    // opening rate, main_LH.f90:2096-2100
+    if (P.has_open) A.xn[i] = fmax(0.0, P.minsig - y[3 * i + 2]) * fabs(v);
 }
 
 // ordered (guard, value) pair of the reference's errmax loop: a guard element
@@ -134,6 +138,10 @@ __global__ __launch_bounds__(kRkThreads) void rk_stage_kernel(RkPar P, const RkC
             const double hy = A.hx_n[i];
             gn = P.hdiv != 0.0 ? hy / P.hdiv : hy;
         }
+        if (P.has_open) {   // main_LH.f90:2129-2143
+            gs = gs + (P.hdiv != 0.0 ? A.hxo[i] / P.hdiv : A.hxo[i]);
+            gn = gn + (P.hdiv != 0.0 ? A.hxo_n[i] / P.hdiv : A.hxo_n[i]);
+        }
         gn = gn + A.sigdot[i];
         gs = gs + A.taudot[i];
         const double *ys = (s == 0) ? A.y : A.ytmp;
@@ -167,6 +175,7 @@ __global__ __launch_bounds__(kRkThreads) void rk_stage_kernel(RkPar P, const RkC
                 const double v = slip_rate(yt[0], yt[1], yt[2], A.a[i], P.vref);
                 A.vel[i] = v;
                 A.x[i] = P.backslip ? v - A.vpl[i] : v;
+                if (P.has_open) A.xn[i] = fmax(0.0, P.minsig - yt[2]) * fabs(v);
             } else {
                 for (int q = 0; q < 3; q++) {
                     const size_t e = 3 * (size_t)i + q;
@@ -253,10 +262,15 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
         double sl = A.slip[i];
         sl = sl + A.velc[i] * dt * 0.5;
         const double v = slip_rate(A.y[3 * i], A.y[3 * i + 1], A.y[3 * i + 2], A.a[i], P.vref);
+        const double sig = A.y[3 * i + 2];
+        if (P.has_open) {   // main_LH.f90:1279-1283
+            const double vn = v * (fmax(0.0, P.minsig - sig) + fmin(0.0, P.maxsig - sig));
+            A.veln[i] = vn;
+            A.slipn[i] = A.slipn[i] + vn * dt;
+        }
         sl = sl + v * dt * 0.5;
         A.slip[i] = sl;
         A.velc[i] = v;
-        const double sig = A.y[3 * i + 2];
         m[0] = fabs(v);
         m[1] = fabs(sl) * A.area[i];
         m[2] = A.y[3 * i + 1] / sig * A.area[i];
@@ -425,11 +439,16 @@ __global__ void rk_output_kernel(RkPar P, const RkCtl *__restrict__ c, RkArrays 
     const size_t n = P.n;
     const double sl = A.slip[i];
     if (c->out_now) {
-        double *s = A.snap + (size_t)c->snap_slot * 4 * n;
+        const int nf = P.has_open ? 6 : 4;
+        double *s = A.snap + (size_t)c->snap_slot * nf * n;
         s[i] = A.velc[i];
         s[n + i] = sl;
         s[2 * n + i] = A.y[3 * (size_t)i + 2];
         s[3 * n + i] = A.y[3 * (size_t)i + 1];
+        if (P.has_open) {
+            s[4 * n + i] = A.veln[i];
+            s[5 * n + i] = A.slipn[i];
+        }
     }
     if (c->onset_now) A.islip[i] = sl;
     if (c->end_now) A.eqslip[(size_t)c->ev_slot * n + i] = fabs(sl - A.islip[i]);
@@ -441,6 +460,7 @@ __global__ void rk_output_kernel(RkPar P, const RkCtl *__restrict__ c, RkArrays 
 struct hbi_rk {
     hipStream_t stream = nullptr;
     hbi_hmat *hs = nullptr, *hn = nullptr;
+    hbi_hmat *hs2 = nullptr, *hn2 = nullptr;   // opening matrices (hbi_rk_set_opening)
     hbi_rk_opts opts{};
     hbi::RkPar P{};
     int n = 0, nblk = 0;
@@ -454,6 +474,7 @@ struct hbi_rk {
     int trace_cap = 0;
     hbi::DevBuf<float> mlp_p, mlp_mu, mlp_sd;   // copy of the step-size MLP
     hbi::DevBuf<uint16_t> mlp_w;
+    hbi::DevBuf<double> xn, zo, zo_n, ypart_o, ypart_on, hxo, hxo_n, veln, slipn;   // opening
     hbi::DevBuf<double> snap, snap_t, islip, eqslip;   // outputs (hbi_rk_set_output)
     hbi::DevBuf<long long> snap_k;
     hbi::DevBuf<hbi_rk_event> events;
@@ -476,6 +497,7 @@ static RkArrays arrays(hbi_rk *r)
     A.y = r->y.p; A.ytmp = r->ytmp.p; A.yout = r->yout.p; A.yerr = r->yerr.p; A.k = r->k.p; A.vel = r->vel.p;
     A.x = r->x.p; A.slip = r->slip.p; A.velc = r->velc.p;
     A.hx = r->hx.p; A.hx_n = r->hx_n.p;
+    A.hxo = r->hxo.p; A.hxo_n = r->hxo_n.p; A.xn = r->xn.p; A.veln = r->veln.p; A.slipn = r->slipn.p;
     A.errpart = r->errpart.p; A.monpart = r->monpart.p;
     A.bins = r->bins.p;
     A.trace = r->trace.p;
@@ -503,6 +525,12 @@ static void launch_stages(hbi_rk *r, int nstages, int mode)
         if (r->hn) {
             hmat_phase1(r->hn, r->opts.prec, r->x.p, r->zn.p, r->ypart_n.p, r->stream, gate);
             hmat_phase2(r->hn, r->opts.prec, r->zn.p, r->ypart_n.p, r->hx_n.p, r->stream, gate);
+        }
+        if (r->P.has_open) {
+            hmat_phase1(r->hs2, r->opts.prec, r->xn.p, r->zo.p, r->ypart_o.p, r->stream, gate);
+            hmat_phase2(r->hs2, r->opts.prec, r->zo.p, r->ypart_o.p, r->hxo.p, r->stream, gate);
+            hmat_phase1(r->hn2, r->opts.prec, r->xn.p, r->zo_n.p, r->ypart_on.p, r->stream, gate);
+            hmat_phase2(r->hn2, r->opts.prec, r->zo_n.p, r->ypart_on.p, r->hxo_n.p, r->stream, gate);
         }
         rk_stage_kernel<<<g, kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A, s, mode);
     }
@@ -642,6 +670,10 @@ int hbi_rk_set_state(hbi_rk *r, const double *y, const double *slip, const doubl
         hbi::RkCtl c{};
         c.h = dt_next;
         c.t = t;
+        if (r->P.has_open) {   // veln = slipn = 0 (main_LH.f90:967, 1004)
+            HBI_HIP(hipMemsetAsync(r->veln.p, 0, n * 8, r->stream));
+            HBI_HIP(hipMemsetAsync(r->slipn.p, 0, n * 8, r->stream));
+        }
         // a fresh event detector (main_LH.f90:1188-1195); undrained output is dropped
         c.dtout = r->P.dtout_inter;
         c.tout = r->P.dtout_inter;
@@ -833,7 +865,7 @@ int hbi_rk_set_output(hbi_rk *r, const hbi_rk_out_opts *o)
                             o->dtout_co > 0 && o->k0 >= 0,
                         "bad output options");
             const size_t n = r->n;
-            r->snap.alloc((size_t)o->capacity * 4 * n);
+            r->snap.alloc((size_t)o->capacity * (P.has_open ? 6 : 4) * n);
             r->snap_t.alloc(o->capacity);
             r->snap_k.alloc(o->capacity);
             r->islip.alloc(n);
@@ -857,6 +889,45 @@ int hbi_rk_set_output(hbi_rk *r, const hbi_rk_out_opts *o)
     });
 }
 
+int hbi_rk_set_opening(hbi_rk *r, hbi_hmat *hs2, hbi_hmat *hn2)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && hs2 && hn2, "NULL argument");
+        HBI_REQUIRE(r->hn, "opening needs the normal-stress matrix (hn of hbi_rk_create)");
+        HBI_REQUIRE(hs2->n == r->n && hn2->n == r->n, "opening matrix size differs");
+        HBI_REQUIRE(!r->P.out_on, "set the opening before hbi_rk_set_output");
+        const int n = r->n;
+        const int prec = r->opts.prec;
+        r->hs2 = hs2;
+        r->hn2 = hn2;
+        r->xn.alloc(n + 8);
+        HBI_HIP(hipMemsetAsync(r->xn.p, 0, (n + 8) * 8, r->stream));
+        r->zo.alloc(hbi::hmat_z_size(hs2, prec));
+        r->zo_n.alloc(hbi::hmat_z_size(hn2, prec));
+        r->ypart_o.alloc(hbi::hmat_ypart_size(hs2, prec));
+        r->ypart_on.alloc(hbi::hmat_ypart_size(hn2, prec));
+        r->hxo.alloc(n);
+        r->hxo_n.alloc(n);
+        r->veln.alloc(n);
+        r->slipn.alloc(n);
+        HBI_HIP(hipMemsetAsync(r->veln.p, 0, n * 8, r->stream));
+        HBI_HIP(hipMemsetAsync(r->slipn.p, 0, n * 8, r->stream));
+        HBI_HIP(hipStreamSynchronize(r->stream));
+        r->P.has_open = 1;
+        r->dirty = true;
+    });
+}
+
+int hbi_rk_get_opening(hbi_rk *r, double *veln, double *slipn)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && r->P.has_open, "no opening (hbi_rk_set_opening)");
+        if (veln) r->veln.download(veln, r->n, r->stream);
+        if (slipn) r->slipn.download(slipn, r->n, r->stream);
+        HBI_HIP(hipStreamSynchronize(r->stream));
+    });
+}
+
 int hbi_rk_get_output(hbi_rk *r, int *nsnap, long long *snap_k, double *snap_t, double *fields, int *nev,
                       hbi_rk_event *events, double *eqslip)
 {
@@ -871,7 +942,7 @@ int hbi_rk_get_output(hbi_rk *r, int *nsnap, long long *snap_k, double *snap_t, 
         if (c.nsnap > 0) {
             if (snap_k) r->snap_k.download(snap_k, c.nsnap, r->stream);
             if (snap_t) r->snap_t.download(snap_t, c.nsnap, r->stream);
-            if (fields) r->snap.download(fields, (size_t)c.nsnap * 4 * n, r->stream);
+            if (fields) r->snap.download(fields, (size_t)c.nsnap * (r->P.has_open ? 6 : 4) * n, r->stream);
         }
         if (c.nev > 0) {
             if (events) r->events.download(events, c.nev, r->stream);

@@ -94,7 +94,8 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_ens_step", "hbi_ens_last_ms", "hbi_mlp_create", "hbi_mlp_destroy", "hbi_mlp_nparams",
            "hbi_mlp_set_params", "hbi_mlp_get_params", "hbi_mlp_forward", "hbi_mlp_loss_grad", "hbi_mlp_train",
            "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace", "hbi_ens_set_controller",
-           "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_rk_step_many", "hbi_rk_set_output", "hbi_rk_get_output")
+           "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_rk_step_many", "hbi_rk_set_output", "hbi_rk_get_output",
+           "hbi_rk_set_opening", "hbi_rk_get_opening")
 
 
 def lib():
@@ -147,6 +148,8 @@ def lib():
         L.hbi_ens_set_trace.argtypes = [C.c_void_p, C.c_int]
         L.hbi_rk_step_many.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(RkInfo)]
         L.hbi_rk_set_output.argtypes = [C.c_void_p, C.POINTER(RkOutOpts)]
+        L.hbi_rk_set_opening.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hbi_rk_get_opening.argtypes = [C.c_void_p, _dp, _dp]
         L.hbi_rk_get_output.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), _dp, _dp,
                                         C.POINTER(C.c_int), C.POINTER(RkEvent), _dp]
         L.hbi_ens_get_trace.argtypes = [C.c_void_p, _dp, _dp, _ip]
@@ -360,14 +363,28 @@ class RK:
         _check(lib().hbi_rk_set_output(self.r, C.byref(o)))
         self.out_caps = (int(capacity), int(ev_capacity))
 
+    def set_opening(self, hs2: HMatrix, hn2: HMatrix):
+        """Opening (tensile dislocation matrices onto shear and normal traction;
+        main_LH.f90:705-716, 2096-2143, 1279-1283).  Before set_output."""
+        self._open = (hs2, hn2)
+        _check(lib().hbi_rk_set_opening(self.r, hs2.h, hn2.h))
+
+    def get_opening(self):
+        """(veln, slipn) after the last accepted step, cluster order."""
+        vn, sn = np.empty(self.n), np.empty(self.n)
+        _check(lib().hbi_rk_get_opening(self.r, _p(vn), _p(sn)))
+        return vn, sn
+
     def get_output(self) -> dict:
         """Drain the held output: snapshot steps, times and fields (vel, slip,
-        sigma, tau; cluster order), finished events and their coseismic slip."""
+        sigma, tau, and veln, slipn with opening; cluster order), finished
+        events and their coseismic slip."""
         cap, ecap = self.out_caps
         n = self.n
+        nf = 6 if getattr(self, "_open", None) else 4
         ks = np.zeros(cap, dtype=np.int64)
         ts = np.zeros(cap)
-        f = np.zeros((cap, 4, n))
+        f = np.zeros((cap, nf, n))
         ev = (RkEvent * ecap)()
         eq = np.zeros((ecap, n))
         ns, ne = C.c_int(0), C.c_int(0)
@@ -375,8 +392,11 @@ class RK:
                                        _p(f), C.byref(ne), ev, _p(eq)))
         ns, ne = ns.value, ne.value
         events = [{k: getattr(ev[i], k) for k, _ in RkEvent._fields_ if k != "pad"} for i in range(ne)]
-        return {"k": ks[:ns].copy(), "t": ts[:ns].copy(), "vel": f[:ns, 0].copy(), "slip": f[:ns, 1].copy(),
-                "sigma": f[:ns, 2].copy(), "tau": f[:ns, 3].copy(), "events": events, "eqslip": eq[:ne].copy()}
+        out = {"k": ks[:ns].copy(), "t": ts[:ns].copy(), "vel": f[:ns, 0].copy(), "slip": f[:ns, 1].copy(),
+               "sigma": f[:ns, 2].copy(), "tau": f[:ns, 3].copy(), "events": events, "eqslip": eq[:ne].copy()}
+        if nf == 6:
+            out["veln"], out["slipn"] = f[:ns, 4].copy(), f[:ns, 5].copy()
+        return out
 
     @staticmethod
     def step_many(runs, nsteps):

@@ -193,6 +193,17 @@ int hbi_rk_step_many(hbi_rk *const *rs, int m, int nsteps, hbi_rk_info *infos);
 /* Device time of the last hbi_rk_step call, milliseconds (HIP events) */
 int hbi_rk_last_ms(const hbi_rk *r, double *ms);
 
+/* Opening (main_LH.f90:705-716, 2096-2143, 1279-1283): hs2, hn2 are the
+   matrices of a tensile dislocation (slip mode 'open') onto shear and normal
+   traction.  Every stage adds hs2 xn to the shear and hn2 xn to the normal
+   stress rate (divided by hdiv like the others), xn = max(0, minsig - sigma)
+   |V|; after each accepted step veln = V (max(0, minsig - sigma) + min(0,
+   maxsig - sigma)) and slipn += veln dt (both start at 0 with
+   hbi_rk_set_state).  Needs the normal-stress matrix (hn of hbi_rk_create);
+   call before hbi_rk_set_output (snapshots then also hold veln, slipn). */
+int hbi_rk_set_opening(hbi_rk *r, hbi_hmat *hs2, hbi_hmat *hn2);
+int hbi_rk_get_opening(hbi_rk *r, double *veln, double *slipn);
+
 /* Field snapshots and the event list, decided on the device after every
    accepted step k = k0 + accepted (main_LH.f90:1339-1407):
    - event detector: an event starts when max |V| > velth (onset time, the
@@ -232,8 +243,9 @@ typedef struct hbi_rk_event {
 int hbi_rk_set_output(hbi_rk *r, const hbi_rk_out_opts *o);
 /* Copies out and clears the held output.  nsnap, nev: counts (outputs).
    Any array may be NULL (not copied); sizes: snap_k, snap_t: capacity;
-   fields: capacity x 4 x N (vel, slip, sigma, tau of each snapshot); events:
-   ev_capacity; eqslip: ev_capacity x N. */
+   fields: capacity x 4 x N (vel, slip, sigma, tau of each snapshot; with
+   opening capacity x 6 x N, + veln, slipn); events: ev_capacity; eqslip:
+   ev_capacity x N. */
 int hbi_rk_get_output(hbi_rk *r, int *nsnap, long long *snap_k, double *snap_t, double *fields, int *nev,
                       hbi_rk_event *events, double *eqslip);
 

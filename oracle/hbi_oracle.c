@@ -1199,8 +1199,10 @@ typedef struct {
     const double *a, *b, *dc, *f0, *vpl, *taudot, *sigdot;   /* cluster order */
     const og_hmat *hs;   /* shear matrix */
     const og_hmat *hn;   /* normal matrix or NULL (sigmaconst) */
+    const og_hmat *hs2, *hn2;   /* opening -> shear / normal matrices, or NULL (no opening) */
     int nthreads;
     double *vel, *xv, *hx, *hy;   /* workspaces */
+    double *xn, *hx2, *hy2;       /* opening rate and its products */
     long long nmatvec;
 } og_sim;
 
@@ -1224,7 +1226,19 @@ og_sim *og_sim_create(const og_hmat *hs, const og_hmat *hn, int law, int backsli
 void og_sim_free(og_sim *s)
 {
     if (!s) return;
-    free(s->vel); free(s->xv); free(s->hx); free(s->hy); free(s);
+    free(s->vel); free(s->xv); free(s->hx); free(s->hy);
+    free(s->xn); free(s->hx2); free(s->hy2); free(s);
+}
+/* opening (main_LH.f90:705-716): the matrices of a tensile dislocation ('open')
+   onto shear and normal traction; needs the normal matrix */
+int og_sim_set_opening(og_sim *s, const og_hmat *hs2, const og_hmat *hn2)
+{
+    if (!s->hn || !hs2 || !hn2 || hs2->n != s->n || hn2->n != s->n) return -1;
+    s->hs2 = hs2; s->hn2 = hn2;
+    s->xn = (double *)malloc(sizeof(double) * s->n);
+    s->hx2 = (double *)malloc(sizeof(double) * s->n);
+    s->hy2 = (double *)malloc(sizeof(double) * s->n);
+    return 0;
 }
 long long og_sim_nmatvec(const og_sim *s) { return s->nmatvec; }
 
@@ -1264,15 +1278,26 @@ void og_derivs(og_sim *s, const double *y, double *dydx)
         double psi = y[3 * i], tau = y[3 * i + 1], sig = y[3 * i + 2];
         s->vel[i] = 2 * s->vref * exp(-psi / s->a[i]) * sinh(tau / sig / s->a[i]);
         s->xv[i] = s->backslip ? s->vel[i] - s->vpl[i] : s->vel[i];
+        /* opening rate, main_LH.f90:2096-2100 */
+        if (s->hs2) s->xn[i] = fmax(0.0, s->minsig - sig) * fabs(s->vel[i]);
     }
     og_hmat_matvec(s->hs, s->xv, s->hx, s->nthreads);
     s->nmatvec++;
     if (s->hn) { og_hmat_matvec(s->hn, s->xv, s->hy, s->nthreads); s->nmatvec++; }
+    if (s->hs2) {
+        og_hmat_matvec(s->hs2, s->xn, s->hx2, s->nthreads);
+        og_hmat_matvec(s->hn2, s->xn, s->hy2, s->nthreads);
+        s->nmatvec += 2;
+    }
 #pragma omp parallel for num_threads(s->nthreads > 0 ? s->nthreads : 1)
     for (int i = 0; i < n; i++) {
         double gs = s->hdiv != 0.0 ? s->hx[i] / s->hdiv : s->hx[i];
         double gn = 0.0;
         if (s->hn) gn = s->hdiv != 0.0 ? s->hy[i] / s->hdiv : s->hy[i];
+        if (s->hs2) {   /* main_LH.f90:2129-2143 */
+            gs = gs + (s->hdiv != 0.0 ? s->hx2[i] / s->hdiv : s->hx2[i]);
+            gn = gn + (s->hdiv != 0.0 ? s->hy2[i] / s->hdiv : s->hy2[i]);
+        }
         gn = gn + s->sigdot[i];
         gs = gs + s->taudot[i];
         og_deriv(s->law, s->limitsigma, s->vref, s->minsig, s->maxsig, gs, gn, y[3 * i], y[3 * i + 1], y[3 * i + 2],

@@ -67,6 +67,7 @@ def lib():
         L.og_sim_create.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                     C.c_double, C.c_double, C.c_double] + [_dp] * 7 + [C.c_int]
         L.og_sim_free.argtypes = [C.c_void_p]
+        L.og_sim_set_opening.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.og_sim_nmatvec.restype = C.c_longlong
         L.og_sim_nmatvec.argtypes = [C.c_void_p]
         L.og_derivs.argtypes = [C.c_void_p, _dp, _dp]
@@ -207,6 +208,12 @@ class Sim:
         if getattr(self, "s", None):
             lib().og_sim_free(self.s)
             self.s = None
+
+    def set_opening(self, hs2: "HMatrix", hn2: "HMatrix"):
+        """Opening (main_LH.f90:705-716, 2096-2143): tensile matrices onto shear and normal traction."""
+        self._open = (hs2, hn2)
+        if lib().og_sim_set_opening(self.s, hs2.h, hn2.h) != 0:
+            raise ValueError("opening needs the normal matrix and matrices of the same size")
 
     @property
     def nmatvec(self):
