@@ -12,7 +12,9 @@
 ! 1518-1631, events and snapshots decided on the device), xyz/ind<n>.dat
 ! (output_coord), and the final `time(s)` line.  `restart T` continues a run
 ! from the snapshot of the last interval step in its monitor file (found
-! through time<n>.dat, main_LH.f90:836-957).  `forward T` writes H*1
+! through time<n>.dat, main_LH.f90:836-957).  `sigmaconst F` adds the
+! normal-stress matrix and `opening T` the two tensile-dislocation matrices
+! with veln/slipn<n>.dat streams (main_LH.f90:683-716).  `forward T` writes H*1
 ! (main_LH.f90:2586-2664) and stops.  Keys beyond the reference's: leaf
 ! (param(21)), precision (f64|f32|mixed), bf16margin, outbatch (steps per
 ! device batch).
@@ -29,7 +31,12 @@ program lhbiem_gpu
   real(8) :: a0, b0, dc0, mu0, vpl, velinit, tauinit, sigmainit, sr, bf16margin, t1, t2, sg
   real(8) :: velth, dtout, dtout_co, x0, dtnxt, dummy(9)
   integer :: interval, kstart, kend, kmon, nsnap_c, ksnap, kt_read
-  logical :: backslip, sigmaconst, forward, parameterfromfile, limitsigma, restart, found
+  logical :: backslip, sigmaconst, forward, parameterfromfile, limitsigma, restart, found, opening
+  type(st_HACApK_lcontrol) :: st_ctl2, st_ctl3, st_ctl4
+  type(st_HACApK_leafmtxp) :: st_leafmtxp_n, st_leafmtxp_s2, st_leafmtxp_n2
+  type(st_HACApK_LHp) :: st_LHp_n, st_LHp_s2, st_LHp_n2
+  type(c_ptr) :: hn_ptr
+  integer :: nf
   type(hbi_rk_out_opts) :: oo
   integer(c_int) :: nsnap, nev
   integer(c_long_long), allocatable :: snap_k(:)
@@ -66,7 +73,7 @@ program lhbiem_gpu
   limitsigma = .true.; backslip = .true.; evlaw = 'aging'; crake = 0d0; dipangle = 90d0; sr = 0d0
   geofile = 'default'; parameter_file = 'default'; ncol = 0; nstep = 1000; leaf = 15
   precision = 'f64'; bf16margin = 1d0; outbatch = 200
-  interval = 0; velth = 1e-2; dtout = 365 * 24 * 3600; dtout_co = 1000.0; restart = .false.
+  interval = 0; velth = 1e-2; dtout = 365 * 24 * 3600; dtout_co = 1000.0; restart = .false.; opening = .false.
   a0 = 0.01d0; b0 = 0.015d0; dc0 = 0.01d0; mu0 = 0.6d0; vpl = 1d-9; velinit = 1d-9; tauinit = 0d0; sigmainit = 50d0
   imax = 0; jmax = 0; ncellg = 0; ds0 = 0d0; problem = ' '
   do
@@ -119,6 +126,7 @@ program lhbiem_gpu
     case ('dtout'); read(pvalue, *) dtout
     case ('dtout_co'); read(pvalue, *) dtout_co
     case ('restart'); read(pvalue, *) restart
+    case ('opening'); read(pvalue, *) opening
     case default
       write(*, *) 'WARNING: ', trim(param), ' is an unknown parameter'
     end select
@@ -231,12 +239,28 @@ program lhbiem_gpu
   end select
   allocate(coord(ncellg, 3))
   coord(:, 1) = xcol; coord(:, 2) = ycol; coord(:, 3) = zcol
+  ! main_LH.f90:683-716: the shear matrix; the normal one unless sigma is
+  ! constant; with opening the tensile-dislocation matrices too
+  select case (trim(problem))
+  case ('2dp', '2dn3', '3dp', '2dvs')
+    sigmaconst = .true.
+  end select
+  if (opening) sigmaconst = .false.
   st_bemv%md = 's'
   st_bemv%v = 's'
   call cpu_time(t1)
   lrtrn = HACApK_generate(st_leafmtxp_s, st_bemv, st_ctl, coord, eps_h)
   if (lrtrn /= 0) stop 2
   lrtrn = HACApK_construct_LH(st_LHp_s, st_leafmtxp_s, st_bemv, st_ctl, coord, eps_h)
+  if (.not. sigmaconst .and. .not. forward) then
+    call build_more(st_ctl2, st_leafmtxp_n, st_LHp_n, 's', 'n')
+  end if
+  if (opening .and. .not. forward) then
+    call build_more(st_ctl3, st_leafmtxp_s2, st_LHp_s2, 'open', 's')
+    call build_more(st_ctl4, st_leafmtxp_n2, st_LHp_n2, 'open', 'n')
+  end if
+  st_bemv%md = 's'
+  st_bemv%v = 's'
   allocate(wws(ncellg))
   lrtrn = HACApK_gen_lattice_vector(st_vel, st_leafmtxp_s, st_ctl)
   lrtrn = HACApK_gen_lattice_vector(st_sum, st_leafmtxp_s, st_ctl)
@@ -281,11 +305,21 @@ program lhbiem_gpu
   ro%hdiv = 0d0
   if (trim(problem) == '3dph') ro%hdiv = ds0
   ro%eps_r = eps_r; ro%velmin = velmin; ro%velmax = velmax
-  rc = hbi_rk_create(st_LHp_s%h, c_null_ptr, ro, a, b, dc, f0, vplv, taudot, sigdot, area, rk)
+  hn_ptr = c_null_ptr
+  if (.not. sigmaconst) hn_ptr = st_LHp_n%h
+  rc = hbi_rk_create(st_LHp_s%h, hn_ptr, ro, a, b, dc, f0, vplv, taudot, sigdot, area, rk)
   if (rc /= HBI_OK) then
     write(*, *) 'hbi_rk_create: ', trim(hbi_error_text())
     stop 2
   end if
+  if (opening) then
+    rc = hbi_rk_set_opening(rk, st_LHp_s2%h, st_LHp_n2%h)
+    if (rc /= HBI_OK) then
+      write(*, *) 'hbi_rk_set_opening: ', trim(hbi_error_text())
+      stop 2
+    end if
+  end if
+  nf = merge(6, 4, opening)
   sg = sum(area)
   x0 = 0d0
   dtnxt = dtinit
@@ -345,7 +379,7 @@ program lhbiem_gpu
   oo%dtout_inter = dtout * 365 * 24 * 3600
   oo%dtout_co = dtout_co
   rc = hbi_rk_set_output(rk, oo)
-  allocate(snap_k(oo%capacity), snap_t(oo%capacity), fields(4 * int(oo%capacity) * ncellg))
+  allocate(snap_k(oo%capacity), snap_t(oo%capacity), fields(nf * int(oo%capacity) * ncellg))
   allocate(events(oo%ev_capacity), eqslip(int(oo%ev_capacity) * ncellg))
   write(fname, '("output/monitor",i0,".dat")') number
   if (restart) then
@@ -362,6 +396,10 @@ program lhbiem_gpu
   call open_stream(82, 'sigma', restart)
   call open_stream(83, 'tau', restart)
   call open_stream(84, 'EQslip', restart)
+  if (opening) then
+    call open_stream(85, 'veln', restart)
+    call open_stream(86, 'slipn', restart)
+  end if
   if (.not. restart) call output_coord()
   allocate(hist(9 * outbatch), vhist(1))
   nrec = 0
@@ -412,6 +450,10 @@ program lhbiem_gpu
   do i = 80, 84
     close(i)
   end do
+  if (opening) then
+    close(85)
+    close(86)
+  end if
   rc = hbi_rk_get_state(rk, y, slip, vel, info)
   write(*, *) 'time(s)', t2 - t1, ' steps', kdone, ' rejected', info%rejected, ' t_end', info%t
   rc = hbi_rk_destroy(rk)
@@ -437,13 +479,33 @@ contains
     do j = 1, nsnap
       if (snap_k(j) > kend) then
         write(50, '(i7,f19.4)') snap_k(j), snap_t(j)
-        write(80) fields((j - 1) * 4 * n + 1:(j - 1) * 4 * n + n)
-        write(81) fields((j - 1) * 4 * n + n + 1:(j - 1) * 4 * n + 2 * n)
-        write(82) fields((j - 1) * 4 * n + 2 * n + 1:(j - 1) * 4 * n + 3 * n)
-        write(83) fields((j - 1) * 4 * n + 3 * n + 1:j * 4 * n)
+        write(80) fields((j - 1) * nf * n + 1:(j - 1) * nf * n + n)
+        write(81) fields((j - 1) * nf * n + n + 1:(j - 1) * nf * n + 2 * n)
+        write(82) fields((j - 1) * nf * n + 2 * n + 1:(j - 1) * nf * n + 3 * n)
+        write(83) fields((j - 1) * nf * n + 3 * n + 1:(j - 1) * nf * n + 4 * n)
+        if (opening) then
+          write(85) fields((j - 1) * nf * n + 4 * n + 1:(j - 1) * nf * n + 5 * n)
+          write(86) fields((j - 1) * nf * n + 5 * n + 1:j * nf * n)
+        end if
       end if
     end do
   end subroutine write_outputs
+
+  ! another H-matrix of the same mesh: slip mode md onto traction v
+  ! (main_LH.f90:698-716)
+  subroutine build_more(ctl, leafmtxp, lhp, md, v)
+    type(st_HACApK_lcontrol), intent(inout) :: ctl
+    type(st_HACApK_leafmtxp), intent(inout) :: leafmtxp
+    type(st_HACApK_LHp), intent(inout) :: lhp
+    character(len=*), intent(in) :: md, v
+    lrtrn = HACApK_init(ncellg, ctl, st_bemv)
+    ctl%param = st_ctl%param
+    st_bemv%md = md
+    st_bemv%v = v
+    lrtrn = HACApK_generate(leafmtxp, st_bemv, ctl, coord, eps_h)
+    if (lrtrn /= 0) stop 2
+    lrtrn = HACApK_construct_LH(lhp, leafmtxp, st_bemv, ctl, coord, eps_h)
+  end subroutine build_more
 
   subroutine open_text(u, path, append)
     integer, intent(in) :: u

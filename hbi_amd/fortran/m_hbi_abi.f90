@@ -109,6 +109,10 @@ module m_hbi_abi
       real(c_double), intent(out) :: y(*), slip(*), vel(*)
       type(hbi_rk_info), intent(out) :: info
     end function
+    integer(c_int) function hbi_rk_set_opening(r, hs2, hn2) bind(C, name="hbi_rk_set_opening")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: r, hs2, hn2
+    end function
     integer(c_int) function hbi_rk_set_output(r, o) bind(C, name="hbi_rk_set_output")
       import :: c_int, c_ptr, hbi_rk_out_opts
       type(c_ptr), value :: r

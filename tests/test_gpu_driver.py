@@ -192,3 +192,65 @@ def test_gpu_driver_restart_continues_the_run(tmp_path):
     assert len(vel2) == len(k2)
     # the first steps of the restarted file are the old file's, unchanged
     assert np.array_equal(vel2[int(np.where(k2 == 1200)[0][0])], snap1200)
+
+
+def test_gpu_driver_normal_stress_and_opening(tmp_path):
+    """sigmaconst F / opening T (main_LH.f90:683-716) on a rough triangular
+    fault read from STL: the driver builds the normal-stress matrix and the two
+    tensile-dislocation matrices; its monitor rows match a library run with the
+    same four matrices, and veln / slipn<n>.dat hold each snapshot's opening
+    rate and opening (post-step update, main_LH.f90:1279-1283)."""
+    import dataclasses
+    minsig, maxsig, nstep = 1.0, 300.0, 20
+    P.write_stl(str(tmp_path / "fault.stl"),
+                P.rough_surface_triangles(6.0, 3.0, 0.25, hurst=0.8, alpha=1e-2, seed=1))
+    gs = P.mesh_triangles(P.read_stl(str(tmp_path / "fault.stl")))
+    n = gs.n
+    fr = P.friction_c3(gs)
+    kw = dict(leaf=32, eps=1e-6)
+    Hs = H.HMatrix(gs, **kw)
+    lod = Hs.lod
+    y0 = P.init_state(fr, lod)
+    P.perturb_state(y0, gs, lod, amp=0.02, depth=2.0, width=1.0)
+    # a shallow patch below minsig (tau / sigma kept): it opens
+    low = np.asarray(gs.zcol)[lod] > -1.0
+    scale = np.where(low, 0.6 / y0[2::3], 1.0)
+    y0[1::3] *= scale
+    y0[2::3] *= scale
+    a = fr.a[lod]
+    for name, val in (("vel", 2 * fr.vref * np.exp(-y0[0::3] / a) * np.sinh(y0[1::3] / y0[2::3] / a)),
+                      ("tau", y0[1::3]), ("sigma", y0[2::3])):
+        arr = np.empty(n)
+        arr[lod] = val
+        setattr(fr, name, arr)
+    P.write_param_file(str(tmp_path / "param.dat"), fr)
+    _run(tmp_path, {"problem": "3dnt", "geometry_file": "fault.stl", "nstep": nstep, "eps_r": 1e-6,
+                    "eps_h": 1e-6, "leaf": 32, "filenumber": 4, "parameterfromfile": True,
+                    "parameter_file": "param.dat", "parameter_file_ncol": 8, "outbatch": 8, "evlaw": "slip",
+                    "vref": 1e-6, "tmax": 1e4, "interval": 1, "velth": 1e3, "sigmaconst": False,
+                    "opening": True, "minsig": minsig, "maxsig": maxsig})
+    out = tmp_path / "output"
+    mon = np.loadtxt(out / "monitor4.dat")[1:]
+    Hn, Hos, Hon = (H.HMatrix(dataclasses.replace(gs, **r), **kw)
+                    for r in ({"vcomp": 1}, {"md": 3}, {"md": 3, "vcomp": 1}))
+    rk = H.RK(Hs, fr, lod, gs.ds, eps_r=1e-6, hn=Hn, minsig=minsig, maxsig=maxsig)
+    rk.set_opening(Hos, Hon)
+    rk.set_state(P.init_state(fr, lod), 0.0, 1.0)
+    info, hist, _ = rk.step(nstep, history=True)
+    assert info["accepted"] == nstep and len(mon) == nstep
+    assert np.allclose(mon[:, 1], hist[:, 0], rtol=1e-6, atol=1e-3)
+    # log10 Vmax, mean slip, mean mu, max / min sigma (sigma evolves: hn is on)
+    for col, hcol in ((2, 4), (3, 5), (4, 6), (5, 7), (6, 8)):
+        assert np.allclose(mon[:, col], hist[:, hcol], rtol=6e-5, atol=1e-30), col
+    ks = np.loadtxt(out / "time4.dat", ndmin=2)[:, 0].astype(int)
+    assert list(ks) == list(range(1, nstep + 1))
+    vel, sig = _stream(out / "vel4.dat", n), _stream(out / "sigma4.dat", n)
+    veln, slipn = _stream(out / "veln4.dat", n), _stream(out / "slipn4.dat", n)
+    assert len(veln) == len(slipn) == len(vel) == nstep
+    assert np.abs(sig[-1] - y0[2::3]).max() > 1e-9 * np.abs(y0[2::3]).max()
+    for j in range(nstep):
+        vn = vel[j] * (np.maximum(0.0, minsig - sig[j]) + np.minimum(0.0, maxsig - sig[j]))
+        assert np.array_equal(veln[j], vn)
+    assert np.abs(slipn[-1][low]).min() > 0 and np.abs(slipn[-1][~low]).max() == 0
+    vn, sn = rk.get_opening()
+    assert np.allclose(veln[-1], vn, rtol=1e-6, atol=1e-30) and np.allclose(slipn[-1], sn, rtol=1e-6, atol=1e-30)
