@@ -6,6 +6,9 @@
 //   3dph   Okada (1992) rectangle, half-space  m_HACApK_calc_entry_ij.f90:1065-1145, okada.f90
 //   3dnr   the same without the image part
 //   3dnt   Nikkhoo & Walter (2015) triangle     m_HACApK_calc_entry_ij.f90:802-851, TDstressFS.f90
+//   3dht   the same in a half-space             m_HACApK_calc_entry_ij.f90:852-898, TDstressFS.f90:7-44,
+//          (main + image + free-surface correction; the correction's strains
+//          are forward-mode derivatives of its displacement field, see fsc_strain)
 // The closed-form expressions are those of the published solutions written
 // term by term in the reference's evaluation order; FP contraction is off in
 // every function here so that the device result tracks the FP64 CPU result
@@ -468,6 +471,224 @@ __device__ __noinline__ void td_stress_fs(double X, double Y, double Z, const do
     Sout[5] = 2 * mu * E[5];
 }
 
+// ------------------------------------------------- half-space triangles ----
+// Value and gradient with respect to the three coordinates of the point.
+struct Dual {
+    double v, d[3];
+};
+__device__ __forceinline__ Dual dvar(double v, int k)
+{
+    Dual r{v, {0.0, 0.0, 0.0}};
+    r.d[k] = 1.0;
+    return r;
+}
+__device__ __forceinline__ Dual operator+(Dual a, Dual b)
+{
+#pragma clang fp contract(off)
+    return {a.v + b.v, {a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2]}};
+}
+__device__ __forceinline__ Dual operator-(Dual a, Dual b)
+{
+#pragma clang fp contract(off)
+    return {a.v - b.v, {a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2]}};
+}
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, {-a.d[0], -a.d[1], -a.d[2]}}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b)
+{
+#pragma clang fp contract(off)
+    return {a.v * b.v, {a.d[0] * b.v + a.v * b.d[0], a.d[1] * b.v + a.v * b.d[1], a.d[2] * b.v + a.v * b.d[2]}};
+}
+__device__ __forceinline__ Dual operator/(Dual a, Dual b)
+{
+#pragma clang fp contract(off)
+    const double q = a.v / b.v;
+    return {q, {(a.d[0] - q * b.d[0]) / b.v, (a.d[1] - q * b.d[1]) / b.v, (a.d[2] - q * b.d[2]) / b.v}};
+}
+__device__ __forceinline__ Dual operator+(Dual a, double c) { return {a.v + c, {a.d[0], a.d[1], a.d[2]}}; }
+__device__ __forceinline__ Dual operator+(double c, Dual a) { return a + c; }
+__device__ __forceinline__ Dual operator-(Dual a, double c) { return {a.v - c, {a.d[0], a.d[1], a.d[2]}}; }
+__device__ __forceinline__ Dual operator-(double c, Dual a) { return {c - a.v, {-a.d[0], -a.d[1], -a.d[2]}}; }
+__device__ __forceinline__ Dual operator*(Dual a, double c) { return {a.v * c, {a.d[0] * c, a.d[1] * c, a.d[2] * c}}; }
+__device__ __forceinline__ Dual operator*(double c, Dual a) { return a * c; }
+__device__ __forceinline__ Dual operator/(Dual a, double c) { return {a.v / c, {a.d[0] / c, a.d[1] / c, a.d[2] / c}}; }
+__device__ __forceinline__ Dual operator/(double c, Dual a)
+{
+#pragma clang fp contract(off)
+    const double q = c / a.v;
+    return {q, {-q * a.d[0] / a.v, -q * a.d[1] / a.v, -q * a.d[2] / a.v}};
+}
+__device__ __forceinline__ Dual dsqrt(Dual a)
+{
+#pragma clang fp contract(off)
+    const double r = sqrt(a.v);
+    return {r, {a.d[0] / (2 * r), a.d[1] / (2 * r), a.d[2] / (2 * r)}};
+}
+__device__ __forceinline__ Dual dlog(Dual a) { return {log(a.v), {a.d[0] / a.v, a.d[1] / a.v, a.d[2] / a.v}}; }
+
+// Displacement of the free-surface correction of an angular dislocation at
+// depth a with angle beta (Nikkhoo & Walter 2015, GJI 201:1117-1139, the
+// harmonic half of the half-space solution), one Burgers component at a time;
+// fib carries the gradient of the Burgers function (its value never reaches
+// a strain).
+__device__ __noinline__ void fsc_disp(Dual y1, Dual y2, Dual y3, Dual fib, double beta, double b1, double b2,
+                                      double b3, double nu, double a, Dual *u)
+{
+#pragma clang fp contract(off)
+    const double pi = dpi();
+    const double sinB = sin(beta), cosB = cos(beta), cotB = cosB / sinB, cot2 = cotB * cotB;
+    const double N1 = 1 - 2 * nu, f = 1 / (4 * pi * (1 - nu));
+    const Dual y3b = y3 + 2 * a;
+    const Dual z1b = y1 * cosB + y3b * sinB;
+    const Dual z3b = -y1 * sinB + y3b * cosB;
+    const Dual rb = dsqrt(y1 * y1 + y2 * y2 + y3b * y3b);
+    const Dual rr = rb * rb, r3 = rr * rb;
+    const Dual ar = a / rb, ar2 = a / rr;
+    const Dual p6 = rb + y3b, p7 = rb + z3b;
+    const Dual h = y3b - a;
+    const Dual q = rb * cosB + y3b;
+    Dual u1{0.0, {0.0, 0.0, 0.0}}, u2 = u1, u3 = u1;
+    if (b1 != 0.0) {
+        u1 = b1 * f * (-2 * (1 - nu) * N1 * fib * cot2 + N1 * y2 / p6 * ((N1 - ar) * cotB - y1 / p6 * (nu + ar))
+             + N1 * y2 * cosB * cotB / p7 * (cosB + ar) + a * y2 * h * cotB / r3
+             + y2 * h / (rb * p6) * (-N1 * cotB + y1 / p6 * (2 * nu + ar) + a * y1 / rr)
+             + y2 * h / (rb * p7) * (cosB / p7 * (q * (N1 * cosB - ar) * cotB + 2 * (1 - nu) * (rb * sinB - y1) * cosB)
+                                     - a * y3b * cosB * cotB / rr));
+        u2 = b1 * f * (N1 * ((2 * (1 - nu) * cot2 - nu) * dlog(p6) - (2 * (1 - nu) * cot2 + N1) * cosB * dlog(p7))
+             - N1 / p6 * (y1 * cotB * (N1 - ar) + nu * y3b - a + y2 * y2 / p6 * (nu + ar))
+             - N1 * z1b * cotB / p7 * (cosB + ar) - a * y1 * h * cotB / r3
+             + h / p6 * (-2 * nu + (N1 * y1 * cotB - a) / rb + y2 * y2 / (rb * p6) * (2 * nu + ar) + a * y2 * y2 / r3)
+             + h / p7 * (cosB * cosB - (N1 * z1b * cotB + a * cosB) / rb + a * y3b * z1b * cotB / r3
+                         - (y2 * y2 * cosB * cosB - a * z1b * cotB / rb * q) / (rb * p7)));
+        u3 = b1 * f * (2 * (1 - nu) * (N1 * fib * cotB + y2 / p6 * (2 * nu + ar) - y2 * cosB / p7 * (cosB + ar))
+             + y2 * h / rb * (2 * nu / p6 + ar2) + y2 * h * cosB / (rb * p7) * (N1 - q / p7 * (cosB + ar) - a * y3b / rr));
+    }
+    if (b2 != 0.0) {
+        u1 = u1 + b2 * f * (N1 * ((2 * (1 - nu) * cot2 + nu) * dlog(p6) - (2 * (1 - nu) * cot2 + 1) * cosB * dlog(p7))
+             + N1 / p6 * (-N1 * y1 * cotB + nu * y3b - a + a * y1 * cotB / rb + y1 * y1 / p6 * (nu + ar))
+             - N1 * cotB / p7 * (z1b * cosB - a * (rb * sinB - y1) / (rb * cosB)) - a * y1 * h * cotB / r3
+             + h / p6 * (2 * nu + (N1 * y1 * cotB + a) / rb - y1 * y1 / (rb * p6) * (2 * nu + ar) - a * y1 * y1 / r3)
+             + h * cotB / p7 * (-cosB * sinB + a * y1 * y3b / (r3 * cosB)
+                                + (rb * sinB - y1) / rb * (2 * (1 - nu) * cosB - q / p7 * (1 + a / (rb * cosB)))));
+        u2 = u2 + b2 * f * (2 * (1 - nu) * N1 * fib * cot2 + N1 * y2 / p6 * (-(N1 - ar) * cotB + y1 / p6 * (nu + ar))
+             - N1 * y2 * cotB / p7 * (1 + a / (rb * cosB)) - a * y2 * h * cotB / r3
+             + y2 * h / (rb * p6) * (N1 * cotB - 2 * nu * y1 / p6 - a * y1 / rb * (1 / rb + 1 / p6))
+             + y2 * h * cotB / (rb * p7) * (-2 * (1 - nu) * cosB + q / p7 * (1 + a / (rb * cosB)) + a * y3b / (rr * cosB)));
+        u3 = u3 + b2 * f * (-2 * (1 - nu) * N1 * cotB * (dlog(p6) - cosB * dlog(p7)) - 2 * (1 - nu) * y1 / p6 * (2 * nu + ar)
+             + 2 * (1 - nu) * z1b / p7 * (cosB + ar) + h / rb * (N1 * cotB - 2 * nu * y1 / p6 - a * y1 / rr)
+             - h / p7 * (cosB * sinB + q * cotB / rb * (2 * (1 - nu) * cosB - q / p7)
+                         + ar * (sinB - y3b * z1b / rr - z1b * q / (rb * p7))));
+    }
+    if (b3 != 0.0) {
+        u1 = u1 + b3 * f * (N1 * (y2 / p6 * (1 + ar) - y2 * cosB / p7 * (cosB + ar)) - y2 * h / rb * (ar2 + 1 / p6)
+             + y2 * h * cosB / (rb * p7) * (q / p7 * (cosB + ar) + a * y3b / rr));
+        u2 = u2 + b3 * f * (N1 * (-sinB * dlog(p7) - y1 / p6 * (1 + ar) + z1b / p7 * (cosB + ar)) + y1 * h / rb * (ar2 + 1 / p6)
+             - h / p7 * (sinB * (cosB - ar) + z1b / rb * (1 + a * y3b / rr)
+                         - (y2 * y2 * cosB * sinB - a * z1b / rb * q) / (rb * p7)));
+        u3 = u3 + b3 * f * (2 * (1 - nu) * fib + 2 * (1 - nu) * y2 * sinB / p7 * (cosB + ar)
+             + y2 * h * sinB / (rb * p7) * (1 + q / p7 * (cosB + ar) + a * y3b / rr));
+    }
+    u[0] = u1; u[1] = u2; u[2] = u3;
+}
+
+// free-surface correction strains v11 v22 v33 v12 v13 v23 of an angular
+// dislocation (the quantity of AngDisStrainFSC, TDstressFS.f90:629-1011)
+__device__ __noinline__ void fsc_strain(double y1, double y2, double y3, double beta, double b1, double b2, double b3,
+                                        double nu, double a, double *v)
+{
+#pragma clang fp contract(off)
+    const double y3b = y3 + 2 * a;
+    const double rb = sqrt(y1 * y1 + y2 * y2 + y3b * y3b);
+    const double c = cos(beta / 2) / sin(beta / 2);
+    const double D = y1 - (rb + y3b) * c, den = D * D + y2 * y2;
+    // gradient of 2 atan(-y2 / (y1 - (r + y3bar) cot(beta/2)))
+    const Dual fib{0.0, {2 * y2 * (1 - c * y1 / rb) / den, -2 * (D + c * y2 * y2 / rb) / den,
+                         -2 * y2 * c * (y3b / rb + 1) / den}};
+    Dual u[3];
+    fsc_disp(dvar(y1, 0), dvar(y2, 1), dvar(y3, 2), fib, beta, b1, b2, b3, nu, a, u);
+    v[0] = u[0].d[0];
+    v[1] = u[1].d[1];
+    v[2] = u[2].d[2];
+    v[3] = (u[0].d[1] + u[1].d[0]) / 2;
+    v[4] = (u[0].d[2] + u[2].d[0]) / 2;
+    v[5] = (u[1].d[2] + u[2].d[1]) / 2;
+}
+
+// AngSetupFSC_S, TDstressFS.f90:405-558: the correction stress of side PA->PB
+// (zero for a vertical side)
+__device__ __noinline__ void fsc_side(double X, double Y, double Z, const double *bv, const double *PA, const double *PB,
+                                      double mu, double lambda, double *S)
+{
+#pragma clang fp contract(off)
+    const double pi = dpi();
+    const double nu = lambda / (lambda + mu) / 2;
+    const double sv[3] = {PB[0] - PA[0], PB[1] - PA[1], PB[2] - PA[2]};
+    const double beta = acos(-sv[2] / sqrt(dot3(sv, sv)));
+    for (int k = 0; k < 6; k++) S[k] = 0.0;
+    if (fabs(beta) < 1e-3 || fabs(pi - beta) < 1e-3) return;
+    double e1[3] = {sv[0], sv[1], 0.0};
+    const double nn = sqrt(dot3(e1, e1));
+    for (int k = 0; k < 3; k++) e1[k] = e1[k] / nn;
+    const double e3[3] = {0.0, 0.0, -1.0};
+    double e2[3];
+    cross3(e3, e1, e2);
+    double A[3][3];
+    for (int k = 0; k < 3; k++) { A[0][k] = e1[k]; A[1][k] = e2[k]; A[2][k] = e3[k]; }
+    const double rel[3] = {X - PA[0], Y - PA[1], Z - PA[2]};
+    double ya[3], yab[3], bb[3];
+    for (int r = 0; r < 3; r++) {
+        ya[r] = A[r][0] * rel[0] + A[r][1] * rel[1] + A[r][2] * rel[2];
+        yab[r] = A[r][0] * sv[0] + A[r][1] * sv[1] + A[r][2] * sv[2];
+        bb[r] = A[r][0] * bv[0] + A[r][1] * bv[1] + A[r][2] * bv[2];
+    }
+    const double ang = (beta * ya[0] >= 0.0) ? -pi + beta : beta;
+    double vA[6], vB[6], v[6], E[6];
+    fsc_strain(ya[0], ya[1], ya[2], ang, bb[0], bb[1], bb[2], nu, -PA[2], vA);
+    fsc_strain(ya[0] - yab[0], ya[1] - yab[1], ya[2] - yab[2], ang, bb[0], bb[1], bb[2], nu, -PB[2], vB);
+    for (int k = 0; k < 6; k++) v[k] = vB[k] - vA[k];
+    tens_trans(v, A, E);
+    const double tr = E[0] + E[1] + E[2];
+    S[0] = 2 * mu * E[0] + lambda * tr;
+    S[1] = 2 * mu * E[1] + lambda * tr;
+    S[2] = 2 * mu * E[2] + lambda * tr;
+    S[3] = 2 * mu * E[3];
+    S[4] = 2 * mu * E[4];
+    S[5] = 2 * mu * E[5];
+}
+
+// TDstressHS + TDstress_HarFunc, TDstressFS.f90:7-44, 236-293
+__device__ __noinline__ void td_stress_hs(double X, double Y, double Z, const double *P1, const double *P2, const double *P3,
+                                          double Ss, double Ds, double Ts, double mu, double lambda, double *S)
+{
+#pragma clang fp contract(off)
+    double Sm[6], Si[6], s1[6], s2[6], s3[6];
+    td_stress_fs(X, Y, Z, P1, P2, P3, Ss, Ds, Ts, mu, lambda, Sm);
+    double a[3], b[3], vn[3], vs[3], vd[3];
+    for (int k = 0; k < 3; k++) { a[k] = P2[k] - P1[k]; b[k] = P3[k] - P1[k]; }
+    cross3(a, b, vn);
+    double nn = sqrt(dot3(vn, vn));
+    for (int k = 0; k < 3; k++) vn[k] = vn[k] / nn;
+    const double eY[3] = {0.0, 1.0, 0.0}, eZ[3] = {0.0, 0.0, 1.0};
+    cross3(eZ, vn, vs);
+    // normalised before the zero test, as there: a horizontal triangle gives NaN
+    nn = sqrt(dot3(vs, vs));
+    for (int k = 0; k < 3; k++) vs[k] = vs[k] / nn;
+    if (dot3(vs, vs) == 0.0)
+        for (int k = 0; k < 3; k++) vs[k] = eY[k] * vn[2];
+    nn = sqrt(dot3(vs, vs));
+    for (int k = 0; k < 3; k++) vs[k] = vs[k] / nn;
+    cross3(vn, vs, vd);
+    double bg[3];
+    for (int k = 0; k < 3; k++) bg[k] = vn[k] * Ts + vs[k] * Ss + vd[k] * Ds;
+    fsc_side(X, Y, Z, bg, P1, P2, mu, lambda, s1);
+    fsc_side(X, Y, Z, bg, P2, P3, mu, lambda, s2);
+    fsc_side(X, Y, Z, bg, P3, P1, mu, lambda, s3);
+    const double Q1[3] = {P1[0], P1[1], -P1[2]}, Q2[3] = {P2[0], P2[1], -P2[2]}, Q3[3] = {P3[0], P3[1], -P3[2]};
+    td_stress_fs(X, Y, Z, Q1, Q2, Q3, Ss, Ds, Ts, mu, lambda, Si);
+    for (int k = 0; k < 6; k++) S[k] = Sm[k] + Si[k] + (s1[k] + s2[k] + s3[k]);
+}
+
+// matel3dn_ij / matel3dh_ij (m_HACApK_calc_entry_ij.f90:802-898): mode 's'
+// takes the receiver's rake in the full space, the source's in the half-space
 __device__ __noinline__ double tri_entry(const DevGeom &g, int i, int j)
 {
 #pragma clang fp contract(off)
@@ -475,13 +696,15 @@ __device__ __noinline__ double tri_entry(const DevGeom &g, int i, int j)
     const double P1[3] = {g.xs1[j], g.ys1[j], g.zs1[j]};
     const double P2[3] = {g.xs2[j], g.ys2[j], g.zs2[j]};
     const double P3[3] = {g.xs3[j], g.ys3[j], g.zs3[j]};
+    const bool half = g.kind == 5;
     double ss = 0.0, dd = 0.0, tt = 0.0;
     if (g.md == 1) ss = 1.0;
     else if (g.md == 2) dd = 1.0;
     else if (g.md == 3) tt = 1.0;
-    else { ss = cos(g.rake[i]); dd = sin(g.rake[i]); }
+    else { const int r = half ? j : i; ss = cos(g.rake[r]); dd = sin(g.rake[r]); }
     double S[6];
-    td_stress_fs(g.xcol[i], g.ycol[i], g.zcol[i], P1, P2, P3, ss, dd, tt, kRigid, lambda, S);
+    if (half) td_stress_hs(g.xcol[i], g.ycol[i], g.zcol[i], P1, P2, P3, ss, dd, tt, kRigid, lambda, S);
+    else td_stress_fs(g.xcol[i], g.ycol[i], g.zcol[i], P1, P2, P3, ss, dd, tt, kRigid, lambda, S);
     const double A[3][3] = {{g.ev11[i], g.ev21[i], g.ev31[i]},
                             {g.ev12[i], g.ev22[i], g.ev32[i]},
                             {g.ev13[i], g.ev23[i], g.ev33[i]}};
@@ -505,7 +728,8 @@ __device__ __forceinline__ double entry(const DevGeom &g, int i, int j)
     case 1: return antiplane(g.xcol[i], g.xel[j], g.xer[j]) - antiplane(g.xcol[i], -g.xel[j], -g.xer[j]);
     case 2: return okada_entry(g, i, j, false);
     case 3: return okada_entry(g, i, j, true);
-    case 4: return tri_entry(g, i, j);
+    case 4:
+    case 5: return tri_entry(g, i, j);
     }
     return __builtin_nan("");
 }

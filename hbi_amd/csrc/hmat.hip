@@ -998,13 +998,13 @@ static void upload_geom(hbi_hmat *h, const hbi_geom *g)
 static void check_geom(const hbi_geom *g)
 {
     HBI_REQUIRE(g && g->n > 0, "geometry with n > 0 required");
-    HBI_REQUIRE(g->kind >= 1 && g->kind <= 4, "unknown geometry kind");
+    HBI_REQUIRE(g->kind >= 1 && g->kind <= 5, "unknown geometry kind");
     HBI_REQUIRE(g->xcol, "xcol required");
     if (g->kind == HBI_KIND_2DVS) HBI_REQUIRE(g->xel && g->xer, "2dvs needs xel, xer");
     if (g->kind == HBI_KIND_OKADA_HS || g->kind == HBI_KIND_OKADA_FS)
         HBI_REQUIRE(g->ycol && g->zcol && g->ang && g->angd && g->dsl && g->dsd && g->rake && g->w > 0,
                     "okada kernels need ycol, zcol, ang, angd, dsl, dsd, rake and w > 0");
-    if (g->kind == HBI_KIND_TRI_FS)
+    if (g->kind == HBI_KIND_TRI_FS || g->kind == HBI_KIND_TRI_HS)
         HBI_REQUIRE(g->ycol && g->zcol && g->xs1 && g->xs2 && g->xs3 && g->ys1 && g->ys2 && g->ys3 && g->zs1 &&
                         g->zs2 && g->zs3 && g->ev11 && g->ev12 && g->ev13 && g->ev21 && g->ev22 && g->ev23 &&
                         g->ev31 && g->ev32 && g->ev33 && g->rake,

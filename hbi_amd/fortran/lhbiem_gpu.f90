@@ -3,7 +3,8 @@
 ! Reads the reference's input formats (the `key value` .in file of
 ! main_LH.f90:182-328 with its defaults 133-180, the column-named parameter
 ! file 530-581, ASCII STL facets 485-510), generates the meshes of the
-! reference (2dp/2dvs 1701-1718, 3dph 1781-1835, STL frames 1940-1990), builds
+! reference (2dp/2dvs 1701-1718, 3dph 1781-1835, STL frames 1940-1990 for 3dnt
+! and 3dht), builds
 ! the H-matrix through the HACApK call sites (m_HACApK_gpu) and advances the
 ! quasi-dynamic earthquake cycle with the device RK45 (hbi_rk_step), writing
 ! the reference's output files: monitor<n>.dat (main_LH.f90:1489-1493
@@ -140,7 +141,7 @@ program lhbiem_gpu
   select case (trim(problem))
   case ('3dph')
     ncellg = imax * jmax
-  case ('3dnt')
+  case ('3dnt', '3dht')
     ncellg = count_stl_facets(geofile)
   end select
   if (ncellg == 0) then
@@ -164,7 +165,7 @@ program lhbiem_gpu
     ds = ds0 * ds0
     dsl = ds0
     dsd = ds0
-  case ('3dnt')
+  case ('3dnt', '3dht')
     allocate(xs(ncellg, 9), ev(ncellg, 9))
     call read_stl(geofile, ncellg, xs, xcol, ycol, zcol)
     call evcalc(ncellg, xs, ev, ds)
@@ -229,7 +230,7 @@ program lhbiem_gpu
   case ('3dph')
     st_bemv%ang => ang; st_bemv%angd => angd; st_bemv%dsl => dsl; st_bemv%dsd => dsd
     st_bemv%w = ds0
-  case ('3dnt')
+  case ('3dnt', '3dht')
     st_bemv%xs1 => xs(:, 1); st_bemv%xs2 => xs(:, 2); st_bemv%xs3 => xs(:, 3)
     st_bemv%ys1 => xs(:, 4); st_bemv%ys2 => xs(:, 5); st_bemv%ys3 => xs(:, 6)
     st_bemv%zs1 => xs(:, 7); st_bemv%zs2 => xs(:, 8); st_bemv%zs3 => xs(:, 9)

@@ -98,6 +98,8 @@ contains
       problem_kind = HBI_KIND_OKADA_FS
     case ('3dnt', '3dn')
       problem_kind = HBI_KIND_TRI_FS
+    case ('3dht', '3dh')
+      problem_kind = HBI_KIND_TRI_HS
     case default
       problem_kind = 0
     end select

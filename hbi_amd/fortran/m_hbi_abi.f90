@@ -4,7 +4,8 @@ module m_hbi_abi
   implicit none
 
   integer(c_int), parameter :: HBI_OK = 0, HBI_E_STEP = -4, HBI_RK_OUTPUT_FULL = 2
-  integer(c_int), parameter :: HBI_KIND_2DVS = 1, HBI_KIND_OKADA_HS = 2, HBI_KIND_OKADA_FS = 3, HBI_KIND_TRI_FS = 4
+  integer(c_int), parameter :: HBI_KIND_2DVS = 1, HBI_KIND_OKADA_HS = 2, HBI_KIND_OKADA_FS = 3, HBI_KIND_TRI_FS = 4, &
+    & HBI_KIND_TRI_HS = 5
   integer(c_int), parameter :: HBI_PREC_F64 = 0, HBI_PREC_F32 = 1, HBI_PREC_MIXED = 2
 
   type, bind(C) :: hbi_geom

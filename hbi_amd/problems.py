@@ -30,6 +30,7 @@ KIND_2DVS = 1
 KIND_OKADA_HS = 2     # 3dph / 3dhr
 KIND_OKADA_FS = 3     # 3dnr
 KIND_TRI_FS = 4       # 3dnt
+KIND_TRI_HS = 5       # 3dht
 
 _GEO_FIELDS = ("xcol", "ycol", "zcol", "xel", "xer", "ang", "angd", "dsl", "dsd", "rake",
                "xs1", "xs2", "xs3", "ys1", "ys2", "ys3", "zs1", "zs2", "zs3",
@@ -145,8 +146,9 @@ def evcalc(xs1, xs2, xs3, ys1, ys2, ys3, zs1, zs2, zs3):
                 ev31=ev31, ev32=ev32, ev33=ev33), ds
 
 
-def mesh_triangles(tri: np.ndarray, crake: float = 0.0) -> Geometry:
-    """Triangle mesh (3dnt) from an (n, 3, 3) vertex array [element, vertex, xyz]."""
+def mesh_triangles(tri: np.ndarray, crake: float = 0.0, halfspace: bool = False) -> Geometry:
+    """Triangle mesh from an (n, 3, 3) vertex array [element, vertex, xyz]:
+    full space (3dnt) or, with ``halfspace``, below a free surface at z = 0 (3dht)."""
     tri = np.asarray(tri, dtype=np.float64)
     n = tri.shape[0]
     xs1, xs2, xs3 = tri[:, 0, 0].copy(), tri[:, 1, 0].copy(), tri[:, 2, 0].copy()
@@ -159,7 +161,7 @@ def mesh_triangles(tri: np.ndarray, crake: float = 0.0) -> Geometry:
     arr = dict(xcol=xcol, ycol=ycol, zcol=zcol, xs1=xs1, xs2=xs2, xs3=xs3, ys1=ys1, ys2=ys2, ys3=ys3,
                zs1=zs1, zs2=zs2, zs3=zs3, rake=np.full(n, crake / 180.0 * PI), **ev)
     arr = {k: _c(v) for k, v in arr.items()}
-    return Geometry(kind=KIND_TRI_FS, n=n, ds=ds, ds0=float(np.sqrt(2 * ds).min()), arrays=arr)
+    return Geometry(kind=KIND_TRI_HS if halfspace else KIND_TRI_FS, n=n, ds=ds, ds0=float(np.sqrt(2 * ds).min()), arrays=arr)
 
 
 def rough_surface_triangles(length: float, width: float, edge: float, hurst: float = 0.8,
@@ -197,6 +199,26 @@ def rough_surface_triangles(length: float, width: float, edge: float, hurst: flo
             p11 = (xs[i + 1], h[i + 1, j + 1], zs[j + 1])
             tris.append((p00, p01, p10))
             tris.append((p10, p01, p11))
+    return np.asarray(tris, dtype=np.float64)
+
+
+def plane_triangles(length: float, width: float, edge: float, dip: float = 90.0) -> np.ndarray:
+    """Planar fault from the free surface down dip, meshed as right triangles
+    on a regular (along-strike, down-dip) grid of spacing ``edge``: x in
+    [-length/2, length/2], down-dip extent ``width`` at ``dip`` degrees
+    (y = -s cos(dip), z = -s sin(dip) at down-dip distance s).  Returns
+    (n, 3, 3) vertices."""
+    nx = max(1, int(round(length / edge)))
+    ns = max(1, int(round(width / edge)))
+    xs = np.linspace(-0.5 * length, 0.5 * length, nx + 1)
+    ss = np.linspace(0.0, width, ns + 1)
+    cd, sd = np.cos(dip * PI / 180.0), np.sin(dip * PI / 180.0)
+    tris = []
+    for i in range(nx):
+        for j in range(ns):
+            p = {(a, b): (xs[i + a], -ss[j + b] * cd, -ss[j + b] * sd) for a in (0, 1) for b in (0, 1)}
+            tris.append((p[0, 0], p[0, 1], p[1, 0]))
+            tris.append((p[1, 0], p[0, 1], p[1, 1]))
     return np.asarray(tris, dtype=np.float64)
 
 

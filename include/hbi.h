@@ -43,6 +43,7 @@ extern "C" {
 #define HBI_KIND_OKADA_HS 2   /* rectangles, half-space: '3dph', '3dhr' */
 #define HBI_KIND_OKADA_FS 3   /* rectangles, full space: '3dnr' */
 #define HBI_KIND_TRI_FS 4     /* triangles, full space: '3dnt' */
+#define HBI_KIND_TRI_HS 5     /* triangles, half-space: '3dht' (z <= 0) */
 
 /* storage precision of the H-matrix factors used by a product */
 #define HBI_PREC_F64 0        /* FP64 factors (the reference's arithmetic) */

@@ -11,6 +11,9 @@
  *   - 2-D antiplane kernel with image            m_HACApK_calc_entry_ij.f90:55-57, 685-695
  *   - Okada rectangular kernel                   m_HACApK_calc_entry_ij.f90:1065-1145, okada.f90:10-597
  *   - TD full-space kernel                       m_HACApK_calc_entry_ij.f90:802-851, TDstressFS.f90:45-403, 560-627, 1012-1019
+ *   - TD half-space kernel                       m_HACApK_calc_entry_ij.f90:852-898, TDstressFS.f90:7-44, 236-293, 405-558,
+ *                                                629-1011 (free-surface correction: strains as complex-step
+ *                                                derivatives of the published harmonic displacement field)
  *   - geometric cluster tree + bounding boxes    m_HACApK_base.f90:3191-3333
  *   - leaf partition (np=1 lattice) + sort       m_HACApK_base.f90:2372-2460, 2937-2995, 3114-3166
  *   - ACA+ and dense fill                        m_HACApK_base.f90:1638-1946, 1949-2043; HACApK_lib.f90:31-116
@@ -33,6 +36,7 @@
  *   RK45         -> order tests on y' = lambda y, tableau identities
  *   friction     -> steady-state invariants, finite differences
  */
+#include <complex.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -51,7 +55,8 @@ static const double og_vs = 3.464;
 /* geometry                                                                  */
 /* ------------------------------------------------------------------------- */
 typedef struct {
-    int kind;   /* 1: 2dvs antiplane, 2: okada half-space (3dph/3dhr), 3: okada full-space (3dnr), 4: TD full-space (3dnt) */
+    int kind;   /* 1: 2dvs antiplane, 2: okada half-space (3dph/3dhr), 3: okada full-space (3dnr), 4: TD full-space (3dnt),
+                   5: TD half-space (3dht) */
     int n;
     int vcomp;  /* 0: shear traction ('s'), 1: normal traction ('n') */
     int md;     /* TD slip mode: 0 's' (receiver rake), 1 'st', 2 'dp', 3 'open' */
@@ -613,7 +618,194 @@ void og_tdstressfs(double X, double Y, double Z, const double *P1, const double 
     S[5] = 2 * mu * E[5];
 }
 
-/* matel3dn_ij, m_HACApK_calc_entry_ij.f90:802-851 */
+/* ---- half-space triangles ------------------------------------------------
+ * The free-surface correction of an angular dislocation in a half-space
+ * (Nikkhoo & Walter 2015, GJI 201:1117-1139; the strain routine is
+ * AngDisStrainFSC, TDstressFS.f90:629-1011) is the gradient of a harmonic
+ * displacement field.  Here the displacement is written out and its gradient
+ * taken by the complex-step method: u(y + i h e_k) = u(y) + i h du/dy_k + O(h^2)
+ * for the real-analytic terms (products, quotients, sqrt, log of positive
+ * arguments), so Im(u)/h is the derivative to rounding with h far below it.
+ * The Burgers function enters only through its gradient, supplied in closed
+ * form (its value carries a branch cut, its gradient does not).
+ */
+typedef double complex og_c;
+
+static void og_fsc_disp(og_c y1, og_c y2, og_c y3, og_c fib, double beta, double b1, double b2, double b3,
+                        double nu, double a, og_c *u)
+{
+    const double pi = OG_PI;
+    double sinB = sin(beta), cosB = cos(beta), cotB = cosB / sinB, cot2 = cotB * cotB;
+    double N1 = 1 - 2 * nu, f = 1 / (4 * pi * (1 - nu));
+    og_c y3b = y3 + 2 * a;
+    og_c z1b = y1 * cosB + y3b * sinB;
+    og_c z3b = -y1 * sinB + y3b * cosB;
+    og_c rb = csqrt(y1 * y1 + y2 * y2 + y3b * y3b);
+    og_c ar = a / rb, ar2 = a / (rb * rb);
+    og_c p6 = rb + y3b;            /* r + y3bar */
+    og_c p7 = rb + z3b;            /* r + z3bar */
+    og_c h = y3b - a;              /* y3 + a */
+    og_c q = rb * cosB + y3b;
+    og_c lg6 = clog(p6), lg7 = clog(p7);
+    og_c u1 = 0, u2 = 0, u3 = 0;
+    if (b1 != 0.0) {
+        u1 += b1 * f * (-2 * (1 - nu) * N1 * fib * cot2 + N1 * y2 / p6 * ((N1 - ar) * cotB - y1 / p6 * (nu + ar))
+              + N1 * y2 * cosB * cotB / p7 * (cosB + ar) + a * y2 * h * cotB / (rb * rb * rb)
+              + y2 * h / (rb * p6) * (-N1 * cotB + y1 / p6 * (2 * nu + ar) + a * y1 / (rb * rb))
+              + y2 * h / (rb * p7) * (cosB / p7 * (q * (N1 * cosB - ar) * cotB + 2 * (1 - nu) * (rb * sinB - y1) * cosB)
+                                      - a * y3b * cosB * cotB / (rb * rb)));
+        u2 += b1 * f * (N1 * ((2 * (1 - nu) * cot2 - nu) * lg6 - (2 * (1 - nu) * cot2 + N1) * cosB * lg7)
+              - N1 / p6 * (y1 * cotB * (N1 - ar) + nu * y3b - a + y2 * y2 / p6 * (nu + ar))
+              - N1 * z1b * cotB / p7 * (cosB + ar) - a * y1 * h * cotB / (rb * rb * rb)
+              + h / p6 * (-2 * nu + (N1 * y1 * cotB - a) / rb + y2 * y2 / (rb * p6) * (2 * nu + ar) + a * y2 * y2 / (rb * rb * rb))
+              + h / p7 * (cosB * cosB - (N1 * z1b * cotB + a * cosB) / rb + a * y3b * z1b * cotB / (rb * rb * rb)
+                          - (y2 * y2 * cosB * cosB - a * z1b * cotB / rb * q) / (rb * p7)));
+        u3 += b1 * f * (2 * (1 - nu) * (N1 * fib * cotB + y2 / p6 * (2 * nu + ar) - y2 * cosB / p7 * (cosB + ar))
+              + y2 * h / rb * (2 * nu / p6 + ar2) + y2 * h * cosB / (rb * p7) * (N1 - q / p7 * (cosB + ar) - a * y3b / (rb * rb)));
+    }
+    if (b2 != 0.0) {
+        u1 += b2 * f * (N1 * ((2 * (1 - nu) * cot2 + nu) * lg6 - (2 * (1 - nu) * cot2 + 1) * cosB * lg7)
+              + N1 / p6 * (-N1 * y1 * cotB + nu * y3b - a + a * y1 * cotB / rb + y1 * y1 / p6 * (nu + ar))
+              - N1 * cotB / p7 * (z1b * cosB - a * (rb * sinB - y1) / (rb * cosB)) - a * y1 * h * cotB / (rb * rb * rb)
+              + h / p6 * (2 * nu + (N1 * y1 * cotB + a) / rb - y1 * y1 / (rb * p6) * (2 * nu + ar) - a * y1 * y1 / (rb * rb * rb))
+              + h * cotB / p7 * (-cosB * sinB + a * y1 * y3b / (rb * rb * rb * cosB)
+                                 + (rb * sinB - y1) / rb * (2 * (1 - nu) * cosB - q / p7 * (1 + a / (rb * cosB)))));
+        u2 += b2 * f * (2 * (1 - nu) * N1 * fib * cot2 + N1 * y2 / p6 * (-(N1 - ar) * cotB + y1 / p6 * (nu + ar))
+              - N1 * y2 * cotB / p7 * (1 + a / (rb * cosB)) - a * y2 * h * cotB / (rb * rb * rb)
+              + y2 * h / (rb * p6) * (N1 * cotB - 2 * nu * y1 / p6 - a * y1 / rb * (1 / rb + 1 / p6))
+              + y2 * h * cotB / (rb * p7) * (-2 * (1 - nu) * cosB + q / p7 * (1 + a / (rb * cosB)) + a * y3b / (rb * rb * cosB)));
+        u3 += b2 * f * (-2 * (1 - nu) * N1 * cotB * (lg6 - cosB * lg7) - 2 * (1 - nu) * y1 / p6 * (2 * nu + ar)
+              + 2 * (1 - nu) * z1b / p7 * (cosB + ar) + h / rb * (N1 * cotB - 2 * nu * y1 / p6 - a * y1 / (rb * rb))
+              - h / p7 * (cosB * sinB + q * cotB / rb * (2 * (1 - nu) * cosB - q / p7)
+                          + ar * (sinB - y3b * z1b / (rb * rb) - z1b * q / (rb * p7))));
+    }
+    if (b3 != 0.0) {
+        u1 += b3 * f * (N1 * (y2 / p6 * (1 + ar) - y2 * cosB / p7 * (cosB + ar)) - y2 * h / rb * (ar2 + 1 / p6)
+              + y2 * h * cosB / (rb * p7) * (q / p7 * (cosB + ar) + a * y3b / (rb * rb)));
+        u2 += b3 * f * (N1 * (-sinB * lg7 - y1 / p6 * (1 + ar) + z1b / p7 * (cosB + ar)) + y1 * h / rb * (ar2 + 1 / p6)
+              - h / p7 * (sinB * (cosB - ar) + z1b / rb * (1 + a * y3b / (rb * rb))
+                          - (y2 * y2 * cosB * sinB - a * z1b / rb * q) / (rb * p7)));
+        u3 += b3 * f * (2 * (1 - nu) * fib + 2 * (1 - nu) * y2 * sinB / p7 * (cosB + ar)
+              + y2 * h * sinB / (rb * p7) * (1 + q / p7 * (cosB + ar) + a * y3b / (rb * rb)));
+    }
+    u[0] = u1; u[1] = u2; u[2] = u3;
+}
+
+/* free-surface correction strains v11 v22 v33 v12 v13 v23 of an angular
+ * dislocation (the quantity of AngDisStrainFSC, TDstressFS.f90:629-1011) */
+void og_angfsc(double y1, double y2, double y3, double beta, double b1, double b2, double b3, double nu, double a,
+               double *v)
+{
+    const double hstep = 1e-60;
+    double y3b = y3 + 2 * a;
+    double rb = sqrt(y1 * y1 + y2 * y2 + y3b * y3b);
+    double c = cos(beta / 2) / sin(beta / 2);
+    double D = y1 - (rb + y3b) * c, den = D * D + y2 * y2;
+    /* gradient of 2 atan(-y2 / (y1 - (r + y3bar) cot(beta/2))) */
+    double gf[3] = { 2 * y2 * (1 - c * y1 / rb) / den, -2 * (D + c * y2 * y2 / rb) / den,
+                     -2 * y2 * c * (y3b / rb + 1) / den };
+    double du[3][3];   /* du[m][k] = d u_m / d y_k */
+    for (int k = 0; k < 3; k++) {
+        og_c yy[3] = { y1, y2, y3 };
+        yy[k] += I * hstep;
+        og_c u[3];
+        og_fsc_disp(yy[0], yy[1], yy[2], I * hstep * gf[k], beta, b1, b2, b3, nu, a, u);
+        for (int m = 0; m < 3; m++) du[m][k] = cimag(u[m]) / hstep;
+    }
+    v[0] = du[0][0];
+    v[1] = du[1][1];
+    v[2] = du[2][2];
+    v[3] = (du[0][1] + du[1][0]) / 2;
+    v[4] = (du[0][2] + du[2][0]) / 2;
+    v[5] = (du[1][2] + du[2][1]) / 2;
+}
+
+/* AngSetupFSC_S, TDstressFS.f90:405-558: free-surface correction stress of
+ * the side PA -> PB (zero for a vertical side) */
+static void og_angsetup_fsc(double X, double Y, double Z, double bX, double bY, double bZ, const double *PA,
+                            const double *PB, double mu, double lambda, double *S)
+{
+    const double pi = OG_PI;
+    double nu = lambda / (lambda + mu) / 2;
+    double sv[3] = { PB[0] - PA[0], PB[1] - PA[1], PB[2] - PA[2] };
+    double beta = acos(-sv[2] / sqrt(og_dot3(sv, sv)));
+    for (int k = 0; k < 6; k++) S[k] = 0.0;
+    if (fabs(beta) < 1e-3 || fabs(pi - beta) < 1e-3) return;
+    double e1[3] = { sv[0], sv[1], 0.0 };
+    double nn = sqrt(og_dot3(e1, e1));
+    for (int k = 0; k < 3; k++) e1[k] = e1[k] / nn;
+    const double e3[3] = { 0.0, 0.0, -1.0 };
+    double e2[3];
+    og_cross(e3, e1, e2);
+    double A[3][3];   /* rows: the angular-dislocation axes */
+    for (int k = 0; k < 3; k++) { A[0][k] = e1[k]; A[1][k] = e2[k]; A[2][k] = e3[k]; }
+    double ya[3], yab[3], bb[3];
+    double rel[3] = { X - PA[0], Y - PA[1], Z - PA[2] }, bv[3] = { bX, bY, bZ };
+    for (int r = 0; r < 3; r++) {
+        ya[r] = A[r][0] * rel[0] + A[r][1] * rel[1] + A[r][2] * rel[2];
+        yab[r] = A[r][0] * sv[0] + A[r][1] * sv[1] + A[r][2] * sv[2];
+        bb[r] = A[r][0] * bv[0] + A[r][1] * bv[1] + A[r][2] * bv[2];
+    }
+    /* configuration I or II (the artefact-free one for the point) */
+    double ang = (beta * ya[0] >= 0.0) ? -pi + beta : beta;
+    double vA[6], vB[6], v[6], E[6];
+    og_angfsc(ya[0], ya[1], ya[2], ang, bb[0], bb[1], bb[2], nu, -PA[2], vA);
+    og_angfsc(ya[0] - yab[0], ya[1] - yab[1], ya[2] - yab[2], ang, bb[0], bb[1], bb[2], nu, -PB[2], vB);
+    for (int k = 0; k < 6; k++) v[k] = vB[k] - vA[k];
+    og_tenstrans(v, A, E);
+    double tr = E[0] + E[1] + E[2];
+    S[0] = 2 * mu * E[0] + lambda * tr;
+    S[1] = 2 * mu * E[1] + lambda * tr;
+    S[2] = 2 * mu * E[2] + lambda * tr;
+    S[3] = 2 * mu * E[3];
+    S[4] = 2 * mu * E[4];
+    S[5] = 2 * mu * E[5];
+}
+
+/* TDstress_HarFunc, TDstressFS.f90:236-293: the harmonic part, one term per side */
+void og_td_harfunc(double X, double Y, double Z, const double *P1, const double *P2, const double *P3, double Ss,
+                   double Ds, double Ts, double mu, double lambda, double *S)
+{
+    double a[3], b[3], vn[3], vs[3], vd[3];
+    for (int k = 0; k < 3; k++) { a[k] = P2[k] - P1[k]; b[k] = P3[k] - P1[k]; }
+    og_cross(a, b, vn);
+    double nn = sqrt(og_dot3(vn, vn));
+    for (int k = 0; k < 3; k++) vn[k] = vn[k] / nn;
+    const double eY[3] = { 0.0, 1.0, 0.0 }, eZ[3] = { 0.0, 0.0, 1.0 };
+    og_cross(eZ, vn, vs);
+    /* normalised before the zero test, as there: a horizontal triangle gives NaN */
+    nn = sqrt(og_dot3(vs, vs));
+    for (int k = 0; k < 3; k++) vs[k] = vs[k] / nn;
+    if (og_dot3(vs, vs) == 0.0)
+        for (int k = 0; k < 3; k++) vs[k] = eY[k] * vn[2];
+    nn = sqrt(og_dot3(vs, vs));
+    for (int k = 0; k < 3; k++) vs[k] = vs[k] / nn;
+    og_cross(vn, vs, vd);
+    /* slip (tensile, strike, dip) in the triangle frame -> global */
+    double bg[3];
+    for (int k = 0; k < 3; k++) bg[k] = vn[k] * Ts + vs[k] * Ss + vd[k] * Ds;
+    double s1[6], s2[6], s3[6];
+    og_angsetup_fsc(X, Y, Z, bg[0], bg[1], bg[2], P1, P2, mu, lambda, s1);
+    og_angsetup_fsc(X, Y, Z, bg[0], bg[1], bg[2], P2, P3, mu, lambda, s2);
+    og_angsetup_fsc(X, Y, Z, bg[0], bg[1], bg[2], P3, P1, mu, lambda, s3);
+    for (int k = 0; k < 6; k++) S[k] = s1[k] + s2[k] + s3[k];
+}
+
+/* TDstressHS, TDstressFS.f90:7-44: main dislocation + harmonic part + image */
+void og_tdstresshs(double X, double Y, double Z, const double *P1, const double *P2, const double *P3,
+                   double Ss, double Ds, double Ts, double mu, double lambda, double *S)
+{
+    double Sm[6], Sh[6], Si[6];
+    og_tdstressfs(X, Y, Z, P1, P2, P3, Ss, Ds, Ts, mu, lambda, Sm);
+    og_td_harfunc(X, Y, Z, P1, P2, P3, Ss, Ds, Ts, mu, lambda, Sh);
+    double Q1[3] = { P1[0], P1[1], -P1[2] }, Q2[3] = { P2[0], P2[1], -P2[2] }, Q3[3] = { P3[0], P3[1], -P3[2] };
+    og_tdstressfs(X, Y, Z, Q1, Q2, Q3, Ss, Ds, Ts, mu, lambda, Si);
+    for (int k = 0; k < 6; k++) S[k] = Sm[k] + Si[k] + Sh[k];
+}
+
+/* matel3dn_ij / matel3dh_ij, m_HACApK_calc_entry_ij.f90:802-898: the
+ * full-space kernel takes the receiver's rake for mode 's', the half-space
+ * kernel the source's */
 static double og_tri_ij(const og_geom *g, int i, int j)
 {
     double lambda = 2 * og_pois / (1 - 2 * og_pois) * og_rigid;
@@ -625,10 +817,16 @@ static double og_tri_ij(const og_geom *g, int i, int j)
     case 1: ss = 1.0; break;
     case 2: dd = 1.0; break;
     case 3: tt = 1.0; break;
-    default: ss = cos(g->rake[i]); dd = sin(g->rake[i]); break;
+    default: {
+        int r = g->kind == 5 ? j : i;
+        ss = cos(g->rake[r]); dd = sin(g->rake[r]); break;
+    }
     }
     double S[6];
-    og_tdstressfs(g->xcol[i], g->ycol[i], g->zcol[i], P1, P2, P3, ss, dd, tt, og_rigid, lambda, S);
+    if (g->kind == 5)
+        og_tdstresshs(g->xcol[i], g->ycol[i], g->zcol[i], P1, P2, P3, ss, dd, tt, og_rigid, lambda, S);
+    else
+        og_tdstressfs(g->xcol[i], g->ycol[i], g->zcol[i], P1, P2, P3, ss, dd, tt, og_rigid, lambda, S);
     double A[3][3] = { { g->ev11[i], g->ev21[i], g->ev31[i] },
                        { g->ev12[i], g->ev22[i], g->ev32[i] },
                        { g->ev13[i], g->ev23[i], g->ev33[i] } };
@@ -646,7 +844,8 @@ double og_entry(const og_geom *g, int i, int j)
         return og_antiplane(g->xcol[i], g->xel[j], g->xer[j]) - og_antiplane(g->xcol[i], -g->xel[j], -g->xer[j]);
     case 2: return og_okada_ij(g, i, j, 0);
     case 3: return og_okada_ij(g, i, j, 1);
-    case 4: return og_tri_ij(g, i, j);
+    case 4:
+    case 5: return og_tri_ij(g, i, j);
     }
     return NAN;
 }

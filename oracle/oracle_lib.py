@@ -85,6 +85,9 @@ def lib():
         L.og_mesh_2dp.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp]
         L.og_dc3d.argtypes = [C.c_double] * 11 + [C.c_double, C.c_int, _dp]
         L.og_tdstressfs.argtypes = [C.c_double] * 3 + [_dp] * 3 + [C.c_double] * 5 + [_dp]
+        L.og_tdstresshs.argtypes = [C.c_double] * 3 + [_dp] * 3 + [C.c_double] * 5 + [_dp]
+        L.og_td_harfunc.argtypes = [C.c_double] * 3 + [_dp] * 3 + [C.c_double] * 5 + [_dp]
+        L.og_angfsc.argtypes = [C.c_double] * 9 + [_dp]
         _lib = L
     return _lib
 
@@ -134,6 +137,23 @@ def tdstressfs(X, Y, Z, P1, P2, P3, ss, ds, ts, mu, lam):
     S = np.empty(6)
     lib().og_tdstressfs(X, Y, Z, _ptr(P1), _ptr(P2), _ptr(P3), ss, ds, ts, mu, lam, _ptr(S))
     return S
+
+
+def tdstresshs(X, Y, Z, P1, P2, P3, ss, ds, ts, mu, lam, part="total"):
+    """Half-space stress of a triangular dislocation (TDstressHS), or only
+    its harmonic part (part="harmonic", TDstress_HarFunc)."""
+    P1, P2, P3 = (np.ascontiguousarray(p, dtype=np.float64) for p in (P1, P2, P3))
+    S = np.empty(6)
+    f = lib().og_tdstresshs if part == "total" else lib().og_td_harfunc
+    f(X, Y, Z, _ptr(P1), _ptr(P2), _ptr(P3), ss, ds, ts, mu, lam, _ptr(S))
+    return S
+
+
+def angfsc(y1, y2, y3, beta, b1, b2, b3, nu, a) -> np.ndarray:
+    """Free-surface correction strains (v11 v22 v33 v12 v13 v23) of an angular dislocation."""
+    v = np.empty(6)
+    lib().og_angfsc(y1, y2, y3, beta, b1, b2, b3, nu, a, _ptr(v))
+    return v
 
 
 def tableau() -> np.ndarray:
@@ -277,3 +297,28 @@ def ref_entries(g, ii, jj) -> np.ndarray:
     _ref.hbiref_entries(int(g.kind), int(g.n), int(g.vcomp), int(g.md), float(g.w),
                         geo.ctypes.data_as(_dp), len(ii), _iptr(ii), _iptr(jj), _ptr(out))
     return out
+
+
+def _reflib():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REFLIB)
+        _ref.hbiref_entries.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int, _ip, _ip, _dp]
+    _ref.hbiref_angfsc.argtypes = [C.c_double] * 9 + [_dp]
+    _ref.hbiref_td_hs.argtypes = [C.c_double] * 3 + [_dp] * 3 + [C.c_double] * 5 + [_dp]
+    return _ref
+
+
+def ref_angfsc(y1, y2, y3, beta, b1, b2, b3, nu, a) -> np.ndarray:
+    """AngDisStrainFSC of the reference itself."""
+    v = np.empty(6)
+    _reflib().hbiref_angfsc(y1, y2, y3, beta, b1, b2, b3, nu, a, _ptr(v))
+    return v
+
+
+def ref_td_hs(X, Y, Z, P1, P2, P3, ss, ds, ts, mu, lam) -> np.ndarray:
+    """TDstressHS (first 6) and TDstress_HarFunc (last 6) of the reference itself."""
+    P1, P2, P3 = (np.ascontiguousarray(p, dtype=np.float64) for p in (P1, P2, P3))
+    o = np.empty(12)
+    _reflib().hbiref_td_hs(X, Y, Z, _ptr(P1), _ptr(P2), _ptr(P3), ss, ds, ts, mu, lam, _ptr(o))
+    return o

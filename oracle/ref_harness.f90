@@ -10,11 +10,12 @@
 ! geo(n,28) columns: xcol ycol zcol xel xer ang angd dsl dsd rake
 !                    xs1 xs2 xs3 ys1 ys2 ys3 zs1 zs2 zs3
 !                    ev11 ev12 ev13 ev21 ev22 ev23 ev31 ev32 ev33
-! kind: 1 '2dvs', 2 '3dph', 3 '3dnr', 4 '3dnt'; vcomp: 0 's', 1 'n';
+! kind: 1 '2dvs', 2 '3dph', 3 '3dnr', 4 '3dnt', 5 '3dht'; vcomp: 0 's', 1 'n';
 ! md: 0 's', 1 'st', 2 'dp', 3 'open'.  ii, jj are 0-based.
 module hbiref_harness
   use iso_c_binding
   use m_HACApK_calc_entry_ij
+  use dtriangular
   implicit none
 contains
   subroutine hbiref_entries(kind, n, vcomp, md, w, geo, np, ii, jj, out) bind(C, name="hbiref_entries")
@@ -35,6 +36,8 @@ contains
       bem%problem = '3dnr'
     case (4)
       bem%problem = '3dnt'
+    case (5)
+      bem%problem = '3dht'
     end select
     if (vcomp == 1) then
       bem%v = 'n'
@@ -75,4 +78,24 @@ contains
     deallocate(bem%xs1, bem%xs2, bem%xs3, bem%ys1, bem%ys2, bem%ys3, bem%zs1, bem%zs2, bem%zs3)
     deallocate(bem%ev11, bem%ev12, bem%ev13, bem%ev21, bem%ev22, bem%ev23, bem%ev31, bem%ev32, bem%ev33)
   end subroutine hbiref_entries
+
+  ! free-surface correction strains of one angular dislocation
+  ! (AngDisStrainFSC, TDstressFS.f90:629-1011): out = v11 v22 v33 v12 v13 v23
+  subroutine hbiref_angfsc(y1, y2, y3, beta, b1, b2, b3, nu, a, out) bind(C, name="hbiref_angfsc")
+    real(c_double), value :: y1, y2, y3, beta, b1, b2, b3, nu, a
+    real(c_double), intent(out) :: out(6)
+    call AngDisStrainFSC(y1, y2, y3, beta, b1, b2, b3, nu, a, out(1), out(2), out(3), out(4), out(5), out(6))
+  end subroutine hbiref_angfsc
+
+  ! half-space stress of a triangular dislocation and its parts
+  ! (TDstressHS, TDstressFS.f90:7-44; TDstress_HarFunc 236-293):
+  ! out(1:6) total, out(7:12) harmonic part
+  subroutine hbiref_td_hs(x, y, z, p1, p2, p3, ss, ds, ts, mu, lambda, out) bind(C, name="hbiref_td_hs")
+    real(c_double), value :: x, y, z, ss, ds, ts, mu, lambda
+    real(c_double), intent(in) :: p1(3), p2(3), p3(3)
+    real(c_double), intent(out) :: out(12)
+    call TDstressHS(x, y, z, p1, p2, p3, ss, ds, ts, mu, lambda, out(1), out(2), out(3), out(4), out(5), out(6))
+    call TDstress_HarFunc(x, y, z, p1, p2, p3, ss, ds, ts, mu, lambda, out(7), out(8), out(9), out(10), out(11), &
+      & out(12))
+  end subroutine hbiref_td_hs
 end module hbiref_harness

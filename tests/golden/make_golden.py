@@ -81,6 +81,43 @@ def main():
         g.md = md
         ii, jj = sample_pairs(g.n, rng, 500)
         save("kernel_entries_3dnt_md%s.npz" % name, g, ii, jj, O.ref_entries(g, ii, jj))
+    # 3dht: the rough surface below a free surface at z = 0 (top edge on it),
+    # rake 30 degrees (the source's rake sets the slip there), all modes
+    tri = P.rough_surface_triangles(6.0, 3.0, 0.25, hurst=0.8, alpha=1e-2, seed=12)
+    g = P.mesh_triangles(tri, crake=30.0, halfspace=True)
+    near = [(i, i) for i in range(0, g.n, 3)] + [(i, i + 1) for i in range(0, g.n - 1, 5)]
+    ii, jj = sample_pairs(g.n, rng, 2000, near)
+    save("kernel_entries_3dht.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    g.vcomp = 1
+    ii, jj = sample_pairs(g.n, rng, 800, near[:100])
+    save("kernel_entries_3dht_n.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    for md, name in ((1, "st"), (2, "dp"), (3, "open")):
+        g.vcomp = 0
+        g.md = md
+        ii, jj = sample_pairs(g.n, rng, 400)
+        save("kernel_entries_3dht_md%s.npz" % name, g, ii, jj, O.ref_entries(g, ii, jj))
+    # a planar thrust dipping 30 degrees from the surface: shallow sides in
+    # both artefact-free configurations, receivers close to the free surface
+    g = P.mesh_triangles(P.plane_triangles(8.0, 4.0, 0.5, dip=30.0), crake=90.0, halfspace=True)
+    ii, jj = sample_pairs(g.n, rng, 1500, [(i, i) for i in range(g.n)])
+    save("kernel_entries_3dht_dip30.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    g.vcomp = 1
+    ii, jj = sample_pairs(g.n, rng, 800)
+    save("kernel_entries_3dht_dip30_n.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    # free-surface correction strains of single angular dislocations
+    # (AngDisStrainFSC): points below the surface, both signs of beta, each
+    # Burgers component, depths of the vertex a
+    m = 400
+    args = np.empty((m, 9))
+    args[:, 0:2] = rng.uniform(-3, 3, (m, 2))
+    args[:, 2] = -rng.uniform(0.0, 3.0, m)
+    args[:, 3] = rng.uniform(0.05, np.pi - 0.05, m) * rng.choice([-1.0, 1.0], m)
+    args[:, 4:7] = np.eye(3)[rng.integers(0, 3, m)]
+    args[:, 7] = 0.25
+    args[:, 8] = rng.uniform(0.05, 3.0, m)
+    vals = np.array([O.ref_angfsc(*a) for a in args])
+    np.savez_compressed(os.path.join(HERE, "angfsc_points.npz"), args=args, vals=vals)
+    print("angfsc_points.npz", m)
 
 
 if __name__ == "__main__":

@@ -254,3 +254,23 @@ def test_gpu_driver_normal_stress_and_opening(tmp_path):
     assert np.abs(slipn[-1][low]).min() > 0 and np.abs(slipn[-1][~low]).max() == 0
     vn, sn = rk.get_opening()
     assert np.allclose(veln[-1], vn, rtol=1e-6, atol=1e-30) and np.allclose(slipn[-1], sn, rtol=1e-6, atol=1e-30)
+
+
+def test_gpu_driver_forward_halfspace_triangles(tmp_path):
+    """problem 3dht from an STL file (main_LH.f90:353-368, 479-510, 852-898):
+    H*1 of the forward mode against the library's product on the same mesh
+    (triangles are not divided by ds0, main_LH.f90:2645)."""
+    P.write_stl(str(tmp_path / "thrust.stl"), P.plane_triangles(8.0, 4.0, 0.5, dip=30.0))
+    _run(tmp_path, {"problem": "3dht", "geometry_file": "thrust.stl", "forward": True, "eps_h": 1e-6, "leaf": 32,
+                    "filenumber": 8, "crake": 90.0})
+    out = np.loadtxt(tmp_path / "output" / "stress8")
+    g = P.mesh_triangles(P.read_stl(str(tmp_path / "thrust.stl")), crake=90.0, halfspace=True)
+    Hg = H.HMatrix(g, leaf=32, eps=1e-6)
+    y = Hg.matvec(np.ones(g.n), H.PREC_F64)
+    q = lambda v: 5.01e-4 * np.abs(v) + 1e-12
+    assert out.shape == (g.n, 6)
+    assert (np.abs(out[:, 4] - y) <= q(y)).all()
+    # the free surface matters: the full-space kernel gives another H*1
+    yf = H.HMatrix(P.mesh_triangles(P.read_stl(str(tmp_path / "thrust.stl")), crake=90.0), leaf=32,
+                   eps=1e-6).matvec(np.ones(g.n), H.PREC_F64)
+    assert np.abs(yf - y).max() > 1e-2 * np.abs(y).max()
