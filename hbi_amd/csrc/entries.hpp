@@ -9,6 +9,11 @@
 //   3dht   the same in a half-space             m_HACApK_calc_entry_ij.f90:852-898, TDstressFS.f90:7-44,
 //          (main + image + free-surface correction; the correction's strains
 //          are forward-mode derivatives of its displacement field, see fsc_strain)
+//   2dp    in-plane segment, full space         m_HACApK_calc_entry_ij.f90:648-661
+//   3dp    planar quadrilateral, full space     m_HACApK_calc_entry_ij.f90:412-421, 440-470
+//   2dph   in-plane dipping segment, half-space m_HACApK_calc_entry_ij.f90:92-116, 143-166 (also '2dnh')
+//   2dn    in-plane nonplanar, full space       m_HACApK_calc_entry_ij.f90:300-339, 697-750
+//   25d    2.5-D rectangles                     m_HACApK_calc_entry_ij.f90:934-1064
 // The closed-form expressions are those of the published solutions written
 // term by term in the reference's evaluation order; FP contraction is off in
 // every function here so that the device result tracks the FP64 CPU result
@@ -31,6 +36,7 @@ struct DevGeom {
     const double *xcol, *ycol, *zcol, *xel, *xer, *ang, *angd, *dsl, *dsd, *rake;
     const double *xs1, *xs2, *xs3, *ys1, *ys2, *ys3, *zs1, *zs2, *zs3;
     const double *ev11, *ev12, *ev13, *ev21, *ev22, *ev23, *ev31, *ev32, *ev33;
+    const double *yel, *yer, *xs4, *zs4, *ds;
 };
 
 __device__ __forceinline__ double dpi() { return 4.0 * atan(1.0); }
@@ -721,6 +727,170 @@ __device__ __forceinline__ double antiplane(double xc, double xl, double xr)
     return factor * (1.0 / (xc - xr) - 1.0 / (xc - xl));
 }
 
+// ------------------------------------------------------------ 2-D, 2.5-D ----
+// planar in-plane shear crack element, full space (matels2dp_ij)
+__device__ __forceinline__ double inplane(double xc, double xl, double xr)
+{
+#pragma clang fp contract(off)
+    const double factor = kRigid / (2.0 * dpi() * (1.0 - kPois));
+    return factor * (1.0 / (xc - xr) - 1.0 / (xc - xl));
+}
+
+// planar rectangle in a full space seen in its own plane (ret3dp): shear for
+// slip modes 's'/'o', normal for 'open'; the other combinations are zero
+__device__ __noinline__ double quad_entry(const DevGeom &g, int i, int j)
+{
+#pragma clang fp contract(off)
+    const double xc = g.xcol[i], zc = g.zcol[i];
+    const double dx1 = xc - g.xs1[j], dx2 = xc - g.xs2[j], dx3 = xc - g.xs3[j], dx4 = xc - g.xs4[j];
+    const double dz1 = zc - g.zs1[j], dz2 = zc - g.zs2[j], dz3 = zc - g.zs3[j], dz4 = zc - g.zs4[j];
+    const bool shear = g.md == 0 && g.vcomp == 0, open = g.md == 3 && g.vcomp == 1;
+    if (!shear && !open) return 0.0;
+    const double factor = kRigid / (4.0 * dpi());
+    const double r1 = sqrt(dx1 * dx1 + dz1 * dz1), r2 = sqrt(dx2 * dx2 + dz2 * dz2);
+    const double r3 = sqrt(dx3 * dx3 + dz3 * dz3), r4 = sqrt(dx4 * dx4 + dz4 * dz4);
+    const double alpha = 1 / (2 - 2 * kPois);
+    // corner sum of r / (dx dz): the part both modes share
+    const double corners = 2.0 * (1 - alpha) * (r4 / (dx4 * dz4) + r2 / (dx2 * dz2) - r3 / (dx3 * dz3) - r1 / (dx1 * dz1));
+    if (open) return factor * corners;
+    const double e1 = -(1 - 2 * alpha) / dx1 * (r4 / dz4 + dz4 / r4 - r1 / dz1 - dz1 / r1);
+    const double e2 = -(1 - 2 * alpha) / dx2 * (r2 / dz2 + dz2 / r2 - r3 / dz3 - dz3 / r3);
+    return factor * (corners + e1 + e2);
+}
+
+// stress (sxx, syy, sxy) at (X, Y) of an edge dislocation at (xs, ys) with
+// Burgers vector (s1, s2) in a half-space y > 0 with a free surface y = 0
+// (D2dip2, m_HACApK_calc_entry_ij.f90:143-166)
+__device__ __noinline__ void edge_dislocation_hs(double X, double Y, double xs, double ys, double s1, double s2,
+                                                 double *o)
+{
+#pragma clang fp contract(off)
+    const double dx = X - xs, ym = Y - ys, yp = Y + ys;
+    const double dx2 = dx * dx, ym2 = ym * ym, yp2 = yp * yp;
+    const double r1s = dx2 + ym2, r2s = dx2 + yp2;        // r1^2, r2^2
+    const double r14 = r1s * r1s, r24 = r2s * r2s, r26 = r24 * r2s;
+    const double coef = kRigid / 2.0 / dpi() / (1.0 - kPois);
+    const double q = 6 * Y * yp * dx2;
+    const double sxx1 = s1 * coef * (ym * (ym2 + 3 * dx2) / r14 - yp * (yp2 + 3 * dx2) / r24 +
+                                     2 * ys / r26 * (q - ym * (yp2 * yp) - dx2 * dx2));
+    const double sxx2 = s2 * coef * (dx * (ym2 - dx2) / r14 - dx * (yp2 - dx2) / r24 +
+                                     4 * ys * dx / r26 * ((2 * ys - Y) * yp2 + (3 * Y + 2 * ys) * dx2));
+    const double syy1 = s1 * coef * (ym * (ym2 - dx2) / r14 - yp * (yp2 - dx2) / r24 -
+                                     2 * ys / r26 * (q - (3 * Y + ys) * (yp2 * yp) + dx2 * dx2));
+    const double syy2 = -s2 * coef * (dx * (3 * ym2 + dx2) / r14 - dx * (3 * yp2 + dx2) / r24 -
+                                      4 * ys * dx * Y / r26 * (3 * yp2 - dx2));
+    const double sxy1 = s1 * coef * (dx * (ym2 - dx2) / r14 - dx * (yp2 - dx2) / r24 +
+                                     4 * ys * dx * Y / r26 * (3 * yp2 - dx2));
+    const double sxy2 = s2 * coef * (ym * (ym2 - dx2) / r14 - yp * (yp2 - dx2) / r24 +
+                                     2 * ys / r26 * (q - dx2 * dx2 + (ys - Y) * (yp2 * yp)));
+    o[0] = sxx1 + sxx2;
+    o[1] = syy1 + syy2;
+    o[2] = sxy1 + sxy2;
+}
+
+// dipping segment in a half-space: the two end dislocations (matel2dph2_ij)
+__device__ __noinline__ double dip2d_entry(const DevGeom &g, int i, int j)
+{
+#pragma clang fp contract(off)
+    const double s1 = cos(g.ang[j]), s2 = sin(g.ang[j]);
+    double p[3], q[3];
+    edge_dislocation_hs(g.xcol[i], g.ycol[i], g.xel[j], g.yel[j], s1, s2, p);
+    edge_dislocation_hs(g.xcol[i], g.ycol[i], g.xer[j], g.yer[j], s1, s2, q);
+    const double sxx = p[0] - q[0], syy = p[1] - q[1], sxy = p[2] - q[2];
+    const double c2 = cos(2 * g.ang[i]), s2i = sin(2 * g.ang[i]);
+    if (g.vcomp == 1) return -0.5 * (sxx + syy) + 0.5 * (sxx - syy) * c2 + sxy * s2i;
+    return -0.5 * (sxx - syy) * s2i + sxy * c2;
+}
+
+// nonplanar in-plane segment, full space (tensor2d_ij): the segment's kernel
+// in its own frame from the end-point integrals (inte11/12/22 s|o), rotated
+// to global axes and projected on the receiver
+__device__ __noinline__ double seg2d_entry(const DevGeom &g, int i, int j)
+{
+#pragma clang fp contract(off)
+    const double pi = dpi();
+    const double cj = cos(g.ang[j]), sj = sin(g.ang[j]);
+    const double xp = cj * (g.xcol[i] - g.xel[j]) + sj * (g.ycol[i] - g.yel[j]);
+    const double xm = cj * (g.xcol[i] - g.xer[j]) + sj * (g.ycol[i] - g.yer[j]);
+    const double yp = -sj * (g.xcol[i] - g.xel[j]) + cj * (g.ycol[i] - g.yel[j]);
+    const double ym = -sj * (g.xcol[i] - g.xer[j]) + cj * (g.ycol[i] - g.yer[j]);
+    const double pa = kVs / 6.0;                        // vs / vp, m_const.f90:7-8
+    const double c = 2 * kVs / pi * (1 - pa * pa);
+    const bool open = g.md == 3;
+    // the three integrals at one end point (x1, x2), shear or opening
+    auto ends = [&](double x1, double x2, double *t) {
+        const double r = sqrt(x1 * x1 + x2 * x2);
+        const double r4 = (r * r) * (r * r);
+        if (!open) {
+            t[0] = -c * x2 * (3 * x1 * x1 + x2 * x2) / r4;
+            t[1] = c * x1 * (x1 * x1 - x2 * x2) / r4;
+            t[2] = c * x2 * (x1 * x1 - x2 * x2) / r4;
+        } else {
+            t[0] = c * x1 * (x1 * x1 - x2 * x2) / r4;
+            t[1] = c * x2 * (x1 * x1 - x2 * x2) / r4;
+            t[2] = c * x1 * (3 * x1 * x1 + x2 * x2) / r4;
+        }
+    };
+    double tp[3], tm[3];
+    ends(xp, yp, tp);
+    ends(xm, ym, tm);
+    const double k11 = -0.5 * kRigid / kVs * (tp[0] - tm[0]);
+    const double k12 = -0.5 * kRigid / kVs * (tp[1] - tm[1]);
+    const double k22 = -0.5 * kRigid / kVs * (tp[2] - tm[2]);
+    const double sin2 = sin(-2 * g.ang[j]), cos2 = cos(-2 * g.ang[j]);
+    const double sxx = 0.5 * (k11 + k22) + 0.5 * (k11 - k22) * cos2 + k12 * sin2;
+    const double sxy = -0.5 * (k11 - k22) * sin2 + k12 * cos2;
+    const double syy = 0.5 * (k11 + k22) - 0.5 * (k11 - k22) * cos2 - k12 * sin2;
+    if (g.vcomp == 1)
+        return -(0.5 * (sxx + syy) - 0.5 * (sxx - syy) * cos(2 * g.ang[i]) - sxy * sin(2 * g.ang[i]));
+    return 0.5 * (sxx - syy) * sin(-2 * g.ang[i]) + sxy * cos(-2 * g.ang[i]);
+}
+
+// 2.5-D: a rectangle of length ds along strike and width w, slip along
+// strike, stresses in its plane (matelrec_ij, dc3d25, ua25)
+__device__ __noinline__ double rect25_entry(const DevGeom &g, int i, int j)
+{
+#pragma clang fp contract(off)
+    const double pi = dpi();
+    const double alpha = (1.0 + (0.5 / kPois - 1.0)) / (1.0 + 2.0 * (0.5 / kPois - 1.0));
+    const double st = g.ang[j];
+    const double x = cos(st) * (g.xcol[i] - g.xcol[j]) + sin(st) * (g.ycol[i] - g.ycol[j]);
+    const double q = -sin(st) * (g.xcol[i] - g.xcol[j]) + cos(st) * (g.ycol[i] - g.ycol[j]);
+    const double dl = g.ds[j], dw = g.w;
+    const double xi[2] = {x + 0.5 * dl, x - 0.5 * dl};
+    const double et[2] = {0.5 * dw, -0.5 * dw};
+    // displacement gradients ux,x uy,x ux,y uy,y uz,z summed over the corners
+    double uxx = 0.0, uyx = 0.0, uxy = 0.0, uyy = 0.0, uzz = 0.0;
+    for (int k = 0; k < 2; k++)
+        for (int jj = 0; jj < 2; jj++) {
+            const double X = xi[jj], E = et[k];
+            const double r = sqrt(X * X + E * E + q * q);
+            const double y11 = 1.0 / (r * (r + E));
+            const double x11 = 1.0 / (r * (r + X));
+            const double y32 = (r + (r + E)) * y11 * y11 / r;
+            const double r3 = r * r * r;
+            const double d4 = (-(1.0 - alpha) / 2.0 * q * y11 - alpha / 2 * (X * X) * q * y32) / (pi * 2);
+            const double d6 = ((1.0 - alpha) / 2.0 * X * y11 + alpha / 2 * X * (q * q) * y32) / (pi * 2);
+            const double d7 = ((1.0 - alpha) / 2.0 * X * y11 + alpha / 2 * X * (E / r3 + (X * X) * y32) + E / 2.0 * x11) /
+                              (pi * 2);
+            const double d9 = ((1.0 - alpha) / 2.0 * (q * y11) - alpha / 2 * q * (E / r3 + (X * X) * y32)) / (pi * 2);
+            const double d11 = (alpha / 2 * (E * q / r3)) / (pi * 2);
+            const double sg = (jj + k == 1) ? -1.0 : 1.0;
+            uxx = uxx + sg * -d4;
+            uyx = uyx + sg * d6;
+            uxy = uxy + sg * -d7;
+            uyy = uyy + sg * d9;
+            uzz = uzz + sg * d11;
+        }
+    const double exx = -uxx, eyy = -uyy, ezz = -uzz, exy = -0.5 * (uxy + uyx);
+    const double sxx = kRigid * (exx + eyy + ezz) + 2 * kRigid * exx;
+    const double syy = kRigid * (exx + eyy + ezz) + 2 * kRigid * eyy;
+    const double sxy = 2 * kRigid * exy;
+    const double d = 2 * (g.ang[i] - g.ang[j]);
+    if (g.vcomp == 1) return -0.5 * (sxx + syy) + 0.5 * (sxx - syy) * cos(d) + sxy * sin(d);
+    return -0.5 * (sxx - syy) * sin(d) + sxy * cos(d);
+}
+
 // HACApK_entry_ij: receiver i, source j (mesh order, 0-based)
 __device__ __forceinline__ double entry(const DevGeom &g, int i, int j)
 {
@@ -730,6 +900,11 @@ __device__ __forceinline__ double entry(const DevGeom &g, int i, int j)
     case 3: return okada_entry(g, i, j, true);
     case 4:
     case 5: return tri_entry(g, i, j);
+    case 6: return inplane(g.xcol[i], g.xel[j], g.xer[j]);
+    case 7: return quad_entry(g, i, j);
+    case 8: return dip2d_entry(g, i, j);
+    case 9: return seg2d_entry(g, i, j);
+    case 10: return rect25_entry(g, i, j);
     }
     return __builtin_nan("");
 }

@@ -971,17 +971,21 @@ void hmat_apply(hbi_hmat *h, int prec, const double *x, double *y)
 // ------------------------------------------------------------- geometry --
 static void upload_geom(hbi_hmat *h, const hbi_geom *g)
 {
-    const double *src[28] = {g->xcol, g->ycol, g->zcol, g->xel, g->xer, g->ang, g->angd, g->dsl, g->dsd, g->rake,
-                             g->xs1, g->xs2, g->xs3, g->ys1, g->ys2, g->ys3, g->zs1, g->zs2, g->zs3,
-                             g->ev11, g->ev12, g->ev13, g->ev21, g->ev22, g->ev23, g->ev31, g->ev32, g->ev33};
-    const double **dst[28] = {&h->dgeo.xcol, &h->dgeo.ycol, &h->dgeo.zcol, &h->dgeo.xel, &h->dgeo.xer, &h->dgeo.ang,
-                              &h->dgeo.angd, &h->dgeo.dsl, &h->dgeo.dsd, &h->dgeo.rake, &h->dgeo.xs1, &h->dgeo.xs2,
-                              &h->dgeo.xs3, &h->dgeo.ys1, &h->dgeo.ys2, &h->dgeo.ys3, &h->dgeo.zs1, &h->dgeo.zs2,
-                              &h->dgeo.zs3, &h->dgeo.ev11, &h->dgeo.ev12, &h->dgeo.ev13, &h->dgeo.ev21,
-                              &h->dgeo.ev22, &h->dgeo.ev23, &h->dgeo.ev31, &h->dgeo.ev32, &h->dgeo.ev33};
+    constexpr int kArrays = 33;
+    const double *src[kArrays] = {g->xcol, g->ycol, g->zcol, g->xel, g->xer, g->ang, g->angd, g->dsl, g->dsd, g->rake,
+                                  g->xs1, g->xs2, g->xs3, g->ys1, g->ys2, g->ys3, g->zs1, g->zs2, g->zs3,
+                                  g->ev11, g->ev12, g->ev13, g->ev21, g->ev22, g->ev23, g->ev31, g->ev32, g->ev33,
+                                  g->yel, g->yer, g->xs4, g->zs4, g->ds};
+    const double **dst[kArrays] = {&h->dgeo.xcol, &h->dgeo.ycol, &h->dgeo.zcol, &h->dgeo.xel, &h->dgeo.xer,
+                                   &h->dgeo.ang, &h->dgeo.angd, &h->dgeo.dsl, &h->dgeo.dsd, &h->dgeo.rake,
+                                   &h->dgeo.xs1, &h->dgeo.xs2, &h->dgeo.xs3, &h->dgeo.ys1, &h->dgeo.ys2,
+                                   &h->dgeo.ys3, &h->dgeo.zs1, &h->dgeo.zs2, &h->dgeo.zs3, &h->dgeo.ev11,
+                                   &h->dgeo.ev12, &h->dgeo.ev13, &h->dgeo.ev21, &h->dgeo.ev22, &h->dgeo.ev23,
+                                   &h->dgeo.ev31, &h->dgeo.ev32, &h->dgeo.ev33, &h->dgeo.yel, &h->dgeo.yer,
+                                   &h->dgeo.xs4, &h->dgeo.zs4, &h->dgeo.ds};
     h->geo.clear();
-    h->geo.reserve(28);
-    for (int k = 0; k < 28; k++) {
+    h->geo.reserve(kArrays);
+    for (int k = 0; k < kArrays; k++) {
         *dst[k] = nullptr;
         if (!src[k]) continue;
         h->geo.emplace_back(g->n);
@@ -998,9 +1002,17 @@ static void upload_geom(hbi_hmat *h, const hbi_geom *g)
 static void check_geom(const hbi_geom *g)
 {
     HBI_REQUIRE(g && g->n > 0, "geometry with n > 0 required");
-    HBI_REQUIRE(g->kind >= 1 && g->kind <= 5, "unknown geometry kind");
+    HBI_REQUIRE(g->kind >= 1 && g->kind <= 10, "unknown geometry kind");
     HBI_REQUIRE(g->xcol, "xcol required");
-    if (g->kind == HBI_KIND_2DVS) HBI_REQUIRE(g->xel && g->xer, "2dvs needs xel, xer");
+    if (g->kind == HBI_KIND_2DVS || g->kind == HBI_KIND_2DP) HBI_REQUIRE(g->xel && g->xer, "2dvs / 2dp need xel, xer");
+    if (g->kind == HBI_KIND_3DP)
+        HBI_REQUIRE(g->zcol && g->xs1 && g->xs2 && g->xs3 && g->xs4 && g->zs1 && g->zs2 && g->zs3 && g->zs4,
+                    "3dp needs zcol and the four corners xs1..xs4, zs1..zs4");
+    if (g->kind == HBI_KIND_2DPH || g->kind == HBI_KIND_2DN)
+        HBI_REQUIRE(g->ycol && g->xel && g->xer && g->yel && g->yer && g->ang,
+                    "2dph / 2dnh / 2dn need ycol, segment ends xel, xer, yel, yer and ang");
+    if (g->kind == HBI_KIND_25D)
+        HBI_REQUIRE(g->ycol && g->ang && g->ds && g->w > 0, "25d needs ycol, ang, ds and w > 0");
     if (g->kind == HBI_KIND_OKADA_HS || g->kind == HBI_KIND_OKADA_FS)
         HBI_REQUIRE(g->ycol && g->zcol && g->ang && g->angd && g->dsl && g->dsd && g->rake && g->w > 0,
                     "okada kernels need ycol, zcol, ang, angd, dsl, dsd, rake and w > 0");

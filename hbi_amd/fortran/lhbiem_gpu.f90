@@ -2,9 +2,10 @@
 !
 ! Reads the reference's input formats (the `key value` .in file of
 ! main_LH.f90:182-328 with its defaults 133-180, the column-named parameter
-! file 530-581, ASCII STL facets 485-510), generates the meshes of the
-! reference (2dp/2dvs 1701-1718, 3dph 1781-1835, STL frames 1940-1990 for 3dnt
-! and 3dht), builds
+! file 530-581, ASCII STL facets 485-510, the 2-D segment and 3dnr/3dhr
+! rectangle geometry files 443-471), generates the meshes of the reference
+! (2dp/2dvs 1701-1718, 2dph 1720-1737, 2dn/2dnh/25d 1739-1750, 3dp 1752-1779,
+! 3dph 1781-1835, STL frames 1940-1990 for 3dnt and 3dht), builds
 ! the H-matrix through the HACApK call sites (m_HACApK_gpu) and advances the
 ! quasi-dynamic earthquake cycle with the device RK45 (hbi_rk_step), writing
 ! the reference's output files: monitor<n>.dat (main_LH.f90:1489-1493
@@ -50,7 +51,8 @@ program lhbiem_gpu
   type(st_HACApK_latticevec) :: st_vel, st_sum
   real(8), allocatable :: coord(:, :), wws(:), values(:, :), y(:), slip(:), vel(:), hist(:), vhist(:)
   real(8), allocatable, target :: xcol(:), ycol(:), zcol(:), xel(:), xer(:), ang(:), angd(:), dsl(:), dsd(:), rake(:)
-  real(8), allocatable, target :: xs(:, :), ev(:, :)
+  real(8), allocatable, target :: xs(:, :), ev(:, :), yel(:), yer(:), xq(:, :), dseg(:)
+  real(8) :: fwid, rows(7)
   real(8), allocatable :: ds(:), ag(:), bg(:), dcg(:), f0g(:), taug(:), sigmag(:), velg(:), vplg(:), taudotg(:), sigdotg(:)
   real(8), allocatable :: a(:), b(:), dc(:), f0(:), vplv(:), taudot(:), sigdot(:), area(:)
   type(hbi_rk_opts) :: ro
@@ -75,6 +77,7 @@ program lhbiem_gpu
   geofile = 'default'; parameter_file = 'default'; ncol = 0; nstep = 1000; leaf = 15
   precision = 'f64'; bf16margin = 1d0; outbatch = 200
   interval = 0; velth = 1e-2; dtout = 365 * 24 * 3600; dtout_co = 1000.0; restart = .false.; opening = .false.
+  fwid = 1e8
   a0 = 0.01d0; b0 = 0.015d0; dc0 = 0.01d0; mu0 = 0.6d0; vpl = 1d-9; velinit = 1d-9; tauinit = 0d0; sigmainit = 50d0
   imax = 0; jmax = 0; ncellg = 0; ds0 = 0d0; problem = ' '
   do
@@ -128,6 +131,7 @@ program lhbiem_gpu
     case ('dtout_co'); read(pvalue, *) dtout_co
     case ('restart'); read(pvalue, *) restart
     case ('opening'); read(pvalue, *) opening
+    case ('fwid'); read(pvalue, *) fwid
     case default
       write(*, *) 'WARNING: ', trim(param), ' is an unknown parameter'
     end select
@@ -139,7 +143,7 @@ program lhbiem_gpu
 
   ! ---------------------------------------------------------------- mesh --
   select case (trim(problem))
-  case ('3dph')
+  case ('3dph', '3dp')
     ncellg = imax * jmax
   case ('3dnt', '3dht')
     ncellg = count_stl_facets(geofile)
@@ -159,6 +163,67 @@ program lhbiem_gpu
       xer(i) = i * ds0
       xcol(i) = 0.5d0 * (xel(i) + xer(i))
     end do
+  case ('2dph', '2dn', '2dnh', '25d')
+    ! dipping from the surface (coordinate2dph, main_LH.f90:1720-1737) or
+    ! segments from the geometry file (main_LH.f90:443-452, coordinate2dn 1739-1750)
+    allocate(xel(ncellg), xer(ncellg), yel(ncellg), yer(ncellg), ang(ncellg), dseg(ncellg))
+    if (trim(problem) == '2dph') then
+      do i = 1, ncellg
+        xel(i) = (i - 1) * ds0 * cos(dipangle * pi / 180)
+        xer(i) = i * ds0 * cos(dipangle * pi / 180)
+        yel(i) = (i - 1) * ds0 * sin(dipangle * pi / 180)
+        yer(i) = i * ds0 * sin(dipangle * pi / 180)
+      end do
+    else
+      open(20, file=geofile, status='old', iostat=ios)
+      if (ios /= 0) then
+        write(*, *) 'ERROR: Failed to open geometry file'
+        stop 1
+      end if
+      do i = 1, ncellg
+        read(20, *) xel(i), xer(i), yel(i), yer(i)
+      end do
+      close(20)
+    end if
+    do i = 1, ncellg
+      ds(i) = sqrt((xer(i) - xel(i))**2 + (yer(i) - yel(i))**2)
+      if (trim(problem) == '2dph') ds(i) = ds0
+      ang(i) = datan2(yer(i) - yel(i), xer(i) - xel(i))
+      xcol(i) = 0.5d0 * (xel(i) + xer(i))
+      ycol(i) = 0.5d0 * (yel(i) + yer(i))
+    end do
+    dseg = ds
+  case ('3dp')
+    ! coordinate3dp, main_LH.f90:1752-1779: squares in the x-z plane
+    allocate(xq(ncellg, 8))
+    do i = 1, imax
+      do k = 1, jmax
+        i_ = (i - 1) * jmax + k
+        xcol(i_) = (i - imax / 2 - 0.5d0) * ds0
+        zcol(i_) = -(k - jmax / 2 - 0.5d0) * ds0
+        xq(i_, 1) = xcol(i_) + 0.5d0 * ds0; xq(i_, 2) = xcol(i_) - 0.5d0 * ds0
+        xq(i_, 3) = xcol(i_) - 0.5d0 * ds0; xq(i_, 4) = xcol(i_) + 0.5d0 * ds0
+        xq(i_, 5) = zcol(i_) + 0.5d0 * ds0; xq(i_, 6) = zcol(i_) + 0.5d0 * ds0
+        xq(i_, 7) = zcol(i_) - 0.5d0 * ds0; xq(i_, 8) = zcol(i_) - 0.5d0 * ds0
+      end do
+    end do
+    ds = ds0 * ds0
+  case ('3dnr', '3dhr')
+    ! rectangles from the geometry file, main_LH.f90:460-471
+    allocate(ang(ncellg), angd(ncellg), dsl(ncellg), dsd(ncellg))
+    open(20, file=geofile, status='old', iostat=ios)
+    if (ios /= 0) then
+      write(*, *) 'ERROR: Failed to open geometry file'
+      stop 1
+    end if
+    do i = 1, ncellg
+      read(20, *) rows
+      xcol(i) = rows(1); ycol(i) = rows(2); zcol(i) = rows(3); ang(i) = rows(4); angd(i) = rows(5)
+      dsl(i) = rows(6); dsd(i) = rows(7)
+      ds(i) = dsl(i) * dsd(i)
+    end do
+    close(20)
+    ds0 = minval(dsl)
   case ('3dph')
     allocate(ang(ncellg), angd(ncellg), dsl(ncellg), dsd(ncellg))
     call mesh_3ddip(imax, jmax, ds0, dipangle, xcol, ycol, zcol, ang, angd)
@@ -225,8 +290,17 @@ program lhbiem_gpu
   st_bemv%xcol => xcol; st_bemv%ycol => ycol; st_bemv%zcol => zcol; st_bemv%rake => rake
   select case (trim(problem))
   case ('2dp', '2dvs')
-    st_bemv%problem = '2dvs'
     st_bemv%xel => xel; st_bemv%xer => xer
+  case ('2dph', '2dn', '2dnh', '25d')
+    st_bemv%xel => xel; st_bemv%xer => xer; st_bemv%yel => yel; st_bemv%yer => yer
+    st_bemv%ang => ang; st_bemv%ds => dseg
+    st_bemv%w = fwid
+  case ('3dp')
+    st_bemv%xs1 => xq(:, 1); st_bemv%xs2 => xq(:, 2); st_bemv%xs3 => xq(:, 3); st_bemv%xs4 => xq(:, 4)
+    st_bemv%zs1 => xq(:, 5); st_bemv%zs2 => xq(:, 6); st_bemv%zs3 => xq(:, 7); st_bemv%zs4 => xq(:, 8)
+  case ('3dnr', '3dhr')
+    st_bemv%ang => ang; st_bemv%angd => angd; st_bemv%dsl => dsl; st_bemv%dsd => dsd
+    st_bemv%w = ds0
   case ('3dph')
     st_bemv%ang => ang; st_bemv%angd => angd; st_bemv%dsl => dsl; st_bemv%dsd => dsd
     st_bemv%w = ds0
@@ -272,7 +346,10 @@ program lhbiem_gpu
     ! H*1 with the okada scaling, main_LH.f90:2641-2664
     st_vel%vs = 1d0
     call HACApK_adot_lattice_hyp(st_sum, st_LHp_s, st_ctl, wws, st_vel)
-    if (trim(problem) == '3dph') st_sum%vs = st_sum%vs / ds0
+    select case (trim(problem))
+    case ('3dph', '3dnr', '3dhr')
+      st_sum%vs = st_sum%vs / ds0
+    end select
     write(fname, '("output/stress",i0)') number
     open(29, file=fname)
     do i = 1, ncellg
@@ -304,7 +381,10 @@ program lhbiem_gpu
   ro%prec = int(st_ctl%param(P_PREC))
   ro%vref = vref; ro%minsig = minsig; ro%maxsig = maxsig; ro%dtmax = dtmax
   ro%hdiv = 0d0
-  if (trim(problem) == '3dph') ro%hdiv = ds0
+  select case (trim(problem))
+  case ('3dph', '3dnr', '3dhr')
+    ro%hdiv = ds0
+  end select
   ro%eps_r = eps_r; ro%velmin = velmin; ro%velmax = velmax
   hn_ptr = c_null_ptr
   if (.not. sigmaconst) hn_ptr = st_LHp_n%h

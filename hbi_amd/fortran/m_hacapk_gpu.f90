@@ -31,6 +31,7 @@ module m_HACApK_gpu
     real(8), pointer :: ev11(:) => null(), ev12(:) => null(), ev13(:) => null()
     real(8), pointer :: ev21(:) => null(), ev22(:) => null(), ev23(:) => null()
     real(8), pointer :: ev31(:) => null(), ev32(:) => null(), ev33(:) => null()
+    real(8), pointer :: yel(:) => null(), yer(:) => null(), xs4(:) => null(), zs4(:) => null(), ds(:) => null()
   end type st_HACApK_calc_entry
 
   type :: st_HACApK_lcontrol
@@ -90,8 +91,18 @@ contains
   integer function problem_kind(problem)
     character(*), intent(in) :: problem
     select case (trim(problem))
-    case ('2dvs', '2dp')
+    case ('2dvs')
       problem_kind = HBI_KIND_2DVS
+    case ('2dp')
+      problem_kind = HBI_KIND_2DP
+    case ('3dp')
+      problem_kind = HBI_KIND_3DP
+    case ('2dph', '2dnh')
+      problem_kind = HBI_KIND_2DPH
+    case ('2dn')
+      problem_kind = HBI_KIND_2DN
+    case ('25d')
+      problem_kind = HBI_KIND_25D
     case ('3dph', '3dhr')
       problem_kind = HBI_KIND_OKADA_HS
     case ('3dnr')
@@ -150,6 +161,8 @@ contains
     call set_ptr(g%ev11, st_bemv%ev11); call set_ptr(g%ev12, st_bemv%ev12); call set_ptr(g%ev13, st_bemv%ev13)
     call set_ptr(g%ev21, st_bemv%ev21); call set_ptr(g%ev22, st_bemv%ev22); call set_ptr(g%ev23, st_bemv%ev23)
     call set_ptr(g%ev31, st_bemv%ev31); call set_ptr(g%ev32, st_bemv%ev32); call set_ptr(g%ev33, st_bemv%ev33)
+    call set_ptr(g%yel, st_bemv%yel); call set_ptr(g%yer, st_bemv%yer)
+    call set_ptr(g%xs4, st_bemv%xs4); call set_ptr(g%zs4, st_bemv%zs4); call set_ptr(g%ds, st_bemv%ds)
     o%leaf = int(st_ctl%param(21))
     o%eta = st_ctl%param(51)
     o%eps_aca = ztol

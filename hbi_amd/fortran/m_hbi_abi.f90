@@ -5,7 +5,8 @@ module m_hbi_abi
 
   integer(c_int), parameter :: HBI_OK = 0, HBI_E_STEP = -4, HBI_RK_OUTPUT_FULL = 2
   integer(c_int), parameter :: HBI_KIND_2DVS = 1, HBI_KIND_OKADA_HS = 2, HBI_KIND_OKADA_FS = 3, HBI_KIND_TRI_FS = 4, &
-    & HBI_KIND_TRI_HS = 5
+    & HBI_KIND_TRI_HS = 5, HBI_KIND_2DP = 6, HBI_KIND_3DP = 7, HBI_KIND_2DPH = 8, HBI_KIND_2DN = 9, &
+    & HBI_KIND_25D = 10
   integer(c_int), parameter :: HBI_PREC_F64 = 0, HBI_PREC_F32 = 1, HBI_PREC_MIXED = 2
 
   type, bind(C) :: hbi_geom
@@ -20,6 +21,7 @@ module m_hbi_abi
     type(c_ptr) :: ev11 = c_null_ptr, ev12 = c_null_ptr, ev13 = c_null_ptr
     type(c_ptr) :: ev21 = c_null_ptr, ev22 = c_null_ptr, ev23 = c_null_ptr
     type(c_ptr) :: ev31 = c_null_ptr, ev32 = c_null_ptr, ev33 = c_null_ptr
+    type(c_ptr) :: yel = c_null_ptr, yer = c_null_ptr, xs4 = c_null_ptr, zs4 = c_null_ptr, ds = c_null_ptr
   end type hbi_geom
 
   type, bind(C) :: hbi_hmat_opts

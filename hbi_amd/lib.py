@@ -19,7 +19,8 @@ E_STEP = -4
 
 _FIELDS = ("xcol", "ycol", "zcol", "xel", "xer", "ang", "angd", "dsl", "dsd", "rake",
            "xs1", "xs2", "xs3", "ys1", "ys2", "ys3", "zs1", "zs2", "zs3",
-           "ev11", "ev12", "ev13", "ev21", "ev22", "ev23", "ev31", "ev32", "ev33")
+           "ev11", "ev12", "ev13", "ev21", "ev22", "ev23", "ev31", "ev32", "ev33",
+           "yel", "yer", "xs4", "zs4", "ds")
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 

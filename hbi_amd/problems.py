@@ -31,10 +31,16 @@ KIND_OKADA_HS = 2     # 3dph / 3dhr
 KIND_OKADA_FS = 3     # 3dnr
 KIND_TRI_FS = 4       # 3dnt
 KIND_TRI_HS = 5       # 3dht
+KIND_2DP = 6          # 2dp: in-plane, full space
+KIND_3DP = 7          # 3dp: planar quadrilaterals, full space
+KIND_2DPH = 8         # 2dph, 2dnh: in-plane dipping segments, half-space
+KIND_2DN = 9          # 2dn: in-plane nonplanar segments, full space
+KIND_25D = 10         # 25d: 2.5-D rectangles
 
 _GEO_FIELDS = ("xcol", "ycol", "zcol", "xel", "xer", "ang", "angd", "dsl", "dsd", "rake",
                "xs1", "xs2", "xs3", "ys1", "ys2", "ys3", "zs1", "zs2", "zs3",
-               "ev11", "ev12", "ev13", "ev21", "ev22", "ev23", "ev31", "ev32", "ev33")
+               "ev11", "ev12", "ev13", "ev21", "ev22", "ev23", "ev31", "ev32", "ev33",
+               "yel", "yer", "xs4", "zs4", "ds")
 
 
 @dataclass
@@ -59,8 +65,8 @@ class Geometry:
         return _GEO_FIELDS
 
     def geo_matrix(self) -> np.ndarray:
-        """(n, 28) column-major table in the field order of ``_GEO_FIELDS``."""
-        g = np.zeros((28, self.n), dtype=np.float64)
+        """(n, 33) column-major table in the field order of ``_GEO_FIELDS``."""
+        g = np.zeros((len(_GEO_FIELDS), self.n), dtype=np.float64)
         for k, f in enumerate(_GEO_FIELDS):
             a = self.arrays.get(f)
             if a is not None:
@@ -82,6 +88,67 @@ def mesh_2dvs(n: int, ds0: float) -> Geometry:
     g = Geometry(kind=KIND_2DVS, n=n, ds=np.full(n, ds0), ds0=ds0,
                  arrays=dict(xcol=_c(xcol), ycol=_c(z), zcol=_c(z.copy()), xel=_c(xel), xer=_c(xer)))
     return g
+
+
+def mesh_2dp(n: int, ds0: float) -> Geometry:
+    """2-D in-plane planar fault in a full space (coordinate2dp, main_LH.f90:1701-1718)."""
+    g = mesh_2dvs(n, ds0)
+    g.kind = KIND_2DP
+    return g
+
+
+def mesh_3dp(imax: int, jmax: int, ds0: float) -> Geometry:
+    """Planar fault of imax x jmax squares in a full space, in the x-z plane
+    (coordinate3dp, main_LH.f90:1752-1779): corners 1..4 at (+,+), (-,+), (-,-), (+,-)."""
+    n = imax * jmax
+    i = np.repeat(np.arange(1, imax + 1), jmax).astype(np.float64)
+    j = np.tile(np.arange(1, jmax + 1), imax).astype(np.float64)
+    xcol = (i - imax // 2 - 0.5) * ds0
+    zcol = -(j - jmax // 2 - 0.5) * ds0
+    h = 0.5 * ds0
+    arr = dict(xcol=xcol, ycol=np.zeros(n), zcol=zcol, xs1=xcol + h, xs2=xcol - h, xs3=xcol - h, xs4=xcol + h,
+               zs1=zcol + h, zs2=zcol + h, zs3=zcol - h, zs4=zcol - h)
+    return Geometry(kind=KIND_3DP, n=n, ds=np.full(n, ds0 * ds0), ds0=ds0, arrays={k: _c(v) for k, v in arr.items()})
+
+
+def mesh_2dph(n: int, ds0: float, dipangle: float) -> Geometry:
+    """2-D in-plane fault dipping from the free surface (y = depth) in a
+    half-space (coordinate2dph, main_LH.f90:1720-1737)."""
+    i = np.arange(1, n + 1, dtype=np.float64)
+    c, s = math.cos(dipangle * PI / 180), math.sin(dipangle * PI / 180)
+    return mesh_segments((i - 1) * ds0 * c, i * ds0 * c, (i - 1) * ds0 * s, i * ds0 * s, KIND_2DPH)
+
+
+def mesh_segments(xel, xer, yel, yer, kind: int = KIND_2DN, w: float = 1e8) -> Geometry:
+    """2-D segments from their end points (coordinate2dn, main_LH.f90:1739-1750):
+    lengths ds, angles atan2, midpoints; kind 2dn, 2dnh (KIND_2DPH) or 25d
+    (element width ``w``, the reference's fwid)."""
+    xel, xer, yel, yer = (_c(a) for a in (xel, xer, yel, yer))
+    n = len(xel)
+    ds = np.sqrt((xer - xel) ** 2 + (yer - yel) ** 2)
+    ang = np.arctan2(yer - yel, xer - xel)
+    arr = dict(xcol=0.5 * (xel + xer), ycol=0.5 * (yel + yer), zcol=np.zeros(n), xel=xel, xer=xer, yel=yel, yer=yer,
+               ang=ang, ds=ds)
+    return Geometry(kind=kind, n=n, w=float(w) if kind == KIND_25D else 0.0, ds=_c(ds), ds0=float(ds.min()),
+                    arrays={k: _c(v) for k, v in arr.items()})
+
+
+def read_segments(path: str) -> np.ndarray:
+    """Geometry file of the 2-D problems: one `xel xer yel yer` line per element
+    (main_LH.f90:443-452)."""
+    return np.loadtxt(path, ndmin=2)[:, :4]
+
+
+def mesh_rectangles(rows: np.ndarray, halfspace: bool = True, crake: float = 0.0) -> Geometry:
+    """Rectangles from the 3dnr / 3dhr geometry file rows `xcol ycol zcol ang
+    angd dsl dsd` (main_LH.f90:460-471): ds = dsl dsd, w = ds0 = min dsl."""
+    rows = np.asarray(rows, dtype=np.float64)
+    n = rows.shape[0]
+    arr = dict(xcol=rows[:, 0], ycol=rows[:, 1], zcol=rows[:, 2], ang=rows[:, 3], angd=rows[:, 4], dsl=rows[:, 5],
+               dsd=rows[:, 6], rake=np.full(n, crake / 180.0 * PI))
+    ds0 = float(rows[:, 5].min())
+    return Geometry(kind=KIND_OKADA_HS if halfspace else KIND_OKADA_FS, n=n, w=ds0, ds=_c(rows[:, 5] * rows[:, 6]),
+                    ds0=ds0, arrays={k: _c(v) for k, v in arr.items()})
 
 
 def mesh_3dph(imax: int, jmax: int, ds0: float, dipangle: float = 90.0, crake: float = 0.0,

@@ -44,6 +44,11 @@ extern "C" {
 #define HBI_KIND_OKADA_FS 3   /* rectangles, full space: '3dnr' */
 #define HBI_KIND_TRI_FS 4     /* triangles, full space: '3dnt' */
 #define HBI_KIND_TRI_HS 5     /* triangles, half-space: '3dht' (z <= 0) */
+#define HBI_KIND_2DP 6        /* 2-D in-plane, full space, planar: '2dp' (matels2dp_ij) */
+#define HBI_KIND_3DP 7        /* 3-D planar full space, quadrilaterals: '3dp' (matel3dp_ij) */
+#define HBI_KIND_2DPH 8       /* 2-D in-plane, half-space, dipping segments: '2dph', '2dnh' (matel2dph2_ij) */
+#define HBI_KIND_2DN 9        /* 2-D in-plane, full space, nonplanar segments: '2dn' (tensor2d_ij) */
+#define HBI_KIND_25D 10       /* 2.5-D: rectangles of width w seen in their plane: '25d' (matelrec_ij) */
 
 /* storage precision of the H-matrix factors used by a product */
 #define HBI_PREC_F64 0        /* FP64 factors (the reference's arithmetic) */
@@ -55,13 +60,16 @@ extern "C" {
 typedef struct hbi_geom {
     int kind, n;
     int vcomp;   /* 0: shear traction ('s'), 1: normal traction ('n') */
-    int md;      /* triangle slip mode: 0 's' (receiver rake), 1 'st', 2 'dp', 3 'open' */
-    double w;    /* okada normalisation length (st_bemv%w) */
+    int md;      /* slip mode (st_bemv%md): 0 's' / 'o', 1 'st', 2 'dp', 3 'open' */
+    double w;    /* st_bemv%w: okada normalisation length; 25d element width */
     const double *xcol, *ycol, *zcol;
     const double *xel, *xer;
     const double *ang, *angd, *dsl, *dsd, *rake;
     const double *xs1, *xs2, *xs3, *ys1, *ys2, *ys3, *zs1, *zs2, *zs3;
     const double *ev11, *ev12, *ev13, *ev21, *ev22, *ev23, *ev31, *ev32, *ev33;
+    const double *yel, *yer;   /* 2-D segment end points (2dph, 2dnh, 2dn, 25d) */
+    const double *xs4, *zs4;   /* fourth corner of 3dp quadrilaterals */
+    const double *ds;          /* st_bemv%ds: 2-D segment lengths (25d) */
 } hbi_geom;
 
 /* H-matrix construction options (the st_ctl%param entries that matter) */

@@ -11,6 +11,8 @@
  *   - 2-D antiplane kernel with image            m_HACApK_calc_entry_ij.f90:55-57, 685-695
  *   - Okada rectangular kernel                   m_HACApK_calc_entry_ij.f90:1065-1145, okada.f90:10-597
  *   - TD full-space kernel                       m_HACApK_calc_entry_ij.f90:802-851, TDstressFS.f90:45-403, 560-627, 1012-1019
+ *   - 2-D / 2.5-D / planar kernels 2dp, 3dp,     m_HACApK_calc_entry_ij.f90:92-166, 300-339, 412-470, 648-661,
+ *     2dph (2dnh), 2dn, 25d                      697-750, 934-1064
  *   - TD half-space kernel                       m_HACApK_calc_entry_ij.f90:852-898, TDstressFS.f90:7-44, 236-293, 405-558,
  *                                                629-1011 (free-surface correction: strains as complex-step
  *                                                derivatives of the published harmonic displacement field)
@@ -56,7 +58,7 @@ static const double og_vs = 3.464;
 /* ------------------------------------------------------------------------- */
 typedef struct {
     int kind;   /* 1: 2dvs antiplane, 2: okada half-space (3dph/3dhr), 3: okada full-space (3dnr), 4: TD full-space (3dnt),
-                   5: TD half-space (3dht) */
+                   5: TD half-space (3dht), 6: 2dp, 7: 3dp, 8: 2dph/2dnh, 9: 2dn, 10: 25d */
     int n;
     int vcomp;  /* 0: shear traction ('s'), 1: normal traction ('n') */
     int md;     /* TD slip mode: 0 's' (receiver rake), 1 'st', 2 'dp', 3 'open' */
@@ -66,6 +68,7 @@ typedef struct {
     const double *ang, *angd, *dsl, *dsd, *rake;
     const double *xs1, *xs2, *xs3, *ys1, *ys2, *ys3, *zs1, *zs2, *zs3;
     const double *ev11, *ev12, *ev13, *ev21, *ev22, *ev23, *ev31, *ev32, *ev33;
+    const double *yel, *yer, *xs4, *zs4, *ds;
 } og_geom;
 
 /* coordinate2dp, main_LH.f90:1701-1718 */
@@ -836,6 +839,143 @@ static double og_tri_ij(const og_geom *g, int i, int j)
     return p[4] * cos(g->rake[i]) + p[5] * sin(g->rake[i]);
 }
 
+/* ---- 2-D, 2.5-D and planar 3-D kernels ------------------------------------ */
+
+/* matels2dp_ij, m_HACApK_calc_entry_ij.f90:648-661 */
+static double og_2dp_ij(const og_geom *g, int i, int j)
+{
+    double factor = og_rigid / (2.0 * OG_PI * (1.0 - og_pois));
+    return factor * (1.0 / (g->xcol[i] - g->xer[j]) - 1.0 / (g->xcol[i] - g->xel[j]));
+}
+
+/* matel3dp_ij + ret3dp, m_HACApK_calc_entry_ij.f90:412-421, 440-470 */
+static double og_3dp_ij(const og_geom *g, int i, int j)
+{
+    double dx[4] = { g->xcol[i] - g->xs1[j], g->xcol[i] - g->xs2[j], g->xcol[i] - g->xs3[j], g->xcol[i] - g->xs4[j] };
+    double dz[4] = { g->zcol[i] - g->zs1[j], g->zcol[i] - g->zs2[j], g->zcol[i] - g->zs3[j], g->zcol[i] - g->zs4[j] };
+    double dr[4];
+    for (int k = 0; k < 4; k++) dr[k] = sqrt(dx[k] * dx[k] + dz[k] * dz[k]);
+    double factor = og_rigid / (4.0 * OG_PI), alpha = 1 / (2 - 2 * og_pois);
+    double t1 = 2.0 * (1 - alpha) * (dr[3] / (dx[3] * dz[3]) + dr[1] / (dx[1] * dz[1]) - dr[2] / (dx[2] * dz[2]) -
+                                     dr[0] / (dx[0] * dz[0]));
+    if (g->md == 0 && g->vcomp == 0) {
+        double t2 = -(1 - 2 * alpha) / dx[0] * (dr[3] / dz[3] + dz[3] / dr[3] - dr[0] / dz[0] - dz[0] / dr[0]);
+        double t3 = -(1 - 2 * alpha) / dx[1] * (dr[1] / dz[1] + dz[1] / dr[1] - dr[2] / dz[2] - dz[2] / dr[2]);
+        return factor * (t1 + t2 + t3);
+    }
+    if (g->md == 3 && g->vcomp == 1) return factor * t1;
+    return 0.0;
+}
+
+/* D2dip2, m_HACApK_calc_entry_ij.f90:143-166: edge dislocation (s1, s2) at
+ * (xs, ys) in the half-space y > 0, stress at (X, Y) */
+static void og_d2dip2(double X, double Y, double xs, double ys, double s1, double s2, double *sxx, double *syy,
+                      double *sxy)
+{
+    double a = X - xs, b = Y - ys, c = Y + ys;
+    double r1 = sqrt(a * a + b * b), r2 = sqrt(a * a + c * c);
+    double r14 = r1 * r1 * r1 * r1, r24 = r2 * r2 * r2 * r2, r26 = r24 * r2 * r2;
+    double coef = og_rigid / 2.0 / OG_PI / (1.0 - og_pois);
+    double a2 = a * a, b2 = b * b, c2 = c * c, c3 = c2 * c, a4 = a2 * a2;
+    double sxx1 = s1 * coef * (b * (b2 + 3 * a2) / r14 - c * (c2 + 3 * a2) / r24 +
+                               2 * ys / r26 * (6 * Y * c * a2 - b * c3 - a4));
+    double sxx2 = s2 * coef * (a * (b2 - a2) / r14 - a * (c2 - a2) / r24 +
+                               4 * ys * a / r26 * ((2 * ys - Y) * c2 + (3 * Y + 2 * ys) * a2));
+    double syy1 = s1 * coef * (b * (b2 - a2) / r14 - c * (c2 - a2) / r24 -
+                               2 * ys / r26 * (6 * Y * c * a2 - (3 * Y + ys) * c3 + a4));
+    double syy2 = -s2 * coef * (a * (3 * b2 + a2) / r14 - a * (3 * c2 + a2) / r24 - 4 * ys * a * Y / r26 * (3 * c2 - a2));
+    double sxy1 = s1 * coef * (a * (b2 - a2) / r14 - a * (c2 - a2) / r24 + 4 * ys * a * Y / r26 * (3 * c2 - a2));
+    double sxy2 = s2 * coef * (b * (b2 - a2) / r14 - c * (c2 - a2) / r24 +
+                               2 * ys / r26 * (6 * Y * c * a2 - a4 + (ys - Y) * c3));
+    *sxx = sxx1 + sxx2;
+    *syy = syy1 + syy2;
+    *sxy = sxy1 + sxy2;
+}
+
+/* matel2dph2_ij, m_HACApK_calc_entry_ij.f90:92-116 */
+static double og_2dph_ij(const og_geom *g, int i, int j)
+{
+    double s1 = cos(g->ang[j]), s2 = sin(g->ang[j]);
+    double p[3], q[3];
+    og_d2dip2(g->xcol[i], g->ycol[i], g->xel[j], g->yel[j], s1, s2, &p[0], &p[1], &p[2]);
+    og_d2dip2(g->xcol[i], g->ycol[i], g->xer[j], g->yer[j], s1, s2, &q[0], &q[1], &q[2]);
+    double sxx = p[0] - q[0], syy = p[1] - q[1], sxy = p[2] - q[2];
+    if (g->vcomp == 1) return -0.5 * (sxx + syy) + 0.5 * (sxx - syy) * cos(2 * g->ang[i]) + sxy * sin(2 * g->ang[i]);
+    return -0.5 * (sxx - syy) * sin(2 * g->ang[i]) + sxy * cos(2 * g->ang[i]);
+}
+
+/* inte11s .. inte22o, m_HACApK_calc_entry_ij.f90:697-750: the in-plane
+ * segment integrals at one end (k = 0: 11, 1: 12, 2: 22), shear or opening */
+static double og_inte(int k, int open, double x1, double x2)
+{
+    double r = sqrt(x1 * x1 + x2 * x2), pa = og_vs / 6.0, c = 2 * og_vs / OG_PI * (1 - pa * pa);
+    double r4 = r * r * r * r;
+    if (!open) {
+        if (k == 0) return -c * x2 * (3 * x1 * x1 + x2 * x2) / r4;
+        if (k == 1) return c * x1 * (x1 * x1 - x2 * x2) / r4;
+        return c * x2 * (x1 * x1 - x2 * x2) / r4;
+    }
+    if (k == 0) return c * x1 * (x1 * x1 - x2 * x2) / r4;
+    if (k == 1) return c * x2 * (x1 * x1 - x2 * x2) / r4;
+    return c * x1 * (3 * x1 * x1 + x2 * x2) / r4;
+}
+
+/* tensor2d_ij, m_HACApK_calc_entry_ij.f90:300-339 */
+static double og_2dn_ij(const og_geom *g, int i, int j)
+{
+    double cj = cos(g->ang[j]), sj = sin(g->ang[j]);
+    double ax = g->xcol[i] - g->xel[j], ay = g->ycol[i] - g->yel[j];
+    double bx = g->xcol[i] - g->xer[j], by = g->ycol[i] - g->yer[j];
+    double xp = cj * ax + sj * ay, xm = cj * bx + sj * by;
+    double yp = -sj * ax + cj * ay, ym = -sj * bx + cj * by;
+    int open = g->md == 3;
+    double k[3];
+    for (int c = 0; c < 3; c++) k[c] = -0.5 * og_rigid / og_vs * (og_inte(c, open, xp, yp) - og_inte(c, open, xm, ym));
+    double sin2 = sin(-2 * g->ang[j]), cos2 = cos(-2 * g->ang[j]);
+    double sxx = 0.5 * (k[0] + k[2]) + 0.5 * (k[0] - k[2]) * cos2 + k[1] * sin2;
+    double sxy = -0.5 * (k[0] - k[2]) * sin2 + k[1] * cos2;
+    double syy = 0.5 * (k[0] + k[2]) - 0.5 * (k[0] - k[2]) * cos2 - k[1] * sin2;
+    if (g->vcomp == 1)
+        return -(0.5 * (sxx + syy) - 0.5 * (sxx - syy) * cos(2 * g->ang[i]) - sxy * sin(2 * g->ang[i]));
+    return 0.5 * (sxx - syy) * sin(-2 * g->ang[i]) + sxy * cos(-2 * g->ang[i]);
+}
+
+/* matelrec_ij + dc3d25 + ua25, m_HACApK_calc_entry_ij.f90:934-1064 */
+static double og_25d_ij(const og_geom *g, int i, int j)
+{
+    double alpha = (1.0 + (0.5 / og_pois - 1.0)) / (1.0 + 2.0 * (0.5 / og_pois - 1.0));
+    double st = g->ang[j];
+    double x = cos(st) * (g->xcol[i] - g->xcol[j]) + sin(st) * (g->ycol[i] - g->ycol[j]);
+    double q = -sin(st) * (g->xcol[i] - g->xcol[j]) + cos(st) * (g->ycol[i] - g->ycol[j]);
+    double xi[2] = { x + 0.5 * g->ds[j], x - 0.5 * g->ds[j] }, et[2] = { 0.5 * g->w, -0.5 * g->w };
+    double u4 = 0, u5 = 0, u7 = 0, u8 = 0, u12 = 0;
+    for (int k = 0; k < 2; k++)
+        for (int m = 0; m < 2; m++) {
+            double X = xi[m], E = et[k];
+            double r = sqrt(X * X + E * E + q * q);
+            double x11 = 1.0 / (r * (r + X)), y11 = 1.0 / (r * (r + E));
+            double y32 = (r + (r + E)) * y11 * y11 / r, r3 = r * r * r;
+            double d4 = -(1.0 - alpha) / 2.0 * q * y11 - alpha / 2 * X * X * q * y32;
+            double d6 = (1.0 - alpha) / 2.0 * X * y11 + alpha / 2 * X * q * q * y32;
+            double d7 = (1.0 - alpha) / 2.0 * X * y11 + alpha / 2 * X * (E / r3 + X * X * y32) + E / 2.0 * x11;
+            double d9 = (1.0 - alpha) / 2.0 * (q * y11) - alpha / 2 * q * (E / r3 + X * X * y32);
+            double d11 = alpha / 2 * (E * q / r3);
+            double s = (m + k == 1) ? -1.0 : 1.0, tp = OG_PI * 2;
+            u4 += s * (-(d4 / tp));
+            u5 += s * (d6 / tp);
+            u7 += s * (-(d7 / tp));
+            u8 += s * (d9 / tp);
+            u12 += s * (d11 / tp);
+        }
+    double exx = -u4, eyy = -u8, ezz = -u12, exy = -0.5 * (u7 + u5);
+    double sxx = og_rigid * (exx + eyy + ezz) + 2 * og_rigid * exx;
+    double syy = og_rigid * (exx + eyy + ezz) + 2 * og_rigid * eyy;
+    double sxy = 2 * og_rigid * exy;
+    double d = 2 * (g->ang[i] - g->ang[j]);
+    if (g->vcomp == 1) return -0.5 * (sxx + syy) + 0.5 * (sxx - syy) * cos(d) + sxy * sin(d);
+    return -0.5 * (sxx - syy) * sin(d) + sxy * cos(d);
+}
+
 /* HACApK_entry_ij, m_HACApK_calc_entry_ij.f90:24-90 (0-based, mesh order) */
 double og_entry(const og_geom *g, int i, int j)
 {
@@ -846,6 +986,11 @@ double og_entry(const og_geom *g, int i, int j)
     case 3: return og_okada_ij(g, i, j, 1);
     case 4:
     case 5: return og_tri_ij(g, i, j);
+    case 6: return og_2dp_ij(g, i, j);
+    case 7: return og_3dp_ij(g, i, j);
+    case 8: return og_2dph_ij(g, i, j);
+    case 9: return og_2dn_ij(g, i, j);
+    case 10: return og_25d_ij(g, i, j);
     }
     return NAN;
 }

@@ -7,10 +7,12 @@
 ! oracle/_ref/libhbiref.so.  It exists to pin the oracle's kernel restatement
 ! and to write the golden vectors under tests/golden/.
 !
-! geo(n,28) columns: xcol ycol zcol xel xer ang angd dsl dsd rake
+! geo(n,33) columns: xcol ycol zcol xel xer ang angd dsl dsd rake
 !                    xs1 xs2 xs3 ys1 ys2 ys3 zs1 zs2 zs3
 !                    ev11 ev12 ev13 ev21 ev22 ev23 ev31 ev32 ev33
-! kind: 1 '2dvs', 2 '3dph', 3 '3dnr', 4 '3dnt', 5 '3dht'; vcomp: 0 's', 1 'n';
+!                    yel yer xs4 zs4 ds
+! kind: 1 '2dvs', 2 '3dph', 3 '3dnr', 4 '3dnt', 5 '3dht', 6 '2dp', 7 '3dp',
+!       8 '2dph', 9 '2dn', 10 '25d'; vcomp: 0 's', 1 'n';
 ! md: 0 's', 1 'st', 2 'dp', 3 'open'.  ii, jj are 0-based.
 module hbiref_harness
   use iso_c_binding
@@ -21,7 +23,7 @@ contains
   subroutine hbiref_entries(kind, n, vcomp, md, w, geo, np, ii, jj, out) bind(C, name="hbiref_entries")
     integer(c_int), value :: kind, n, vcomp, md, np
     real(c_double), value :: w
-    real(c_double), intent(in) :: geo(n, 28)
+    real(c_double), intent(in) :: geo(n, 33)
     integer(c_int), intent(in) :: ii(np), jj(np)
     real(c_double), intent(out) :: out(np)
     type(st_HACApK_calc_entry) :: bem
@@ -38,6 +40,16 @@ contains
       bem%problem = '3dnt'
     case (5)
       bem%problem = '3dht'
+    case (6)
+      bem%problem = '2dp'
+    case (7)
+      bem%problem = '3dp'
+    case (8)
+      bem%problem = '2dph'
+    case (9)
+      bem%problem = '2dn'
+    case (10)
+      bem%problem = '25d'
     end select
     if (vcomp == 1) then
       bem%v = 'n'
@@ -61,6 +73,7 @@ contains
     allocate(bem%zs1(n), bem%zs2(n), bem%zs3(n))
     allocate(bem%ev11(n), bem%ev12(n), bem%ev13(n), bem%ev21(n), bem%ev22(n), bem%ev23(n))
     allocate(bem%ev31(n), bem%ev32(n), bem%ev33(n))
+    allocate(bem%yel(n), bem%yer(n), bem%xs4(n), bem%zs4(n), bem%ds(n))
     bem%xcol = geo(:, 1); bem%ycol = geo(:, 2); bem%zcol = geo(:, 3)
     bem%xel = geo(:, 4); bem%xer = geo(:, 5)
     bem%ang = geo(:, 6); bem%angd = geo(:, 7); bem%dsl = geo(:, 8); bem%dsd = geo(:, 9)
@@ -71,12 +84,14 @@ contains
     bem%ev11 = geo(:, 20); bem%ev12 = geo(:, 21); bem%ev13 = geo(:, 22)
     bem%ev21 = geo(:, 23); bem%ev22 = geo(:, 24); bem%ev23 = geo(:, 25)
     bem%ev31 = geo(:, 26); bem%ev32 = geo(:, 27); bem%ev33 = geo(:, 28)
+    bem%yel = geo(:, 29); bem%yer = geo(:, 30); bem%xs4 = geo(:, 31); bem%zs4 = geo(:, 32); bem%ds = geo(:, 33)
     do p = 1, np
       out(p) = HACApK_entry_ij(ii(p) + 1, jj(p) + 1, bem)
     end do
     deallocate(bem%xcol, bem%ycol, bem%zcol, bem%xel, bem%xer, bem%ang, bem%angd, bem%dsl, bem%dsd, bem%rake)
     deallocate(bem%xs1, bem%xs2, bem%xs3, bem%ys1, bem%ys2, bem%ys3, bem%zs1, bem%zs2, bem%zs3)
     deallocate(bem%ev11, bem%ev12, bem%ev13, bem%ev21, bem%ev22, bem%ev23, bem%ev31, bem%ev32, bem%ev33)
+    deallocate(bem%yel, bem%yer, bem%xs4, bem%zs4, bem%ds)
   end subroutine hbiref_entries
 
   ! free-surface correction strains of one angular dislocation

@@ -118,6 +118,45 @@ def main():
     vals = np.array([O.ref_angfsc(*a) for a in args])
     np.savez_compressed(os.path.join(HERE, "angfsc_points.npz"), args=args, vals=vals)
     print("angfsc_points.npz", m)
+    # the other kernels, on the configurations of the reference's examples
+    # (examples/2dp.in: 1000 x 0.1 km; bp3dip30thrust.in: 1600 x 0.025 km at
+    # 30 degrees; 3dp squares) and on curved 2-D faults
+    def pairs(g, nrand):
+        near = [(i, i) for i in range(0, g.n, 4)] + [(i, i + 1) for i in range(0, g.n - 1, 9)]
+        return sample_pairs(g.n, rng, nrand, near)
+
+    def curve(n, length, amp, y0):
+        x = np.linspace(0.0, length, n + 1)
+        y = y0 + amp * np.sin(6 * np.pi * x / length) + 0.3 * amp * np.sin(22 * np.pi * x / length)
+        return x[:-1], x[1:], y[:-1], y[1:]
+
+    g = P.mesh_2dp(1000, 0.1)
+    ii, jj = pairs(g, 1500)
+    save("kernel_entries_2dp.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    g = P.mesh_3dp(32, 16, 0.5)
+    ii, jj = pairs(g, 1500)
+    save("kernel_entries_3dp.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    g.md, g.vcomp = 3, 1
+    ii, jj = pairs(g, 600)
+    save("kernel_entries_3dp_mdopen_n.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    g = P.mesh_2dph(1600, 0.025, 30.0)
+    ii, jj = pairs(g, 1500)
+    save("kernel_entries_2dph.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    g.vcomp = 1
+    ii, jj = pairs(g, 600)
+    save("kernel_entries_2dph_n.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    for kind, tag, y0, extra in ((P.KIND_2DN, "2dn", 0.0, {}), (P.KIND_2DPH, "2dnh", 2.0, {}),
+                                 (P.KIND_25D, "25d", 0.0, {"w": 10.0})):
+        g = P.mesh_segments(*curve(600, 30.0, 0.5, y0), kind=kind, **extra)
+        ii, jj = pairs(g, 1000)
+        save("kernel_entries_%s.npz" % tag, g, ii, jj, O.ref_entries(g, ii, jj))
+        g.vcomp = 1
+        ii, jj = pairs(g, 500)
+        save("kernel_entries_%s_n.npz" % tag, g, ii, jj, O.ref_entries(g, ii, jj))
+    g = P.mesh_segments(*curve(600, 30.0, 0.5, 0.0), kind=P.KIND_2DN)
+    g.md, g.vcomp = 3, 1
+    ii, jj = pairs(g, 500)
+    save("kernel_entries_2dn_mdopen_n.npz", g, ii, jj, O.ref_entries(g, ii, jj))
 
 
 if __name__ == "__main__":

@@ -274,3 +274,71 @@ def test_gpu_driver_forward_halfspace_triangles(tmp_path):
     yf = H.HMatrix(P.mesh_triangles(P.read_stl(str(tmp_path / "thrust.stl")), crake=90.0), leaf=32,
                    eps=1e-6).matvec(np.ones(g.n), H.PREC_F64)
     assert np.abs(yf - y).max() > 1e-2 * np.abs(y).max()
+
+
+def _curve(n, length, amp, y0):
+    x = np.linspace(0.0, length, n + 1)
+    y = y0 + amp * np.sin(6 * np.pi * x / length) + 0.3 * amp * np.sin(22 * np.pi * x / length)
+    return np.stack([x[:-1], x[1:], y[:-1], y[1:]], axis=1)
+
+
+def _forward_cases():
+    seg = _curve(400, 20.0, 0.4, 0.0)
+    segh = _curve(400, 20.0, 0.4, 1.5)
+    rect = P.mesh_3dph(24, 12, 1.0, 60.0)
+    rows = np.stack([rect.xcol, rect.ycol, rect.zcol, rect.ang, rect.angd, rect.dsl, rect.dsd], axis=1)
+    return [
+        # problem, keys, files, library geometry, divided by ds0
+        ("2dp", {"ncellg": 500, "ds": 0.1}, {}, lambda: P.mesh_2dp(500, 0.1), None),
+        ("2dph", {"ncellg": 800, "ds": 0.025, "dipangle": 30.0}, {}, lambda: P.mesh_2dph(800, 0.025, 30.0), None),
+        ("3dp", {"imax": 32, "jmax": 16, "ds": 0.5}, {}, lambda: P.mesh_3dp(32, 16, 0.5), None),
+        ("2dn", {"ncellg": 400, "geometry_file": "seg.dat"}, {"seg.dat": seg},
+         lambda: P.mesh_segments(*seg.T, kind=P.KIND_2DN), None),
+        ("2dnh", {"ncellg": 400, "geometry_file": "seg.dat"}, {"seg.dat": segh},
+         lambda: P.mesh_segments(*segh.T, kind=P.KIND_2DPH), None),
+        ("25d", {"ncellg": 400, "geometry_file": "seg.dat", "fwid": 5.0}, {"seg.dat": seg},
+         lambda: P.mesh_segments(*seg.T, kind=P.KIND_25D, w=5.0), None),
+        ("3dhr", {"ncellg": rect.n, "geometry_file": "rect.dat"}, {"rect.dat": rows},
+         lambda: P.mesh_rectangles(rows, halfspace=True), 1.0),
+        ("3dnr", {"ncellg": rect.n, "geometry_file": "rect.dat"}, {"rect.dat": rows},
+         lambda: P.mesh_rectangles(rows, halfspace=False), 1.0),
+    ]
+
+
+@pytest.mark.parametrize("case", _forward_cases(), ids=lambda c: c[0])
+def test_gpu_driver_forward_other_problems(tmp_path, case):
+    """The other problems of the reference's mesh switch (main_LH.f90:432-471,
+    595-612): the driver's H*1 against the library's product on the geometry
+    built the same way (the 2-D segments, the 3dnr / 3dhr rectangles from
+    their geometry files; only the rectangles are divided by ds0)."""
+    problem, keys, files, make, div = case
+    for name, data in files.items():
+        np.savetxt(tmp_path / name, data, fmt="%.17e")
+    _run(tmp_path, dict(problem=problem, forward=True, eps_h=1e-6, leaf=32, filenumber=9, **keys))
+    out = np.loadtxt(tmp_path / "output" / "stress9")
+    g = make()
+    Hg = H.HMatrix(g, leaf=32, eps=1e-6)
+    y = Hg.matvec(np.ones(g.n), H.PREC_F64)
+    if div:
+        y = y / g.ds0
+    q = lambda v: 5.01e-4 * np.abs(v) + 1e-12
+    assert out.shape == (g.n, 6)
+    assert (np.abs(out[:, 4] - y) <= q(y) + 1e-9 * np.abs(y).max()).all()
+    lod = Hg.lod
+    for c, arr in ((0, g.xcol), (1, g.ycol)):
+        assert (np.abs(out[:, c] - arr[lod]) <= 5.01e-4 * np.abs(arr[lod]) + 1e-9).all()
+
+
+def test_gpu_driver_3dhr_file_reproduces_3dph(tmp_path):
+    """3dhr reads rectangles from a file (main_LH.f90:460-471); written from
+    the 3dph mesh it gives the 3dph matrix: identical forward outputs."""
+    rect = P.mesh_3dph(24, 12, 1.0, 60.0)
+    rows = np.stack([rect.xcol, rect.ycol, rect.zcol, rect.ang, rect.angd, rect.dsl, rect.dsd], axis=1)
+    np.savetxt(tmp_path / "rect.dat", rows, fmt="%.17e")
+    _run(tmp_path, {"problem": "3dhr", "ncellg": rect.n, "geometry_file": "rect.dat", "forward": True,
+                    "eps_h": 1e-6, "leaf": 32, "filenumber": 1})
+    _run(tmp_path, {"problem": "3dph", "imax": 24, "jmax": 12, "ds": 1.0, "dipangle": 60.0, "forward": True,
+                    "eps_h": 1e-6, "leaf": 32, "filenumber": 2})
+    a = np.loadtxt(tmp_path / "output" / "stress1")
+    b = np.loadtxt(tmp_path / "output" / "stress2")
+    assert np.array_equal(a, b)

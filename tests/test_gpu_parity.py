@@ -54,12 +54,23 @@ def test_gpu_partition_identical_to_oracle(make, leaf):
     assert np.array_equal(Hg.leaves[:, :5], Ho.leaves[:, :5])
 
 
+def _curve(n, length, amp, y0):
+    x = np.linspace(0.0, length, n + 1)
+    y = y0 + amp * np.sin(6 * np.pi * x / length) + 0.3 * amp * np.sin(22 * np.pi * x / length)
+    return x[:-1], x[1:], y[:-1], y[1:]
+
+
 CASES = [
     ("2dvs", lambda: P.mesh_2dvs(2048, 40 / 2048), 32, 1e-6, 1e-9),
     ("3dph", lambda: P.mesh_3dph(48, 24, 1.0), 32, 1e-5, 1e-6),
     ("3dnt", lambda: P.mesh_triangles(P.rough_surface_triangles(8.0, 4.0, 0.25, seed=9)), 32, 1e-4, 1e-4),
     ("3dht", lambda: P.mesh_triangles(P.rough_surface_triangles(8.0, 4.0, 0.25, seed=9), halfspace=True), 32, 1e-4,
      1e-4),
+    ("2dp", lambda: P.mesh_2dp(1000, 0.1), 32, 1e-6, 1e-6),
+    ("3dp", lambda: P.mesh_3dp(48, 24, 0.5), 32, 1e-5, 1e-5),
+    ("2dph", lambda: P.mesh_2dph(1600, 0.025, 30.0), 32, 1e-6, 1e-6),
+    ("2dn", lambda: P.mesh_segments(*_curve(800, 30.0, 0.5, 0.0)), 32, 1e-6, 1e-6),
+    ("25d", lambda: P.mesh_segments(*_curve(800, 30.0, 0.5, 0.0), kind=P.KIND_25D, w=10.0), 32, 1e-6, 1e-6),
 ]
 
 
