@@ -395,6 +395,7 @@ struct hbi_ens {
     int shared = 0;   // arrays kept once per element (bit k: EnsArrays order)
     hbi::DevBuf<double> a, b, dc, f0, vpl, taudot, sigdot;
     hbi::DevBuf<double> y, ytmp, yout, k, vel, x, slip, velc, errpart, monpart, z, zpart, ypart;
+    hbi::DevBuf<float> x32;      // FP32 copy of x for the FP32-core product
     hbi::DevBuf<double> fold;   // first-level reduction partials (kFold x 2 x 64)
     hbi::DevBuf<double> hist;   // nreal-major rows of kHistCols (graph argument)
     long long hist_rows = 0;
@@ -434,7 +435,7 @@ static void e_attempt(hbi_ens *E)
     const int *gate = &E->ctl.p->done;
     ens_stage0_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
     for (int s = 0; s < 6; s++) {
-        mr_apply_parts(E->hs, E->opts.prec, E->x.p, E->zpart.p, E->z.p, E->ypart.p, E->stream, gate);
+        mr_apply_parts(E->hs, E->opts.prec, E->x.p, E->x32.p, E->zpart.p, E->z.p, E->ypart.p, E->stream, gate);
         ens_stage_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A, s);
     }
     ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->errpart.p, g, E->fold.p, 0);
@@ -528,6 +529,7 @@ int hbi_ens_create(hbi_hmat *hs, const hbi_rk_opts *opts, int nreal, const doubl
             P.ratio_hi = 8.0;
             E->z.alloc((size_t)std::max(hs->nz, 1) * hbi::kR);
             E->zpart.alloc(hbi::mr_zpart_size(hs, opts->prec));
+            E->x32.alloc(hbi::mr_x32_size(hs, opts->prec));
             E->ypart.alloc(hbi::mr_ypart_size(hs, opts->prec));
             for (auto *d : {&E->y, &E->ytmp, &E->yout, &E->k, &E->vel, &E->x, &E->slip, &E->velc})
                 HBI_HIP(hipMemsetAsync(d->p, 0, d->n * 8, E->stream));

@@ -120,12 +120,15 @@ void hmat_phase2(hbi_hmat *h, int prec, const double *z, const double *ypart, do
                  const int *gate);
 const Plan &hmat_plan(hbi_hmat *h, int prec);
 // multi-RHS product (RR = 64 columns, row-major N x 64)
-void mr_apply_parts(hbi_hmat *h, int prec, const double *X, double *zpart, double *Z, double *ypart, hipStream_t s,
-                    const int *gate);
+void mr_apply_parts(hbi_hmat *h, int prec, const double *X, float *X32, double *zpart, double *Z, double *ypart,
+                    hipStream_t s, const int *gate);
 void mr_reduce(hbi_hmat *h, int prec, const double *ypart, double *Y, hipStream_t s);
-void mr_phase1(hbi_hmat *h, int prec, const double *X, double *zpart, double *Z, hipStream_t s, const int *gate);
-void mr_phase2(hbi_hmat *h, int prec, const double *X, const double *Z, double *ypart, hipStream_t s, const int *gate);
+void mr_phase1(hbi_hmat *h, int prec, const double *X, float *X32, double *zpart, double *Z, hipStream_t s,
+               const int *gate);
+void mr_phase2(hbi_hmat *h, int prec, const double *X, const float *X32, const double *Z, double *ypart,
+               hipStream_t s, const int *gate);
 size_t mr_zpart_size(hbi_hmat *h, int prec);
+size_t mr_x32_size(hbi_hmat *h, int prec);
 size_t mr_ypart_size(hbi_hmat *h, int prec);
 const int *mr_part0(hbi_hmat *h, int prec);
 void mr_forget(const hbi_hmat *h);

@@ -7,12 +7,18 @@
 // unused columns zero).
 //
 //   phase 1  Zpart(item) = a1^T[:, t-range] X[t-range, :] for column chunks of
-//            every low-rank block (FP64 matrix cores); zreduce adds the
-//            chunks of a block in order -> Z (sum_kt x RR)
+//            every low-rank block; zreduce adds the chunks of a block in
+//            order -> Z (sum_kt x RR)
 //   phase 2  per tile item: Ypart = sum over segments of A_seg (TR x ncol)
-//            X_seg (ncol x RR) on the FP64 matrix cores, 16-deep k-slices
-//            staged through LDS
+//            X_seg (ncol x RR), k-slices staged through LDS
 // Both phases reuse the packed single-vector layout of the precision's plan.
+// FP64 factors multiply on the FP64 matrix cores.  FP32 and BF16 factors
+// multiply on the FP32 matrix cores (twice the FP64 rate on this chip): X is
+// rounded once to an FP32 copy and Z is stored in FP32 (halving the bytes the
+// k-slices load), products accumulate in FP32 for at most one k-slice (24 or
+// 32 columns) and every slice's sums are added into FP64 accumulators, so the
+// accumulation error stays at a few FP32 roundings whatever the block width;
+// the phase-1 partials, the phase-2 partials and Y stay FP64.
 #include "hmat.hpp"
 
 #include <algorithm>
@@ -38,6 +44,7 @@ struct MItem1 {
     int part;          // partial slot zpart[part][kP1Rows][RR], or -1: the item is the rows' whole
                        // column range and writes Z rows zrow[zr .. zr + kt) directly
     int dtype, zr;
+    int r0;            // first row of the item within its partial slot (FP32-core items: 0 or 32)
 };
 struct MBlk1 {         // rows roff .. roff+kt of partial slots [p0, p1) -> z rows zoff .. zoff+kt
     int p0, p1, kt, zoff, roff;
@@ -64,12 +71,18 @@ template <> struct Run2<float> {
     fvec2 v;
     __device__ void load(const float *p) { v = __builtin_nontemporal_load((const fvec2 *)p); }
     __device__ double get(int e) const { return (double)v[e]; }
+    __device__ float getf(int e) const { return v[e]; }
 };
 template <> struct Run2<uint16_t> {
     uint32_t v;
     __device__ void load(const uint16_t *p) { v = __builtin_nontemporal_load((const uint32_t *)p); }
     __device__ double get(int e) const { return (double)__uint_as_float(e ? (v & 0xffff0000u) : (v << 16)); }
+    __device__ float getf(int e) const { return __uint_as_float(e ? (v & 0xffff0000u) : (v << 16)); }
 };
+
+template <typename T> __device__ __forceinline__ float narrow(T v);
+template <> __device__ __forceinline__ float narrow<float>(float v) { return v; }
+template <> __device__ __forceinline__ float narrow<uint16_t>(uint16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
 
 // One wave per item: Zpart (kt x 64) = a1^T[rows, t0 .. t0+tn) X[t0 .. t0+tn, :]
 // on the FP64 matrix cores (16x16x4 MFMA; lane l = li + 16 lk supplies
@@ -161,6 +174,120 @@ __device__ __forceinline__ void mr_p1_wave(const MItem1 &it, const uint8_t *__re
         }
 }
 
+// The same item on the FP32 matrix cores (16x16x4 FP32 MFMA: operands as in
+// the FP64 form, result element q of lane l is D[4 (l / 16) + q][l % 16]).
+// Items have at most 32 rows (NRB <= 2) so the FP64 sums fit beside the FP32
+// accumulators at two or more waves per SIMD; the FP32 sums of every 24
+// columns (one pass of the three buffers) are added into the FP64 sums.
+template <typename T, int NRB>
+__device__ __forceinline__ void mr_p1_wave32(const MItem1 &it, const uint8_t *__restrict__ buf,
+                                             const float *__restrict__ X, double *__restrict__ zpart,
+                                             const int *__restrict__ zrow, float *__restrict__ Z)
+{
+    const int lane = threadIdx.x % 64, li = lane % 16, lk = lane / 16;
+    const T *a = (const T *)(buf + it.off);
+    const int tend = it.t0 + it.tn;
+    struct Buf {
+        Run2<T> a[NRB];
+        fvec4 x[2];
+    };
+    Buf b0, b1, b2;
+    auto load = [&](Buf &B, int k) {
+        const int tb = it.t0 + 8 * k + 2 * lk;
+        const bool ok = tb < tend;
+#pragma unroll
+        for (int rb = 0; rb < NRB; rb++) {
+            const int row = 16 * rb + li;
+            B.a[rb].load(ok && row < it.kt ? a + (size_t)row * it.ldt + tb : (const T *)g_zero_row);
+        }
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+            B.x[s] = tb + s < tend ? *(const fvec4 *)(X + (size_t)(it.xoff + tb + s) * kRR + 4 * li)
+                                   : *(const fvec4 *)g_zero_row;
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    fvec4 acc[NRB][4];
+    v4d sum[NRB][4];
+#pragma unroll
+    for (int rb = 0; rb < NRB; rb++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            acc[rb][j] = fvec4{0.f, 0.f, 0.f, 0.f};
+            sum[rb][j] = v4d{0.0, 0.0, 0.0, 0.0};
+        }
+    auto mul = [&](const Buf &B) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+#pragma unroll
+            for (int rb = 0; rb < NRB; rb++) {
+                const float av = B.a[rb].getf(s);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[rb][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, B.x[s][j], acc[rb][j], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    auto flush = [&] {
+#pragma unroll
+        for (int rb = 0; rb < NRB; rb++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) sum[rb][j][q] += (double)acc[rb][j][q];
+                acc[rb][j] = fvec4{0.f, 0.f, 0.f, 0.f};
+            }
+    };
+    const int nk = (it.tn + 7) / 8;
+    load(b0, 0);
+    load(b1, 1);
+    int k = 0;
+    for (; k + 3 <= nk; k += 3) {
+        load(b2, k + 2);
+        mul(b0);
+        load(b0, k + 3);
+        mul(b1);
+        load(b1, k + 4);
+        mul(b2);
+        flush();
+    }
+    if (k < nk) mul(b0);
+    if (k + 1 < nk) mul(b1);
+    flush();
+#pragma unroll
+    for (int rb = 0; rb < NRB; rb++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int row = 16 * rb + 4 * lk + q;
+            if (row < it.kt) {
+                if (it.part < 0) {
+                    *(fvec4 *)(Z + (size_t)zrow[it.zr + row] * kRR + 4 * li) =
+                        fvec4{(float)sum[rb][0][q], (float)sum[rb][1][q], (float)sum[rb][2][q], (float)sum[rb][3][q]};
+                } else {
+                    dvec2 *dst = (dvec2 *)(zpart + ((size_t)it.part * kP1Rows + it.r0 + row) * kRR + 4 * li);
+                    dst[0] = dvec2{sum[rb][0][q], sum[rb][1][q]};
+                    dst[1] = dvec2{sum[rb][2][q], sum[rb][3][q]};
+                }
+            }
+        }
+}
+
+__global__ __launch_bounds__(64) void mr_lrvt32_kernel(const MItem1 *__restrict__ items,
+                                                       const uint8_t *__restrict__ buf, const float *__restrict__ X,
+                                                       double *__restrict__ zpart, const int *__restrict__ zrow,
+                                                       float *__restrict__ Z, const int *__restrict__ gate)
+{
+    if (gate && *gate) return;
+    const MItem1 it = items[blockIdx.x];
+    if (it.dtype == kF32) {
+        if (it.kt > 16) mr_p1_wave32<float, 2>(it, buf, X, zpart, zrow, Z);
+        else mr_p1_wave32<float, 1>(it, buf, X, zpart, zrow, Z);
+    } else {
+        if (it.kt > 16) mr_p1_wave32<uint16_t, 2>(it, buf, X, zpart, zrow, Z);
+        else mr_p1_wave32<uint16_t, 1>(it, buf, X, zpart, zrow, Z);
+    }
+}
+
 __global__ __launch_bounds__(64) void mr_lrvt_kernel(const MItem1 *__restrict__ items,
                                                      const uint8_t *__restrict__ buf, const double *__restrict__ X,
                                                      double *__restrict__ zpart, const int *__restrict__ zrow,
@@ -182,8 +309,20 @@ __global__ __launch_bounds__(64) void mr_lrvt_kernel(const MItem1 *__restrict__ 
 #undef HBI_P1
 }
 
+// X rounded to the FP32 copy the FP32-core kernels read (4 values a thread)
+__global__ void mr_narrow_kernel(size_t n4, const double *__restrict__ X, float *__restrict__ X32,
+                                 const int *__restrict__ gate)
+{
+    if (gate && *gate) return;
+    const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (q >= n4) return;
+    const dvec2 a = ((const dvec2 *)X)[2 * q], b = ((const dvec2 *)X)[2 * q + 1];
+    ((fvec4 *)X32)[q] = fvec4{(float)a[0], (float)a[1], (float)b[0], (float)b[1]};
+}
+
+template <typename OT>
 __global__ void mr_zreduce_kernel(const MBlk1 *__restrict__ blks, int nblk, const double *__restrict__ zpart,
-                                  double *__restrict__ Z, const int *__restrict__ gate)
+                                  OT *__restrict__ Z, const int *__restrict__ gate)
 {
     if (gate && *gate) return;
     const int b = blockIdx.x;
@@ -193,7 +332,7 @@ __global__ void mr_zreduce_kernel(const MBlk1 *__restrict__ blks, int nblk, cons
         const int k = q / kRR, rr = q % kRR;
         double s = 0.0;
         for (int p = B.p0; p < B.p1; p++) s += zpart[((size_t)p * kP1Rows + B.roff + k) * kRR + rr];
-        Z[(size_t)(B.zoff + k) * kRR + rr] = s;
+        Z[(size_t)(B.zoff + k) * kRR + rr] = (OT)s;
     }
 }
 
@@ -298,6 +437,112 @@ __global__ __launch_bounds__(kMrThreads) void mr_tile_kernel(const Item2 *__rest
                 o[(size_t)(RB * 16 * wr + 16 * ib + lk + 4 * q) * kRR + 32 * wc + 16 * jb + li] = acc[ib][jb][q];
 }
 
+// The same product on the FP32 matrix cores for FP32 / BF16 factors: 32-deep
+// k-slices of FP32 values in LDS (X and Z rounded at the LDS store); the FP32
+// sums of every slice are added into FP64 sums.
+template <typename T, int TR>
+__global__ __launch_bounds__(kMrThreads) void mr_tile32_kernel(const Item2 *__restrict__ items,
+                                                               const Seg *__restrict__ segs,
+                                                               const uint8_t *__restrict__ buf,
+                                                               const float *__restrict__ X,
+                                                               const float *__restrict__ Z,
+                                                               double *__restrict__ ypart, const int *__restrict__ gate)
+{
+    constexpr int KS = 32;
+    constexpr int RB = TR / 32;
+    constexpr int AP = TR + 16, XP = kRR + 16;
+    constexpr int NA = KS * TR / kMrThreads, NX = KS * kRR / kMrThreads;
+    __shared__ float as[KS][AP];
+    __shared__ float xs[KS][XP];
+    if (gate && *gate) return;
+    const Item2 it = items[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+    const int wr = w / 2, wc = w % 2;
+    const int li = lane % 16, lk = lane / 16;
+    T ra[NA];
+    float rx[NX];
+    auto load = [&](const Seg &g, int c) {
+        const T *a = (const T *)(buf + g.off);
+        const float *xin = (g.src ? Z : X) + (size_t)g.xoff * kRR;
+        const int kn = min(KS, g.ncol - c);
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int q = tid + u * kMrThreads;
+            ra[u] = q / TR < kn ? a[(size_t)c * TR + q] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < NX; u++) {
+            const int q = tid + u * kMrThreads;
+            rx[u] = q / kRR < kn ? xin[(size_t)c * kRR + q] : 0.f;
+        }
+    };
+    fvec4 acc[RB][2];
+    v4d sum[RB][2];
+#pragma unroll
+    for (int ib = 0; ib < RB; ib++)
+#pragma unroll
+        for (int jb = 0; jb < 2; jb++) {
+            acc[ib][jb] = fvec4{0.f, 0.f, 0.f, 0.f};
+            sum[ib][jb] = v4d{0.0, 0.0, 0.0, 0.0};
+        }
+    int s = it.seg0, c0 = 0;
+    Seg sg{};
+    if (s < it.seg1) {
+        sg = segs[s];
+        load(sg, 0);
+    }
+    while (s < it.seg1) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int q = tid + u * kMrThreads;
+            as[q / TR][q % TR] = narrow<T>(ra[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < NX; u++) {
+            const int q = tid + u * kMrThreads;
+            xs[q / kRR][q % kRR] = rx[u];
+        }
+        __syncthreads();
+        c0 += KS;
+        if (c0 >= sg.ncol) {
+            s++;
+            c0 = 0;
+            if (s < it.seg1) sg = segs[s];
+        }
+        if (s < it.seg1) load(sg, c0);
+#pragma unroll
+        for (int kk = 0; kk < KS; kk += 4) {
+            float bf[2];
+#pragma unroll
+            for (int jb = 0; jb < 2; jb++) bf[jb] = xs[kk + lk][32 * wc + 16 * jb + li];
+#pragma unroll
+            for (int ib = 0; ib < RB; ib++) {
+                const float af = as[kk + lk][RB * 16 * wr + 16 * ib + li];
+#pragma unroll
+                for (int jb = 0; jb < 2; jb++)
+                    acc[ib][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af, bf[jb], acc[ib][jb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int ib = 0; ib < RB; ib++)
+#pragma unroll
+            for (int jb = 0; jb < 2; jb++) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) sum[ib][jb][q] += (double)acc[ib][jb][q];
+                acc[ib][jb] = fvec4{0.f, 0.f, 0.f, 0.f};
+            }
+    }
+    double *o = ypart + (size_t)it.part * TR * kRR;
+#pragma unroll
+    for (int ib = 0; ib < RB; ib++)
+#pragma unroll
+        for (int jb = 0; jb < 2; jb++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                o[(size_t)(RB * 16 * wr + 16 * ib + 4 * lk + q) * kRR + 32 * wc + 16 * jb + li] = sum[ib][jb][q];
+}
+
 __global__ void mr_tile_reduce_kernel(int n, int tr, const int *__restrict__ part0, const double *__restrict__ ypart,
                                       double *__restrict__ Y)
 {
@@ -314,6 +559,7 @@ __global__ void mr_tile_reduce_kernel(int n, int tr, const int *__restrict__ par
 struct MrPlan {
     bool built = false;
     int kc1 = 512, cols2 = 256;    // phase-1 column chunk, phase-2 columns per item (autotuned)
+    bool f32 = false;              // FP32 / BF16 factors only: the FP32-core kernels, X32 and Z in FP32
     int r2[3][2] = {};             // phase-2 item range per dtype
     std::vector<MItem1> it1;
     std::vector<MBlk1> blk1;
@@ -387,7 +633,20 @@ static void mr_items1(const Plan &P, MrPlan &M, int kc1)
                 MItem1 it{};
                 it.off = grp[g0].off; it.ldt = f.ldt; it.kt = kt; it.t0 = t0; it.tn = std::min(kc1, f.nc - t0);
                 it.xoff = f.xoff; it.part = direct ? -1 : M.nzpart++; it.dtype = f.dtype; it.zr = grp[g0].sr;
-                M.it1.push_back(it);
+                it.r0 = 0;
+                if (f.dtype == kF64 || kt <= 32) {
+                    M.it1.push_back(it);
+                } else {
+                    // FP32-core items: rows 0..31 and 32..kt of the slot
+                    MItem1 lo = it, hi = it;
+                    lo.kt = 32;
+                    hi.kt = kt - 32;
+                    hi.r0 = 32;
+                    hi.zr = it.zr + 32;
+                    hi.off = it.off + (uint64_t)32 * f.ldt * dtype_size(f.dtype);
+                    M.it1.push_back(lo);
+                    M.it1.push_back(hi);
+                }
             }
             if (!direct) {
                 int roff = 0;
@@ -443,6 +702,17 @@ static void mr_items2(const hbi_hmat *h, const Plan &P, MrPlan &M, int cols2)
     M.cols2 = cols2;
 }
 
+// FP32-core mode: a plan holds either FP64 factors only or none (the
+// precision policies never mix them), so Z has one type
+static void mr_mode(const Plan &P, MrPlan &M)
+{
+    bool has64 = false, has32 = false;
+    for (const Seg &g : P.segs) (g.dtype == kF64 ? has64 : has32) = true;
+    for (const Row1 &r : P.rows1) (r.dtype == kF64 ? has64 : has32) = true;
+    HBI_REQUIRE(!(has64 && has32), "64-column product: FP64 and reduced-precision factors in one plan");
+    M.f32 = has32;
+}
+
 static void mr_upload(hbi_hmat *h, MrPlan &M)
 {
     M.d_it1.upload(M.it1, h->stream);
@@ -461,6 +731,7 @@ const MrPlan &mr_plan(hbi_hmat *h, int prec)
     MrCache *c = new MrCache{h, prec, MrPlan()};
     try {
         MrPlan &M = c->P;
+        mr_mode(P, M);
         mr_items1(P, M, 512);
         mr_items2(h, P, M, 256);
         mr_upload(h, M);
@@ -490,56 +761,74 @@ void mr_forget(const hbi_hmat *h)
 static size_t zpart_cap(const MrPlan &M) { return (size_t)std::max(M.nzpart, 1) * kP1Rows * kRR; }
 static size_t ypart_cap(const hbi_hmat *h, const MrPlan &M) { return (size_t)std::max(M.nparts, 1) * h->tr * kRR; }
 size_t mr_zpart_size(hbi_hmat *h, int prec) { return zpart_cap(mr_plan(h, prec)); }
+size_t mr_x32_size(hbi_hmat *h, int prec) { return mr_plan(h, prec).f32 ? (size_t)h->n * kRR : 1; }
 size_t mr_ypart_size(hbi_hmat *h, int prec) { return ypart_cap(h, mr_plan(h, prec)); }
 const int *mr_part0(hbi_hmat *h, int prec) { return mr_plan(h, prec).d_part0.p; }
 
-// Z = a1^T X for every low-rank block (partials, then the ordered reduction)
-static void phase1(const Plan &P, const MrPlan &M, const double *X, double *zpart, double *Z, hipStream_t s,
-                   const int *gate)
+// Z = a1^T X for every low-rank block (partials, then the ordered reduction);
+// in FP32-core mode X is first rounded into X32 (which phase 2 reads too) and
+// Z is stored as FP32 in the first half of its buffer
+static void phase1(hbi_hmat *h, const Plan &P, const MrPlan &M, const double *X, float *X32, double *zpart,
+                   double *Z, hipStream_t s, const int *gate)
 {
-    if (!M.it1.empty()) {
+    if (M.f32) {
+        const size_t n4 = (size_t)h->n * kRR / 4;
+        mr_narrow_kernel<<<(int)((n4 + 255) / 256), 256, 0, s>>>(n4, X, X32, gate);
+    }
+    if (M.it1.empty()) return;
+    if (!M.f32) {
         mr_lrvt_kernel<<<(int)M.it1.size(), 64, 0, s>>>(M.d_it1.p, P.buf.p, X, zpart, P.d_zrow.p, Z, gate);
         if (!M.blk1.empty())
-            mr_zreduce_kernel<<<(int)M.blk1.size(), 256, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(), zpart, Z, gate);
+            mr_zreduce_kernel<double><<<(int)M.blk1.size(), 256, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(), zpart, Z, gate);
+    } else {
+        float *Z32 = (float *)Z;
+        mr_lrvt32_kernel<<<(int)M.it1.size(), 64, 0, s>>>(M.d_it1.p, P.buf.p, X32, zpart, P.d_zrow.p, Z32, gate);
+        if (!M.blk1.empty())
+            mr_zreduce_kernel<float><<<(int)M.blk1.size(), 256, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(), zpart, Z32,
+                                                                        gate);
     }
 }
 
 // Y partials (per tile item) of the dense and a2 Z products
-static void phase2(const hbi_hmat *h, const Plan &P, const MrPlan &M, const double *X, const double *Z, double *ypart,
-                   hipStream_t s, const int *gate)
+static void phase2(const hbi_hmat *h, const Plan &P, const MrPlan &M, const double *X, const float *X32,
+                   const double *Z, double *ypart, hipStream_t s, const int *gate)
 {
+    const float *Z32 = (const float *)Z;
     for (int d = 0; d < 3; d++) {
         const int n = M.r2[d][1] - M.r2[d][0];
         if (n <= 0) continue;
         const Item2 *it = M.d_it2.p + M.r2[d][0];
         if (h->tr == 64) {
             if (d == kF64) mr_tile_kernel<double, 64><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X, Z, ypart, gate);
-            else if (d == kF32) mr_tile_kernel<float, 64><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X, Z, ypart, gate);
-            else mr_tile_kernel<uint16_t, 64><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X, Z, ypart, gate);
+            else if (d == kF32) mr_tile32_kernel<float, 64><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
+            else mr_tile32_kernel<uint16_t, 64><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
         } else {
             if (d == kF64) mr_tile_kernel<double, 32><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X, Z, ypart, gate);
-            else if (d == kF32) mr_tile_kernel<float, 32><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X, Z, ypart, gate);
-            else mr_tile_kernel<uint16_t, 32><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X, Z, ypart, gate);
+            else if (d == kF32) mr_tile32_kernel<float, 32><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
+            else mr_tile32_kernel<uint16_t, 32><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
         }
     }
 }
 
-void mr_phase1(hbi_hmat *h, int prec, const double *X, double *zpart, double *Z, hipStream_t s, const int *gate)
+void mr_phase1(hbi_hmat *h, int prec, const double *X, float *X32, double *zpart, double *Z, hipStream_t s,
+               const int *gate)
 {
-    phase1(hmat_plan(h, prec), mr_plan(h, prec), X, zpart, Z, s, gate);
+    phase1(h, hmat_plan(h, prec), mr_plan(h, prec), X, X32, zpart, Z, s, gate);
 }
 
-void mr_phase2(hbi_hmat *h, int prec, const double *X, const double *Z, double *ypart, hipStream_t s, const int *gate)
+void mr_phase2(hbi_hmat *h, int prec, const double *X, const float *X32, const double *Z, double *ypart,
+               hipStream_t s, const int *gate)
 {
-    phase2(h, hmat_plan(h, prec), mr_plan(h, prec), X, Z, ypart, s, gate);
+    phase2(h, hmat_plan(h, prec), mr_plan(h, prec), X, X32, Z, ypart, s, gate);
 }
 
-// Y partials of H X (device pointers, row-major N x 64); Y itself through mr_reduce or a consumer
-void mr_apply_parts(hbi_hmat *h, int prec, const double *X, double *zpart, double *Z, double *ypart, hipStream_t s,
-                    const int *gate)
+// Y partials of H X (device pointers, row-major N x 64); Y itself through mr_reduce or a consumer.
+// X32 (mr_x32_size floats) is scratch for the FP32 copy of X.
+void mr_apply_parts(hbi_hmat *h, int prec, const double *X, float *X32, double *zpart, double *Z, double *ypart,
+                    hipStream_t s, const int *gate)
 {
-    mr_phase1(h, prec, X, zpart, Z, s, gate);
-    mr_phase2(h, prec, X, Z, ypart, s, gate);
+    mr_phase1(h, prec, X, X32, zpart, Z, s, gate);
+    mr_phase2(h, prec, X, X32, Z, ypart, s, gate);
 }
 
 void mr_reduce(hbi_hmat *h, int prec, const double *ypart, double *Y, hipStream_t s)
@@ -566,6 +855,7 @@ static void mr_tune(hbi_hmat *h, int prec, MrPlan &M)
     mr_items2(h, P, M, c2s[0]);
     DevBuf<double> dX((size_t)h->n * kRR), dZ((size_t)std::max(h->nz, 1) * kRR);
     DevBuf<double> zp(zpart_cap(M)), yp(ypart_cap(h, M)), dY((size_t)h->n * kRR);
+    DevBuf<float> x32(M.f32 ? (size_t)h->n * kRR : 1);
     HBI_HIP(hipMemsetAsync(dX.p, 0, dX.n * 8, h->stream));
     HBI_HIP(hipMemsetAsync(dZ.p, 0, dZ.n * 8, h->stream));
     auto time_it = [&](int which) {
@@ -574,9 +864,9 @@ static void mr_tune(hbi_hmat *h, int prec, MrPlan &M)
         // consumer pays (fewer, larger items: fewer slots to read)
         auto run = [&] {
             if (which == 1) {
-                phase1(P, M, dX.p, zp.p, dZ.p, h->stream, nullptr);
+                phase1(h, P, M, dX.p, x32.p, zp.p, dZ.p, h->stream, nullptr);
             } else {
-                phase2(h, P, M, dX.p, dZ.p, yp.p, h->stream, nullptr);
+                phase2(h, P, M, dX.p, x32.p, dZ.p, yp.p, h->stream, nullptr);
                 const size_t tot = (size_t)h->n * kRR;
                 mr_tile_reduce_kernel<<<(int)((tot + 255) / 256), 256, 0, h->stream>>>(h->n, h->tr, M.d_part0.p,
                                                                                         yp.p, dY.p);
@@ -620,6 +910,7 @@ extern "C" int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs
         (void)P;
         hbi::DevBuf<double> dX((size_t)n * hbi::kRR), dY((size_t)n * hbi::kRR), dZ((size_t)(std::max(h->nz, 1)) * hbi::kRR);
         hbi::DevBuf<double> zp(hbi::mr_zpart_size(h, prec)), yp(hbi::mr_ypart_size(h, prec));
+        hbi::DevBuf<float> x32(hbi::mr_x32_size(h, prec));
         std::vector<double> hx((size_t)n * hbi::kRR), hy((size_t)n * hbi::kRR);
         for (int c0 = 0; c0 < nrhs; c0 += hbi::kRR) {
             const int nc = std::min(hbi::kRR, nrhs - c0);
@@ -627,7 +918,7 @@ extern "C" int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs
             for (int r = 0; r < nc; r++)
                 for (int i = 0; i < n; i++) hx[(size_t)i * hbi::kRR + r] = X[(size_t)(c0 + r) * n + i];
             dX.upload(hx, h->stream);
-            hbi::mr_apply_parts(h, prec, dX.p, zp.p, dZ.p, yp.p, h->stream, nullptr);
+            hbi::mr_apply_parts(h, prec, dX.p, x32.p, zp.p, dZ.p, yp.p, h->stream, nullptr);
             hbi::mr_reduce(h, prec, yp.p, dY.p, h->stream);
             HBI_HIP(hipGetLastError());
             dY.download(hy.data(), hy.size(), h->stream);
@@ -646,16 +937,17 @@ extern "C" int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms
         const hbi::Plan &P = hbi::hmat_plan(h, prec);
         hbi::DevBuf<double> dX((size_t)n * hbi::kRR), dY((size_t)n * hbi::kRR), dZ((size_t)(std::max(h->nz, 1)) * hbi::kRR);
         hbi::DevBuf<double> zp(hbi::mr_zpart_size(h, prec)), yp(hbi::mr_ypart_size(h, prec));
+        hbi::DevBuf<float> x32(hbi::mr_x32_size(h, prec));
         HBI_HIP(hipMemsetAsync(dX.p, 0, dX.n * 8, h->stream));
         HBI_HIP(hipMemsetAsync(dZ.p, 0, dZ.n * 8, h->stream));
-        for (int w = 0; w < 2; w++) hbi::mr_apply_parts(h, prec, dX.p, zp.p, dZ.p, yp.p, h->stream, nullptr);
+        for (int w = 0; w < 2; w++) hbi::mr_apply_parts(h, prec, dX.p, x32.p, zp.p, dZ.p, yp.p, h->stream, nullptr);
         auto timed = [&](int which) {
             HBI_HIP(hipEventRecord(h->ev[0], h->stream));
             for (int it = 0; it < iters; it++) {
-                if (which == 0) hbi::mr_phase1(h, prec, dX.p, zp.p, dZ.p, h->stream, nullptr);
-                else if (which == 1) hbi::mr_phase2(h, prec, dX.p, dZ.p, yp.p, h->stream, nullptr);
+                if (which == 0) hbi::mr_phase1(h, prec, dX.p, x32.p, zp.p, dZ.p, h->stream, nullptr);
+                else if (which == 1) hbi::mr_phase2(h, prec, dX.p, x32.p, dZ.p, yp.p, h->stream, nullptr);
                 else {
-                    hbi::mr_apply_parts(h, prec, dX.p, zp.p, dZ.p, yp.p, h->stream, nullptr);
+                    hbi::mr_apply_parts(h, prec, dX.p, x32.p, zp.p, dZ.p, yp.p, h->stream, nullptr);
                     hbi::mr_reduce(h, prec, yp.p, dY.p, h->stream);
                 }
             }

@@ -1,7 +1,10 @@
 """GPU: the 64-column H product and the lockstep ensemble integrator.
 
 * hbi_hmat_matmul against hbi_hmat_matvec column by column, same factors:
-  both accumulate in FP64 in different orders -> |dY| <= 1e-12 max|Y|
+  FP64 factors accumulate in FP64 in both, in different orders -> |dY| <=
+  1e-12 max|Y|; FP32 / BF16 factors run on the FP32 matrix cores (X, Z
+  rounded to FP32, FP32 sums of at most 32 columns added in FP64) ->
+  |dY_i| <= 64 u32 sum_j |A_ij| |x_j| with A the dense kernel matrix
 * ensemble realisation r against a single-run integration of the same
   friction parameters and initial state: the products differ in summation
   order only, so the step grids coincide over the first steps (states to
@@ -28,12 +31,22 @@ def test_gpu_matmul_matches_matvec_columns(label, make, leaf, eps):
     print(label, g.n, "kt_max", st["kt_max"], flush=True)
     rng = np.random.default_rng(3)
     X = rng.standard_normal((g.n, 70))      # one full 64-column pass and a partial one
+    lod = Hg.lod
+    A = np.abs(H.entries(g, np.repeat(lod, g.n).astype(np.int32), np.tile(lod, g.n).astype(np.int32))).reshape(g.n, g.n)
+    u32 = 2.0 ** -24
+    worst = {}
     for prec in (H.PREC_F64, H.PREC_F32, H.PREC_MIXED):
         Y = Hg.matmul(X, prec)
-        tol = 1e-12
         for c in range(X.shape[1]):
             y = Hg.matvec(X[:, c], prec)
-            assert np.abs(Y[:, c] - y).max() <= tol * np.abs(y).max(), (prec, c)
+            d = np.abs(Y[:, c] - y)
+            if prec == H.PREC_F64:
+                assert d.max() <= 1e-12 * np.abs(y).max(), (prec, c)
+            else:
+                bound = 64 * u32 * (A @ np.abs(X[:, c]))
+                assert (d <= bound).all(), (prec, c, (d / bound).max())
+                worst[prec] = max(worst.get(prec, 0.0), (d / bound).max())
+        print("  max |dY| / bound", worst, flush=True)
     if label == "3dph-highrank":
         assert st["kt_max"] > 64      # exercises the split of ranks above 64
 
@@ -74,8 +87,10 @@ def test_gpu_ensemble_matches_single_runs(nreal, prec):
     and the oracle part there too), so step grids are compared only before;
     after that, the times at which log10 Vmax crosses levels -7.5 .. -1
     (interpolated within each history, so independent of the other grid)
-    agree to 2e-4 as a guard against gross errors (the coarse early steps,
-    ~2e5 s, limit the interpolation to ~5e-5)."""
+    agree to 1e-3 as a guard against gross errors (the coarse early steps,
+    ~2e5 s, move the interpolated crossings of the flat early levels by up to
+    2e-4 already between two FP64 integrations; FP32 matrix cores in the
+    ensemble add ~2e-4)."""
     g, frs = _c1_realisations(1024, nreal)
     Hg = H.HMatrix(g, leaf=32, eps=1e-6)
     lod = Hg.lod
@@ -123,7 +138,7 @@ def test_gpu_ensemble_matches_single_runs(nreal, prec):
               % (r, on_e, on_s, len(both), rel), flush=True)
         if on_s is not None:
             assert abs(on_e - on_s) <= 1e-5 * on_s
-        assert len(both) >= 3 and rel <= 2e-4
+        assert len(both) >= 3 and rel <= 1e-3
 
 
 def test_gpu_ensemble_stop_is_per_realisation():
