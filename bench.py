@@ -38,11 +38,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
+FP32_MFMA_PEAK_TFS = 157.3
 # HBM bytes per launch of the two roofline kernels (mixed precision), from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE passes over
 # hbi_amd/profile_kernels.py, scaled by the counters of a 256 MiB device copy
-# (fetch x 2.00, write x 1.00)
-TRAFFIC_MR_TILE_MIXED = 623.5e6
+# (fetch x 2.00, write x 1.00): mr_tile32_kernel<float,64,4> and stream_kernel
+TRAFFIC_MR_TILE_MIXED = 436.3e6
 TRAFFIC_STREAM_MIXED = 225.7e6
 IMAX, JMAX, DS0, LEAF, EPS_H, EPS_R = 256, 128, 0.390625, 64, 1e-5, 1e-6
 NREAL = 64
@@ -93,8 +94,10 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
                     "p2_TFs": p2_entries * 64 * 2 / (t["p2_ms"] * 1e-3) / 1e12,
                     "per_rhs_vs_matvec": mv[name]["ms"] * 64 / t["total_ms"]}
     out["matmul64"] = mm
-    # what the FP64 matrix cores sustain here for this instruction (no memory traffic)
+    out["matmul64_plan"] = {name: Hm.matmul_plan(prec) for prec, name in ((H.PREC_F64, "f64"), (H.PREC_MIXED, "mixed"))}
+    # what the matrix cores sustain here for these instructions (no memory traffic)
     out["mfma_f64_loop_tfs"] = {str(w): round(H.mfma_f64_rate(w), 2) for w in (1, 2, 4)}
+    out["mfma_f32_loop_tfs"] = {str(w): round(H.mfma_f32_rate(w), 2) for w in (1, 2, 4)}
 
     # single runs: FP64 (the plain GPU form) and mixed, with slip-rate histories
     fr = P.friction_c2(g, IMAX, JMAX, DS0, seed=1)
@@ -238,17 +241,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 state / f32 H factors (FP64 MFMA accumulation)",
+        "dtype": "f64 state / f32 H factors (FP32 MFMA, FP64 sums of 32-column slices)",
         "data": "synthetic",
         "config": {"workload": "3dph 256x128 planar fault, N=32768, leaf 64, ACA eps 1e-5, RK45 eps_r 1e-6; "
                                "64 friction realisations in lockstep, one 64-column H product per RK stage, "
                                "FP32 factors, learned step-size proposals (rkqs check), hipGraph attempts",
                    "n_elements": IMAX * JMAX, "realisations": NREAL},
-        "roofline": {"bound": "mfma", "kernel": "mr_tile_kernel<float,64> (64-RHS row tiles, FP64 MFMA)",
-                     "achieved": round(mmx["p2_TFs"], 2), "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
-                     "frac": round(mmx["p2_TFs"] / FP64_MFMA_PEAK_TFS, 4), "traffic": TRAFFIC_MR_TILE_MIXED,
-                     "measured_loop_ceiling": max(r["mfma_f64_loop_tfs"].values()),
-                     "frac_of_measured_ceiling": round(mmx["p2_TFs"] / max(r["mfma_f64_loop_tfs"].values()), 4)},
+        "roofline": {"bound": "mfma", "kernel": "mr_tile32_kernel<float,64,4> (64-RHS row tiles, FP32 MFMA)",
+                     "achieved": round(mmx["p2_TFs"], 2), "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                     "frac": round(mmx["p2_TFs"] / FP32_MFMA_PEAK_TFS, 4), "traffic": TRAFFIC_MR_TILE_MIXED,
+                     "measured_loop_ceiling": max(r["mfma_f32_loop_tfs"].values()),
+                     "frac_of_measured_ceiling": round(mmx["p2_TFs"] / max(r["mfma_f32_loop_tfs"].values()), 4)},
         "roofline_matvec": {"bound": "hbm", "kernel": "stream_kernel<64> (single-vector product, first launch: "
                                                       "stacked a1^T tiles + dense tiles, mixed)",
                             "achieved": round(mvx["a_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -266,6 +269,8 @@ def main():
         "matvec": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in r["matvec"].items()},
         "matmul64": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in r["matmul64"].items()},
+        "matmul64_plan": r["matmul64_plan"],
+        "mfma_loop_tfs": {"f64": r["mfma_f64_loop_tfs"], "f32": r["mfma_f32_loop_tfs"]},
         "single": r["single"],
         "build_s": r["build_s"],
         "hmat": r["hmat"],

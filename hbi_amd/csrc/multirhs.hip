@@ -177,8 +177,10 @@ __device__ __forceinline__ void mr_p1_wave(const MItem1 &it, const uint8_t *__re
 // The same item on the FP32 matrix cores (16x16x4 FP32 MFMA: operands as in
 // the FP64 form, result element q of lane l is D[4 (l / 16) + q][l % 16]).
 // Items have at most 32 rows (NRB <= 2) so the FP64 sums fit beside the FP32
-// accumulators at two or more waves per SIMD; the FP32 sums of every 24
-// columns (one pass of the three buffers) are added into the FP64 sums.
+// accumulators at two waves per SIMD.  A step multiplies half as long as an
+// FP64 step, so a ring of NB register buffers keeps NB - 1 steps of loads in
+// flight; the FP32 sums of every NB steps (8 NB columns) are added into the
+// FP64 sums.
 template <typename T, int NRB>
 __device__ __forceinline__ void mr_p1_wave32(const MItem1 &it, const uint8_t *__restrict__ buf,
                                              const float *__restrict__ X, double *__restrict__ zpart,
@@ -191,7 +193,8 @@ __device__ __forceinline__ void mr_p1_wave32(const MItem1 &it, const uint8_t *__
         Run2<T> a[NRB];
         fvec4 x[2];
     };
-    Buf b0, b1, b2;
+    constexpr int NB = 6;
+    Buf bb[NB];
     auto load = [&](Buf &B, int k) {
         const int tb = it.t0 + 8 * k + 2 * lk;
         const bool ok = tb < tend;
@@ -239,20 +242,21 @@ __device__ __forceinline__ void mr_p1_wave32(const MItem1 &it, const uint8_t *__
             }
     };
     const int nk = (it.tn + 7) / 8;
-    load(b0, 0);
-    load(b1, 1);
+#pragma unroll
+    for (int b = 0; b < NB - 1; b++) load(bb[b], b);
     int k = 0;
-    for (; k + 3 <= nk; k += 3) {
-        load(b2, k + 2);
-        mul(b0);
-        load(b0, k + 3);
-        mul(b1);
-        load(b1, k + 4);
-        mul(b2);
+    for (; k + NB <= nk; k += NB) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            load(bb[(b + NB - 1) % NB], k + b + NB - 1);   // steps past the item's end load zeros
+            mul(bb[b]);
+        }
         flush();
     }
-    if (k < nk) mul(b0);
-    if (k + 1 < nk) mul(b1);
+    // steps k .. nk - 1 (fewer than NB) are in bb[0 ..]
+#pragma unroll
+    for (int b = 0; b < NB - 1; b++)
+        if (k + b < nk) mul(bb[b]);
     flush();
 #pragma unroll
     for (int rb = 0; rb < NRB; rb++)
@@ -438,10 +442,15 @@ __global__ __launch_bounds__(kMrThreads) void mr_tile_kernel(const Item2 *__rest
 }
 
 // The same product on the FP32 matrix cores for FP32 / BF16 factors: 32-deep
-// k-slices of FP32 values in LDS (X and Z rounded at the LDS store); the FP32
-// sums of every slice are added into FP64 sums.
-template <typename T, int TR>
-__global__ __launch_bounds__(kMrThreads) void mr_tile32_kernel(const Item2 *__restrict__ items,
+// k-slices of FP32 values in LDS; the FP32 sums of every slice are added into
+// FP64 sums.  Every thread moves a slice in 16-byte pieces through a ring of
+// register stages, so the loads of the next slices are in flight while one
+// slice is multiplied.  Masked pieces read the zero row (no selects on loaded values).
+// NWC waves share the right-hand sides (64 / NWC each) in each of the two
+// row halves: 8 waves for 64-row tiles, so a tile item keeps two waves on
+// every SIMD.
+template <typename T, int TR, int NWC>
+__global__ __launch_bounds__(2 * NWC * 64) void mr_tile32_kernel(const Item2 *__restrict__ items,
                                                                const Seg *__restrict__ segs,
                                                                const uint8_t *__restrict__ buf,
                                                                const float *__restrict__ X,
@@ -449,98 +458,147 @@ __global__ __launch_bounds__(kMrThreads) void mr_tile32_kernel(const Item2 *__re
                                                                double *__restrict__ ypart, const int *__restrict__ gate)
 {
     constexpr int KS = 32;
-    constexpr int RB = TR / 32;
+    constexpr int NT = 2 * NWC * 64;                          // threads
+    constexpr int RB = TR / 32;                               // 16-row blocks per wave
+    constexpr int CW = kRR / NWC;                             // right-hand sides per wave
+    constexpr int JB = CW / 16;
     constexpr int AP = TR + 16, XP = kRR + 16;
-    constexpr int NA = KS * TR / kMrThreads, NX = KS * kRR / kMrThreads;
-    __shared__ float as[KS][AP];
-    __shared__ float xs[KS][XP];
+    constexpr int EPC = 16 / (int)sizeof(T);                 // factor values per 16-byte piece
+    constexpr int CA = KS * TR / EPC;                         // pieces of a factor slice
+    constexpr int NA = (CA + NT - 1) / NT;
+    constexpr int CX = KS * kRR / 4;                          // pieces of an X slice
+    constexpr int NX = (CX + NT - 1) / NT;
+    static_assert(JB >= 1, "slice split");
+    __shared__ __align__(16) float as[KS][AP];
+    __shared__ __align__(16) float xs[KS][XP];
     if (gate && *gate) return;
     const Item2 it = items[blockIdx.x];
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
-    const int wr = w / 2, wc = w % 2;
+    const int wr = w / NWC, wc = w % NWC;
     const int li = lane % 16, lk = lane / 16;
-    T ra[NA];
-    float rx[NX];
-    auto load = [&](const Seg &g, int c) {
-        const T *a = (const T *)(buf + g.off);
-        const float *xin = (g.src ? Z : X) + (size_t)g.xoff * kRR;
+    struct Stage {
+        uvec4 a[NA];
+        fvec4 x[NX];
+    };
+    auto load = [&](Stage &S, const Seg &g, int c) {
+        const uint8_t *a = buf + g.off + (size_t)c * TR * sizeof(T);
+        const float *xin = (g.src ? Z : X) + ((size_t)g.xoff + c) * kRR;
         const int kn = min(KS, g.ncol - c);
 #pragma unroll
         for (int u = 0; u < NA; u++) {
-            const int q = tid + u * kMrThreads;
-            ra[u] = q / TR < kn ? a[(size_t)c * TR + q] : (T)0;
+            const int q = tid + u * NT;
+            const bool ok = q < CA && q * EPC / TR < kn;
+            S.a[u] = *(ok ? (const uvec4 *)(a + (size_t)q * 16) : (const uvec4 *)g_zero_row);
         }
 #pragma unroll
         for (int u = 0; u < NX; u++) {
-            const int q = tid + u * kMrThreads;
-            rx[u] = q / kRR < kn ? xin[(size_t)c * kRR + q] : 0.f;
+            const int q = tid + u * NT;
+            S.x[u] = *(q < CX && q * 4 / kRR < kn ? (const fvec4 *)(xin + (size_t)q * 4) : (const fvec4 *)g_zero_row);
         }
     };
-    fvec4 acc[RB][2];
-    v4d sum[RB][2];
+    auto store = [&](const Stage &S) {
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int q = tid + u * NT;
+            if (q < CA) {
+                const int e0 = q * EPC, col = e0 / TR, row = e0 % TR;
+                if constexpr (sizeof(T) == 4) {
+                    *(fvec4 *)&as[col][row] = fvec4{__uint_as_float(S.a[u][0]), __uint_as_float(S.a[u][1]),
+                                                    __uint_as_float(S.a[u][2]), __uint_as_float(S.a[u][3])};
+                } else {
+#pragma unroll
+                    for (int h = 0; h < 2; h++)
+                        *(fvec4 *)&as[col][row + 4 * h] =
+                            fvec4{__uint_as_float(S.a[u][2 * h] << 16), __uint_as_float(S.a[u][2 * h] & 0xffff0000u),
+                                  __uint_as_float(S.a[u][2 * h + 1] << 16),
+                                  __uint_as_float(S.a[u][2 * h + 1] & 0xffff0000u)};
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NX; u++) {
+            const int q = tid + u * NT;
+            if (q < CX) *(fvec4 *)&xs[q * 4 / kRR][q * 4 % kRR] = S.x[u];
+        }
+    };
+    fvec4 acc[RB][JB];
+    v4d sum[RB][JB];
 #pragma unroll
     for (int ib = 0; ib < RB; ib++)
 #pragma unroll
-        for (int jb = 0; jb < 2; jb++) {
+        for (int jb = 0; jb < JB; jb++) {
             acc[ib][jb] = fvec4{0.f, 0.f, 0.f, 0.f};
             sum[ib][jb] = v4d{0.0, 0.0, 0.0, 0.0};
         }
-    int s = it.seg0, c0 = 0;
+    // position of the next slice to load: segment s, first column c
+    int s = it.seg0, c = 0;
     Seg sg{};
-    if (s < it.seg1) {
-        sg = segs[s];
-        load(sg, 0);
-    }
-    while (s < it.seg1) {
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < NA; u++) {
-            const int q = tid + u * kMrThreads;
-            as[q / TR][q % TR] = narrow<T>(ra[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < NX; u++) {
-            const int q = tid + u * kMrThreads;
-            xs[q / kRR][q % kRR] = rx[u];
-        }
-        __syncthreads();
-        c0 += KS;
-        if (c0 >= sg.ncol) {
+    if (s < it.seg1) sg = segs[s];
+    auto next = [&](Stage &S) {
+        if (s >= it.seg1) return false;
+        load(S, sg, c);
+        c += KS;
+        if (c >= sg.ncol) {
             s++;
-            c0 = 0;
+            c = 0;
             if (s < it.seg1) sg = segs[s];
         }
-        if (s < it.seg1) load(sg, c0);
+        return true;
+    };
+    auto multiply = [&] {
 #pragma unroll
         for (int kk = 0; kk < KS; kk += 4) {
-            float bf[2];
+            float bf[JB];
 #pragma unroll
-            for (int jb = 0; jb < 2; jb++) bf[jb] = xs[kk + lk][32 * wc + 16 * jb + li];
+            for (int jb = 0; jb < JB; jb++) bf[jb] = xs[kk + lk][CW * wc + 16 * jb + li];
 #pragma unroll
             for (int ib = 0; ib < RB; ib++) {
                 const float af = as[kk + lk][RB * 16 * wr + 16 * ib + li];
 #pragma unroll
-                for (int jb = 0; jb < 2; jb++)
+                for (int jb = 0; jb < JB; jb++)
                     acc[ib][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af, bf[jb], acc[ib][jb], 0, 0, 0);
             }
         }
 #pragma unroll
         for (int ib = 0; ib < RB; ib++)
 #pragma unroll
-            for (int jb = 0; jb < 2; jb++) {
+            for (int jb = 0; jb < JB; jb++) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) sum[ib][jb][q] += (double)acc[ib][jb][q];
                 acc[ib][jb] = fvec4{0.f, 0.f, 0.f, 0.f};
             }
+    };
+    // a ring of NS register stages: slice i sits in stage i % NS; storing it
+    // frees the stage for slice i + NS, whose loads then have NS - 1 multiply
+    // phases to arrive
+    constexpr int NS = 3;
+    Stage st[NS];
+    bool have[NS];
+#pragma unroll
+    for (int b = 0; b < NS; b++) have[b] = next(st[b]);
+    bool more = have[0];
+    while (more) {
+#pragma unroll
+        for (int b = 0; b < NS; b++) {
+            if (!have[b]) {
+                more = false;
+                break;
+            }
+            __syncthreads();
+            store(st[b]);
+            __syncthreads();
+            have[b] = next(st[b]);
+            multiply();
+        }
     }
     double *o = ypart + (size_t)it.part * TR * kRR;
 #pragma unroll
     for (int ib = 0; ib < RB; ib++)
 #pragma unroll
-        for (int jb = 0; jb < 2; jb++)
+        for (int jb = 0; jb < JB; jb++)
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                o[(size_t)(RB * 16 * wr + 16 * ib + 4 * lk + q) * kRR + 32 * wc + 16 * jb + li] = sum[ib][jb][q];
+                o[(size_t)(RB * 16 * wr + 16 * ib + 4 * lk + q) * kRR + CW * wc + 16 * jb + li] = sum[ib][jb][q];
 }
 
 __global__ void mr_tile_reduce_kernel(int n, int tr, const int *__restrict__ part0, const double *__restrict__ ypart,
@@ -710,6 +768,8 @@ static void mr_mode(const Plan &P, MrPlan &M)
     for (const Seg &g : P.segs) (g.dtype == kF64 ? has64 : has32) = true;
     for (const Row1 &r : P.rows1) (r.dtype == kF64 ? has64 : has32) = true;
     HBI_REQUIRE(!(has64 && has32), "64-column product: FP64 and reduced-precision factors in one plan");
+    for (const Seg &g : P.segs)
+        HBI_REQUIRE(g.off % 16 == 0, "64-column product: segment not 16-byte aligned");
     M.f32 = has32;
 }
 
@@ -800,12 +860,12 @@ static void phase2(const hbi_hmat *h, const Plan &P, const MrPlan &M, const doub
         const Item2 *it = M.d_it2.p + M.r2[d][0];
         if (h->tr == 64) {
             if (d == kF64) mr_tile_kernel<double, 64><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X, Z, ypart, gate);
-            else if (d == kF32) mr_tile32_kernel<float, 64><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
-            else mr_tile32_kernel<uint16_t, 64><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
+            else if (d == kF32) mr_tile32_kernel<float, 64, 4><<<n, 512, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
+            else mr_tile32_kernel<uint16_t, 64, 4><<<n, 512, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
         } else {
             if (d == kF64) mr_tile_kernel<double, 32><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X, Z, ypart, gate);
-            else if (d == kF32) mr_tile32_kernel<float, 32><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
-            else mr_tile32_kernel<uint16_t, 32><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
+            else if (d == kF32) mr_tile32_kernel<float, 32, 2><<<n, 256, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
+            else mr_tile32_kernel<uint16_t, 32, 2><<<n, 256, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
         }
     }
 }
@@ -929,6 +989,20 @@ extern "C" int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs
     });
 }
 
+extern "C" int hbi_hmat_matmul_plan(hbi_hmat *h, int prec, int *info)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(h && info, "bad arguments");
+        const hbi::MrPlan &M = hbi::mr_plan(h, prec);
+        info[0] = M.kc1;
+        info[1] = M.cols2;
+        info[2] = (int)M.it1.size();
+        info[3] = (int)M.it2.size();
+        info[4] = M.nzpart;
+        info[5] = M.f32 ? 1 : 0;
+    });
+}
+
 extern "C" int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms)
 {
     return hbi::guarded([&] {
@@ -964,11 +1038,11 @@ extern "C" int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms
     });
 }
 
-// Rate of the FP64 matrix cores on this device for the instruction both
-// 64-column kernels use: every wave runs iters rounds of 8 independent
-// v_mfma_f64_16x16x4f64 chains (no memory traffic), waves_per_simd waves on
-// each SIMD.  The ceiling the products are compared with beside the data-sheet
-// peak.
+// Rate of the matrix cores on this device for the instructions of the
+// 64-column kernels: every wave runs iters rounds of 8 independent
+// v_mfma_f64_16x16x4f64 (FP64 factors) or v_mfma_f32_16x16x4f32 (FP32 / BF16
+// factors) chains, no memory traffic, waves_per_simd waves on each SIMD.  The
+// ceiling the products are compared with beside the data-sheet peak.
 namespace {
 __global__ __launch_bounds__(64) void mfma_rate_kernel(double *out, int iters, double a0)
 {
@@ -985,9 +1059,24 @@ __global__ __launch_bounds__(64) void mfma_rate_kernel(double *out, int iters, d
     for (int i = 0; i < 8; i++) s += acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3];
     out[(size_t)blockIdx.x * 64 + threadIdx.x] = s;
 }
-}  // namespace
 
-extern "C" int hbi_mfma_f64_rate(int waves_per_simd, int iters, double *tflops)
+__global__ __launch_bounds__(64) void mfma32_rate_kernel(double *out, int iters, double a0)
+{
+    using hbi::fvec4;
+    fvec4 acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i] = fvec4{0.f, 0.f, 0.f, 0.f};
+    const float a = (float)(a0 + threadIdx.x), b = (float)(1.0 - a0 * threadIdx.x);
+    for (int it = 0; it < iters; it++)
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i], 0, 0, 0);
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) s += (double)(acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3]);
+    out[(size_t)blockIdx.x * 64 + threadIdx.x] = s;
+}
+
+int mfma_rate(bool f32, int waves_per_simd, int iters, double *tflops)
 {
     return hbi::guarded([&] {
         HBI_REQUIRE(tflops && waves_per_simd > 0 && waves_per_simd <= 8 && iters > 0, "bad arguments");
@@ -1001,9 +1090,13 @@ extern "C" int hbi_mfma_f64_rate(int waves_per_simd, int iters, double *tflops)
         hipEvent_t e0 = nullptr, e1 = nullptr;
         HBI_HIP(hipEventCreate(&e0));
         HBI_HIP(hipEventCreate(&e1));
-        mfma_rate_kernel<<<nwaves, 64, 0, s>>>(out.p, iters, 0.5);
+        auto launch = [&] {
+            if (f32) mfma32_rate_kernel<<<nwaves, 64, 0, s>>>(out.p, iters, 0.5);
+            else mfma_rate_kernel<<<nwaves, 64, 0, s>>>(out.p, iters, 0.5);
+        };
+        launch();
         HBI_HIP(hipEventRecord(e0, s));
-        mfma_rate_kernel<<<nwaves, 64, 0, s>>>(out.p, iters, 0.5);
+        launch();
         HBI_HIP(hipEventRecord(e1, s));
         HBI_HIP(hipEventSynchronize(e1));
         float ms = 0;
@@ -1013,4 +1106,15 @@ extern "C" int hbi_mfma_f64_rate(int waves_per_simd, int iters, double *tflops)
         (void)hipStreamDestroy(s);
         *tflops = (double)nwaves * iters * 8 * 2048.0 / (ms * 1e-3) / 1e12;
     });
+}
+}  // namespace
+
+extern "C" int hbi_mfma_f64_rate(int waves_per_simd, int iters, double *tflops)
+{
+    return mfma_rate(false, waves_per_simd, iters, tflops);
+}
+
+extern "C" int hbi_mfma_f32_rate(int waves_per_simd, int iters, double *tflops)
+{
+    return mfma_rate(true, waves_per_simd, iters, tflops);
 }

@@ -91,7 +91,7 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_hmat_stats_get", "hbi_hmat_lod", "hbi_hmat_leaves", "hbi_hmat_leaf_data", "hbi_hmat_matvec",
            "hbi_hmat_matmul", "hbi_hmat_time_matvec", "hbi_rk_create", "hbi_rk_destroy", "hbi_rk_set_state",
            "hbi_rk_get_state", "hbi_rk_derivs", "hbi_rk_rkck", "hbi_rk_step", "hbi_rk_last_ms",
-           "hbi_hmat_time_matmul", "hbi_mfma_f64_rate", "hbi_ens_create", "hbi_ens_destroy", "hbi_ens_set_state", "hbi_ens_get_state",
+           "hbi_hmat_time_matmul", "hbi_hmat_matmul_plan", "hbi_mfma_f64_rate", "hbi_mfma_f32_rate", "hbi_ens_create", "hbi_ens_destroy", "hbi_ens_set_state", "hbi_ens_get_state",
            "hbi_ens_step", "hbi_ens_last_ms", "hbi_mlp_create", "hbi_mlp_destroy", "hbi_mlp_nparams",
            "hbi_mlp_set_params", "hbi_mlp_get_params", "hbi_mlp_forward", "hbi_mlp_loss_grad", "hbi_mlp_train",
            "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace", "hbi_ens_set_controller",
@@ -125,7 +125,9 @@ def lib():
         L.hbi_rk_step.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip, _dp, C.POINTER(RkInfo)]
         L.hbi_rk_last_ms.argtypes = [C.c_void_p, _dp]
         L.hbi_hmat_time_matmul.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.hbi_hmat_matmul_plan.argtypes = [C.c_void_p, C.c_int, _ip]
         L.hbi_mfma_f64_rate.argtypes = [C.c_int, C.c_int, _dp]
+        L.hbi_mfma_f32_rate.argtypes = [C.c_int, C.c_int, _dp]
         _lp = C.POINTER(C.c_longlong)
         L.hbi_ens_create.argtypes = [C.c_void_p, C.POINTER(RkOpts), C.c_int] + [_dp] * 7 + [C.POINTER(C.c_void_p)]
         L.hbi_ens_destroy.argtypes = [C.c_void_p]
@@ -275,11 +277,25 @@ class HMatrix:
         _check(lib().hbi_hmat_time_matmul(self.h, int(prec), int(iters), _p(ms)))
         return {"p1_ms": ms[0], "p2_ms": ms[1], "total_ms": ms[2]}
 
+    def matmul_plan(self, prec=PREC_F64) -> dict:
+        """The autotuned 64-column plan: chunk sizes, item counts, FP32 cores."""
+        info = np.zeros(6, dtype=np.int32)
+        _check(lib().hbi_hmat_matmul_plan(self.h, int(prec), info.ctypes.data_as(_ip)))
+        return {"kc1": int(info[0]), "cols2": int(info[1]), "items1": int(info[2]), "items2": int(info[3]),
+                "zpart_slots": int(info[4]), "fp32_cores": bool(info[5])}
+
 
 def mfma_f64_rate(waves_per_simd=2, iters=4000) -> float:
     """Measured TFLOP/s of the FP64 16x16x4 MFMA with no memory traffic."""
     t = np.zeros(1)
     _check(lib().hbi_mfma_f64_rate(int(waves_per_simd), int(iters), _p(t)))
+    return float(t[0])
+
+
+def mfma_f32_rate(waves_per_simd=2, iters=4000) -> float:
+    """Measured TFLOP/s of the FP32 16x16x4 MFMA with no memory traffic."""
+    t = np.zeros(1)
+    _check(lib().hbi_mfma_f32_rate(int(waves_per_simd), int(iters), _p(t)))
     return float(t[0])
 
 

@@ -130,19 +130,29 @@ int hbi_hmat_leaf_data(const hbi_hmat *h, int ip, double *out);
 int hbi_hmat_matvec(hbi_hmat *h, const double *x, double *y, int prec, int on_device);
 /* Y = H X for nrhs right-hand sides, X and Y column-major N x nrhs (host).
    Computed 64 columns at a time by the multi-RHS kernels (factors read once
-   per 64 columns, FP64 accumulation); no reference counterpart (the
-   reference applies H to one vector at a time). */
+   per 64 columns): FP64 factors on the FP64 matrix cores; FP32 / BF16 factors
+   on the FP32 matrix cores with X and the low-rank intermediates rounded to
+   FP32 and the FP32 sums of at most 48 columns added in FP64.  No reference
+   counterpart (the reference applies H to one vector at a time). */
 int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs, int prec);
 /* Average device time per call over iters 64-column products on device-resident
    data, HIP events on the library stream: ms[0] = phase 1 (a1^T X of the
    low-rank blocks), ms[1] = phase 2 (row tiles), ms[2] = whole product. */
 int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms);
+/* The autotuned 64-column plan of a precision (built on first use):
+   info[0] phase-1 column chunk, [1] phase-2 columns per item, [2] phase-1
+   items, [3] phase-2 items (= FP64 partial row tiles written), [4] phase-1
+   partial slots, [5] 1 if the FP32 matrix cores are used. */
+int hbi_hmat_matmul_plan(hbi_hmat *h, int prec, int *info);
 /* Measured FP64 matrix-core rate (TFLOP/s) of v_mfma_f64_16x16x4f64, the
    instruction of both 64-column kernels: waves_per_simd (1..8) waves per SIMD
    each run iters rounds of 8 independent chains, no memory traffic.  The
    ceiling the 64-column products are set against beside the data-sheet peak;
    no reference counterpart. */
 int hbi_mfma_f64_rate(int waves_per_simd, int iters, double *tflops);
+/* The same for v_mfma_f32_16x16x4f32, the instruction of the 64-column
+   kernels for FP32 / BF16 factors (2048 flops per instruction). */
+int hbi_mfma_f32_rate(int waves_per_simd, int iters, double *tflops);
 /* Average device time per call of the two matvec launches over iters products
    on device-resident vectors, from HIP events on the library stream:
    ms[0] = first launch (a1^T x of the stacked low-rank row tiles, and the
