@@ -20,7 +20,8 @@ struct Seg {
 };
 // Phase-2 work item: segments [seg0, seg1) of one row tile, one dtype.
 struct Item2 {
-    int tile, seg0, seg1, part, dtype, pad;
+    int tile, seg0, seg1, part, dtype;
+    int pad;        // 64-column FP32-core items: slices of packed segment runs (mr_slices)
 };
 // Low-rank work item of the single-vector product: the low-rank segments of
 // one row tile, for each dtype d ncol[d] contiguous columns from byte offset

@@ -29,6 +29,8 @@ constexpr int kRR = 64;      // right-hand sides per product
 constexpr int kP1Rows = 64;  // a1^T rows per phase-1 item (one wave)
 constexpr int kKS = 32;      // k-slice staged in LDS
 constexpr int kMrThreads = 256;
+constexpr int kKS32 = 32;         // columns per slice of the FP32-core phase-2 kernel
+constexpr int kMaxCols32 = 4096;  // columns of one FP32-core phase-2 item at most
 
 template <typename T> __device__ __forceinline__ double widen(T v);
 template <> __device__ __forceinline__ double widen<double>(double v) { return v; }
@@ -45,6 +47,13 @@ struct MItem1 {
                        // column range and writes Z rows zrow[zr .. zr + kt) directly
     int dtype, zr;
     int r0;            // first row of the item within its partial slot (FP32-core items: 0 or 32)
+};
+// FP32-core phase-2 item: the item's segments are consecutive in the packed
+// layout, so they form one TR x ncol column-major block at byte offset off;
+// column k multiplies X row cmap[cm0 + k] (Z row when bit 31 is set)
+struct MItem2 {
+    uint64_t off;
+    int ncol, cm0, part, pad;
 };
 struct MBlk1 {         // rows roff .. roff+kt of partial slots [p0, p1) -> z rows zoff .. zoff+kt
     int p0, p1, kt, zoff, roff;
@@ -443,21 +452,24 @@ __global__ __launch_bounds__(kMrThreads) void mr_tile_kernel(const Item2 *__rest
 
 // The same product on the FP32 matrix cores for FP32 / BF16 factors: 32-deep
 // k-slices of FP32 values in LDS; the FP32 sums of every slice are added into
-// FP64 sums.  Every thread moves a slice in 16-byte pieces through a ring of
-// register stages, so the loads of the next slices are in flight while one
-// slice is multiplied.  Masked pieces read the zero row (no selects on loaded values).
+// FP64 sums.  An item is one contiguous column block (its segments back to
+// back), so a slice is a plain 32-column stride of it whatever the segment
+// widths (low-rank segments have k_t columns, often far fewer than 32); the
+// X / Z row of each column comes from the item's column map, staged in LDS
+// once.  Every thread moves a slice in 16-byte pieces through a ring of
+// register stages; no branch surrounds a load, so the wait before a stage's
+// LDS store covers that stage only and the next NS - 1 slices stay in flight.
 // NWC waves share the right-hand sides (64 / NWC each) in each of the two
-// row halves: 8 waves for 64-row tiles, so a tile item keeps two waves on
-// every SIMD.
+// row halves: 8 waves for 64-row tiles, two waves on every SIMD per item.
 template <typename T, int TR, int NWC>
-__global__ __launch_bounds__(2 * NWC * 64) void mr_tile32_kernel(const Item2 *__restrict__ items,
-                                                               const Seg *__restrict__ segs,
-                                                               const uint8_t *__restrict__ buf,
-                                                               const float *__restrict__ X,
-                                                               const float *__restrict__ Z,
-                                                               double *__restrict__ ypart, const int *__restrict__ gate)
+__global__ __launch_bounds__(2 * NWC * 64) void mr_tile32_kernel(const MItem2 *__restrict__ items,
+                                                                 const int *__restrict__ cmap,
+                                                                 const uint8_t *__restrict__ buf,
+                                                                 const float *__restrict__ X,
+                                                                 const float *__restrict__ Z,
+                                                                 double *__restrict__ ypart, const int *__restrict__ gate)
 {
-    constexpr int KS = 32;
+    constexpr int KS = kKS32;
     constexpr int NT = 2 * NWC * 64;                          // threads
     constexpr int RB = TR / 32;                               // 16-row blocks per wave
     constexpr int CW = kRR / NWC;                             // right-hand sides per wave
@@ -469,56 +481,64 @@ __global__ __launch_bounds__(2 * NWC * 64) void mr_tile32_kernel(const Item2 *__
     constexpr int CX = KS * kRR / 4;                          // pieces of an X slice
     constexpr int NX = (CX + NT - 1) / NT;
     static_assert(JB >= 1, "slice split");
-    __shared__ __align__(16) float as[KS][AP];
-    __shared__ __align__(16) float xs[KS][XP];
+    // one spare row: pieces past a slice's end are stored there, so no store is guarded
+    __shared__ __align__(16) float as[KS + 1][AP];
+    __shared__ __align__(16) float xs[KS + 1][XP];
+    __shared__ int cm[kMaxCols32];
     if (gate && *gate) return;
-    const Item2 it = items[blockIdx.x];
+    const MItem2 it = items[blockIdx.x];
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
     const int wr = w / NWC, wc = w % NWC;
     const int li = lane % 16, lk = lane / 16;
+    for (int k = tid; k < it.ncol; k += NT) cm[k] = cmap[it.cm0 + k];
+    __syncthreads();
+    const uint8_t *fb = buf + it.off;
     struct Stage {
         uvec4 a[NA];
         fvec4 x[NX];
-    };
-    auto load = [&](Stage &S, const Seg &g, int c) {
-        const uint8_t *a = buf + g.off + (size_t)c * TR * sizeof(T);
-        const float *xin = (g.src ? Z : X) + ((size_t)g.xoff + c) * kRR;
-        const int kn = min(KS, g.ncol - c);
-#pragma unroll
-        for (int u = 0; u < NA; u++) {
-            const int q = tid + u * NT;
-            const bool ok = q < CA && q * EPC / TR < kn;
-            S.a[u] = *(ok ? (const uvec4 *)(a + (size_t)q * 16) : (const uvec4 *)g_zero_row);
-        }
-#pragma unroll
-        for (int u = 0; u < NX; u++) {
-            const int q = tid + u * NT;
-            S.x[u] = *(q < CX && q * 4 / kRR < kn ? (const fvec4 *)(xin + (size_t)q * 4) : (const fvec4 *)g_zero_row);
-        }
     };
     auto store = [&](const Stage &S) {
 #pragma unroll
         for (int u = 0; u < NA; u++) {
             const int q = tid + u * NT;
-            if (q < CA) {
-                const int e0 = q * EPC, col = e0 / TR, row = e0 % TR;
-                if constexpr (sizeof(T) == 4) {
-                    *(fvec4 *)&as[col][row] = fvec4{__uint_as_float(S.a[u][0]), __uint_as_float(S.a[u][1]),
-                                                    __uint_as_float(S.a[u][2]), __uint_as_float(S.a[u][3])};
-                } else {
+            const int e0 = q * EPC, col = q < CA ? e0 / TR : KS, row = q < CA ? e0 % TR : 0;
+            if constexpr (sizeof(T) == 4) {
+                *(fvec4 *)&as[col][row] = fvec4{__uint_as_float(S.a[u][0]), __uint_as_float(S.a[u][1]),
+                                                __uint_as_float(S.a[u][2]), __uint_as_float(S.a[u][3])};
+            } else {
 #pragma unroll
-                    for (int h = 0; h < 2; h++)
-                        *(fvec4 *)&as[col][row + 4 * h] =
-                            fvec4{__uint_as_float(S.a[u][2 * h] << 16), __uint_as_float(S.a[u][2 * h] & 0xffff0000u),
-                                  __uint_as_float(S.a[u][2 * h + 1] << 16),
-                                  __uint_as_float(S.a[u][2 * h + 1] & 0xffff0000u)};
-                }
+                for (int h = 0; h < 2; h++)
+                    *(fvec4 *)&as[col][row + 4 * h] =
+                        fvec4{__uint_as_float(S.a[u][2 * h] << 16), __uint_as_float(S.a[u][2 * h] & 0xffff0000u),
+                              __uint_as_float(S.a[u][2 * h + 1] << 16),
+                              __uint_as_float(S.a[u][2 * h + 1] & 0xffff0000u)};
             }
         }
 #pragma unroll
         for (int u = 0; u < NX; u++) {
             const int q = tid + u * NT;
-            if (q < CX) *(fvec4 *)&xs[q * 4 / kRR][q * 4 % kRR] = S.x[u];
+            const int col = q < CX ? q * 4 / kRR : KS, e = q < CX ? q * 4 % kRR : 0;
+            *(fvec4 *)&xs[col][e] = S.x[u];
+        }
+    };
+    // loads of slice j (columns 32 j ..); past the item's end: the zero row
+    auto gather = [&](Stage &S, int j) {
+        const int c0 = j * KS;
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int q = tid + u * NT;
+            const int col = c0 + q * EPC / TR, row = q * EPC % TR;
+            const bool ok = q < CA && col < it.ncol;
+            S.a[u] = *(ok ? (const uvec4 *)(fb + ((size_t)col * TR + row) * sizeof(T)) : (const uvec4 *)g_zero_row);
+        }
+#pragma unroll
+        for (int u = 0; u < NX; u++) {
+            const int q = tid + u * NT;
+            const int col = c0 + q * 4 / kRR, e = q * 4 % kRR;
+            const bool ok = q < CX && col < it.ncol;
+            const int m = cm[ok ? col : 0];
+            const float *src = m < 0 ? Z : X;
+            S.x[u] = *(ok ? (const fvec4 *)(src + (size_t)(m & 0x7fffffff) * kRR + e) : (const fvec4 *)g_zero_row);
         }
     };
     fvec4 acc[RB][JB];
@@ -530,21 +550,6 @@ __global__ __launch_bounds__(2 * NWC * 64) void mr_tile32_kernel(const Item2 *__
             acc[ib][jb] = fvec4{0.f, 0.f, 0.f, 0.f};
             sum[ib][jb] = v4d{0.0, 0.0, 0.0, 0.0};
         }
-    // position of the next slice to load: segment s, first column c
-    int s = it.seg0, c = 0;
-    Seg sg{};
-    if (s < it.seg1) sg = segs[s];
-    auto next = [&](Stage &S) {
-        if (s >= it.seg1) return false;
-        load(S, sg, c);
-        c += KS;
-        if (c >= sg.ncol) {
-            s++;
-            c = 0;
-            if (s < it.seg1) sg = segs[s];
-        }
-        return true;
-    };
     auto multiply = [&] {
 #pragma unroll
         for (int kk = 0; kk < KS; kk += 4) {
@@ -568,27 +573,21 @@ __global__ __launch_bounds__(2 * NWC * 64) void mr_tile32_kernel(const Item2 *__
                 acc[ib][jb] = fvec4{0.f, 0.f, 0.f, 0.f};
             }
     };
-    // a ring of NS register stages: slice i sits in stage i % NS; storing it
-    // frees the stage for slice i + NS, whose loads then have NS - 1 multiply
-    // phases to arrive
-    constexpr int NS = 3;
+    // a ring of NS register stages: slice j sits in stage j % NS; storing it
+    // frees the stage for slice j + NS
+    constexpr int NS = 4;
+    const int nsl = (it.ncol + KS - 1) / KS;
     Stage st[NS];
-    bool have[NS];
 #pragma unroll
-    for (int b = 0; b < NS; b++) have[b] = next(st[b]);
-    bool more = have[0];
-    while (more) {
+    for (int b = 0; b < NS; b++) gather(st[b], b);
+    for (int i = 0; i < nsl; i += NS) {
 #pragma unroll
         for (int b = 0; b < NS; b++) {
-            if (!have[b]) {
-                more = false;
-                break;
-            }
             __syncthreads();
             store(st[b]);
             __syncthreads();
-            have[b] = next(st[b]);
-            multiply();
+            gather(st[b], i + b + NS);
+            if (i + b < nsl) multiply();
         }
     }
     double *o = ypart + (size_t)it.part * TR * kRR;
@@ -622,10 +621,14 @@ struct MrPlan {
     std::vector<MItem1> it1;
     std::vector<MBlk1> blk1;
     std::vector<Item2> it2;
+    std::vector<MItem2> it2f;      // the same items as contiguous column blocks (FP32-core kernel)
+    std::vector<int> cmap;         // their columns' X / Z rows (bit 31: Z)
     std::vector<int> part0;        // ntiles + 1: partial slots of each row tile
     DevBuf<MItem1> d_it1;
     DevBuf<MBlk1> d_blk1;
     DevBuf<Item2> d_it2;
+    DevBuf<MItem2> d_it2f;
+    DevBuf<int> d_cmap;
     DevBuf<int> d_part0;
     int nzpart = 0, nparts = 0;
 };
@@ -757,6 +760,24 @@ static void mr_items2(const hbi_hmat *h, const Plan &P, MrPlan &M, int cols2)
         if (M.r2[d][1] == 0) M.r2[d][0] = q;
         M.r2[d][1] = q + 1;
     }
+    // contiguous-block form: the segments of an item follow each other in the layout
+    M.it2f.clear();
+    M.cmap.clear();
+    for (const Item2 &it : M.it2) {
+        MItem2 f{};
+        f.off = P.segs[it.seg0].off;
+        f.cm0 = (int)M.cmap.size();
+        f.part = it.part;
+        const size_t colbytes = (size_t)h->tr * dtype_size(it.dtype);
+        for (int q = it.seg0; q < it.seg1; q++) {
+            const Seg &g = P.segs[q];
+            HBI_REQUIRE(g.off == f.off + (uint64_t)f.ncol * colbytes, "64-column product: item segments not contiguous");
+            for (int j = 0; j < g.ncol; j++) M.cmap.push_back((g.src ? (int)0x80000000u : 0) | (g.xoff + j));
+            f.ncol += g.ncol;
+        }
+        HBI_REQUIRE(f.ncol <= kMaxCols32, "64-column product: item wider than the column map");
+        M.it2f.push_back(f);
+    }
     M.cols2 = cols2;
 }
 
@@ -778,6 +799,8 @@ static void mr_upload(hbi_hmat *h, MrPlan &M)
     M.d_it1.upload(M.it1, h->stream);
     M.d_blk1.upload(M.blk1, h->stream);
     M.d_it2.upload(M.it2, h->stream);
+    M.d_it2f.upload(M.it2f, h->stream);
+    M.d_cmap.upload(M.cmap.empty() ? std::vector<int>{0} : M.cmap, h->stream);
     M.d_part0.upload(M.part0, h->stream);
 }
 
@@ -858,14 +881,16 @@ static void phase2(const hbi_hmat *h, const Plan &P, const MrPlan &M, const doub
         const int n = M.r2[d][1] - M.r2[d][0];
         if (n <= 0) continue;
         const Item2 *it = M.d_it2.p + M.r2[d][0];
+        const MItem2 *itf = M.d_it2f.p + M.r2[d][0];
+        const int *cm = M.d_cmap.p;
         if (h->tr == 64) {
             if (d == kF64) mr_tile_kernel<double, 64><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X, Z, ypart, gate);
-            else if (d == kF32) mr_tile32_kernel<float, 64, 4><<<n, 512, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
-            else mr_tile32_kernel<uint16_t, 64, 4><<<n, 512, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
+            else if (d == kF32) mr_tile32_kernel<float, 64, 4><<<n, 512, 0, s>>>(itf, cm, P.buf.p, X32, Z32, ypart, gate);
+            else mr_tile32_kernel<uint16_t, 64, 4><<<n, 512, 0, s>>>(itf, cm, P.buf.p, X32, Z32, ypart, gate);
         } else {
             if (d == kF64) mr_tile_kernel<double, 32><<<n, kMrThreads, 0, s>>>(it, P.d_segs.p, P.buf.p, X, Z, ypart, gate);
-            else if (d == kF32) mr_tile32_kernel<float, 32, 2><<<n, 256, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
-            else mr_tile32_kernel<uint16_t, 32, 2><<<n, 256, 0, s>>>(it, P.d_segs.p, P.buf.p, X32, Z32, ypart, gate);
+            else if (d == kF32) mr_tile32_kernel<float, 32, 2><<<n, 256, 0, s>>>(itf, cm, P.buf.p, X32, Z32, ypart, gate);
+            else mr_tile32_kernel<uint16_t, 32, 2><<<n, 256, 0, s>>>(itf, cm, P.buf.p, X32, Z32, ypart, gate);
         }
     }
 }
