@@ -46,7 +46,7 @@ struct MItem1 {
     int part;          // partial slot zpart[part][kP1Rows][RR], or -1: the item is the rows' whole
                        // column range and writes Z rows zrow[zr .. zr + kt) directly
     int dtype, zr;
-    int r0;            // first row of the item within its partial slot (FP32-core items: 0 or 32)
+    int r0;            // first row of the item within its partial slot (set per wave by the FP32-core kernel)
 };
 // FP32-core phase-2 item: the item's segments are consecutive in the packed
 // layout, so they form one TR x ncol column-major block at byte offset off;
@@ -185,8 +185,9 @@ __device__ __forceinline__ void mr_p1_wave(const MItem1 &it, const uint8_t *__re
 
 // The same item on the FP32 matrix cores (16x16x4 FP32 MFMA: operands as in
 // the FP64 form, result element q of lane l is D[4 (l / 16) + q][l % 16]).
-// Items have at most 32 rows (NRB <= 2) so the FP64 sums fit beside the FP32
-// accumulators at two waves per SIMD.  A step multiplies half as long as an
+// Each wave takes 32 rows of an item (NRB <= 2) so the FP64 sums fit beside
+// the FP32 accumulators at two waves per SIMD; the two waves of an item read
+// the same X rows close together (the second read hits the CU's cache).  A step multiplies half as long as an
 // FP64 step, so a ring of NB register buffers keeps NB - 1 steps of loads in
 // flight; the FP32 sums of every NB steps (8 NB columns) are added into the
 // FP64 sums.
@@ -285,13 +286,20 @@ __device__ __forceinline__ void mr_p1_wave32(const MItem1 &it, const uint8_t *__
         }
 }
 
-__global__ __launch_bounds__(64) void mr_lrvt32_kernel(const MItem1 *__restrict__ items,
-                                                       const uint8_t *__restrict__ buf, const float *__restrict__ X,
-                                                       double *__restrict__ zpart, const int *__restrict__ zrow,
-                                                       float *__restrict__ Z, const int *__restrict__ gate)
+__global__ __launch_bounds__(128) void mr_lrvt32_kernel(const MItem1 *__restrict__ items,
+                                                        const uint8_t *__restrict__ buf, const float *__restrict__ X,
+                                                        double *__restrict__ zpart, const int *__restrict__ zrow,
+                                                        float *__restrict__ Z, const int *__restrict__ gate)
 {
     if (gate && *gate) return;
-    const MItem1 it = items[blockIdx.x];
+    MItem1 it = items[blockIdx.x];
+    // wave w: rows 32 w .. 32 w + 31 of the item (none: the wave ends; no barrier follows)
+    const int w = threadIdx.x / 64;
+    if (it.kt <= 32 * w) return;
+    it.kt = min(32, it.kt - 32 * w);
+    it.r0 = 32 * w;
+    it.zr += 32 * w;
+    it.off += (uint64_t)32 * w * it.ldt * (it.dtype == kF32 ? 4 : 2);
     if (it.dtype == kF32) {
         if (it.kt > 16) mr_p1_wave32<float, 2>(it, buf, X, zpart, zrow, Z);
         else mr_p1_wave32<float, 1>(it, buf, X, zpart, zrow, Z);
@@ -337,16 +345,15 @@ template <typename OT>
 __global__ void mr_zreduce_kernel(const MBlk1 *__restrict__ blks, int nblk, const double *__restrict__ zpart,
                                   OT *__restrict__ Z, const int *__restrict__ gate)
 {
+    // workgroup (b, k): row k of block b, one right-hand side per thread (64 threads)
     if (gate && *gate) return;
-    const int b = blockIdx.x;
+    const int b = blockIdx.x / kP1Rows, k = blockIdx.x % kP1Rows, rr = threadIdx.x;
     if (b >= nblk) return;
     const MBlk1 B = blks[b];
-    for (int q = threadIdx.x; q < B.kt * kRR; q += blockDim.x) {
-        const int k = q / kRR, rr = q % kRR;
-        double s = 0.0;
-        for (int p = B.p0; p < B.p1; p++) s += zpart[((size_t)p * kP1Rows + B.roff + k) * kRR + rr];
-        Z[(size_t)(B.zoff + k) * kRR + rr] = (OT)s;
-    }
+    if (k >= B.kt) return;
+    double s = 0.0;
+    for (int p = B.p0; p < B.p1; p++) s += zpart[((size_t)p * kP1Rows + B.roff + k) * kRR + rr];
+    Z[(size_t)(B.zoff + k) * kRR + rr] = (OT)s;
 }
 
 // ------------------------------------------------------------- phase 2 --
@@ -695,19 +702,7 @@ static void mr_items1(const Plan &P, MrPlan &M, int kc1)
                 it.off = grp[g0].off; it.ldt = f.ldt; it.kt = kt; it.t0 = t0; it.tn = std::min(kc1, f.nc - t0);
                 it.xoff = f.xoff; it.part = direct ? -1 : M.nzpart++; it.dtype = f.dtype; it.zr = grp[g0].sr;
                 it.r0 = 0;
-                if (f.dtype == kF64 || kt <= 32) {
-                    M.it1.push_back(it);
-                } else {
-                    // FP32-core items: rows 0..31 and 32..kt of the slot
-                    MItem1 lo = it, hi = it;
-                    lo.kt = 32;
-                    hi.kt = kt - 32;
-                    hi.r0 = 32;
-                    hi.zr = it.zr + 32;
-                    hi.off = it.off + (uint64_t)32 * f.ldt * dtype_size(f.dtype);
-                    M.it1.push_back(lo);
-                    M.it1.push_back(hi);
-                }
+                M.it1.push_back(it);
             }
             if (!direct) {
                 int roff = 0;
@@ -862,13 +857,14 @@ static void phase1(hbi_hmat *h, const Plan &P, const MrPlan &M, const double *X,
     if (!M.f32) {
         mr_lrvt_kernel<<<(int)M.it1.size(), 64, 0, s>>>(M.d_it1.p, P.buf.p, X, zpart, P.d_zrow.p, Z, gate);
         if (!M.blk1.empty())
-            mr_zreduce_kernel<double><<<(int)M.blk1.size(), 256, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(), zpart, Z, gate);
+            mr_zreduce_kernel<double><<<(int)M.blk1.size() * kP1Rows, kRR, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(), zpart,
+                                                                                  Z, gate);
     } else {
         float *Z32 = (float *)Z;
-        mr_lrvt32_kernel<<<(int)M.it1.size(), 64, 0, s>>>(M.d_it1.p, P.buf.p, X32, zpart, P.d_zrow.p, Z32, gate);
+        mr_lrvt32_kernel<<<(int)M.it1.size(), 128, 0, s>>>(M.d_it1.p, P.buf.p, X32, zpart, P.d_zrow.p, Z32, gate);
         if (!M.blk1.empty())
-            mr_zreduce_kernel<float><<<(int)M.blk1.size(), 256, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(), zpart, Z32,
-                                                                        gate);
+            mr_zreduce_kernel<float><<<(int)M.blk1.size() * kP1Rows, kRR, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(), zpart,
+                                                                                 Z32, gate);
     }
 }
 
