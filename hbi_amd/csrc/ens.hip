@@ -50,7 +50,8 @@ enum { kPa, kPb, kPdc, kPf0, kPvpl, kPtaudot, kPsigdot, kNPar };
 struct EnsArrays {
     const double *a, *b, *dc, *f0, *vpl, *taudot, *sigdot;
     int shared;
-    double *y, *ytmp, *yout, *k, *vel, *x, *slip, *velc, *errpart, *monpart;
+    double *y, *yout, *k, *vel, *x, *slip, *velc, *errpart, *monpart;
+    float *x32;           // FP32 copy of x for an FP32-core product (written instead of x), else null
     const double *ypart;
     const int *part0;
     int *bins;            // kR x kBins histograms of log10 |V| after an accepted step
@@ -72,12 +73,33 @@ __global__ void ens_stage0_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArr
     const int r = threadIdx.x % kR, i = blockIdx.x * 4 + threadIdx.x / kR;
     if (i >= P.n) return;
     const size_t q = (size_t)i * kR + r;
-    if (r >= P.nreal) { A.x[q] = 0.0; return; }
+    if (r >= P.nreal) {
+        if (A.x32) A.x32[q] = 0.f;
+        else A.x[q] = 0.0;
+        return;
+    }
     const double *y = A.y;
     const double v = slip_rate(y[(3 * (size_t)i) * kR + r], y[(3 * (size_t)i + 1) * kR + r],
                                  y[(3 * (size_t)i + 2) * kR + r], prm(A, A.a, kPa, i, r), P.vref);
     A.vel[q] = v;
-    A.x[q] = P.backslip ? v - prm(A, A.vpl, kPvpl, i, r) : v;
+    const double xv = P.backslip ? v - prm(A, A.vpl, kPvpl, i, r) : v;
+    if (A.x32) A.x32[q] = (float)xv;
+    else A.x[q] = xv;
+}
+
+// State of Cash-Karp stage t (1..5) from y and the slopes kv[0 .. t-1]
+// (main_LH.f90:2501-2536); the stage that forms it and the stage that uses
+// it evaluate this same expression, so both see the same bits.
+__device__ __forceinline__ double stage_state(const EnsPar &P, int t, double h, double y, const double *kv)
+{
+#pragma clang fp contract(off)
+    switch (t) {
+    case 1: return y + P.T.b21 * h * kv[0];
+    case 2: return y + h * (P.T.b31 * kv[0] + P.T.b32 * kv[1]);
+    case 3: return y + h * (P.T.b41 * kv[0] + P.T.b42 * kv[1] + P.T.b43 * kv[2]);
+    case 4: return y + h * (P.T.b51 * kv[0] + P.T.b52 * kv[1] + P.T.b53 * kv[2] + P.T.b54 * kv[3]);
+    default: return y + h * (P.T.b61 * kv[0] + P.T.b62 * kv[1] + P.T.b63 * kv[2] + P.T.b64 * kv[3] + P.T.b65 * kv[4]);
+    }
 }
 
 struct EPair {
@@ -90,7 +112,8 @@ __device__ __forceinline__ EPair e_combine(EPair l, EPair r)
     return EPair{l.g, fmax(l.v, r.v)};
 }
 
-__global__ __launch_bounds__(256) void ens_stage_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A, int s)
+template <int s>
+__global__ __launch_bounds__(256) void ens_stage_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A)
 {
 #pragma clang fp contract(off)
     __shared__ int sg[4][kR];
@@ -107,44 +130,46 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(EnsPar P, const EnsCtl *
         double gn = 0.0;
         gn = gn + prm(A, A.sigdot, kPsigdot, i, r);
         gs = gs + prm(A, A.taudot, kPtaudot, i, r);
-        const double *ys = (s == 0) ? A.y : A.ytmp;
         const size_t e0 = (3 * (size_t)i) * kR + r;
+        const size_t m3 = 3 * (size_t)P.n * kR;
+        const double h = c->h[r];
+        // y and the earlier slopes, read once; the stage state is formed from them
+        // again rather than stored by the previous stage and read back
+        double yv[3], kv[3][6], ys[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+            const size_t e = e0 + cc * kR;
+            yv[cc] = A.y[e];
+#pragma unroll
+            for (int j = 0; j < s; j++) kv[cc][j] = A.k[j * m3 + e];
+            ys[cc] = s == 0 ? yv[cc] : stage_state(P, s, h, yv[cc], kv[cc]);
+        }
         double kk[3];
         const double ai = prm(A, A.a, kPa, i, r);
-        deriv(P, gs, gn, ys[e0], ys[e0 + kR], ys[e0 + 2 * kR], A.vel[q], ai, prm(A, A.b, kPb, i, r),
-              prm(A, A.dc, kPdc, i, r), prm(A, A.f0, kPf0, i, r), kk);
-        const size_t m3 = 3 * (size_t)P.n * kR;
+        deriv(P, gs, gn, ys[0], ys[1], ys[2], A.vel[q], ai, prm(A, A.b, kPb, i, r), prm(A, A.dc, kPdc, i, r),
+              prm(A, A.f0, kPf0, i, r), kk);
         double *ks = A.k + s * m3;
-        ks[e0] = kk[0];
-        ks[e0 + kR] = kk[1];
-        ks[e0 + 2 * kR] = kk[2];
-        const double h = c->h[r];
-        const double *k1 = A.k, *k2 = A.k + m3, *k3 = A.k + 2 * m3, *k4 = A.k + 3 * m3, *k5 = A.k + 4 * m3,
-                     *k6 = A.k + 5 * m3;
-        if (s < 5) {
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+            ks[e0 + cc * kR] = kk[cc];
+            kv[cc][s] = kk[cc];
+        }
+        if constexpr (s < 5) {
             double yt[3];
-            for (int cc = 0; cc < 3; cc++) {
-                const size_t e = e0 + cc * kR;
-                const double y = A.y[e];
-                switch (s) {
-                case 0: yt[cc] = y + P.T.b21 * h * k1[e]; break;
-                case 1: yt[cc] = y + h * (P.T.b31 * k1[e] + P.T.b32 * k2[e]); break;
-                case 2: yt[cc] = y + h * (P.T.b41 * k1[e] + P.T.b42 * k2[e] + P.T.b43 * k3[e]); break;
-                case 3: yt[cc] = y + h * (P.T.b51 * k1[e] + P.T.b52 * k2[e] + P.T.b53 * k3[e] + P.T.b54 * k4[e]); break;
-                default:
-                    yt[cc] = y + h * (P.T.b61 * k1[e] + P.T.b62 * k2[e] + P.T.b63 * k3[e] + P.T.b64 * k4[e] + P.T.b65 * k5[e]);
-                    break;
-                }
-                A.ytmp[e] = yt[cc];
-            }
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) yt[cc] = stage_state(P, s + 1, h, yv[cc], kv[cc]);
             const double v = slip_rate(yt[0], yt[1], yt[2], ai, P.vref);
             A.vel[q] = v;
-            A.x[q] = P.backslip ? v - prm(A, A.vpl, kPvpl, i, r) : v;
+            const double xv = P.backslip ? v - prm(A, A.vpl, kPvpl, i, r) : v;
+            if (A.x32) A.x32[q] = (float)xv;
+            else A.x[q] = xv;
         } else {
+#pragma unroll
             for (int cc = 0; cc < 3; cc++) {
+                const double *kc = kv[cc];
+                const double yo = yv[cc] + h * (P.T.c1 * kc[0] + P.T.c3 * kc[2] + P.T.c4 * kc[3] + P.T.c6 * kc[5]);
+                const double ye = h * (P.T.dc1 * kc[0] + P.T.dc3 * kc[2] + P.T.dc4 * kc[3] + P.T.dc5 * kc[4] + P.T.dc6 * kc[5]);
                 const size_t e = e0 + cc * kR;
-                const double yo = A.y[e] + h * (P.T.c1 * k1[e] + P.T.c3 * k3[e] + P.T.c4 * k4[e] + P.T.c6 * k6[e]);
-                const double ye = h * (P.T.dc1 * k1[e] + P.T.dc3 * k3[e] + P.T.dc4 * k4[e] + P.T.dc5 * k5[e] + P.T.dc6 * k6[e]);
                 A.yout[e] = yo;
                 const double er = fabs(ye / yo) / P.eps;
                 EPair cur{0, er > 0.0 ? er : 0.0};
@@ -153,7 +178,7 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(EnsPar P, const EnsCtl *
             }
         }
     }
-    if (s != 5) return;
+    if constexpr (s != 5) return;
     sg[li][r] = ep.g;
     sv[li][r] = ep.v;
     __syncthreads();
@@ -394,8 +419,9 @@ struct hbi_ens {
     int n = 0, nreal = 0, nblk = 0;
     int shared = 0;   // arrays kept once per element (bit k: EnsArrays order)
     hbi::DevBuf<double> a, b, dc, f0, vpl, taudot, sigdot;
-    hbi::DevBuf<double> y, ytmp, yout, k, vel, x, slip, velc, errpart, monpart, z, zpart, ypart;
+    hbi::DevBuf<double> y, yout, k, vel, x, slip, velc, errpart, monpart, z, zpart, ypart;
     hbi::DevBuf<float> x32;      // FP32 copy of x for the FP32-core product
+    bool x32_direct = false;     // the stage kernels write x32 themselves (FP32-core product)
     hbi::DevBuf<double> fold;   // first-level reduction partials (kFold x 2 x 64)
     hbi::DevBuf<double> hist;   // nreal-major rows of kHistCols (graph argument)
     long long hist_rows = 0;
@@ -417,8 +443,9 @@ static EnsArrays e_arrays(hbi_ens *E)
     EnsArrays A{};
     A.shared = E->shared;
     A.a = E->a.p; A.b = E->b.p; A.dc = E->dc.p; A.f0 = E->f0.p; A.vpl = E->vpl.p; A.taudot = E->taudot.p;
-    A.sigdot = E->sigdot.p; A.y = E->y.p; A.ytmp = E->ytmp.p; A.yout = E->yout.p; A.k = E->k.p; A.vel = E->vel.p;
-    A.x = E->x.p; A.slip = E->slip.p; A.velc = E->velc.p; A.errpart = E->errpart.p; A.monpart = E->monpart.p;
+    A.sigdot = E->sigdot.p; A.y = E->y.p; A.yout = E->yout.p; A.k = E->k.p; A.vel = E->vel.p;
+    A.x = E->x.p;
+    A.x32 = E->x32_direct ? E->x32.p : nullptr; A.slip = E->slip.p; A.velc = E->velc.p; A.errpart = E->errpart.p; A.monpart = E->monpart.p;
     A.ypart = E->ypart.p;
     A.part0 = mr_part0(E->hs, E->opts.prec);
     A.bins = E->bins.p;
@@ -434,9 +461,13 @@ static void e_attempt(hbi_ens *E)
     EnsArrays A = e_arrays(E);
     const int *gate = &E->ctl.p->done;
     ens_stage0_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
+    using StageFn = void (*)(EnsPar, const EnsCtl *, EnsArrays);
+    static const StageFn stage[6] = {ens_stage_kernel<0>, ens_stage_kernel<1>, ens_stage_kernel<2>,
+                                     ens_stage_kernel<3>, ens_stage_kernel<4>, ens_stage_kernel<5>};
     for (int s = 0; s < 6; s++) {
-        mr_apply_parts(E->hs, E->opts.prec, E->x.p, E->x32.p, E->zpart.p, E->z.p, E->ypart.p, E->stream, gate);
-        ens_stage_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A, s);
+        mr_apply_parts(E->hs, E->opts.prec, E->x32_direct ? nullptr : E->x.p, E->x32.p, E->zpart.p, E->z.p,
+                       E->ypart.p, E->stream, gate);
+        stage[s]<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
     }
     ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->errpart.p, g, E->fold.p, 0);
     ens_decide_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold);
@@ -517,7 +548,7 @@ int hbi_ens_create(hbi_hmat *hs, const hbi_rk_opts *opts, int nreal, const doubl
                 }
             }
             const size_t m = (size_t)n * hbi::kR;
-            E->y.alloc(3 * m); E->ytmp.alloc(3 * m); E->yout.alloc(3 * m); E->k.alloc(18 * m);
+            E->y.alloc(3 * m); E->yout.alloc(3 * m); E->k.alloc(18 * m);
             E->vel.alloc(m); E->x.alloc(m); E->slip.alloc(m); E->velc.alloc(m);
             E->errpart.alloc(2 * (size_t)E->nblk * hbi::kR);
             E->monpart.alloc((size_t)E->nblk * hbi::kR);
@@ -530,8 +561,10 @@ int hbi_ens_create(hbi_hmat *hs, const hbi_rk_opts *opts, int nreal, const doubl
             E->z.alloc((size_t)std::max(hs->nz, 1) * hbi::kR);
             E->zpart.alloc(hbi::mr_zpart_size(hs, opts->prec));
             E->x32.alloc(hbi::mr_x32_size(hs, opts->prec));
+            E->x32_direct = hbi::mr_x32_size(hs, opts->prec) > 1;
+            if (E->x32_direct) HBI_HIP(hipMemsetAsync(E->x32.p, 0, E->x32.n * sizeof(float), E->stream));
             E->ypart.alloc(hbi::mr_ypart_size(hs, opts->prec));
-            for (auto *d : {&E->y, &E->ytmp, &E->yout, &E->k, &E->vel, &E->x, &E->slip, &E->velc})
+            for (auto *d : {&E->y, &E->yout, &E->k, &E->vel, &E->x, &E->slip, &E->velc})
                 HBI_HIP(hipMemsetAsync(d->p, 0, d->n * 8, E->stream));
             E->ctl.alloc(1);
             hbi::EnsCtl c{};

@@ -849,7 +849,7 @@ const int *mr_part0(hbi_hmat *h, int prec) { return mr_plan(h, prec).d_part0.p; 
 static void phase1(hbi_hmat *h, const Plan &P, const MrPlan &M, const double *X, float *X32, double *zpart,
                    double *Z, hipStream_t s, const int *gate)
 {
-    if (M.f32) {
+    if (M.f32 && X) {   // X null: the caller wrote X32 itself
         const size_t n4 = (size_t)h->n * kRR / 4;
         mr_narrow_kernel<<<(int)((n4 + 255) / 256), 256, 0, s>>>(n4, X, X32, gate);
     }
@@ -904,7 +904,8 @@ void mr_phase2(hbi_hmat *h, int prec, const double *X, const float *X32, const d
 }
 
 // Y partials of H X (device pointers, row-major N x 64); Y itself through mr_reduce or a consumer.
-// X32 (mr_x32_size floats) is scratch for the FP32 copy of X.
+// X32 (mr_x32_size floats) is scratch for the FP32 copy of X; with the FP32
+// cores a caller may fill X32 itself and pass X = null.
 void mr_apply_parts(hbi_hmat *h, int prec, const double *X, float *X32, double *zpart, double *Z, double *ypart,
                     hipStream_t s, const int *gate)
 {
