@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kDecThreads) void ens_decide_kernel(EnsPar P, EnsCt
 // (lane = realisation); the slip-rate histogram of the learned step size is
 // filled with run-length aggregated atomics (neighbouring elements mostly
 // share a bin).
-constexpr int kCommitRun = 16, kCommitPerBlock = 4 * kCommitRun;
+constexpr int kCommitRun = 4, kCommitPerBlock = 4 * kCommitRun;
 
 __global__ __launch_bounds__(256) void ens_commit_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A)
 {
@@ -295,35 +295,71 @@ __global__ __launch_bounds__(256) void ens_commit_kernel(EnsPar P, const EnsCtl 
     double vmax = 0.0;
     if (r < P.nreal && c->acc[r]) {
         const double dt = c->hdid[r];
-        int cur_b = -1, cur_n = 0;
+        // distinct arrays: the loads of an element need not wait for the stores of the previous one
+        double *__restrict__ y = A.y, *__restrict__ slip = A.slip, *__restrict__ velc = A.velc;
+        const double *__restrict__ yout = A.yout;
         for (int k = 0; k < kCommitRun; k++) {
             const int i = i0 + k;
             if (i >= P.n) break;
             const size_t q = (size_t)i * kR + r, e0 = (3 * (size_t)i) * kR + r;
-            for (int cc = 0; cc < 3; cc++) A.y[e0 + cc * kR] = A.yout[e0 + cc * kR];
-            double sl = A.slip[q];
-            sl = sl + A.velc[q] * dt * 0.5;
-            const double v = slip_rate(A.y[e0], A.y[e0 + kR], A.y[e0 + 2 * kR], prm(A, A.a, kPa, i, r), P.vref);
-            sl = sl + v * dt * 0.5;
-            A.slip[q] = sl;
-            A.velc[q] = v;
-            vmax = fmax(vmax, fabs(v));
-            if (P.learn) {
-                const int bb = vbin(fabs(v));
-                if (bb == cur_b) {
-                    cur_n++;
-                } else {
-                    if (cur_n) atomicAdd(&A.bins[(size_t)r * kBins + cur_b], cur_n);
-                    cur_b = bb;
-                    cur_n = 1;
-                }
+            double yv[3];
+            for (int cc = 0; cc < 3; cc++) {
+                yv[cc] = yout[e0 + cc * kR];
+                y[e0 + cc * kR] = yv[cc];
             }
+            double sl = slip[q];
+            sl = sl + velc[q] * dt * 0.5;
+            const double v = slip_rate(yv[0], yv[1], yv[2], prm(A, A.a, kPa, i, r), P.vref);
+            sl = sl + v * dt * 0.5;
+            slip[q] = sl;
+            velc[q] = v;
+            vmax = fmax(vmax, fabs(v));
         }
-        if (cur_n) atomicAdd(&A.bins[(size_t)r * kBins + cur_b], cur_n);
     }
     vm[w][r] = vmax;
     __syncthreads();
     if (w == 0) A.monpart[(size_t)blockIdx.x * kR + r] = fmax(fmax(vm[0][r], vm[1][r]), fmax(vm[2][r], vm[3][r]));
+}
+
+// log10 |V| histograms of the accepted realisations (the learned controller's
+// features): workgroup (g, m) counts realisations 4 g .. 4 g + 3 over element
+// chunk m of kHistChunks into LDS -- each thread a contiguous run of elements
+// of one realisation, counted as runs of equal bins (most elements of a fault
+// share a few bins) -- then adds its nonzero bins to the global histograms (at
+// most kHistChunks adds per address)
+constexpr int kHistR = 4, kHistChunks = 16;
+
+__global__ __launch_bounds__(256) void ens_hist_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A)
+{
+    __shared__ int hl[kHistR][kBins];
+    if (c->done) return;
+    const int g = blockIdx.x / kHistChunks, m = blockIdx.x % kHistChunks;
+    const int rl = threadIdx.x % kHistR, r = kHistR * g + rl;
+    for (int b = threadIdx.x; b < kHistR * kBins; b += blockDim.x) hl[b / kBins][b % kBins] = 0;
+    __syncthreads();
+    const int per = (P.n + kHistChunks - 1) / kHistChunks;
+    const int lanes = (int)blockDim.x / kHistR, run = (per + lanes - 1) / lanes;
+    const int i0 = m * per + ((int)threadIdx.x / kHistR) * run;
+    const int i1 = min(min(P.n, (m + 1) * per), i0 + run);
+    if (r < P.nreal && c->acc[r]) {
+        int cur_b = -1, cur_n = 0;
+        for (int i = i0; i < i1; i++) {
+            const int b = vbin(fabs(A.velc[(size_t)i * kR + r]));
+            if (b == cur_b) {
+                cur_n++;
+            } else {
+                if (cur_n) atomicAdd(&hl[rl][cur_b], cur_n);
+                cur_b = b;
+                cur_n = 1;
+            }
+        }
+        if (cur_n) atomicAdd(&hl[rl][cur_b], cur_n);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kHistR * kBins; b += blockDim.x) {
+        const int rr = kHistR * g + b / kBins, n = hl[b / kBins][b % kBins];
+        if (n) atomicAdd(&A.bins[(size_t)rr * kBins + b % kBins], n);
+    }
 }
 
 // history row per accepted step of realisation r: (t, dt, errmax, rejections before it, log10 Vmax)
@@ -473,7 +509,10 @@ static void e_attempt(hbi_ens *E)
     ens_decide_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold);
     const int gc = (E->n + kCommitPerBlock - 1) / kCommitPerBlock;
     ens_commit_kernel<<<gc, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
-    if (E->P.learn) ens_learn_kernel<<<E->nreal, kHid, 0, E->stream>>>(E->P, E->ctl.p, A);
+    if (E->P.learn) {
+        ens_hist_kernel<<<(kR / kHistR) * kHistChunks, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
+        ens_learn_kernel<<<E->nreal, kHid, 0, E->stream>>>(E->P, E->ctl.p, A);
+    }
     ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->monpart.p, gc, E->fold.p, 1);
     ens_monitor_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold, E->hist.p);
 }
