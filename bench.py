@@ -43,7 +43,7 @@ FP32_MFMA_PEAK_TFS = 157.3
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE passes over
 # hbi_amd/profile_kernels.py, scaled by the counters of a 256 MiB device copy
 # (fetch x 2.00, write x 1.00): mr_tile32_kernel<float,64,4> and stream_kernel
-TRAFFIC_MR_TILE_MIXED = 436.3e6
+TRAFFIC_MR_TILE_MIXED = 420.6e6
 TRAFFIC_STREAM_MIXED = 225.7e6
 IMAX, JMAX, DS0, LEAF, EPS_H, EPS_R = 256, 128, 0.390625, 64, 1e-5, 1e-6
 NREAL = 64

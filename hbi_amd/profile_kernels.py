@@ -9,7 +9,9 @@ product of every precision and the 64-column product (FP64 and mixed) over
 `iters` calls each; the same kernels and work items the RK graphs launch.
 With --copy-mb > 0 it also runs a plain device-to-device copy of that many
 MiB (HIP runtime blit kernel) `iters` times, the reference for scaling the
-profiler's byte counters.
+profiler's byte counters.  With --ensemble K it instead runs the bench's
+ensemble (64 friction realisations, learned step size, FP32 factors) for 20
+warm-up and K timed lockstep steps, for a kernel trace of the attempt.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -29,9 +31,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--copy-mb", type=int, default=0)
+    ap.add_argument("--ensemble", type=int, default=0)
     a = ap.parse_args()
     g = P.mesh_3dph(IMAX, JMAX, DS0, 90.0)
     Hm = H.HMatrix(g, leaf=LEAF, eps=EPS_H, bf16_margin=1.0)
+    if a.ensemble > 0:
+        print(json.dumps(ensemble(g, Hm, a.ensemble)), flush=True)
+        return
     st = Hm.stats
     lv = Hm.leaves
     dense, lr = lv[:, 0] != 1, lv[:, 0] == 1
@@ -46,6 +52,27 @@ def main():
     if a.copy_mb > 0:
         out["copy"] = device_copy(a.copy_mb * (1 << 20), a.iters)
     print(json.dumps(out), flush=True)
+
+
+def ensemble(g, Hm, steps: int) -> dict:
+    """The bench's learned-step ensemble: 20 warm-up, then `steps` lockstep steps."""
+    import os
+    from . import learned as L
+    lod = Hm.lod
+    frs = [P.friction_c2(g, IMAX, JMAX, DS0, seed=1 + r) for r in range(64)]
+    ys = []
+    for f in frs:
+        y = P.init_state(f, lod)
+        P.perturb_state(y, g, lod, amp=0.02, depth=15.0, width=3.0)
+        ys.append(y)
+    E = H.Ensemble(Hm, frs, lod, hdiv=DS0, eps_r=1e-6, prec=H.PREC_MIXED)
+    E.set_controller(1, L.load(os.path.join(os.path.dirname(__file__), "data", "step_mlp_c2.npz")), 0.9)
+    E.set_state(np.array(ys), 0.0, 1.0)
+    E.step(20)
+    att = E.step(steps)
+    out = {"steps": steps, "attempts": int(att), "device_ms": E.last_ms()}
+    E.close()
+    return out
 
 
 def device_copy(nbytes: int, iters: int) -> dict:
