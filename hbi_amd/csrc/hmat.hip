@@ -399,6 +399,10 @@ template <> struct Raw<uint16_t> {
 
 constexpr int kMvThreads = 256;
 
+// zeros read in place of masked factor pieces and x values (never written):
+// masked loads choose an address, not a value, so no select waits on a load
+static __device__ double g_mv_zero[8];
+
 // phase 1: z rows of one stacked row tile = (tile, tr x ncol, column-major)
 // times x[xoff ..].  One workgroup per item; lane l owns V rows (16 bytes) of
 // the column csub = l / (tr / V) of every column group, so a wave load covers
@@ -449,8 +453,9 @@ __device__ __forceinline__ double stack_rows(const Item1 &it, const uint8_t *__r
 
 // phase 2: partial y of one work item (one workgroup).  A lane owns V rows
 // of the TR-row tile and streams columns; U column groups are loaded before
-// they are used so U 16-byte loads per lane are in flight.
-template <typename T, int TR, int U>
+// they are used so U 16-byte loads per lane are in flight.  XG: xin is in
+// global memory (masked lanes then read the zero buffer); else in LDS.
+template <typename T, int TR, int U, bool XG = true>
 __device__ __forceinline__ void tile_seg(const Seg &sg, const uint8_t *__restrict__ buf, const double *__restrict__ xin,
                                          int rsub, int csub, int w, double (&acc)[Raw<T>::V])
 {
@@ -468,8 +473,9 @@ __device__ __forceinline__ void tile_seg(const Seg &sg, const uint8_t *__restric
             const int g = g0 + u * NW;
             const int c = g * CPI + csub;
             const bool ok = g < ngroups && c < sg.ncol;
-            a[u] = ok ? Raw<T>::load(base + (size_t)c * TR * sizeof(T)) : Raw<T>::zero();
-            xv[u] = ok ? xin[c] : 0.0;
+            a[u] = Raw<T>::load(ok ? base + (size_t)c * TR * sizeof(T) : (const uint8_t *)g_mv_zero);
+            if constexpr (XG) xv[u] = *(ok ? xin + c : (const double *)g_mv_zero);
+            else xv[u] = ok ? xin[c] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; u++)
@@ -579,7 +585,7 @@ __device__ __forceinline__ double lowrank_rows(uint64_t off, int ncol, const int
         Seg sg{};
         sg.off = off + (uint64_t)c0 * TR * sizeof(T);
         sg.ncol = nb;
-        tile_seg<T, TR, U>(sg, buf, zl, rsub, csub, w, acc);
+        tile_seg<T, TR, U, false>(sg, buf, zl, rsub, csub, w, acc);
         __syncthreads();
     }
 #pragma unroll
