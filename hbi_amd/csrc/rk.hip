@@ -60,6 +60,9 @@ struct RkArrays {
     double *hist, *vhist;
     const int *rec;
     int nrec;
+    double *probe;        // per accepted step, kRkProbe values for each probed element
+    const int *probe_idx;
+    int nprobe;
     int *bins;            // histogram of log10 |V| over the elements after an accepted step
     double *trace;        // rows of 12 features + (h, errmax)
     int trace_cap;
@@ -80,6 +83,11 @@ constexpr int kMon = 6;
 // t, dt, errmax, rejected attempts before the step, log10 max |V|, mean |slip|,
 // mean tau / sigma (area weighted), max sigma, min sigma
 constexpr int kRkHist = 9;
+// probe row per accepted step and probed element (the local output of
+// main_LH.f90:1495-1499): V, slip, tau, sigma, mu = tau / sigma, psi, veln, slipn
+// (veln and slipn 0 without opening)
+constexpr int kRkProbe = 8;
+constexpr int kRkMaxProbes = 16;
 
 
 __global__ void rk_arm_kernel(RkCtl *c, long long nsteps, double h_override, int set_h)
@@ -418,6 +426,16 @@ __global__ void rk_monitor_kernel(RkPar P, RkCtl *c, RkArrays A, int nblk)
         }
         if (A.vhist)
             for (int r = threadIdx.x; r < A.nrec; r += blockDim.x) A.vhist[row * A.nrec + r] = A.velc[A.rec[r]];
+        if (A.probe)
+            for (int q = threadIdx.x; q < A.nprobe; q += blockDim.x) {
+                const size_t i = A.probe_idx[q];
+                const double tau = A.y[3 * i + 1], sig = A.y[3 * i + 2];
+                double *pr = A.probe + ((size_t)row * A.nprobe + q) * kRkProbe;
+                pr[0] = A.velc[i]; pr[1] = A.slip[i]; pr[2] = tau; pr[3] = sig; pr[4] = tau / sig;
+                pr[5] = A.y[3 * i];
+                pr[6] = P.has_open ? A.veln[i] : 0.0;
+                pr[7] = P.has_open ? A.slipn[i] : 0.0;
+            }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -468,6 +486,9 @@ struct hbi_rk {
     hbi::DevBuf<double> y, ytmp, yout, yerr, k, vel, x, slip, velc, z, zn, ypart, ypart_n, hx, hx_n, errpart, monpart;
     hbi::DevBuf<double> hist, vhist;
     hbi::DevBuf<int> rec;
+    hbi::DevBuf<double> probe;                 // probe rows of the last step call (hbi_rk_set_probes)
+    hbi::DevBuf<int> probe_idx;
+    long long probe_rows = 0;
     hbi::DevBuf<hbi::RkCtl> ctl;
     hbi::DevBuf<int> bins;
     hbi::DevBuf<double> trace;
@@ -509,6 +530,9 @@ static RkArrays arrays(hbi_rk *r)
     A.vhist = r->vhist.n ? r->vhist.p : nullptr;
     A.rec = r->rec.p;
     A.nrec = (int)r->rec.n;
+    A.probe = r->probe.n ? r->probe.p : nullptr;
+    A.probe_idx = r->probe_idx.p;
+    A.nprobe = (int)r->probe_idx.n;
     return A;
 }
 
@@ -758,6 +782,12 @@ static void step_prepare(hbi_rk *r, int nsteps, bool hist, int nrec, const int *
         recapture = true;
     }
     if (nrec) r->rec.upload(rec, nrec, r->stream);
+    const size_t pneed = (size_t)nsteps * r->probe_idx.n * kRkProbe;
+    if (pneed > r->probe.n || (pneed == 0 && r->probe.n)) {
+        if (pneed) r->probe.alloc(pneed); else r->probe.release();
+        recapture = true;
+    }
+    r->probe_rows = 0;
     if (r->dirty) recapture = true;
     if (recapture) capture(r);
     r->dirty = false;
@@ -773,6 +803,7 @@ static void step_finish(hbi_rk *r, const RkCtl &c, hbi_rk_info *info)
     HBI_HIP(hipEventElapsedTime(&ms, r->e0, r->e1));
     r->last_ms = ms;
     r->out_full = c.out_full != 0;
+    r->probe_rows = std::max<long long>(0, std::min(c.accepted - c.hist_base, c.hist_cap));
     if (info) {
         info->t = c.t; info->dt_next = c.h; info->dt_last = c.hdid; info->errmax_last = c.errmax;
         info->attempts = c.attempts; info->accepted = c.accepted; info->rejected = c.rejected;
@@ -925,6 +956,35 @@ int hbi_rk_get_opening(hbi_rk *r, double *veln, double *slipn)
         if (veln) r->veln.download(veln, r->n, r->stream);
         if (slipn) r->slipn.download(slipn, r->n, r->stream);
         HBI_HIP(hipStreamSynchronize(r->stream));
+    });
+}
+
+int hbi_rk_set_probes(hbi_rk *r, int nprobe, const int *idx)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && nprobe >= 0 && nprobe <= hbi::kRkMaxProbes && (nprobe == 0 || idx), "bad arguments");
+        for (int q = 0; q < nprobe; q++) HBI_REQUIRE(idx[q] >= 0 && idx[q] < r->n, "probe index out of range");
+        if (nprobe) {
+            r->probe_idx.alloc(nprobe);
+            r->probe_idx.upload(idx, nprobe, r->stream);
+        } else {
+            r->probe_idx.release();
+        }
+        r->probe.release();
+        r->probe_rows = 0;
+        r->dirty = true;
+    });
+}
+
+int hbi_rk_get_probes(hbi_rk *r, long long *nrows, double *rows)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(r && nrows, "NULL argument");
+        *nrows = r->probe_rows;
+        if (rows && r->probe_rows > 0) {
+            r->probe.download(rows, (size_t)r->probe_rows * r->probe_idx.n * hbi::kRkProbe, r->stream);
+            HBI_HIP(hipStreamSynchronize(r->stream));
+        }
     });
 }
 

@@ -12,7 +12,10 @@
 ! columns), time<n>.dat, event<n>.dat, the vel/slip/sigma/tau<n>.dat FP64
 ! streams (cluster order) and EQslip<n>.dat (main_LH.f90:1339-1407,
 ! 1518-1631, events and snapshots decided on the device), xyz/ind<n>.dat
-! (output_coord), and the final `time(s)` line.  `restart T` continues a run
+! (output_coord), local<n>-<id>.dat for each `outloc id` key (at most 10,
+! 1-based mesh-order ids; main_LH.f90:322-324, 1099-1122, 1495-1499, from
+! probe rows the device records after every step), and the final `time(s)`
+! line.  `restart T` continues a run
 ! from the snapshot of the last interval step in its monitor file (found
 ! through time<n>.dat, main_LH.f90:836-957).  `sigmaconst F` adds the
 ! normal-stress matrix and `opening T` the two tensile-dislocation matrices
@@ -60,6 +63,10 @@ program lhbiem_gpu
   type(c_ptr) :: rk
   integer(c_int) :: rc, nrec
   integer :: lrtrn, i_, kdone
+  integer :: noutloc, locid(10), nprobe, punit(10)
+  integer(c_int) :: probe_idx(10)
+  integer(c_long_long) :: nprow
+  real(8), allocatable :: prow(:)
   integer(8) :: c0, c1, crate
   real(8) :: mu_avg
 
@@ -78,6 +85,7 @@ program lhbiem_gpu
   precision = 'f64'; bf16margin = 1d0; outbatch = 200
   interval = 0; velth = 1e-2; dtout = 365 * 24 * 3600; dtout_co = 1000.0; restart = .false.; opening = .false.
   fwid = 1e8
+  noutloc = 0; locid = 0
   a0 = 0.01d0; b0 = 0.015d0; dc0 = 0.01d0; mu0 = 0.6d0; vpl = 1d-9; velinit = 1d-9; tauinit = 0d0; sigmainit = 50d0
   imax = 0; jmax = 0; ncellg = 0; ds0 = 0d0; problem = ' '
   do
@@ -132,6 +140,13 @@ program lhbiem_gpu
     case ('restart'); read(pvalue, *) restart
     case ('opening'); read(pvalue, *) opening
     case ('fwid'); read(pvalue, *) fwid
+    case ('outloc')
+      if (noutloc == 10) then
+        write(*, *) 'ERROR: more than 10 outloc keys'
+        stop 1
+      end if
+      noutloc = noutloc + 1
+      read(pvalue, *) locid(noutloc)
     case default
       write(*, *) 'WARNING: ', trim(param), ' is an unknown parameter'
     end select
@@ -482,6 +497,25 @@ program lhbiem_gpu
     call open_stream(86, 'slipn', restart)
   end if
   if (.not. restart) call output_coord()
+  ! local output (main_LH.f90:1099-1122): each outloc id (mesh order) found
+  ! among the cells is probed at its cluster-order index; the file is opened
+  ! afresh, as there, also on restart (rows of steps k > kend only)
+  nprobe = 0
+  do k = 1, noutloc
+    do i = 1, ncellg
+      if (st_sum%lodc(i) == locid(k)) then
+        nprobe = nprobe + 1
+        probe_idx(nprobe) = i - 1
+        punit(nprobe) = 53 + k
+        write(*, *) xcol(locid(k)), ycol(locid(k)), zcol(locid(k))
+        write(fname, '("output/local",i0,"-",i0,".dat")') number, locid(k)
+        open(53 + k, file=fname, status='replace')
+        exit
+      end if
+    end do
+  end do
+  rc = hbi_rk_set_probes(rk, int(nprobe, c_int), probe_idx)
+  allocate(prow(8 * max(nprobe, 1) * outbatch))
   allocate(hist(9 * outbatch), vhist(1))
   nrec = 0
   nrec_dummy = 0
@@ -509,6 +543,7 @@ program lhbiem_gpu
           hist(9 * k - 2), hist(9 * k - 1), hist(9 * k), hist(9 * k - 6), hist(9 * k - 7), int(hist(9 * k - 5)), &
           real(c1 - c0, 8) / real(crate, 8)
     end do
+    if (nprobe > 0) call write_local()
     kdone = int(info%accepted) + kstart - 1
     call write_outputs()
     if (rc == HBI_E_STEP) then
@@ -535,6 +570,9 @@ program lhbiem_gpu
     close(85)
     close(86)
   end if
+  do k = 1, nprobe
+    close(punit(k))
+  end do
   rc = hbi_rk_get_state(rk, y, slip, vel, info)
   write(*, *) 'time(s)', t2 - t1, ' steps', kdone, ' rejected', info%rejected, ' t_end', info%t
   rc = hbi_rk_destroy(rk)
@@ -542,6 +580,22 @@ program lhbiem_gpu
   lrtrn = HACApK_finalize(st_ctl)
 
 contains
+
+  ! rows of output_local (main_LH.f90:1495-1499) for the steps of the last
+  ! batch: k, t, log10 V, slip, tau, sigma, mu, psi, vflow (0: no viscous
+  ! flow here), veln, slipn
+  subroutine write_local()
+    integer :: j, r, o
+    rc = hbi_rk_get_probes(rk, nprow, prow)
+    do r = 1, int(nprow)
+      if (kdone + r <= kend) cycle
+      do j = 1, nprobe
+        o = 8 * (nprobe * (r - 1) + j - 1)
+        write(punit(j), '(i7,f19.4,9e16.5)') kdone + r, hist(9 * r - 8), log10(prow(o + 1)), prow(o + 2), &
+          prow(o + 3), prow(o + 4), prow(o + 5), prow(o + 6), 0d0, prow(o + 7), prow(o + 8)
+      end do
+    end do
+  end subroutine write_local
 
   ! the decided outputs of the last batch: time rows and field snapshots
   ! (main_LH.f90:1410-1425, output_field), event rows and coseismic slip

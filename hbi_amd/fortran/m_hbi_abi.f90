@@ -116,6 +116,18 @@ module m_hbi_abi
       import :: c_int, c_ptr
       type(c_ptr), value :: r, hs2, hn2
     end function
+    integer(c_int) function hbi_rk_set_probes(r, nprobe, idx) bind(C, name="hbi_rk_set_probes")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: r
+      integer(c_int), value :: nprobe
+      integer(c_int), intent(in) :: idx(*)
+    end function
+    integer(c_int) function hbi_rk_get_probes(r, nrows, rows) bind(C, name="hbi_rk_get_probes")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: r
+      integer(c_long_long), intent(out) :: nrows
+      real(c_double), intent(out) :: rows(*)
+    end function
     integer(c_int) function hbi_rk_set_output(r, o) bind(C, name="hbi_rk_set_output")
       import :: c_int, c_ptr, hbi_rk_out_opts
       type(c_ptr), value :: r

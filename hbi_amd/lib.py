@@ -96,7 +96,7 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_mlp_set_params", "hbi_mlp_get_params", "hbi_mlp_forward", "hbi_mlp_loss_grad", "hbi_mlp_train",
            "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace", "hbi_ens_set_controller",
            "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_rk_step_many", "hbi_rk_set_output", "hbi_rk_get_output",
-           "hbi_rk_set_opening", "hbi_rk_get_opening")
+           "hbi_rk_set_opening", "hbi_rk_get_opening", "hbi_rk_set_probes", "hbi_rk_get_probes")
 
 
 def lib():
@@ -153,6 +153,8 @@ def lib():
         L.hbi_rk_set_output.argtypes = [C.c_void_p, C.POINTER(RkOutOpts)]
         L.hbi_rk_set_opening.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.hbi_rk_get_opening.argtypes = [C.c_void_p, _dp, _dp]
+        L.hbi_rk_set_probes.argtypes = [C.c_void_p, C.c_int, _ip]
+        L.hbi_rk_get_probes.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), _dp]
         L.hbi_rk_get_output.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), _dp, _dp,
                                         C.POINTER(C.c_int), C.POINTER(RkEvent), _dp]
         L.hbi_ens_get_trace.argtypes = [C.c_void_p, _dp, _dp, _ip]
@@ -391,6 +393,23 @@ class RK:
         vn, sn = np.empty(self.n), np.empty(self.n)
         _check(lib().hbi_rk_get_opening(self.r, _p(vn), _p(sn)))
         return vn, sn
+
+    def set_probes(self, idx):
+        """Per-element local output (main_LH.f90:1099-1122, 1495-1499): after
+        every accepted step of the following step calls, record V, slip, tau,
+        sigma, mu, psi, veln, slipn at up to 16 cluster-order indices."""
+        a = np.ascontiguousarray(idx, dtype=np.int32)
+        self._nprobe = int(a.size)
+        _check(lib().hbi_rk_set_probes(self.r, int(a.size), a.ctypes.data_as(_ip) if a.size else None))
+
+    def get_probes(self) -> np.ndarray:
+        """Probe rows of the last step call: (accepted steps, probes, 8)."""
+        nr = C.c_longlong(0)
+        _check(lib().hbi_rk_get_probes(self.r, C.byref(nr), None))
+        out = np.empty((nr.value, getattr(self, "_nprobe", 0), 8))
+        if out.size:
+            _check(lib().hbi_rk_get_probes(self.r, C.byref(nr), _p(out)))
+        return out
 
     def get_output(self) -> dict:
         """Drain the held output: snapshot steps, times and fields (vel, slip,

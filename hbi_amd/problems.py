@@ -448,9 +448,11 @@ def write_param_file(path: str, fr: Friction, columns=("a", "b", "dc", "f0", "ta
 
 
 def write_in_file(path: str, keys: Dict[str, object]) -> None:
-    """``key value`` input file parsed by main_LH.f90:182-328."""
+    """``key value`` input file parsed by main_LH.f90:182-328; a list value
+    repeats its key once per item (``outloc``, main_LH.f90:322-324)."""
     with open(path, "w") as f:
         for k, v in keys.items():
-            if isinstance(v, bool):
-                v = "T" if v else "F"
-            f.write("%s %s\n" % (k, v))
+            for item in (v if isinstance(v, (list, tuple)) else [v]):
+                if isinstance(item, bool):
+                    item = "T" if item else "F"
+                f.write("%s %s\n" % (k, item))

@@ -223,6 +223,18 @@ int hbi_rk_last_ms(const hbi_rk *r, double *ms);
 int hbi_rk_set_opening(hbi_rk *r, hbi_hmat *hs2, hbi_hmat *hn2);
 int hbi_rk_get_opening(hbi_rk *r, double *veln, double *slipn);
 
+/* Per-element local output (main_LH.f90:1099-1122, 1430-1434, output_local
+   at 1495-1499; the reference's 'outloc' ids mapped to cluster order through
+   lodc): hbi_rk_set_probes names up to 16 cluster-order elements (nprobe 0
+   clears them).  After every accepted step of the following step calls the
+   device records, for each probe p, 8 values: V, slip, tau, sigma, mu =
+   tau / sigma, psi, veln, slipn (veln, slipn 0 without opening).
+   hbi_rk_get_probes gives the rows of the last hbi_rk_step (or
+   hbi_rk_step_many) call: *nrows accepted steps x nprobe x 8, step-major;
+   rows may be NULL to query *nrows only. */
+int hbi_rk_set_probes(hbi_rk *r, int nprobe, const int *idx);
+int hbi_rk_get_probes(hbi_rk *r, long long *nrows, double *rows);
+
 /* Field snapshots and the event list, decided on the device after every
    accepted step k = k0 + accepted (main_LH.f90:1339-1407):
    - event detector: an event starts when max |V| > velth (onset time, the

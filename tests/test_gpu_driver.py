@@ -131,7 +131,9 @@ def test_gpu_driver_events_and_snapshots(tmp_path):
     n = 2048
     g, fr, lod, ds0 = _c1_nucleating(n)
     P.write_param_file(str(tmp_path / "param.dat"), fr)
-    _run(tmp_path, _keys(n, ds0, 2500))
+    # outloc (main_LH.f90:322-324): 1-based mesh ids; one that is no cell gets no file
+    loc_ids = [int(lod[7]) + 1, n // 2, n + 5]
+    _run(tmp_path, _keys(n, ds0, 2500, outloc=loc_ids))
     out = tmp_path / "output"
     trows = np.loadtxt(out / "time5.dat", ndmin=2)
     vel, slip = _stream(out / "vel5.dat", n), _stream(out / "slip5.dat", n)
@@ -167,6 +169,28 @@ def test_gpu_driver_events_and_snapshots(tmp_path):
     xyz = np.loadtxt(out / "xyz5.dat")
     ind = np.loadtxt(out / "ind5.dat", dtype=int)
     assert np.array_equal(ind, lod + 1)
+    # local<n>-<id>.dat (output_local, main_LH.f90:1495-1499): a row per step,
+    # k, t, log10 V, slip, tau, sigma, mu, psi, vflow, veln, slipn; at the
+    # snapshot steps the element's snapshot values (e16.5: 5 significant digits)
+    assert not (out / ("local5-%d.dat" % (n + 5))).exists()
+    for mid in loc_ids[:2]:
+        loc = np.loadtxt(out / ("local5-%d.dat" % mid))
+        i = int(np.where(lod == mid - 1)[0][0])
+        assert np.array_equal(loc[:, 0], np.arange(1, 2501))
+        assert np.array_equal(loc[:, 1], mon[1:2501, 1])
+        assert (loc[:, 8] == 0).all() and (loc[:, 9] == 0).all() and (loc[:, 10] == 0).all()
+        for j, k in enumerate(ks):
+            r = loc[k - 1]
+            assert abs(r[2] - np.log10(vel[j, i])) <= 5.01e-5 * abs(r[2]) + 1e-12
+            for c, f in ((3, slip[j, i]), (4, tau[j, i]), (5, sig[j, i]), (6, tau[j, i] / sig[j, i])):
+                assert abs(r[c] - f) <= 5.01e-5 * abs(f) + 1e-300, (k, c)
+        # psi from V, tau, sigma (main_LH.f90:1004 inverted), on the rows of
+        # the snapshot steps
+        a = fr.a[mid - 1]
+        for j, k in enumerate(ks[:20]):
+            r = loc[k - 1]
+            psi = a * np.log(2 * fr.vref / vel[j, i] * np.sinh(tau[j, i] / sig[j, i] / a))
+            assert abs(r[7] - psi) <= 1e-4 * abs(psi) + 1e-6, (k, r[7], psi)
     assert np.allclose(xyz[:, 0], g.xcol[lod], rtol=5.01e-6, atol=1e-12)     # e15.6: 6 significant digits
 
 

@@ -95,3 +95,65 @@ def test_gpu_opening_post_step_update():
     assert np.abs(o["slipn"][-1][low]).min() > 0 and np.abs(o["slipn"][-1][~low]).max() == 0
     vn, snf = rk.get_opening()
     assert np.array_equal(vn, o["veln"][-1]) and np.array_equal(snf, o["slipn"][-1])
+
+
+def test_gpu_probes_record_the_local_output():
+    """hbi_rk_set_probes / hbi_rk_get_probes (output_local, main_LH.f90:1495-1499):
+    after every accepted step, V, slip, tau, sigma, mu, psi, veln, slipn of
+    each probed element, equal to that step's snapshot (psi: to the state
+    after single-step calls); rows belong to the last step call only."""
+    gs, fr, lod, y0, (Hs, Hn, Hos, Hon), _, low = _setup()
+    rk = H.RK(Hs, fr, lod, gs.ds, eps_r=1e-6, hn=Hn, minsig=MINSIG, maxsig=MAXSIG)
+    rk.set_opening(Hos, Hon)
+    rk.set_output(velth=1e3, interval=1, capacity=64)
+    idx = np.array([int(np.where(low)[0][0]), int(np.where(low)[0][-1]), int(np.where(~low)[0][3])])
+    rk.set_probes(idx)
+    rk.set_state(y0, 0.0, 1.0)
+    rk.step(12)
+    pr = rk.get_probes()
+    o = rk.get_output()
+    assert pr.shape == (12, 3, 8)
+    for c, f in enumerate(("vel", "slip", "tau", "sigma")):
+        assert np.array_equal(pr[:, :, c], o[f][:, idx]), f
+    assert np.array_equal(pr[:, :, 4], o["tau"][:, idx] / o["sigma"][:, idx])
+    assert np.array_equal(pr[:, :, 6], o["veln"][:, idx]) and np.array_equal(pr[:, :, 7], o["slipn"][:, idx])
+    assert np.abs(pr[-1, :2, 7]).min() > 0 and pr[-1, 2, 7] == 0
+    y, slip, vel, _ = rk.get_state()
+    assert np.array_equal(pr[-1, :, 5], y[0::3][idx]) and np.array_equal(pr[-1, :, 0], vel[idx])
+    # a later call replaces the rows; psi after each single step
+    for _ in range(3):
+        rk.step(1)
+        pr1 = rk.get_probes()
+        y, slip, vel, _ = rk.get_state()
+        assert pr1.shape == (1, 3, 8)
+        assert np.array_equal(pr1[0, :, 5], y[0::3][idx]) and np.array_equal(pr1[0, :, 1], slip[idx])
+    rk.get_output()
+    rk.set_probes([])
+    rk.step(2)
+    assert rk.get_probes().shape == (2, 0, 8)
+
+
+def test_gpu_probes_without_opening():
+    """Without opening the veln and slipn columns are 0; rows equal the state
+    after the step; a probe index outside the elements is refused."""
+    g = P.mesh_2dvs(512, 40.0 / 512)
+    fr = P.friction_c1(g)
+    Hm = H.HMatrix(g, leaf=32, eps=1e-6)
+    lod = Hm.lod
+    y0 = P.init_state(fr, lod)
+    P.perturb_state(y0, g, lod)
+    rk = H.RK(Hm, fr, lod, g.ds, eps_r=1e-6)
+    with pytest.raises(H.HbiError):
+        rk.set_probes([0, 512])
+    rk.set_probes([0, 100, 511])
+    rk.set_state(y0, 0.0, 1.0)
+    info, hist, _ = rk.step(30, history=True)
+    pr = rk.get_probes()
+    assert pr.shape == (30, 3, 8)
+    assert (pr[:, :, 6:] == 0).all()
+    y, slip, vel, _ = rk.get_state()
+    idx = [0, 100, 511]
+    assert np.array_equal(pr[-1, :, 0], vel[idx]) and np.array_equal(pr[-1, :, 1], slip[idx])
+    assert np.array_equal(pr[-1, :, 2], y[1::3][idx]) and np.array_equal(pr[-1, :, 3], y[2::3][idx])
+    # log10 max |V| of the history bounds each probe's V
+    assert (np.log10(np.abs(pr[:, :, 0])).max(axis=1) <= hist[:, 4] + 1e-12).all()
