@@ -16,7 +16,7 @@ FSRC = os.path.join(HERE, "fortran")
 LIB = os.path.join(HERE, "libhbi.so")
 DRIVER = os.path.join(HERE, "lhbiem_gpu")
 
-SOURCES = ["partition.cpp", "hmat.hip", "rk.hip", "multirhs.hip", "ens.hip", "learn.hip"]
+SOURCES = ["partition.cpp", "hmat.hip", "rk.hip", "multirhs.hip", "ens.hip", "learn.hip", "fv.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("FC_HBI", "/opt/rocm/bin/amdflang")
 ARCH = os.environ.get("HBI_ARCH", "gfx950")

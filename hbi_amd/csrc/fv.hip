@@ -1,0 +1,945 @@
+// fv.hip -- the fluid-driven cycle of main_fv.f90 on the device.
+//
+// State per element in cluster order, interleaved (psi, tau, sigma_e, ks)
+// as main_fv.f90:995-1003; pf, sigma, disp beside it.  One RK attempt is a
+// fixed kernel sequence, captured once as a HIP graph and replayed:
+//   stage0                  V, matvec input from y                   2064-2077
+//   6 x [ H-matvec (shear, and normal unless sigma is constant) -> stage s ]
+//       stage s: derivs (4 components) -> k_s, then the next stage's state
+//       and V, or after the 6th yout, yerr and the block maxima of
+//       |yerr / yout| / eps                                        2259-2342
+//   decide                  accept / shrink, hnext, underflow       2149-2256
+//   commit                  y <- yout, V, sigma = sigma_e + pf       1016-1028
+//   pressure (1 workgroup)  implicit pore-pressure step: Beuler by
+//                           conjugate gradients, Picard iterations when
+//                           the diffusivity depends on pf; dt bound   1678-1940
+//   fields                  sigma_e = sigma - pf, disp, V, mu, kp,
+//                           qflow, monitor partials                   1045-1082
+//   monitor                 history row, stop criteria                1083-1152
+// Every kernel first reads the device flag and returns when the batch target
+// is reached or the run stopped (rejected attempts skip commit .. monitor).
+#include "rkcore.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace hbi {
+
+struct FvPar {
+    int n, law, backslip, has_hn, pdep, pfconst, bcl, bcr, setting, outpertime;
+    double vref, eps, hdiv, ds0, kL, kpmin, s0, eta, beta, Bs, pbcl, pbcr, q0, pinj, tinj, dtout;
+    double velmin, velmax, tmax, sg;
+    Tab T;
+};
+
+struct FvCtl {
+    double h, t, hdid, errmax, vmax;
+    long long attempts, accepted, rejected, target, hist_base, hist_cap;
+    int done, status, acc, niter;
+};
+
+struct FvArrays {
+    const double *a, *b, *dc, *f0, *vpl, *kpmax, *kT, *phi, *pfhyd, *qin, *area;
+    double *y, *ytmp, *yout, *k, *vel, *x, *hx, *hx_n;
+    double *pf, *sigma, *disp, *velpre, *qflow, *errpart, *monpart, *hist;
+};
+
+constexpr int kFvThreads = 256;
+constexpr int kFvRed = 16;   // reduction slots: waves of the largest solve workgroup
+constexpr int kFvMon = 6;    // max |V|, sum disp area, sum mu area, max sigma_e, min sigma_e, first argmax |V|
+constexpr int kFvHist = 10;
+constexpr int kFvMaxCells = HBI_FV_MAX_CELLS;
+
+// main_fv.f90:2039-2146 for one element: o = (dpsi, dtau, dsigma_e, dks)/dt
+__device__ __forceinline__ void fv_deriv(const FvPar &P, double gs, double gn, double psi, double tau, double sig,
+                                         double ks, double vel, double a, double b, double dc, double f0,
+                                         double kpmax, double kT, double *o)
+{
+#pragma clang fp contract(off)
+    const double rigid = kRigid, vs = kVs, vref = P.vref;
+    const double dksdt = -vel / P.kL * (ks - kpmax) - (ks - P.kpmin) / kT;
+    double dpsidt;
+    if (P.law == 0) {
+        if (b == 0) dpsidt = 0.0;
+        else dpsidt = b / dc * vref * exp((f0 - psi) / b) - b * vel / dc;
+    } else {
+        const double fss = f0 + (a - b) * log(fabs(vel) / vref);
+        dpsidt = -fabs(vel) / dc * (fabs(tau) / sig - fss);
+    }
+    const double dsigdt = (1 - P.Bs) * gn;
+    const double dvdtau = 2 * vref * exp(-psi / a) * cosh(tau / sig / a) / (a * sig);
+    const double dvdsig = -2 * vref * exp(-psi / a) * cosh(tau / sig / a) * tau / (a * (sig * sig));
+    const double dvdpsi = -vel / a;
+    double dtaudt = gs - 0.5 * rigid / vs * (dvdpsi * dpsidt + dvdsig * dsigdt);
+    dtaudt = dtaudt / (1.0 + 0.5 * rigid / vs * dvdtau);
+    o[0] = dpsidt;
+    o[1] = dtaudt;
+    o[2] = dsigdt;
+    o[3] = dksdt;
+}
+
+__global__ void fv_arm_kernel(FvCtl *c, long long nsteps)
+{
+    if (c->status == 0) c->done = 0;
+    c->target = c->accepted + nsteps;
+    c->hist_base = c->accepted;
+    c->hist_cap = nsteps;
+    c->acc = 0;
+}
+
+// V and the matvec input from y (or from ytmp for a derivs evaluation)
+__global__ void fv_stage0_kernel(FvPar P, const int *__restrict__ gate, FvArrays A, int from_tmp)
+{
+#pragma clang fp contract(off)
+    if (*gate) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const double *y = from_tmp ? A.ytmp : A.y;
+    const double v = slip_rate(y[4 * i], y[4 * i + 1], y[4 * i + 2], A.a[i], P.vref);
+    A.vel[i] = v;
+    A.x[i] = P.backslip ? v - A.vpl[i] : v;
+}
+
+// stage s in 0..5; mode 0: full attempt, 1: derivs only of ytmp (writes k_0)
+__global__ __launch_bounds__(kFvThreads) void fv_stage_kernel(FvPar P, const int *__restrict__ gate,
+                                                               const FvCtl *__restrict__ c, FvArrays A, int s,
+                                                               int mode)
+{
+#pragma clang fp contract(off)
+    __shared__ double sv[kFvThreads / kWave];
+    if (*gate) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = P.n;
+    const Tab &T = P.T;
+    double emax = 0.0;
+    if (i < n) {
+        const double hx = A.hx[i];
+        const double gs = P.hdiv != 0.0 ? hx / P.hdiv : hx;
+        double gn = 0.0;
+        if (P.has_hn) gn = P.hdiv != 0.0 ? A.hx_n[i] / P.hdiv : A.hx_n[i];
+        const double *ys = (s == 0 && mode == 0) ? A.y : A.ytmp;
+        double kk[4];
+        fv_deriv(P, gs, gn, ys[4 * i], ys[4 * i + 1], ys[4 * i + 2], ys[4 * i + 3], A.vel[i], A.a[i], A.b[i],
+                 A.dc[i], A.f0[i], A.kpmax[i], A.kT[i], kk);
+        const size_t m4 = 4 * (size_t)n;
+        double *ks = A.k + s * m4;
+        for (int q = 0; q < 4; q++) ks[4 * i + q] = kk[q];
+        if (mode == 0) {
+            const double h = c->h;
+            const double *k1 = A.k, *k2 = A.k + m4, *k3 = A.k + 2 * m4, *k4 = A.k + 3 * m4, *k5 = A.k + 4 * m4,
+                         *k6 = A.k + 5 * m4;
+            if (s < 5) {
+                double yt[4];
+                for (int q = 0; q < 4; q++) {
+                    const size_t e = 4 * (size_t)i + q;
+                    const double y = A.y[e];
+                    switch (s) {
+                    case 0: yt[q] = y + T.b21 * h * k1[e]; break;
+                    case 1: yt[q] = y + h * (T.b31 * k1[e] + T.b32 * k2[e]); break;
+                    case 2: yt[q] = y + h * (T.b41 * k1[e] + T.b42 * k2[e] + T.b43 * k3[e]); break;
+                    case 3: yt[q] = y + h * (T.b51 * k1[e] + T.b52 * k2[e] + T.b53 * k3[e] + T.b54 * k4[e]); break;
+                    default:
+                        yt[q] = y + h * (T.b61 * k1[e] + T.b62 * k2[e] + T.b63 * k3[e] + T.b64 * k4[e] + T.b65 * k5[e]);
+                        break;
+                    }
+                    A.ytmp[e] = yt[q];
+                }
+                const double v = slip_rate(yt[0], yt[1], yt[2], A.a[i], P.vref);
+                A.vel[i] = v;
+                A.x[i] = P.backslip ? v - A.vpl[i] : v;
+            } else {
+                // max |yerr / yout| / eps over the 4 components (main_fv.f90:2192-2197);
+                // a NaN ratio never wins, as in the reference's comparison
+                for (int q = 0; q < 4; q++) {
+                    const size_t e = 4 * (size_t)i + q;
+                    const double yo = A.y[e] + h * (T.c1 * k1[e] + T.c3 * k3[e] + T.c4 * k4[e] + T.c6 * k6[e]);
+                    const double ye = h * (T.dc1 * k1[e] + T.dc3 * k3[e] + T.dc4 * k4[e] + T.dc5 * k5[e] + T.dc6 * k6[e]);
+                    A.yout[e] = yo;
+                    const double r = fabs(ye / yo) / P.eps;
+                    if (r > emax) emax = r;
+                }
+            }
+        }
+    }
+    if (mode != 0 || s != 5) return;
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(emax, o, kWave);
+        if (v2 > emax) emax = v2;
+    }
+    if (threadIdx.x % kWave == 0) sv[threadIdx.x / kWave] = emax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sv[0];
+        for (int q = 1; q < kFvThreads / kWave; q++)
+            if (sv[q] > m) m = sv[q];
+        A.errpart[blockIdx.x] = m;
+    }
+}
+
+// rkqs decision, main_fv.f90:2182-2252
+__global__ void fv_decide_kernel(FvPar P, FvCtl *c, const double *__restrict__ errpart, int nblk)
+{
+#pragma clang fp contract(off)
+    if (c->done || threadIdx.x != 0) return;
+    const double SAFETY = (double)0.9f, PGROW = (double)-0.2f, PSHRNK = (double)-0.25f;
+    double errmax = 0.0;
+    for (int b = 0; b < nblk; b++)
+        if (errpart[b] > errmax) errmax = errpart[b];
+    const double h = c->h;
+    c->attempts++;
+    c->errmax = errmax;
+    if (errmax < 1.0 && errmax > 1e-15) {
+        double hn = fmin(2 * h, SAFETY * h * pow(errmax, PGROW));
+        if (P.outpertime) hn = fmin(hn, P.dtout);
+        c->hdid = h;
+        c->t = c->t + h;
+        c->h = hn;
+        c->acc = 1;
+        c->accepted++;
+    } else {
+        double hnew;
+        if (errmax > 1e-15) hnew = fmax(0.5 * h, SAFETY * h * pow(errmax, PSHRNK));
+        else hnew = 0.5 * h;
+        c->acc = 0;
+        c->rejected++;
+        c->h = hnew;
+        const double xnew = c->t + hnew;
+        if (xnew - c->t < 1.e-15) {
+            c->status = HBI_E_STEP;
+            c->done = 1;
+        }
+    }
+}
+
+// y <- yout; V of the new state (the first half of the displacement
+// update uses it, main_fv.f90:1023, 1054); sigma = sigma_e + pf
+__global__ void fv_commit_kernel(FvPar P, const FvCtl *__restrict__ c, FvArrays A)
+{
+#pragma clang fp contract(off)
+    if (c->done || !c->acc) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    double yo[4];
+    for (int q = 0; q < 4; q++) {
+        yo[q] = A.yout[4 * (size_t)i + q];
+        A.y[4 * (size_t)i + q] = yo[q];
+    }
+    A.velpre[i] = slip_rate(yo[0], yo[1], yo[2], A.a[i], P.vref);
+    A.sigma[i] = yo[2] + A.pf[i];
+}
+
+// ---------------------------------------------------------- pore pressure --
+struct FvSolveIO {
+    const double *pf;       // pf before the step
+    const double *y;        // ks (component 3)
+    const double *sigma;    // total normal stress (pressure-dependent diffusivity)
+    double *pfnew;
+    double *rows;           // 3N: the matrix rows of the solve
+    double *work;           // 3N: right-hand side, solution, Picard trial
+    double h, t;            // used when not gated (the graph path reads hdid and t)
+    double *dtnxt;          // in/out (gated: &ctl->h)
+    int *niter;
+    int gated;
+};
+
+// block sum over NT threads; every thread returns the same value.  Two LDS
+// slots alternate so one barrier per reduction suffices.
+template <int NT>
+__device__ __forceinline__ double fv_bsum(double v, double (*red)[kFvRed], int &slot)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    if (l == 0) red[slot][w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int q = 0; q < NT / kWave; q++) s += red[slot][q];
+    slot ^= 1;
+    return s;
+}
+
+template <int NT>
+__device__ __forceinline__ double fv_bmax(double v, double (*red)[kFvRed], int &slot)
+{
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(v, o, kWave);
+        if (v2 > v) v = v2;
+    }
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    if (l == 0) red[slot][w] = v;
+    __syncthreads();
+    double s = red[slot][0];
+    for (int q = 1; q < NT / kWave; q++)
+        if (red[slot][q] > s) s = red[slot][q];
+    slot ^= 1;
+    return s;
+}
+
+// Dxx row e of Beuler (main_fv.f90:1786-1816) from the diffusivities in LDS:
+// the coefficients of pf(e-1), pf(e), pf(e+1), before the 1 / ds0^2 scaling
+__device__ __forceinline__ void fv_dxx_row(const FvPar &P, const double *cd, int e, int n, double &d1, double &d2,
+                                           double &d3)
+{
+#pragma clang fp contract(off)
+    d1 = 0.0; d2 = 0.0; d3 = 0.0;
+    if (e == 0) {
+        d1 = -cd[0] - cd[1];
+        d2 = P.bcl == HBI_FV_DIRICHLET ? cd[1] - cd[0] : -d1;
+    } else if (e == 1) {
+        d1 = P.bcl == HBI_FV_DIRICHLET ? cd[1] / 2 - cd[0] / 2 : cd[0] / 2 + cd[1] / 2;
+        d2 = -cd[0] / 2 - cd[1] - cd[2] / 2;
+        d3 = cd[1] / 2 + cd[2] / 2;
+    } else if (e == n - 2) {
+        d1 = cd[n - 3] / 2 + cd[n - 2] / 2;
+        d2 = -cd[n - 3] / 2 - cd[n - 2] - cd[n - 1] / 2;
+        d3 = P.bcr == HBI_FV_DIRICHLET ? -cd[n - 1] / 2 + cd[n - 2] / 2 : cd[n - 2] / 2 + cd[n - 1] / 2;
+    } else if (e == n - 1) {
+        // row n keeps its diagonal in column 3 and the sub-diagonal in column 2
+        d3 = -cd[n - 1] - cd[n - 2];
+        d2 = P.bcr == HBI_FV_DIRICHLET ? cd[n - 2] - cd[n - 1] : -d3;
+    } else {
+        d1 = cd[e - 1] / 2 + cd[e] / 2;
+        d2 = -cd[e - 1] / 2 - cd[e] - cd[e + 1] / 2;
+        d3 = cd[e] / 2 + cd[e + 1] / 2;
+    }
+}
+
+// One Beuler solve (main_fv.f90:1776-1940) by a whole workgroup of NT
+// threads.  The set-up pass writes each element's matrix row (sub, diagonal,
+// super) to rows[3e..3e+2], the right-hand side to w[e] and the start x =
+// pf - pfhyd to w[n + e]; in the conjugate-gradient loop element e = j * NT +
+// threadIdx.x (j < NPT) of x and r lives in registers, its row is read from
+// global memory by its own thread only, p (read at neighbours) sits in ps
+// and A p in cd (the diffusivities on entry).  pfnew goes to w[n + e];
+// returns the iteration count.
+template <int NT, int NPT>
+__device__ __forceinline__ int fv_beuler(const FvPar &P, double *cd, double *ps, double (*red)[kFvRed], int &slot,
+                                         const double *pf, const double *pfhyd, const double *qin, const double *phi,
+                                         double h, double t, double *__restrict__ rows, double *__restrict__ w)
+{
+#pragma clang fp contract(off)
+    const int n = P.n;
+    const int tid = threadIdx.x;
+    const double ds0 = P.ds0;
+    const double c1e9 = (double)1e-9f;
+    const int half = n / 2 - 1;    // 1-based row N/2
+    for (int e = tid; e < n; e += NT) {
+        double d1, d2, d3;
+        fv_dxx_row(P, cd, e, n, d1, d2, d3);
+        d1 = d1 / ds0 / ds0;
+        d2 = d2 / ds0 / ds0;
+        d3 = d3 / ds0 / ds0;
+        double lo = 0.0, di, up = 0.0;
+        if (e == 0) {
+            di = 1.0 - h * d1;
+            up = -h * d2;
+        } else if (e == n - 1) {
+            di = 1.0 - h * d3;
+            lo = -h * d2;
+        } else {
+            lo = -h * d1;
+            di = 1.0 - h * d2;
+            up = -h * d3;
+        }
+        // penalty vector (main_fv.f90:1842-1868)
+        double sat = 0.0;
+        if (P.bcl == HBI_FV_DIRICHLET) {
+            if (e == 0) sat = -1.0 * cd[0] / ds0 / ds0 * (P.pbcl - pfhyd[0]) * h * 2;
+            if (e == 1) sat = -0.5 * cd[0] / ds0 / ds0 * (P.pbcl - pfhyd[0]) * h * 2;
+        } else if (P.setting == HBI_FV_STEADYSTATE && e == 0) {
+            sat = -P.q0 / P.beta / phi[0] * c1e9 * h / ds0 * 2;
+        }
+        if (P.bcr == HBI_FV_DIRICHLET) {
+            const double satn = -1 * cd[n - 1] / ds0 / ds0 * (P.pbcr - pfhyd[n - 1]) * h * 2;
+            if (e == n - 1) sat = satn;
+            if (e == n - 2) sat = satn / 2;
+        } else if (e == n - 1) {
+            if (P.setting == HBI_FV_STEADYSTATE) sat = P.q0 / P.beta / phi[n - 1] * c1e9 * h / ds0 * 2;
+            if (P.setting == HBI_FV_THRUST) sat = qin[1999] / P.beta / phi[n - 1] * c1e9 * h / ds0 * 2;
+        }
+        if (P.setting == HBI_FV_THRUST && e >= 2 && e <= n - 3) sat = -qin[e] / P.beta / phi[e] * c1e9 * h / ds0;
+        double b = pf[e] - sat - pfhyd[e];
+        if (e == half) {
+            if (P.setting == HBI_FV_INJECTIONP) {
+                b = b + h * P.pinj / 1e1;
+                di = di + h / 1e1;
+            } else if (P.setting == HBI_FV_INJECTIONQ && t < P.tinj) {
+                b = b + h * qin[half] / P.beta / phi[half] * c1e9 / ds0;
+            }
+        }
+        if (e == 0 || e == n - 1) {
+            b = b / 2;
+            lo = lo / 2;
+            di = di / 2;
+            up = up / 2;
+        }
+        rows[3 * (size_t)e] = lo;
+        rows[3 * (size_t)e + 1] = di;
+        rows[3 * (size_t)e + 2] = up;
+        w[e] = b;
+        const double x0 = pf[e] - pfhyd[e];
+        w[n + e] = x0;
+        ps[e] = x0;
+    }
+    // the matrix product in the reference's order of terms (1895-1903)
+    auto apply = [&](const double *v, int e) -> double {
+        const double *rw = rows + 3 * (size_t)e;
+        if (e == 0) return v[0] * rw[1] + v[1] * rw[2];
+        if (e == n - 1) return v[n - 1] * rw[1] + v[n - 2] * rw[0];
+        return v[e - 1] * rw[0] + v[e] * rw[1] + v[e + 1] * rw[2];
+    };
+    __syncthreads();   // ps holds x; every row has read cd
+    double x[NPT], r[NPT], rr = 0.0;
+#pragma unroll
+    for (int j = 0; j < NPT; j++) {
+        const int e = j * NT + tid;
+        x[j] = r[j] = 0.0;
+        if (e < n) {
+            x[j] = ps[e];
+            r[j] = w[e] - apply(ps, e);
+            rr += r[j] * r[j];
+        }
+    }
+    double rsold = fv_bsum<NT>(rr, red, slot);   // its barrier ends the reads of ps
+#pragma unroll
+    for (int j = 0; j < NPT; j++) {
+        const int e = j * NT + tid;
+        if (e < n) ps[e] = r[j];
+    }
+    const double tol = (double)1e-4f;
+    const double thr = tol * tol * (double)n;
+    const int itermax = 10000;
+    int niter = 0;
+    if (!(rsold < thr)) {
+        niter = itermax;
+        for (int it = 1; it <= itermax; it++) {
+            const double tmp1 = rsold;   // = sum(r * r) of the current r
+            __syncthreads();             // ps holds p
+            double mp = 0.0;
+#pragma unroll
+            for (int j = 0; j < NPT; j++) {
+                const int e = j * NT + tid;
+                if (e < n) {
+                    const double m = apply(ps, e);
+                    cd[e] = m;
+                    mp += m * ps[e];
+                }
+            }
+            const double tmp2 = fv_bsum<NT>(mp, red, slot);
+            const double alpha = tmp1 / tmp2;
+            double rs = 0.0;
+#pragma unroll
+            for (int j = 0; j < NPT; j++) {
+                const int e = j * NT + tid;
+                if (e < n) {
+                    x[j] = x[j] + alpha * ps[e];
+                    r[j] = r[j] - alpha * cd[e];
+                    rs += r[j] * r[j];
+                }
+            }
+            const double rsnew = fv_bsum<NT>(rs, red, slot);
+            if (rsnew < thr) {
+                niter = it;
+                break;
+            }
+#pragma unroll
+            for (int j = 0; j < NPT; j++) {
+                const int e = j * NT + tid;
+                if (e < n) ps[e] = r[j] + (rsnew / rsold) * ps[e];
+            }
+            rsold = rsnew;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NPT; j++) {
+        const int e = j * NT + tid;
+        if (e < n) w[n + e] = x[j] + pfhyd[e];
+    }
+    return niter;
+}
+
+// implicitsolver2 (main_fv.f90:1727-1774) or, pressure dependent,
+// implicitsolver (1678-1724): pfnew, CG iterations and the dt bound.  The
+// work vector w (3N) holds the right-hand side, pfnew and the Picard trial.
+template <int NT, int NPT>
+__global__ __launch_bounds__(NT) void fv_pressure_kernel(FvPar P, FvCtl *c, FvArrays A, FvSolveIO io)
+{
+#pragma clang fp contract(off)
+    __shared__ double cd[kFvMaxCells];
+    __shared__ double ps[kFvMaxCells];
+    __shared__ double red[2][kFvRed];
+    if (io.gated && (c->done || !c->acc)) return;
+    const int n = P.n;
+    const int tid = threadIdx.x;
+    const double h = io.gated ? c->hdid : io.h;
+    const double t = io.gated ? c->t : io.t;
+    double *dtnxt = io.gated ? &c->h : io.dtnxt;
+    double *w = io.work, *pft = io.work + 2 * (size_t)n;
+    int slot = 0;
+    for (int e = tid; e < n; e += NT) pft[e] = io.pf[e];
+    int niter = 0;
+    const int kitmax = P.pdep ? 20 : 1;
+    for (int kit = 1; kit <= kitmax; kit++) {
+        __syncthreads();   // the previous solve's readers of cd and ps are done
+        for (int e = tid; e < n; e += NT) {
+            const double cc = P.eta * P.beta * A.phi[e];
+            const double ks = io.y[4 * (size_t)e + 3];
+            cd[e] = P.pdep ? ks * exp(-(io.sigma[e] - pft[e]) / P.s0) / cc * 1e-6 : ks / cc * 1e-6;
+        }
+        __syncthreads();
+        niter = fv_beuler<NT, NPT>(P, cd, ps, red, slot, io.pf, A.pfhyd, A.qin, A.phi, h, t, io.rows, w);
+        if (!P.pdep) break;
+        double err = 0.0;
+        for (int e = tid; e < n; e += NT) {
+            const double d = fabs(pft[e] - w[n + e]);
+            if (d > err) err = d;
+        }
+        err = fv_bmax<NT>(err, red, slot);
+        if (err < (double)1e-3f) break;
+        for (int e = tid; e < n; e += NT) pft[e] = w[n + e];
+    }
+    // pf(e) is read and written by its own thread only, so pfnew may alias pf
+    double dmax = -HUGE_VAL;
+    for (int e = tid; e < n; e += NT) {
+        const double old = io.pf[e];
+        const double x = w[n + e];
+        io.pfnew[e] = x;
+        const double d = x - old;
+        if (d > dmax) dmax = d;
+    }
+    dmax = fv_bmax<NT>(dmax, red, slot);
+    if (tid == 0) {
+        const double dpth = P.pdep ? (double)0.3f : (double)0.1f;
+        double dt = *dtnxt;
+        if (dt / h * dmax > dpth) dt = dpth * h / dmax;
+        if (P.pdep && P.outpertime) dt = fmin(dt, P.dtout);
+        *dtnxt = dt;
+        *io.niter = niter;
+    }
+}
+
+// sigma_e = sigma - pf, displacement (trapezoid over the post-RK and the final
+// V), mu, kp, qflow and the monitor partials (main_fv.f90:1045-1082, 1083-1134)
+__global__ __launch_bounds__(kFvThreads) void fv_fields_kernel(FvPar P, const FvCtl *__restrict__ c, FvArrays A)
+{
+#pragma clang fp contract(off)
+    __shared__ double sp[kFvThreads / kWave][kFvMon];
+    if (c->done || !c->acc) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = P.n;
+    const double dt = c->hdid;
+    double m[kFvMon] = {-1.0, 0.0, 0.0, -HUGE_VAL, HUGE_VAL, -1.0};
+    if (i < n) {
+        const size_t e = 4 * (size_t)i;
+        const double psi = A.y[e], tau = A.y[e + 1], ks = A.y[e + 3];
+        const double sige = A.sigma[i] - A.pf[i];
+        A.y[e + 2] = sige;
+        double d = A.disp[i];
+        d = d + A.velpre[i] * dt * 0.5;
+        const double v = slip_rate(psi, tau, sige, A.a[i], P.vref);
+        d = d + v * dt * 0.5;
+        A.disp[i] = d;
+        A.vel[i] = v;
+        const double mu = tau / sige;
+        // qflow with kp of this element and pf of the neighbours (final pf)
+        const double kp = P.pdep ? ks * exp(-sige / P.s0) : ks;
+        const double *pf = A.pf, *ph = A.pfhyd;
+        double q;
+        if (i == 0) q = -kp / P.eta * 1e3 * (pf[1] - ph[1] - pf[0] + ph[0]) / P.ds0;
+        else if (i == n - 1) q = -kp / P.eta * 1e3 * (pf[n - 1] - ph[n - 1] - pf[n - 2] + ph[n - 2]) / P.ds0;
+        else q = -kp / P.eta * 1e3 * (pf[i + 1] - ph[i + 1] - pf[i - 1] + ph[i - 1]) / 2 / P.ds0;
+        A.qflow[i] = q;
+        m[0] = fabs(v);
+        m[1] = d * A.area[i];
+        m[2] = mu * A.area[i];
+        m[3] = sige;
+        m[4] = sige;
+        m[5] = (double)i;
+    }
+    for (int o = 1; o < kWave; o <<= 1) {
+        double o2[kFvMon];
+        for (int k = 0; k < kFvMon; k++) o2[k] = __shfl_down(m[k], o, kWave);
+        const int lane = threadIdx.x % kWave;
+        if ((lane & (2 * o - 1)) == 0 && lane + o < kWave) {
+            if (o2[0] > m[0]) { m[0] = o2[0]; m[5] = o2[5]; }
+            m[1] = m[1] + o2[1];
+            m[2] = m[2] + o2[2];
+            m[3] = fmax(m[3], o2[3]);
+            m[4] = fmin(m[4], o2[4]);
+        }
+    }
+    const int w = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0)
+        for (int k = 0; k < kFvMon; k++) sp[w][k] = m[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b[kFvMon];
+        for (int k = 0; k < kFvMon; k++) b[k] = sp[0][k];
+        for (int q = 1; q < kFvThreads / kWave; q++) {
+            if (sp[q][0] > b[0]) { b[0] = sp[q][0]; b[5] = sp[q][5]; }
+            b[1] = b[1] + sp[q][1];
+            b[2] = b[2] + sp[q][2];
+            b[3] = fmax(b[3], sp[q][3]);
+            b[4] = fmin(b[4], sp[q][4]);
+        }
+        for (int k = 0; k < kFvMon; k++) A.monpart[kFvMon * (size_t)blockIdx.x + k] = b[k];
+    }
+}
+
+__global__ void fv_monitor_kernel(FvPar P, FvCtl *c, FvArrays A, int nblk)
+{
+#pragma clang fp contract(off)
+    if (c->done || !c->acc || threadIdx.x != 0) return;
+    double b[kFvMon];
+    for (int k = 0; k < kFvMon; k++) b[k] = A.monpart[k];
+    for (int q = 1; q < nblk; q++) {
+        const double *mp = A.monpart + kFvMon * (size_t)q;
+        if (mp[0] > b[0]) { b[0] = mp[0]; b[5] = mp[5]; }
+        b[1] = b[1] + mp[1];
+        b[2] = b[2] + mp[2];
+        b[3] = fmax(b[3], mp[3]);
+        b[4] = fmin(b[4], mp[4]);
+    }
+    const double vm = b[0];
+    c->vmax = vm;
+    const long long row = c->accepted - c->hist_base - 1;
+    if (A.hist && row >= 0 && row < c->hist_cap) {
+        double *hr = A.hist + kFvHist * row;
+        hr[0] = c->t; hr[1] = log10(vm); hr[2] = b[1] / P.sg; hr[3] = b[2] / P.sg; hr[4] = b[3]; hr[5] = b[4];
+        hr[6] = b[5] + 1.0; hr[7] = c->errmax; hr[8] = c->hdid; hr[9] = (double)c->niter;
+    }
+    // stop controls, main_fv.f90:1136-1152
+    if (vm > P.velmax || vm < P.velmin) c->status = 1;
+    else if (c->t > P.tmax) c->status = 3;
+    else if (b[4] < (double)0.2f) c->status = 4;
+    if (c->status || c->accepted >= c->target) c->done = 1;
+}
+
+}  // namespace hbi
+
+// ------------------------------------------------------------------ object --
+struct hbi_fv {
+    hipStream_t stream = nullptr;
+    hbi_hmat *hs = nullptr, *hn = nullptr;
+    hbi_fv_opts opts{};
+    hbi::FvPar P{};
+    int n = 0, nblk = 0;
+    hbi::DevBuf<double> a, b, dc, f0, vpl, kpmax, kT, phi, pfhyd, qin, area;
+    hbi::DevBuf<double> y, ytmp, yout, k, vel, x, z, zn, ypart, ypart_n, hx, hx_n;
+    hbi::DevBuf<double> pf, sigma, disp, velpre, qflow, errpart, monpart, hist, rows, work;
+    hbi::DevBuf<double> io_pf, io_y, io_sig, io_out, io_dt;
+    hbi::DevBuf<int> io_niter, zero;
+    hbi::DevBuf<hbi::FvCtl> ctl;
+    hipGraphExec_t graph = nullptr;
+    double *graph_hist = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double last_ms = 0.0;
+};
+
+namespace hbi {
+
+static FvArrays fv_arrays(hbi_fv *f)
+{
+    FvArrays A{};
+    A.a = f->a.p; A.b = f->b.p; A.dc = f->dc.p; A.f0 = f->f0.p; A.vpl = f->vpl.p; A.kpmax = f->kpmax.p;
+    A.kT = f->kT.p; A.phi = f->phi.p; A.pfhyd = f->pfhyd.p; A.qin = f->qin.p; A.area = f->area.p;
+    A.y = f->y.p; A.ytmp = f->ytmp.p; A.yout = f->yout.p; A.k = f->k.p; A.vel = f->vel.p; A.x = f->x.p;
+    A.hx = f->hx.p; A.hx_n = f->hx_n.p;
+    A.pf = f->pf.p; A.sigma = f->sigma.p; A.disp = f->disp.p; A.velpre = f->velpre.p; A.qflow = f->qflow.p;
+    A.errpart = f->errpart.p; A.monpart = f->monpart.p;
+    A.hist = f->hist.n ? f->hist.p : nullptr;
+    return A;
+}
+
+static void fv_launch_pressure(hbi_fv *f, FvArrays A, const FvSolveIO &io)
+{
+    FvCtl *c = f->ctl.p;
+    // elements per thread: registers hold 6 x NPT doubles, so the larger grids
+    // take 512-thread workgroups (256 VGPRs per lane)
+    if (f->n <= 1024) fv_pressure_kernel<1024, 1><<<1, 1024, 0, f->stream>>>(f->P, c, A, io);
+    else if (f->n <= 2048) fv_pressure_kernel<1024, 2><<<1, 1024, 0, f->stream>>>(f->P, c, A, io);
+    else if (f->n <= 4096) fv_pressure_kernel<512, 8><<<1, 512, 0, f->stream>>>(f->P, c, A, io);
+    else fv_pressure_kernel<1024, 8><<<1, 1024, 0, f->stream>>>(f->P, c, A, io);
+}
+
+// nstages derivs evaluations: mode 0 a whole attempt from y, mode 1 k_0 of ytmp
+static void fv_launch_stages(hbi_fv *f, int nstages, int mode, const int *gate)
+{
+    const int g = f->nblk;
+    FvArrays A = fv_arrays(f);
+    const int prec = f->opts.prec;
+    fv_stage0_kernel<<<g, kFvThreads, 0, f->stream>>>(f->P, gate, A, mode == 1);
+    for (int s = 0; s < nstages; s++) {
+        hmat_phase1(f->hs, prec, f->x.p, f->z.p, f->ypart.p, f->stream, gate);
+        hmat_phase2(f->hs, prec, f->z.p, f->ypart.p, f->hx.p, f->stream, gate);
+        if (f->hn) {
+            hmat_phase1(f->hn, prec, f->x.p, f->zn.p, f->ypart_n.p, f->stream, gate);
+            hmat_phase2(f->hn, prec, f->zn.p, f->ypart_n.p, f->hx_n.p, f->stream, gate);
+        }
+        fv_stage_kernel<<<g, kFvThreads, 0, f->stream>>>(f->P, gate, f->ctl.p, A, s, mode);
+    }
+}
+
+static void fv_launch_attempt(hbi_fv *f)
+{
+    const int g = f->nblk;
+    FvArrays A = fv_arrays(f);
+    FvCtl *c = f->ctl.p;
+    fv_launch_stages(f, 6, 0, &c->done);
+    fv_decide_kernel<<<1, 64, 0, f->stream>>>(f->P, c, f->errpart.p, g);
+    fv_commit_kernel<<<g, kFvThreads, 0, f->stream>>>(f->P, c, A);
+    if (!f->P.pfconst) {
+        FvSolveIO io{};
+        io.pf = f->pf.p; io.y = f->y.p; io.sigma = f->sigma.p; io.pfnew = f->pf.p; io.rows = f->rows.p;
+        io.work = f->work.p;
+        io.niter = &c->niter; io.gated = 1;
+        fv_launch_pressure(f, A, io);
+    }
+    fv_fields_kernel<<<g, kFvThreads, 0, f->stream>>>(f->P, c, A);
+    fv_monitor_kernel<<<1, 64, 0, f->stream>>>(f->P, c, A, g);
+}
+
+static void fv_capture(hbi_fv *f)
+{
+    if (f->graph) {
+        (void)hipGraphExecDestroy(f->graph);
+        f->graph = nullptr;
+    }
+    hipGraph_t gr;
+    HBI_HIP(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
+    fv_launch_attempt(f);
+    HBI_HIP(hipStreamEndCapture(f->stream, &gr));
+    HBI_HIP(hipGraphInstantiate(&f->graph, gr, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(gr);
+    f->graph_hist = f->hist.p;
+}
+
+}  // namespace hbi
+
+extern "C" {
+
+int hbi_fv_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_fv_opts *o, const double *a, const double *b,
+                  const double *dc, const double *f0, const double *vpl, const double *kpmaxv, const double *kTv,
+                  const double *phi, const double *pfhyd, const double *qin, const double *area, hbi_fv **out)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(hs && o && out && a && b && dc && f0 && vpl && kpmaxv && kTv && phi && pfhyd && qin && area,
+                    "NULL argument");
+        HBI_REQUIRE(!hn || hn->n == hs->n, "normal-stress matrix size differs");
+        HBI_REQUIRE(o->prec >= 0 && o->prec <= 2, "bad precision");
+        HBI_REQUIRE(o->eps_r > 0 && o->ds0 > 0, "eps_r and ds0 must be > 0");
+        HBI_REQUIRE((o->bcl == HBI_FV_NEUMANN || o->bcl == HBI_FV_DIRICHLET) &&
+                        (o->bcr == HBI_FV_NEUMANN || o->bcr == HBI_FV_DIRICHLET),
+                    "bad boundary condition");
+        HBI_REQUIRE(o->setting >= HBI_FV_INJECTIONQ && o->setting <= HBI_FV_OTHER, "bad setting");
+        const int n = hs->n;
+        HBI_REQUIRE(n >= 4 && n <= HBI_FV_MAX_CELLS, "the pore-pressure grid needs 4 to 8192 elements");
+        HBI_REQUIRE(o->setting != HBI_FV_THRUST || n >= 2000, "setting thrust reads qin(2000): needs N >= 2000");
+        *out = nullptr;
+        hbi_fv *f = new hbi_fv();
+        try {
+            f->hs = hs;
+            f->hn = hn;
+            f->opts = *o;
+            f->n = n;
+            f->nblk = (n + hbi::kFvThreads - 1) / hbi::kFvThreads;
+            HBI_HIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+            HBI_HIP(hipEventCreate(&f->e0));
+            HBI_HIP(hipEventCreate(&f->e1));
+            hbi::FvPar &P = f->P;
+            P.n = n; P.law = o->law; P.backslip = o->backslip; P.has_hn = hn ? 1 : 0;
+            P.pdep = o->pressure_dependent; P.pfconst = o->pfconst; P.bcl = o->bcl; P.bcr = o->bcr;
+            P.setting = o->setting; P.outpertime = o->outpertime;
+            P.vref = o->vref; P.eps = o->eps_r; P.hdiv = o->hdiv; P.ds0 = o->ds0; P.kL = o->kL; P.kpmin = o->kpmin;
+            P.s0 = o->s0; P.eta = o->eta; P.beta = o->beta; P.Bs = o->Bs; P.pbcl = o->pbcl; P.pbcr = o->pbcr;
+            P.q0 = o->q0; P.pinj = o->pinj; P.tinj = o->tinj_s; P.dtout = o->dtout_s;
+            P.velmin = o->velmin; P.velmax = o->velmax; P.tmax = o->tmax_s;
+            P.T = hbi::rk_tableau();
+            double sg = 0.0;
+            for (int i = 0; i < n; i++) sg = sg + area[i];
+            P.sg = sg;
+            hipStream_t s = f->stream;
+            f->a.upload(a, n, s); f->b.upload(b, n, s); f->dc.upload(dc, n, s); f->f0.upload(f0, n, s);
+            f->vpl.upload(vpl, n, s); f->kpmax.upload(kpmaxv, n, s); f->kT.upload(kTv, n, s);
+            f->phi.upload(phi, n, s); f->pfhyd.upload(pfhyd, n, s); f->qin.upload(qin, n, s);
+            f->area.upload(area, n, s);
+            const size_t m4 = 4 * (size_t)n;
+            f->y.alloc(m4); f->ytmp.alloc(m4); f->yout.alloc(m4); f->k.alloc(6 * m4);
+            f->vel.alloc(n); f->x.alloc(n + 8); f->hx.alloc(n);
+            HBI_HIP(hipMemsetAsync(f->x.p, 0, (n + 8) * 8, s));
+            HBI_HIP(hipMemsetAsync(f->k.p, 0, 6 * m4 * 8, s));
+            f->z.alloc(hbi::hmat_z_size(hs, o->prec));
+            f->ypart.alloc(hbi::hmat_ypart_size(hs, o->prec));
+            if (hn) {
+                f->zn.alloc(hbi::hmat_z_size(hn, o->prec));
+                f->ypart_n.alloc(hbi::hmat_ypart_size(hn, o->prec));
+                f->hx_n.alloc(n);
+            }
+            f->pf.alloc(n); f->sigma.alloc(n); f->disp.alloc(n); f->velpre.alloc(n); f->qflow.alloc(n);
+            HBI_HIP(hipMemsetAsync(f->pf.p, 0, n * 8, s));
+            HBI_HIP(hipMemsetAsync(f->disp.p, 0, n * 8, s));
+            HBI_HIP(hipMemsetAsync(f->qflow.p, 0, n * 8, s));
+            f->rows.alloc(3 * (size_t)n);
+            f->work.alloc(3 * (size_t)n);
+            f->errpart.alloc(f->nblk);
+            f->monpart.alloc(hbi::kFvMon * (size_t)f->nblk);
+            f->io_pf.alloc(n); f->io_y.alloc(m4); f->io_sig.alloc(n); f->io_out.alloc(n); f->io_dt.alloc(1);
+            f->io_niter.alloc(1);
+            f->zero.alloc(1);
+            HBI_HIP(hipMemsetAsync(f->zero.p, 0, sizeof(int), s));
+            f->ctl.alloc(1);
+            hbi::FvCtl c{};
+            c.h = 1.0;
+            hbi::to_device_sync(f->ctl.p, &c, sizeof(c), s);
+        } catch (...) {
+            hbi_fv_destroy(f);
+            throw;
+        }
+        *out = f;
+    });
+}
+
+int hbi_fv_destroy(hbi_fv *f)
+{
+    if (!f) return HBI_OK;
+    if (f->stream) (void)hipStreamSynchronize(f->stream);
+    if (f->graph) (void)hipGraphExecDestroy(f->graph);
+    if (f->e0) (void)hipEventDestroy(f->e0);
+    if (f->e1) (void)hipEventDestroy(f->e1);
+    hipStream_t s = f->stream;
+    delete f;
+    if (s) (void)hipStreamDestroy(s);
+    return HBI_OK;
+}
+
+int hbi_fv_set_state(hbi_fv *f, const double *y, const double *pf, const double *disp, double t, double dt_next)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(f && y && pf, "NULL argument");
+        const int n = f->n;
+        f->y.upload(y, 4 * (size_t)n, f->stream);
+        f->pf.upload(pf, n, f->stream);
+        if (disp) f->disp.upload(disp, n, f->stream);
+        else HBI_HIP(hipMemsetAsync(f->disp.p, 0, n * 8, f->stream));
+        hbi::FvCtl c{};
+        hbi::to_device_sync(f->ctl.p, &c, sizeof(c), f->stream);
+        // V of the state (main_fv.f90:995-1003)
+        hbi::fv_stage0_kernel<<<f->nblk, hbi::kFvThreads, 0, f->stream>>>(f->P, f->zero.p, hbi::fv_arrays(f), 0);
+        HBI_HIP(hipGetLastError());
+        c.h = dt_next;
+        c.t = t;
+        hbi::to_device_sync(f->ctl.p, &c, sizeof(c), f->stream);
+        HBI_HIP(hipStreamSynchronize(f->stream));
+    });
+}
+
+int hbi_fv_get_state(hbi_fv *f, double *y, double *pf, double *disp, double *vel, double *qflow, hbi_fv_info *info)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(f, "NULL argument");
+        const int n = f->n;
+        hbi::FvCtl c;
+        HBI_HIP(hipMemcpyAsync(&c, f->ctl.p, sizeof(c), hipMemcpyDeviceToHost, f->stream));
+        if (y) f->y.download(y, 4 * (size_t)n, f->stream);
+        if (pf) f->pf.download(pf, n, f->stream);
+        if (disp) f->disp.download(disp, n, f->stream);
+        if (vel) f->vel.download(vel, n, f->stream);
+        if (qflow) f->qflow.download(qflow, n, f->stream);
+        HBI_HIP(hipStreamSynchronize(f->stream));
+        if (info) {
+            info->t = c.t; info->dt_next = c.h; info->dt_last = c.hdid; info->errmax_last = c.errmax;
+            info->attempts = c.attempts; info->accepted = c.accepted; info->rejected = c.rejected;
+            info->vmax = c.vmax; info->niter = c.niter; info->status = c.status;
+        }
+    });
+}
+
+int hbi_fv_derivs(hbi_fv *f, const double *y, double *dydx)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(f && y && dydx, "NULL argument");
+        const size_t m4 = 4 * (size_t)f->n;
+        hbi::to_device_sync(f->ytmp.p, y, m4 * 8, f->stream);
+        hbi::fv_launch_stages(f, 1, 1, f->zero.p);
+        HBI_HIP(hipGetLastError());
+        HBI_HIP(hipMemcpyAsync(dydx, f->k.p, m4 * 8, hipMemcpyDeviceToHost, f->stream));
+        HBI_HIP(hipStreamSynchronize(f->stream));
+    });
+}
+
+int hbi_fv_diffuse(hbi_fv *f, const double *pf, const double *y, const double *sigma, double h, double t,
+                   double *pfnew, int *niter, double *dtnxt)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(f && pf && y && sigma && pfnew && niter && dtnxt, "NULL argument");
+        HBI_REQUIRE(h > 0, "h must be > 0");
+        const int n = f->n;
+        f->io_pf.upload(pf, n, f->stream);
+        f->io_y.upload(y, 4 * (size_t)n, f->stream);
+        f->io_sig.upload(sigma, n, f->stream);
+        f->io_dt.upload(dtnxt, 1, f->stream);
+        hbi::FvSolveIO io{};
+        io.pf = f->io_pf.p; io.y = f->io_y.p; io.sigma = f->io_sig.p; io.pfnew = f->io_out.p; io.rows = f->rows.p;
+        io.work = f->work.p;
+        io.h = h; io.t = t; io.dtnxt = f->io_dt.p; io.niter = f->io_niter.p; io.gated = 0;
+        hbi::fv_launch_pressure(f, hbi::fv_arrays(f), io);
+        HBI_HIP(hipGetLastError());
+        f->io_out.download(pfnew, n, f->stream);
+        f->io_niter.download(niter, 1, f->stream);
+        f->io_dt.download(dtnxt, 1, f->stream);
+        HBI_HIP(hipStreamSynchronize(f->stream));
+    });
+}
+
+int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(f && nsteps >= 0, "bad arguments");
+        const size_t hneed = hist ? hbi::kFvHist * (size_t)nsteps : 0;
+        bool recapture = !f->graph;
+        if (hneed > f->hist.n || (hneed == 0 && f->hist.n)) {
+            if (hneed) f->hist.alloc(hneed); else f->hist.release();
+            recapture = true;
+        }
+        if (recapture) hbi::fv_capture(f);
+        hbi::fv_arm_kernel<<<1, 1, 0, f->stream>>>(f->ctl.p, nsteps);
+        HBI_HIP(hipEventRecord(f->e0, f->stream));
+        hbi::FvCtl c;
+        long long launched = 0;
+        const long long budget = 50LL * nsteps + 100;
+        int batch = nsteps;
+        for (;;) {
+            for (int q = 0; q < batch; q++) HBI_HIP(hipGraphLaunch(f->graph, f->stream));
+            launched += batch;
+            HBI_HIP(hipMemcpyAsync(&c, f->ctl.p, sizeof(c), hipMemcpyDeviceToHost, f->stream));
+            HBI_HIP(hipStreamSynchronize(f->stream));
+            if (c.done || launched >= budget || nsteps == 0) break;
+            batch = (int)std::max<long long>(1, std::min<long long>(c.target - c.accepted + 2, budget - launched));
+        }
+        HBI_HIP(hipEventRecord(f->e1, f->stream));
+        HBI_HIP(hipEventSynchronize(f->e1));
+        float ms = 0;
+        HBI_HIP(hipEventElapsedTime(&ms, f->e0, f->e1));
+        f->last_ms = ms;
+        const long long done = c.accepted - c.hist_base;
+        if (hist && done > 0) {
+            f->hist.download(hist, hbi::kFvHist * (size_t)std::min<long long>(done, nsteps), f->stream);
+            HBI_HIP(hipStreamSynchronize(f->stream));
+        }
+        if (info) {
+            info->t = c.t; info->dt_next = c.h; info->dt_last = c.hdid; info->errmax_last = c.errmax;
+            info->attempts = c.attempts; info->accepted = c.accepted; info->rejected = c.rejected;
+            info->vmax = c.vmax; info->niter = c.niter; info->status = c.status;
+        }
+        if (c.status == HBI_E_STEP) throw hbi::Error{HBI_E_STEP, "Runge-Kutta step size underflow"};
+    });
+}
+
+int hbi_fv_last_ms(const hbi_fv *f, double *ms)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(f && ms, "NULL argument");
+        *ms = f->last_ms;
+    });
+}
+
+}  // extern "C"

@@ -80,6 +80,28 @@ class RkInfo(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+FV_NEUMANN, FV_DIRICHLET = 0, 1
+FV_INJECTIONQ, FV_INJECTIONP, FV_STEADYSTATE, FV_THRUST, FV_OTHER = 0, 1, 2, 3, 4
+FV_MAX_CELLS = 8192
+
+
+class FvOpts(C.Structure):
+    _fields_ = [("law", C.c_int), ("backslip", C.c_int), ("prec", C.c_int), ("pressure_dependent", C.c_int),
+                ("pfconst", C.c_int), ("bcl", C.c_int), ("bcr", C.c_int), ("setting", C.c_int),
+                ("outpertime", C.c_int)] + [(k, C.c_double) for k in (
+                    "vref", "eps_r", "hdiv", "ds0", "kL", "kpmin", "s0", "eta", "beta", "Bs", "pbcl", "pbcr", "q0",
+                    "pinj", "tinj_s", "dtout_s", "velmin", "velmax", "tmax_s")]
+
+
+class FvInfo(C.Structure):
+    _fields_ = [("t", C.c_double), ("dt_next", C.c_double), ("dt_last", C.c_double), ("errmax_last", C.c_double),
+                ("attempts", C.c_longlong), ("accepted", C.c_longlong), ("rejected", C.c_longlong),
+                ("vmax", C.c_double), ("niter", C.c_int), ("status", C.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class HbiError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("hbi error %d: %s" % (code, msg))
@@ -96,7 +118,9 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_mlp_set_params", "hbi_mlp_get_params", "hbi_mlp_forward", "hbi_mlp_loss_grad", "hbi_mlp_train",
            "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace", "hbi_ens_set_controller",
            "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_rk_step_many", "hbi_rk_set_output", "hbi_rk_get_output",
-           "hbi_rk_set_opening", "hbi_rk_get_opening", "hbi_rk_set_probes", "hbi_rk_get_probes")
+           "hbi_rk_set_opening", "hbi_rk_get_opening", "hbi_rk_set_probes", "hbi_rk_get_probes",
+           "hbi_fv_create", "hbi_fv_destroy", "hbi_fv_set_state", "hbi_fv_get_state", "hbi_fv_derivs",
+           "hbi_fv_diffuse", "hbi_fv_step", "hbi_fv_last_ms")
 
 
 def lib():
@@ -155,6 +179,14 @@ def lib():
         L.hbi_rk_get_opening.argtypes = [C.c_void_p, _dp, _dp]
         L.hbi_rk_set_probes.argtypes = [C.c_void_p, C.c_int, _ip]
         L.hbi_rk_get_probes.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), _dp]
+        L.hbi_fv_create.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(FvOpts)] + [_dp] * 11 + [C.POINTER(C.c_void_p)]
+        L.hbi_fv_destroy.argtypes = [C.c_void_p]
+        L.hbi_fv_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double]
+        L.hbi_fv_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.POINTER(FvInfo)]
+        L.hbi_fv_derivs.argtypes = [C.c_void_p, _dp, _dp]
+        L.hbi_fv_diffuse.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _ip, _dp]
+        L.hbi_fv_step.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(FvInfo)]
+        L.hbi_fv_last_ms.argtypes = [C.c_void_p, _dp]
         L.hbi_rk_get_output.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), _dp, _dp,
                                         C.POINTER(C.c_int), C.POINTER(RkEvent), _dp]
         L.hbi_ens_get_trace.argtypes = [C.c_void_p, _dp, _dp, _ip]
@@ -604,3 +636,84 @@ class Ensemble:
         cnt = np.zeros(self.nreal, dtype=np.int32)
         _check(lib().hbi_ens_get_trace(self.e, _p(feat), _p(raw), _ipp(cnt)))
         return [(feat[r, :cnt[r]], raw[r, :cnt[r]]) for r in range(self.nreal)]
+
+
+FV_ARRAYS = ("a", "b", "dc", "f0", "vpl", "kpmaxv", "kTv", "phi", "pfhyd", "qin", "area")
+
+
+class FV:
+    """Device fluid-driven cycle of main_fv.f90 (hbi_fv_*): 4-component
+    rate-and-state RK45 on effective normal stress with permeability
+    evolution, and the implicit pore-pressure step after every accepted step.
+    ``opts``: FvOpts field values; ``arrays``: the FV_ARRAYS in cluster order."""
+
+    def __init__(self, hs: HMatrix, opts: dict, arrays: dict, hn: Optional[HMatrix] = None):
+        self.hs, self.hn = hs, hn
+        self.n = hs.n
+        self.keep = [_f64(arrays[k]) for k in FV_ARRAYS]
+        for a in self.keep:
+            if a.size != self.n:
+                raise ValueError("per-element arrays must have N entries")
+        o = FvOpts(**opts)
+        f = C.c_void_p()
+        _check(lib().hbi_fv_create(hs.h, hn.h if hn is not None else None, C.byref(o),
+                                   *[_p(a) for a in self.keep], C.byref(f)))
+        self.f = f
+
+    def close(self):
+        if getattr(self, "f", None):
+            lib().hbi_fv_destroy(self.f)
+            self.f = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, y, pf, disp=None, t=0.0, dt=1.0):
+        y, pf = _f64(y), _f64(pf)
+        disp = _f64(disp) if disp is not None else None
+        _check(lib().hbi_fv_set_state(self.f, _p(y), _p(pf), _p(disp), float(t), float(dt)))
+        self._accepted = 0
+
+    def get_state(self) -> dict:
+        n = self.n
+        out = {"y": np.empty(4 * n), "pf": np.empty(n), "disp": np.empty(n), "vel": np.empty(n),
+               "qflow": np.empty(n)}
+        info = FvInfo()
+        _check(lib().hbi_fv_get_state(self.f, _p(out["y"]), _p(out["pf"]), _p(out["disp"]), _p(out["vel"]),
+                                      _p(out["qflow"]), C.byref(info)))
+        out["info"] = info.as_dict()
+        return out
+
+    def derivs(self, y):
+        y = _f64(y)
+        d = np.empty_like(y)
+        _check(lib().hbi_fv_derivs(self.f, _p(y), _p(d)))
+        return d
+
+    def diffuse(self, pf, y, sigma, h, t, dtnxt):
+        """One implicit pore-pressure step: (pfnew, CG iterations, dt bound)."""
+        pf, y, sigma = _f64(pf), _f64(y), _f64(sigma)
+        out = np.empty(self.n)
+        it = C.c_int(0)
+        dt = C.c_double(dtnxt)
+        _check(lib().hbi_fv_diffuse(self.f, _p(pf), _p(y), _p(sigma), float(h), float(t), _p(out), C.byref(it),
+                                    C.byref(dt)))
+        return out, it.value, dt.value
+
+    def step(self, nsteps, history=False):
+        """nsteps accepted steps; history rows: t, log10 max|V|, mean disp,
+        mean mu, max sigma_e, min sigma_e, argmax (1-based), errmax, dt, CG iterations."""
+        hist = np.zeros((nsteps, 10)) if history else None
+        info = FvInfo()
+        _check(lib().hbi_fv_step(self.f, int(nsteps), _p(hist), C.byref(info)))
+        d = info.as_dict()
+        done = d["accepted"] - getattr(self, "_accepted", 0)
+        self._accepted = d["accepted"]
+        if history:
+            hist = hist[:max(0, min(nsteps, done))]
+        return d, hist
+
+    def last_ms(self) -> float:
+        ms = C.c_double(0)
+        _check(lib().hbi_fv_last_ms(self.f, C.byref(ms)))
+        return ms.value

@@ -456,3 +456,49 @@ def write_in_file(path: str, keys: Dict[str, object]) -> None:
                 if isinstance(item, bool):
                     item = "T" if item else "F"
                 f.write("%s %s\n" % (k, item))
+
+
+# ------------------------------------------------------------ fluid drivers --
+YEAR_S = 365 * 24 * 3600
+
+
+def fluid_injection(g: Geometry, lod, setting: str = "injectionq", a=0.007, b=0.005, dc=4e-3, f0=0.6, vref=1e-6,
+                    vpl=1e-9, velinit=1e-9, sigmainit=50.0, pfinit=0.0, tauinit=0.0, muinit=0.58, phi0=0.1,
+                    beta=1e-8, eta=1e-3, kpmin=1e-13, kpmax=1e-12, kL=1e8, kT=1e12, s0=30.0, q0=1.25e-6,
+                    pinj=10.0, tinj=1.0, pbcl=0.0, pbcr=0.0, bcl="Neumann", bcr="Neumann", law=0,
+                    pressure_dependent=False, pfconst=False, eps_r=1e-4, backslip=True, prec=0,
+                    velmin=1e-16, velmax=1e7, tmax=1e12, outpertime=False, dtout=1e3, Bs=0.0):
+    """Options, per-element arrays (cluster order) and the initial state
+    (y, pf) of the injection settings of main_fv.f90: initcond_injectionq
+    (1596-1643: a source q0 at element N/2, active while t < tinj years) and
+    initcond_injectionp (1548-1594: pf = pinj at N/2, held there by the
+    penalty of Beuler).  Uniform friction; the defaults of main_fv.f90:130-162
+    for kL, kT, Bs, tmax, velmin, velmax, minsig; 2-D grid spacing ds0."""
+    n = g.n
+    lod = np.asarray(lod)
+    one = np.ones(n)
+    kind = {"injectionq": 0, "injectionp": 1, "steadystate": 2, "thrust": 3}.get(setting, 4)
+    qin = np.zeros(n)
+    if setting == "injectionq":
+        qin[n // 2 - 1] = q0
+    arrays = dict(a=a * one, b=b * one, dc=dc * one, f0=f0 * one, vpl=vpl * one, kpmaxv=kpmax * one, kTv=kT * one,
+                  phi=phi0 * one, pfhyd=np.zeros(n), qin=qin, area=np.asarray(g.ds, dtype=np.float64)[lod])
+    pf = pfinit * one
+    if setting == "injectionp":
+        pf[n // 2 - 1] = pinj
+    sigma = sigmainit * one
+    sige = sigma - pf
+    tau = tauinit * one
+    if muinit != 0.0 or setting == "injectionp":
+        tau = sige * muinit
+    vel = velinit * one
+    psi = arrays["a"] * np.log(2 * vref / vel * np.sinh(tau / sige / arrays["a"]))
+    y = np.empty(4 * n)
+    y[0::4], y[1::4], y[2::4], y[3::4] = psi, tau, sige, kpmin * one
+    opts = dict(law=law, backslip=int(backslip), prec=prec, pressure_dependent=int(pressure_dependent),
+                pfconst=int(pfconst), bcl=0 if bcl == "Neumann" else 1, bcr=0 if bcr == "Neumann" else 1,
+                setting=kind, outpertime=int(outpertime), vref=vref, eps_r=eps_r, hdiv=0.0, ds0=float(g.ds0),
+                kL=kL, kpmin=kpmin, s0=s0, eta=eta, beta=beta, Bs=Bs, pbcl=pbcl, pbcr=pbcr, q0=q0, pinj=pinj,
+                tinj_s=tinj * 365 * 24 * 3600, dtout_s=dtout * 365 * 24 * 3600, velmin=velmin, velmax=velmax,
+                tmax_s=tmax * 365 * 24 * 3600)
+    return opts, arrays, y, pf

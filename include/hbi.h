@@ -349,6 +349,98 @@ int hbi_ens_set_trace(hbi_ens *e, int cap);
 int hbi_ens_get_trace(hbi_ens *e, double *feat, double *raw, int *count);
 int hbi_ens_last_ms(const hbi_ens *e, double *ms);
 
+/* ---------------------------------------------------------------------------
+   Fluid-driven cycle (main_fv.f90): rate-and-state friction on effective
+   normal stress sigma_e = sigma - pf with permeability evolution, and
+   pore-pressure diffusion along the fault solved implicitly after every
+   accepted step.
+   State per element (cluster order, interleaved as main_fv.f90:995-1003):
+   y = (psi, tau, sigma_e, ks); pf, disp, vel alongside.  One accepted step:
+     rkqs with the 4-component derivs (main_fv.f90:2039-2146, 2149-2256: the
+       error is max |yerr / yout| / eps over all 4N components);
+     post-RK update (1016-1028: sigma = sigma_e + pf, V from the new y);
+     the implicit pore-pressure step (implicitsolver2 1727-1774, or with
+       pressure_dependent implicitsolver 1678-1724, Picard iterations on the
+       diffusivity) through Beuler (1776-1940): backward Euler on the 1-D
+       finite-volume grid of spacing ds0 over cluster-order neighbours,
+       boundary penalties (Neumann / Dirichlet), the setting's source terms,
+       solved by conjugate gradients to |r|^2 < tol^2 N (tol = 1e-4, at most
+       10000 iterations); the next step size is limited by dpf;
+     field update (1045-1082: sigma_e = sigma - pf, disp trapezoid, V, mu, kp,
+       qflow) and the stop criteria (1136-1152).
+   The pore-pressure grid holds at most 8192 elements (one workgroup solves
+   it, the vectors in registers and LDS). */
+#define HBI_FV_NEUMANN 0
+#define HBI_FV_DIRICHLET 1
+#define HBI_FV_INJECTIONQ 0
+#define HBI_FV_INJECTIONP 1
+#define HBI_FV_STEADYSTATE 2
+#define HBI_FV_THRUST 3
+#define HBI_FV_OTHER 4       /* 'swarm' and others: no source terms */
+#define HBI_FV_MAX_CELLS 8192
+
+typedef struct hbi_fv_opts {
+    int law;                 /* 0 aging, 1 slip (evlaw) */
+    int backslip;            /* matvec input V - Vpl (1) or V (0) */
+    int prec;                /* HBI_PREC_* of the matvecs */
+    int pressure_dependent;  /* implicitsolver (1) or implicitsolver2 (0) */
+    int pfconst;             /* 1: pf is not updated */
+    int bcl, bcr;            /* HBI_FV_NEUMANN / HBI_FV_DIRICHLET */
+    int setting;             /* HBI_FV_INJECTIONQ ... */
+    int outpertime;          /* hnext <= dtout_s (main_fv.f90:2233-2235, 1718-1720) */
+    double vref, eps_r;
+    double hdiv;             /* matvec results divided by this (ds0 for 3dph / 3dhr), 0 = none */
+    double ds0;              /* grid spacing of the diffusion */
+    double kL, kpmin, s0, eta, beta, Bs;
+    double pbcl, pbcr;       /* Dirichlet boundary pressures */
+    double q0, pinj;         /* source rate (steadystate / injectionq through qin) and injection pressure */
+    double tinj_s;           /* injectionq: source active while t < tinj_s (seconds) */
+    double dtout_s;          /* outpertime bound, seconds */
+    double velmin, velmax, tmax_s;   /* stop criteria */
+} hbi_fv_opts;
+
+typedef struct hbi_fv_info {
+    double t, dt_next, dt_last, errmax_last;
+    long long attempts, accepted, rejected;
+    double vmax;
+    int niter;               /* conjugate-gradient iterations of the last pore-pressure solve */
+    int status;              /* 0 running; 1 V outside [velmin, velmax]; 3 t > tmax; 4 min sigma_e < 0.2;
+                                HBI_E_STEP on step-size underflow */
+} hbi_fv_info;
+
+typedef struct hbi_fv hbi_fv;
+/* Per-element arrays (cluster order, host pointers): friction a, b, dc, f0,
+   vpl; permeability kpmaxv (kpmax of the evolution law), kTv (healing time),
+   phi (porosity), pfhyd (hydrostatic pressure), qin (source rates; thrust
+   reads qin at 1-based index 2000 for its right boundary, so it needs N >=
+   2000), area (ds, the monitor weights).  hn may be NULL (sigmaconst). */
+int hbi_fv_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_fv_opts *opts, const double *a, const double *b,
+                  const double *dc, const double *f0, const double *vpl, const double *kpmaxv,
+                  const double *kTv, const double *phi, const double *pfhyd, const double *qin,
+                  const double *area, hbi_fv **out);
+int hbi_fv_destroy(hbi_fv *f);
+/* y: 4N as above; pf, disp: N (disp may be NULL -> 0) */
+int hbi_fv_set_state(hbi_fv *f, const double *y, const double *pf, const double *disp, double t, double dt_next);
+/* any output pointer may be NULL; vel = V after the last step; qflow
+   (main_fv.f90:1071-1076) */
+int hbi_fv_get_state(hbi_fv *f, double *y, double *pf, double *disp, double *vel, double *qflow,
+                     hbi_fv_info *info);
+/* dydx of main_fv.f90:2039-2146 at y (4N) */
+int hbi_fv_derivs(hbi_fv *f, const double *y, double *dydx);
+/* One implicit pore-pressure step of length h at time t (implicitsolver2, or
+   implicitsolver with pressure_dependent) from pf, y (ks and sigma_e) and
+   sigma: pfnew, the conjugate-gradient iterations and the step bound dtnxt
+   (in: the proposed next step; out: after the dpf limit). */
+int hbi_fv_diffuse(hbi_fv *f, const double *pf, const double *y, const double *sigma, double h, double t,
+                   double *pfnew, int *niter, double *dtnxt);
+/* Advance nsteps accepted steps (one captured HIP graph per attempt, the
+   pore-pressure solve inside it).  hist (may be NULL): nsteps rows of 10:
+   t, log10 max|V|, mean disp, mean mu, max sigma_e, min sigma_e, 1-based
+   cluster index of the first max |V|, errmax, dt, CG iterations (the
+   output_monitor columns of main_fv.f90:1307-1311). */
+int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info);
+int hbi_fv_last_ms(const hbi_fv *f, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,163 @@
+"""GPU: the fluid-driven cycle (hbi_fv_*, main_fv.f90) against the oracle
+(oracle/fluid.py) on the same inputs:
+
+* derivs (4 components) with and without the normal-stress matrix, aging and
+  slip laws
+* the implicit pore-pressure step for every setting and boundary pair, both
+  solvers, at the four grid sizes of the solve kernel's variants
+* the first steps of injection runs: identical step grids, CG iteration
+  counts, monitor rows and states
+* at 8192 elements (the largest grid) the run holds the injected volume
+"""
+import numpy as np
+import pytest
+
+from hbi_amd import lib as H
+from hbi_amd import problems as P
+from oracle import fluid as F
+from oracle import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = float(np.float32(1e-4))    # the solver's residual tolerance (main_fv.f90:1785)
+
+
+def _pair(g, kw, normal=False, eps=1e-8):
+    Hs = H.HMatrix(g, leaf=32, eps=eps)
+    Os = O.HMatrix(g, leaf=32, eps=eps, nthreads=8)
+    Hn = On = None
+    if normal:
+        import dataclasses
+        gn = dataclasses.replace(g, vcomp=1)
+        Hn, On = H.HMatrix(gn, leaf=32, eps=eps), O.HMatrix(gn, leaf=32, eps=eps, nthreads=8)
+    opts, arr, y, pf = P.fluid_injection(g, Hs.lod, **kw)
+    dev = H.FV(Hs, opts, arr, hn=Hn)
+    ora = F.FvOracle(Os, opts, arr, hn=On)
+    return dev, ora, y, pf, (Hs, Hn)
+
+
+@pytest.mark.parametrize("law,normal", [(0, False), (1, False), (0, True)])
+def test_gpu_fluid_derivs_against_oracle(law, normal):
+    g = P.mesh_2dph(384, 0.01, 45.0) if normal else P.mesh_2dp(384, 0.01)
+    dev, ora, y, pf, _ = _pair(g, dict(law=law), normal=normal)
+    rng = np.random.default_rng(5)
+    y[1::4] *= 1 + 0.02 * rng.standard_normal(g.n)
+    y[3::4] *= 1 + rng.random(g.n)
+    dg, do = dev.derivs(y), ora.derivs(y)
+    print("derivs law", law, "normal", normal, flush=True)
+    for c in range(4):
+        scale = np.abs(do[c::4]).max()
+        assert np.abs(dg[c::4] - do[c::4]).max() <= 1e-9 * scale + 1e-300, c
+    if normal:
+        assert np.abs(do[2::4]).max() > 0
+
+
+DIFFUSE_CASES = [
+    # (n, setting, bcl, bcr, pressure_dependent)
+    (1000, "injectionq", "Neumann", "Neumann", False),
+    (1000, "injectionp", "Dirichlet", "Neumann", True),
+    (2000, "steadystate", "Neumann", "Neumann", False),
+    (2000, "thrust", "Neumann", "Neumann", False),
+    (4000, "other", "Dirichlet", "Dirichlet", True),
+    (8192, "injectionq", "Neumann", "Dirichlet", False),
+]
+
+
+@pytest.mark.parametrize("n,setting,bcl,bcr,pdep", DIFFUSE_CASES)
+def test_gpu_fluid_pressure_step_against_oracle(n, setting, bcl, bcr, pdep):
+    g = P.mesh_2dp(n, 0.01)
+    kw = dict(setting=setting, bcl=bcl, bcr=bcr, pressure_dependent=pdep, pbcl=4.0, pbcr=1.0, pinj=8.0)
+    dev, ora, y, pf, _ = _pair(g, kw, eps=1e-4)
+    rng = np.random.default_rng(n)
+    if setting == "thrust":   # sources along the fault and at qin(2000)
+        ora.arr["qin"][:] = 1e-7 * rng.standard_normal(n)
+        dev = H.FV(dev.hs, ora.o, ora.arr)
+    pf = pf + rng.random(n)
+    y[3::4] = 1e-13 * (1 + 4 * rng.random(n))
+    sigma = y[2::4] + pf
+    for h in (2e3, 3e5):
+        print("pressure step", n, setting, bcl, bcr, pdep, h, flush=True)
+        pg, ig, dtg = dev.diffuse(pf, y, sigma, h, 100.0, 1e8)
+        po, io, dto = ora.diffuse(pf, y, sigma, h, 100.0, 1e8)
+        # the stopping iteration may move by one under other summation orders
+        assert ig > 0 and abs(ig - io) <= max(1, io // 50), (ig, io)
+        # both results solve the same system to the residual bound: they are
+        # 2 tol sqrt(N) / lambda_min apart at most (lambda_min >= 1/2)
+        assert np.linalg.norm(pg - po) <= 4 * TOL * np.sqrt(n)
+        if ig == io and io < 60:   # few iterations: rounding has not grown
+            assert np.abs(pg - po).max() <= 1e-9 * np.abs(po).max(), (ig, np.abs(pg - po).max())
+        if not pdep:
+            # the device's pfnew is a converged solution of the reference's system
+            lo, di, up, b, _ = ora.beuler_system(pf, ora.cdiff(y[3::4], sigma, pf), h, 100.0)
+            r = b - ora.apply(lo, di, up, pg - ora.arr["pfhyd"])
+            assert F.seqsum(r * r) < 1.001 * TOL ** 2 * n
+        assert abs(dtg - dto) <= 1e-5 * abs(dto)
+
+
+@pytest.mark.parametrize("setting,pdep", [("injectionq", False), ("injectionp", True)])
+def test_gpu_fluid_run_against_oracle(setting, pdep):
+    n = 256
+    g = P.mesh_2dp(n, 0.01)
+    dev, ora, y, pf, _ = _pair(g, dict(setting=setting, pressure_dependent=pdep))
+    dev.set_state(y, pf, t=0.0, dt=1.0)
+    st = dict(y=y.copy(), pf=pf.copy(), disp=np.zeros(n), t=0.0, h=1.0)
+    rows = [ora.step(st) for _ in range(25)]
+    info, hist = dev.step(25, history=True)
+    print("run", setting, info, flush=True)
+    assert info["status"] == 0 and info["accepted"] == 25 and len(hist) == 25
+    assert info["rejected"] == sum(r["rejected"] for r in rows)
+    for k, r in enumerate(rows):
+        hr = hist[k]
+        assert abs(hr[8] - r["dt"]) <= 1e-9 * r["dt"], k
+        assert abs(hr[0] - r["t"]) <= 1e-9 * r["t"], k
+        assert int(hr[9]) == r["niter"] or abs(int(hr[9]) - r["niter"]) <= 1, k
+        assert int(hr[6]) == r["argmax"], k
+        assert abs(hr[1] - r["log10vmax"]) <= 1e-9 * abs(r["log10vmax"]), k
+        assert abs(hr[3] - r["meanmu"]) <= 1e-12 * abs(r["meanmu"]), k
+        assert abs(hr[5] - r["minnorm"]) <= 1e-9 * abs(r["minnorm"]), k
+    s = dev.get_state()
+    assert np.abs(s["pf"] - st["pf"]).max() <= 1e-6 * max(1.0, np.abs(st["pf"]).max())
+    for c in range(4):
+        ref = st["y"][c::4]
+        assert np.abs(s["y"][c::4] - ref).max() <= 1e-9 * np.abs(ref).max(), c
+    assert np.abs(s["disp"] - st["disp"]).max() <= 1e-8 * np.abs(st["disp"]).max()
+    assert np.abs(s["vel"] - st["vel"]).max() <= 1e-8 * np.abs(st["vel"]).max()
+    assert np.abs(s["qflow"] - st["qflow"]).max() <= 1e-6 * np.abs(st["qflow"]).max() + 1e-300
+    assert s["info"]["niter"] == int(hist[-1, 9])
+
+
+def test_gpu_fluid_full_grid_holds_the_injected_volume():
+    """8192 elements (the largest pore-pressure grid): over 60 steps the
+    half-weighted pressure sum equals the injected volume to the solver's
+    residual bound, and pf peaks at the source."""
+    n = H.FV_MAX_CELLS
+    g = P.mesh_2dp(n, 0.01)
+    Hs = H.HMatrix(g, leaf=32, eps=1e-4)
+    opts, arr, y, pf = P.fluid_injection(g, Hs.lod, setting="injectionq")
+    dev = H.FV(Hs, opts, arr)
+    dev.set_state(y, pf, t=0.0, dt=1.0)
+    info, hist = dev.step(60, history=True)
+    print("full grid", info, flush=True)
+    assert info["status"] == 0 and len(hist) == 60
+    s = dev.get_state()
+    w = np.ones(n)
+    w[0] = w[-1] = 0.5
+    k = n // 2 - 1
+    rate = arr["qin"][k] / opts["beta"] / arr["phi"][k] * float(np.float32(1e-9)) / opts["ds0"]
+    vol = rate * hist[:, 8].sum()
+    assert vol > 0
+    assert abs(np.dot(w, s["pf"]) - vol) <= 60 * TOL * n
+    assert int(np.argmax(s["pf"])) == k and hist[:, 9].max() > 0
+
+
+def test_gpu_fluid_rejects_unsupported_grids():
+    g = P.mesh_2dp(H.FV_MAX_CELLS + 8, 0.01)
+    Hs = H.HMatrix(g, leaf=32, eps=1e-4)
+    opts, arr, y, pf = P.fluid_injection(g, Hs.lod)
+    with pytest.raises(H.HbiError):
+        H.FV(Hs, opts, arr)
+    g = P.mesh_2dp(512, 0.01)
+    Hs = H.HMatrix(g, leaf=32, eps=1e-4)
+    opts, arr, y, pf = P.fluid_injection(g, Hs.lod, setting="thrust")
+    with pytest.raises(H.HbiError):
+        H.FV(Hs, opts, arr)
