@@ -47,6 +47,8 @@ TRAFFIC_MR_TILE_MIXED = 420.6e6
 TRAFFIC_STREAM_MIXED = 225.7e6
 IMAX, JMAX, DS0, LEAF, EPS_H, EPS_R = 256, 128, 0.390625, 64, 1e-5, 1e-6
 NREAL = 64
+FLUID_STEPS = 500      # timed accepted steps of the bp6a fluid-driven cycle
+FLUID_CPU_STEPS = 200
 
 
 def _onset(t, lv, thr=-3.0):
@@ -132,6 +134,23 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
         "max_abs_dlnV_to_half_onset": dlnv,
         "note": "slip-rate histories at %d elements compared on a common time grid (step grids differ)" % len(rec)}
 
+    # the fluid-driven cycle (main_fv.f90) on the reference's bp6a example
+    print("bench: fluid-driven cycle bp6a, 4000 elements", file=sys.stderr, flush=True)
+    gf = P.mesh_2dp(P.BP6A_N, P.BP6A_DS0)
+    Hf = H.HMatrix(gf, leaf=32, eps=1e-4)
+    fo, fa, fy, fpf = P.fluid_bp6a(gf, Hf.lod)
+    fv = H.FV(Hf, fo, fa)
+    fv.set_state(fy, fpf, t=0.0, dt=1.0)
+    fv.step(20)
+    t0 = time.perf_counter()
+    finfo, fhist = fv.step(FLUID_STEPS, history=True)
+    wall = time.perf_counter() - t0
+    out["fluid_bp6a"] = {"steps_per_s": round(FLUID_STEPS / wall, 1), "ms_per_step": round(1e3 * wall / FLUID_STEPS, 4),
+                         "device_ms": round(fv.last_ms(), 3), "attempts": finfo["attempts"],
+                         "cg_iter_mean": round(float(fhist[:, 9].mean()), 1), "t_end_s": finfo["t"],
+                         "log10_vmax_end": round(float(fhist[-1, 1]), 4)}
+    fv.close()
+
     # batched form: 64 realisations in lockstep, mixed precision, rkqs and learned step size
     from hbi_amd import learned as L
     mlp = L.load(os.path.join(ROOT, "hbi_amd", "data", "step_mlp_c2.npz"))
@@ -206,9 +225,27 @@ def cpu_part(nsteps: int):
     t0 = time.perf_counter()
     done, *_ = sim.run(y, 0.0, 1.0, EPS_R, nsteps, slip, vel, np.asarray(g.ds)[lod])
     wall = time.perf_counter() - t0
-    return {"value": done / wall, "unit": "accepted RK steps/s (one realisation)", "cores": ncores, "kind": "port",
-            "sample": "%d accepted RK45 steps (FP64 oracle, OpenMP) of one realisation of the 32768-element fault "
-                      "after a %.1f s H-matrix build" % (done, t_build)}
+    out = {"value": done / wall, "unit": "accepted RK steps/s (one realisation)", "cores": ncores, "kind": "port",
+           "sample": "%d accepted RK45 steps (FP64 oracle, OpenMP) of one realisation of the 32768-element fault "
+                     "after a %.1f s H-matrix build" % (done, t_build)}
+    Ho = None
+    # the fluid-driven cycle's oracle (numpy, C oracle products) on bp6a
+    from oracle import fluid as F
+    gf = P.mesh_2dp(P.BP6A_N, P.BP6A_DS0)
+    Hf = O.HMatrix(gf, leaf=32, eps=1e-4, nthreads=ncores)
+    fo, fa, fy, fpf = P.fluid_bp6a(gf, Hf.lod)
+    ora = F.FvOracle(Hf, fo, fa)
+    st = dict(y=fy.copy(), pf=fpf.copy(), disp=np.zeros(gf.n), t=0.0, h=1.0)
+    for _ in range(20):
+        ora.step(st)
+    t0 = time.perf_counter()
+    for _ in range(FLUID_CPU_STEPS):
+        ora.step(st)
+    wall = time.perf_counter() - t0
+    out["fluid_bp6a"] = {"steps_per_s": round(FLUID_CPU_STEPS / wall, 2), "cores": ncores,
+                         "sample": "%d accepted steps after 20 warm-up steps (numpy oracle, OpenMP products)"
+                                   % FLUID_CPU_STEPS}
+    return out
 
 
 def main():
@@ -272,6 +309,7 @@ def main():
         "matmul64_plan": r["matmul64_plan"],
         "mfma_loop_tfs": {"f64": r["mfma_f64_loop_tfs"], "f32": r["mfma_f32_loop_tfs"]},
         "single": r["single"],
+        "fluid_bp6a": r["fluid_bp6a"],
         "build_s": r["build_s"],
         "hmat": r["hmat"],
     }

@@ -40,6 +40,22 @@ module m_hbi_abi
     real(c_double) :: hdiv = 0d0, eps_r = 1d-4, velmin = 1d-20, velmax = 1d7
   end type hbi_rk_opts
 
+  ! fluid-driven cycle of main_fv.f90 (hbi_fv_*); defaults of main_fv.f90:130-162
+  type, bind(C) :: hbi_fv_opts
+    integer(c_int) :: law = 0, backslip = 0, prec = 0, pressure_dependent = 0, pfconst = 0
+    integer(c_int) :: bcl = 0, bcr = 0, setting = 0, outpertime = 0
+    real(c_double) :: vref = 1d-6, eps_r = 1d-4, hdiv = 0d0, ds0 = 0d0, kL = 1d8, kpmin = 0d0, s0 = 0d0
+    real(c_double) :: eta = 0d0, beta = 0d0, Bs = 0d0, pbcl = 0d0, pbcr = 0d0, q0 = 0d0, pinj = 0d0
+    real(c_double) :: tinj_s = 0d0, dtout_s = 0d0, velmin = 1d-16, velmax = 1d7, tmax_s = 1d12
+  end type hbi_fv_opts
+
+  type, bind(C) :: hbi_fv_info
+    real(c_double) :: t, dt_next, dt_last, errmax_last
+    integer(c_long_long) :: attempts, accepted, rejected
+    real(c_double) :: vmax
+    integer(c_int) :: niter, status
+  end type hbi_fv_info
+
   type, bind(C) :: hbi_rk_info
     real(c_double) :: t, dt_next, dt_last, errmax_last
     integer(c_long_long) :: attempts, accepted, rejected
@@ -227,6 +243,38 @@ module m_hbi_abi
       type(c_ptr), value :: r, m
       integer(c_int), value :: mode
       real(c_double), value :: safety, ratio_lo, ratio_hi
+    end function
+    integer(c_int) function hbi_fv_create(hs, hn, o, a, b, dc, f0, vpl, kpmaxv, kTv, phi, pfhyd, qin, area, f) &
+        bind(C, name="hbi_fv_create")
+      import :: c_int, c_ptr, c_double, hbi_fv_opts
+      type(c_ptr), value :: hs, hn
+      type(hbi_fv_opts), intent(in) :: o
+      real(c_double), intent(in) :: a(*), b(*), dc(*), f0(*), vpl(*), kpmaxv(*), kTv(*), phi(*), pfhyd(*)
+      real(c_double), intent(in) :: qin(*), area(*)
+      type(c_ptr), intent(out) :: f
+    end function
+    integer(c_int) function hbi_fv_destroy(f) bind(C, name="hbi_fv_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: f
+    end function
+    integer(c_int) function hbi_fv_set_state(f, y, pf, disp, t, dt) bind(C, name="hbi_fv_set_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: f
+      real(c_double), intent(in) :: y(*), pf(*), disp(*)
+      real(c_double), value :: t, dt
+    end function
+    integer(c_int) function hbi_fv_get_state(f, y, pf, disp, vel, qflow, info) bind(C, name="hbi_fv_get_state")
+      import :: c_int, c_ptr, c_double, hbi_fv_info
+      type(c_ptr), value :: f
+      real(c_double), intent(out) :: y(*), pf(*), disp(*), vel(*), qflow(*)
+      type(hbi_fv_info), intent(out) :: info
+    end function
+    integer(c_int) function hbi_fv_step(f, nsteps, hist, info) bind(C, name="hbi_fv_step")
+      import :: c_int, c_ptr, c_double, hbi_fv_info
+      type(c_ptr), value :: f
+      integer(c_int), value :: nsteps
+      real(c_double), intent(out) :: hist(*)
+      type(hbi_fv_info), intent(out) :: info
     end function
   end interface
 

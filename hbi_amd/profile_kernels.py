@@ -11,8 +11,9 @@ With --copy-mb > 0 it also runs a plain device-to-device copy of that many
 MiB (HIP runtime blit kernel) `iters` times, the reference for scaling the
 profiler's byte counters.  With --ensemble K it instead runs the bench's
 ensemble (64 friction realisations, learned step size, FP32 factors) for 20
-warm-up and K timed lockstep steps, for a kernel trace of the attempt.
-Prints one JSON line.
+warm-up and K timed lockstep steps, for a kernel trace of the attempt; with
+--fluid K the fluid-driven cycle of the bp6a example (4000 elements) for 20
+warm-up and K timed steps.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -32,7 +33,11 @@ def main():
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--copy-mb", type=int, default=0)
     ap.add_argument("--ensemble", type=int, default=0)
+    ap.add_argument("--fluid", type=int, default=0)
     a = ap.parse_args()
+    if a.fluid > 0:
+        print(json.dumps(fluid(a.fluid)), flush=True)
+        return
     g = P.mesh_3dph(IMAX, JMAX, DS0, 90.0)
     Hm = H.HMatrix(g, leaf=LEAF, eps=EPS_H, bf16_margin=1.0)
     if a.ensemble > 0:
@@ -73,6 +78,19 @@ def ensemble(g, Hm, steps: int) -> dict:
     out = {"steps": steps, "attempts": int(att), "device_ms": E.last_ms()}
     E.close()
     return out
+
+
+def fluid(steps: int) -> dict:
+    """bp6a (main_fv): 20 warm-up and `steps` timed accepted steps."""
+    g = P.mesh_2dp(P.BP6A_N, P.BP6A_DS0)
+    Hm = H.HMatrix(g, leaf=32, eps=1e-4)
+    opts, arr, y, pf = P.fluid_bp6a(g, Hm.lod)
+    fv = H.FV(Hm, opts, arr)
+    fv.set_state(y, pf, t=0.0, dt=1.0)
+    fv.step(20)
+    info, hist = fv.step(steps, history=True)
+    return {"steps": steps, "device_ms": fv.last_ms(), "ms_per_step": fv.last_ms() / max(1, len(hist)),
+            "cg_iter_mean": float(hist[:, 9].mean()), "attempts": info["attempts"], "t_end_s": info["t"]}
 
 
 def device_copy(nbytes: int, iters: int) -> dict:
