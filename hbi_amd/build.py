@@ -15,6 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 FSRC = os.path.join(HERE, "fortran")
 LIB = os.path.join(HERE, "libhbi.so")
 DRIVER = os.path.join(HERE, "lhbiem_gpu")
+DRIVER_FV = os.path.join(HERE, "lhbiem_fv_gpu")
 
 SOURCES = ["partition.cpp", "hmat.hip", "rk.hip", "multirhs.hip", "ens.hip", "learn.hip", "fv.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -62,19 +63,23 @@ def build_lib(jobs: int = 4) -> str:
 
 
 def build_driver() -> str:
-    """Fortran host (amdflang) linked against libhbi.so."""
-    srcs = ["m_hbi_abi.f90", "m_hacapk_gpu.f90", "lhbiem_gpu.f90"]
+    """Fortran hosts (amdflang) linked against libhbi.so: lhbiem_gpu (main_LH)
+    and lhbiem_fv_gpu (main_fv)."""
+    srcs = ["m_hbi_abi.f90", "m_hacapk_gpu.f90", "lhbiem_gpu.f90", "lhbiem_fv_gpu.f90"]
     paths = [os.path.join(FSRC, s) for s in srcs]
     if not all(os.path.exists(p) for p in paths) or not shutil.which(FC) and not os.path.exists(FC):
         return ""
     objdir = os.path.join(HERE, "_build", "f")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs = {}
     for p in paths:
         o = os.path.join(objdir, os.path.basename(p) + ".o")
         _run([FC, "-O2", "-c", p, "-o", o, "-module-dir", objdir, "-I", objdir])
-        objs.append(o)
-    _run([FC, "-o", DRIVER] + objs + ["-L" + HERE, "-lhbi", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + HERE])
+        objs[os.path.basename(p)] = o
+    common = [objs["m_hbi_abi.f90"], objs["m_hacapk_gpu.f90"]]
+    link = ["-L" + HERE, "-lhbi", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + HERE]
+    _run([FC, "-o", DRIVER] + common + [objs["lhbiem_gpu.f90"]] + link)
+    _run([FC, "-o", DRIVER_FV] + common + [objs["lhbiem_fv_gpu.f90"]] + link)
     return DRIVER
 
 

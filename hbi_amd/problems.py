@@ -513,7 +513,9 @@ def fluid_bp6a(g: Geometry, lod):
     the centre for 0.27397 yr, constant permeability 1e-13, tau 29.2 MPa,
     sigma 50 MPa, V 1e-12 m/s; main_fv's defaults otherwise (injectionq,
     Neumann boundaries, eps_r 1e-4, kL 1e8, kT 1e12).  main_fv leaves vref
-    undeclared and backslip unset: vref = 1e-6 (main_LH's) and no back slip."""
+    undeclared and backslip unset: vref = 1e-6 (main_LH's) and no back slip;
+    its kT default is the single-precision literal 1e12."""
     return fluid_injection(g, lod, setting="injectionq", a=0.007, b=0.005, dc=0.004, f0=0.6, velinit=1e-12,
                            sigmainit=50.0, tauinit=29.2, muinit=0.0, phi0=0.1, beta=1e-8, eta=1e-3, kpmin=1e-13,
-                           kpmax=1e-13, q0=1.25e-6, tinj=0.27397, tmax=2.0, backslip=False, vpl=0.0, eps_r=1e-4)
+                           kpmax=1e-13, q0=1.25e-6, tinj=0.27397, tmax=2.0, backslip=False, vpl=0.0, eps_r=1e-4,
+                           kT=float(np.float32(1e12)))

@@ -161,3 +161,46 @@ def test_gpu_fluid_rejects_unsupported_grids():
     opts, arr, y, pf = P.fluid_injection(g, Hs.lod, setting="thrust")
     with pytest.raises(H.HbiError):
         H.FV(Hs, opts, arr)
+
+
+def test_gpu_fluid_driver_matches_the_library_run(tmp_path):
+    """lhbiem_fv_gpu (the main_fv host) on bp6a at 1000 elements: its monitor
+    rows (main_fv.f90:1307-1311) are the library run's history rows, the
+    streams hold the initial and every interval-th state, and the last pf
+    record is the library's final pf."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    n, nstep = 1000, 300
+    keys = {"problem": "2dp", "ncellg": n, "filenumber": 3, "interval": 100, "nstep": nstep, "tmax": 2.0,
+            "tinj": 0.27397, "ds": 0.01, "kpmax": 1e-13, "kpmin": 1e-13, "eta": 1e-3, "beta": 1e-8, "phi0": 0.1,
+            "q0": 1.25e-6, "a": 0.007, "b": 0.005, "dc": 0.004, "f0": 0.6, "tauinit": 29.2, "sigmainit": 50.0,
+            "velinit": "1d-12"}
+    P.write_in_file(str(tmp_path / "bp6.in"), keys)
+    r = subprocess.run([os.path.join(root, "hbi_amd", "lhbiem_fv_gpu"), "bp6.in"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    out = tmp_path / "output"
+    mon = np.loadtxt(out / "monitor3.dat")
+    assert mon.shape == (nstep + 1, 12) and np.array_equal(mon[:, 0], np.arange(nstep + 1))
+    g = P.mesh_2dp(n, 0.01)
+    Hs = H.HMatrix(g, leaf=32, eps=1e-4)
+    opts, arr, y, pf = P.fluid_bp6a(g, Hs.lod)
+    fv = H.FV(Hs, opts, arr)
+    fv.set_state(y, pf, t=0.0, dt=1.0)
+    info, hist = fv.step(nstep, history=True)
+    assert len(hist) == nstep
+    assert np.allclose(mon[1:, 1], hist[:, 0], rtol=1e-9, atol=6e-5)          # f19.4
+    for c_mon, c_hist in ((2, 1), (3, 2), (4, 3), (5, 4), (6, 5), (8, 7), (9, 8)):   # e16.5
+        assert np.allclose(mon[1:, c_mon], hist[:, c_hist], rtol=6e-5, atol=1e-300), c_mon
+    assert np.array_equal(mon[1:, 11], hist[:, 9])
+    tr = np.loadtxt(out / "time3.dat", ndmin=2)
+    assert list(tr[:, 0].astype(int)) == [0, 100, 200, 300]
+    for name in ("vel", "slip", "sigma", "tau", "ks", "pf", "qflow"):
+        d = np.fromfile(out / ("%s3.dat" % name), dtype=np.float64)
+        assert d.size == 4 * n, name
+    pfs = np.fromfile(out / "pf3.dat", dtype=np.float64).reshape(4, n)
+    s = fv.get_state()
+    assert np.abs(pfs[-1] - s["pf"]).max() <= 1e-9 * np.abs(s["pf"]).max()
+    assert pfs[-1].max() > 0 and np.array_equal(pfs[0], np.zeros(n))
