@@ -26,7 +26,11 @@
 namespace hbi {
 
 constexpr int kRR = 64;      // right-hand sides per product
-constexpr int kP1Rows = 64;  // a1^T rows per phase-1 item (one wave)
+constexpr int kP1Rows = 64;     // a1^T rows per phase-1 item on the FP64 cores (one wave)
+// the same on the FP32 cores (two waves of 32 rows share X; 128-row items
+// halve the X reads but leave too few items for 256 CUs: 0.085 vs 0.080 ms
+// for phase 1 of the 32768-element fault)
+constexpr int kP1Rows32 = 64;
 constexpr int kKS = 32;      // k-slice staged in LDS
 constexpr int kMrThreads = 256;
 constexpr int kKS32 = 32;         // columns per slice of the FP32-core phase-2 kernel
@@ -40,10 +44,10 @@ template <> __device__ __forceinline__ double widen<uint16_t>(uint16_t v) { retu
 // ------------------------------------------------------------- phase 1 --
 struct MItem1 {
     uint64_t off;      // byte offset of the first a1^T row
-    int ldt, kt;       // row stride (elements), rows (<= kP1Rows: whole blocks of one column cluster, stacked)
+    int ldt, kt;       // row stride (elements), rows (<= the plan's cap: whole blocks of one column cluster, stacked)
     int t0, tn;        // column chunk [t0, t0+tn)
     int xoff;          // first x row of the column cluster
-    int part;          // partial slot zpart[part][kP1Rows][RR], or -1: the item is the rows' whole
+    int part;          // partial slot zpart[part][rows cap][RR], or -1: the item is the rows' whole
                        // column range and writes Z rows zrow[zr .. zr + kt) directly
     int dtype, zr;
     int r0;            // first row of the item within its partial slot (set per wave by the FP32-core kernel)
@@ -196,6 +200,7 @@ __device__ __forceinline__ void mr_p1_wave32(const MItem1 &it, const uint8_t *__
                                              const float *__restrict__ X, double *__restrict__ zpart,
                                              const int *__restrict__ zrow, float *__restrict__ Z)
 {
+    constexpr int prow = kP1Rows32;
     const int lane = threadIdx.x % 64, li = lane % 16, lk = lane / 16;
     const T *a = (const T *)(buf + it.off);
     const int tend = it.t0 + it.tn;
@@ -278,7 +283,7 @@ __device__ __forceinline__ void mr_p1_wave32(const MItem1 &it, const uint8_t *__
                     *(fvec4 *)(Z + (size_t)zrow[it.zr + row] * kRR + 4 * li) =
                         fvec4{(float)sum[rb][0][q], (float)sum[rb][1][q], (float)sum[rb][2][q], (float)sum[rb][3][q]};
                 } else {
-                    dvec2 *dst = (dvec2 *)(zpart + ((size_t)it.part * kP1Rows + it.r0 + row) * kRR + 4 * li);
+                    dvec2 *dst = (dvec2 *)(zpart + ((size_t)it.part * prow + it.r0 + row) * kRR + 4 * li);
                     dst[0] = dvec2{sum[rb][0][q], sum[rb][1][q]};
                     dst[1] = dvec2{sum[rb][2][q], sum[rb][3][q]};
                 }
@@ -286,7 +291,7 @@ __device__ __forceinline__ void mr_p1_wave32(const MItem1 &it, const uint8_t *__
         }
 }
 
-__global__ __launch_bounds__(128) void mr_lrvt32_kernel(const MItem1 *__restrict__ items,
+__global__ __launch_bounds__(2 * kP1Rows32) void mr_lrvt32_kernel(const MItem1 *__restrict__ items,
                                                         const uint8_t *__restrict__ buf, const float *__restrict__ X,
                                                         double *__restrict__ zpart, const int *__restrict__ zrow,
                                                         float *__restrict__ Z, const int *__restrict__ gate)
@@ -342,17 +347,18 @@ __global__ void mr_narrow_kernel(size_t n4, const double *__restrict__ X, float 
 }
 
 template <typename OT>
-__global__ void mr_zreduce_kernel(const MBlk1 *__restrict__ blks, int nblk, const double *__restrict__ zpart,
+__global__ void mr_zreduce_kernel(const MBlk1 *__restrict__ blks, int nblk, int prow, const double *__restrict__ zpart,
                                   OT *__restrict__ Z, const int *__restrict__ gate)
 {
-    // workgroup (b, k): row k of block b, one right-hand side per thread (64 threads)
+    // workgroup (b, k): row k of block b, one right-hand side per thread (64
+    // threads); prow: rows per partial slot
     if (gate && *gate) return;
-    const int b = blockIdx.x / kP1Rows, k = blockIdx.x % kP1Rows, rr = threadIdx.x;
+    const int b = blockIdx.x / prow, k = blockIdx.x % prow, rr = threadIdx.x;
     if (b >= nblk) return;
     const MBlk1 B = blks[b];
     if (k >= B.kt) return;
     double s = 0.0;
-    for (int p = B.p0; p < B.p1; p++) s += zpart[((size_t)p * kP1Rows + B.roff + k) * kRR + rr];
+    for (int p = B.p0; p < B.p1; p++) s += zpart[((size_t)p * prow + B.roff + k) * kRR + rr];
     Z[(size_t)(B.zoff + k) * kRR + rr] = (OT)s;
 }
 
@@ -624,6 +630,7 @@ struct MrPlan {
     bool built = false;
     int kc1 = 512, cols2 = 256;    // phase-1 column chunk, phase-2 columns per item (autotuned)
     bool f32 = false;              // FP32 / BF16 factors only: the FP32-core kernels, X32 and Z in FP32
+    int p1rows = kP1Rows;          // a1^T rows per phase-1 item (kP1Rows32 on the FP32 cores)
     int r2[3][2] = {};             // phase-2 item range per dtype
     std::vector<MItem1> it1;
     std::vector<MBlk1> blk1;
@@ -656,10 +663,11 @@ namespace hbi {
 
 // Phase-1 items.  Runs of blocks with the same column cluster and dtype are
 // adjacent in the packed layout (same ldt): whole blocks are packed into row
-// groups of at most kP1Rows rows, each read against its x rows once per column
-// chunk of kc1.  Ranks above kP1Rows are split into pieces of their own.
+// groups of at most M.p1rows rows, each read against its x rows once per
+// column chunk of kc1.  Ranks above it are split into pieces of their own.
 static void mr_items1(const Plan &P, MrPlan &M, int kc1)
 {
+    const int cap = M.p1rows;
     M.it1.clear();
     M.blk1.clear();
     M.nzpart = 0;
@@ -673,15 +681,15 @@ static void mr_items1(const Plan &P, MrPlan &M, int kc1)
         std::vector<Piece> grp;
         int rows = 0;
         const Row1 &f = R[i];
-        if (f.kt > kP1Rows) {
-            for (int k0 = 0; k0 < f.kt; k0 += kP1Rows)
-                grp.push_back({f.off + (uint64_t)k0 * f.ldt * dtype_size(f.dtype), std::min(kP1Rows, f.kt - k0),
+        if (f.kt > cap) {
+            for (int k0 = 0; k0 < f.kt; k0 += cap)
+                grp.push_back({f.off + (uint64_t)k0 * f.ldt * dtype_size(f.dtype), std::min(cap, f.kt - k0),
                                f.zoff + k0, srow[i] + k0});
             i++;
         } else {
             size_t j = i;
             while (j < R.size() && R[j].xoff == f.xoff && R[j].nc == f.nc && R[j].dtype == f.dtype &&
-                   R[j].kt <= kP1Rows && rows + R[j].kt <= kP1Rows &&
+                   R[j].kt <= cap && rows + R[j].kt <= cap &&
                    (j == i || R[j].off == R[j - 1].off + (uint64_t)R[j - 1].kt * R[j - 1].ldt * dtype_size(f.dtype))) {
                 grp.push_back({R[j].off, R[j].kt, R[j].zoff, srow[j]});
                 rows += R[j].kt;
@@ -690,7 +698,7 @@ static void mr_items1(const Plan &P, MrPlan &M, int kc1)
             i = j;
         }
         // one packed run per group, except split pieces: each piece is its own run
-        const bool split = f.kt > kP1Rows;
+        const bool split = f.kt > cap;
         for (size_t g0 = 0; g0 < grp.size();) {
             const size_t g1 = split ? g0 + 1 : grp.size();
             int kt = 0;
@@ -787,6 +795,7 @@ static void mr_mode(const Plan &P, MrPlan &M)
     for (const Seg &g : P.segs)
         HBI_REQUIRE(g.off % 16 == 0, "64-column product: segment not 16-byte aligned");
     M.f32 = has32;
+    M.p1rows = has32 ? kP1Rows32 : kP1Rows;
 }
 
 static void mr_upload(hbi_hmat *h, MrPlan &M)
@@ -836,7 +845,7 @@ void mr_forget(const hbi_hmat *h)
 }
 
 // buffer sizes cover every candidate of the tuner
-static size_t zpart_cap(const MrPlan &M) { return (size_t)std::max(M.nzpart, 1) * kP1Rows * kRR; }
+static size_t zpart_cap(const MrPlan &M) { return (size_t)std::max(M.nzpart, 1) * M.p1rows * kRR; }
 static size_t ypart_cap(const hbi_hmat *h, const MrPlan &M) { return (size_t)std::max(M.nparts, 1) * h->tr * kRR; }
 size_t mr_zpart_size(hbi_hmat *h, int prec) { return zpart_cap(mr_plan(h, prec)); }
 size_t mr_x32_size(hbi_hmat *h, int prec) { return mr_plan(h, prec).f32 ? (size_t)h->n * kRR : 1; }
@@ -857,14 +866,15 @@ static void phase1(hbi_hmat *h, const Plan &P, const MrPlan &M, const double *X,
     if (!M.f32) {
         mr_lrvt_kernel<<<(int)M.it1.size(), 64, 0, s>>>(M.d_it1.p, P.buf.p, X, zpart, P.d_zrow.p, Z, gate);
         if (!M.blk1.empty())
-            mr_zreduce_kernel<double><<<(int)M.blk1.size() * kP1Rows, kRR, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(), zpart,
-                                                                                  Z, gate);
+            mr_zreduce_kernel<double><<<(int)M.blk1.size() * M.p1rows, kRR, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(),
+                                                                                   M.p1rows, zpart, Z, gate);
     } else {
         float *Z32 = (float *)Z;
-        mr_lrvt32_kernel<<<(int)M.it1.size(), 128, 0, s>>>(M.d_it1.p, P.buf.p, X32, zpart, P.d_zrow.p, Z32, gate);
+        mr_lrvt32_kernel<<<(int)M.it1.size(), 2 * kP1Rows32, 0, s>>>(M.d_it1.p, P.buf.p, X32, zpart, P.d_zrow.p, Z32,
+                                                                     gate);
         if (!M.blk1.empty())
-            mr_zreduce_kernel<float><<<(int)M.blk1.size() * kP1Rows, kRR, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(), zpart,
-                                                                                 Z32, gate);
+            mr_zreduce_kernel<float><<<(int)M.blk1.size() * M.p1rows, kRR, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(),
+                                                                                  M.p1rows, zpart, Z32, gate);
     }
 }
 
