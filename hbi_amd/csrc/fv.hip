@@ -307,11 +307,12 @@ __device__ __forceinline__ void fv_dxx_row(const FvPar &P, const double *cd, int
 // threads.  The set-up pass writes each element's matrix row (sub, diagonal,
 // super) to rows[3e..3e+2], the right-hand side to w[e] and the start x =
 // pf - pfhyd to w[n + e]; in the conjugate-gradient loop element e = j * NT +
-// threadIdx.x (j < NPT) of x and r lives in registers, its row is read from
-// global memory by its own thread only, p (read at neighbours) sits in ps
-// and A p in cd (the diffusivities on entry).  pfnew goes to w[n + e];
-// returns the iteration count.
-template <int NT, int NPT>
+// threadIdx.x (j < NPT) of x, r and A p lives in registers, p (read at
+// neighbours) in ps.  With RL (N <= 4096) the rows are then copied into LDS
+// over cd (the diffusivities, no longer needed: 3N doubles from cd on),
+// else each thread reads its rows from global memory.  pfnew goes to
+// w[n + e]; returns the iteration count.
+template <int NT, int NPT, bool RL>
 __device__ __forceinline__ int fv_beuler(const FvPar &P, double *cd, double *ps, double (*red)[kFvRed], int &slot,
                                          const double *pf, const double *pfhyd, const double *qin, const double *phi,
                                          double h, double t, double *__restrict__ rows, double *__restrict__ w)
@@ -380,14 +381,19 @@ __device__ __forceinline__ int fv_beuler(const FvPar &P, double *cd, double *ps,
         w[n + e] = x0;
         ps[e] = x0;
     }
+    __syncthreads();   // ps holds x; every row has read cd; the rows are written
+    if constexpr (RL) {
+        for (int q = tid; q < 3 * n; q += NT) cd[q] = rows[q];
+        __syncthreads();
+    }
+    const double *R = RL ? cd : rows;
     // the matrix product in the reference's order of terms (1895-1903)
     auto apply = [&](const double *v, int e) -> double {
-        const double *rw = rows + 3 * (size_t)e;
+        const double *rw = R + 3 * (size_t)e;
         if (e == 0) return v[0] * rw[1] + v[1] * rw[2];
         if (e == n - 1) return v[n - 1] * rw[1] + v[n - 2] * rw[0];
         return v[e - 1] * rw[0] + v[e] * rw[1] + v[e + 1] * rw[2];
     };
-    __syncthreads();   // ps holds x; every row has read cd
     double x[NPT], r[NPT], rr = 0.0;
 #pragma unroll
     for (int j = 0; j < NPT; j++) {
@@ -414,14 +420,14 @@ __device__ __forceinline__ int fv_beuler(const FvPar &P, double *cd, double *ps,
         for (int it = 1; it <= itermax; it++) {
             const double tmp1 = rsold;   // = sum(r * r) of the current r
             __syncthreads();             // ps holds p
-            double mp = 0.0;
+            double mp = 0.0, m[NPT];
 #pragma unroll
             for (int j = 0; j < NPT; j++) {
                 const int e = j * NT + tid;
+                m[j] = 0.0;
                 if (e < n) {
-                    const double m = apply(ps, e);
-                    cd[e] = m;
-                    mp += m * ps[e];
+                    m[j] = apply(ps, e);
+                    mp += m[j] * ps[e];
                 }
             }
             const double tmp2 = fv_bsum<NT>(mp, red, slot);
@@ -432,7 +438,7 @@ __device__ __forceinline__ int fv_beuler(const FvPar &P, double *cd, double *ps,
                 const int e = j * NT + tid;
                 if (e < n) {
                     x[j] = x[j] + alpha * ps[e];
-                    r[j] = r[j] - alpha * cd[e];
+                    r[j] = r[j] - alpha * m[j];
                     rs += r[j] * r[j];
                 }
             }
@@ -460,13 +466,15 @@ __device__ __forceinline__ int fv_beuler(const FvPar &P, double *cd, double *ps,
 // implicitsolver2 (main_fv.f90:1727-1774) or, pressure dependent,
 // implicitsolver (1678-1724): pfnew, CG iterations and the dt bound.  The
 // work vector w (3N) holds the right-hand side, pfnew and the Picard trial.
-template <int NT, int NPT>
+template <int NT, int NPT, bool RL>
 __global__ __launch_bounds__(NT) void fv_pressure_kernel(FvPar P, FvCtl *c, FvArrays A, FvSolveIO io)
 {
 #pragma clang fp contract(off)
-    __shared__ double cd[kFvMaxCells];
-    __shared__ double ps[kFvMaxCells];
+    // RL: ps = [0, 4096), cd and then the rows from 4096 on; else cd, ps halves
+    __shared__ double lds[2 * kFvMaxCells];
     __shared__ double red[2][kFvRed];
+    double *cd = RL ? lds + kFvMaxCells / 2 : lds;
+    double *ps = RL ? lds : lds + kFvMaxCells;
     if (io.gated && (c->done || !c->acc)) return;
     const int n = P.n;
     const int tid = threadIdx.x;
@@ -486,7 +494,7 @@ __global__ __launch_bounds__(NT) void fv_pressure_kernel(FvPar P, FvCtl *c, FvAr
             cd[e] = P.pdep ? ks * exp(-(io.sigma[e] - pft[e]) / P.s0) / cc * 1e-6 : ks / cc * 1e-6;
         }
         __syncthreads();
-        niter = fv_beuler<NT, NPT>(P, cd, ps, red, slot, io.pf, A.pfhyd, A.qin, A.phi, h, t, io.rows, w);
+        niter = fv_beuler<NT, NPT, RL>(P, cd, ps, red, slot, io.pf, A.pfhyd, A.qin, A.phi, h, t, io.rows, w);
         if (!P.pdep) break;
         double err = 0.0;
         for (int e = tid; e < n; e += NT) {
@@ -653,12 +661,13 @@ static FvArrays fv_arrays(hbi_fv *f)
 static void fv_launch_pressure(hbi_fv *f, FvArrays A, const FvSolveIO &io)
 {
     FvCtl *c = f->ctl.p;
-    // elements per thread: registers hold 6 x NPT doubles, so the larger grids
-    // take 512-thread workgroups (256 VGPRs per lane)
-    if (f->n <= 1024) fv_pressure_kernel<1024, 1><<<1, 1024, 0, f->stream>>>(f->P, c, A, io);
-    else if (f->n <= 2048) fv_pressure_kernel<1024, 2><<<1, 1024, 0, f->stream>>>(f->P, c, A, io);
-    else if (f->n <= 4096) fv_pressure_kernel<512, 8><<<1, 512, 0, f->stream>>>(f->P, c, A, io);
-    else fv_pressure_kernel<1024, 8><<<1, 1024, 0, f->stream>>>(f->P, c, A, io);
+    // elements per thread: registers hold x, r and A p of NPT elements, so
+    // 2049-4096 elements take 512-thread workgroups (256 VGPRs per lane); up
+    // to 4096 elements the matrix rows sit in LDS
+    if (f->n <= 1024) fv_pressure_kernel<1024, 1, true><<<1, 1024, 0, f->stream>>>(f->P, c, A, io);
+    else if (f->n <= 2048) fv_pressure_kernel<1024, 2, true><<<1, 1024, 0, f->stream>>>(f->P, c, A, io);
+    else if (f->n <= 4096) fv_pressure_kernel<512, 8, true><<<1, 512, 0, f->stream>>>(f->P, c, A, io);
+    else fv_pressure_kernel<1024, 8, false><<<1, 1024, 0, f->stream>>>(f->P, c, A, io);
 }
 
 // nstages derivs evaluations: mode 0 a whole attempt from y, mode 1 k_0 of ytmp
