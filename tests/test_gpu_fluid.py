@@ -234,3 +234,27 @@ def test_gpu_fluid_constant_pressure_and_output_step_bound():
     for c in range(4):
         ref = st["y"][c::4]
         assert np.abs(s["y"][c::4] - ref).max() <= 1e-9 * np.abs(ref).max(), c
+
+
+@pytest.mark.parametrize("kw,status", [(dict(velmax=1e-9), 1), (dict(velmin=1e-8), 1), (dict(tmax=1e-7), 3)])
+def test_gpu_fluid_stop_controls(kw, status):
+    """The stop controls of main_fv.f90:1136-1152 (V outside [velmin,
+    velmax], t > tmax) stop the device run at the same step as the
+    oracle's, with the step counted."""
+    n = 128
+    g = P.mesh_2dp(n, 0.01)
+    base = dict(setting="injectionq", velinit=1e-9)
+    base.update(kw)
+    dev, ora, y, pf, _ = _pair(g, base)
+    dev.set_state(y, pf, t=0.0, dt=1.0)
+    st = dict(y=y.copy(), pf=pf.copy(), disp=np.zeros(n), t=0.0, h=1.0)
+    k_ora = None
+    for k in range(200):
+        r = ora.step(st)
+        if r["status"]:
+            k_ora = k + 1
+            assert r["status"] == status
+            break
+    assert k_ora is not None
+    info, hist = dev.step(200, history=True)
+    assert info["status"] == status and info["accepted"] == k_ora and len(hist) == k_ora
