@@ -42,12 +42,19 @@ struct FvArrays {
     const double *a, *b, *dc, *f0, *vpl, *kpmax, *kT, *phi, *pfhyd, *qin, *area;
     double *y, *ytmp, *yout, *k, *vel, *x, *hx, *hx_n;
     double *pf, *sigma, *disp, *velpre, *qflow, *errpart, *monpart, *hist;
+    double *probe;          // per accepted step, kFvProbe values for each probed element
+    const int *probe_idx;
+    int nprobe;
 };
 
 constexpr int kFvThreads = 256;
 constexpr int kFvRed = 16;   // reduction slots: waves of the largest solve workgroup
 constexpr int kFvMon = 6;    // max |V|, sum disp area, sum mu area, max sigma_e, min sigma_e, first argmax |V|
 constexpr int kFvHist = 10;
+// probe row (the local output of main_fv.f90:1312-1316): V, disp, sigma_e,
+// tau, pf, sigma, mu, psi, kp
+constexpr int kFvProbe = 9;
+constexpr int kFvMaxProbes = 16;
 constexpr int kFvMaxCells = HBI_FV_MAX_CELLS;
 
 // main_fv.f90:2039-2146 for one element: o = (dpsi, dtau, dsigma_e, dks)/dt
@@ -615,6 +622,15 @@ __global__ void fv_monitor_kernel(FvPar P, FvCtl *c, FvArrays A, int nblk)
         hr[0] = c->t; hr[1] = log10(vm); hr[2] = b[1] / P.sg; hr[3] = b[2] / P.sg; hr[4] = b[3]; hr[5] = b[4];
         hr[6] = b[5] + 1.0; hr[7] = c->errmax; hr[8] = c->hdid; hr[9] = (double)c->niter;
     }
+    if (A.probe && row >= 0 && row < c->hist_cap)
+        for (int q = 0; q < A.nprobe; q++) {
+            const size_t i = A.probe_idx[q];
+            const double *yi = A.y + 4 * i;
+            const double sige = yi[2], tau = yi[1], ks = yi[3];
+            double *pr = A.probe + ((size_t)row * A.nprobe + q) * kFvProbe;
+            pr[0] = A.vel[i]; pr[1] = A.disp[i]; pr[2] = sige; pr[3] = tau; pr[4] = A.pf[i]; pr[5] = A.sigma[i];
+            pr[6] = tau / sige; pr[7] = yi[0]; pr[8] = P.pdep ? ks * exp(-sige / P.s0) : ks;
+        }
     // stop controls, main_fv.f90:1136-1152
     if (vm > P.velmax || vm < P.velmin) c->status = 1;
     else if (c->t > P.tmax) c->status = 3;
@@ -636,9 +652,13 @@ struct hbi_fv {
     hbi::DevBuf<double> pf, sigma, disp, velpre, qflow, errpart, monpart, hist, rows, work;
     hbi::DevBuf<double> io_pf, io_y, io_sig, io_out, io_dt;
     hbi::DevBuf<int> io_niter, zero;
+    hbi::DevBuf<double> probe;                 // probe rows of the last step call (hbi_fv_set_probes)
+    hbi::DevBuf<int> probe_idx;
+    long long probe_rows = 0;
     hbi::DevBuf<hbi::FvCtl> ctl;
     hipGraphExec_t graph = nullptr;
-    double *graph_hist = nullptr;
+    double *graph_hist = nullptr, *graph_probe = nullptr;
+    bool dirty = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0.0;
 };
@@ -655,6 +675,9 @@ static FvArrays fv_arrays(hbi_fv *f)
     A.pf = f->pf.p; A.sigma = f->sigma.p; A.disp = f->disp.p; A.velpre = f->velpre.p; A.qflow = f->qflow.p;
     A.errpart = f->errpart.p; A.monpart = f->monpart.p;
     A.hist = f->hist.n ? f->hist.p : nullptr;
+    A.probe = f->probe.n ? f->probe.p : nullptr;
+    A.probe_idx = f->probe_idx.p;
+    A.nprobe = (int)f->probe_idx.n;
     return A;
 }
 
@@ -720,6 +743,7 @@ static void fv_capture(hbi_fv *f)
     HBI_HIP(hipGraphInstantiate(&f->graph, gr, nullptr, nullptr, 0));
     (void)hipGraphDestroy(gr);
     f->graph_hist = f->hist.p;
+    f->graph_probe = f->probe.p;
 }
 
 }  // namespace hbi
@@ -909,6 +933,14 @@ int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info)
             if (hneed) f->hist.alloc(hneed); else f->hist.release();
             recapture = true;
         }
+        const size_t pneed = (size_t)nsteps * f->probe_idx.n * hbi::kFvProbe;
+        if (pneed > f->probe.n || (pneed == 0 && f->probe.n)) {
+            if (pneed) f->probe.alloc(pneed); else f->probe.release();
+            recapture = true;
+        }
+        f->probe_rows = 0;
+        if (f->dirty) recapture = true;
+        f->dirty = false;
         if (recapture) hbi::fv_capture(f);
         hbi::fv_arm_kernel<<<1, 1, 0, f->stream>>>(f->ctl.p, nsteps);
         HBI_HIP(hipEventRecord(f->e0, f->stream));
@@ -930,6 +962,7 @@ int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info)
         HBI_HIP(hipEventElapsedTime(&ms, f->e0, f->e1));
         f->last_ms = ms;
         const long long done = c.accepted - c.hist_base;
+        f->probe_rows = std::max<long long>(0, std::min<long long>(done, nsteps));
         if (hist && done > 0) {
             f->hist.download(hist, hbi::kFvHist * (size_t)std::min<long long>(done, nsteps), f->stream);
             HBI_HIP(hipStreamSynchronize(f->stream));
@@ -940,6 +973,35 @@ int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info)
             info->vmax = c.vmax; info->niter = c.niter; info->status = c.status;
         }
         if (c.status == HBI_E_STEP) throw hbi::Error{HBI_E_STEP, "Runge-Kutta step size underflow"};
+    });
+}
+
+int hbi_fv_set_probes(hbi_fv *f, int nprobe, const int *idx)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(f && nprobe >= 0 && nprobe <= hbi::kFvMaxProbes && (nprobe == 0 || idx), "bad arguments");
+        for (int q = 0; q < nprobe; q++) HBI_REQUIRE(idx[q] >= 0 && idx[q] < f->n, "probe index out of range");
+        if (nprobe) {
+            f->probe_idx.alloc(nprobe);
+            f->probe_idx.upload(idx, nprobe, f->stream);
+        } else {
+            f->probe_idx.release();
+        }
+        f->probe.release();
+        f->probe_rows = 0;
+        f->dirty = true;
+    });
+}
+
+int hbi_fv_get_probes(hbi_fv *f, long long *nrows, double *rows)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(f && nrows, "NULL argument");
+        *nrows = f->probe_rows;
+        if (rows && f->probe_rows > 0) {
+            f->probe.download(rows, (size_t)f->probe_rows * f->probe_idx.n * hbi::kFvProbe, f->stream);
+            HBI_HIP(hipStreamSynchronize(f->stream));
+        }
     });
 }
 

@@ -10,8 +10,8 @@
 ! device) and writes the reference's files: monitor<n>.dat (output_monitor,
 ! 1307-1311), time<n>.dat, event<n>.dat (1197-1239, decided from the
 ! per-step monitor values), the vel / slip / sigma / tau / ks / pf / qflow<n>.dat
-! FP64 streams of every output step (1155-1198) and xyz / ind<n>.dat
-! (811-824).  Output steps: every interval-th step exactly (a batch ends
+! FP64 streams of every output step (1155-1198), xyz / ind<n>.dat
+! (811-824) and the five local<n>-<id>.dat files (862-868, 1312-1316).  Output steps: every interval-th step exactly (a batch ends
 ! there); t > tout is checked at the end of each batch of `outbatch` steps.
 ! Keys beyond the reference's: vref (main_fv leaves it undeclared; 1e-6),
 ! leaf, precision, outbatch.
@@ -42,6 +42,11 @@ program lhbiem_fv_gpu
   type(hbi_fv_info) :: info
   type(c_ptr) :: fv
   integer(8) :: c0, c1, crate
+  integer, parameter :: nl = 5
+  integer :: locid(nl), nprobe, punit(nl)
+  integer(c_int) :: probe_idx(nl)
+  integer(c_long_long) :: nprow
+  real(8), allocatable :: prow(:)
 
   call get_command_argument(1, input_file)
   open(33, file=input_file, status='old', iostat=ios)
@@ -254,6 +259,19 @@ program lhbiem_fv_gpu
     sum(y(2::4) / y(3::4) * ds0) / (ds0 * ncellg), maxval(y(3::4)), minval(y(3::4)), &
     xcol(maxloc(abs(vel_of_state()), 1)), 0d0, 0d0, 0d0, 0
   write(50, '(i7,f19.4)') 0, 0d0
+  ! local output points, main_fv.f90:862-868: cluster-order indices
+  nprobe = 0
+  do i = 1, nl
+    locid(i) = int(dble(i - 0.5) / nl * ncellg)
+    if (locid(i) < 1 .or. locid(i) > ncellg) cycle
+    nprobe = nprobe + 1
+    probe_idx(nprobe) = locid(i) - 1
+    punit(nprobe) = 52 + i
+    write(fname, '("output/local",i0,"-",i0,".dat")') number, locid(i)
+    open(52 + i, file=fname, status='replace')
+  end do
+  rc = hbi_fv_set_probes(fv, int(nprobe, c_int), probe_idx)
+  allocate(prow(9 * max(nprobe, 1) * outbatch))
 
   allocate(hist(10 * outbatch))
   tout = dtout * 365 * 24 * 60 * 60
@@ -288,6 +306,7 @@ program lhbiem_fv_gpu
           (log10(moment * rigid * ds0 * ncellg) + 5.9) / 1.5
       end if
     end do
+    if (nprobe > 0) call write_local()
     kdone = int(info%accepted)
     outfield = interval > 0 .and. mod(kdone, max(interval, 1)) == 0 .and. nrow > 0
     if (info%t > tout) then
@@ -312,11 +331,28 @@ program lhbiem_fv_gpu
     close(i)
   end do
   close(50); close(51); close(52)
+  do i = 1, nprobe
+    close(punit(i))
+  end do
   rc = hbi_fv_destroy(fv)
   lrtrn = HACApK_free_leafmtxp(st_leafmtxp_s)
   lrtrn = HACApK_finalize(st_ctl)
 
 contains
+
+  ! output_local rows (main_fv.f90:1312-1316) of the last batch: k, t,
+  ! log10 V, disp, sigma_e, tau, pf, sigma, mu, psi, kp
+  subroutine write_local()
+    integer :: j, rr, o
+    rc = hbi_fv_get_probes(fv, nprow, prow)
+    do rr = 1, int(nprow)
+      do j = 1, nprobe
+        o = 9 * (nprobe * (rr - 1) + j - 1)
+        write(punit(j), '(i7,f19.4,9e16.6)') kdone + rr, hist(10 * rr - 9), log10(prow(o + 1)), prow(o + 2), &
+          prow(o + 3), prow(o + 4), prow(o + 5), prow(o + 6), prow(o + 7), prow(o + 8), prow(o + 9)
+      end do
+    end do
+  end subroutine write_local
 
   ! a mesh-order array in cluster order
   function perm(v) result(w)

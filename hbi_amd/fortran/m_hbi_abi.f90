@@ -269,6 +269,18 @@ module m_hbi_abi
       real(c_double), intent(out) :: y(*), pf(*), disp(*), vel(*), qflow(*)
       type(hbi_fv_info), intent(out) :: info
     end function
+    integer(c_int) function hbi_fv_set_probes(f, nprobe, idx) bind(C, name="hbi_fv_set_probes")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: f
+      integer(c_int), value :: nprobe
+      integer(c_int), intent(in) :: idx(*)
+    end function
+    integer(c_int) function hbi_fv_get_probes(f, nrows, rows) bind(C, name="hbi_fv_get_probes")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: f
+      integer(c_long_long), intent(out) :: nrows
+      real(c_double), intent(out) :: rows(*)
+    end function
     integer(c_int) function hbi_fv_step(f, nsteps, hist, info) bind(C, name="hbi_fv_step")
       import :: c_int, c_ptr, c_double, hbi_fv_info
       type(c_ptr), value :: f

@@ -120,7 +120,7 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_rk_step_many", "hbi_rk_set_output", "hbi_rk_get_output",
            "hbi_rk_set_opening", "hbi_rk_get_opening", "hbi_rk_set_probes", "hbi_rk_get_probes",
            "hbi_fv_create", "hbi_fv_destroy", "hbi_fv_set_state", "hbi_fv_get_state", "hbi_fv_derivs",
-           "hbi_fv_diffuse", "hbi_fv_step", "hbi_fv_last_ms")
+           "hbi_fv_diffuse", "hbi_fv_step", "hbi_fv_last_ms", "hbi_fv_set_probes", "hbi_fv_get_probes")
 
 
 def lib():
@@ -187,6 +187,8 @@ def lib():
         L.hbi_fv_diffuse.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _ip, _dp]
         L.hbi_fv_step.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(FvInfo)]
         L.hbi_fv_last_ms.argtypes = [C.c_void_p, _dp]
+        L.hbi_fv_set_probes.argtypes = [C.c_void_p, C.c_int, _ip]
+        L.hbi_fv_get_probes.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), _dp]
         L.hbi_rk_get_output.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), _dp, _dp,
                                         C.POINTER(C.c_int), C.POINTER(RkEvent), _dp]
         L.hbi_ens_get_trace.argtypes = [C.c_void_p, _dp, _dp, _ip]
@@ -717,3 +719,19 @@ class FV:
         ms = C.c_double(0)
         _check(lib().hbi_fv_last_ms(self.f, C.byref(ms)))
         return ms.value
+
+    def set_probes(self, idx):
+        """Local output (main_fv.f90:1312-1316) at up to 16 cluster-order indices."""
+        a = np.ascontiguousarray(idx, dtype=np.int32)
+        self._nprobe = int(a.size)
+        _check(lib().hbi_fv_set_probes(self.f, int(a.size), a.ctypes.data_as(_ip) if a.size else None))
+
+    def get_probes(self) -> np.ndarray:
+        """Rows of the last step call: (steps, probes, 9): V, disp, sigma_e,
+        tau, pf, sigma, mu, psi, kp."""
+        nr = C.c_longlong(0)
+        _check(lib().hbi_fv_get_probes(self.f, C.byref(nr), None))
+        out = np.empty((nr.value, getattr(self, "_nprobe", 0), 9))
+        if out.size:
+            _check(lib().hbi_fv_get_probes(self.f, C.byref(nr), _p(out)))
+        return out

@@ -440,6 +440,13 @@ int hbi_fv_diffuse(hbi_fv *f, const double *pf, const double *y, const double *s
    output_monitor columns of main_fv.f90:1307-1311). */
 int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info);
 int hbi_fv_last_ms(const hbi_fv *f, double *ms);
+/* Local output (main_fv.f90:862-868, output_local at 1312-1316): up to 16
+   cluster-order elements; after every accepted step of the following step
+   calls the device records for each 9 values: V, disp, sigma_e, tau, pf,
+   sigma, mu, psi, kp.  hbi_fv_get_probes gives the last hbi_fv_step call's
+   rows (*nrows x nprobe x 9, step-major; rows NULL: count only). */
+int hbi_fv_set_probes(hbi_fv *f, int nprobe, const int *idx);
+int hbi_fv_get_probes(hbi_fv *f, long long *nrows, double *rows);
 
 #ifdef __cplusplus
 }

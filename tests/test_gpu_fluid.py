@@ -99,6 +99,8 @@ def test_gpu_fluid_run_against_oracle(setting, pdep):
     n = 256
     g = P.mesh_2dp(n, 0.01)
     dev, ora, y, pf, _ = _pair(g, dict(setting=setting, pressure_dependent=pdep))
+    pidx = [0, 100, n // 2 - 1, n - 1]
+    dev.set_probes(pidx)
     dev.set_state(y, pf, t=0.0, dt=1.0)
     st = dict(y=y.copy(), pf=pf.copy(), disp=np.zeros(n), t=0.0, h=1.0)
     rows = [ora.step(st) for _ in range(25)]
@@ -124,6 +126,15 @@ def test_gpu_fluid_run_against_oracle(setting, pdep):
     assert np.abs(s["vel"] - st["vel"]).max() <= 1e-8 * np.abs(st["vel"]).max()
     assert np.abs(s["qflow"] - st["qflow"]).max() <= 1e-6 * np.abs(st["qflow"]).max() + 1e-300
     assert s["info"]["niter"] == int(hist[-1, 9])
+    # local output rows (main_fv.f90:1312-1316) of the last step = the state
+    pr = dev.get_probes()
+    assert pr.shape == (25, 4, 9)
+    yl = s["y"]
+    last = pr[-1]
+    assert np.array_equal(last[:, 0], s["vel"][pidx]) and np.array_equal(last[:, 1], s["disp"][pidx])
+    assert np.array_equal(last[:, 2], yl[2::4][pidx]) and np.array_equal(last[:, 3], yl[1::4][pidx])
+    assert np.array_equal(last[:, 4], s["pf"][pidx]) and np.array_equal(last[:, 7], yl[0::4][pidx])
+    assert np.array_equal(last[:, 6], yl[1::4][pidx] / yl[2::4][pidx])
 
 
 def test_gpu_fluid_full_grid_holds_the_injected_volume():
@@ -201,6 +212,22 @@ def test_gpu_fluid_driver_matches_the_library_run(tmp_path):
         assert np.allclose(mon[1:k + 1, c_mon], hist[:k, c_hist], rtol=6e-5, atol=1e-300), c_mon
     assert np.array_equal(mon[1:k + 1, 11], hist[:k, 9])
     assert abs(mon[-1, 1] - hist[-1, 0]) <= 1e-3 * hist[-1, 0]
+    # local files (main_fv.f90:862-868, 1312-1316): five points, a row per
+    # step; at the output steps their values are the field streams' (e16.6)
+    vel = np.fromfile(out / "vel3.dat").reshape(4, n)
+    slp = np.fromfile(out / "slip3.dat").reshape(4, n)
+    sgm = np.fromfile(out / "sigma3.dat").reshape(4, n)
+    tau = np.fromfile(out / "tau3.dat").reshape(4, n)
+    pff = np.fromfile(out / "pf3.dat").reshape(4, n)
+    for i in range(1, 6):
+        lid = int((i - 0.5) / 5 * n)
+        loc = np.loadtxt(out / ("local3-%d.dat" % lid))
+        assert loc.shape == (nstep, 11) and np.array_equal(loc[:, 0], np.arange(1, nstep + 1))
+        for rec, k in ((1, 100), (2, 200), (3, 300)):
+            r = loc[k - 1]
+            for c, ref in ((2, np.log10(vel[rec, lid - 1])), (3, slp[rec, lid - 1]), (4, sgm[rec, lid - 1]),
+                           (5, tau[rec, lid - 1]), (6, pff[rec, lid - 1])):
+                assert abs(r[c] - ref) <= 6e-6 * abs(ref) + 1e-300, (lid, k, c)
     tr = np.loadtxt(out / "time3.dat", ndmin=2)
     assert list(tr[:, 0].astype(int)) == [0, 100, 200, 300]
     for name in ("vel", "slip", "sigma", "tau", "ks", "pf", "qflow"):
