@@ -4,8 +4,8 @@
 ! Reads main_fv's `key value` input (main_fv.f90:166-316, defaults 130-162),
 ! meshes a 2dp fault (coordinate2dp, 1327-1344), builds the lattice H-matrix
 ! through the HACApK call sites (m_HACApK_gpu), sets the initial condition
-! of the setting (injectionq 1596-1643, injectionp 1548-1594, swarm
-! 1645-1675), advances with hbi_fv_step (rkqs with the 4-component derivs,
+! of the setting (injectionq 1596-1643, injectionp 1548-1594, steadystate
+! 1452-1498, swarm 1645-1675), advances with hbi_fv_step (rkqs with the 4-component derivs,
 ! the implicit pore-pressure step, field update and stop controls on the
 ! device) and writes the reference's files: monitor<n>.dat (output_monitor,
 ! 1307-1311), time<n>.dat, event<n>.dat (1197-1239, decided from the
@@ -25,7 +25,7 @@ program lhbiem_fv_gpu
   integer :: ios, number, ncellg, nstep, interval, leaf, outbatch, i, k, r, nb, nrow, kdone, u
   real(8) :: ds0, tmax, tinj, kpmax, kpmin, eta, beta, phi0, q0, a0, b0, dc0, f00, tauinit, sigmainit
   real(8) :: velinit, pfinit, muinit, pbcl, pbcr, pinj, kL, kT, s0, eps_r, eps_h, vpl, dtout, velmin, velmax
-  real(8) :: vref, Bs, dtinit, tout, meandisp0, moment, onset_time
+  real(8) :: vref, Bs, dtinit, tout, meandisp0, moment, onset_time, kp0, lf
   logical :: backslip, pressuredependent, pfconst, outpertime, slipping, outfield
   integer :: eventcount, hypoloc, lrtrn
   integer(c_int) :: rc
@@ -61,7 +61,7 @@ program lhbiem_fv_gpu
   evlaw = 'aging'; kT = 1e12; kL = 1e8; Bs = 0d0; outpertime = .false.; backslip = .false.
   tinj = 0d0; kpmax = 0d0; kpmin = 0d0; eta = 0d0; beta = 0d0; phi0 = 0d0; q0 = 0d0; a0 = 0d0; b0 = 0d0
   dc0 = 0d0; f00 = 0d0; tauinit = 0d0; sigmainit = 0d0; velinit = 0d0; pfinit = 0d0; muinit = 0d0
-  pbcl = 0d0; pbcr = 0d0; pinj = 0d0; s0 = 0d0; vpl = 0d0
+  pbcl = 0d0; pbcr = 0d0; pinj = 0d0; s0 = 0d0; vpl = 0d0; kp0 = 0d0
   vref = 1d-6; leaf = 32; precision = 'f64'; outbatch = 200
   do
     read(33, *, iostat=ios) param, pvalue
@@ -92,6 +92,7 @@ program lhbiem_fv_gpu
     case ('beta'); read(pvalue, *) beta
     case ('kpmax'); read(pvalue, *) kpmax
     case ('kpmin'); read(pvalue, *) kpmin
+    case ('kp0'); read(pvalue, *) kp0
     case ('s0'); read(pvalue, *) s0
     case ('kL'); read(pvalue, *) kL
     case ('kT'); read(pvalue, *) kT
@@ -183,6 +184,22 @@ program lhbiem_fv_gpu
     sige = sigma - pf
     tau = sige * muinit
     fo%setting = 1
+  case ('steadystate')  ! main_fv.f90:1452-1498: the linear pf profile carrying q0 along the fault
+    lf = ncellg * ds0
+    kp0 = kp0 * exp(sigmainit / s0)
+    do i = 1, ncellg
+      kTv(i) = kT
+      if (abs(xcol(i) - lf / 2) > lf / 6) kTv(i) = kT * 100
+    end do
+    kpmaxv = kp0 * (1 + kL / kTv / vpl) - kpmin * kL / kTv / vpl
+    ks = kp0
+    pbcr = 0d0
+    pbcl = pbcr + q0 * lf / (ks(1) * exp(-sigmainit / s0)) * eta * 1d-3
+    pf = pbcl + (pbcr - pbcl) * xcol / lf
+    sigma = sigmainit + pf
+    sige = sigma - pf
+    tau = sige * (f00 + (a - b) * log(velinit / vref)) * 1.001
+    fo%setting = 2
   case ('swarm')        ! main_fv.f90:1645-1675
     kTv = 1e12
     ks = kpmin

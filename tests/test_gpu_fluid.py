@@ -285,3 +285,33 @@ def test_gpu_fluid_stop_controls(kw, status):
     assert k_ora is not None
     info, hist = dev.step(200, history=True)
     assert info["status"] == status and info["accepted"] == k_ora and len(hist) == k_ora
+
+
+def test_gpu_fluid_driver_steadystate_setting(tmp_path):
+    """lhbiem_fv_gpu with setting steadystate (initcond_ss, main_fv.f90:1452-1498):
+    the initial pf record is the linear profile from pbcl = q0 lf eta 1e-3 / kp0
+    down to 0, kTv is raised outside the central third (through kpmaxv), and the
+    run advances with the steadystate flux terms of Beuler."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    n, nstep, ds0 = 512, 40, 0.01
+    keys = {"problem": "2dp", "ncellg": n, "filenumber": 4, "interval": 20, "nstep": nstep, "ds": ds0,
+            "setting": "steadystate", "kp0": 1e-13, "kpmin": 1e-14, "s0": 30.0, "vpl": 1e-9, "q0": 1e-7,
+            "eta": 1e-3, "beta": 1e-8, "phi0": 0.1, "sigmainit": 50.0, "velinit": "1d-9", "a": 0.01, "b": 0.005,
+            "dc": 0.01, "f0": 0.6}
+    P.write_in_file(str(tmp_path / "ss.in"), keys)
+    r = subprocess.run([os.path.join(root, "hbi_amd", "lhbiem_fv_gpu"), "ss.in"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    out = tmp_path / "output"
+    mon = np.loadtxt(out / "monitor4.dat")
+    assert mon.shape[0] == nstep + 1 and np.isfinite(mon).all()
+    pfs = np.fromfile(out / "pf4.dat").reshape(-1, n)
+    assert pfs.shape[0] == 3
+    lf = n * ds0
+    pbcl = 1e-7 * lf / 1e-13 * 1e-3 * 1e-3
+    xcol = (np.arange(n) + 0.5) * ds0
+    assert np.allclose(pfs[0], pbcl + (0.0 - pbcl) * xcol / lf, rtol=1e-12, atol=1e-12 * pbcl)
+    assert np.abs(pfs[-1] - pfs[0]).max() > 0
