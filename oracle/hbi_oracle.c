@@ -1710,34 +1710,57 @@ double og_errmax(int m, const double *ytemp, const double *yerr, double eps)
     return errmax;
 }
 
+/* Step-size decision of rkqs, main_LH.f90:2335-2357, for one attempt of
+   step h at time x with error ratio errmax:
+     accept iff tiny < errmax < 1 -> *hnew = min(2h, SAFETY h errmax^PGROW, dtmax)
+                                     (the next step's trial size), returns 1;
+     reject, errmax > tiny        -> *hnew = max(h/2, SAFETY h errmax^PSHRNK), returns 0;
+     reject, errmax <= tiny       -> *hnew = h/2, returns 0;
+     reject and x + hnew - x < 1e-15 (step underflow; the reference stops)
+                                  -> returns -1.
+   SAFETY, PGROW, PSHRNK and tiny are single-precision literals stored in
+   real(8) (main_LH.f90:2283). */
+int og_rkqs_control(double h, double errmax, double x, double dtmax, double *hnew)
+{
+    const double SAFETY = (double)0.9f, PGROW = (double)-0.2f, PSHRNK = (double)-0.25f, tiny = (double)1e-20f;
+    if (errmax < 1.0 && errmax > tiny) {
+        double hn = 2 * h, hg = SAFETY * h * pow(errmax, PGROW);
+        hn = hn < hg ? hn : hg;
+        hn = hn < dtmax ? hn : dtmax;
+        *hnew = hn;
+        return 1;
+    }
+    if (errmax > tiny) {
+        double h1 = 0.5 * h, h2 = SAFETY * h * pow(errmax, PSHRNK);
+        h = h1 > h2 ? h1 : h2;
+    } else {
+        h = 0.5 * h;
+    }
+    *hnew = h;
+    double xnew = x + h;
+    if (xnew - x < 1.e-15) return -1;
+    return 0;
+}
+
 /* rkqs, main_LH.f90:2270-2370.  Returns 0, or -1 when the step underflows
    (the reference stops there). */
 int og_rkqs(og_sim *s, double *y, double *x, double htry, double eps, double *hdid, double *hnext,
             double *errmax_out, int *nrjct)
 {
-    const double SAFETY = (double)0.9f, PGROW = (double)-0.2f, PSHRNK = (double)-0.25f, tiny = (double)1e-20f;
     int m = 3 * s->n;
     double *yerr = (double *)malloc(sizeof(double) * m * 2), *ytemp = yerr + m;
-    double h = htry, errmax;
+    double h = htry, errmax, hnew;
     *nrjct = 0;
     for (;;) {
         og_rkck(s, y, h, ytemp, yerr);
         errmax = og_errmax(m, ytemp, yerr, eps);
-        if (errmax < 1.0 && errmax > tiny) break;
+        int st = og_rkqs_control(h, errmax, *x, s->dtmax, &hnew);
+        if (st == 1) break;
         (*nrjct)++;
-        if (errmax > tiny) {
-            double h1 = 0.5 * h, h2 = SAFETY * h * pow(errmax, PSHRNK);
-            h = h1 > h2 ? h1 : h2;
-        } else {
-            h = 0.5 * h;
-        }
-        double xnew = *x + h;
-        if (xnew - *x < 1.e-15) { free(yerr); *errmax_out = errmax; return -1; }
+        h = hnew;
+        if (st < 0) { free(yerr); *errmax_out = errmax; return -1; }
     }
-    double hn = 2 * h, hg = SAFETY * h * pow(errmax, PGROW);
-    hn = hn < hg ? hn : hg;
-    hn = hn < s->dtmax ? hn : s->dtmax;
-    *hnext = hn;
+    *hnext = hnew;
     *hdid = h;
     *x = *x + h;
     memcpy(y, ytemp, sizeof(double) * m);
@@ -1756,39 +1779,54 @@ void og_init_state(int n, const double *a, const double *tau, const double *sigm
     }
 }
 
+/* Post-step update of main_LH.f90:1266-1287 after an accepted step dtdid
+   (viscous flow and opening off): slip advances by the trapezoid rule over
+   the old and the new slip rate, half a step each (1271 with the old V,
+   1284 with the new one); V is recomputed from the new state
+   (V = 2 vref e^(-psi/a) sinh(tau/(sigma a))).  Returns max |V| (1290);
+   *meanslip = sum |slip| ds / sum ds, summed in element order (1316-1324). */
+double og_post_step(int n, const double *y, const double *a, double vref, double dtdid, const double *ds,
+                    double *slip, double *vel, double *meanslip)
+{
+    double mvel = 0.0, ms = 0.0, sg = 0.0;
+    for (int i = 0; i < n; i++) {
+        double psi = y[3 * i], tau = y[3 * i + 1], sig = y[3 * i + 2];
+        slip[i] = slip[i] + vel[i] * dtdid * 0.5;
+        vel[i] = 2 * vref * exp(-psi / a[i]) * sinh(tau / sig / a[i]);
+        slip[i] = slip[i] + vel[i] * dtdid * 0.5;
+        if (fabs(vel[i]) > mvel) mvel = fabs(vel[i]);
+        ms = ms + fabs(slip[i]) * ds[i];
+        sg = sg + ds[i];
+    }
+    *meanslip = ms / sg;
+    return mvel;
+}
+
 /* The time loop of main_LH.f90:1219-1336 for nsteps accepted steps.
    slip, vel: in/out (cluster order).  ds: element areas (cluster order).
    hist (optional, nsteps x 6): t, dtdid, errmax, nrjct, log10(max V), mean slip.
-   vhist (optional, nsteps x nrec): V at the recorded cluster indices rec[].
+   vhist, shist (optional, nsteps x nrec): V and slip at the recorded cluster
+   indices rec[].
    Returns the number of steps done (< nsteps when the step underflowed or V
    left [velmin, velmax]). */
 int og_run(og_sim *s, double *y, double *x, double *dtnxt, double eps_r, int nsteps,
            double *slip, double *vel, const double *ds, double velmin, double velmax,
-           double *hist, int nrec, const int *rec, double *vhist)
+           double *hist, int nrec, const int *rec, double *vhist, double *shist)
 {
     int n = s->n;
-    double sg = 0.0;
-    for (int i = 0; i < n; i++) sg = sg + ds[i];
     for (int k = 0; k < nsteps; k++) {
         double dtdid, hnext, errmax;
         int nrjct;
         if (og_rkqs(s, y, x, *dtnxt, eps_r, &dtdid, &hnext, &errmax, &nrjct) != 0) return k;
         *dtnxt = hnext;
-        double mvel = 0.0, meanslip = 0.0;
-        for (int i = 0; i < n; i++) {
-            double psi = y[3 * i], tau = y[3 * i + 1], sig = y[3 * i + 2];
-            slip[i] = slip[i] + vel[i] * dtdid * 0.5;
-            vel[i] = 2 * s->vref * exp(-psi / s->a[i]) * sinh(tau / sig / s->a[i]);
-            slip[i] = slip[i] + vel[i] * dtdid * 0.5;
-            if (fabs(vel[i]) > mvel) mvel = fabs(vel[i]);
-            meanslip = meanslip + fabs(slip[i]) * ds[i];
-        }
-        meanslip = meanslip / sg;
+        double meanslip;
+        double mvel = og_post_step(n, y, s->a, s->vref, dtdid, ds, slip, vel, &meanslip);
         if (hist) {
             double *hrow = hist + (size_t)6 * k;
             hrow[0] = *x; hrow[1] = dtdid; hrow[2] = errmax; hrow[3] = nrjct; hrow[4] = log10(mvel); hrow[5] = meanslip;
         }
         if (vhist) for (int r = 0; r < nrec; r++) vhist[(size_t)k * nrec + r] = vel[rec[r]];
+        if (shist) for (int r = 0; r < nrec; r++) shist[(size_t)k * nrec + r] = slip[rec[r]];
         if (mvel > velmax || mvel < velmin) return k + 1;
     }
     return nsteps;

@@ -77,9 +77,13 @@ def lib():
         L.og_errmax.argtypes = [C.c_int, _dp, _dp, C.c_double]
         L.og_rkqs.restype = C.c_int
         L.og_rkqs.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, _ip]
+        L.og_rkqs_control.restype = C.c_int
+        L.og_rkqs_control.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+        L.og_post_step.restype = C.c_double
+        L.og_post_step.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.og_run.restype = C.c_int
         L.og_run.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_int, _dp, _dp, _dp, C.c_double,
-                             C.c_double, _dp, C.c_int, _ip, _dp]
+                             C.c_double, _dp, C.c_int, _ip, _dp, _dp]
         L.og_deriv.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double] + [C.c_double] * 10 + [_dp] * 3
         L.og_tableau_out.argtypes = [_dp]
         L.og_mesh_3ddip.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double] + [_dp] * 5
@@ -95,6 +99,27 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def rkqs_control(h, errmax, x, dtmax):
+    """Step-size decision of rkqs (main_LH.f90:2335-2357): (status, h_new) with
+    status 1 = accepted (h_new is the next trial step), 0 = rejected (h_new is
+    the retry), -1 = step-size underflow."""
+    hn = np.zeros(1)
+    st = lib().og_rkqs_control(float(h), float(errmax), float(x), float(dtmax), _ptr(hn))
+    return int(st), float(hn[0])
+
+
+def post_step(y, a, vref, dtdid, ds, slip, vel):
+    """Post-step update (main_LH.f90:1266-1324) in place on slip and vel;
+    returns (max |V|, mean slip)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    ds = np.ascontiguousarray(ds, dtype=np.float64)
+    ms = np.zeros(1)
+    mv = lib().og_post_step(len(slip), _ptr(y), _ptr(a), float(vref), float(dtdid), _ptr(ds), _ptr(slip),
+                            _ptr(vel), _ptr(ms))
+    return float(mv), float(ms[0])
 
 
 def _iptr(a):
@@ -263,16 +288,20 @@ class Sim:
         st = lib().og_rkqs(self.s, _ptr(y), _ptr(xx), htry, eps, _ptr(hd), _ptr(hn), _ptr(em), _iptr(nr))
         return st, float(xx[0]), float(hd[0]), float(hn[0]), float(em[0]), int(nr[0])
 
-    def run(self, y, x, dt, eps_r, nsteps, slip, vel, ds, velmin=1e-20, velmax=1e7, rec=None):
-        """Run nsteps accepted steps in place; returns (steps_done, x, dt, hist, vhist)."""
+    def run(self, y, x, dt, eps_r, nsteps, slip, vel, ds, velmin=1e-20, velmax=1e7, rec=None, slips=False):
+        """Run nsteps accepted steps in place; returns (steps_done, x, dt, hist, vhist),
+        and with slips=True also the slip history at rec (steps_done, nrec)."""
         xx = np.array([x], dtype=np.float64)
         dd = np.array([dt], dtype=np.float64)
         hist = np.zeros((nsteps, 6))
         rec = np.ascontiguousarray(rec if rec is not None else [], dtype=np.int32)
         vh = np.zeros((nsteps, max(len(rec), 1)))
+        sh = np.zeros((nsteps, max(len(rec), 1))) if slips else None
         done = lib().og_run(self.s, _ptr(y), _ptr(xx), _ptr(dd), eps_r, nsteps, _ptr(slip), _ptr(vel), _ptr(ds),
-                            velmin, velmax, _ptr(hist), len(rec), _iptr(rec), _ptr(vh))
-        return done, float(xx[0]), float(dd[0]), hist[:done], vh[:done, :len(rec)]
+                            velmin, velmax, _ptr(hist), len(rec), _iptr(rec), _ptr(vh),
+                            _ptr(sh) if slips else None)
+        out = (done, float(xx[0]), float(dd[0]), hist[:done], vh[:done, :len(rec)])
+        return out + (sh[:done, :len(rec)],) if slips else out
 
 
 # ---------------------------------------------------------------------------

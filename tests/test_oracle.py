@@ -311,3 +311,149 @@ def test_oracle_spring_slider_cycle_runs_and_rejects_bad_state():
     ybad[1] = np.nan
     st, *_ = sim.rkqs(ybad, 0.0, 1.0, 1e-6)
     assert st == -1
+
+
+# ---------------------------------------------------------------------------
+# rkqs step-size decision and the post-step update, against hand-worked values
+# ---------------------------------------------------------------------------
+# The constants are main_LH.f90:2283's single-precision literals stored in
+# real(8): SAFETY = 0.9f = 0.89999997615814208984375, PGROW = -0.2f =
+# -0.20000000298023223876953125, PSHRNK = -0.25f = -0.25, tiny = 1e-20f =
+# 9.999999682655225e-21.  Inputs are picked so the powers are exact or nearly so.
+
+def test_rkqs_control_accepts_and_grows_by_the_error_ratio():
+    # errmax = 1/32: 32^0.2 = 2, and the extra 2.98e-9 of -0.2f multiplies it
+    # by 32^(2.98e-9) = 1 + 1.0329e-8, so hnext = 0.9f * 2.0000000206576 h
+    # = 1.79999997090793995 h (< 2h, < dtmax)
+    st, hn = O.rkqs_control(1.0, 1.0 / 32, 0.0, 1e10)
+    assert st == 1 and abs(hn - 1.79999997090793995) <= 2e-16
+    st, hn = O.rkqs_control(4.0, 1.0 / 32, 100.0, 1e10)
+    assert st == 1 and abs(hn - 4 * 1.79999997090793995) <= 8e-16
+
+
+def test_rkqs_control_growth_caps_at_2h_and_dtmax():
+    # errmax = 1e-6: 0.9 * 1e-6^-0.2 = 14.3 > 2, so hnext = 2h exactly
+    assert O.rkqs_control(3.0, 1e-6, 0.0, 1e10) == (1, 6.0)
+    # dtmax below both: hnext = dtmax
+    assert O.rkqs_control(10.0, 1.0 / 32, 0.0, 5.0) == (1, 5.0)
+    # just above tiny: accepted, growth capped at 2h
+    assert O.rkqs_control(1.0, 2e-20, 0.0, 1e10) == (1, 2.0)
+
+
+def test_rkqs_control_rejects_and_shrinks():
+    # errmax = 81/16: (81/16)^-0.25 = 2/3 exactly, h_retry = 0.9f * 2/3 h
+    # = 0.59999998410542806 h (> h/2)
+    st, hn = O.rkqs_control(1.0, 81.0 / 16, 0.0, 1e10)
+    assert st == 0 and abs(hn - 0.59999998410542806) <= 2e-16
+    # errmax = 16: 16^-0.25 = 1/2, 0.9f/2 h < h/2, so the retry is h/2
+    assert O.rkqs_control(1.0, 16.0, 0.0, 1e10) == (0, 0.5)
+    # errmax = 1 is a rejection (accept needs errmax < 1): retry 0.9f h
+    assert O.rkqs_control(1.0, 1.0, 0.0, 1e10) == (0, 0.89999997615814208984375)
+    # errmax <= tiny (incl. zero and tiny itself) and NaN: retry h/2
+    for e in (0.0, 9.999999682655225e-21, 1e-30, float("nan")):
+        assert O.rkqs_control(2.0, e, 0.0, 1e10) == (0, 1.0), e
+
+
+def test_rkqs_control_step_underflow():
+    # at t = 1e10 s the spacing of doubles is 1.9e-6 s: a retry of
+    # max(5e-7, 0.9f * 1e-6 * 2^-0.25 = 7.568e-7) rounds away (x + h - x = 0)
+    st, hn = O.rkqs_control(1e-6, 2.0, 1e10, 1e10)
+    assert st == -1 and abs(hn - 7.568067536798102e-07) <= 1e-21
+    # the same retry at t = 1e3 s is representable
+    assert O.rkqs_control(1e-6, 2.0, 1e3, 1e10)[0] == 0
+
+
+def test_rkqs_attempt_sequence_follows_the_control():
+    """og_rkqs = rkck + errmax + the decision above, repeated from htry until
+    an accept: replaying the attempts with the oracle's rkck and errmax gives
+    the same hdid, hnext, errmax and rejection count."""
+    H, fr, sim, y = _slider()
+    y = y.copy()
+    y[0::3] -= 0.12
+    L = O.lib()
+    for htry in (1e4, 1e6, 3e7):
+        yr = y.copy()
+        st, x, hdid, hnext, em, nr = sim.rkqs(yr, 5.0, htry, 1e-6)
+        h, rej = htry, 0
+        while True:
+            yt, ye = sim.rkck(y, h)
+            e = L.og_errmax(len(y), O._ptr(np.ascontiguousarray(yt)), O._ptr(np.ascontiguousarray(ye)), 1e-6)
+            s2, hn = O.rkqs_control(h, e, 5.0, 1e10)
+            if s2 == 1:
+                break
+            rej += 1
+            h = hn
+        assert st == 0 and (hdid, hnext, em, nr) == (h, hn, e, rej)
+        assert x == 5.0 + h and np.array_equal(yr, yt)
+        if htry == 3e7:
+            assert nr >= 1
+
+
+def _state_for(v, a, vref=1e-6, sig=50.0):
+    """(psi, tau, sigma) rows with psi = 0 whose slip rate is v."""
+    v, a = np.asarray(v, float), np.asarray(a, float)
+    y = np.zeros(3 * len(v))
+    y[1::3] = sig * a * np.arcsinh(v / (2 * vref))
+    y[2::3] = sig
+    return y
+
+
+def test_post_step_trapezoid_slip_hand_worked():
+    """main_LH.f90:1266-1287: slip advances by half a step at the old V and
+    half at the new V.  Three elements (areas 1, 3, 2):
+    step 1 (dt 10 s): V (1, 2, -1) nm/s -> (3, 4, -3) nm/s
+        slip = 5(1+3), 5(2+4), 5(-1-3) nm = (20, 30, -20) nm
+        mean |slip| = (20 + 90 + 40)/6 = 25 nm, max |V| = 4 nm/s
+    step 2 (dt 4 s): V -> (1, 1, -1) nm/s
+        slip = (20 + 2(3+1), 30 + 2(4+1), -20 - 2(3+1)) = (28, 40, -28) nm
+        mean |slip| = (28 + 120 + 56)/6 = 34 nm, max |V| = 1 nm/s"""
+    a = np.array([0.01, 0.02, 0.015])
+    ds = np.array([1.0, 3.0, 2.0])
+    slip = np.zeros(3)
+    vel = np.array([1e-9, 2e-9, -1e-9])
+    mv, ms = O.post_step(_state_for([3e-9, 4e-9, -3e-9], a), a, 1e-6, 10.0, ds, slip, vel)
+    assert np.allclose(slip, [20e-9, 30e-9, -20e-9], rtol=1e-13, atol=0)
+    assert np.allclose(vel, [3e-9, 4e-9, -3e-9], rtol=1e-13, atol=0)
+    assert abs(ms - 25e-9) <= 1e-13 * 25e-9 and abs(mv - 4e-9) <= 1e-13 * 4e-9
+    mv, ms = O.post_step(_state_for([1e-9, 1e-9, -1e-9], a), a, 1e-6, 4.0, ds, slip, vel)
+    assert np.allclose(slip, [28e-9, 40e-9, -28e-9], rtol=1e-13, atol=0)
+    assert abs(ms - 34e-9) <= 1e-13 * 34e-9 and abs(mv - 1e-9) <= 1e-13 * 1e-9
+
+
+def test_post_step_uniform_sliding_slip_is_vpl_t():
+    """Steady sliding at Vpl: after any sequence of steps, slip = Vpl t on
+    every element and the mean slip is Vpl t."""
+    vpl, n = 1e-9, 5
+    a = np.full(n, 0.012)
+    y = _state_for(np.full(n, vpl), a)
+    ds = np.linspace(1.0, 2.0, n)
+    slip, vel = np.zeros(n), np.full(n, vpl)
+    vel[:] = 2e-6 * np.sinh(y[1::3] / y[2::3] / a)
+    v0 = vel.copy()
+    t = 0.0
+    for dt in np.random.default_rng(3).uniform(1e3, 1e7, 50):
+        mv, ms = O.post_step(y, a, 1e-6, dt, ds, slip, vel)
+        t += dt
+    assert np.allclose(slip, v0 * t, rtol=1e-12, atol=0) and abs(v0 / vpl - 1).max() < 1e-12
+    assert abs(ms - vpl * t) <= 1e-12 * vpl * t
+
+
+def test_run_history_rows_are_the_post_step_monitors():
+    """og_run's history: t, dtdid, errmax, rejections, log10 max|V| and mean
+    slip of each accepted step equal a replay with rkqs + post_step."""
+    H, fr, sim, y = _slider()
+    y = y.copy()
+    y[0::3] -= 0.05
+    n = len(y) // 3
+    a = np.asarray(fr.a)[H.lod] if hasattr(H, "lod") else np.asarray(fr.a)
+    vel0 = 2 * fr.vref * np.exp(-y[0::3] / a) * np.sinh(y[1::3] / y[2::3] / a)
+    ds = np.ones(n)
+    y1, slip1, vel1 = y.copy(), np.zeros(n), vel0.copy()
+    done, x, dt, hist, _ = sim.run(y1, 0.0, 1.0, 1e-6, 40, slip1, vel1, ds)
+    y2, slip2, vel2 = y.copy(), np.zeros(n), vel0.copy()
+    xx, h = 0.0, 1.0
+    for k in range(done):
+        st, xx, hdid, h, em, nr = sim.rkqs(y2, xx, h, 1e-6)
+        mv, ms = O.post_step(y2, a, fr.vref, hdid, ds, slip2, vel2)
+        assert st == 0 and tuple(hist[k]) == (xx, hdid, em, nr, math.log10(mv), ms)
+    assert done == 40 and np.array_equal(slip1, slip2) and x == xx and dt == h
