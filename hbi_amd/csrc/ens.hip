@@ -22,7 +22,8 @@
 namespace hbi {
 
 constexpr int kR = 64;
-constexpr int kHistCols = 5;
+constexpr int kHistCols = 5;      // (t, dt, errmax, rejections before the step, log10 Vmax)
+constexpr int kEnsProbes = 16;    // probe elements at most; each adds (V, slip) to a history row
 
 struct EnsPar {
     int n, nreal, law, backslip, limitsigma, tr;
@@ -30,6 +31,12 @@ struct EnsPar {
     Tab T;
     int learn, use_mlp;                   // step features on; MLP proposals on
     double safety, ratio_lo, ratio_hi;    // learned proposal: factor and bounds relative to rkqs
+    int nprobe;                           // history rows carry (V, slip) of these elements
+    int probe[kEnsProbes];
+    // replayed step grid (or null): accepted step k of realisation r has size
+    // grid[r * gcap + k]; every attempt is accepted
+    const double *grid;
+    long long gcap;
 };
 
 struct EnsCtl {           // per realisation
@@ -255,7 +262,14 @@ __global__ __launch_bounds__(kDecThreads) void ens_decide_kernel(EnsPar P, EnsCt
         for (int q = 1; q < kDecWaves; q++) t = e_combine(t, EPair{sg[q][r], sv[q][r]});
         const double errmax = t.v, h = c->h[r];
         c->errmax[r] = errmax;
-        if (errmax < 1.0 && errmax > tiny) {
+        if (P.grid) {
+            // replay: the step is taken whatever errmax; the next size comes from the grid
+            c->hdid[r] = h;
+            c->t[r] = c->t[r] + h;
+            c->acc[r] = 1;
+            c->accepted[r]++;
+            if (c->accepted[r] < P.gcap) c->h[r] = P.grid[(size_t)r * P.gcap + c->accepted[r]];
+        } else if (errmax < 1.0 && errmax > tiny) {
             double hn = fmin(2 * h, SAFETY * h * pow(errmax, PGROW));
             hn = fmin(hn, P.dtmax);
             c->hdid[r] = h;
@@ -362,9 +376,12 @@ __global__ __launch_bounds__(256) void ens_hist_kernel(EnsPar P, const EnsCtl *_
     }
 }
 
-// history row per accepted step of realisation r: (t, dt, errmax, rejections before it, log10 Vmax)
+// history row per accepted step of realisation r: (t, dt, errmax, rejections before it, log10 Vmax),
+// then (V, slip) of every probe element
 __global__ __launch_bounds__(kDecThreads) void ens_monitor_kernel(EnsPar P, EnsCtl *c, const double *__restrict__ monpart,
-                                                                  int nblk, double *__restrict__ hist)
+                                                                  int nblk, double *__restrict__ hist,
+                                                                  const double *__restrict__ velc,
+                                                                  const double *__restrict__ slip)
 {
     __shared__ double sm[kDecWaves][kR];
     if (c->done) return;
@@ -383,12 +400,17 @@ __global__ __launch_bounds__(kDecThreads) void ens_monitor_kernel(EnsPar P, EnsC
             c->vmax[r] = v;
             const long long row = c->accepted[r] - 1 - c->hist_base[r];
             if (hist && row >= 0 && row < c->hist_cap) {
-                double *o = hist + ((size_t)r * c->hist_cap + row) * kHistCols;
+                double *o = hist + ((size_t)r * c->hist_cap + row) * (kHistCols + 2 * P.nprobe);
                 o[0] = c->t[r];
                 o[1] = c->hdid[r];
                 o[2] = c->errmax[r];
                 o[3] = (double)c->nrjct[r];
                 o[4] = log10(v);
+                for (int q = 0; q < P.nprobe; q++) {
+                    const size_t e = (size_t)P.probe[q] * kR + r;
+                    o[kHistCols + 2 * q] = velc[e];
+                    o[kHistCols + 2 * q + 1] = slip[e];
+                }
             }
             c->nrjct[r] = 0;
             if (v > P.velmax || v < P.velmin) { c->status[r] = 1; c->frozen[r] = 1; }
@@ -418,7 +440,7 @@ __global__ __launch_bounds__(kHid) void ens_learn_kernel(EnsPar P, EnsCtl *c, En
         row[13] = c->errmax[r];
         c->trace_n[r]++;
     }
-    if (P.use_mlp) {
+    if (P.use_mlp && !P.grid) {
         const float out = mlp_infer_block(A.mlp, x, h1, red);
         if (j == 0) c->h[r] = learned_h(x[11], out, c->h[r], P.safety, P.ratio_lo, P.ratio_hi, P.dtmax);
     }
@@ -432,11 +454,13 @@ __global__ __launch_bounds__(kHid) void ens_learn_kernel(EnsPar P, EnsCtl *c, En
     for (int b = 0; b < kBins / kHid; b++) bins[j * (kBins / kHid) + b] = 0;
 }
 
-__global__ void ens_arm_kernel(EnsCtl *c, int nreal, long long nsteps, long long hist_cap)
+__global__ void ens_arm_kernel(EnsCtl *c, int nreal, long long nsteps, long long hist_cap, const double *grid,
+                               long long gcap)
 {
     const int r = threadIdx.x;
     if (r == 0) c->hist_cap = hist_cap;
     if (r < nreal) {
+        if (grid && c->accepted[r] < gcap) c->h[r] = grid[(size_t)r * gcap + c->accepted[r]];
         c->hist_base[r] = c->accepted[r];
         c->target[r] = c->accepted[r] + nsteps;
         c->frozen[r] = c->status[r] != 0 ? 1 : 0;
@@ -459,10 +483,11 @@ struct hbi_ens {
     hbi::DevBuf<float> x32;      // FP32 copy of x for the FP32-core product
     bool x32_direct = false;     // the stage kernels write x32 themselves (FP32-core product)
     hbi::DevBuf<double> fold;   // first-level reduction partials (kFold x 2 x 64)
-    hbi::DevBuf<double> hist;   // nreal-major rows of kHistCols (graph argument)
-    long long hist_rows = 0;
+    hbi::DevBuf<double> hist;   // nreal-major rows of kHistCols + 2 nprobe (graph argument)
+    long long hist_rows = 0;    // rows the buffer holds
     hbi::DevBuf<hbi::EnsCtl> ctl;
     hbi::DevBuf<int> bins;
+    hbi::DevBuf<double> grid;   // replayed step sizes, nreal x gcap (hbi_ens_set_step_grid)
     hbi::DevBuf<double> trace;
     int trace_cap = 0;
     hbi::DevBuf<float> mlp_p, mlp_mu, mlp_sd;   // copy of the step-size MLP
@@ -514,7 +539,8 @@ static void e_attempt(hbi_ens *E)
         ens_learn_kernel<<<E->nreal, kHid, 0, E->stream>>>(E->P, E->ctl.p, A);
     }
     ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->monpart.p, gc, E->fold.p, 1);
-    ens_monitor_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold, E->hist.p);
+    ens_monitor_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold, E->hist.p, E->velc.p,
+                                                        E->slip.p);
 }
 
 // host (realisation-major, nreal x n x comps) -> device (element-major,
@@ -676,16 +702,17 @@ int hbi_ens_step(hbi_ens *E, int nsteps, double *hist, long long *attempts)
     return hbi::guarded([&] {
         HBI_REQUIRE(E && nsteps >= 0, "bad arguments");
         const size_t hrows = (size_t)E->nreal * nsteps;
+        const size_t hcols = hbi::kHistCols + 2 * (size_t)E->P.nprobe;
         if (hist && (long long)hrows > E->hist_rows) {
             // the history buffer is a graph argument: a larger one needs a new capture
-            E->hist.alloc(hrows * hbi::kHistCols);
+            E->hist.alloc(hrows * (hbi::kHistCols + 2 * hbi::kEnsProbes));
             E->hist_rows = (long long)hrows;
             if (E->graph) {
                 HBI_HIP(hipGraphExecDestroy(E->graph));
                 E->graph = nullptr;
             }
         }
-        if (hist) HBI_HIP(hipMemsetAsync(E->hist.p, 0, hrows * hbi::kHistCols * sizeof(double), E->stream));
+        if (hist) HBI_HIP(hipMemsetAsync(E->hist.p, 0, hrows * hcols * sizeof(double), E->stream));
         if (!E->graph) {
             hipGraph_t gr;
             HBI_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
@@ -694,7 +721,17 @@ int hbi_ens_step(hbi_ens *E, int nsteps, double *hist, long long *attempts)
             HBI_HIP(hipGraphInstantiate(&E->graph, gr, nullptr, nullptr, 0));
             (void)hipGraphDestroy(gr);
         }
-        hbi::ens_arm_kernel<<<1, hbi::kR, 0, E->stream>>>(E->ctl.p, E->nreal, nsteps, hist ? nsteps : 0);
+        if (E->P.grid) {
+            // a replayed grid must cover the steps asked for
+            hbi::EnsCtl c0;
+            HBI_HIP(hipMemcpyAsync(&c0, E->ctl.p, sizeof(c0), hipMemcpyDeviceToHost, E->stream));
+            HBI_HIP(hipStreamSynchronize(E->stream));
+            for (int r = 0; r < E->nreal; r++)
+                HBI_REQUIRE(c0.status[r] != 0 || c0.accepted[r] + nsteps <= E->P.gcap,
+                            "the replayed step grid ends before these steps");
+        }
+        hbi::ens_arm_kernel<<<1, hbi::kR, 0, E->stream>>>(E->ctl.p, E->nreal, nsteps, hist ? nsteps : 0, E->P.grid,
+                                                          E->P.gcap);
         HBI_HIP(hipEventRecord(E->e0, E->stream));
         hbi::EnsCtl c;
         long long launched = 0, budget = 50LL * nsteps + 100;
@@ -717,7 +754,7 @@ int hbi_ens_step(hbi_ens *E, int nsteps, double *hist, long long *attempts)
         E->last_ms = ms;
         if (attempts) *attempts = launched;
         if (hist) {
-            E->hist.download(hist, hrows * hbi::kHistCols, E->stream);
+            E->hist.download(hist, hrows * hcols, E->stream);
             HBI_HIP(hipStreamSynchronize(E->stream));
         }
     });
@@ -768,6 +805,37 @@ int hbi_ens_set_controller(hbi_ens *E, int mode, const hbi_mlp *m, double safety
         }
         E->P.use_mlp = mode;
         E->P.learn = (mode == 1 || E->trace_cap > 0) ? 1 : 0;
+        drop_graph(E);
+    });
+}
+
+int hbi_ens_set_probes(hbi_ens *E, int nprobe, const int *idx)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(E && nprobe >= 0 && nprobe <= hbi::kEnsProbes && (nprobe == 0 || idx), "0 <= nprobe <= 16");
+        for (int q = 0; q < nprobe; q++) HBI_REQUIRE(idx[q] >= 0 && idx[q] < E->n, "probe index out of range");
+        E->P.nprobe = nprobe;
+        for (int q = 0; q < nprobe; q++) E->P.probe[q] = idx[q];
+        drop_graph(E);
+    });
+}
+
+int hbi_ens_set_step_grid(hbi_ens *E, long long cap, const double *h)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(E && cap >= 0 && (cap == 0 || h), "bad arguments");
+        if (cap > 0) {
+            for (size_t q = 0; q < (size_t)E->nreal * cap; q++)
+                HBI_REQUIRE(h[q] > 0.0 && std::isfinite(h[q]), "step sizes must be positive and finite");
+            std::vector<double> t((size_t)hbi::kR * cap, 1.0);
+            std::memcpy(t.data(), h, sizeof(double) * (size_t)E->nreal * cap);
+            E->grid.upload(t, E->stream);
+            E->P.grid = E->grid.p;
+        } else {
+            E->grid.release();
+            E->P.grid = nullptr;
+        }
+        E->P.gcap = cap;
         drop_graph(E);
     });
 }

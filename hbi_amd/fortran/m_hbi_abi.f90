@@ -215,6 +215,18 @@ module m_hbi_abi
       real(c_double), intent(out) :: hist(*)
       integer(c_long_long), intent(out) :: attempts
     end function
+    integer(c_int) function hbi_ens_set_probes(e, nprobe, idx) bind(C, name="hbi_ens_set_probes")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: nprobe
+      integer(c_int), intent(in) :: idx(*)
+    end function
+    integer(c_int) function hbi_ens_set_step_grid(e, cap, h) bind(C, name="hbi_ens_set_step_grid")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: e
+      integer(c_long_long), value :: cap
+      real(c_double), intent(in) :: h(*)
+    end function
     integer(c_int) function hbi_ens_set_controller(e, mode, m, safety, ratio_lo, ratio_hi) &
         bind(C, name="hbi_ens_set_controller")
       import :: c_int, c_ptr, c_double

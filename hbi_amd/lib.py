@@ -117,7 +117,7 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_ens_step", "hbi_ens_last_ms", "hbi_mlp_create", "hbi_mlp_destroy", "hbi_mlp_nparams",
            "hbi_mlp_set_params", "hbi_mlp_get_params", "hbi_mlp_forward", "hbi_mlp_loss_grad", "hbi_mlp_train",
            "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace", "hbi_ens_set_controller",
-           "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_rk_step_many", "hbi_rk_set_output", "hbi_rk_get_output",
+           "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_ens_set_probes", "hbi_ens_set_step_grid", "hbi_rk_step_many", "hbi_rk_set_output", "hbi_rk_get_output",
            "hbi_rk_set_opening", "hbi_rk_get_opening", "hbi_rk_set_probes", "hbi_rk_get_probes",
            "hbi_fv_create", "hbi_fv_destroy", "hbi_fv_set_state", "hbi_fv_get_state", "hbi_fv_derivs",
            "hbi_fv_diffuse", "hbi_fv_step", "hbi_fv_last_ms", "hbi_fv_set_probes", "hbi_fv_get_probes")
@@ -159,6 +159,8 @@ def lib():
         L.hbi_ens_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _lp, _lp, _ip]
         L.hbi_ens_step.argtypes = [C.c_void_p, C.c_int, _dp, _lp]
         L.hbi_ens_last_ms.argtypes = [C.c_void_p, _dp]
+        L.hbi_ens_set_probes.argtypes = [C.c_void_p, C.c_int, _ip]
+        L.hbi_ens_set_step_grid.argtypes = [C.c_void_p, C.c_longlong, _dp]
         _fp = C.POINTER(C.c_float)
         L.hbi_mlp_create.argtypes = [C.c_ulonglong, C.POINTER(C.c_void_p)]
         L.hbi_mlp_destroy.argtypes = [C.c_void_p]
@@ -607,12 +609,31 @@ class Ensemble:
                                        rej.ctypes.data_as(C.POINTER(C.c_longlong)), _ipp(st)))
         return {"y": y, "slip": slip, "vel": vel, "t": t, "accepted": acc, "rejected": rej, "status": st}
 
+    def set_probes(self, idx):
+        """History rows of the following step calls also carry (V, slip) of up
+        to 16 elements (cluster-order indices), per realisation."""
+        a = np.ascontiguousarray(idx, dtype=np.int32)
+        _check(lib().hbi_ens_set_probes(self.e, int(a.size), a.ctypes.data_as(_ip) if a.size else None))
+        self._nprobe = int(a.size)
+
+    def set_step_grid(self, h=None):
+        """Replay step sizes h (nreal, cap): accepted step k of realisation r
+        takes h[r, k], every attempt accepted; None returns to rkqs."""
+        if h is None:
+            _check(lib().hbi_ens_set_step_grid(self.e, 0, None))
+            return
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        if h.ndim != 2 or h.shape[0] != self.nreal:
+            raise ValueError("h must be (nreal, cap)")
+        _check(lib().hbi_ens_set_step_grid(self.e, h.shape[1], _p(h)))
+
     def step(self, nsteps, history=False):
         """Advance every running realisation nsteps accepted steps.  Returns the
-        lockstep attempts, and with history=True also (nreal, nsteps, 5) rows of
-        (t, dt, errmax, rejections before the step, log10 Vmax)."""
+        lockstep attempts, and with history=True also (nreal, nsteps, 5 + 2 p)
+        rows of (t, dt, errmax, rejections before the step, log10 Vmax) and
+        (V, slip) of the p probe elements."""
         att = C.c_longlong(0)
-        hist = np.zeros((self.nreal, nsteps, 5)) if history else None
+        hist = np.zeros((self.nreal, nsteps, 5 + 2 * getattr(self, "_nprobe", 0))) if history else None
         _check(lib().hbi_ens_step(self.e, int(nsteps), _p(hist), C.byref(att)))
         return (att.value, hist) if history else att.value
 

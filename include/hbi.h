@@ -338,8 +338,21 @@ int hbi_ens_get_state(hbi_ens *e, double *y, double *slip, double *vel, double *
    A realisation that underflows (HBI_E_STEP) or leaves [velmin, velmax]
    stops there; the others go on.  hist (may be NULL): nreal x nsteps rows of
    (t, dt, errmax, rejected attempts before the step, log10 Vmax) per accepted
-   step, zero after a stop. */
+   step, followed by (V, slip) of each probe element (hbi_ens_set_probes),
+   zero after a stop. */
 int hbi_ens_step(hbi_ens *e, int nsteps, double *hist, long long *attempts);
+/* Probe elements (cluster-order indices, at most 16) whose slip rate and
+   slip every history row of hbi_ens_step carries (main_LH.f90:1099-1122's
+   local output, per realisation); nprobe = 0 removes them. */
+int hbi_ens_set_probes(hbi_ens *e, int nprobe, const int *idx);
+/* Replay a step grid (verification of reduced-precision runs against FP64 on
+   a common time grid): h is nreal x cap, accepted step k (counted since
+   hbi_ens_set_state) of realisation r has size h[r * cap + k]; every attempt
+   is accepted (errmax is still computed and recorded), the learned proposal
+   is ignored, and a step call that would run past cap steps fails.  The
+   first step's size is h[r * cap], whatever dt_next was.  cap = 0 returns to
+   rkqs.  No counterpart in the reference. */
+int hbi_ens_set_step_grid(hbi_ens *e, long long cap, const double *h);
 /* Step-size control of every realisation, as hbi_rk_set_controller (the
    features and proposals are per realisation; the MLP is shared). */
 int hbi_ens_set_controller(hbi_ens *e, int mode, const hbi_mlp *m, double safety, double ratio_lo, double ratio_hi);

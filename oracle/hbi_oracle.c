@@ -1831,3 +1831,31 @@ int og_run(og_sim *s, double *y, double *x, double *dtnxt, double eps_r, int nst
     }
     return nsteps;
 }
+
+/* The same loop on a given step grid: accepted step k has size hs[k] (one
+   rkck per step, taken whatever its errmax; the error estimate is recorded),
+   followed by the post-step update.  A replay of another run's step grid, so
+   that two runs can be compared at identical times; the rkqs decision is
+   not involved.  hist, vhist, shist as og_run. */
+void og_run_grid(og_sim *s, double *y, double *x, const double *hs, int nsteps, double eps_r, double *slip,
+                 double *vel, const double *ds, double *hist, int nrec, const int *rec, double *vhist, double *shist)
+{
+    int n = s->n, m = 3 * n;
+    double *yerr = (double *)malloc(sizeof(double) * m * 2), *ytemp = yerr + m;
+    for (int k = 0; k < nsteps; k++) {
+        double h = hs[k];
+        og_rkck(s, y, h, ytemp, yerr);
+        double errmax = og_errmax(m, ytemp, yerr, eps_r);
+        memcpy(y, ytemp, sizeof(double) * m);
+        *x = *x + h;
+        double meanslip;
+        double mvel = og_post_step(n, y, s->a, s->vref, h, ds, slip, vel, &meanslip);
+        if (hist) {
+            double *hrow = hist + (size_t)6 * k;
+            hrow[0] = *x; hrow[1] = h; hrow[2] = errmax; hrow[3] = 0; hrow[4] = log10(mvel); hrow[5] = meanslip;
+        }
+        if (vhist) for (int r = 0; r < nrec; r++) vhist[(size_t)k * nrec + r] = vel[rec[r]];
+        if (shist) for (int r = 0; r < nrec; r++) shist[(size_t)k * nrec + r] = slip[rec[r]];
+    }
+    free(yerr);
+}

@@ -77,6 +77,8 @@ def lib():
         L.og_errmax.argtypes = [C.c_int, _dp, _dp, C.c_double]
         L.og_rkqs.restype = C.c_int
         L.og_rkqs.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, _ip]
+        L.og_run_grid.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_int,
+                                  _ip, _dp, _dp]
         L.og_rkqs_control.restype = C.c_int
         L.og_rkqs_control.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp]
         L.og_post_step.restype = C.c_double
@@ -302,6 +304,21 @@ class Sim:
                             _ptr(sh) if slips else None)
         out = (done, float(xx[0]), float(dd[0]), hist[:done], vh[:done, :len(rec)])
         return out + (sh[:done, :len(rec)],) if slips else out
+
+
+    def run_grid(self, y, x, hs, eps_r, slip, vel, ds, rec=None):
+        """Replay the step sizes hs in place (every step taken); returns (x,
+        hist, vhist, shist) with hist rows as run's (rejections 0)."""
+        hs = np.ascontiguousarray(hs, dtype=np.float64)
+        n = len(hs)
+        xx = np.array([x], dtype=np.float64)
+        hist = np.zeros((n, 6))
+        rec = np.ascontiguousarray(rec if rec is not None else [], dtype=np.int32)
+        vh = np.zeros((n, max(len(rec), 1)))
+        sh = np.zeros((n, max(len(rec), 1)))
+        lib().og_run_grid(self.s, _ptr(y), _ptr(xx), _ptr(hs), n, eps_r, _ptr(slip), _ptr(vel), _ptr(ds), _ptr(hist),
+                          len(rec), _iptr(rec), _ptr(vh), _ptr(sh))
+        return float(xx[0]), hist, vh[:, :len(rec)], sh[:, :len(rec)]
 
 
 # ---------------------------------------------------------------------------

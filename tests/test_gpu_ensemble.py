@@ -51,6 +51,36 @@ def test_gpu_matmul_matches_matvec_columns(label, make, leaf, eps):
         assert st["kt_max"] > 64      # exercises the split of ranks above 64
 
 
+@pytest.mark.parametrize("label,make,leaf,eps,tol", [
+    ("2dvs", lambda: P.mesh_2dvs(2048, 40 / 2048), 32, 1e-6, 1e-9),
+    ("3dph", lambda: P.mesh_3dph(48, 24, 1.0), 32, 1e-5, 1e-6),
+])
+def test_gpu_matmul_against_oracle_and_dense(label, make, leaf, eps, tol):
+    """Every column of the 64-column product against the oracle's H-matrix
+    product and the dense kernel matrix (cluster order): FP64 factors within
+    the ACA tolerance of the dense product (1e-9 at eps 1e-6, 1e-6 at 1e-5,
+    as the single-vector product) and within 2 tol of the oracle's ACA
+    product; FP32 / mixed factors within 2e-6 of the oracle's product (FP32
+    rounding of factors, X and the slice sums)."""
+    from oracle import oracle_lib as O
+    g = make()
+    Hg = H.HMatrix(g, leaf=leaf, eps=eps, bf16_margin=1.0)
+    Ho = O.HMatrix(g, leaf=leaf, eps=eps, nthreads=8)
+    assert np.array_equal(Hg.lod, Ho.lod)
+    A = Ho.dense()
+    X = np.random.default_rng(5).standard_normal((g.n, 64))
+    YD = A @ X
+    YO = np.stack([Ho.matvec(X[:, c], nthreads=8) for c in range(64)], axis=1)
+    for prec, bd, bo in ((H.PREC_F64, tol, 2 * tol), (H.PREC_F32, None, 2e-6), (H.PREC_MIXED, None, 2e-6)):
+        Y = Hg.matmul(X, prec)
+        ed = max(np.linalg.norm(Y[:, c] - YD[:, c]) / np.linalg.norm(YD[:, c]) for c in range(64))
+        eo = max(np.linalg.norm(Y[:, c] - YO[:, c]) / np.linalg.norm(YO[:, c]) for c in range(64))
+        print("%s prec %d: max column error vs dense %.3e, vs oracle %.3e" % (label, prec, ed, eo), flush=True)
+        if bd is not None:
+            assert ed <= bd
+        assert eo <= bo
+
+
 def _c1_realisations(n, nreal):
     g = P.mesh_2dvs(n, 40.0 / n)
     base = P.friction_c1(g)

@@ -457,3 +457,21 @@ def test_run_history_rows_are_the_post_step_monitors():
         mv, ms = O.post_step(y2, a, fr.vref, hdid, ds, slip2, vel2)
         assert st == 0 and tuple(hist[k]) == (xx, hdid, em, nr, math.log10(mv), ms)
     assert done == 40 and np.array_equal(slip1, slip2) and x == xx and dt == h
+
+
+def test_run_grid_replays_the_run_bit_for_bit():
+    """og_run_grid on the accepted step sizes of og_run (rejected attempts
+    included in that run) reproduces its states, slips and history rows."""
+    H, fr, sim, y = _slider()
+    y = y.copy()
+    y[0::3] -= 0.1
+    n = len(y) // 3
+    a = np.asarray(fr.a)
+    vel0 = 2 * fr.vref * np.exp(-y[0::3] / a) * np.sinh(y[1::3] / y[2::3] / a)
+    y1, s1, v1 = y.copy(), np.zeros(n), vel0.copy()
+    done, x1, _, h1, vh1 = sim.run(y1, 0.0, 1.0, 1e-6, 60, s1, v1, np.ones(n), rec=[0])
+    assert done == 60 and h1[:, 3].sum() > 0
+    y2, s2, v2 = y.copy(), np.zeros(n), vel0.copy()
+    x2, h2, vh2, sh2 = sim.run_grid(y2, 0.0, h1[:, 1], 1e-6, s2, v2, np.ones(n), rec=[0])
+    assert x2 == x1 and np.array_equal(y1, y2) and np.array_equal(s1, s2)
+    assert np.array_equal(h1[:, [0, 1, 2, 4, 5]], h2[:, [0, 1, 2, 4, 5]]) and np.array_equal(vh1, vh2)
