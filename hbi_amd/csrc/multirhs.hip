@@ -11,30 +11,29 @@
 //            order -> Z (sum_kt x RR)
 //   phase 2  per tile item: Ypart = sum over segments of A_seg (TR x ncol)
 //            X_seg (ncol x RR), k-slices staged through LDS
-// Both phases reuse the packed single-vector layout of the precision's plan.
-// FP64 factors multiply on the FP64 matrix cores.  FP32 and BF16 factors
-// multiply on the FP32 matrix cores (twice the FP64 rate on this chip): X is
-// rounded once to an FP32 copy and Z is stored in FP32 (halving the bytes the
-// k-slices load), products accumulate in FP32 for at most one k-slice (24 or
-// 32 columns) and every slice's sums are added into FP64 accumulators, so the
-// accumulation error stays at a few FP32 roundings whatever the block width;
-// the phase-1 partials, the phase-2 partials and Y stay FP64.
+// FP64 factors multiply on the FP64 matrix cores, in the packed layout of the
+// single-vector product.  FP32 and BF16 factors multiply, in HBI_PREC_F32
+// plans, on the FP32 matrix cores (twice the FP64 rate on this chip) and, in
+// HBI_PREC_MIXED plans, on the BF16 matrix cores with every FP32 operand
+// split into three BF16 parts (FP32-sized products; see the BF16-core
+// section and mr_mode for how the two round).  In both, X is rounded once to
+// an FP32 copy, Z is stored in FP32, FP32 sums of at most one (FP32 cores)
+// or two (BF16 cores) 32-column slices are added into FP64 sums, and the
+// phase-1 partials, the phase-2 partials and Y stay FP64.
 #include "hmat.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace hbi {
 
 constexpr int kRR = 64;      // right-hand sides per product
 constexpr int kP1Rows = 64;     // a1^T rows per phase-1 item on the FP64 cores (one wave)
-// the same on the FP32 cores (two waves of 32 rows share X; 128-row items
-// halve the X reads but leave too few items for 256 CUs: 0.085 vs 0.080 ms
-// for phase 1 of the 32768-element fault)
-constexpr int kP1Rows32 = 64;
 constexpr int kKS = 32;      // k-slice staged in LDS
 constexpr int kMrThreads = 256;
 constexpr int kKS32 = 32;         // columns per slice of the FP32-core phase-2 kernel
-constexpr int kMaxCols32 = 4096;  // columns of one FP32-core phase-2 item at most
+constexpr int kP1Rows32 = 64;     // a1^T rows per phase-1 item on the FP32 cores (two waves of 32 rows)
+constexpr int kMaxCols32 = 4096;  // columns of one phase-2 item on the FP32 / BF16 cores at most (column map in LDS)
 
 template <typename T> __device__ __forceinline__ double widen(T v);
 template <> __device__ __forceinline__ double widen<double>(double v) { return v; }
@@ -93,9 +92,6 @@ template <> struct Run2<uint16_t> {
     __device__ float getf(int e) const { return __uint_as_float(e ? (v & 0xffff0000u) : (v << 16)); }
 };
 
-template <typename T> __device__ __forceinline__ float narrow(T v);
-template <> __device__ __forceinline__ float narrow<float>(float v) { return v; }
-template <> __device__ __forceinline__ float narrow<uint16_t>(uint16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
 
 // One wave per item: Zpart (kt x 64) = a1^T[rows, t0 .. t0+tn) X[t0 .. t0+tn, :]
 // on the FP64 matrix cores (16x16x4 MFMA; lane l = li + 16 lk supplies
@@ -625,12 +621,289 @@ __global__ void mr_tile_reduce_kernel(int n, int tr, const int *__restrict__ par
     Y[q] = s;
 }
 
+// --------------------------------------------- the BF16 matrix cores --
+// The same two phases for FP32 / BF16 factors on the BF16 matrix cores
+// (v_mfma_f32_16x16x32_bf16).  Every FP32 operand value is split exactly
+// into three BF16 parts, x = x_h + x_m + x_l (round to nearest at each step;
+// the remainder is below 2^-24 |x|), and each 16 x 16 x 32 block product is
+// the sum of the six part products with i + j <= 2 (the dropped ones are
+// below 2^-24 |a| |x|): FP32-level products at six BF16 instructions, which
+// run faster than the sixteen FP32 16x16x4 ones the same block needs.  The
+// FP32 sums of every 32-column slice are added into FP64 sums, as on the FP32
+// cores.
+//
+// A workgroup of NW waves takes an item of 16 NW rows; wave w owns rows
+// 16 w .. 16 w + 15 and all 64 right-hand sides (four 16-column MFMA blocks).
+// Per 32-column slice, the workgroup stages the slice's 32 X / Z rows once in
+// LDS, already split: three planes of BF16 pairs (columns 2 kp, 2 kp + 1 of a
+// right-hand side in one word), each right-hand side a row of 16 words plus 4
+// of padding, the four 16-byte word groups of a row XOR-swizzled by the row's
+// bits 4-5 -- so both the transposing stores (4 consecutive columns x 16
+// right-hand-side quads per wave) and the 16-byte operand reads (16 rows per
+// read phase) hit 64 distinct banks.  The A operand never passes through LDS:
+// phase 2 reads a copy of its factors packed in MFMA lane order (lane l of a
+// 16 x 32 block holds row l % 16, columns 8 (l / 16) .. + 7: 2 KB per block,
+// two 16-byte loads per lane), phase 1 reads its a1^T rows directly (eight
+// consecutive columns of a row are contiguous).  Two LDS buffers: the loads of
+// slice s + 1 are issued before slice s is multiplied and stored after it,
+// one barrier per slice.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+constexpr int kBfStages = 3;              // register stages of the load ring (4 measured slower: more VGPRs)
+constexpr int kBfRW = 20;                 // LDS words per right-hand-side row of a B plane
+constexpr int kBfPlane = kRR * kBfRW;     // words per plane
+constexpr int kP1RowsBf = 64;             // a1^T rows per phase-1 item at most (4 waves; 128-row items on 8
+                                          // waves read X half as often but measured slower: 0.076 vs 0.055 ms)
+
+// phase-2 item of the BF16-core kernel: its factors in lane order from float
+// index off16, columns through the column map as MItem2
+struct MItem2b {
+    uint64_t off16;
+    int ncol, cm0, part, pad;
+};
+
+__device__ __forceinline__ uint32_t bf_pack(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(fvec2{a, b}, bf16x2));
+}
+__device__ __forceinline__ fvec2 bf_unpack(uint32_t w)
+{
+    return fvec2{__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u)};
+}
+// (a, b) -> three packed BF16 words (high, middle, low parts)
+__device__ __forceinline__ void bf_split3(float a, float b, uint32_t &h, uint32_t &m, uint32_t &l)
+{
+#pragma clang fp contract(off)
+    h = bf_pack(a, b);
+    const fvec2 hf = bf_unpack(h);
+    const float ra = a - hf[0], rb = b - hf[1];
+    m = bf_pack(ra, rb);
+    const fvec2 mf = bf_unpack(m);
+    l = bf_pack(ra - mf[0], rb - mf[1]);
+}
+__device__ __forceinline__ bf16x8 as_bf8(uvec4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+template <int NW, bool P2, int NS>
+__global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restrict__ items2,
+                                                          const MItem1 *__restrict__ items1,
+                                                          const int *__restrict__ cmap,
+                                                          const float *__restrict__ a16,
+                                                          const uint8_t *__restrict__ buf,
+                                                          const float *__restrict__ X, const float *__restrict__ Z,
+                                                          double *__restrict__ out, const int *__restrict__ zrow,
+                                                          float *__restrict__ Zout, int prow,
+                                                          const int *__restrict__ gate)
+{
+    constexpr int NT = 64 * NW;
+    constexpr int PP = (256 + NT - 1) / NT;   // B pairs (two columns x four right-hand sides) per thread
+    __shared__ __align__(16) uint32_t bs[2][3][kBfPlane];
+    __shared__ int cm[P2 ? kMaxCols32 : 1];
+    if (gate && *gate) return;
+    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+    const int c = lane % 16, q = lane / 16;
+    MItem2b it2{};
+    MItem1 it1{};
+    int ncol;
+    if constexpr (P2) {
+        it2 = items2[blockIdx.x];
+        ncol = it2.ncol;
+        for (int k = tid; k < ncol; k += NT) cm[k] = cmap[it2.cm0 + k];
+        __syncthreads();
+    } else {
+        it1 = items1[blockIdx.x];
+        ncol = it1.tn;
+    }
+    const int nsl = (ncol + 31) / 32;
+    const float *zf = (const float *)g_zero_row;
+    // input row of item column col (past the item's end: zeros)
+    auto brow = [&](int col) -> const float * {
+        if (col >= ncol) return zf;
+        if constexpr (P2) {
+            const int m = cm[col];
+            return (m < 0 ? Z : X) + (size_t)(m & 0x7fffffff) * kRR;
+        } else {
+            return X + (size_t)(it1.xoff + it1.t0 + col) * kRR;
+        }
+    };
+    // a ring of NS register stages: slice s is loaded into stage s % NS three
+    // slices before it is multiplied; no branch surrounds a load (slices past
+    // the end read zeros), so each wait covers one stage
+    struct Stage {
+        fvec4 b0[PP], b1[PP];   // B: columns 2 kp, 2 kp + 1 of right-hand sides 4 m .. 4 m + 3
+        fvec4 a0, a1;           // A: row 16 w + c, columns 8 q .. 8 q + 7 of the slice
+    };
+    Stage st[NS];
+    auto load = [&](Stage &S, int s) {
+#pragma unroll
+        for (int u = 0; u < PP; u++) {
+            const int p = min(tid + u * NT, 255), kp = p / 16, m = p % 16;
+            S.b0[u] = *(const fvec4 *)(brow(32 * s + 2 * kp) + 4 * m);
+            S.b1[u] = *(const fvec4 *)(brow(32 * s + 2 * kp + 1) + 4 * m);
+        }
+        if constexpr (P2) {
+            const float *ap = s < nsl ? a16 + it2.off16 + ((size_t)(s * NW + w) * 64 + lane) * 8 : zf;
+            S.a0 = *(const fvec4 *)ap;
+            S.a1 = *(const fvec4 *)(ap + 4);
+        } else {
+            // FP32 rows: 8 values in two 16-byte loads; BF16 rows: one 16-byte load (read twice)
+            const int row = 16 * w + c, col = it1.t0 + 32 * s + 8 * q;
+            const bool ok = row < it1.kt && col < it1.t0 + it1.tn;
+            const bool f32 = it1.dtype == kF32;
+            const uint8_t *ap = ok ? buf + it1.off + ((size_t)row * it1.ldt + col) * (f32 ? 4 : 2)
+                                   : (const uint8_t *)zf;
+            const fvec4 v0 = *(const fvec4 *)ap, v1 = *(const fvec4 *)(ap + (f32 ? 16 : 0));
+            S.a0 = v0;
+            S.a1 = v1;
+        }
+    };
+    auto widen_a = [&](const Stage &S, fvec4 &a0, fvec4 &a1) {
+        a0 = S.a0;
+        a1 = S.a1;
+        if constexpr (!P2) {
+            if (it1.dtype != kF32) {   // BF16 factors: the 16 bytes hold 8 values
+                const uvec4 v = __builtin_bit_cast(uvec4, S.a0);
+                a0 = fvec4{__uint_as_float(v[0] << 16), __uint_as_float(v[0] & 0xffff0000u),
+                           __uint_as_float(v[1] << 16), __uint_as_float(v[1] & 0xffff0000u)};
+                a1 = fvec4{__uint_as_float(v[2] << 16), __uint_as_float(v[2] & 0xffff0000u),
+                           __uint_as_float(v[3] << 16), __uint_as_float(v[3] & 0xffff0000u)};
+            }
+        }
+    };
+    auto bstore = [&](const Stage &S, int b) {
+#pragma unroll
+        for (int u = 0; u < PP; u++) {
+            const int p = tid + u * NT;
+            if (p >= 256) break;
+            const int kp = p / 16, m = p % 16;
+            const int word = 4 * ((kp >> 2) ^ ((m >> 2) & 3)) + (kp & 3);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                uint32_t h, mm, l;
+                bf_split3(S.b0[u][i], S.b1[u][i], h, mm, l);
+                const int o = (4 * m + i) * kBfRW + word;
+                bs[b][0][o] = h;
+                bs[b][1][o] = mm;
+                bs[b][2][o] = l;
+            }
+        }
+    };
+    v4d sum[4];
+    fvec4 acc[4];
+#pragma unroll
+    for (int jb = 0; jb < 4; jb++) {
+        sum[jb] = v4d{0.0, 0.0, 0.0, 0.0};
+        acc[jb] = fvec4{0.f, 0.f, 0.f, 0.f};
+    }
+    // the six part products of one slice into the FP32 accumulators
+    auto multiply = [&](int b, const Stage &S) {
+        fvec4 a0, a1;
+        widen_a(S, a0, a1);
+        uint32_t ah[4], am[4], al[4];
+        bf_split3(a0[0], a0[1], ah[0], am[0], al[0]);
+        bf_split3(a0[2], a0[3], ah[1], am[1], al[1]);
+        bf_split3(a1[0], a1[1], ah[2], am[2], al[2]);
+        bf_split3(a1[2], a1[3], ah[3], am[3], al[3]);
+        const bf16x8 Ah = as_bf8(uvec4{ah[0], ah[1], ah[2], ah[3]}), Am = as_bf8(uvec4{am[0], am[1], am[2], am[3]}),
+                     Al = as_bf8(uvec4{al[0], al[1], al[2], al[3]});
+        bf16x8 Bh[4], Bm[4], Bl[4];
+#pragma unroll
+        for (int jb = 0; jb < 4; jb++) {
+            const int o = (16 * jb + c) * kBfRW + 4 * (q ^ (jb & 3));
+            Bh[jb] = as_bf8(*(const uvec4 *)&bs[b][0][o]);
+            Bm[jb] = as_bf8(*(const uvec4 *)&bs[b][1][o]);
+            Bl[jb] = as_bf8(*(const uvec4 *)&bs[b][2][o]);
+        }
+#pragma unroll
+        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh[jb], acc[jb], 0, 0, 0);
+#pragma unroll
+        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bl[jb], acc[jb], 0, 0, 0);
+#pragma unroll
+        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am, Bm[jb], acc[jb], 0, 0, 0);
+#pragma unroll
+        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am, Bh[jb], acc[jb], 0, 0, 0);
+#pragma unroll
+        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bm[jb], acc[jb], 0, 0, 0);
+#pragma unroll
+        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bh[jb], acc[jb], 0, 0, 0);
+    };
+    // FP32 sums of at most two slices (64 columns) are added into the FP64 sums
+    auto flush = [&] {
+#pragma unroll
+        for (int jb = 0; jb < 4; jb++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) sum[jb][i] += (double)acc[jb][i];
+            acc[jb] = fvec4{0.f, 0.f, 0.f, 0.f};
+        }
+    };
+#pragma unroll
+    for (int b = 0; b < NS - 1; b++) load(st[b], b);
+    bstore(st[0], 0);
+    __syncthreads();
+    // slice sl: stage sl % NS (i is a multiple of NS), LDS buffer sl % 2
+    for (int i = 0; i < nsl; i += NS) {
+#pragma unroll
+        for (int b = 0; b < NS; b++) {
+            const int sl = i + b;
+            load(st[(b + NS - 1) % NS], sl + NS - 1);
+            if (sl < nsl) multiply(sl & 1, st[b]);
+            bstore(st[(b + 1) % NS], (sl + 1) & 1);
+            __syncthreads();
+            if (sl & 1) flush();
+        }
+    }
+    flush();
+    // rows 16 w + 4 q + i, right-hand sides 16 jb + c
+    if constexpr (P2) {
+        double *o = out + (size_t)it2.part * (16 * NW) * kRR;
+#pragma unroll
+        for (int jb = 0; jb < 4; jb++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) o[(size_t)(16 * w + 4 * q + i) * kRR + 16 * jb + c] = sum[jb][i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = 16 * w + 4 * q + i;
+            if (row >= it1.kt) continue;
+#pragma unroll
+            for (int jb = 0; jb < 4; jb++) {
+                if (it1.part < 0) Zout[(size_t)zrow[it1.zr + row] * kRR + 16 * jb + c] = (float)sum[jb][i];
+                else out[((size_t)it1.part * prow + row) * kRR + 16 * jb + c] = sum[jb][i];
+            }
+        }
+    }
+}
+
+// Lane-order copy of phase-2 factors: block b of the table is (item, slice,
+// dtype); its 16 NW rows x 32 columns are written as 64-lane x 8 groups per
+// 16-row block (zeros past the item's columns)
+struct MPack {
+    uint64_t src, dst;   // byte offset in the plan buffer, float index in the copy
+    int ncol, s, dtype, nw;
+};
+__global__ __launch_bounds__(256) void mr_pack_bf_kernel(const MPack *__restrict__ tab, const uint8_t *__restrict__ buf,
+                                                         float *__restrict__ dst)
+{
+    const MPack t = tab[blockIdx.x];
+    const int tr = 16 * t.nw;
+    for (int e = threadIdx.x; e < t.nw * 64 * 8; e += blockDim.x) {
+        const int rb = e / 512, lane = (e / 8) % 64, j = e % 8;
+        const int row = 16 * rb + lane % 16, col = 32 * t.s + 8 * (lane / 16) + j;
+        float v = 0.f;
+        if (col < t.ncol) {
+            const size_t k = (size_t)col * tr + row;
+            v = t.dtype == kF32 ? ((const float *)(buf + t.src))[k]
+                                : __uint_as_float((uint32_t)((const uint16_t *)(buf + t.src))[k] << 16);
+        }
+        dst[t.dst + ((size_t)t.s * t.nw * 64) * 8 + e] = v;
+    }
+}
+
 // ------------------------------------------------------------ host side --
 struct MrPlan {
     bool built = false;
     int kc1 = 512, cols2 = 256;    // phase-1 column chunk, phase-2 columns per item (autotuned)
     bool f32 = false;              // FP32 / BF16 factors only: the FP32-core kernels, X32 and Z in FP32
-    int p1rows = kP1Rows;          // a1^T rows per phase-1 item (kP1Rows32 on the FP32 cores)
+    int p1rows = kP1Rows;          // a1^T rows per phase-1 item (kP1RowsBf on the BF16 cores)
     int r2[3][2] = {};             // phase-2 item range per dtype
     std::vector<MItem1> it1;
     std::vector<MBlk1> blk1;
@@ -645,6 +918,15 @@ struct MrPlan {
     DevBuf<int> d_cmap;
     DevBuf<int> d_part0;
     int nzpart = 0, nparts = 0;
+    // BF16 matrix cores (FP32 / BF16 factors, split operands)
+    bool bf = false;
+    int r1b[2] = {0, 0};           // phase-1 items [0, r1b[0]) fit 4 waves (64 rows), [r1b[0], r1b[1]) 8 waves
+    std::vector<MItem2b> it2b;     // phase-2 items with their lane-order factor copies
+    std::vector<MPack> pack;       // (item, slice) blocks of the copy
+    size_t a16_floats = 0;
+    DevBuf<MItem2b> d_it2b;
+    DevBuf<MPack> d_pack;
+    DevBuf<float> a16;
 };
 
 }  // namespace hbi
@@ -726,6 +1008,11 @@ static void mr_items1(const Plan &P, MrPlan &M, int kc1)
     std::stable_sort(M.it1.begin(), M.it1.end(), [](const MItem1 &x, const MItem1 &y) {
         return (x.kt + 15) / 16 * ((x.tn + 31) / 32) > (y.kt + 15) / 16 * ((y.tn + 31) / 32);
     });
+    if (M.bf) {   // BF16 cores: items of up to 64 rows on 4 waves, the rest on 8
+        auto mid = std::stable_partition(M.it1.begin(), M.it1.end(), [](const MItem1 &x) { return x.kt <= 64; });
+        M.r1b[0] = (int)(mid - M.it1.begin());
+        M.r1b[1] = (int)M.it1.size();
+    }
     M.kc1 = kc1;
 }
 
@@ -781,12 +1068,34 @@ static void mr_items2(const hbi_hmat *h, const Plan &P, MrPlan &M, int cols2)
         HBI_REQUIRE(f.ncol <= kMaxCols32, "64-column product: item wider than the column map");
         M.it2f.push_back(f);
     }
+    if (M.bf) {
+        // lane-order copies: 16 x 32 blocks of 64 lanes x 8 floats
+        const int nw = h->tr / 16;
+        M.it2b.clear();
+        M.pack.clear();
+        uint64_t off = 0;
+        for (size_t q = 0; q < M.it2.size(); q++) {
+            const MItem2 &f = M.it2f[q];
+            M.it2b.push_back({off, f.ncol, f.cm0, f.part, 0});
+            const int nsl = (f.ncol + 31) / 32;
+            for (int sl = 0; sl < nsl; sl++) M.pack.push_back({f.off, off, f.ncol, sl, M.it2[q].dtype, nw});
+            off += (uint64_t)nsl * nw * 64 * 8;
+        }
+        M.a16_floats = off;
+    }
     M.cols2 = cols2;
 }
 
-// FP32-core mode: a plan holds either FP64 factors only or none (the
-// precision policies never mix them), so Z has one type
-static void mr_mode(const Plan &P, MrPlan &M)
+// Core selection: a plan holds either FP64 factors only or none (the
+// precision policies never mix them), so Z has one type.  FP32 / BF16
+// factors of HBI_PREC_MIXED multiply on the BF16 cores (split operands, the
+// faster form), those of HBI_PREC_F32 on the FP32 cores.  The two differ in
+// their rounding: the FP32 cores round each addition to nearest, while the
+// BF16 cores align the products of each group of 8 columns to the largest and
+// truncate the parts of the others below half an FP32 unit of it (toward
+// zero), a biased error of the same size that ensembles integrated over many
+// steps accumulate (event onsets ~1e-5 relative from FP64 against ~1e-6).
+static void mr_mode(const Plan &P, MrPlan &M, int prec)
 {
     bool has64 = false, has32 = false;
     for (const Seg &g : P.segs) (g.dtype == kF64 ? has64 : has32) = true;
@@ -795,10 +1104,11 @@ static void mr_mode(const Plan &P, MrPlan &M)
     for (const Seg &g : P.segs)
         HBI_REQUIRE(g.off % 16 == 0, "64-column product: segment not 16-byte aligned");
     M.f32 = has32;
-    M.p1rows = has32 ? kP1Rows32 : kP1Rows;
+    M.bf = has32 && prec == HBI_PREC_MIXED;
+    M.p1rows = M.bf ? kP1RowsBf : (has32 ? kP1Rows32 : kP1Rows);
 }
 
-static void mr_upload(hbi_hmat *h, MrPlan &M)
+static void mr_upload(hbi_hmat *h, const Plan &P, MrPlan &M)
 {
     M.d_it1.upload(M.it1, h->stream);
     M.d_blk1.upload(M.blk1, h->stream);
@@ -806,6 +1116,14 @@ static void mr_upload(hbi_hmat *h, MrPlan &M)
     M.d_it2f.upload(M.it2f, h->stream);
     M.d_cmap.upload(M.cmap.empty() ? std::vector<int>{0} : M.cmap, h->stream);
     M.d_part0.upload(M.part0, h->stream);
+    if (M.bf && !M.pack.empty()) {
+        M.d_it2b.upload(M.it2b, h->stream);
+        M.d_pack.upload(M.pack, h->stream);
+        if (M.a16.n < M.a16_floats) M.a16.alloc(M.a16_floats);
+        mr_pack_bf_kernel<<<(int)M.pack.size(), 256, 0, h->stream>>>(M.d_pack.p, P.buf.p, M.a16.p);
+        HBI_HIP(hipGetLastError());
+        HBI_HIP(hipStreamSynchronize(h->stream));
+    }
 }
 
 static void mr_tune(hbi_hmat *h, int prec, MrPlan &M);
@@ -818,10 +1136,10 @@ const MrPlan &mr_plan(hbi_hmat *h, int prec)
     MrCache *c = new MrCache{h, prec, MrPlan()};
     try {
         MrPlan &M = c->P;
-        mr_mode(P, M);
+        mr_mode(P, M, prec);
         mr_items1(P, M, 512);
         mr_items2(h, P, M, 256);
-        mr_upload(h, M);
+        mr_upload(h, P, M);
         M.built = true;
         mr_tune(h, prec, M);
     } catch (...) {
@@ -868,6 +1186,15 @@ static void phase1(hbi_hmat *h, const Plan &P, const MrPlan &M, const double *X,
         if (!M.blk1.empty())
             mr_zreduce_kernel<double><<<(int)M.blk1.size() * M.p1rows, kRR, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(),
                                                                                    M.p1rows, zpart, Z, gate);
+    } else if (M.bf) {
+        float *Z32 = (float *)Z;
+        // every item fits 4 waves (kP1RowsBf rows)
+        mr_bf16_kernel<4, false, kBfStages><<<(int)M.it1.size(), 256, 0, s>>>(nullptr, M.d_it1.p, nullptr, nullptr,
+                                                                            P.buf.p, X32, nullptr, zpart, P.d_zrow.p,
+                                                                            Z32, M.p1rows, gate);
+        if (!M.blk1.empty())
+            mr_zreduce_kernel<float><<<(int)M.blk1.size() * M.p1rows, kRR, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(),
+                                                                                  M.p1rows, zpart, Z32, gate);
     } else {
         float *Z32 = (float *)Z;
         mr_lrvt32_kernel<<<(int)M.it1.size(), 2 * kP1Rows32, 0, s>>>(M.d_it1.p, P.buf.p, X32, zpart, P.d_zrow.p, Z32,
@@ -883,6 +1210,17 @@ static void phase2(const hbi_hmat *h, const Plan &P, const MrPlan &M, const doub
                    const double *Z, double *ypart, hipStream_t s, const int *gate)
 {
     const float *Z32 = (const float *)Z;
+    if (M.bf) {   // every FP32 / BF16 item in one launch (the copies are FP32)
+        const int n = (int)M.it2b.size();
+        if (n == 0) return;
+        if (h->tr == 64)
+            mr_bf16_kernel<4, true, kBfStages><<<n, 256, 0, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr, X32,
+                                                                 Z32, ypart, nullptr, nullptr, 0, gate);
+        else
+            mr_bf16_kernel<2, true, kBfStages><<<n, 128, 0, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr, X32,
+                                                                 Z32, ypart, nullptr, nullptr, 0, gate);
+        return;
+    }
     for (int d = 0; d < 3; d++) {
         const int n = M.r2[d][1] - M.r2[d][0];
         if (n <= 0) continue;
@@ -940,8 +1278,8 @@ namespace hbi {
 static void mr_tune(hbi_hmat *h, int prec, MrPlan &M)
 {
     const Plan &P = hmat_plan(h, prec);
-    const int kc1s[] = {64, 128, 256, 512, 1024};
-    const int c2s[] = {128, 256, 512, 1024, 2048};
+    const int kc1s[] = {64, 128, 256, 512, 1024, 2048};
+    const int c2s[] = {128, 256, 512, 1024, 2048, 4096};
     // buffers sized for the smallest chunks (most partial slots)
     mr_items1(P, M, kc1s[0]);
     mr_items2(h, P, M, c2s[0]);
@@ -951,7 +1289,7 @@ static void mr_tune(hbi_hmat *h, int prec, MrPlan &M)
     HBI_HIP(hipMemsetAsync(dX.p, 0, dX.n * 8, h->stream));
     HBI_HIP(hipMemsetAsync(dZ.p, 0, dZ.n * 8, h->stream));
     auto time_it = [&](int which) {
-        mr_upload(h, M);
+        mr_upload(h, P, M);
         // phase 2 is timed with the reduction of its partial slots, which every
         // consumer pays (fewer, larger items: fewer slots to read)
         auto run = [&] {
@@ -988,7 +1326,7 @@ static void mr_tune(hbi_hmat *h, int prec, MrPlan &M)
     }
     mr_items1(P, M, b1);
     mr_items2(h, P, M, b2);
-    mr_upload(h, M);
+    mr_upload(h, P, M);
 }
 
 }  // namespace hbi
