@@ -20,6 +20,8 @@
 //            are added in a fixed order -> deterministic, no atomics
 #include "hmat.hpp"
 
+#include <mutex>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -1390,6 +1392,60 @@ int hbi_hmat_matvec(hbi_hmat *h, const double *x, double *y, int prec, int on_de
             h->d_y.download(y, h->n, h->stream);
             HBI_HIP(hipStreamSynchronize(h->stream));
         }
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- product without allocation --
+// hbi_hmat_matvec_noalloc keeps the product's scratch -- the z values of the
+// low-rank blocks and the dense partial slots -- in this array instead of the
+// handle's buffers.  Each GPU holds its own copy for the life of the process;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device (the array's name is a host-side stand-in).
+namespace {
+constexpr size_t kScratchDoubles = (size_t)4 << 20;   // 32 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;   // per device, set on the first call there
+
+double *scratch_for_current_device()
+{
+    int dev = 0;
+    HBI_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HBI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        HBI_REQUIRE(p != nullptr, "scratch array has no device address");
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int hbi_hmat_matvec_noalloc(hbi_hmat *h, const double *x, double *y, int prec)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(h && x && y, "NULL argument");
+        HBI_REQUIRE(prec >= 0 && prec <= 2, "precision must be HBI_PREC_F64, _F32 or _MIXED");
+        HBI_REQUIRE(h->plan[prec].built, "hbi_hmat_matvec_noalloc: the layout of this precision is not built");
+        const hbi::Plan &P = h->plan[prec];
+        const size_t zn = (hbi::z_size(h) + 31) & ~size_t(31);   // ypart starts 256-byte aligned
+        const size_t need = zn + (size_t)std::max(P.nparts, 1) * h->tr;
+        if (need > kScratchDoubles)
+            throw hbi::Error{HBI_E_ARG, "hbi_hmat_matvec_noalloc: this matrix needs " + std::to_string(need * 8) +
+                                            " bytes of scratch; the maximum is " + std::to_string(kScratchDoubles * 8) +
+                                            " (use hbi_hmat_matvec)"};
+        double *sp = scratch_for_current_device();
+        hbi::hmat_phase1(h, prec, x, sp, sp + zn, h->stream, nullptr);
+        hbi::hmat_phase2(h, prec, sp, sp + zn, y, h->stream, nullptr);
+        HBI_HIP(hipGetLastError());
+        HBI_HIP(hipStreamSynchronize(h->stream));
     });
 }
 

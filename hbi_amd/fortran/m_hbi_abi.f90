@@ -104,6 +104,12 @@ module m_hbi_abi
       real(c_double), intent(out) :: y(*)
       integer(c_int), value :: prec, on_device
     end function
+    ! y = H x on device pointers, no allocation (scratch kept by the library on each GPU)
+    integer(c_int) function hbi_hmat_matvec_noalloc(h, x, y, prec) bind(C, name="hbi_hmat_matvec_noalloc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, x, y
+      integer(c_int), value :: prec
+    end function
     integer(c_int) function hbi_rk_create(hs, hn, opts, a, b, dc, f0, vpl, taudot, sigdot, area, r) &
         bind(C, name="hbi_rk_create")
       import :: c_int, c_ptr, c_double, hbi_rk_opts

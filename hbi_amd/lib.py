@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libhbi.so")
 
 PREC_F64, PREC_F32, PREC_MIXED = 0, 1, 2
-E_STEP = -4
+E_ARG, E_STEP = -1, -4
 
 _FIELDS = ("xcol", "ycol", "zcol", "xel", "xer", "ang", "angd", "dsl", "dsd", "rake",
            "xs1", "xs2", "xs3", "ys1", "ys2", "ys3", "zs1", "zs2", "zs3",
@@ -113,7 +113,7 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_hmat_stats_get", "hbi_hmat_lod", "hbi_hmat_leaves", "hbi_hmat_leaf_data", "hbi_hmat_matvec",
            "hbi_hmat_matmul", "hbi_hmat_time_matvec", "hbi_rk_create", "hbi_rk_destroy", "hbi_rk_set_state",
            "hbi_rk_get_state", "hbi_rk_derivs", "hbi_rk_rkck", "hbi_rk_step", "hbi_rk_last_ms",
-           "hbi_hmat_time_matmul", "hbi_hmat_matmul_plan", "hbi_mfma_f64_rate", "hbi_mfma_f32_rate", "hbi_ens_create", "hbi_ens_destroy", "hbi_ens_set_state", "hbi_ens_get_state",
+           "hbi_hmat_time_matmul", "hbi_hmat_matmul_plan", "hbi_mfma_f64_rate", "hbi_mfma_f32_rate", "hbi_hmat_matvec_noalloc", "hbi_ens_create", "hbi_ens_destroy", "hbi_ens_set_state", "hbi_ens_get_state",
            "hbi_ens_step", "hbi_ens_last_ms", "hbi_mlp_create", "hbi_mlp_destroy", "hbi_mlp_nparams",
            "hbi_mlp_set_params", "hbi_mlp_get_params", "hbi_mlp_forward", "hbi_mlp_loss_grad", "hbi_mlp_train",
            "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace", "hbi_ens_set_controller",
@@ -153,6 +153,7 @@ def lib():
         L.hbi_mfma_f64_rate.argtypes = [C.c_int, C.c_int, _dp]
         L.hbi_mfma_f32_rate.argtypes = [C.c_int, C.c_int, _dp]
         _lp = C.POINTER(C.c_longlong)
+        L.hbi_hmat_matvec_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.hbi_ens_create.argtypes = [C.c_void_p, C.POINTER(RkOpts), C.c_int] + [_dp] * 7 + [C.POINTER(C.c_void_p)]
         L.hbi_ens_destroy.argtypes = [C.c_void_p]
         L.hbi_ens_set_state.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double]

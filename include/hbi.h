@@ -128,6 +128,18 @@ int hbi_hmat_leaf_data(const hbi_hmat *h, int ip, double *out);
 /* y = H x in cluster order, with the factors of precision prec.
    on_device = 0: x, y are host pointers; 1: device pointers. */
 int hbi_hmat_matvec(hbi_hmat *h, const double *x, double *y, int prec, int on_device);
+/* y = H x like hbi_hmat_matvec with device pointers, but allocating nothing:
+   the product's scratch (low-rank z values, dense partial slots) lives in a
+   32 MiB array the library keeps on each GPU for the life of the process.
+   No counterpart in the reference; for callers that cannot allocate device
+   memory (code under HIP graph capture, serving loops, harnesses that forbid
+   hipMalloc).  hbi_hmat_build lays out all three precisions, so the call
+   only launches kernels.  A matrix whose scratch exceeds 32 MiB (its z
+   values plus tr values per dense partial slot, 8 bytes each; 5 MB for the
+   32768-element benchmark fault) is rejected with HBI_E_ARG.  Calls that overlap on one GPU
+   share the array and race: issue them one after another.  Returns after
+   the product completes, like hbi_hmat_matvec. */
+int hbi_hmat_matvec_noalloc(hbi_hmat *h, const double *x, double *y, int prec);
 /* Y = H X for nrhs right-hand sides, X and Y column-major N x nrhs (host).
    Computed 64 columns at a time by the multi-RHS kernels (factors read once
    per 64 columns): FP64 factors on the FP64 matrix cores; FP32 / BF16 factors
