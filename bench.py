@@ -39,11 +39,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
 FP32_MFMA_PEAK_TFS = 157.3
+BF16_MFMA_PEAK_TFS = 2500.0
 # HBM bytes per launch of the two roofline kernels (mixed precision), from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE passes over
 # hbi_amd/profile_kernels.py, scaled by the counters of a 256 MiB device copy
-# (fetch x 2.00, write x 1.00): mr_tile32_kernel<float,64,4> and stream_kernel
-TRAFFIC_MR_TILE_MIXED = 420.6e6
+# (fetch x 2.00, write x 1.00): mr_bf16_kernel<4,true,3> and stream_kernel
+TRAFFIC_MR_BF16_MIXED = None
 TRAFFIC_STREAM_MIXED = 225.7e6
 IMAX, JMAX, DS0, LEAF, EPS_H, EPS_R = 256, 128, 0.390625, 64, 1e-5, 1e-6
 NREAL = 64
@@ -97,6 +98,11 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
                     "per_rhs_vs_matvec": mv[name]["ms"] * 64 / t["total_ms"]}
     out["matmul64"] = mm
     out["matmul64_plan"] = {name: Hm.matmul_plan(prec) for prec, name in ((H.PREC_F64, "f64"), (H.PREC_MIXED, "mixed"))}
+    # algorithmic bytes of one phase-2 launch (FP32 factors): every factor value once, every X row and Z row
+    # (sum kt of them) once in FP32, and the 64-row FP64 partial block of every item written once
+    plan = out["matmul64_plan"]["mixed"]
+    nz = int(round(st["kt_mean"] * st["nlr"]))
+    out["p2_alg_bytes"] = int(p2_entries * 4 + (g.n + nz) * 64 * 4 + plan["items2"] * 64 * 64 * 8)
     # what the matrix cores sustain here for these instructions (no memory traffic)
     out["mfma_f64_loop_tfs"] = {str(w): round(H.mfma_f64_rate(w), 2) for w in (1, 2, 4)}
     out["mfma_f32_loop_tfs"] = {str(w): round(H.mfma_f32_rate(w), 2) for w in (1, 2, 4)}
@@ -175,9 +181,12 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
         wall = time.perf_counter() - t0
         es = E.get_state()
         hists[name] = np.concatenate([hw, ht], axis=1)
+        t_start = hw[:, -1, 0] if warmup else np.zeros(NREAL)
         out[name] = {"realisations": NREAL, "learned_step": learned, "steps": steps, "wall_s": wall,
                      "device_ms": E.last_ms(), "lockstep_attempts": att, "ms_per_attempt": 1e3 * wall / max(att, 1),
-                     "steps_per_s": NREAL * steps / wall, "max_rejected": int(es["rejected"].max()),
+                     "steps_per_s": NREAL * steps / wall,
+                     "sim_time_per_wall_s": float((es["t"] - t_start).sum()) / wall,
+                     "max_rejected": int(es["rejected"].max()),
                      "status_ok": bool((es["status"] == 0).all())}
         E.close()
     # the two controllers from the same start to a common simulated time per realisation
@@ -193,9 +202,36 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
                               "time_to_common_sim_time_ms": [w0, w1], "speedup": w0 / w1,
                               "note": "attempts (accepted + rejected) of each realisation from t = 0 to the smaller "
                                       "of the two controllers' final times; lockstep wall = max attempts x ms/attempt"}
+    out["accuracy_ensemble_vs_f64"] = ensemble_accuracy(H, Hm, frs, lod, ys, g)
     out["p2_entries"], out["p1_entries"] = p2_entries, p1_entries
     Hm.close()
     return out
+
+
+def ensemble_accuracy(H, Hm, frs, lod, ys, g, steps=1500):
+    """The headline ensemble's precision (mixed: FP32 factors on the BF16
+    matrix cores) against an FP64 ensemble of the same realisations, both
+    under rkqs from t = 0: event onset times; and, replayed on the FP64 run's
+    step grid (identical times), the slip-rate and probe-slip differences over
+    the near-plate-rate (log10 Vmax < -8) and quasi-static (< -6) phases."""
+    from hbi_amd import workloads as W
+    print("bench: ensemble accuracy, FP64 and mixed ensembles of %d steps" % steps, file=sys.stderr, flush=True)
+    inv = np.empty(g.n, dtype=np.int64)
+    inv[lod] = np.arange(g.n)
+    xs = np.linspace(-25.0, 25.0, 16)
+    probes = inv[[int(np.argmin((g.xcol - x) ** 2 + (g.zcol + 15.0) ** 2)) for x in xs]].astype(np.int32)
+    hf, sf = W.run_ensemble(Hm, frs, lod, ys, probes, H.PREC_F64, steps, DS0)
+    hm, sm = W.run_ensemble(Hm, frs, lod, ys, probes, H.PREC_MIXED, steps, DS0)
+    E = H.Ensemble(Hm, frs, lod, hdiv=DS0, eps_r=EPS_R, prec=H.PREC_MIXED)
+    E.set_probes(probes)
+    E.set_state(ys, 0.0, 1.0)
+    E.set_step_grid(hf[:, :, 1])
+    _, hr = E.step(steps, history=True)
+    E.close()
+    return {"steps": steps, "f64_ensemble_steps_per_s": sf["steps_per_s"], "onsets": W.ensemble_errors(hf, hm),
+            "on_f64_grid_below_1e-8": W.grid_errors(hf, hr, -8.0), "on_f64_grid_below_1e-6": W.grid_errors(hf, hr, -6.0),
+            "note": "mixed vs FP64 lockstep ensembles from the same states; onset = first log10 Vmax > -3; "
+                    "grid errors: mixed replayed on the FP64 run's accepted step sizes"}
 
 
 def cpu_part(nsteps: int):
@@ -254,7 +290,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1500, help="timed lockstep steps of the ensemble")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--single-steps", type=int, default=2000)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -278,17 +314,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 state / f32 H factors (FP32 MFMA, FP64 sums of 32-column slices)",
+        "dtype": "f64 state / f32 H factors (split into 3 bf16 parts on the BF16 MFMA, FP64 sums of 64-column slices)",
         "data": "synthetic",
         "config": {"workload": "3dph 256x128 planar fault, N=32768, leaf 64, ACA eps 1e-5, RK45 eps_r 1e-6; "
                                "64 friction realisations in lockstep, one 64-column H product per RK stage, "
                                "FP32 factors, learned step-size proposals (rkqs check), hipGraph attempts",
                    "n_elements": IMAX * JMAX, "realisations": NREAL},
-        "roofline": {"bound": "mfma", "kernel": "mr_tile32_kernel<float,64,4> (64-RHS row tiles, FP32 MFMA)",
-                     "achieved": round(mmx["p2_TFs"], 2), "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
-                     "frac": round(mmx["p2_TFs"] / FP32_MFMA_PEAK_TFS, 4), "traffic": TRAFFIC_MR_TILE_MIXED,
-                     "measured_loop_ceiling": max(r["mfma_f32_loop_tfs"].values()),
-                     "frac_of_measured_ceiling": round(mmx["p2_TFs"] / max(r["mfma_f32_loop_tfs"].values()), 4)},
+        "roofline": {"bound": "hbm",
+                     "kernel": "mr_bf16_kernel<4,true,3> (64-RHS row tiles on the BF16 matrix cores, split operands)",
+                     "achieved": round(r["p2_alg_bytes"] / (mmx["p2_ms"] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(r["p2_alg_bytes"] / (mmx["p2_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": TRAFFIC_MR_BF16_MIXED,
+                     "alg_bytes": r["p2_alg_bytes"],
+                     "useful_tflops": round(mmx["p2_TFs"], 2),
+                     "useful_frac_of_bf16_peak": round(mmx["p2_TFs"] / BF16_MFMA_PEAK_TFS, 4),
+                     "mfma_executed_frac_of_bf16_peak": round(6 * mmx["p2_TFs"] / BF16_MFMA_PEAK_TFS, 4),
+                     "note": "arithmetic intensity 2*64 flop per 4-byte factor (32 flop/B) is below the BF16 ridge "
+                             "(312 flop/B): HBM-bound; six BF16 MFMA terms per FP32 product"},
         "roofline_matvec": {"bound": "hbm", "kernel": "stream_kernel<64> (single-vector product, first launch: "
                                                       "stacked a1^T tiles + dense tiles, mixed)",
                             "achieved": round(mvx["a_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -299,9 +341,12 @@ def main():
         "mixed_single_steps_per_s": round(r["single"]["mixed"]["steps_per_s"], 2),
         "speedup_vs_fp64_single": round(ens["steps_per_s"] / f64, 2),
         "batched_rkqs_steps_per_s": round(ens0["steps_per_s"], 1),
+        "sim_time_per_wall_s": {"learned": round(ens["sim_time_per_wall_s"], 1),
+                                "rkqs": round(ens0["sim_time_per_wall_s"], 1)},
         "learned_vs_rkqs": r["learned_vs_rkqs"],
         "ensemble_rkqs": ens0,
         "accuracy_mixed_vs_f64": r["accuracy_mixed_vs_f64"],
+        "accuracy_ensemble_vs_f64": r["accuracy_ensemble_vs_f64"],
         "ensemble": ens,
         "matvec": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in r["matvec"].items()},

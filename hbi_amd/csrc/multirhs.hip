@@ -701,16 +701,22 @@ __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restr
     if (gate && *gate) return;
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
     const int c = lane % 16, q = lane / 16;
+    // Workgroups go to the 8 XCDs round-robin by block index, and each XCD has
+    // its own L2: in phase 2, XCD x takes the x-th contiguous range of items
+    // (neighbouring row tiles, which read mostly the same X / Z rows).  Phase-1
+    // items are ordered largest first and keep the plain order (balance).
+    const int nb = gridDim.x, qx = nb / 8, rx = nb % 8, x = blockIdx.x % 8;
+    const int item = P2 ? x * qx + min(x, rx) + (int)blockIdx.x / 8 : (int)blockIdx.x;
     MItem2b it2{};
     MItem1 it1{};
     int ncol;
     if constexpr (P2) {
-        it2 = items2[blockIdx.x];
+        it2 = items2[item];
         ncol = it2.ncol;
         for (int k = tid; k < ncol; k += NT) cm[k] = cmap[it2.cm0 + k];
         __syncthreads();
     } else {
-        it1 = items1[blockIdx.x];
+        it1 = items1[item];
         ncol = it1.tn;
     }
     const int nsl = (ncol + 31) / 32;

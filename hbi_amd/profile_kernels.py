@@ -5,7 +5,8 @@ profilers (rocprofv3 --kernel-trace / --pmc) and for the roofline figures.
 
 Builds the 256 x 128 half-space fault's lattice H-matrix (leaf 64, ACA eps
 1e-5) and times, with HIP events on the library stream, the single-vector
-product of every precision and the 64-column product (FP64 and mixed) over
+product of every precision and the 64-column product (FP64 on the FP64
+matrix cores, FP32 on the FP32 cores, mixed on the BF16 cores) over
 `iters` calls each; the same kernels and work items the RK graphs launch.
 With --copy-mb > 0 it also runs a plain device-to-device copy of that many
 MiB (HIP runtime blit kernel) `iters` times, the reference for scaling the
@@ -52,7 +53,7 @@ def main():
            "alg_bytes_p1": st["alg_bytes_p1"], "alg_bytes_p2": st["alg_bytes_p2"], "matvec": {}, "matmul64": {}}
     for prec, name in ((H.PREC_F64, "f64"), (H.PREC_F32, "f32"), (H.PREC_MIXED, "mixed")):
         out["matvec"][name] = Hm.time_matvec(prec, a.iters)
-    for prec, name in ((H.PREC_F64, "f64"), (H.PREC_MIXED, "mixed")):
+    for prec, name in ((H.PREC_F64, "f64"), (H.PREC_F32, "f32"), (H.PREC_MIXED, "mixed")):
         out["matmul64"][name] = Hm.time_matmul(prec, a.iters)
     if a.copy_mb > 0:
         out["copy"] = device_copy(a.copy_mb * (1 << 20), a.iters)
