@@ -169,7 +169,9 @@ int hbi_mfma_f32_rate(int waves_per_simd, int iters, double *tflops);
    on device-resident vectors, from HIP events on the library stream:
    ms[0] = first launch (a1^T x of the stacked low-rank row tiles, and the
    dense blocks' row tiles), ms[1] = second launch (the low-rank blocks' a2 z
-   per row tile, adding the tile's dense partials into y), ms[2] = whole product. */
+   per row tile, adding the tile's dense partials into y), each bracketed by
+   events; ms[2] = whole product, from iters products launched back to back
+   between two events. */
 int hbi_hmat_time_matvec(hbi_hmat *h, int prec, int iters, double *ms);
 
 /* Friction / RK45 options (main_LH.f90 variables) */
