@@ -851,34 +851,63 @@ int hbi_rk_step_many(hbi_rk *const *rs, int m, int nsteps, hbi_rk_info *infos)
         for (int f = 0; f < m; f++)
             for (int g = f + 1; g < m; g++) HBI_REQUIRE(rs[f] != rs[g], "a run appears twice");
         for (int f = 0; f < m; f++) hbi::step_prepare(rs[f], nsteps, false, 0, nullptr);
-        std::vector<hbi::RkCtl> c(m);
-        std::vector<char> fin(m, 0);
-        std::vector<long long> launched(m, 0);
-        const long long budget = 50LL * nsteps + 100;
-        int batch = nsteps;
-        for (;;) {
-            // interleaved over the runs' streams: the device overlaps the runs
-            for (int q = 0; q < batch; q++)
+        for (int f = 0; f < m; f++) HBI_HIP(hipStreamSynchronize(rs[f]->stream));
+        // One graph for one attempt of every run: a branch per run (its own
+        // attempt graph's kernels, on its stream), forked from and joined to a
+        // launcher stream, so each lockstep round is a single graph launch and
+        // the device runs the faults side by side.  A run that reached its
+        // target is gated off inside its kernels (its control block's done flag).
+        hipStream_t s0 = nullptr;
+        hipEvent_t fork = nullptr;
+        std::vector<hipEvent_t> join(m, nullptr);
+        hipGraphExec_t all = nullptr;
+        auto cleanup = [&] {
+            if (all) (void)hipGraphExecDestroy(all);
+            for (hipEvent_t e : join)
+                if (e) (void)hipEventDestroy(e);
+            if (fork) (void)hipEventDestroy(fork);
+            if (s0) (void)hipStreamDestroy(s0);
+        };
+        try {
+            HBI_HIP(hipStreamCreateWithFlags(&s0, hipStreamNonBlocking));
+            HBI_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+            for (int f = 0; f < m; f++) HBI_HIP(hipEventCreateWithFlags(&join[f], hipEventDisableTiming));
+            hipGraph_t gr;
+            HBI_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+            HBI_HIP(hipEventRecord(fork, s0));
+            for (int f = 0; f < m; f++) {
+                HBI_HIP(hipStreamWaitEvent(rs[f]->stream, fork, 0));
+                hbi::launch_attempt(rs[f]);
+                HBI_HIP(hipEventRecord(join[f], rs[f]->stream));
+                HBI_HIP(hipStreamWaitEvent(s0, join[f], 0));
+            }
+            HBI_HIP(hipStreamEndCapture(s0, &gr));
+            HBI_HIP(hipGraphInstantiate(&all, gr, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(gr);
+            std::vector<hbi::RkCtl> c(m);
+            long long launched = 0;
+            const long long budget = 50LL * nsteps + 100;
+            int batch = nsteps;
+            for (;;) {
+                for (int q = 0; q < batch; q++) HBI_HIP(hipGraphLaunch(all, s0));
+                launched += batch;
                 for (int f = 0; f < m; f++)
-                    if (!fin[f] && launched[f] + q < budget) HBI_HIP(hipGraphLaunch(rs[f]->graph, rs[f]->stream));
-            for (int f = 0; f < m; f++) {
-                if (fin[f]) continue;
-                launched[f] += batch;
-                HBI_HIP(hipMemcpyAsync(&c[f], rs[f]->ctl.p, sizeof(hbi::RkCtl), hipMemcpyDeviceToHost, rs[f]->stream));
+                    HBI_HIP(hipMemcpyAsync(&c[f], rs[f]->ctl.p, sizeof(hbi::RkCtl), hipMemcpyDeviceToHost, s0));
+                HBI_HIP(hipStreamSynchronize(s0));
+                long long need = 0;
+                for (int f = 0; f < m; f++)
+                    if (!c[f].done) need = std::max(need, c[f].target - c[f].accepted + 2);
+                if (need == 0 || launched >= budget || nsteps == 0) break;
+                batch = (int)std::max<long long>(1, std::min<long long>(need, budget - launched));
             }
-            long long need = 0;
-            for (int f = 0; f < m; f++) {
-                if (fin[f]) continue;
-                HBI_HIP(hipStreamSynchronize(rs[f]->stream));
-                if (c[f].done || launched[f] >= budget || nsteps == 0) fin[f] = 1;
-                else need = std::max(need, std::min<long long>(c[f].target - c[f].accepted + 2, budget - launched[f]));
-            }
-            if (need == 0) break;
-            batch = (int)std::max<long long>(1, need);
+            for (int f = 0; f < m; f++) hbi::step_finish(rs[f], c[f], infos ? &infos[f] : nullptr);
+            for (int f = 0; f < m; f++)
+                if (c[f].status == HBI_E_STEP) throw hbi::Error{HBI_E_STEP, "Runge-Kutta step size underflow"};
+        } catch (...) {
+            cleanup();
+            throw;
         }
-        for (int f = 0; f < m; f++) hbi::step_finish(rs[f], c[f], infos ? &infos[f] : nullptr);
-        for (int f = 0; f < m; f++)
-            if (c[f].status == HBI_E_STEP) throw hbi::Error{HBI_E_STEP, "Runge-Kutta step size underflow"};
+        cleanup();
     });
 }
 

@@ -217,11 +217,13 @@ int hbi_rk_rkck(hbi_rk *r, const double *y, double h, double *yout, double *yerr
 int hbi_rk_step(hbi_rk *r, int nsteps, double *hist, int nrec, const int *rec, double *vhist,
                 hbi_rk_info *info);
 /* Advance m independent runs (e.g. distinct faults, each with its own
-   H-matrix and step control) by nsteps accepted steps each: their attempt
-   graphs are launched interleaved on their streams, so the device runs them
-   concurrently; each run's result is the one hbi_rk_step gives.  infos (may
-   be NULL): m entries.  No reference counterpart (the reference runs one
-   fault per job). */
+   H-matrix and step control) by nsteps accepted steps each: one attempt of
+   every run is captured into one HIP graph (a branch per run on its own
+   stream), so each lockstep round is a single graph launch and the device
+   runs the faults concurrently; a run that has reached its target is gated
+   off on the device.  Each run's result is bit-identical to the one
+   hbi_rk_step gives.  infos (may be NULL): m entries.  No reference
+   counterpart (the reference runs one fault per job). */
 int hbi_rk_step_many(hbi_rk *const *rs, int m, int nsteps, hbi_rk_info *infos);
 /* Device time of the last hbi_rk_step call, milliseconds (HIP events) */
 int hbi_rk_last_ms(const hbi_rk *r, double *ms);
