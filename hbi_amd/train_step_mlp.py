@@ -4,10 +4,14 @@
         [--seeds N] [--steps K] [--rounds R] [--epochs E]
 
 c1: the 2-D antiplane fault (2048 elements, 40 km, leaf 32, ACA eps 1e-6),
-    single runs of `seeds` perturbed realisations (default 8 x 8000 steps);
+    `seeds` perturbed realisations (default 44 x 8000 steps) advanced as one
+    lockstep ensemble, FP64;
 c2: the 32768-element 3-D planar fault (256 x 128, leaf 64, ACA eps 1e-5,
-    FP32 factors), `seeds` realisations of its lognormal friction (default
-    64) advanced as one lockstep ensemble (default 1500 steps);
+    mixed precision), `seeds` realisations of its lognormal friction
+    (default 192: seeds 1-64, 201-264, 301-364, clear of the held-out
+    101-164) advanced 64 at a time as lockstep ensembles (default 1750
+    steps);
+the defaults give about 1e6 (features, step) pairs over three rounds;
 RK eps_r 1e-6 for both.  Round 1 records traces of accepted steps under the
 reference's controller (rkqs); every further round adds traces recorded under
 the model of the previous round (the states the learned controller itself
@@ -32,11 +36,15 @@ C2_OUT = os.path.join(DATA, "step_mlp_c2.npz")
 C2 = dict(imax=256, jmax=128, ds0=0.390625, leaf=64, eps=1e-5)
 
 
-def c2_setup(seeds, first_seed=1):
-    """The 3-D fault, its H-matrix and `seeds` friction realisations with
-    perturbed initial states (seeds first_seed ...)."""
-    g = P.mesh_3dph(C2["imax"], C2["jmax"], C2["ds0"], 90.0)
-    Hm = H.HMatrix(g, leaf=C2["leaf"], eps=C2["eps"], bf16_margin=1.0)
+def c2_setup(seeds, first_seed=1, hmat=None):
+    """The 3-D fault, its H-matrix (built, or hmat = (g, Hm) reused) and
+    `seeds` friction realisations with perturbed initial states (seeds
+    first_seed ...)."""
+    if hmat is None:
+        g = P.mesh_3dph(C2["imax"], C2["jmax"], C2["ds0"], 90.0)
+        Hm = H.HMatrix(g, leaf=C2["leaf"], eps=C2["eps"], bf16_margin=1.0)
+    else:
+        g, Hm = hmat
     lod = Hm.lod
     frs, ys = [], []
     for s in range(first_seed, first_seed + seeds):
@@ -50,13 +58,24 @@ def c2_setup(seeds, first_seed=1):
     return g, Hm, lod, frs, np.array(ys)
 
 
-def train_c2(seeds=64, steps=1500, rounds=3, epochs=40, log=print):
-    g, Hm, lod, frs, ys = c2_setup(seeds)
+C2_TRAIN_FIRST_SEEDS = (1, 201, 301)
+
+
+def train_c2(seeds=192, steps=1750, rounds=3, epochs=40, log=print):
+    g, Hm, lod, frs, ys = c2_setup(min(seeds, 64))
+    batches = [(frs, ys)]
+    for f0 in C2_TRAIN_FIRST_SEEDS[1:]:
+        k = min(64, seeds - 64 * len(batches))
+        if k <= 0:
+            break
+        _, _, _, fb, yb = c2_setup(k, first_seed=f0, hmat=(g, Hm))
+        batches.append((fb, yb))
     traces = []
     m = None
     t0 = time.time()
     for r in range(rounds):
-        traces += L.ensemble_trace_run(Hm, g, frs, lod, ys, steps, hdiv=C2["ds0"], prec=H.PREC_MIXED, mlp=m)
+        for fb, yb in batches:
+            traces += L.ensemble_trace_run(Hm, g, fb, lod, yb, steps, hdiv=C2["ds0"], prec=H.PREC_MIXED, mlp=m)
         X, T = L.dataset(traces)
         m, loss = L.fit(X, T, epochs=epochs)
         log("round %d: %d samples, loss %.3e (target variance %.3e), %.1f s" % (r + 1, len(X), loss[-1], T.var(),
@@ -64,19 +83,18 @@ def train_c2(seeds=64, steps=1500, rounds=3, epochs=40, log=print):
     return m
 
 
-def train(seeds=8, steps=8000, rounds=3, epochs=40, n=2048, log=print):
+def train(seeds=44, steps=8000, rounds=3, epochs=40, n=2048, log=print):
     g = P.mesh_2dvs(n, 40.0 / n)
     Hm = H.HMatrix(g, leaf=32, eps=1e-6)
     lod = Hm.lod
-    setups = []
-    for s in range(1, seeds + 1):
-        fr = L.c1_realisation(g, s)
-        setups.append((fr, L.initial_state(g, fr, lod, s)))
+    frs = [L.c1_realisation(g, s) for s in range(1, seeds + 1)]
+    ys = np.array([L.initial_state(g, fr, lod, s) for s, fr in zip(range(1, seeds + 1), frs)])
     traces = []
     m = None
     t0 = time.time()
     for r in range(rounds):
-        traces += [L.trace_run(Hm, g, fr, lod, y0, steps, mlp=m) for fr, y0 in setups]
+        for b in range(0, seeds, 64):
+            traces += L.ensemble_trace_run(Hm, g, frs[b:b + 64], lod, ys[b:b + 64], steps, mlp=m)
         X, T = L.dataset(traces)
         m, loss = L.fit(X, T, epochs=epochs)
         log("round %d: %d samples, loss %.3e (target variance %.3e), %.1f s" % (r + 1, len(X), loss[-1], T.var(),
@@ -96,10 +114,10 @@ def main():
     log = lambda s: print(s, flush=True)
     if a.problem == "c1":
         out = a.out or DEFAULT_OUT
-        m = train(a.seeds or 8, a.steps or 8000, a.rounds, a.epochs, log=log)
+        m = train(a.seeds or 44, a.steps or 8000, a.rounds, a.epochs, log=log)
     else:
         out = a.out or C2_OUT
-        m = train_c2(a.seeds or 64, a.steps or 1500, a.rounds, a.epochs, log=log)
+        m = train_c2(a.seeds or 192, a.steps or 1750, a.rounds, a.epochs, log=log)
     a.out = out
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     L.save(m, a.out)

@@ -145,10 +145,11 @@ def test_gpu_training_pipeline_short():
 
 
 def test_gpu_learned_step_gain_on_held_out_seeds():
-    """The committed model (python -m hbi_amd.train_step_mlp: 8 seeds x 8000
-    steps, 3 rounds) against rkqs on three realisations it was not trained
-    on, 2048 elements, 8000 rkqs steps (several seismic cycles): RK attempts
-    to reach the same simulated time, event onset times."""
+    """The committed model (python -m hbi_amd.train_step_mlp: 44 seeds x 8000
+    steps, 3 rounds, 1.06e6 pairs) against rkqs on three realisations it was
+    not trained on, 2048 elements, 8000 rkqs steps (several seismic cycles):
+    RK attempts to reach the same simulated time (measured 1.07, 1.10, 1.10
+    times fewer), event onset times."""
     from hbi_amd import train_step_mlp as TS
     m = L.load(TS.DEFAULT_OUT)
     g = P.mesh_2dvs(2048, 40.0 / 2048)
@@ -210,8 +211,8 @@ def test_gpu_ensemble_traces_and_learned_steps_match_single_runs():
 
 def test_gpu_learned_step_gain_on_the_3d_fault_ensemble():
     """The committed 3-D model (python -m hbi_amd.train_step_mlp --problem c2:
-    64 realisations x 1500 lockstep steps, 3 rounds) on 64 realisations it
-    was not trained on (seeds 101-164), mixed precision, in one ensemble: RK
+    192 realisations x 1750 lockstep steps, 3 rounds, 1.01e6 pairs) on 64
+    realisations it was not trained on (seeds 101-164), mixed precision, in one ensemble: RK
     attempts of each realisation to reach the simulated time of 1500 rkqs
     steps, and the lockstep attempts until every realisation got there."""
     from hbi_amd import train_step_mlp as TS
