@@ -202,10 +202,57 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
                               "time_to_common_sim_time_ms": [w0, w1], "speedup": w0 / w1,
                               "note": "attempts (accepted + rejected) of each realisation from t = 0 to the smaller "
                                       "of the two controllers' final times; lockstep wall = max attempts x ms/attempt"}
+    out["controllers_by_phase"] = controllers_by_phase(H, Hm, frs, lod, ys, mlp, warmup, steps)
     out["accuracy_ensemble_vs_f64"] = ensemble_accuracy(H, Hm, frs, lod, ys, g)
     out["p2_entries"], out["p1_entries"] = p2_entries, p1_entries
     Hm.close()
     return out
+
+
+PHASES = ((300, "nucleation"), (1000, "first event"), (2000, "interseismic"), (3000, "second event"))
+
+
+def controllers_by_phase(H, Hm, frs, lod, ys, mlp, warmup, steps):
+    """The two controllers over the cycle: an rkqs ensemble from t = 0 is
+    spun up to each of the PHASES' lockstep step counts (first event
+    nucleating at ~300, rupturing at ~600-1200, interseismic by 2000, second
+    event at 3000); from that state (per-realisation t and next step) each
+    controller takes `warmup` untimed and `steps` timed lockstep steps:
+    accepted steps/s, lockstep attempts and simulated time per wall-second."""
+    S = H.Ensemble(Hm, frs, lod, hdiv=DS0, eps_r=EPS_R, prec=H.PREC_MIXED)
+    S.set_state(ys, 0.0, 1.0)
+    done, rows = 0, []
+    for spin, name in PHASES:
+        S.step(spin - done)
+        done = spin
+        st = S.get_state()
+        t, h = S.get_clock()
+        print("bench: controllers from the %s state (%d steps)" % (name, spin), file=sys.stderr, flush=True)
+        row = {"phase": name, "spin_steps": spin, "t_median_s": float(np.median(t)),
+               "log10_vmax_median": float(np.median(np.log10(np.abs(st["vel"]).max(axis=1))))}
+        for key, learned in (("rkqs", False), ("learned", True)):
+            E = H.Ensemble(Hm, frs, lod, hdiv=DS0, eps_r=EPS_R, prec=H.PREC_MIXED)
+            if learned:
+                E.set_controller(1, mlp, 0.9)
+            E.set_state(st["y"], 0.0, 1.0)
+            E.set_clock(t, h)
+            if warmup:
+                E.step(warmup)
+            t1 = E.get_clock()[0]
+            w0 = time.perf_counter()
+            att = E.step(steps)
+            wall = time.perf_counter() - w0
+            es = E.get_state()
+            row[key] = {"lockstep_attempts": int(att), "steps_per_s": round(NREAL * steps / wall, 1),
+                        "sim_time_per_wall_s": float((es["t"] - t1).sum()) / wall,
+                        "status_ok": bool((es["status"] == 0).all())}
+            E.close()
+        rows.append(row)
+    S.close()
+    mean = {k: float(np.mean([r[k]["steps_per_s"] for r in rows])) for k in ("rkqs", "learned")}
+    return {"phases": rows, "mean_steps_per_s": mean,
+            "note": "both controllers from the same spun-up states (rkqs spin-up from t = 0); the headline window "
+                    "starts at t = 0, where rkqs grows h by 2x per step without rejections"}
 
 
 def ensemble_accuracy(H, Hm, frs, lod, ys, g, steps=1500):
@@ -344,6 +391,7 @@ def main():
         "sim_time_per_wall_s": {"learned": round(ens["sim_time_per_wall_s"], 1),
                                 "rkqs": round(ens0["sim_time_per_wall_s"], 1)},
         "learned_vs_rkqs": r["learned_vs_rkqs"],
+        "controllers_by_phase": r["controllers_by_phase"],
         "ensemble_rkqs": ens0,
         "accuracy_mixed_vs_f64": r["accuracy_mixed_vs_f64"],
         "accuracy_ensemble_vs_f64": r["accuracy_ensemble_vs_f64"],

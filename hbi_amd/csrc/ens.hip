@@ -677,6 +677,30 @@ int hbi_ens_set_state(hbi_ens *E, const double *y, double t, double dt_next)
     });
 }
 
+int hbi_ens_set_clock(hbi_ens *E, const double *t, const double *dt_next)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(E && t && dt_next, "NULL argument");
+        for (int r = 0; r < E->nreal; r++)
+            HBI_REQUIRE(std::isfinite(t[r]) && std::isfinite(dt_next[r]) && dt_next[r] > 0.0,
+                        "t must be finite and dt_next finite and > 0");
+        hbi::EnsCtl c;
+        HBI_HIP(hipMemcpy(&c, E->ctl.p, sizeof(c), hipMemcpyDeviceToHost));
+        for (int r = 0; r < E->nreal; r++) { c.t[r] = t[r]; c.h[r] = dt_next[r]; }
+        hbi::to_device_sync(E->ctl.p, &c, sizeof(c), E->stream);
+    });
+}
+
+int hbi_ens_get_clock(hbi_ens *E, double *t, double *dt_next)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(E && t && dt_next, "NULL argument");
+        hbi::EnsCtl c;
+        HBI_HIP(hipMemcpy(&c, E->ctl.p, sizeof(c), hipMemcpyDeviceToHost));
+        for (int r = 0; r < E->nreal; r++) { t[r] = c.t[r]; dt_next[r] = c.h[r]; }
+    });
+}
+
 int hbi_ens_get_state(hbi_ens *E, double *y, double *slip, double *vel, double *t, long long *accepted,
                       long long *rejected, int *status)
 {

@@ -1375,7 +1375,7 @@ extern "C" int hbi_hmat_matmul_plan(hbi_hmat *h, int prec, int *info)
         info[2] = (int)M.it1.size();
         info[3] = (int)M.it2.size();
         info[4] = M.nzpart;
-        info[5] = M.f32 ? 1 : 0;
+        info[5] = M.bf ? 2 : (M.f32 ? 1 : 0);
     });
 }
 

@@ -206,6 +206,16 @@ module m_hbi_abi
       real(c_double), intent(in) :: y(*)
       real(c_double), value :: t, dt
     end function
+    integer(c_int) function hbi_ens_set_clock(e, t, dt_next) bind(C, name="hbi_ens_set_clock")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double), intent(in) :: t(*), dt_next(*)
+    end function
+    integer(c_int) function hbi_ens_get_clock(e, t, dt_next) bind(C, name="hbi_ens_get_clock")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double), intent(out) :: t(*), dt_next(*)
+    end function
     integer(c_int) function hbi_ens_get_state(e, y, slip, vel, t, accepted, rejected, status) &
         bind(C, name="hbi_ens_get_state")
       import :: c_int, c_ptr, c_double, c_long_long

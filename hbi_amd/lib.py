@@ -113,7 +113,7 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_hmat_stats_get", "hbi_hmat_lod", "hbi_hmat_leaves", "hbi_hmat_leaf_data", "hbi_hmat_matvec",
            "hbi_hmat_matmul", "hbi_hmat_time_matvec", "hbi_rk_create", "hbi_rk_destroy", "hbi_rk_set_state",
            "hbi_rk_get_state", "hbi_rk_derivs", "hbi_rk_rkck", "hbi_rk_step", "hbi_rk_last_ms",
-           "hbi_hmat_time_matmul", "hbi_hmat_matmul_plan", "hbi_mfma_f64_rate", "hbi_mfma_f32_rate", "hbi_hmat_matvec_noalloc", "hbi_ens_create", "hbi_ens_destroy", "hbi_ens_set_state", "hbi_ens_get_state",
+           "hbi_hmat_time_matmul", "hbi_hmat_matmul_plan", "hbi_mfma_f64_rate", "hbi_mfma_f32_rate", "hbi_hmat_matvec_noalloc", "hbi_ens_create", "hbi_ens_destroy", "hbi_ens_set_state", "hbi_ens_get_state", "hbi_ens_set_clock", "hbi_ens_get_clock",
            "hbi_ens_step", "hbi_ens_last_ms", "hbi_mlp_create", "hbi_mlp_destroy", "hbi_mlp_nparams",
            "hbi_mlp_set_params", "hbi_mlp_get_params", "hbi_mlp_forward", "hbi_mlp_loss_grad", "hbi_mlp_train",
            "hbi_rk_set_controller", "hbi_rk_set_trace", "hbi_rk_get_trace", "hbi_ens_set_controller",
@@ -158,6 +158,8 @@ def lib():
         L.hbi_ens_destroy.argtypes = [C.c_void_p]
         L.hbi_ens_set_state.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double]
         L.hbi_ens_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _lp, _lp, _ip]
+        L.hbi_ens_set_clock.argtypes = [C.c_void_p, _dp, _dp]
+        L.hbi_ens_get_clock.argtypes = [C.c_void_p, _dp, _dp]
         L.hbi_ens_step.argtypes = [C.c_void_p, C.c_int, _dp, _lp]
         L.hbi_ens_last_ms.argtypes = [C.c_void_p, _dp]
         L.hbi_ens_set_probes.argtypes = [C.c_void_p, C.c_int, _ip]
@@ -317,11 +319,11 @@ class HMatrix:
         return {"p1_ms": ms[0], "p2_ms": ms[1], "total_ms": ms[2]}
 
     def matmul_plan(self, prec=PREC_F64) -> dict:
-        """The autotuned 64-column plan: chunk sizes, item counts, FP32 cores."""
+        """The autotuned 64-column plan: chunk sizes, item counts, matrix cores."""
         info = np.zeros(6, dtype=np.int32)
         _check(lib().hbi_hmat_matmul_plan(self.h, int(prec), info.ctypes.data_as(_ip)))
         return {"kc1": int(info[0]), "cols2": int(info[1]), "items1": int(info[2]), "items2": int(info[3]),
-                "zpart_slots": int(info[4]), "fp32_cores": bool(info[5])}
+                "zpart_slots": int(info[4]), "cores": ("fp64", "fp32", "bf16")[int(info[5])]}
 
 
 def mfma_f64_rate(waves_per_simd=2, iters=4000) -> float:
@@ -609,6 +611,21 @@ class Ensemble:
                                        acc.ctypes.data_as(C.POINTER(C.c_longlong)),
                                        rej.ctypes.data_as(C.POINTER(C.c_longlong)), _ipp(st)))
         return {"y": y, "slip": slip, "vel": vel, "t": t, "accepted": acc, "rejected": rej, "status": st}
+
+    def set_clock(self, t, dt):
+        """Per-realisation t and next step size (after set_state)."""
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        if t.shape != (self.nreal,) or dt.shape != (self.nreal,):
+            raise ValueError("t and dt must have nreal entries")
+        _check(lib().hbi_ens_set_clock(self.e, _p(t), _p(dt)))
+
+    def get_clock(self):
+        """(t, next step size) per realisation."""
+        t = np.empty(self.nreal)
+        dt = np.empty(self.nreal)
+        _check(lib().hbi_ens_get_clock(self.e, _p(t), _p(dt)))
+        return t, dt
 
     def set_probes(self, idx):
         """History rows of the following step calls also carry (V, slip) of up

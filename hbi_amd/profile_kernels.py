@@ -11,7 +11,7 @@ matrix cores, FP32 on the FP32 cores, mixed on the BF16 cores) over
 With --copy-mb > 0 it also runs a plain device-to-device copy of that many
 MiB (HIP runtime blit kernel) `iters` times, the reference for scaling the
 profiler's byte counters.  With --ensemble K it instead runs the bench's
-ensemble (64 friction realisations, learned step size, FP32 factors) for 20
+ensemble (64 friction realisations, learned step size, mixed precision) for 20
 warm-up and K timed lockstep steps, for a kernel trace of the attempt; with
 --fluid K the fluid-driven cycle of the bp6a example (4000 elements) for 20
 warm-up and K timed steps.  Prints one JSON line.

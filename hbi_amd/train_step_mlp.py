@@ -7,11 +7,13 @@ c1: the 2-D antiplane fault (2048 elements, 40 km, leaf 32, ACA eps 1e-6),
     `seeds` perturbed realisations (default 44 x 8000 steps) advanced as one
     lockstep ensemble, FP64;
 c2: the 32768-element 3-D planar fault (256 x 128, leaf 64, ACA eps 1e-5,
-    mixed precision), `seeds` realisations of its lognormal friction
-    (default 192: seeds 1-64, 201-264, 301-364, clear of the held-out
-    101-164) advanced 64 at a time as lockstep ensembles (default 1750
-    steps);
-the defaults give about 1e6 (features, step) pairs over three rounds;
+    mixed precision), realisations of its lognormal friction advanced 64 at
+    a time as lockstep ensembles: seeds 1-64 for `steps` steps (default
+    5300: the first event cycle and the second event), seeds 201-264 and
+    301-364 (clear of the held-out 101-164) for 1750 steps each (the first
+    event); --seeds N takes the first N of these 192;
+the defaults give 1.06e6 (c1) and 1.69e6 (c2) (features, step) pairs over
+three rounds;
 RK eps_r 1e-6 for both.  Round 1 records traces of accepted steps under the
 reference's controller (rkqs); every further round adds traces recorded under
 the model of the previous round (the states the learned controller itself
@@ -59,23 +61,24 @@ def c2_setup(seeds, first_seed=1, hmat=None):
 
 
 C2_TRAIN_FIRST_SEEDS = (1, 201, 301)
+C2_SHORT_STEPS = 1750
 
 
-def train_c2(seeds=192, steps=1750, rounds=3, epochs=40, log=print):
+def train_c2(seeds=192, steps=5300, rounds=3, epochs=40, log=print):
     g, Hm, lod, frs, ys = c2_setup(min(seeds, 64))
-    batches = [(frs, ys)]
+    batches = [(frs, ys, steps)]
     for f0 in C2_TRAIN_FIRST_SEEDS[1:]:
         k = min(64, seeds - 64 * len(batches))
         if k <= 0:
             break
         _, _, _, fb, yb = c2_setup(k, first_seed=f0, hmat=(g, Hm))
-        batches.append((fb, yb))
+        batches.append((fb, yb, min(steps, C2_SHORT_STEPS)))
     traces = []
     m = None
     t0 = time.time()
     for r in range(rounds):
-        for fb, yb in batches:
-            traces += L.ensemble_trace_run(Hm, g, fb, lod, yb, steps, hdiv=C2["ds0"], prec=H.PREC_MIXED, mlp=m)
+        for fb, yb, k in batches:
+            traces += L.ensemble_trace_run(Hm, g, fb, lod, yb, k, hdiv=C2["ds0"], prec=H.PREC_MIXED, mlp=m)
         X, T = L.dataset(traces)
         m, loss = L.fit(X, T, epochs=epochs)
         log("round %d: %d samples, loss %.3e (target variance %.3e), %.1f s" % (r + 1, len(X), loss[-1], T.var(),
@@ -117,7 +120,7 @@ def main():
         m = train(a.seeds or 44, a.steps or 8000, a.rounds, a.epochs, log=log)
     else:
         out = a.out or C2_OUT
-        m = train_c2(a.seeds or 192, a.steps or 1750, a.rounds, a.epochs, log=log)
+        m = train_c2(a.seeds or 192, a.steps or 5300, a.rounds, a.epochs, log=log)
     a.out = out
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     L.save(m, a.out)

@@ -154,7 +154,8 @@ int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms);
 /* The autotuned 64-column plan of a precision (built on first use):
    info[0] phase-1 column chunk, [1] phase-2 columns per item, [2] phase-1
    items, [3] phase-2 items (= FP64 partial row tiles written), [4] phase-1
-   partial slots, [5] 1 if the FP32 matrix cores are used. */
+   partial slots, [5] the matrix cores used: 0 FP64, 1 FP32, 2 BF16 (split
+   FP32 operands). */
 int hbi_hmat_matmul_plan(hbi_hmat *h, int prec, int *info);
 /* Measured FP64 matrix-core rate (TFLOP/s) of v_mfma_f64_16x16x4f64, the
    instruction of both 64-column kernels: waves_per_simd (1..8) waves per SIMD
@@ -345,6 +346,14 @@ int hbi_ens_create(hbi_hmat *hs, const hbi_rk_opts *opts, int nreal,
 int hbi_ens_destroy(hbi_ens *e);
 /* Same initial time and first trial step for every realisation; slip reset to 0 */
 int hbi_ens_set_state(hbi_ens *e, const double *y, double t, double dt_next);
+/* Per-realisation clock: t and the next step size of each of the nreal
+   realisations (call after hbi_ens_set_state, which gives every realisation
+   the same t and dt_next; counters and states are kept).  With
+   hbi_ens_get_clock, a run can continue from another run's states, e.g. a
+   spun-up ensemble.  No counterpart in the reference (its runs restart from
+   a snapshot with their own t and dt). */
+int hbi_ens_set_clock(hbi_ens *e, const double *t, const double *dt_next);
+int hbi_ens_get_clock(hbi_ens *e, double *t, double *dt_next);
 /* Any output pointer may be NULL; t, accepted, rejected, status have nreal
    entries (status as hbi_rk_info.status) */
 int hbi_ens_get_state(hbi_ens *e, double *y, double *slip, double *vel, double *t,

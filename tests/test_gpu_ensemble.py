@@ -203,3 +203,37 @@ def test_gpu_ensemble_stop_is_per_realisation():
     vel = 2 * frs[2].vref * np.exp(-y[0::3] / a) * np.sinh(y[1::3] / y[2::3] / a)
     done, *_ = so.run(y, 0.0, 1.0, 1e-6, 10, np.zeros(g.n), vel, np.asarray(g.ds)[lod])
     assert done == 0
+
+
+def test_gpu_ensemble_continues_from_another_ensembles_clock():
+    """hbi_ens_set_clock / hbi_ens_get_clock: an ensemble set to another's
+    states, times and next step sizes continues exactly as that ensemble
+    does (rkqs: the next step depends on the state and h only), also with
+    realisations at different times."""
+    g, frs = _c1_realisations(1024, 6)
+    Hg = H.HMatrix(g, leaf=32, eps=1e-6)
+    lod = Hg.lod
+    ys = np.array([P.init_state(f, lod) for f in frs])
+    for y in ys:
+        P.perturb_state(y, g, lod)
+    A = H.Ensemble(Hg, frs, lod, eps_r=1e-6)
+    A.set_state(ys, 0.0, 1.0)
+    A.step(150)
+    st = A.get_state()
+    t, h = A.get_clock()
+    assert np.array_equal(t, st["t"]) and (h > 0).all() and np.unique(t).size > 1
+    B = H.Ensemble(Hg, frs, lod, eps_r=1e-6)
+    B.set_state(st["y"], 0.0, 1.0)
+    B.set_clock(t, h)
+    tb, hb = B.get_clock()
+    assert np.array_equal(tb, t) and np.array_equal(hb, h)
+    A.step(40)
+    B.step(40)
+    sa, sb = A.get_state(), B.get_state()
+    assert np.array_equal(sa["t"], sb["t"])
+    assert np.array_equal(sa["y"], sb["y"])
+    assert list(sb["accepted"]) == [40] * 6 and np.array_equal(sb["rejected"], sa["rejected"] - st["rejected"])
+    with pytest.raises(H.HbiError):
+        B.set_clock(t, -h)
+    A.close()
+    B.close()
