@@ -12,7 +12,8 @@ device.
 
 Headline (``value``): the mixed-precision, batched, learned-step form -- 64
 realisations of the fault's friction (seeds 1..64) advanced in lockstep with
-one 64-column H product per RK stage, FP32 factors (FP64 state), each
+one 64-column H product per RK stage, FP32 factors split into three BF16
+parts on the BF16 matrix cores (FP64 state), each
 realisation's next step proposed by the step-size MLP trained on this fault
 (hbi_amd/data/step_mlp_c2.npz; the embedded error check judges every step)
 -- as accepted steps/s summed over the realisations, over K lockstep steps
@@ -43,9 +44,14 @@ BF16_MFMA_PEAK_TFS = 2500.0
 # HBM bytes per launch of the two roofline kernels (mixed precision), from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE passes over
 # hbi_amd/profile_kernels.py, scaled by the counters of a 256 MiB device copy
-# (fetch x 2.00, write x 1.00): mr_bf16_kernel<4,true,3> and stream_kernel
-TRAFFIC_MR_BF16_MIXED = None
-TRAFFIC_STREAM_MIXED = 225.7e6
+# (fetch x 2.00, write x 1.00): mr_bf16_kernel<4,true,3> (512 items of 2048
+# columns: 134033.5 KiB fetched, 512 x 32 KiB partial tiles written) and
+# stream_kernel (110083.2 KiB fetched, 2887.5 KiB written)
+TRAFFIC_MR_BF16_MIXED = 291.3e6
+TRAFFIC_STREAM_MIXED = 228.4e6
+# rocprofv3 --pmc MfmaUtil over the same program: fraction of cycles the
+# matrix cores of mr_bf16_kernel<4,true,3> are busy
+MFMA_UTIL_MR_BF16 = 0.193
 IMAX, JMAX, DS0, LEAF, EPS_H, EPS_R = 256, 128, 0.390625, 64, 1e-5, 1e-6
 NREAL = 64
 FLUID_STEPS = 500      # timed accepted steps of the bp6a fluid-driven cycle
@@ -365,7 +371,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "3dph 256x128 planar fault, N=32768, leaf 64, ACA eps 1e-5, RK45 eps_r 1e-6; "
                                "64 friction realisations in lockstep, one 64-column H product per RK stage, "
-                               "FP32 factors, learned step-size proposals (rkqs check), hipGraph attempts",
+                               "FP32 factors on the BF16 matrix cores (3-part split), learned step-size proposals "
+                               "(rkqs check), hipGraph attempts",
                    "n_elements": IMAX * JMAX, "realisations": NREAL},
         "roofline": {"bound": "hbm",
                      "kernel": "mr_bf16_kernel<4,true,3> (64-RHS row tiles on the BF16 matrix cores, split operands)",
@@ -376,6 +383,7 @@ def main():
                      "useful_tflops": round(mmx["p2_TFs"], 2),
                      "useful_frac_of_bf16_peak": round(mmx["p2_TFs"] / BF16_MFMA_PEAK_TFS, 4),
                      "mfma_executed_frac_of_bf16_peak": round(6 * mmx["p2_TFs"] / BF16_MFMA_PEAK_TFS, 4),
+                     "mfma_util_counter": MFMA_UTIL_MR_BF16,
                      "note": "arithmetic intensity 2*64 flop per 4-byte factor (32 flop/B) is below the BF16 ridge "
                              "(312 flop/B): HBM-bound; six BF16 MFMA terms per FP32 product"},
         "roofline_matvec": {"bound": "hbm", "kernel": "stream_kernel<64> (single-vector product, first launch: "
