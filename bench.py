@@ -56,6 +56,8 @@ IMAX, JMAX, DS0, LEAF, EPS_H, EPS_R = 256, 128, 0.390625, 64, 1e-5, 1e-6
 NREAL = 64
 FLUID_STEPS = 500      # timed accepted steps of the bp6a fluid-driven cycle
 FLUID_CPU_STEPS = 200
+C1_STEPS = 20000       # accepted steps of each GPU run of the 2-D antiplane fault (its configured run length)
+C1_CPU_STEPS = 20000   # the CPU oracle runs the same run length
 
 
 def _onset(t, lv, thr=-3.0):
@@ -163,6 +165,8 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
                          "log10_vmax_end": round(float(fhist[-1, 1]), 4)}
     fv.close()
 
+    out["c1"] = c1_gpu(H, P, C1_STEPS)
+
     # batched form: 64 realisations in lockstep, mixed precision, rkqs and learned step size
     from hbi_amd import learned as L
     mlp = L.load(os.path.join(ROOT, "hbi_amd", "data", "step_mlp_c2.npz"))
@@ -211,6 +215,68 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
     out["controllers_by_phase"] = controllers_by_phase(H, Hm, frs, lod, ys, mlp, warmup, steps)
     out["accuracy_ensemble_vs_f64"] = ensemble_accuracy(H, Hm, frs, lod, ys, g)
     out["p2_entries"], out["p1_entries"] = p2_entries, p1_entries
+    Hm.close()
+    return out
+
+
+def _events(t, lv, thr=-3.0):
+    """onset times: upward crossings of log10 Vmax = thr"""
+    up = np.nonzero((lv[1:] > thr) & (lv[:-1] <= thr))[0] + 1
+    return [float(t[k]) for k in up]
+
+
+def c1_gpu(H, P, steps):
+    """The 2-D antiplane fault (2048 elements over 40 km, leaf 32, ACA eps
+    1e-6, aging law) from the steady state at V = Vpl with a seeded
+    nucleation bump (the exact steady state has no dynamics: rkqs underflows
+    on it, as the reference's does), `steps` accepted steps of single GPU runs: FP64, mixed, and mixed with the learned step size
+    (hbi_amd/data/step_mlp_c1.npz); the single-vector product of each
+    precision; event onsets of the mixed runs against the FP64 run's."""
+    from hbi_amd import learned as L
+    print("bench: C1, 2048-element antiplane fault, %d steps per run" % steps, file=sys.stderr, flush=True)
+    g = P.mesh_2dvs(2048, 40.0 / 2048)
+    Hm = H.HMatrix(g, leaf=32, eps=1e-6)
+    st = Hm.stats
+    lod = Hm.lod
+    fr = P.friction_c1(g)
+    y0 = P.init_state(fr, lod)
+    P.perturb_state(y0, g, lod)
+    out = {"n": g.n, "steps": steps, "matvec_ms": {}, "runs": {}}
+    for prec, name in ((H.PREC_F64, "f64"), (H.PREC_F32, "f32"), (H.PREC_MIXED, "mixed")):
+        t = Hm.time_matvec(prec, 50)
+        out["matvec_ms"][name] = round(t["total_ms"], 5)
+    out["matvec_alg_bytes_f64"] = st["alg_bytes"][H.PREC_F64]
+    mlp = L.load(os.path.join(ROOT, "hbi_amd", "data", "step_mlp_c1.npz"))
+    hists = {}
+    for name, prec, learned in (("f64", H.PREC_F64, False), ("mixed", H.PREC_MIXED, False),
+                                ("mixed_learned", H.PREC_MIXED, True)):
+        rk = H.RK(Hm, fr, lod, g.ds, eps_r=EPS_R, prec=prec)
+        if learned:
+            rk.set_controller(1, mlp, 0.9)
+        rk.set_state(y0, 0.0, 1.0)
+        rk.step(20)
+        rk.set_state(y0, 0.0, 1.0)
+        t0 = time.perf_counter()
+        info, hist, _ = rk.step(steps, history=True)
+        wall = time.perf_counter() - t0
+        hists[name] = hist
+        out["runs"][name] = {"steps_per_s": round(steps / wall, 1), "attempts": info["attempts"],
+                             "rejected": info["rejected"], "t_end_s": info["t"],
+                             "sim_time_per_wall_s": info["t"] / wall,
+                             "events": len(_events(hist[:, 0], hist[:, 4])), "status": info["status"]}
+        rk.close()
+    e64 = _events(hists["f64"][:, 0], hists["f64"][:, 4])
+    for name in ("mixed", "mixed_learned"):
+        em = _events(hists[name][:, 0], hists[name][:, 4])
+        k = min(len(e64), len(em))
+        out["runs"][name]["onset_rel_vs_f64"] = (max(abs(a - b) / b for a, b in zip(em[:k], e64[:k]))
+                                                 if k else None)
+    t_common = min(hists["f64"][-1, 0], hists["mixed_learned"][-1, 0])
+    a_p = L.attempts_to_time(hists["mixed"], t_common)
+    a_l = L.attempts_to_time(hists["mixed_learned"], t_common)
+    out["learned_attempt_ratio_to_common_time"] = a_p / a_l
+    out["speedup_mixed_learned_vs_f64_sim_time"] = (out["runs"]["mixed_learned"]["sim_time_per_wall_s"]
+                                                    / out["runs"]["f64"]["sim_time_per_wall_s"])
     Hm.close()
     return out
 
@@ -318,6 +384,28 @@ def cpu_part(nsteps: int):
            "sample": "%d accepted RK45 steps (FP64 oracle, OpenMP) of one realisation of the 32768-element fault "
                      "after a %.1f s H-matrix build" % (done, t_build)}
     Ho = None
+    # C1: the 2-D antiplane fault from the steady state at Vpl with the seeded nucleation bump
+    g1 = P.mesh_2dvs(2048, 40.0 / 2048)
+    fr1 = P.friction_c1(g1)
+    H1 = O.HMatrix(g1, leaf=32, eps=1e-6, nthreads=ncores)
+    x1 = np.random.default_rng(0).standard_normal(g1.n)
+    H1.matvec(x1, nthreads=ncores)
+    t0 = time.perf_counter()
+    for _ in range(50):
+        H1.matvec(x1, nthreads=ncores)
+    mv1 = (time.perf_counter() - t0) / 50
+    y1 = P.init_state(fr1, H1.lod)
+    P.perturb_state(y1, g1, H1.lod)
+    a1 = fr1.a[H1.lod]
+    vel1 = 2 * fr1.vref * np.exp(-y1[0::3] / a1) * np.sinh(y1[1::3] / y1[2::3] / a1)
+    t0 = time.perf_counter()
+    d1, *_ = O.Sim(H1, fr1, H1.lod, nthreads=ncores).run(y1, 0.0, 1.0, EPS_R, C1_CPU_STEPS, np.zeros(g1.n), vel1,
+                                                          np.asarray(g1.ds)[H1.lod])
+    wall = time.perf_counter() - t0
+    out["c1"] = {"steps_per_s": round(d1 / wall, 1), "matvec_ms": round(1e3 * mv1, 4), "cores": ncores,
+                 "sample": "%d accepted steps of the 2048-element antiplane fault from the seeded steady "
+                           "state (FP64 oracle, OpenMP products)" % d1}
+    H1 = None
     # the fluid-driven cycle's oracle (numpy, C oracle products) on bp6a
     from oracle import fluid as F
     gf = P.mesh_2dp(P.BP6A_N, P.BP6A_DS0)
@@ -400,6 +488,7 @@ def main():
                                 "rkqs": round(ens0["sim_time_per_wall_s"], 1)},
         "learned_vs_rkqs": r["learned_vs_rkqs"],
         "controllers_by_phase": r["controllers_by_phase"],
+        "c1": r["c1"],
         "ensemble_rkqs": ens0,
         "accuracy_mixed_vs_f64": r["accuracy_mixed_vs_f64"],
         "accuracy_ensemble_vs_f64": r["accuracy_ensemble_vs_f64"],

@@ -431,6 +431,8 @@ __global__ __launch_bounds__(kHid) void ens_learn_kernel(EnsPar P, EnsCtl *c, En
     __shared__ double q[3];
     const int r = blockIdx.x, j = threadIdx.x;
     if (c->done || !P.learn || !c->acc[r]) return;
+    MlpRow R;
+    if (P.use_mlp && !P.grid) R = mlp_load_row(A.mlp);
     int *bins = A.bins + (size_t)r * kBins;
     step_features(bins, P.n, c->hdid[r], c->errmax[r], c->nrjct[r] > 0, c->mem[r], part, hb, x, q);
     if (j == 0 && A.trace && c->trace_n[r] < A.trace_cap) {
@@ -441,7 +443,7 @@ __global__ __launch_bounds__(kHid) void ens_learn_kernel(EnsPar P, EnsCtl *c, En
         c->trace_n[r]++;
     }
     if (P.use_mlp && !P.grid) {
-        const float out = mlp_infer_block(A.mlp, x, h1, red);
+        const float out = mlp_infer_block(A.mlp, R, x, h1, red);
         if (j == 0) c->h[r] = learned_h(x[11], out, c->h[r], P.safety, P.ratio_lo, P.ratio_hi, P.dtmax);
     }
     if (j == 0) {

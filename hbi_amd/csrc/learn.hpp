@@ -26,24 +26,67 @@ __device__ __forceinline__ float gelu_grad(float x)
     return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
 }
 
-// One sample on one workgroup of kHid threads.  x: kIn raw features (LDS),
-// h1, red: kHid floats of LDS scratch.  Every thread returns the output.
-__device__ __forceinline__ float mlp_infer_block(const MlpDev &M, const float *x, float *h1, float *red)
+// The weights of hidden unit j (thread j of a kHid-thread workgroup) in
+// registers: W2 row j as 16-byte groups of eight BF16 values, W1 row j as
+// 8-byte groups of four, the biases and W3[j].  Loaded together at the top of
+// a kernel, their latency overlaps the work before the inference.
+struct MlpRow {
+    uint4 w2[kHid / 8];
+    uint2 w1[kIn / 4];
+    float b1, b2, w3, b3;
+};
+static_assert(kOffW2 % 8 == 0 && kIn % 4 == 0 && kHid % 8 == 0, "MLP row loads need 16- and 8-byte alignment");
+
+__device__ __forceinline__ MlpRow mlp_load_row(const MlpDev &M)
 {
     const int j = threadIdx.x;
-    float z = M.p[kOffB1 + j];
-    for (int i = 0; i < kIn; i++) z += bf16_to_f32(M.w[kOffW1 + j * kIn + i]) * ((x[i] - M.mu[i]) / M.sd[i]);
+    MlpRow R;
+    const uint4 *w2 = (const uint4 *)(M.w + kOffW2 + (size_t)j * kHid);
+#pragma unroll
+    for (int q = 0; q < kHid / 8; q++) R.w2[q] = w2[q];
+    const uint2 *w1 = (const uint2 *)(M.w + kOffW1 + (size_t)j * kIn);
+#pragma unroll
+    for (int q = 0; q < kIn / 4; q++) R.w1[q] = w1[q];
+    R.b1 = M.p[kOffB1 + j];
+    R.b2 = M.p[kOffB2 + j];
+    R.w3 = bf16_to_f32(M.w[kOffW3 + j]);
+    R.b3 = M.p[kOffB3];
+    return R;
+}
+
+// BF16 value k of a group of packed pairs (low half first)
+__device__ __forceinline__ float bf_at(uint32_t pair, int hi) { return __uint_as_float(hi ? pair & 0xffff0000u : pair << 16); }
+
+// One sample on one workgroup of kHid threads.  x: kIn raw features (LDS),
+// h1, red: kHid floats of LDS scratch.  Every thread returns the output.
+// Layer sums run in the order of the parameter vector (b + w_0 a_0 + w_1 a_1 ...).
+__device__ __forceinline__ float mlp_infer_block(const MlpDev &M, const MlpRow &R, const float *x, float *h1, float *red)
+{
+    const int j = threadIdx.x;
+    float z = R.b1;
+#pragma unroll
+    for (int i = 0; i < kIn; i++) {
+        const uint2 g = R.w1[i / 4];
+        const uint32_t pr = (i % 4) < 2 ? g.x : g.y;
+        z += bf_at(pr, i % 2) * ((x[i] - M.mu[i]) / M.sd[i]);
+    }
     h1[j] = gelu(z);
     __syncthreads();
-    float z2 = M.p[kOffB2 + j];
-    for (int k = 0; k < kHid; k++) z2 += bf16_to_f32(M.w[kOffW2 + j * kHid + k]) * h1[k];
-    red[j] = bf16_to_f32(M.w[kOffW3 + j]) * gelu(z2);
+    float z2 = R.b2;
+#pragma unroll
+    for (int k = 0; k < kHid; k++) {
+        const uint4 g = R.w2[k / 8];
+        const int e = k % 8;
+        const uint32_t pr = e < 2 ? g.x : e < 4 ? g.y : e < 6 ? g.z : g.w;
+        z2 += bf_at(pr, e % 2) * h1[k];
+    }
+    red[j] = R.w3 * gelu(z2);
     __syncthreads();
     for (int s = kHid / 2; s > 0; s >>= 1) {
         if (j < s) red[j] += red[j + s];
         __syncthreads();
     }
-    return red[0] + M.p[kOffB3];
+    return red[0] + R.b3;
 }
 
 // ---- step features (shared by the single-run and the ensemble RK graphs)
@@ -63,46 +106,59 @@ struct StepMem {
     int have_prev;
 };
 
-// Block of kHid threads; bins: the histogram of n elements.  Thread 0 fills
-// q (max, median, p99 of log10 |V|, bin centres) and the 12 features x:
-// q, their changes since the previous step, log10 h, log10 errmax, the
-// rejection flag, log10 h and errmax of the previous step, and the
-// error-optimal step log10(h errmax^(-1/5)).  part: kHid + 1 ints of LDS,
-// hb: kBins ints of LDS.
+// Block of kHid threads; bins: the histogram of n elements.  Fills q (max,
+// median, p99 of log10 |V|, bin centres) and the 12 features x: q, their
+// changes since the previous step, log10 h, log10 errmax, the rejection flag,
+// log10 h and errmax of the previous step, and the error-optimal step
+// log10(h errmax^(-1/5)).  Thread j owns bins per j .. per (j + 1) - 1: the
+// counts of the chunks are scanned across the workgroup (shuffles within each
+// wavefront, then the first wavefront's total added to the second's), and
+// the one thread whose chunk holds a rank walks its bins to it.  part: kHid +
+// 1 ints of LDS, hb: kBins ints of LDS.
+static_assert(kHid == 128, "the chunk scan spans two wavefronts");
 __device__ __forceinline__ void step_features(const int *bins, int n, double hdid, double errmax, bool rejected,
                                               const StepMem &m, int *part, int *hb, float *x, double *q)
 {
-    const int j = threadIdx.x;
+    const int j = threadIdx.x, lane = j % 64;
     constexpr int per = kBins / kHid;
-    int cnt = 0;
-    for (int b = 0; b < per; b++) {
-        const int k = bins[b * kHid + j];      // coalesced; hb in bin order
-        hb[b * kHid + j] = k;
+    constexpr double w = (kBinHi - kBinLo) / kBins;
+    for (int b = 0; b < per; b++) hb[b * kHid + j] = bins[b * kHid + j];   // coalesced; hb in bin order
+    if (j < 3) q[j] = kBinLo + (kBins - 0.5) * w;   // a rank past the histogram's total: the last bin
+    __syncthreads();
+    int s = 0;
+    for (int b = 0; b < per; b++) s += hb[j * per + b];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(s, o, 64);
+        if (lane >= o) s += t;
+    }
+    if (j == 63) part[0] = s;
+    __syncthreads();
+    if (j >= 64) s += part[0];
+    __syncthreads();
+    part[j + 1] = s;   // elements in chunks <= j
+    if (j == 0) part[0] = 0;
+    __syncthreads();
+    // the smallest bin whose cumulative count reaches each 1-based rank
+    const long long rank[3] = {(long long)n, (long long)(n + 1) / 2, (long long)ceil(0.99 * n)};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const long long target = rank[k];
+        if (part[j] < target && target <= part[j + 1]) {
+            long long cum = part[j];
+            int bin = (j + 1) * per - 1;
+            for (int b = j * per; b < (j + 1) * per; b++) {
+                cum += hb[b];
+                if (cum >= target) {
+                    bin = b;
+                    break;
+                }
+            }
+            q[k] = kBinLo + (bin + 0.5) * w;
+        }
     }
     __syncthreads();
-    for (int b = 0; b < per; b++) cnt += hb[j * per + b];
-    part[j + 1] = cnt;
-    __syncthreads();
     if (j == 0) {
-        part[0] = 0;
-        for (int t = 0; t < kHid; t++) part[t + 1] += part[t];   // part[t] = elements in chunks < t
-        const double w = (kBinHi - kBinLo) / kBins;
-        // smallest bin whose cumulative count reaches target (1-based rank)
-        auto rank_bin = [&](long long target) {
-            int t = 0;
-            while (t < kHid - 1 && part[t + 1] < target) t++;
-            long long cum = part[t];
-            for (int b = t * per; b < (t + 1) * per; b++) {
-                cum += hb[b];
-                if (cum >= target) return b;
-            }
-            return (t + 1) * per - 1;
-        };
-        const long long n50 = (n + 1) / 2, n99 = (long long)ceil(0.99 * n);
-        const int b50 = rank_bin(n50), b99 = rank_bin(n99), bmax = rank_bin(n);
-        q[0] = kBinLo + (bmax + 0.5) * w;
-        q[1] = kBinLo + (b50 + 0.5) * w;
-        q[2] = kBinLo + (b99 + 0.5) * w;
         const double lh = log10(hdid), le = log10(errmax);
         const double ph = m.have_prev ? log10(m.prev_h) : lh, pe = m.have_prev ? log10(m.prev_e) : le;
         for (int k = 0; k < 3; k++) {

@@ -259,9 +259,14 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
 {
 #pragma clang fp contract(off)
     __shared__ double sp[kRkThreads / kWave][kMon];
+    __shared__ int lh[kBins];   // the block's log10 |V| histogram (learned step size)
     if (c->done || !c->acc) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const double dt = c->hdid;
+    if (P.learn) {
+        for (int b = threadIdx.x; b < kBins; b += blockDim.x) lh[b] = 0;
+        __syncthreads();
+    }
     // max |V|, sum |slip| area, sum mu area, max sigma, min sigma, index of the
     // first max |V| (identities for i >= n; -1 < |V| so element 0 wins ties with it)
     double m[kMon] = {-1.0, 0.0, 0.0, -HUGE_VAL, HUGE_VAL, -1.0};
@@ -285,7 +290,12 @@ __global__ __launch_bounds__(kRkThreads) void rk_commit_kernel(RkPar P, const Rk
         m[3] = sig;
         m[4] = sig;
         m[5] = (double)i;
-        if (P.learn) atomicAdd(&A.bins[vbin(m[0])], 1);
+        if (P.learn) atomicAdd(&lh[vbin(m[0])], 1);
+    }
+    if (P.learn) {   // most elements share a few bins: one global add per nonzero bin and block
+        __syncthreads();
+        for (int b = threadIdx.x; b < kBins; b += blockDim.x)
+            if (lh[b]) atomicAdd(&A.bins[b], lh[b]);
     }
     // per-block combine in a fixed order (sums ordered by lane, then by wave)
     for (int o = 1; o < kWave; o <<= 1) {
@@ -331,6 +341,8 @@ __global__ __launch_bounds__(kHid) void rk_learn_kernel(RkPar P, RkCtl *c, RkArr
     __shared__ double q[3];
     if (c->done || !c->acc || !P.learn) return;
     const int j = threadIdx.x;
+    MlpRow R;
+    if (P.use_mlp) R = mlp_load_row(A.mlp);
     step_features(A.bins, P.n, c->hdid, c->errmax, c->nrjct > 0, c->mem, part, hb, x, q);
     if (j == 0 && A.trace && c->trace_n < A.trace_cap) {
         double *row = A.trace + 14 * (size_t)c->trace_n;
@@ -340,7 +352,7 @@ __global__ __launch_bounds__(kHid) void rk_learn_kernel(RkPar P, RkCtl *c, RkArr
         c->trace_n++;
     }
     if (P.use_mlp) {
-        const float out = mlp_infer_block(A.mlp, x, h1, red);
+        const float out = mlp_infer_block(A.mlp, R, x, h1, red);
         if (j == 0) c->h = learned_h(x[11], out, c->h, P.safety, P.ratio_lo, P.ratio_hi, P.dtmax);
     }
     if (j == 0) {
