@@ -463,7 +463,7 @@ def main():
                                "(rkqs check), hipGraph attempts",
                    "n_elements": IMAX * JMAX, "realisations": NREAL},
         "roofline": {"bound": "hbm",
-                     "kernel": "mr_bf16_kernel<4,true,3> (64-RHS row tiles on the BF16 matrix cores, split operands)",
+                     "kernel": "mr_bf16_kernel<4,true,2> (64-RHS row tiles on the BF16 matrix cores, split operands)",
                      "achieved": round(r["p2_alg_bytes"] / (mmx["p2_ms"] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r["p2_alg_bytes"] / (mmx["p2_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": TRAFFIC_MR_BF16_MIXED,

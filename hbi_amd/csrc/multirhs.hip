@@ -649,7 +649,8 @@ __global__ void mr_tile_reduce_kernel(int n, int tr, const int *__restrict__ par
 // one barrier per slice.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-constexpr int kBfStages = 3;              // register stages of the load ring (4 measured slower: more VGPRs)
+constexpr int kBfStages = 2;              // register stages of the load ring (2: the fewest registers; with
+                                          // more, narrower items in flight per CU beat deeper prefetch)
 constexpr int kBfRW = 20;                 // LDS words per right-hand-side row of a B plane
 constexpr int kBfPlane = kRR * kBfRW;     // words per plane
 constexpr int kP1RowsBf = 64;             // a1^T rows per phase-1 item at most (4 waves; 128-row items on 8
@@ -697,7 +698,7 @@ __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restr
     constexpr int NT = 64 * NW;
     constexpr int PP = (256 + NT - 1) / NT;   // B pairs (two columns x four right-hand sides) per thread
     __shared__ __align__(16) uint32_t bs[2][3][kBfPlane];
-    __shared__ int cm[P2 ? kMaxCols32 : 1];
+    extern __shared__ int cm[];   // phase 2: the item's column map (dynamic: the plan's widest item)
     if (gate && *gate) return;
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
     const int c = lane % 16, q = lane / 16;
@@ -928,6 +929,7 @@ struct MrPlan {
     bool bf = false;
     int r1b[2] = {0, 0};           // phase-1 items [0, r1b[0]) fit 4 waves (64 rows), [r1b[0], r1b[1]) 8 waves
     std::vector<MItem2b> it2b;     // phase-2 items with their lane-order factor copies
+    int ncol2max = 0;              // columns of the widest phase-2 item (its column map is staged in LDS)
     std::vector<MPack> pack;       // (item, slice) blocks of the copy
     size_t a16_floats = 0;
     DevBuf<MItem2b> d_it2b;
@@ -1078,11 +1080,13 @@ static void mr_items2(const hbi_hmat *h, const Plan &P, MrPlan &M, int cols2)
         // lane-order copies: 16 x 32 blocks of 64 lanes x 8 floats
         const int nw = h->tr / 16;
         M.it2b.clear();
+        M.ncol2max = 0;
         M.pack.clear();
         uint64_t off = 0;
         for (size_t q = 0; q < M.it2.size(); q++) {
             const MItem2 &f = M.it2f[q];
             M.it2b.push_back({off, f.ncol, f.cm0, f.part, 0});
+            M.ncol2max = std::max(M.ncol2max, f.ncol);
             const int nsl = (f.ncol + 31) / 32;
             for (int sl = 0; sl < nsl; sl++) M.pack.push_back({f.off, off, f.ncol, sl, M.it2[q].dtype, nw});
             off += (uint64_t)nsl * nw * 64 * 8;
@@ -1219,11 +1223,12 @@ static void phase2(const hbi_hmat *h, const Plan &P, const MrPlan &M, const doub
     if (M.bf) {   // every FP32 / BF16 item in one launch (the copies are FP32)
         const int n = (int)M.it2b.size();
         if (n == 0) return;
+        const size_t cmb = (size_t)M.ncol2max * sizeof(int);
         if (h->tr == 64)
-            mr_bf16_kernel<4, true, kBfStages><<<n, 256, 0, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr, X32,
+            mr_bf16_kernel<4, true, kBfStages><<<n, 256, cmb, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr, X32,
                                                                  Z32, ypart, nullptr, nullptr, 0, gate);
         else
-            mr_bf16_kernel<2, true, kBfStages><<<n, 128, 0, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr, X32,
+            mr_bf16_kernel<2, true, kBfStages><<<n, 128, cmb, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr, X32,
                                                                  Z32, ypart, nullptr, nullptr, 0, gate);
         return;
     }
