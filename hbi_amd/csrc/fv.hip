@@ -16,9 +16,16 @@
 //   fields                  sigma_e = sigma - pf, disp, V, mu, kp,
 //                           qflow, monitor partials                   1045-1082
 //   monitor                 history row, stop criteria                1083-1152
+// Grid mode (main_fv3dp.f90, imax x jmax 3dp fault): the same sequence with
+// sigma held (not rebuilt from sigma_e + pf), the pressure solved on the 2-D
+// grid by one cooperative launch over all CUs (grid-wide reductions between
+// the conjugate-gradient steps), hnext <= dtmax, the minsig rule and the
+// monitor values of main_fv3dp.f90:725-815.
 // Every kernel first reads the device flag and returns when the batch target
 // is reached or the run stopped (rejected attempts skip commit .. monitor).
 #include "rkcore.hpp"
+
+#include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
 #include <cmath>
@@ -29,6 +36,9 @@ struct FvPar {
     int n, law, backslip, has_hn, pdep, pfconst, bcl, bcr, setting, outpertime;
     double vref, eps, hdiv, ds0, kL, kpmin, s0, eta, beta, Bs, pbcl, pbcr, q0, pinj, tinj, dtout;
     double velmin, velmax, tmax, sg;
+    // grid mode (imax, jmax > 0): main_fv3dp's 2-D pore-pressure grid
+    int grid, imax, jmax, bct, bcb, nwell;
+    double dtmax, minsig, phi0;
     Tab T;
 };
 
@@ -36,6 +46,7 @@ struct FvCtl {
     double h, t, hdid, errmax, vmax;
     long long attempts, accepted, rejected, target, hist_base, hist_cap;
     int done, status, acc, niter;
+    int sw, nn;   // grid mode: the well schedule's switch flag and 1-based position (implicitsolver2)
 };
 
 struct FvArrays {
@@ -45,6 +56,11 @@ struct FvArrays {
     double *probe;          // per accepted step, kFvProbe values for each probed element
     const int *probe_idx;
     int nprobe;
+    // grid mode: cluster index of grid cell l; wells (1-based cells, schedule of
+    // HBI_FV_MAX_SCHEDULE entries each)
+    const int *cell;
+    const int *wi, *wj, *wk;
+    const double *wq, *wt;
 };
 
 constexpr int kFvThreads = 256;
@@ -197,7 +213,8 @@ __global__ void fv_decide_kernel(FvPar P, FvCtl *c, const double *__restrict__ e
     c->errmax = errmax;
     if (errmax < 1.0 && errmax > 1e-15) {
         double hn = fmin(2 * h, SAFETY * h * pow(errmax, PGROW));
-        if (P.outpertime) hn = fmin(hn, P.dtout);
+        if (P.grid) hn = fmin(hn, P.dtmax);
+        else if (P.outpertime) hn = fmin(hn, P.dtout);
         c->hdid = h;
         c->t = c->t + h;
         c->h = hn;
@@ -232,7 +249,7 @@ __global__ void fv_commit_kernel(FvPar P, const FvCtl *__restrict__ c, FvArrays 
         A.y[4 * (size_t)i + q] = yo[q];
     }
     A.velpre[i] = slip_rate(yo[0], yo[1], yo[2], A.a[i], P.vref);
-    A.sigma[i] = yo[2] + A.pf[i];
+    if (!P.grid) A.sigma[i] = yo[2] + A.pf[i];   // grid mode: sigma (total) is held
 }
 
 // ---------------------------------------------------------- pore pressure --
@@ -532,6 +549,344 @@ __global__ __launch_bounds__(NT) void fv_pressure_kernel(FvPar P, FvCtl *c, FvAr
     }
 }
 
+// ------------------------------------------- the 2-D grid of main_fv3dp --
+// Work arrays of the 2-D solve, grid order (cell l = i jmax + j): cd the
+// diffusivities, co the six coefficients of each row (Amx along j, Amy along
+// i, Beuler's storage: first row (diagonal, right), last row (left at 1,
+// diagonal at 2)), b, the direction p (read at neighbours), the pressure at
+// the start pg, the Picard trial pt, the new pressure pn, pfhyd hg; red: the
+// per-workgroup partials of two reductions in flight.
+struct Fv2Work {
+    double *cd, *co, *b, *p, *pg, *pt, *pn, *hg, *red;
+};
+constexpr int kFv2Threads = 256;
+
+// Sum (max) of one value per thread over the whole grid: per workgroup in a
+// fixed order, the partials to red[slot], a grid barrier, then every
+// workgroup folds all partials in the same order (lane k takes partials k,
+// k + 64, ... then a fixed butterfly), so all get the same bits.  Two slots
+// alternate: a slot is written again only after the next reduction's barrier.
+template <bool MAX>
+__device__ __forceinline__ double fv2_reduce(double v, double *red, int &slot, double *lds)
+{
+    namespace cgr = cooperative_groups;
+    const int tid = threadIdx.x, lane = tid % kWave, w = tid / kWave;
+    const int G = gridDim.x;
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double u = __shfl_xor(v, o, kWave);
+        v = MAX ? fmax(v, u) : v + u;
+    }
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    double* out = red + (size_t)slot * G;
+    if (tid == 0) {
+        double b = lds[0];
+        for (int q = 1; q < kFv2Threads / kWave; q++) b = MAX ? fmax(b, lds[q]) : b + lds[q];
+        out[blockIdx.x] = b;
+    }
+    cgr::this_grid().sync();
+    if (w == 0) {
+        double a = MAX ? -HUGE_VAL : 0.0;
+        for (int q = lane; q < G; q += kWave) a = MAX ? fmax(a, out[q]) : a + out[q];
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const double u = __shfl_xor(a, o, kWave);
+            a = MAX ? fmax(a, u) : a + u;
+        }
+        if (lane == 0) lds[kFv2Threads / kWave] = a;
+    }
+    __syncthreads();
+    const double r = lds[kFv2Threads / kWave];
+    __syncthreads();
+    slot ^= 1;
+    return r;
+}
+
+// D(row, 0..2) of one grid line at position k of len (Beuler's Dxx along j
+// or Dyy along i, before the 1 / ds0^2 scaling); c(q) the line's q-th
+// diffusivity.  The same expressions as the 1-D fv_dxx_row.
+template <typename CF>
+__device__ __forceinline__ void fv2_line_row(int k, int len, int lo_bc, int hi_bc, CF c, double &d1, double &d2,
+                                             double &d3)
+{
+#pragma clang fp contract(off)
+    d1 = 0.0; d2 = 0.0; d3 = 0.0;
+    if (k == 0) {
+        d1 = -c(0) - c(1);
+        d2 = lo_bc == HBI_FV_DIRICHLET ? c(1) - c(0) : -d1;
+    } else if (k == 1) {
+        d1 = lo_bc == HBI_FV_DIRICHLET ? c(1) / 2 - c(0) / 2 : c(0) / 2 + c(1) / 2;
+        d2 = -c(0) / 2 - c(1) - c(2) / 2;
+        d3 = c(1) / 2 + c(2) / 2;
+    } else if (k == len - 2) {
+        d1 = c(len - 3) / 2 + c(len - 2) / 2;
+        d2 = -c(len - 3) / 2 - c(len - 2) - c(len - 1) / 2;
+        d3 = hi_bc == HBI_FV_DIRICHLET ? -c(len - 1) / 2 + c(len - 2) / 2 : c(len - 2) / 2 + c(len - 1) / 2;
+    } else if (k == len - 1) {
+        d3 = -c(len - 1) - c(len - 2);
+        d2 = hi_bc == HBI_FV_DIRICHLET ? c(len - 2) - c(len - 1) : -d3;
+    } else {
+        d1 = c(k - 1) / 2 + c(k) / 2;
+        d2 = -c(k - 1) / 2 - c(k) - c(k + 1) / 2;
+        d3 = c(k) / 2 + c(k + 1) / 2;
+    }
+}
+
+// Am of a line row from D (already / ds0^2): 0.5 - h D on the diagonal
+__device__ __forceinline__ void fv2_line_matrix(int k, int len, double h, double d1, double d2, double d3, double *a)
+{
+#pragma clang fp contract(off)
+    if (k == 0) {
+        a[0] = 0.5 - h * d1; a[1] = -h * d2; a[2] = 0.0;
+    } else if (k == len - 1) {
+        a[0] = 0.0; a[1] = -h * d2; a[2] = 0.5 - h * d3;
+    } else {
+        a[0] = -h * d1; a[1] = 0.5 - h * d2; a[2] = -h * d3;
+    }
+}
+
+// (A v)(l): the line products along j and i in the reference's order of terms
+// (main_fv3dp.f90:1566-1585), m = mx + my
+__device__ __forceinline__ double fv2_apply(const FvPar &P, const double *co, const double *v, int l)
+{
+#pragma clang fp contract(off)
+    const int jm = P.jmax, i = l / jm, j = l % jm;
+    const double *a = co + 6 * (size_t)l;
+    double mx, my;
+    if (j == 0) mx = v[l] * a[0] + v[l + 1] * a[1];
+    else if (j == jm - 1) mx = v[l] * a[2] + v[l - 1] * a[1];
+    else mx = v[l - 1] * a[0] + v[l] * a[1] + v[l + 1] * a[2];
+    if (i == 0) my = v[l] * a[3] + v[l + jm] * a[4];
+    else if (i == P.imax - 1) my = v[l] * a[5] + v[l - jm] * a[4];
+    else my = v[l - jm] * a[3] + v[l] * a[4] + v[l + jm] * a[5];
+    return mx + my;
+}
+
+// The rate of well w at time t (main_fv3dp.f90:1553-1565)
+__device__ __forceinline__ double fv2_well_rate(const FvArrays &A, int w, double t)
+{
+#pragma clang fp contract(off)
+    const double *qv = A.wq + (size_t)w * HBI_FV_MAX_SCHEDULE, *qt = A.wt + (size_t)w * HBI_FV_MAX_SCHEDULE;
+    const int kl = A.wk[w];
+    double q = 0.0;
+    for (int k = 0; k < kl - 1; k++) {
+        const double t0 = qt[k], t1 = qt[k + 1], v0 = qv[k], v1 = qv[k + 1];
+        if (t >= t0 && t <= t1) q = (v1 - v0) / (t1 - t0) * (t - t0) + v0;
+        else if (t > t1 && k == kl - 2) q = v1;
+    }
+    return q;
+}
+
+// implicitsolver2 / implicitsolver of main_fv3dp (1314-1424) with Beuler
+// (1426-1649) on the imax x jmax grid: cooperative launch, kFv2Threads
+// threads per workgroup, NPT cells per thread (cell l = g + q G kFv2Threads,
+// g the global thread index); x and r of its cells in registers, p and the
+// coefficients in global memory (neighbours).  Gated like the 1-D kernel.
+template <int NPT>
+__global__ __launch_bounds__(kFv2Threads) void fv2_pressure_kernel(FvPar P, FvCtl *c, FvArrays A, FvSolveIO io,
+                                                                    Fv2Work W)
+{
+#pragma clang fp contract(off)
+    namespace cgr = cooperative_groups;
+    __shared__ double lds[kFv2Threads / kWave + 1];
+    if (io.gated && (c->done || !c->acc)) return;
+    cgr::grid_group grid = cgr::this_grid();
+    const int n = P.n, im = P.imax, jm = P.jmax;
+    const int gt = blockIdx.x * kFv2Threads + threadIdx.x, stride = gridDim.x * kFv2Threads;
+    const double h = io.gated ? c->hdid : io.h;
+    const double t = io.gated ? c->t : io.t;
+    const double ds0 = P.ds0;
+    const double cc = P.eta * P.beta * P.phi0;
+    const double e9 = (double)1e-9f, e12 = (double)1e-12f;
+    int slot = 0;
+    // grid-order copies of pf and pfhyd; the Picard trial starts at pf
+#pragma unroll
+    for (int q = 0; q < NPT; q++) {
+        const int l = gt + q * stride;
+        if (l < n) {
+            const int e = A.cell[l];
+            W.pg[l] = io.pf[e];
+            W.pt[l] = io.pf[e];
+            W.hg[l] = A.pfhyd[e];
+        }
+    }
+    int niter = 0;
+    const int kitmax = P.pdep ? 20 : 1;
+    for (int kit = 1; kit <= kitmax; kit++) {
+#pragma unroll
+        for (int q = 0; q < NPT; q++) {
+            const int l = gt + q * stride;
+            if (l < n) {
+                const int e = A.cell[l];
+                const double ks = io.y[4 * (size_t)e + 3];
+                W.cd[l] = P.pdep ? ks * exp(-(io.sigma[e] - W.pt[l]) / P.s0) / cc * 1e-6 : ks / cc * 1e-6;
+            }
+        }
+        grid.sync();
+        // the rows, the right-hand side and the start x = pf - pfhyd (p holds it for r = b - A x)
+        double x[NPT], r[NPT];
+#pragma unroll
+        for (int q = 0; q < NPT; q++) {
+            const int l = gt + q * stride;
+            x[q] = r[q] = 0.0;
+            if (l >= n) continue;
+            const int i = l / jm, j = l % jm;
+            const double *cd = W.cd;
+            double d1, d2, d3, a[6];
+            fv2_line_row(j, jm, P.bct, P.bcb, [&](int k) { return cd[i * jm + k]; }, d1, d2, d3);
+            fv2_line_matrix(j, jm, h, d1 / ds0 / ds0, d2 / ds0 / ds0, d3 / ds0 / ds0, a);
+            fv2_line_row(i, im, P.bcl, P.bcr, [&](int k) { return cd[k * jm + j]; }, d1, d2, d3);
+            fv2_line_matrix(i, im, h, d1 / ds0 / ds0, d2 / ds0 / ds0, d3 / ds0 / ds0, a + 3);
+            double *co = W.co + 6 * (size_t)l;
+            for (int k = 0; k < 6; k++) co[k] = a[k];
+            // penalty vector (1531-1545)
+            const double hy = W.hg[l];
+            double sat = 0.0;
+            if (P.setting == HBI_FV_STEADYSTATE) {
+                if (i == 0) sat = -P.q0 / P.beta / P.phi0 * e9 * h / ds0 * 2;
+                if (i == im - 1) sat = P.q0 / P.beta / P.phi0 * e9 * h / ds0 * 2;
+            }
+            if (P.setting == HBI_FV_THRUST) {
+                const double c0 = cd[i * jm], h0 = W.hg[i * jm];
+                const double cl = cd[i * jm + jm - 1], hl = W.hg[i * jm + jm - 1];
+                if (j == 0) sat = -1.0 * c0 / ds0 / ds0 * (P.pbcl - h0) * h * 2;
+                if (j == 1) sat = -0.5 * c0 / ds0 / ds0 * (P.pbcl - h0) * h * 2;
+                if (P.bcb == HBI_FV_NEUMANN) {
+                    if (j == jm - 1) sat = -P.q0 / P.beta / P.phi0 * e9 * h / ds0 * 2;
+                } else {
+                    if (j == jm - 1) sat = -1.0 * cl / ds0 / ds0 * (P.pbcr - hl) * h * 2;
+                    if (j == jm - 2) sat = -0.5 * cl / ds0 / ds0 * (P.pbcr - hl) * h * 2;
+                }
+            }
+            const double pfl = W.pg[l];
+            double bb = pfl - sat - hy;
+            if (P.setting == HBI_FV_INJECTIONQ) {   // 'injection' (1546-1565)
+                if (P.nwell > 0) {
+                    for (int w = 0; w < P.nwell; w++)
+                        if (A.wi[w] - 1 == i && A.wj[w] - 1 == j)
+                            bb = bb + h * fv2_well_rate(A, w, t) / P.beta / P.phi0 * e12 / ds0 / ds0;
+                } else if (i == im / 2 - 1 && j == jm / 2 - 1) {
+                    bb = bb + h * P.q0 / P.beta / P.phi0 * e12 / ds0 / ds0;
+                }
+            }
+            W.b[l] = bb;
+            x[q] = pfl - hy;
+            W.p[l] = x[q];
+        }
+        grid.sync();
+        double rr = 0.0;
+#pragma unroll
+        for (int q = 0; q < NPT; q++) {
+            const int l = gt + q * stride;
+            if (l < n) {
+                r[q] = W.b[l] - fv2_apply(P, W.co, W.p, l);
+                rr += r[q] * r[q];
+            }
+        }
+        double rsold = fv2_reduce<false>(rr, W.red, slot, lds);   // its barrier ends the reads of p = x
+#pragma unroll
+        for (int q = 0; q < NPT; q++) {
+            const int l = gt + q * stride;
+            if (l < n) W.p[l] = r[q];
+        }
+        const double tol = (double)1e-6f;
+        const double thr = tol * tol * (double)im * (double)jm;
+        const int itermax = 1000;
+        niter = 0;
+        if (!(rsold < thr)) {
+            niter = itermax;
+            for (int it = 1; it <= itermax; it++) {
+                const double tmp1 = rsold;
+                grid.sync();   // p complete
+                double mp = 0.0, m[NPT];
+#pragma unroll
+                for (int q = 0; q < NPT; q++) {
+                    const int l = gt + q * stride;
+                    m[q] = 0.0;
+                    if (l < n) {
+                        m[q] = fv2_apply(P, W.co, W.p, l);
+                        mp += m[q] * W.p[l];
+                    }
+                }
+                const double tmp2 = fv2_reduce<false>(mp, W.red, slot, lds);
+                const double alpha = tmp1 / tmp2;
+                double rs = 0.0;
+#pragma unroll
+                for (int q = 0; q < NPT; q++) {
+                    const int l = gt + q * stride;
+                    if (l < n) {
+                        x[q] = x[q] + alpha * W.p[l];
+                        r[q] = r[q] - alpha * m[q];
+                        rs += r[q] * r[q];
+                    }
+                }
+                const double rsnew = fv2_reduce<false>(rs, W.red, slot, lds);
+                if (rsnew < thr) {
+                    niter = it;
+                    break;
+                }
+#pragma unroll
+                for (int q = 0; q < NPT; q++) {
+                    const int l = gt + q * stride;
+                    if (l < n) W.p[l] = r[q] + (rsnew / rsold) * W.p[l];
+                }
+                rsold = rsnew;
+            }
+        }
+        double err = 0.0;
+#pragma unroll
+        for (int q = 0; q < NPT; q++) {
+            const int l = gt + q * stride;
+            if (l < n) {
+                const double pn = x[q] + W.hg[l];
+                W.pn[l] = pn;
+                err = fmax(err, fabs(W.pt[l] - pn));
+            }
+        }
+        if (!P.pdep) break;
+        err = fv2_reduce<true>(err, W.red, slot, lds);
+        if (err < (double)1e-5f) break;
+#pragma unroll
+        for (int q = 0; q < NPT; q++) {
+            const int l = gt + q * stride;
+            if (l < n) W.pt[l] = W.pn[l];
+        }
+    }
+    // pf <- pfnew, the dt bound on max |dpf| (dpth 0.1) and the well schedule
+    double amax = 0.0;
+#pragma unroll
+    for (int q = 0; q < NPT; q++) {
+        const int l = gt + q * stride;
+        if (l < n) {
+            const double d = W.pn[l] - W.pg[l];
+            amax = fmax(amax, fabs(d));
+            io.pfnew[A.cell[l]] = W.pn[l];
+        }
+    }
+    amax = fv2_reduce<true>(amax, W.red, slot, lds);
+    if (gt == 0) {
+        double *dtnxt = io.gated ? &c->h : io.dtnxt;
+        double dt = *dtnxt;
+        const double dpth = (double)0.1f;
+        if (dt / h * amax > dpth) dt = dpth * h / amax;
+        if (!P.pdep) {
+            if (c->sw) {
+                dt = 20.0;
+                c->sw = 0;
+            }
+            if (P.nwell > 0 && c->nn <= A.wk[0]) {
+                const double tq = A.wt[c->nn - 1];
+                if (t + dt > tq) {
+                    dt = tq - t - 1.0;
+                    c->sw = 1;
+                    c->nn += 2;
+                }
+            }
+        }
+        *dtnxt = dt;
+        *io.niter = niter;
+    }
+}
+
 // sigma_e = sigma - pf, displacement (trapezoid over the post-RK and the final
 // V), mu, kp, qflow and the monitor partials (main_fv.f90:1045-1082, 1083-1134)
 __global__ __launch_bounds__(kFvThreads) void fv_fields_kernel(FvPar P, const FvCtl *__restrict__ c, FvArrays A)
@@ -542,10 +897,13 @@ __global__ __launch_bounds__(kFvThreads) void fv_fields_kernel(FvPar P, const Fv
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = P.n;
     const double dt = c->hdid;
-    double m[kFvMon] = {-1.0, 0.0, 0.0, -HUGE_VAL, HUGE_VAL, -1.0};
+    double m[kFvMon] = {P.grid ? -HUGE_VAL : -1.0, 0.0, 0.0, -HUGE_VAL, HUGE_VAL, -1.0};
     if (i < n) {
         const size_t e = 4 * (size_t)i;
         const double psi = A.y[e], tau = A.y[e + 1], ks = A.y[e + 3];
+        // grid mode: where the RK's sigma_e fell below minsig, sigma = pf + minsig
+        // (main_fv3dp.f90:765-770: the test reads sigma_e before the pressure step)
+        if (P.grid && A.y[e + 2] < P.minsig) A.sigma[i] = A.pf[i] + P.minsig;
         const double sige = A.sigma[i] - A.pf[i];
         A.y[e + 2] = sige;
         double d = A.disp[i];
@@ -555,15 +913,18 @@ __global__ __launch_bounds__(kFvThreads) void fv_fields_kernel(FvPar P, const Fv
         A.disp[i] = d;
         A.vel[i] = v;
         const double mu = tau / sige;
-        // qflow with kp of this element and pf of the neighbours (final pf)
-        const double kp = P.pdep ? ks * exp(-sige / P.s0) : ks;
-        const double *pf = A.pf, *ph = A.pfhyd;
-        double q;
-        if (i == 0) q = -kp / P.eta * 1e3 * (pf[1] - ph[1] - pf[0] + ph[0]) / P.ds0;
-        else if (i == n - 1) q = -kp / P.eta * 1e3 * (pf[n - 1] - ph[n - 1] - pf[n - 2] + ph[n - 2]) / P.ds0;
-        else q = -kp / P.eta * 1e3 * (pf[i + 1] - ph[i + 1] - pf[i - 1] + ph[i - 1]) / 2 / P.ds0;
-        A.qflow[i] = q;
-        m[0] = fabs(v);
+        if (!P.grid) {
+            // qflow with kp of this element and pf of the neighbours (final pf);
+            // main_fv3dp keeps qflow = q0
+            const double kp = P.pdep ? ks * exp(-sige / P.s0) : ks;
+            const double *pf = A.pf, *ph = A.pfhyd;
+            double q;
+            if (i == 0) q = -kp / P.eta * 1e3 * (pf[1] - ph[1] - pf[0] + ph[0]) / P.ds0;
+            else if (i == n - 1) q = -kp / P.eta * 1e3 * (pf[n - 1] - ph[n - 1] - pf[n - 2] + ph[n - 2]) / P.ds0;
+            else q = -kp / P.eta * 1e3 * (pf[i + 1] - ph[i + 1] - pf[i - 1] + ph[i - 1]) / 2 / P.ds0;
+            A.qflow[i] = q;
+        }
+        m[0] = P.grid ? v : fabs(v);   // main_fv3dp's maxval(vel); main_fv's max |V|
         m[1] = d * A.area[i];
         m[2] = mu * A.area[i];
         m[3] = sige;
@@ -629,7 +990,7 @@ __global__ void fv_monitor_kernel(FvPar P, FvCtl *c, FvArrays A, int nblk)
             const double sige = yi[2], tau = yi[1], ks = yi[3];
             double *pr = A.probe + ((size_t)row * A.nprobe + q) * kFvProbe;
             pr[0] = A.vel[i]; pr[1] = A.disp[i]; pr[2] = sige; pr[3] = tau; pr[4] = A.pf[i]; pr[5] = A.sigma[i];
-            pr[6] = tau / sige; pr[7] = yi[0]; pr[8] = P.pdep ? ks * exp(-sige / P.s0) : ks;
+            pr[6] = tau / sige; pr[7] = yi[0]; pr[8] = P.pdep || P.grid ? ks * exp(-sige / P.s0) : ks;
         }
     // stop controls, main_fv.f90:1136-1152
     if (vm > P.velmax || vm < P.velmin) c->status = 1;
@@ -656,7 +1017,13 @@ struct hbi_fv {
     hbi::DevBuf<int> probe_idx;
     long long probe_rows = 0;
     hbi::DevBuf<hbi::FvCtl> ctl;
+    // grid mode: the 2-D solve's work arrays, grid -> cluster map, wells, launch shape
+    hbi::DevBuf<double> g2;
+    hbi::DevBuf<int> cell, wi, wj, wk;
+    hbi::DevBuf<double> wq, wt;
+    int g2_blocks = 0, g2_npt = 0;
     hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph2 = nullptr;   // grid mode: fields and monitor (after the cooperative pressure launch)
     double *graph_hist = nullptr, *graph_probe = nullptr;
     bool dirty = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -678,11 +1045,61 @@ static FvArrays fv_arrays(hbi_fv *f)
     A.probe = f->probe.n ? f->probe.p : nullptr;
     A.probe_idx = f->probe_idx.p;
     A.nprobe = (int)f->probe_idx.n;
+    A.cell = f->cell.p;
+    A.wi = f->wi.p; A.wj = f->wj.p; A.wk = f->wk.p; A.wq = f->wq.p; A.wt = f->wt.p;
     return A;
+}
+
+static Fv2Work fv2_work(hbi_fv *f)
+{
+    const size_t n = f->n;
+    double *g = f->g2.p;
+    return Fv2Work{g, g + n, g + 7 * n, g + 8 * n, g + 9 * n, g + 10 * n, g + 11 * n, g + 12 * n, g + 13 * n};
+}
+
+// cells per thread and workgroups of the cooperative 2-D solve: the fewest
+// cells per thread whose grid fits on the device at once
+static void fv2_shape(hbi_fv *f)
+{
+    int dev = 0, cus = 0;
+    HBI_HIP(hipGetDevice(&dev));
+    HBI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const void *fn[5] = {(const void *)fv2_pressure_kernel<1>, (const void *)fv2_pressure_kernel<2>,
+                         (const void *)fv2_pressure_kernel<4>, (const void *)fv2_pressure_kernel<8>,
+                         (const void *)fv2_pressure_kernel<16>};
+    for (int k = 0; k < 5; k++) {
+        const int npt = 1 << k;
+        int per = 0;
+        HBI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn[k], kFv2Threads, 0));
+        const long long need = ((long long)f->n + (long long)kFv2Threads * npt - 1) / ((long long)kFv2Threads * npt);
+        if (need <= (long long)per * cus) {
+            f->g2_npt = npt;
+            f->g2_blocks = (int)need;
+            return;
+        }
+    }
+    throw Error{HBI_E_ARG, "pore-pressure grid too large for one cooperative launch"};
+}
+
+static void fv2_launch(hbi_fv *f, FvArrays A, FvSolveIO io)
+{
+    FvCtl *c = f->ctl.p;
+    Fv2Work W = fv2_work(f);
+    void *args[] = {&f->P, &c, &A, &io, &W};
+    const void *fn = f->g2_npt == 1 ? (const void *)fv2_pressure_kernel<1>
+                   : f->g2_npt == 2 ? (const void *)fv2_pressure_kernel<2>
+                   : f->g2_npt == 4 ? (const void *)fv2_pressure_kernel<4>
+                   : f->g2_npt == 8 ? (const void *)fv2_pressure_kernel<8>
+                                    : (const void *)fv2_pressure_kernel<16>;
+    HBI_HIP(hipLaunchCooperativeKernel(fn, dim3(f->g2_blocks), dim3(kFv2Threads), args, 0, f->stream));
 }
 
 static void fv_launch_pressure(hbi_fv *f, FvArrays A, const FvSolveIO &io)
 {
+    if (f->P.grid) {
+        fv2_launch(f, A, io);
+        return;
+    }
     FvCtl *c = f->ctl.p;
     // elements per thread: registers hold x, r and A p of NPT elements, so
     // 2049-4096 elements take 512-thread workgroups (256 VGPRs per lane); up
@@ -711,7 +1128,9 @@ static void fv_launch_stages(hbi_fv *f, int nstages, int mode, const int *gate)
     }
 }
 
-static void fv_launch_attempt(hbi_fv *f)
+// an attempt: part 1 (stages, decide, commit), the pressure step, part 2
+// (fields, monitor)
+static void fv_launch_part1(hbi_fv *f)
 {
     const int g = f->nblk;
     FvArrays A = fv_arrays(f);
@@ -719,31 +1138,67 @@ static void fv_launch_attempt(hbi_fv *f)
     fv_launch_stages(f, 6, 0, &c->done);
     fv_decide_kernel<<<1, 64, 0, f->stream>>>(f->P, c, f->errpart.p, g);
     fv_commit_kernel<<<g, kFvThreads, 0, f->stream>>>(f->P, c, A);
-    if (!f->P.pfconst) {
-        FvSolveIO io{};
-        io.pf = f->pf.p; io.y = f->y.p; io.sigma = f->sigma.p; io.pfnew = f->pf.p; io.rows = f->rows.p;
-        io.work = f->work.p;
-        io.niter = &c->niter; io.gated = 1;
-        fv_launch_pressure(f, A, io);
-    }
+}
+
+static void fv_launch_gated_pressure(hbi_fv *f)
+{
+    if (f->P.pfconst) return;
+    FvCtl *c = f->ctl.p;
+    FvSolveIO io{};
+    io.pf = f->pf.p; io.y = f->y.p; io.sigma = f->sigma.p; io.pfnew = f->pf.p; io.rows = f->rows.p;
+    io.work = f->work.p;
+    io.niter = &c->niter; io.gated = 1;
+    fv_launch_pressure(f, fv_arrays(f), io);
+}
+
+static void fv_launch_part2(hbi_fv *f)
+{
+    const int g = f->nblk;
+    FvArrays A = fv_arrays(f);
+    FvCtl *c = f->ctl.p;
     fv_fields_kernel<<<g, kFvThreads, 0, f->stream>>>(f->P, c, A);
     fv_monitor_kernel<<<1, 64, 0, f->stream>>>(f->P, c, A, g);
 }
 
-static void fv_capture(hbi_fv *f)
+static void fv_capture_one(hbi_fv *f, hipGraphExec_t *out, int part)
 {
-    if (f->graph) {
-        (void)hipGraphExecDestroy(f->graph);
-        f->graph = nullptr;
+    if (*out) {
+        (void)hipGraphExecDestroy(*out);
+        *out = nullptr;
     }
     hipGraph_t gr;
     HBI_HIP(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
-    fv_launch_attempt(f);
+    if (part != 2) fv_launch_part1(f);
+    if (part == 0) fv_launch_gated_pressure(f);
+    if (part != 1) fv_launch_part2(f);
     HBI_HIP(hipStreamEndCapture(f->stream, &gr));
-    HBI_HIP(hipGraphInstantiate(&f->graph, gr, nullptr, nullptr, 0));
+    HBI_HIP(hipGraphInstantiate(out, gr, nullptr, nullptr, 0));
     (void)hipGraphDestroy(gr);
+}
+
+// 1-D: the whole attempt as one graph.  Grid mode: the cooperative pressure
+// launch stays outside the graphs (launched between them on the same
+// stream, gated on the device like the graphs' kernels), so no graph node
+// needs the cooperative launch mode.
+static void fv_capture(hbi_fv *f)
+{
+    if (f->P.grid) {
+        fv_capture_one(f, &f->graph, 1);
+        fv_capture_one(f, &f->graph2, 2);
+    } else {
+        fv_capture_one(f, &f->graph, 0);
+    }
     f->graph_hist = f->hist.p;
     f->graph_probe = f->probe.p;
+}
+
+static void fv_launch_graphs(hbi_fv *f)
+{
+    HBI_HIP(hipGraphLaunch(f->graph, f->stream));
+    if (f->P.grid) {
+        fv_launch_gated_pressure(f);
+        HBI_HIP(hipGraphLaunch(f->graph2, f->stream));
+    }
 }
 
 }  // namespace hbi
@@ -765,8 +1220,20 @@ int hbi_fv_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_fv_opts *o, const double
                     "bad boundary condition");
         HBI_REQUIRE(o->setting >= HBI_FV_INJECTIONQ && o->setting <= HBI_FV_OTHER, "bad setting");
         const int n = hs->n;
-        HBI_REQUIRE(n >= 4 && n <= HBI_FV_MAX_CELLS, "the pore-pressure grid needs 4 to 8192 elements");
-        HBI_REQUIRE(o->setting != HBI_FV_THRUST || n >= 2000, "setting thrust reads qin(2000): needs N >= 2000");
+        const bool grid = o->imax > 0 || o->jmax > 0;
+        if (grid) {
+            HBI_REQUIRE(o->imax >= 4 && o->jmax >= 4 && (long long)o->imax * o->jmax == n,
+                        "grid mode: imax, jmax >= 4 and imax jmax = N");
+            HBI_REQUIRE((o->bct == HBI_FV_NEUMANN || o->bct == HBI_FV_DIRICHLET) &&
+                            (o->bcb == HBI_FV_NEUMANN || o->bcb == HBI_FV_DIRICHLET),
+                        "bad boundary condition");
+            HBI_REQUIRE(o->dtmax > 0 && o->phi0 > 0, "grid mode: dtmax and phi0 must be > 0");
+            HBI_REQUIRE(o->setting != HBI_FV_INJECTIONP, "grid mode: no injection-pressure setting in main_fv3dp");
+            HBI_REQUIRE((int)hs->part.lod.size() == n, "matrix without its cluster permutation");
+        } else {
+            HBI_REQUIRE(n >= 4 && n <= HBI_FV_MAX_CELLS, "the pore-pressure grid needs 4 to 8192 elements");
+            HBI_REQUIRE(o->setting != HBI_FV_THRUST || n >= 2000, "setting thrust reads qin(2000): needs N >= 2000");
+        }
         *out = nullptr;
         hbi_fv *f = new hbi_fv();
         try {
@@ -786,6 +1253,8 @@ int hbi_fv_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_fv_opts *o, const double
             P.s0 = o->s0; P.eta = o->eta; P.beta = o->beta; P.Bs = o->Bs; P.pbcl = o->pbcl; P.pbcr = o->pbcr;
             P.q0 = o->q0; P.pinj = o->pinj; P.tinj = o->tinj_s; P.dtout = o->dtout_s;
             P.velmin = o->velmin; P.velmax = o->velmax; P.tmax = o->tmax_s;
+            P.grid = grid ? 1 : 0; P.imax = o->imax; P.jmax = o->jmax; P.bct = o->bct; P.bcb = o->bcb;
+            P.nwell = 0; P.dtmax = o->dtmax; P.minsig = o->minsig; P.phi0 = o->phi0;
             P.T = hbi::rk_tableau();
             double sg = 0.0;
             for (int i = 0; i < n; i++) sg = sg + area[i];
@@ -819,9 +1288,17 @@ int hbi_fv_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_fv_opts *o, const double
             f->io_niter.alloc(1);
             f->zero.alloc(1);
             HBI_HIP(hipMemsetAsync(f->zero.p, 0, sizeof(int), s));
+            if (grid) {
+                std::vector<int> cell(n);
+                for (int e = 0; e < n; e++) cell[hs->part.lod[e]] = e;
+                f->cell.upload(cell, s);
+                hbi::fv2_shape(f);
+                f->g2.alloc(13 * (size_t)n + 2 * (size_t)f->g2_blocks);
+            }
             f->ctl.alloc(1);
             hbi::FvCtl c{};
             c.h = 1.0;
+            c.nn = 2;
             hbi::to_device_sync(f->ctl.p, &c, sizeof(c), s);
         } catch (...) {
             hbi_fv_destroy(f);
@@ -836,6 +1313,7 @@ int hbi_fv_destroy(hbi_fv *f)
     if (!f) return HBI_OK;
     if (f->stream) (void)hipStreamSynchronize(f->stream);
     if (f->graph) (void)hipGraphExecDestroy(f->graph);
+    if (f->graph2) (void)hipGraphExecDestroy(f->graph2);
     if (f->e0) (void)hipEventDestroy(f->e0);
     if (f->e1) (void)hipEventDestroy(f->e1);
     hipStream_t s = f->stream;
@@ -854,6 +1332,7 @@ int hbi_fv_set_state(hbi_fv *f, const double *y, const double *pf, const double 
         if (disp) f->disp.upload(disp, n, f->stream);
         else HBI_HIP(hipMemsetAsync(f->disp.p, 0, n * 8, f->stream));
         hbi::FvCtl c{};
+        c.nn = 2;
         hbi::to_device_sync(f->ctl.p, &c, sizeof(c), f->stream);
         // V of the state (main_fv.f90:995-1003)
         hbi::fv_stage0_kernel<<<f->nblk, hbi::kFvThreads, 0, f->stream>>>(f->P, f->zero.p, hbi::fv_arrays(f), 0);
@@ -861,7 +1340,42 @@ int hbi_fv_set_state(hbi_fv *f, const double *y, const double *pf, const double 
         c.h = dt_next;
         c.t = t;
         hbi::to_device_sync(f->ctl.p, &c, sizeof(c), f->stream);
+        // grid mode: sigma (total, held through the steps) = sigma_e + pf of the state
+        if (f->P.grid) {
+            std::vector<double> sg(n);
+            for (int i = 0; i < n; i++) sg[i] = y[4 * (size_t)i + 2] + pf[i];
+            f->sigma.upload(sg.data(), n, f->stream);
+        }
         HBI_HIP(hipStreamSynchronize(f->stream));
+    });
+}
+
+int hbi_fv_set_wells(hbi_fv *f, int nwell, const int *iw, const int *jw, const int *kleng, const double *qvals,
+                     const double *qtimes)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(f && nwell >= 0 && (nwell == 0 || (iw && jw && kleng && qvals && qtimes)), "bad arguments");
+        HBI_REQUIRE(f->P.grid || nwell == 0, "wells need the 2-D grid mode");
+        for (int w = 0; w < nwell; w++) {
+            HBI_REQUIRE(iw[w] >= 1 && iw[w] <= f->P.imax && jw[w] >= 1 && jw[w] <= f->P.jmax, "well outside the grid");
+            HBI_REQUIRE(kleng[w] >= 1 && kleng[w] <= HBI_FV_MAX_SCHEDULE, "schedule length 1..50");
+        }
+        HBI_HIP(hipStreamSynchronize(f->stream));
+        if (nwell) {
+            f->wi.upload(iw, nwell, f->stream);
+            f->wj.upload(jw, nwell, f->stream);
+            f->wk.upload(kleng, nwell, f->stream);
+            f->wq.upload(qvals, (size_t)nwell * HBI_FV_MAX_SCHEDULE, f->stream);
+            f->wt.upload(qtimes, (size_t)nwell * HBI_FV_MAX_SCHEDULE, f->stream);
+        }
+        f->P.nwell = nwell;
+        hbi::FvCtl c;
+        HBI_HIP(hipMemcpy(&c, f->ctl.p, sizeof(c), hipMemcpyDeviceToHost));
+        c.sw = 0;
+        c.nn = 2;
+        hbi::to_device_sync(f->ctl.p, &c, sizeof(c), f->stream);
+        HBI_HIP(hipStreamSynchronize(f->stream));
+        f->dirty = true;
     });
 }
 
@@ -949,7 +1463,7 @@ int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info)
         const long long budget = 50LL * nsteps + 100;
         int batch = nsteps;
         for (;;) {
-            for (int q = 0; q < batch; q++) HBI_HIP(hipGraphLaunch(f->graph, f->stream));
+            for (int q = 0; q < batch; q++) hbi::fv_launch_graphs(f);
             launched += batch;
             HBI_HIP(hipMemcpyAsync(&c, f->ctl.p, sizeof(c), hipMemcpyDeviceToHost, f->stream));
             HBI_HIP(hipStreamSynchronize(f->stream));

@@ -649,6 +649,7 @@ __global__ void mr_tile_reduce_kernel(int n, int tr, const int *__restrict__ par
 // one barrier per slice.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+constexpr int kBfBufs = 2;                // LDS buffers of the B slices
 constexpr int kBfStages = 2;              // register stages of the load ring (2: the fewest registers; with
                                           // more, narrower items in flight per CU beat deeper prefetch)
 constexpr int kBfRW = 20;                 // LDS words per right-hand-side row of a B plane
@@ -684,7 +685,7 @@ __device__ __forceinline__ void bf_split3(float a, float b, uint32_t &h, uint32_
 }
 __device__ __forceinline__ bf16x8 as_bf8(uvec4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-template <int NW, bool P2, int NS>
+template <int NW, bool P2, int NS, int NB>
 __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restrict__ items2,
                                                           const MItem1 *__restrict__ items1,
                                                           const int *__restrict__ cmap,
@@ -697,7 +698,7 @@ __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restr
 {
     constexpr int NT = 64 * NW;
     constexpr int PP = (256 + NT - 1) / NT;   // B pairs (two columns x four right-hand sides) per thread
-    __shared__ __align__(16) uint32_t bs[2][3][kBfPlane];
+    __shared__ __align__(16) uint32_t bs[NB][3][kBfPlane];
     extern __shared__ int cm[];   // phase 2: the item's column map (dynamic: the plan's widest item)
     if (gate && *gate) return;
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
@@ -846,14 +847,15 @@ __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restr
     for (int b = 0; b < NS - 1; b++) load(st[b], b);
     bstore(st[0], 0);
     __syncthreads();
-    // slice sl: stage sl % NS (i is a multiple of NS), LDS buffer sl % 2
+    // slice sl: stage sl % NS (i is a multiple of NS), LDS buffer sl % NB
     for (int i = 0; i < nsl; i += NS) {
 #pragma unroll
         for (int b = 0; b < NS; b++) {
             const int sl = i + b;
             load(st[(b + NS - 1) % NS], sl + NS - 1);
-            if (sl < nsl) multiply(sl & 1, st[b]);
-            bstore(st[(b + 1) % NS], (sl + 1) & 1);
+            if (sl < nsl) multiply(NB == 2 ? (sl & 1) : 0, st[b]);
+            if constexpr (NB == 1) __syncthreads();   // one buffer: every wave has read it
+            bstore(st[(b + 1) % NS], NB == 2 ? ((sl + 1) & 1) : 0);
             __syncthreads();
             if (sl & 1) flush();
         }
@@ -1199,7 +1201,7 @@ static void phase1(hbi_hmat *h, const Plan &P, const MrPlan &M, const double *X,
     } else if (M.bf) {
         float *Z32 = (float *)Z;
         // every item fits 4 waves (kP1RowsBf rows)
-        mr_bf16_kernel<4, false, kBfStages><<<(int)M.it1.size(), 256, 0, s>>>(nullptr, M.d_it1.p, nullptr, nullptr,
+        mr_bf16_kernel<4, false, kBfStages, kBfBufs><<<(int)M.it1.size(), 256, 0, s>>>(nullptr, M.d_it1.p, nullptr, nullptr,
                                                                             P.buf.p, X32, nullptr, zpart, P.d_zrow.p,
                                                                             Z32, M.p1rows, gate);
         if (!M.blk1.empty())
@@ -1225,10 +1227,10 @@ static void phase2(const hbi_hmat *h, const Plan &P, const MrPlan &M, const doub
         if (n == 0) return;
         const size_t cmb = (size_t)M.ncol2max * sizeof(int);
         if (h->tr == 64)
-            mr_bf16_kernel<4, true, kBfStages><<<n, 256, cmb, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr, X32,
+            mr_bf16_kernel<4, true, kBfStages, kBfBufs><<<n, 256, cmb, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr, X32,
                                                                  Z32, ypart, nullptr, nullptr, 0, gate);
         else
-            mr_bf16_kernel<2, true, kBfStages><<<n, 128, cmb, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr, X32,
+            mr_bf16_kernel<2, true, kBfStages, kBfBufs><<<n, 128, cmb, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr, X32,
                                                                  Z32, ypart, nullptr, nullptr, 0, gate);
         return;
     }

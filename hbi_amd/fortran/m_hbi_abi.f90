@@ -47,6 +47,8 @@ module m_hbi_abi
     real(c_double) :: vref = 1d-6, eps_r = 1d-4, hdiv = 0d0, ds0 = 0d0, kL = 1d8, kpmin = 0d0, s0 = 0d0
     real(c_double) :: eta = 0d0, beta = 0d0, Bs = 0d0, pbcl = 0d0, pbcr = 0d0, q0 = 0d0, pinj = 0d0
     real(c_double) :: tinj_s = 0d0, dtout_s = 0d0, velmin = 1d-16, velmax = 1d7, tmax_s = 1d12
+    integer(c_int) :: imax = 0, jmax = 0, bct = 0, bcb = 0
+    real(c_double) :: dtmax = 1d10, minsig = 0d0, phi0 = 0d0
   end type hbi_fv_opts
 
   type, bind(C) :: hbi_fv_info
@@ -296,6 +298,13 @@ module m_hbi_abi
       type(c_ptr), value :: f
       real(c_double), intent(out) :: y(*), pf(*), disp(*), vel(*), qflow(*)
       type(hbi_fv_info), intent(out) :: info
+    end function
+    integer(c_int) function hbi_fv_set_wells(f, nwell, iw, jw, kleng, qvals, qtimes) bind(C, name="hbi_fv_set_wells")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: f
+      integer(c_int), value :: nwell
+      integer(c_int), intent(in) :: iw(*), jw(*), kleng(*)
+      real(c_double), intent(in) :: qvals(*), qtimes(*)
     end function
     integer(c_int) function hbi_fv_set_probes(f, nprobe, idx) bind(C, name="hbi_fv_set_probes")
       import :: c_int, c_ptr

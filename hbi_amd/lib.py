@@ -90,7 +90,9 @@ class FvOpts(C.Structure):
                 ("pfconst", C.c_int), ("bcl", C.c_int), ("bcr", C.c_int), ("setting", C.c_int),
                 ("outpertime", C.c_int)] + [(k, C.c_double) for k in (
                     "vref", "eps_r", "hdiv", "ds0", "kL", "kpmin", "s0", "eta", "beta", "Bs", "pbcl", "pbcr", "q0",
-                    "pinj", "tinj_s", "dtout_s", "velmin", "velmax", "tmax_s")]
+                    "pinj", "tinj_s", "dtout_s", "velmin", "velmax", "tmax_s")] + [
+                    (k, C.c_int) for k in ("imax", "jmax", "bct", "bcb")] + [
+                    (k, C.c_double) for k in ("dtmax", "minsig", "phi0")]
 
 
 class FvInfo(C.Structure):
@@ -120,7 +122,7 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_ens_set_probes", "hbi_ens_set_step_grid", "hbi_rk_step_many", "hbi_rk_set_output", "hbi_rk_get_output",
            "hbi_rk_set_opening", "hbi_rk_get_opening", "hbi_rk_set_probes", "hbi_rk_get_probes",
            "hbi_fv_create", "hbi_fv_destroy", "hbi_fv_set_state", "hbi_fv_get_state", "hbi_fv_derivs",
-           "hbi_fv_diffuse", "hbi_fv_step", "hbi_fv_last_ms", "hbi_fv_set_probes", "hbi_fv_get_probes")
+           "hbi_fv_diffuse", "hbi_fv_step", "hbi_fv_last_ms", "hbi_fv_set_probes", "hbi_fv_get_probes", "hbi_fv_set_wells")
 
 
 def lib():
@@ -194,6 +196,7 @@ def lib():
         L.hbi_fv_last_ms.argtypes = [C.c_void_p, _dp]
         L.hbi_fv_set_probes.argtypes = [C.c_void_p, C.c_int, _ip]
         L.hbi_fv_get_probes.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), _dp]
+        L.hbi_fv_set_wells.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp]
         L.hbi_rk_get_output.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), _dp, _dp,
                                         C.POINTER(C.c_int), C.POINTER(RkEvent), _dp]
         L.hbi_ens_get_trace.argtypes = [C.c_void_p, _dp, _dp, _ip]
@@ -774,3 +777,24 @@ class FV:
         if out.size:
             _check(lib().hbi_fv_get_probes(self.f, C.byref(nr), _p(out)))
         return out
+
+    def set_wells(self, wells):
+        """Grid mode: wells (i, j, rates, times) with 1-based grid indices and
+        schedules of up to 50 entries (times in seconds); [] or None: the
+        centre source."""
+        wells = list(wells or [])
+        nw = len(wells)
+        S = 50
+        iw = np.array([w[0] for w in wells], dtype=np.int32)
+        jw = np.array([w[1] for w in wells], dtype=np.int32)
+        kl = np.array([len(w[2]) for w in wells], dtype=np.int32)
+        qv = np.zeros((max(nw, 1), S))
+        qt = np.zeros((max(nw, 1), S))
+        for k, w in enumerate(wells):
+            if len(w[2]) != len(w[3]) or not 1 <= len(w[2]) <= S:
+                raise ValueError("a well needs 1..50 rates with as many times")
+            qv[k, :len(w[2])] = w[2]
+            qt[k, :len(w[3])] = w[3]
+        ptr = lambda a: a.ctypes.data_as(_ip) if nw else None
+        _check(lib().hbi_fv_set_wells(self.f, nw, ptr(iw), ptr(jw), ptr(kl), _p(qv) if nw else None,
+                                      _p(qt) if nw else None))

@@ -519,3 +519,51 @@ def fluid_bp6a(g: Geometry, lod):
                            sigmainit=50.0, tauinit=29.2, muinit=0.0, phi0=0.1, beta=1e-8, eta=1e-3, kpmin=1e-13,
                            kpmax=1e-13, q0=1.25e-6, tinj=0.27397, tmax=2.0, backslip=False, vpl=0.0, eps_r=1e-4,
                            kT=float(np.float32(1e12)))
+
+
+def fluid3dp_injection(imax: int, jmax: int, ds0: float, a=0.013, b=0.0, dc=1e-4, f0=0.6, vref=1e-6, vpl=0.0,
+                       velinit=1e-12, sigmainit=30.0, tauinit=0.0, muinit=0.55, phi0=0.2, beta=5e-8, eta=0.89e-3,
+                       kpmin=1e-14, kpmax=3e-14, kL=1e-3, kT=1e15, s0=50.0, q0=3e-4, bc="Dirichlet", pbc=0.0,
+                       law=0, pressure_dependent=False, pfconst=False, eps_r=1e-4, backslip=True, prec=0,
+                       velmin=1e-16, velmax=1e7, tmax=5.0, dtmax=1e6, minsig=0.0, lod=None):
+    """A main_fv3dp 'injection' run on a 3dp fault of imax x jmax squares of
+    ds0 km (the reference's examples/3dinjection2.in: uniform a 0.013, b 0, Dc
+    0.1 mm, mu 0.55 on sigma 30 MPa, V 1e-12 m/s, permeability 1e-14..3e-14,
+    Dirichlet boundaries at pf 0, dtmax 1e6 s): geometry, options, per-element
+    arrays (cluster order, area 1 so that the monitor means are plain means
+    as main_fv3dp.f90:795-799) and the initial state (y, pf) of
+    initcond_injection (main_fv3dp.f90:1020-1061).  lod: the matrix's cluster
+    permutation (identity if None)."""
+    g = mesh_3dp(imax, jmax, ds0)
+    n = g.n
+    lod = np.arange(n) if lod is None else np.asarray(lod)
+    one = np.ones(n)
+    arrays = dict(a=a * one, b=b * one, dc=dc * one, f0=f0 * one, vpl=vpl * one, kpmaxv=kpmax * one, kTv=kT * one,
+                  phi=phi0 * one, pfhyd=np.zeros(n), qin=np.zeros(n), area=one.copy())
+    pf = np.zeros(n)
+    sige = sigmainit * one - pf
+    tau = tauinit * one
+    if muinit != 0.0:
+        tau = sige * muinit
+    vel = velinit * one
+    psi = arrays["a"] * np.log(2 * vref / vel * np.sinh(tau / sige / arrays["a"]))
+    y = np.empty(4 * n)
+    y[0::4], y[1::4], y[2::4], y[3::4] = psi, tau, sige, kpmin * one
+    bcv = 0 if bc == "Neumann" else 1
+    opts = dict(law=law, backslip=int(backslip), prec=prec, pressure_dependent=int(pressure_dependent),
+                pfconst=int(pfconst), bcl=bcv, bcr=bcv, setting=0, outpertime=0, vref=vref, eps_r=eps_r, hdiv=0.0,
+                ds0=float(ds0), kL=kL, kpmin=kpmin, s0=s0, eta=eta, beta=beta, Bs=0.0, pbcl=pbc, pbcr=pbc, q0=q0,
+                pinj=0.0, tinj_s=0.0, dtout_s=0.0, velmin=velmin, velmax=velmax, tmax_s=tmax * 365 * 24 * 3600,
+                imax=imax, jmax=jmax, bct=bcv, bcb=bcv, dtmax=dtmax, minsig=minsig, phi0=phi0)
+    return g, opts, arrays, y, pf
+
+
+# examples/wells.dat of the reference (input_well, main_fv3dp.f90:1101-1118): three wells on the
+# 100 x 100 grid of examples/3dinjection2.in with 8-entry rate schedules (m/s) at times (s); grid
+# indices 1-based
+_T8 = [0.0, 1e7, 1e7, 2e7, 2e7, 3e7, 3e7, 99e7]
+WELLS_3DINJ2 = [
+    (25, 50, [3e-4, 3e-4, 6e-4, 6e-4, 3e-4, 3e-4, 0.0, 0.0], _T8),
+    (50, 50, [3e-4, 3e-4, 0.0, 0.0, 3e-4, 3e-4, 0.0, 0.0], _T8),
+    (75, 50, [3e-4, 3e-4, 0.0, 0.0, 3e-4, 3e-4, 0.0, 0.0], _T8),
+]

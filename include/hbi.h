@@ -406,8 +406,20 @@ int hbi_ens_last_ms(const hbi_ens *e, double *ms);
        10000 iterations); the next step size is limited by dpf;
      field update (1045-1082: sigma_e = sigma - pf, disp trapezoid, V, mu, kp,
        qflow) and the stop criteria (1136-1152).
-   The pore-pressure grid holds at most 8192 elements (one workgroup solves
-   it, the vectors in registers and LDS). */
+   The 1-D pore-pressure grid holds at most 8192 elements (one workgroup
+   solves it, the vectors in registers and LDS).
+
+   With imax, jmax > 0 (main_fv3dp.f90): the same cycle on a 3dp fault with
+   the pore pressure on the fault's imax x jmax grid -- Beuler of
+   main_fv3dp.f90:1426-1649 (finite-volume operators along j and i with their
+   own boundaries, the penalties of the 'ss' / 'thrust' settings, the
+   'injection' source at the centre or at wells on rate schedules), solved by
+   conjugate gradients on all CUs (one cooperative launch, grid-wide
+   reductions) to |r|^2 < tol^2 imax jmax (tol = 1e-6, at most 1000
+   iterations); implicitsolver (1314-1366) / implicitsolver2 (1368-1424) with
+   dpth 0.1 on max |dpf| and the schedule's step bound; the RK step bounded
+   by dtmax; the minsig rule and fields of 762-790; monitor values of
+   795-799 (max V, mean disp and mu over elements: pass area = 1). */
 #define HBI_FV_NEUMANN 0
 #define HBI_FV_DIRICHLET 1
 #define HBI_FV_INJECTIONQ 0
@@ -435,7 +447,17 @@ typedef struct hbi_fv_opts {
     double tinj_s;           /* injectionq: source active while t < tinj_s (seconds) */
     double dtout_s;          /* outpertime bound, seconds */
     double velmin, velmax, tmax_s;   /* stop criteria */
+    /* main_fv3dp (imax, jmax > 0): the fault is a 3dp mesh of imax x jmax
+       squares and the pore pressure diffuses on that 2-D grid (grid cell (i, j),
+       0-based, is mesh element i jmax + j, coordinate3dp of
+       main_fv3dp.f90:1174-1202); 0, 0: main_fv's 1-D grid */
+    int imax, jmax;
+    int bct, bcb;            /* grid: boundaries of j = 0 (top) and j = jmax - 1 (bottom); bcl / bcr are i = 0 / imax - 1 */
+    double dtmax;            /* grid: hnext <= dtmax (main_fv3dp.f90:1820) */
+    double minsig;           /* grid: where the RK's sigma_e < minsig, sigma = pf + minsig (main_fv3dp.f90:765-770) */
+    double phi0;             /* grid: porosity of the diffusivity, penalties and sources */
 } hbi_fv_opts;
+#define HBI_FV_MAX_SCHEDULE 50   /* rate-schedule entries per well (main_fv3dp.f90:1107) */
 
 typedef struct hbi_fv_info {
     double t, dt_next, dt_last, errmax_last;
@@ -484,6 +506,15 @@ int hbi_fv_last_ms(const hbi_fv *f, double *ms);
    sigma, mu, psi, kp.  hbi_fv_get_probes gives the last hbi_fv_step call's
    rows (*nrows x nprobe x 9, step-major; rows NULL: count only). */
 int hbi_fv_set_probes(hbi_fv *f, int nprobe, const int *idx);
+/* Grid mode, setting 'injection': wells on rate schedules instead of the
+   centre source q0 (input_well, main_fv3dp.f90:1101-1118, used at
+   1546-1565 and 1407-1419): well w at 1-based grid cell (iw[w], jw[w]) with
+   kleng[w] <= HBI_FV_MAX_SCHEDULE rates qvals[w * 50 + k] at times
+   qtimes[w * 50 + k] (seconds), interpolated linearly; after each step the
+   first well's schedule bounds the next step at its rate changes.  nwell = 0
+   returns to the centre source.  Resets the schedule position. */
+int hbi_fv_set_wells(hbi_fv *f, int nwell, const int *iw, const int *jw, const int *kleng, const double *qvals,
+                     const double *qtimes);
 int hbi_fv_get_probes(hbi_fv *f, long long *nrows, double *rows);
 
 #ifdef __cplusplus
