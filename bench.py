@@ -166,6 +166,7 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
     fv.close()
 
     out["c1"] = c1_gpu(H, P, C1_STEPS)
+    out["fluid3dp"] = fluid3dp_gpu(H, P)
 
     # batched form: 64 realisations in lockstep, mixed precision, rkqs and learned step size
     from hbi_amd import learned as L
@@ -217,6 +218,35 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
     out["p2_entries"], out["p1_entries"] = p2_entries, p1_entries
     Hm.close()
     return out
+
+
+FV3_IM = FV3_JM = 200      # examples/3dinjection.in's grid (40 000 cells of 10 m)
+FV3_STEPS = 300
+FV3_LEAF = 32
+
+
+def fluid3dp_gpu(H, P):
+    """main_fv3dp's cycle on the 200 x 200 3dp fault of examples/3dinjection.in
+    (uniform friction of examples/3dinjection2.in, the centre source q0 3e-4,
+    Neumann boundaries, ACA eps 1e-4): FV3_STEPS accepted steps after 20,
+    the 2-D pressure solve by the cooperative conjugate-gradient kernel."""
+    print("bench: main_fv3dp 200 x 200 grid, %d steps" % FV3_STEPS, file=sys.stderr, flush=True)
+    g0, _, _, _, _ = P.fluid3dp_injection(FV3_IM, FV3_JM, 0.01)
+    Hm = H.HMatrix(g0, leaf=FV3_LEAF, eps=1e-4)
+    g, opts, arr, y, pf = P.fluid3dp_injection(FV3_IM, FV3_JM, 0.01, bc="Neumann", q0=3e-4, lod=Hm.lod)
+    fv = H.FV(Hm, opts, arr)
+    fv.set_state(y, pf, t=0.0, dt=1.0)
+    fv.step(20)
+    t0 = time.perf_counter()
+    info, hist = fv.step(FV3_STEPS, history=True)
+    wall = time.perf_counter() - t0
+    res = {"cells": g.n, "steps_per_s": round(FV3_STEPS / wall, 1), "ms_per_step": round(1e3 * wall / FV3_STEPS, 4),
+           "device_ms": round(fv.last_ms(), 3), "attempts": info["attempts"],
+           "cg_iter_mean": round(float(hist[:, 9].mean()), 2), "cg_iter_max": int(hist[:, 9].max()),
+           "t_end_s": info["t"], "log10_vmax_end": round(float(hist[-1, 1]), 4), "status": info["status"]}
+    fv.close()
+    Hm.close()
+    return res
 
 
 def _events(t, lv, thr=-3.0):
@@ -489,6 +519,7 @@ def main():
         "learned_vs_rkqs": r["learned_vs_rkqs"],
         "controllers_by_phase": r["controllers_by_phase"],
         "c1": r["c1"],
+        "fluid3dp": r["fluid3dp"],
         "ensemble_rkqs": ens0,
         "accuracy_mixed_vs_f64": r["accuracy_mixed_vs_f64"],
         "accuracy_ensemble_vs_f64": r["accuracy_ensemble_vs_f64"],

@@ -25,8 +25,6 @@
 // is reached or the run stopped (rejected attempts skip commit .. monitor).
 #include "rkcore.hpp"
 
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 #include <cmath>
 
@@ -553,13 +551,68 @@ __global__ __launch_bounds__(NT) void fv_pressure_kernel(FvPar P, FvCtl *c, FvAr
 // Work arrays of the 2-D solve, grid order (cell l = i jmax + j): cd the
 // diffusivities, co the six coefficients of each row (Amx along j, Amy along
 // i, Beuler's storage: first row (diagonal, right), last row (left at 1,
-// diagonal at 2)), b, the direction p (read at neighbours), the pressure at
-// the start pg, the Picard trial pt, the new pressure pn, pfhyd hg; red: the
-// per-workgroup partials of two reductions in flight.
+// diagonal at 2)), b, the directions p / p2 of alternate iterations and the
+// residual r (read at neighbours), the pressure at the start pg, the Picard
+// trial pt, the new pressure pn, pfhyd hg; red: the per-workgroup partials of
+// two reductions in flight.
 struct Fv2Work {
-    double *cd, *co, *b, *p, *pg, *pt, *pn, *hg, *red;
+    double *cd, *co, *b, *p, *pg, *pt, *pn, *hg, *red, *r, *p2;
+    unsigned *bar;   // grid barrier: release word, then one 128-byte arrival slot per workgroup
+    int *fail;       // set when a barrier wait ran past its spin limit
 };
 constexpr int kFv2Threads = 256;
+constexpr long long kFv2Spin = 1LL << 24;
+constexpr int kFv2Failed = 5;   // run status: a barrier of the 2-D solve timed out
+constexpr int kFv2MaxBlocks = 40;     // workgroups of the 2-D solve at most: 40 of 256 threads measured fastest on
+                                      // the 200 x 200 grid (barrier cost against cells per thread)
+
+// Grid-wide barrier of a cooperative launch (every workgroup resident),
+// without a contended counter: workgroup b publishes generation g in its own
+// 128-byte slot of W.bar, workgroup 0 waits (one thread per slot) until every
+// slot shows g and then publishes g as the release word W.bar[0]; the others
+// wait for it.  g = the release word read at kernel start (every workgroup
+// reads it before it first arrives, and the release word cannot move before
+// all have arrived) plus the barrier's ordinal.  Each thread releases its
+// writes before, and acquires after.  Waits are bounded: past kFv2Spin polls
+// W.fail is raised and the wait ends (the host discards the result).
+constexpr int kFv2Slot = 32;   // unsigned words per arrival slot (128 bytes)
+__device__ __forceinline__ void fv2_barrier(const Fv2Work &W, unsigned &gen)
+{
+    gen++;
+    // every wave's stores complete (workgroup scope: no cache write-back per
+    // wave); the arriving thread's release at device scope then publishes the
+    // workgroup's writes once
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    const int G = gridDim.x;
+    int bad = 0;
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&W.bar[kFv2Slot * (1 + blockIdx.x)], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (blockIdx.x == 0) {
+        for (int q = threadIdx.x; q < G; q += blockDim.x) {
+            long long spins = 0;
+            while (__hip_atomic_load(&W.bar[kFv2Slot * (1 + q)], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen)
+                if (++spins > kFv2Spin) {
+                    bad = 1;
+                    break;
+                }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(&W.bar[0], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (threadIdx.x == 0) {
+        long long spins = 0;
+        while (__hip_atomic_load(&W.bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen)
+            if (++spins > kFv2Spin) {
+                bad = 1;
+                break;
+            }
+    }
+    if (bad) atomicOr(W.fail, 1);
+    // the waiting thread's device-scope acquire invalidated the CU's and the
+    // XCD's caches; the barrier orders the other waves' loads after it
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
 
 // Sum (max) of one value per thread over the whole grid: per workgroup in a
 // fixed order, the partials to red[slot], a grid barrier, then every
@@ -567,9 +620,9 @@ constexpr int kFv2Threads = 256;
 // k + 64, ... then a fixed butterfly), so all get the same bits.  Two slots
 // alternate: a slot is written again only after the next reduction's barrier.
 template <bool MAX>
-__device__ __forceinline__ double fv2_reduce(double v, double *red, int &slot, double *lds)
+__device__ __forceinline__ double fv2_reduce(double v, const Fv2Work &W, int &slot, double *lds, unsigned &gen)
 {
-    namespace cgr = cooperative_groups;
+    double *red = W.red;
     const int tid = threadIdx.x, lane = tid % kWave, w = tid / kWave;
     const int G = gridDim.x;
     for (int o = kWave / 2; o > 0; o >>= 1) {
@@ -584,7 +637,7 @@ __device__ __forceinline__ double fv2_reduce(double v, double *red, int &slot, d
         for (int q = 1; q < kFv2Threads / kWave; q++) b = MAX ? fmax(b, lds[q]) : b + lds[q];
         out[blockIdx.x] = b;
     }
-    cgr::this_grid().sync();
+    fv2_barrier(W, gen);
     if (w == 0) {
         double a = MAX ? -HUGE_VAL : 0.0;
         for (int q = lane; q < G; q += kWave) a = MAX ? fmax(a, out[q]) : a + out[q];
@@ -646,18 +699,20 @@ __device__ __forceinline__ void fv2_line_matrix(int k, int len, double h, double
 
 // (A v)(l): the line products along j and i in the reference's order of terms
 // (main_fv3dp.f90:1566-1585), m = mx + my
-__device__ __forceinline__ double fv2_apply(const FvPar &P, const double *co, const double *v, int l)
+template <typename VF>
+__device__ __forceinline__ double fv2_apply(const FvPar &P, const double *co, VF v, int l)
 {
 #pragma clang fp contract(off)
     const int jm = P.jmax, i = l / jm, j = l % jm;
     const double *a = co + 6 * (size_t)l;
+    const double vl = v(l);
     double mx, my;
-    if (j == 0) mx = v[l] * a[0] + v[l + 1] * a[1];
-    else if (j == jm - 1) mx = v[l] * a[2] + v[l - 1] * a[1];
-    else mx = v[l - 1] * a[0] + v[l] * a[1] + v[l + 1] * a[2];
-    if (i == 0) my = v[l] * a[3] + v[l + jm] * a[4];
-    else if (i == P.imax - 1) my = v[l] * a[5] + v[l - jm] * a[4];
-    else my = v[l - jm] * a[3] + v[l] * a[4] + v[l + jm] * a[5];
+    if (j == 0) mx = vl * a[0] + v(l + 1) * a[1];
+    else if (j == jm - 1) mx = vl * a[2] + v(l - 1) * a[1];
+    else mx = v(l - 1) * a[0] + vl * a[1] + v(l + 1) * a[2];
+    if (i == 0) my = vl * a[3] + v(l + jm) * a[4];
+    else if (i == P.imax - 1) my = vl * a[5] + v(l - jm) * a[4];
+    else my = v(l - jm) * a[3] + vl * a[4] + v(l + jm) * a[5];
     return mx + my;
 }
 
@@ -686,10 +741,9 @@ __global__ __launch_bounds__(kFv2Threads) void fv2_pressure_kernel(FvPar P, FvCt
                                                                     Fv2Work W)
 {
 #pragma clang fp contract(off)
-    namespace cgr = cooperative_groups;
     __shared__ double lds[kFv2Threads / kWave + 1];
     if (io.gated && (c->done || !c->acc)) return;
-    cgr::grid_group grid = cgr::this_grid();
+    unsigned gen = __hip_atomic_load(&W.bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     const int n = P.n, im = P.imax, jm = P.jmax;
     const int gt = blockIdx.x * kFv2Threads + threadIdx.x, stride = gridDim.x * kFv2Threads;
     const double h = io.gated ? c->hdid : io.h;
@@ -721,7 +775,7 @@ __global__ __launch_bounds__(kFv2Threads) void fv2_pressure_kernel(FvPar P, FvCt
                 W.cd[l] = P.pdep ? ks * exp(-(io.sigma[e] - W.pt[l]) / P.s0) / cc * 1e-6 : ks / cc * 1e-6;
             }
         }
-        grid.sync();
+        fv2_barrier(W, gen);
         // the rows, the right-hand side and the start x = pf - pfhyd (p holds it for r = b - A x)
         double x[NPT], r[NPT];
 #pragma unroll
@@ -772,63 +826,66 @@ __global__ __launch_bounds__(kFv2Threads) void fv2_pressure_kernel(FvPar P, FvCt
             x[q] = pfl - hy;
             W.p[l] = x[q];
         }
-        grid.sync();
+        fv2_barrier(W, gen);
         double rr = 0.0;
 #pragma unroll
         for (int q = 0; q < NPT; q++) {
             const int l = gt + q * stride;
             if (l < n) {
-                r[q] = W.b[l] - fv2_apply(P, W.co, W.p, l);
+                r[q] = W.b[l] - fv2_apply(P, W.co, [&](int k) { return W.p[k]; }, l);
+                W.r[l] = r[q];
                 rr += r[q] * r[q];
             }
         }
-        double rsold = fv2_reduce<false>(rr, W.red, slot, lds);   // its barrier ends the reads of p = x
-#pragma unroll
-        for (int q = 0; q < NPT; q++) {
-            const int l = gt + q * stride;
-            if (l < n) W.p[l] = r[q];
-        }
+        double rsold = fv2_reduce<false>(rr, W, slot, lds, gen);   // its barrier ends the reads of p = x
         const double tol = (double)1e-6f;
         const double thr = tol * tol * (double)im * (double)jm;
         const int itermax = 1000;
         niter = 0;
         if (!(rsold < thr)) {
+            // The reference's p = r + (rsnew / rsold) p at the end of an
+            // iteration is taken into the next one: each thread forms the new p
+            // of its cells and of their neighbours from r and the previous p
+            // (same operands, same bits), so an iteration needs two barriers.
             niter = itermax;
+            double beta = 0.0;
             for (int it = 1; it <= itermax; it++) {
                 const double tmp1 = rsold;
-                grid.sync();   // p complete
-                double mp = 0.0, m[NPT];
+                const double *pold = (it & 1) ? W.p2 : W.p;
+                double *pnew = (it & 1) ? W.p : W.p2;
+                const bool first = it == 1;
+                auto pval = [&](int k) { return first ? W.r[k] : W.r[k] + beta * pold[k]; };
+                double mp = 0.0, m[NPT], pv[NPT];
 #pragma unroll
                 for (int q = 0; q < NPT; q++) {
                     const int l = gt + q * stride;
-                    m[q] = 0.0;
+                    m[q] = pv[q] = 0.0;
                     if (l < n) {
-                        m[q] = fv2_apply(P, W.co, W.p, l);
-                        mp += m[q] * W.p[l];
+                        pv[q] = first ? r[q] : r[q] + beta * pold[l];
+                        pnew[l] = pv[q];
+                        m[q] = fv2_apply(P, W.co, pval, l);
+                        mp += m[q] * pv[q];
                     }
                 }
-                const double tmp2 = fv2_reduce<false>(mp, W.red, slot, lds);
+                const double tmp2 = fv2_reduce<false>(mp, W, slot, lds, gen);
                 const double alpha = tmp1 / tmp2;
                 double rs = 0.0;
 #pragma unroll
                 for (int q = 0; q < NPT; q++) {
                     const int l = gt + q * stride;
                     if (l < n) {
-                        x[q] = x[q] + alpha * W.p[l];
+                        x[q] = x[q] + alpha * pv[q];
                         r[q] = r[q] - alpha * m[q];
+                        W.r[l] = r[q];
                         rs += r[q] * r[q];
                     }
                 }
-                const double rsnew = fv2_reduce<false>(rs, W.red, slot, lds);
+                const double rsnew = fv2_reduce<false>(rs, W, slot, lds, gen);
                 if (rsnew < thr) {
                     niter = it;
                     break;
                 }
-#pragma unroll
-                for (int q = 0; q < NPT; q++) {
-                    const int l = gt + q * stride;
-                    if (l < n) W.p[l] = r[q] + (rsnew / rsold) * W.p[l];
-                }
+                beta = rsnew / rsold;
                 rsold = rsnew;
             }
         }
@@ -843,7 +900,7 @@ __global__ __launch_bounds__(kFv2Threads) void fv2_pressure_kernel(FvPar P, FvCt
             }
         }
         if (!P.pdep) break;
-        err = fv2_reduce<true>(err, W.red, slot, lds);
+        err = fv2_reduce<true>(err, W, slot, lds, gen);
         if (err < (double)1e-5f) break;
 #pragma unroll
         for (int q = 0; q < NPT; q++) {
@@ -862,7 +919,7 @@ __global__ __launch_bounds__(kFv2Threads) void fv2_pressure_kernel(FvPar P, FvCt
             io.pfnew[A.cell[l]] = W.pn[l];
         }
     }
-    amax = fv2_reduce<true>(amax, W.red, slot, lds);
+    amax = fv2_reduce<true>(amax, W, slot, lds, gen);
     if (gt == 0) {
         double *dtnxt = io.gated ? &c->h : io.dtnxt;
         double dt = *dtnxt;
@@ -884,6 +941,10 @@ __global__ __launch_bounds__(kFv2Threads) void fv2_pressure_kernel(FvPar P, FvCt
         }
         *dtnxt = dt;
         *io.niter = niter;
+        if (io.gated && __hip_atomic_load(W.fail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) {
+            c->status = kFv2Failed;
+            c->done = 1;
+        }
     }
 }
 
@@ -1019,6 +1080,7 @@ struct hbi_fv {
     hbi::DevBuf<hbi::FvCtl> ctl;
     // grid mode: the 2-D solve's work arrays, grid -> cluster map, wells, launch shape
     hbi::DevBuf<double> g2;
+    hbi::DevBuf<unsigned> g2bar;   // barrier count, generation, failure flag
     hbi::DevBuf<int> cell, wi, wj, wk;
     hbi::DevBuf<double> wq, wt;
     int g2_blocks = 0, g2_npt = 0;
@@ -1054,7 +1116,10 @@ static Fv2Work fv2_work(hbi_fv *f)
 {
     const size_t n = f->n;
     double *g = f->g2.p;
-    return Fv2Work{g, g + n, g + 7 * n, g + 8 * n, g + 9 * n, g + 10 * n, g + 11 * n, g + 12 * n, g + 13 * n};
+    unsigned *bar = f->g2bar.p;
+    return Fv2Work{g, g + n, g + 7 * n, g + 8 * n, g + 9 * n, g + 10 * n, g + 11 * n, g + 12 * n,
+                   g + 15 * n, g + 13 * n, g + 14 * n,
+                   bar, bar ? (int *)(bar + kFv2Slot * (1 + (size_t)f->g2_blocks)) : nullptr};
 }
 
 // cells per thread and workgroups of the cooperative 2-D solve: the fewest
@@ -1072,7 +1137,7 @@ static void fv2_shape(hbi_fv *f)
         int per = 0;
         HBI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn[k], kFv2Threads, 0));
         const long long need = ((long long)f->n + (long long)kFv2Threads * npt - 1) / ((long long)kFv2Threads * npt);
-        if (need <= (long long)per * cus) {
+        if (need <= (long long)per * cus && (need <= kFv2MaxBlocks || npt == 16)) {
             f->g2_npt = npt;
             f->g2_blocks = (int)need;
             return;
@@ -1293,7 +1358,10 @@ int hbi_fv_create(hbi_hmat *hs, hbi_hmat *hn, const hbi_fv_opts *o, const double
                 for (int e = 0; e < n; e++) cell[hs->part.lod[e]] = e;
                 f->cell.upload(cell, s);
                 hbi::fv2_shape(f);
-                f->g2.alloc(13 * (size_t)n + 2 * (size_t)f->g2_blocks);
+                f->g2.alloc(15 * (size_t)n + 2 * (size_t)f->g2_blocks);
+                const size_t nbar = hbi::kFv2Slot * (2 + (size_t)f->g2_blocks);
+                f->g2bar.alloc(nbar);
+                HBI_HIP(hipMemsetAsync(f->g2bar.p, 0, nbar * sizeof(unsigned), s));
             }
             f->ctl.alloc(1);
             hbi::FvCtl c{};
@@ -1430,6 +1498,12 @@ int hbi_fv_diffuse(hbi_fv *f, const double *pf, const double *y, const double *s
         io.h = h; io.t = t; io.dtnxt = f->io_dt.p; io.niter = f->io_niter.p; io.gated = 0;
         hbi::fv_launch_pressure(f, hbi::fv_arrays(f), io);
         HBI_HIP(hipGetLastError());
+        if (f->P.grid) {
+            int failed = 0;
+            HBI_HIP(hipMemcpyAsync(&failed, hbi::fv2_work(f).fail, sizeof(int), hipMemcpyDeviceToHost, f->stream));
+            HBI_HIP(hipStreamSynchronize(f->stream));
+            if (failed) throw hbi::Error{HBI_E_HIP, "pore-pressure solve: a grid barrier timed out"};
+        }
         f->io_out.download(pfnew, n, f->stream);
         f->io_niter.download(niter, 1, f->stream);
         f->io_dt.download(dtnxt, 1, f->stream);
@@ -1487,6 +1561,7 @@ int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info)
             info->vmax = c.vmax; info->niter = c.niter; info->status = c.status;
         }
         if (c.status == HBI_E_STEP) throw hbi::Error{HBI_E_STEP, "Runge-Kutta step size underflow"};
+        if (c.status == hbi::kFv2Failed) throw hbi::Error{HBI_E_HIP, "pore-pressure solve: a grid barrier timed out"};
     });
 }
 

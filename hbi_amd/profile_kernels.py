@@ -14,7 +14,8 @@ profiler's byte counters.  With --ensemble K it instead runs the bench's
 ensemble (64 friction realisations, learned step size, mixed precision) for 20
 warm-up and K timed lockstep steps, for a kernel trace of the attempt; with
 --fluid K the fluid-driven cycle of the bp6a example (4000 elements) for 20
-warm-up and K timed steps.  Prints one JSON line.
+warm-up and K timed steps; with --fluid3dp K main_fv3dp's cycle on the 200 x
+200 grid of examples/3dinjection.in likewise.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -35,9 +36,13 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0)
     ap.add_argument("--ensemble", type=int, default=0)
     ap.add_argument("--fluid", type=int, default=0)
+    ap.add_argument("--fluid3dp", type=int, default=0)
     a = ap.parse_args()
     if a.fluid > 0:
         print(json.dumps(fluid(a.fluid)), flush=True)
+        return
+    if a.fluid3dp > 0:
+        print(json.dumps(fluid3dp(a.fluid3dp)), flush=True)
         return
     g = P.mesh_3dph(IMAX, JMAX, DS0, 90.0)
     Hm = H.HMatrix(g, leaf=LEAF, eps=EPS_H, bf16_margin=1.0)
@@ -86,6 +91,20 @@ def fluid(steps: int) -> dict:
     g = P.mesh_2dp(P.BP6A_N, P.BP6A_DS0)
     Hm = H.HMatrix(g, leaf=32, eps=1e-4)
     opts, arr, y, pf = P.fluid_bp6a(g, Hm.lod)
+    fv = H.FV(Hm, opts, arr)
+    fv.set_state(y, pf, t=0.0, dt=1.0)
+    fv.step(20)
+    info, hist = fv.step(steps, history=True)
+    return {"steps": steps, "device_ms": fv.last_ms(), "ms_per_step": fv.last_ms() / max(1, len(hist)),
+            "cg_iter_mean": float(hist[:, 9].mean()), "attempts": info["attempts"], "t_end_s": info["t"]}
+
+
+def fluid3dp(steps: int) -> dict:
+    """main_fv3dp on the 200 x 200 3dp fault (bench.py's fluid3dp): 20
+    warm-up and `steps` timed accepted steps."""
+    g0, _, _, _, _ = P.fluid3dp_injection(200, 200, 0.01)
+    Hm = H.HMatrix(g0, leaf=32, eps=1e-4)
+    g, opts, arr, y, pf = P.fluid3dp_injection(200, 200, 0.01, bc="Neumann", q0=3e-4, lod=Hm.lod)
     fv = H.FV(Hm, opts, arr)
     fv.set_state(y, pf, t=0.0, dt=1.0)
     fv.step(20)

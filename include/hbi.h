@@ -464,7 +464,8 @@ typedef struct hbi_fv_info {
     long long attempts, accepted, rejected;
     double vmax;
     int niter;               /* conjugate-gradient iterations of the last pore-pressure solve */
-    int status;              /* 0 running; 1 V outside [velmin, velmax]; 3 t > tmax; 4 min sigma_e < 0.2;
+    int status;              /* 0 running; 1 V outside [velmin, velmax]; 3 t > tmax; 4 min sigma_e < 0.2; 5 grid mode: a
+                                barrier of the pore-pressure solve timed out;
                                 HBI_E_STEP on step-size underflow */
 } hbi_fv_info;
 
