@@ -44,14 +44,15 @@ BF16_MFMA_PEAK_TFS = 2500.0
 # HBM bytes per launch of the two roofline kernels (mixed precision), from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE passes over
 # hbi_amd/profile_kernels.py, scaled by the counters of a 256 MiB device copy
-# (fetch x 2.00, write x 1.00): mr_bf16_kernel<4,true,3> (512 items of 2048
-# columns: 134033.5 KiB fetched, 512 x 32 KiB partial tiles written) and
-# stream_kernel (110083.2 KiB fetched, 2887.5 KiB written)
-TRAFFIC_MR_BF16_MIXED = 291.3e6
+# (fetch x 2.00, write x 1.00): mr_bf16_kernel<4,true,2,2> (137317.2 KiB
+# fetched, 1016 x 32 KiB partial tiles written at the tuner's usual 1016 items
+# of up to 1024 columns) and stream_kernel (110083.2 KiB fetched, 2887.5 KiB
+# written)
+TRAFFIC_MR_BF16_MIXED = 314.5e6
 TRAFFIC_STREAM_MIXED = 228.4e6
 # rocprofv3 --pmc MfmaUtil over the same program: fraction of cycles the
-# matrix cores of mr_bf16_kernel<4,true,3> are busy
-MFMA_UTIL_MR_BF16 = 0.193
+# matrix cores of mr_bf16_kernel<4,true,2,2> are busy
+MFMA_UTIL_MR_BF16 = 0.231
 IMAX, JMAX, DS0, LEAF, EPS_H, EPS_R = 256, 128, 0.390625, 64, 1e-5, 1e-6
 NREAL = 64
 FLUID_STEPS = 500      # timed accepted steps of the bp6a fluid-driven cycle
@@ -493,7 +494,7 @@ def main():
                                "(rkqs check), hipGraph attempts",
                    "n_elements": IMAX * JMAX, "realisations": NREAL},
         "roofline": {"bound": "hbm",
-                     "kernel": "mr_bf16_kernel<4,true,2> (64-RHS row tiles on the BF16 matrix cores, split operands)",
+                     "kernel": "mr_bf16_kernel<4,true,2,2> (64-RHS row tiles on the BF16 matrix cores, split operands)",
                      "achieved": round(r["p2_alg_bytes"] / (mmx["p2_ms"] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r["p2_alg_bytes"] / (mmx["p2_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": TRAFFIC_MR_BF16_MIXED,
