@@ -4,7 +4,8 @@ conjugate-gradient kernel -- against the oracle (oracle/fluid3dp.py):
 
 * the pressure step for each setting and boundary set, both solvers, the
   wells of the reference's examples/wells.dat on its 100 x 100 grid, and the
-  200 x 200 grid of examples/3dinjection.in (157 workgroups): iteration
+  200 x 200 grid of examples/3dinjection.in (40 workgroups of 256 threads,
+  four cells per thread): iteration
   counts within one, solutions within the two solvers' residual bounds, the
   step bound;
 * the first steps of a run: identical step grids, monitor rows and states;
