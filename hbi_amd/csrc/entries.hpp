@@ -14,6 +14,9 @@
 //   2dph   in-plane dipping segment, half-space m_HACApK_calc_entry_ij.f90:92-116, 143-166 (also '2dnh')
 //   2dn    in-plane nonplanar, full space       m_HACApK_calc_entry_ij.f90:300-339, 697-750
 //   25d    2.5-D rectangles                     m_HACApK_calc_entry_ij.f90:934-1064
+//   2dpfz  antiplane segment + 10 image pairs   m_HACApK_calc_entry_ij.f90:663-683
+//   2dn_vector in-plane nonplanar (end tractions) m_HACApK_calc_entry_ij.f90:472-562
+//   2dna   antiplane nonplanar, full space      m_HACApK_calc_entry_ij.f90:378-393, 751-765
 // The closed-form expressions are those of the published solutions written
 // term by term in the reference's evaluation order; FP contraction is off in
 // every function here so that the device result tracks the FP64 CPU result
@@ -736,6 +739,76 @@ __device__ __forceinline__ double inplane(double xc, double xl, double xr)
     return factor * (1.0 / (xc - xr) - 1.0 / (xc - xl));
 }
 
+// planar antiplane segment with ten image pairs at k w, k = 1..10
+// (matels2dpFZ_ij, m_HACApK_calc_entry_ij.f90:663-683); the image term's
+// numerator dx^3 - 3 a^2 is the reference's as written
+__device__ __forceinline__ double antiplane_fz(double xc, double xl, double xr, double w)
+{
+#pragma clang fp contract(off)
+    const double dx1 = xc - xr, dx2 = xc - xl;
+    double s = 1.0 / dx1 - 1.0 / dx2;
+    for (int k = 1; k <= 10; k++) {
+        const double a = k * w, a2 = a * a;
+        const double t1 = a2 + dx1 * dx1, t2 = a2 + dx2 * dx2;
+        s = s + 2 * dx1 * (dx1 * dx1 * dx1 - 3 * a2) / (t1 * t1 * t1) -
+            2 * dx2 * (dx2 * dx2 * dx2 - 3 * a2) / (t2 * t2 * t2);
+    }
+    return s * (kRigid / (2.0 * dpi()));
+}
+
+// nonplanar in-plane segment, full space, from the traction vectors of its two
+// end points (matel2dn_ij + kern, m_HACApK_calc_entry_ij.f90:472-562):
+// shear (vcomp 0, 's') or normal (vcomp 1, 'n') traction on the receiver
+__device__ __noinline__ double seg2dv_entry(const DevGeom &g, int i, int j)
+{
+#pragma clang fp contract(off)
+    const double factor = kRigid / (2.0 * dpi() * (1.0 - kPois));
+    const double nx = -sin(g.ang[i]), ny = cos(g.ang[i]);
+    const double xe[2] = {g.xel[j], g.xer[j]}, ye[2] = {g.yel[j], g.yer[j]};
+    double I1[2], I2[2];
+    for (int e = 0; e < 2; e++) {
+        const double dx = g.xcol[i] - xe[e], dy = g.ycol[i] - ye[e];
+        const double r = sqrt(dx * dx + dy * dy);
+        const double gx = dx / r, gy = dy / r;
+        if (g.vcomp == 0) {
+            const double t = 4 * nx * ny * gx * gy + (ny * ny - nx * nx) * (gy * gy - gx * gx);
+            I1[e] = -(gy / r * t);
+            I2[e] = -(gx / r * t);
+        } else {
+            const double t = 2 * nx * ny * (gy * gy - gx * gx) - 2 * (ny * ny - nx * nx) * gx * gy;
+            I1[e] = gx / r + gy / r * t;
+            I2[e] = gy / r + gx / r * t;
+        }
+    }
+    const double sx = -sin(g.ang[j]), sy = cos(g.ang[j]);
+    return factor * (sx * (-I1[1] + I1[0]) + sy * (I2[1] - I2[0]));
+}
+
+// nonplanar antiplane segment, full space (tensor2d3_ij + inte31s / inte32s,
+// m_HACApK_calc_entry_ij.f90:378-393, 751-765): the out-of-plane stress pair
+// of the segment's frame projected on the receiver's normal
+__device__ __noinline__ double seg2da_entry(const DevGeom &g, int i, int j)
+{
+#pragma clang fp contract(off)
+    const double pi = dpi();
+    const double cj = cos(g.ang[j]), sj = sin(g.ang[j]);
+    const double xp = cj * (g.xcol[i] - g.xel[j]) + sj * (g.ycol[i] - g.yel[j]);
+    const double xm = cj * (g.xcol[i] - g.xer[j]) + sj * (g.ycol[i] - g.yer[j]);
+    const double yp = -sj * (g.xcol[i] - g.xel[j]) + cj * (g.ycol[i] - g.yel[j]);
+    const double ym = -sj * (g.xcol[i] - g.xer[j]) + cj * (g.ycol[i] - g.yer[j]);
+    auto i31 = [&](double x1, double x2) {
+        const double r = sqrt(x1 * x1 + x2 * x2);
+        return -(kVs * x2 / pi / (r * r));
+    };
+    auto i32 = [&](double x1, double x2) {
+        const double r = sqrt(x1 * x1 + x2 * x2);
+        return kVs / pi * x1 / (r * r);
+    };
+    const double k31 = -0.5 * kRigid / kVs * (i31(xp, yp) - i31(xm, ym));
+    const double k32 = -0.5 * kRigid / kVs * (i32(xp, yp) - i32(xm, ym));
+    return sin(g.ang[i]) * k31 + cos(g.ang[i]) * k32;
+}
+
 // planar rectangle in a full space seen in its own plane (ret3dp): shear for
 // slip modes 's'/'o', normal for 'open'; the other combinations are zero
 __device__ __noinline__ double quad_entry(const DevGeom &g, int i, int j)
@@ -905,6 +978,9 @@ __device__ __forceinline__ double entry(const DevGeom &g, int i, int j)
     case 8: return dip2d_entry(g, i, j);
     case 9: return seg2d_entry(g, i, j);
     case 10: return rect25_entry(g, i, j);
+    case 11: return antiplane_fz(g.xcol[i], g.xel[j], g.xer[j], g.w);
+    case 12: return seg2dv_entry(g, i, j);
+    case 13: return seg2da_entry(g, i, j);
     }
     return __builtin_nan("");
 }

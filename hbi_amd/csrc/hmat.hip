@@ -563,14 +563,20 @@ __global__ __launch_bounds__(kMvThreads) void stream_kernel(const Item1 *__restr
 // The low-rank columns of one tile and dtype (contiguous in the layout,
 // ncol of them from byte offset off; column j multiplies z[zc[j]]): the z
 // values are gathered into LDS, kZL at a time, and the columns streamed as
-// one run.  Thread r < TR returns row r's sum; red and zl are free again on
-// return.
+// one run.  The whole launch is a single round of workgroups, so its time is
+// the chain of memory latencies of one workgroup: the first U factor loads of
+// every lane (independent of z) are issued before the gather, the gather
+// issues all its index loads and then all its value loads before waiting, and
+// the stream keeps two batches of U loads per lane in flight.  Thread r < TR
+// returns row r's sum; red and zl are free again on return.
 constexpr int kZL = 2048;
+constexpr int kZG = kZL / kMvThreads;   // z values gathered per thread and chunk
 template <typename T, int TR, int U>
 __device__ __forceinline__ double lowrank_rows(uint64_t off, int ncol, const int *__restrict__ zc,
                                                const uint8_t *__restrict__ buf, const double *__restrict__ z,
                                                double *zl, double *red)
 {
+    using R = typename Raw<T>::R;
     constexpr int V = Raw<T>::V;
     constexpr int LPC = TR / V;
     constexpr int CPI = kWave / LPC;
@@ -582,12 +588,45 @@ __device__ __forceinline__ double lowrank_rows(uint64_t off, int ncol, const int
     for (int e = 0; e < V; e++) acc[e] = 0.0;
     for (int c0 = 0; c0 < ncol; c0 += kZL) {
         const int nb = min(kZL, ncol - c0);
-        for (int j = threadIdx.x; j < nb; j += kMvThreads) zl[j] = z[zc[c0 + j]];
+        const int ngroups = (nb + CPI - 1) / CPI;
+        const uint8_t *cb = buf + off + ((size_t)c0 * TR + (size_t)rsub * V) * sizeof(T);
+        // factor values of column group g of this lane (the zero buffer past the run)
+        auto aload = [&](int g) -> R {
+            const int c = g * CPI + csub;
+            const bool ok = g < ngroups && c < nb;
+            return Raw<T>::load(ok ? cb + (size_t)c * TR * sizeof(T) : (const uint8_t *)g_mv_zero);
+        };
+        R a[U], b[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) a[u] = aload(w + u * NW);
+        int zi[kZG];
+        double zv[kZG];
+#pragma unroll
+        for (int k = 0; k < kZG; k++) {
+            const int j = threadIdx.x + k * kMvThreads;
+            zi[k] = *(j < nb ? zc + c0 + j : (const int *)g_mv_zero);
+        }
+#pragma unroll
+        for (int k = 0; k < kZG; k++) zv[k] = z[zi[k]];
+#pragma unroll
+        for (int k = 0; k < kZG; k++) {
+            const int j = threadIdx.x + k * kMvThreads;
+            if (j < nb) zl[j] = zv[k];
+        }
         __syncthreads();
-        Seg sg{};
-        sg.off = off + (uint64_t)c0 * TR * sizeof(T);
-        sg.ncol = nb;
-        tile_seg<T, TR, U, false>(sg, buf, zl, rsub, csub, w, acc);
+        for (int g0 = w; g0 < ngroups; g0 += NW * U) {
+#pragma unroll
+            for (int u = 0; u < U; u++) b[u] = aload(g0 + NW * U + u * NW);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int c = (g0 + u * NW) * CPI + csub;
+                const double xv = c < nb ? zl[c] : 0.0;
+#pragma unroll
+                for (int e = 0; e < V; e++) acc[e] = fma(Raw<T>::get(a[u], e), xv, acc[e]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) a[u] = b[u];
+        }
         __syncthreads();
     }
 #pragma unroll
@@ -606,7 +645,9 @@ __device__ __forceinline__ double lowrank_rows(uint64_t off, int ncol, const int
 
 // Launch B: one item per row tile.  Its low-rank columns (a2 times z; one
 // run per dtype, in dtype order) plus the tile's dense slots from launch A
-// (in slot order) give the tile's rows of y.
+// (in slot order) give the tile's rows of y.  The first kSlotPre slots are
+// loaded before the low-rank runs (they were written by launch A).
+constexpr int kSlotPre = 4;
 template <int TR, int U>
 __global__ __launch_bounds__(kMvThreads) void lowrank_kernel(const ItemB *__restrict__ items,
                                                              const int *__restrict__ zcol,
@@ -620,6 +661,12 @@ __global__ __launch_bounds__(kMvThreads) void lowrank_kernel(const ItemB *__rest
     __shared__ double red[kMvThreads * 8];
     if (gate && *gate) return;
     const ItemB it = items[blockIdx.x];
+    const int r = threadIdx.x, row = it.tile * TR + r;
+    const int p0 = part0[it.tile], p1 = part0[it.tile + 1];
+    double pre[kSlotPre];
+#pragma unroll
+    for (int k = 0; k < kSlotPre; k++)
+        pre[k] = *(r < TR && p0 + k < p1 ? ypart + (size_t)(p0 + k) * TR + r : g_mv_zero);
     double lr = 0.0;
     int zc = it.zc0;
     if (it.ncol[kF64] > 0) lr += lowrank_rows<double, TR, U>(it.off[kF64], it.ncol[kF64], zcol + zc, buf, z, zl, red);
@@ -628,10 +675,12 @@ __global__ __launch_bounds__(kMvThreads) void lowrank_kernel(const ItemB *__rest
     zc += it.ncol[kF32];
     if (it.ncol[kBF16] > 0)
         lr += lowrank_rows<uint16_t, TR, U>(it.off[kBF16], it.ncol[kBF16], zcol + zc, buf, z, zl, red);
-    const int r = threadIdx.x, row = it.tile * TR + r;
     if (r < TR && row < n) {
         double s = 0.0;
-        for (int p = part0[it.tile]; p < part0[it.tile + 1]; p++) s += ypart[(size_t)p * TR + r];
+#pragma unroll
+        for (int k = 0; k < kSlotPre; k++)
+            if (p0 + k < p1) s += pre[k];
+        for (int p = p0 + kSlotPre; p < p1; p++) s += ypart[(size_t)p * TR + r];
         y[row] = s + lr;
     }
 }
@@ -946,7 +995,7 @@ static void launch_b(const hbi_hmat *h, const Plan &P, const double *z, const do
     const int n = (int)P.itb.size();
     if (n == 0) return;
     if (P.unroll3 == 16)
-        lowrank_kernel<TR, 16><<<n, kMvThreads, 0, s>>>(P.d_itb.p, P.d_zcol.p, P.buf.p, z, ypart, P.d_tile_part0.p,
+        lowrank_kernel<TR, 12><<<n, kMvThreads, 0, s>>>(P.d_itb.p, P.d_zcol.p, P.buf.p, z, ypart, P.d_tile_part0.p,
                                                         y, h->n, gate);
     else
         lowrank_kernel<TR, 8><<<n, kMvThreads, 0, s>>>(P.d_itb.p, P.d_zcol.p, P.buf.p, z, ypart, P.d_tile_part0.p,
@@ -1010,15 +1059,16 @@ static void upload_geom(hbi_hmat *h, const hbi_geom *g)
 static void check_geom(const hbi_geom *g)
 {
     HBI_REQUIRE(g && g->n > 0, "geometry with n > 0 required");
-    HBI_REQUIRE(g->kind >= 1 && g->kind <= 10, "unknown geometry kind");
+    HBI_REQUIRE(g->kind >= 1 && g->kind <= 13, "unknown geometry kind");
     HBI_REQUIRE(g->xcol, "xcol required");
     if (g->kind == HBI_KIND_2DVS || g->kind == HBI_KIND_2DP) HBI_REQUIRE(g->xel && g->xer, "2dvs / 2dp need xel, xer");
+    if (g->kind == HBI_KIND_2DPFZ) HBI_REQUIRE(g->xel && g->xer, "2dpfz needs xel, xer (and the image spacing w)");
     if (g->kind == HBI_KIND_3DP)
         HBI_REQUIRE(g->zcol && g->xs1 && g->xs2 && g->xs3 && g->xs4 && g->zs1 && g->zs2 && g->zs3 && g->zs4,
                     "3dp needs zcol and the four corners xs1..xs4, zs1..zs4");
-    if (g->kind == HBI_KIND_2DPH || g->kind == HBI_KIND_2DN)
+    if (g->kind == HBI_KIND_2DPH || g->kind == HBI_KIND_2DN || g->kind == HBI_KIND_2DNV || g->kind == HBI_KIND_2DNA)
         HBI_REQUIRE(g->ycol && g->xel && g->xer && g->yel && g->yer && g->ang,
-                    "2dph / 2dnh / 2dn need ycol, segment ends xel, xer, yel, yer and ang");
+                    "2dph / 2dnh / 2dn / 2dn_vector / 2dna need ycol, segment ends xel, xer, yel, yer and ang");
     if (g->kind == HBI_KIND_25D)
         HBI_REQUIRE(g->ycol && g->ang && g->ds && g->w > 0, "25d needs ycol, ang, ds and w > 0");
     if (g->kind == HBI_KIND_OKADA_HS || g->kind == HBI_KIND_OKADA_FS)

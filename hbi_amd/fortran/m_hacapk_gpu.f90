@@ -103,6 +103,12 @@ contains
       problem_kind = HBI_KIND_2DN
     case ('25d')
       problem_kind = HBI_KIND_25D
+    case ('2dpfz')
+      problem_kind = HBI_KIND_2DPFZ
+    case ('2dn_vector')
+      problem_kind = HBI_KIND_2DNV
+    case ('2dna')
+      problem_kind = HBI_KIND_2DNA
     case ('3dph', '3dhr')
       problem_kind = HBI_KIND_OKADA_HS
     case ('3dnr')

@@ -6,7 +6,7 @@ module m_hbi_abi
   integer(c_int), parameter :: HBI_OK = 0, HBI_E_STEP = -4, HBI_RK_OUTPUT_FULL = 2
   integer(c_int), parameter :: HBI_KIND_2DVS = 1, HBI_KIND_OKADA_HS = 2, HBI_KIND_OKADA_FS = 3, HBI_KIND_TRI_FS = 4, &
     & HBI_KIND_TRI_HS = 5, HBI_KIND_2DP = 6, HBI_KIND_3DP = 7, HBI_KIND_2DPH = 8, HBI_KIND_2DN = 9, &
-    & HBI_KIND_25D = 10
+    & HBI_KIND_25D = 10, HBI_KIND_2DPFZ = 11, HBI_KIND_2DNV = 12, HBI_KIND_2DNA = 13
   integer(c_int), parameter :: HBI_PREC_F64 = 0, HBI_PREC_F32 = 1, HBI_PREC_MIXED = 2
 
   type, bind(C) :: hbi_geom

@@ -36,6 +36,9 @@ KIND_3DP = 7          # 3dp: planar quadrilaterals, full space
 KIND_2DPH = 8         # 2dph, 2dnh: in-plane dipping segments, half-space
 KIND_2DN = 9          # 2dn: in-plane nonplanar segments, full space
 KIND_25D = 10         # 25d: 2.5-D rectangles
+KIND_2DPFZ = 11       # 2dpfz: antiplane planar segments with ten image pairs at multiples of w
+KIND_2DNV = 12        # 2dn_vector: in-plane nonplanar segments (end-point tractions), full space
+KIND_2DNA = 13        # 2dna: antiplane nonplanar segments, full space
 
 _GEO_FIELDS = ("xcol", "ycol", "zcol", "xel", "xer", "ang", "angd", "dsl", "dsd", "rake",
                "xs1", "xs2", "xs3", "ys1", "ys2", "ys3", "zs1", "zs2", "zs3",
@@ -97,6 +100,16 @@ def mesh_2dp(n: int, ds0: float) -> Geometry:
     return g
 
 
+def mesh_2dpfz(n: int, ds0: float, fwid: float) -> Geometry:
+    """2-D antiplane planar fault of n segments (coordinate2dp) with the image
+    pairs of matels2dpFZ_ij at multiples of ``fwid`` (st_bemv%w = fwid,
+    main_LH.f90:606); the reference's mesh switch has no case for this kind."""
+    g = mesh_2dvs(n, ds0)
+    g.kind = KIND_2DPFZ
+    g.w = float(fwid)
+    return g
+
+
 def mesh_3dp(imax: int, jmax: int, ds0: float) -> Geometry:
     """Planar fault of imax x jmax squares in a full space, in the x-z plane
     (coordinate3dp, main_LH.f90:1752-1779): corners 1..4 at (+,+), (-,+), (-,-), (+,-)."""
@@ -121,8 +134,8 @@ def mesh_2dph(n: int, ds0: float, dipangle: float) -> Geometry:
 
 def mesh_segments(xel, xer, yel, yer, kind: int = KIND_2DN, w: float = 1e8) -> Geometry:
     """2-D segments from their end points (coordinate2dn, main_LH.f90:1739-1750):
-    lengths ds, angles atan2, midpoints; kind 2dn, 2dnh (KIND_2DPH) or 25d
-    (element width ``w``, the reference's fwid)."""
+    lengths ds, angles atan2, midpoints; kind 2dn, 2dnh (KIND_2DPH), 2dn_vector
+    (KIND_2DNV), 2dna (KIND_2DNA) or 25d (element width ``w``, the reference's fwid)."""
     xel, xer, yel, yer = (_c(a) for a in (xel, xer, yel, yer))
     n = len(xel)
     ds = np.sqrt((xer - xel) ** 2 + (yer - yel) ** 2)

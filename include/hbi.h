@@ -49,6 +49,9 @@ extern "C" {
 #define HBI_KIND_2DPH 8       /* 2-D in-plane, half-space, dipping segments: '2dph', '2dnh' (matel2dph2_ij) */
 #define HBI_KIND_2DN 9        /* 2-D in-plane, full space, nonplanar segments: '2dn' (tensor2d_ij) */
 #define HBI_KIND_25D 10       /* 2.5-D: rectangles of width w seen in their plane: '25d' (matelrec_ij) */
+#define HBI_KIND_2DPFZ 11     /* 2-D antiplane, planar, ten image pairs at multiples of w: '2dpfz' (matels2dpFZ_ij) */
+#define HBI_KIND_2DNV 12      /* 2-D in-plane, nonplanar, end-point tractions: '2dn_vector' (matel2dn_ij) */
+#define HBI_KIND_2DNA 13      /* 2-D antiplane, nonplanar segments, full space: '2dna' (tensor2d3_ij) */
 
 /* storage precision of the H-matrix factors used by a product */
 #define HBI_PREC_F64 0        /* FP64 factors (the reference's arithmetic) */

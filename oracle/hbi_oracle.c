@@ -58,7 +58,8 @@ static const double og_vs = 3.464;
 /* ------------------------------------------------------------------------- */
 typedef struct {
     int kind;   /* 1: 2dvs antiplane, 2: okada half-space (3dph/3dhr), 3: okada full-space (3dnr), 4: TD full-space (3dnt),
-                   5: TD half-space (3dht), 6: 2dp, 7: 3dp, 8: 2dph/2dnh, 9: 2dn, 10: 25d */
+                   5: TD half-space (3dht), 6: 2dp, 7: 3dp, 8: 2dph/2dnh, 9: 2dn, 10: 25d,
+                   11: 2dpfz, 12: 2dn_vector, 13: 2dna */
     int n;
     int vcomp;  /* 0: shear traction ('s'), 1: normal traction ('n') */
     int md;     /* TD slip mode: 0 's' (receiver rake), 1 'st', 2 'dp', 3 'open' */
@@ -976,6 +977,65 @@ static double og_25d_ij(const og_geom *g, int i, int j)
     return -0.5 * (sxx - syy) * sin(d) + sxy * cos(d);
 }
 
+/* matels2dpFZ_ij, m_HACApK_calc_entry_ij.f90:663-683: the antiplane segment
+   plus ten image pairs at a = k w (the numerator dx^3 - 3 a^2 as written) */
+static double og_2dpfz_ij(const og_geom *g, int i, int j)
+{
+    double d1 = g->xcol[i] - g->xer[j], d2 = g->xcol[i] - g->xel[j];
+    double v = 1.0 / d1 - 1.0 / d2;
+    for (int k = 1; k <= 10; k++) {
+        double a = (double)k * g->w;
+        double p1 = a * a + d1 * d1, p2 = a * a + d2 * d2;
+        double t1 = 2 * d1 * (d1 * d1 * d1 - 3 * (a * a)) / (p1 * p1 * p1);
+        double t2 = 2 * d2 * (d2 * d2 * d2 - 3 * (a * a)) / (p2 * p2 * p2);
+        v = v + t1 - t2;
+    }
+    return v * (og_rigid / (2.0 * OG_PI));
+}
+
+/* matel2dn_ij + kern, m_HACApK_calc_entry_ij.f90:472-562: traction vectors of
+   the segment's end points, shear (vcomp 0) or normal (vcomp 1) */
+static double og_2dnv_ij(const og_geom *g, int i, int j)
+{
+    double factor = og_rigid / (2.0 * OG_PI * (1.0 - og_pois));
+    double nx = -sin(g->ang[i]), ny = cos(g->ang[i]);
+    double I1[2], I2[2];
+    for (int e = 0; e < 2; e++) {
+        double dx = g->xcol[i] - (e ? g->xer[j] : g->xel[j]);
+        double dy = g->ycol[i] - (e ? g->yer[j] : g->yel[j]);
+        double r = sqrt(dx * dx + dy * dy), gx = dx / r, gy = dy / r;
+        double c2 = gy * gy - gx * gx, q = ny * ny - nx * nx;
+        if (g->vcomp == 0) {
+            double t = 4 * nx * ny * gx * gy + q * c2;
+            I1[e] = -(gy / r * t);
+            I2[e] = -(gx / r * t);
+        } else {
+            double t = 2 * nx * ny * c2 - 2 * q * gx * gy;
+            I1[e] = gx / r + gy / r * t;
+            I2[e] = gy / r + gx / r * t;
+        }
+    }
+    double sx = -sin(g->ang[j]), sy = cos(g->ang[j]);
+    return factor * (sx * (-I1[1] + I1[0]) + sy * (I2[1] - I2[0]));
+}
+
+/* tensor2d3_ij + inte31s / inte32s, m_HACApK_calc_entry_ij.f90:378-393, 751-765 */
+static double og_2dna_ij(const og_geom *g, int i, int j)
+{
+    double c = cos(g->ang[j]), s = sin(g->ang[j]);
+    double ex[2] = { g->xcol[i] - g->xel[j], g->xcol[i] - g->xer[j] };
+    double ey[2] = { g->ycol[i] - g->yel[j], g->ycol[i] - g->yer[j] };
+    double k31[2], k32[2];
+    for (int e = 0; e < 2; e++) {
+        double x1 = c * ex[e] + s * ey[e], x2 = -s * ex[e] + c * ey[e];
+        double r = sqrt(x1 * x1 + x2 * x2);
+        k31[e] = -(og_vs * x2 / OG_PI / (r * r));
+        k32[e] = og_vs / OG_PI * x1 / (r * r);
+    }
+    double f = -0.5 * og_rigid / og_vs;
+    return sin(g->ang[i]) * (f * (k31[0] - k31[1])) + cos(g->ang[i]) * (f * (k32[0] - k32[1]));
+}
+
 /* HACApK_entry_ij, m_HACApK_calc_entry_ij.f90:24-90 (0-based, mesh order) */
 double og_entry(const og_geom *g, int i, int j)
 {
@@ -991,6 +1051,9 @@ double og_entry(const og_geom *g, int i, int j)
     case 8: return og_2dph_ij(g, i, j);
     case 9: return og_2dn_ij(g, i, j);
     case 10: return og_25d_ij(g, i, j);
+    case 11: return og_2dpfz_ij(g, i, j);
+    case 12: return og_2dnv_ij(g, i, j);
+    case 13: return og_2dna_ij(g, i, j);
     }
     return NAN;
 }

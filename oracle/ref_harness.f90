@@ -12,7 +12,8 @@
 !                    ev11 ev12 ev13 ev21 ev22 ev23 ev31 ev32 ev33
 !                    yel yer xs4 zs4 ds
 ! kind: 1 '2dvs', 2 '3dph', 3 '3dnr', 4 '3dnt', 5 '3dht', 6 '2dp', 7 '3dp',
-!       8 '2dph', 9 '2dn', 10 '25d'; vcomp: 0 's', 1 'n';
+!       8 '2dph', 9 '2dn', 10 '25d', 11 '2dpfz', 12 '2dn_vector', 13 '2dna';
+!       vcomp: 0 's', 1 'n';
 ! md: 0 's', 1 'st', 2 'dp', 3 'open'.  ii, jj are 0-based.
 module hbiref_harness
   use iso_c_binding
@@ -50,6 +51,12 @@ contains
       bem%problem = '2dn'
     case (10)
       bem%problem = '25d'
+    case (11)
+      bem%problem = '2dpfz'
+    case (12)
+      bem%problem = '2dn_vector'
+    case (13)
+      bem%problem = '2dna'
     end select
     if (vcomp == 1) then
       bem%v = 'n'

@@ -5,7 +5,8 @@ into oracle/_ref/libhbiref.so by oracle/Makefile (``make -C oracle ref``), on
 sampled (receiver, source) pairs of the three benchmark geometries and stores
 geometry + pairs + values as compressed .npz fixtures next to this script.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                # every fixture
+    python tests/golden/make_golden.py --more-kinds   # only 2dpfz, 2dn_vector, 2dna
 """
 from __future__ import annotations
 
@@ -157,7 +158,45 @@ def main():
     g.md, g.vcomp = 3, 1
     ii, jj = pairs(g, 500)
     save("kernel_entries_2dn_mdopen_n.npz", g, ii, jj, O.ref_entries(g, ii, jj))
+    more_kinds()
+
+
+def more_kinds():
+    """The three kinds of HACApK_entry_ij that main_LH's mesh switch never
+    selects: 2dpfz (image pairs at multiples of fwid), 2dn_vector and 2dna,
+    on a straight fault and the curved fault above (own seed, so the files
+    above do not change when this part is re-run alone)."""
+    rng = np.random.default_rng(20261204)
+
+    def pairs(g, nrand):
+        near = [(i, i) for i in range(0, g.n, 4)] + [(i, i + 1) for i in range(0, g.n - 1, 9)] + \
+               [(i + 1, i) for i in range(0, g.n - 1, 11)]
+        return sample_pairs(g.n, rng, nrand, near)
+
+    def curve(n, length, amp, y0):
+        x = np.linspace(0.0, length, n + 1)
+        y = y0 + amp * np.sin(6 * np.pi * x / length) + 0.3 * amp * np.sin(22 * np.pi * x / length)
+        return x[:-1], x[1:], y[:-1], y[1:]
+
+    # 2dpfz: 1000 x 0.1 km, image spacing 0.5 km and 3 km
+    for fwid, tag in ((0.5, "2dpfz"), (3.0, "2dpfz_w3")):
+        g = P.mesh_2dpfz(1000, 0.1, fwid)
+        ii, jj = pairs(g, 1500)
+        save("kernel_entries_%s.npz" % tag, g, ii, jj, O.ref_entries(g, ii, jj))
+    for kind, tag in ((P.KIND_2DNV, "2dnv"), (P.KIND_2DNA, "2dna")):
+        g = P.mesh_segments(*curve(600, 30.0, 0.5, 0.0), kind=kind)
+        ii, jj = pairs(g, 1000)
+        save("kernel_entries_%s.npz" % tag, g, ii, jj, O.ref_entries(g, ii, jj))
+        if kind == P.KIND_2DNV:
+            g.vcomp = 1
+            ii, jj = pairs(g, 600)
+            save("kernel_entries_%s_n.npz" % tag, g, ii, jj, O.ref_entries(g, ii, jj))
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--more-kinds"]:
+        O.build()
+        assert O.ref_available(), "oracle/_ref/libhbiref.so missing (needs /root/reference)"
+        more_kinds()
+    else:
+        main()

@@ -71,6 +71,9 @@ CASES = [
     ("2dph", lambda: P.mesh_2dph(1600, 0.025, 30.0), 32, 1e-6, 1e-6),
     ("2dn", lambda: P.mesh_segments(*_curve(800, 30.0, 0.5, 0.0)), 32, 1e-6, 1e-6),
     ("25d", lambda: P.mesh_segments(*_curve(800, 30.0, 0.5, 0.0), kind=P.KIND_25D, w=10.0), 32, 1e-6, 1e-6),
+    ("2dpfz", lambda: P.mesh_2dpfz(1000, 0.1, 0.5), 32, 1e-6, 1e-6),
+    ("2dn_vector", lambda: P.mesh_segments(*_curve(800, 30.0, 0.5, 0.0), kind=P.KIND_2DNV), 32, 1e-6, 1e-6),
+    ("2dna", lambda: P.mesh_segments(*_curve(800, 30.0, 0.5, 0.0), kind=P.KIND_2DNA), 32, 1e-6, 1e-6),
 ]
 
 

@@ -124,6 +124,63 @@ def test_antiplane_closed_form_and_image():
     assert (d < 0).all() and d[0] > d[-1]
 
 
+def test_unused_2d_kinds_reduce_to_the_planar_kernels():
+    """2dpfz, 2dn_vector and 2dna against independent closed forms:
+    * 2dpfz: the image-pair sum, written out here per pair with a = k w, and
+      at the reference's default fwid = 1e8 it is the full-space antiplane
+      kernel mu / 2 pi (1/(x - xr) - 1/(x - xl)) to rounding;
+    * on a straight fault along x (angle 0), 2dn_vector's shear traction is
+      the in-plane kernel 2dp, mu / (2 pi (1 - nu)) (...), its normal traction
+      vanishes, and 2dna is the full-space antiplane kernel -- so a dropped
+      term, a swapped end point or a wrong sign fails here."""
+    rng = np.random.default_rng(5)
+    n = 200
+    g = P.mesh_2dpfz(n, 0.1, 0.7)
+    ii, jj = rng.integers(0, n, 400), rng.integers(0, n, 400)
+    xc, xl, xr = g.xcol[ii], g.xel[jj], g.xer[jj]
+    fa = 32.04 / (2 * math.pi)
+    expect = 1 / (xc - xr) - 1 / (xc - xl)
+    for k in range(1, 11):
+        a = 0.7 * k
+        for d, s in ((xc - xr, 1.0), (xc - xl, -1.0)):
+            expect = expect + s * 2 * d * (d ** 3 - 3 * a * a) / (a * a + d * d) ** 3
+    assert np.allclose(O.entries(g, ii, jj), fa * expect, rtol=1e-12, atol=1e-12 * np.abs(fa * expect).max())
+    g.w = 1e8
+    anti = fa * (1 / (xc - xr) - 1 / (xc - xl))
+    assert np.allclose(O.entries(g, ii, jj), anti, rtol=1e-13)
+    x = np.arange(n + 1) * 0.1
+    line = (x[:-1], x[1:], np.zeros(n), np.zeros(n))
+    fi = 32.04 / (2 * math.pi * 0.75)
+    gv = P.mesh_segments(*line, kind=P.KIND_2DNV)
+    assert np.allclose(O.entries(gv, ii, jj), fi * (1 / (xc - xr) - 1 / (xc - xl)), rtol=1e-12)
+    gv.vcomp = 1
+    assert np.abs(O.entries(gv, ii, jj)).max() <= 1e-12 * np.abs(anti).max()
+    ga = P.mesh_segments(*line, kind=P.KIND_2DNA)
+    assert np.allclose(O.entries(ga, ii, jj), anti, rtol=1e-12)
+    # on a curved fault, 2dn_vector's shear traction (end-point traction
+    # vectors, scaled by 1 / (1 - nu)) equals 2dn's (tensor2d_ij: end-point
+    # integrals of the stress tensor, scaled by 2 (1 - (vs / vp)^2), another
+    # closed form) up to the ratio of the two scalings.  2dn_vector's normal
+    # traction ('n') is not rotation invariant (the reference's kern as
+    # written): golden vectors only.
+    xs = np.linspace(0, 30, 301)
+    ys = 0.5 * np.sin(6 * np.pi * xs / 30)
+    i2, j2 = rng.integers(0, 300, 400), rng.integers(0, 300, 400)
+    gv = P.mesh_segments(xs[:-1], xs[1:], ys[:-1], ys[1:], kind=P.KIND_2DNV)
+    gt = P.mesh_segments(xs[:-1], xs[1:], ys[:-1], ys[1:], kind=P.KIND_2DN)
+    vv, vt = O.entries(gv, i2, j2), O.entries(gt, i2, j2)
+    ratio = 2 * (1 - (3.464 / 6.0) ** 2) * (1 - 0.25)
+    assert abs(ratio - 1) > 1e-5
+    assert np.allclose(vv * ratio, vt, rtol=1e-9, atol=1e-11 * np.abs(vt).max())
+    # 2dna projects the source-frame stress pair with the receiver's global
+    # angle (tensor2d3_ij as written): on the straight fault turned by theta,
+    # every entry is cos(theta) times the antiplane kernel along the fault
+    th = 0.6
+    c, s = math.cos(th), math.sin(th)
+    gr = P.mesh_segments(c * x[:-1], c * x[1:], s * x[:-1], s * x[1:], kind=P.KIND_2DNA)
+    assert np.allclose(O.entries(gr, ii, jj), c * anti, rtol=1e-9, atol=1e-12 * np.abs(anti).max())
+
+
 def test_mesh_generators_match_oracle():
     imax, jmax, ds0 = 12, 7, 0.390625
     g = P.mesh_3dph(imax, jmax, ds0, 90.0)
