@@ -42,6 +42,7 @@ struct FvPar {
 
 struct FvCtl {
     double h, t, hdid, errmax, vmax;
+    double tbreak;   // the call ends after the first accepted step with t > tbreak
     long long attempts, accepted, rejected, target, hist_base, hist_cap;
     int done, status, acc, niter;
     int sw, nn;   // grid mode: the well schedule's switch flag and 1-based position (implicitsolver2)
@@ -99,9 +100,10 @@ __device__ __forceinline__ void fv_deriv(const FvPar &P, double gs, double gn, d
     o[3] = dksdt;
 }
 
-__global__ void fv_arm_kernel(FvCtl *c, long long nsteps)
+__global__ void fv_arm_kernel(FvCtl *c, long long nsteps, double tbreak)
 {
     if (c->status == 0) c->done = 0;
+    c->tbreak = tbreak;
     c->target = c->accepted + nsteps;
     c->hist_base = c->accepted;
     c->hist_cap = nsteps;
@@ -1057,7 +1059,7 @@ __global__ void fv_monitor_kernel(FvPar P, FvCtl *c, FvArrays A, int nblk)
     if (vm > P.velmax || vm < P.velmin) c->status = 1;
     else if (c->t > P.tmax) c->status = 3;
     else if (b[4] < (double)0.2f) c->status = 4;
-    if (c->status || c->accepted >= c->target) c->done = 1;
+    if (c->status || c->accepted >= c->target || c->t > c->tbreak) c->done = 1;
 }
 
 }  // namespace hbi
@@ -1090,6 +1092,7 @@ struct hbi_fv {
     bool dirty = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0.0;
+    double tbreak = HUGE_VAL;   // hbi_fv_set_break
 };
 
 namespace hbi {
@@ -1184,10 +1187,10 @@ static void fv_launch_stages(hbi_fv *f, int nstages, int mode, const int *gate)
     fv_stage0_kernel<<<g, kFvThreads, 0, f->stream>>>(f->P, gate, A, mode == 1);
     for (int s = 0; s < nstages; s++) {
         hmat_phase1(f->hs, prec, f->x.p, f->z.p, f->ypart.p, f->stream, gate);
-        hmat_phase2(f->hs, prec, f->z.p, f->ypart.p, f->hx.p, f->stream, gate);
+        hmat_phase2(f->hs, prec, f->x.p, f->z.p, f->ypart.p, f->hx.p, f->stream, gate);
         if (f->hn) {
             hmat_phase1(f->hn, prec, f->x.p, f->zn.p, f->ypart_n.p, f->stream, gate);
-            hmat_phase2(f->hn, prec, f->zn.p, f->ypart_n.p, f->hx_n.p, f->stream, gate);
+            hmat_phase2(f->hn, prec, f->x.p, f->zn.p, f->ypart_n.p, f->hx_n.p, f->stream, gate);
         }
         fv_stage_kernel<<<g, kFvThreads, 0, f->stream>>>(f->P, gate, f->ctl.p, A, s, mode);
     }
@@ -1530,7 +1533,7 @@ int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info)
         if (f->dirty) recapture = true;
         f->dirty = false;
         if (recapture) hbi::fv_capture(f);
-        hbi::fv_arm_kernel<<<1, 1, 0, f->stream>>>(f->ctl.p, nsteps);
+        hbi::fv_arm_kernel<<<1, 1, 0, f->stream>>>(f->ctl.p, nsteps, f->tbreak);
         HBI_HIP(hipEventRecord(f->e0, f->stream));
         hbi::FvCtl c;
         long long launched = 0;
@@ -1562,6 +1565,14 @@ int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info)
         }
         if (c.status == HBI_E_STEP) throw hbi::Error{HBI_E_STEP, "Runge-Kutta step size underflow"};
         if (c.status == hbi::kFv2Failed) throw hbi::Error{HBI_E_HIP, "pore-pressure solve: a grid barrier timed out"};
+    });
+}
+
+int hbi_fv_set_break(hbi_fv *f, double t)
+{
+    return hbi::guarded([&] {
+        HBI_REQUIRE(f && !std::isnan(t), "bad arguments");
+        f->tbreak = t;
     });
 }
 

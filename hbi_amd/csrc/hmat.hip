@@ -561,20 +561,22 @@ __global__ __launch_bounds__(kMvThreads) void stream_kernel(const Item1 *__restr
 }
 
 // The low-rank columns of one tile and dtype (contiguous in the layout,
-// ncol of them from byte offset off; column j multiplies z[zc[j]]): the z
-// values are gathered into LDS, kZL at a time, and the columns streamed as
-// one run.  The whole launch is a single round of workgroups, so its time is
+// ncol of them from byte offset off; column j multiplies z[zc[j]], or
+// x[zc[j] & ~kXcol] where the kXcol bit is set: the dense columns that lead
+// the run in dense_b plans): the input values are gathered into LDS, kZL at
+// a time, and the columns streamed as one run.  The whole launch is a single round of workgroups, so its time is
 // the chain of memory latencies of one workgroup: the first U factor loads of
 // every lane (independent of z) are issued before the gather, the gather
 // issues all its index loads and then all its value loads before waiting, and
 // the stream keeps two batches of U loads per lane in flight.  Thread r < TR
 // returns row r's sum; red and zl are free again on return.
 constexpr int kZL = 2048;
+constexpr int kXcol = (int)0x80000000u;   // run column index flag: an x index (dense column), not a z index
 constexpr int kZG = kZL / kMvThreads;   // z values gathered per thread and chunk
 template <typename T, int TR, int U>
 __device__ __forceinline__ double lowrank_rows(uint64_t off, int ncol, const int *__restrict__ zc,
                                                const uint8_t *__restrict__ buf, const double *__restrict__ z,
-                                               double *zl, double *red)
+                                               const double *__restrict__ x, double *zl, double *red)
 {
     using R = typename Raw<T>::R;
     constexpr int V = Raw<T>::V;
@@ -607,7 +609,7 @@ __device__ __forceinline__ double lowrank_rows(uint64_t off, int ncol, const int
             zi[k] = *(j < nb ? zc + c0 + j : (const int *)g_mv_zero);
         }
 #pragma unroll
-        for (int k = 0; k < kZG; k++) zv[k] = z[zi[k]];
+        for (int k = 0; k < kZG; k++) zv[k] = *(zi[k] < 0 ? x + (zi[k] & ~kXcol) : z + zi[k]);
 #pragma unroll
         for (int k = 0; k < kZG; k++) {
             const int j = threadIdx.x + k * kMvThreads;
@@ -646,7 +648,10 @@ __device__ __forceinline__ double lowrank_rows(uint64_t off, int ncol, const int
 // Launch B: one item per row tile.  Its low-rank columns (a2 times z; one
 // run per dtype, in dtype order) plus the tile's dense slots from launch A
 // (in slot order) give the tile's rows of y.  The first kSlotPre slots are
-// loaded before the low-rank runs (they were written by launch A).
+// loaded before the low-rank runs (they were written by launch A).  In plans
+// with dense_b each run also holds the tile's dense columns of its dtype
+// (they lie just before its low-rank columns), there are no slots, and
+// launch A holds the stacked a1^T tiles alone.
 constexpr int kSlotPre = 4;
 template <int TR, int U>
 __global__ __launch_bounds__(kMvThreads) void lowrank_kernel(const ItemB *__restrict__ items,
@@ -655,7 +660,8 @@ __global__ __launch_bounds__(kMvThreads) void lowrank_kernel(const ItemB *__rest
                                                              const double *__restrict__ z,
                                                              const double *__restrict__ ypart,
                                                              const int *__restrict__ part0, double *__restrict__ y,
-                                                             int n, const int *__restrict__ gate)
+                                                             int n, const double *__restrict__ x,
+                                                             const int *__restrict__ gate)
 {
     __shared__ double zl[kZL];
     __shared__ double red[kMvThreads * 8];
@@ -669,12 +675,14 @@ __global__ __launch_bounds__(kMvThreads) void lowrank_kernel(const ItemB *__rest
         pre[k] = *(r < TR && p0 + k < p1 ? ypart + (size_t)(p0 + k) * TR + r : g_mv_zero);
     double lr = 0.0;
     int zc = it.zc0;
-    if (it.ncol[kF64] > 0) lr += lowrank_rows<double, TR, U>(it.off[kF64], it.ncol[kF64], zcol + zc, buf, z, zl, red);
+    if (it.ncol[kF64] > 0)
+        lr += lowrank_rows<double, TR, U>(it.off[kF64], it.ncol[kF64], zcol + zc, buf, z, x, zl, red);
     zc += it.ncol[kF64];
-    if (it.ncol[kF32] > 0) lr += lowrank_rows<float, TR, U>(it.off[kF32], it.ncol[kF32], zcol + zc, buf, z, zl, red);
+    if (it.ncol[kF32] > 0)
+        lr += lowrank_rows<float, TR, U>(it.off[kF32], it.ncol[kF32], zcol + zc, buf, z, x, zl, red);
     zc += it.ncol[kF32];
     if (it.ncol[kBF16] > 0)
-        lr += lowrank_rows<uint16_t, TR, U>(it.off[kBF16], it.ncol[kBF16], zcol + zc, buf, z, zl, red);
+        lr += lowrank_rows<uint16_t, TR, U>(it.off[kBF16], it.ncol[kBF16], zcol + zc, buf, z, x, zl, red);
     if (r < TR && row < n) {
         double s = 0.0;
 #pragma unroll
@@ -695,7 +703,7 @@ static size_t z_size(const hbi_hmat *h) { return (size_t)std::max(h->nz, 1) + 8;
 // segments of one dtype (one workgroup each) writing one partial slot, a
 // tile's slots consecutive; and one item per tile with its low-rank segments.
 // (Phase-1 items are the row tiles.)
-static void make_items(hbi_hmat *h, Plan &P, int chunk2, int unroll1, int unroll2, int unroll3)
+static void make_items(hbi_hmat *h, Plan &P, int chunk2, int unroll1, int unroll2, int unroll3, bool dense_b)
 {
     P.it2.clear();
     P.itb.clear();
@@ -724,6 +732,19 @@ static void make_items(hbi_hmat *h, Plan &P, int chunk2, int unroll1, int unroll
                 ib.ncol[G.dtype] += sg.ncol;
             }
             if (sb == G.s0) continue;
+            if (dense_b) {   // the group's dense columns join its run, ahead of the low-rank ones
+                uint64_t o = P.segs[G.s0].off;
+                int nd = 0;
+                for (int q = G.s0; q < sb; q++) {
+                    HBI_REQUIRE(P.segs[q].off == o, "segments of a tile are not contiguous");
+                    o += (uint64_t)P.segs[q].ncol * h->tr * sz;
+                    nd += P.segs[q].ncol;
+                }
+                HBI_REQUIRE(sb == G.s1 || o == ib.off[G.dtype], "segments of a tile are not contiguous");
+                ib.off[G.dtype] = P.segs[G.s0].off;
+                ib.ncol[G.dtype] += nd;
+                continue;
+            }
             size_t cur = 0;
             int s0 = G.s0;
             for (int q = G.s0; q < sb; q++) {
@@ -737,14 +758,19 @@ static void make_items(hbi_hmat *h, Plan &P, int chunk2, int unroll1, int unroll
             }
             P.it2.push_back({t, s0, sb, P.nparts++, G.dtype, 0});
         }
-        // z index per low-rank column, dtype by dtype
+        // z index per low-rank column, dtype by dtype (dense_b: every column of
+        // the run, dense ones as x index | kXcol)
         for (int d = 0; d < 3; d++)
             for (size_t q = g_first; q < g; q++) {
                 const SegGroup &G = P.groups[q];
                 if (G.dtype != d) continue;
-                for (int k = G.s0; k < G.s1; k++)
-                    if (P.segs[k].src)
-                        for (int j = 0; j < P.segs[k].ncol; j++) P.zcol.push_back(P.segs[k].xoff + j);
+                for (int k = G.s0; k < G.s1; k++) {
+                    const Seg &sg = P.segs[k];
+                    if (sg.src)
+                        for (int j = 0; j < sg.ncol; j++) P.zcol.push_back(sg.xoff + j);
+                    else if (dense_b)
+                        for (int j = 0; j < sg.ncol; j++) P.zcol.push_back((int)((unsigned)(sg.xoff + j) | kXcol));
+                }
             }
         P.itb.push_back(ib);
     }
@@ -753,6 +779,12 @@ static void make_items(hbi_hmat *h, Plan &P, int chunk2, int unroll1, int unroll
     P.unroll1 = unroll1;
     P.unroll2 = unroll2;
     P.unroll3 = unroll3;
+    if (dense_b != P.dense_b) {   // the dense factor bytes move to the other launch
+        const long long d = dense_b ? P.alg_dense : -P.alg_dense;
+        P.alg_bytes_p1 -= d;
+        P.alg_bytes_p2 += d;
+        P.dense_b = dense_b;
+    }
     P.d_it2.upload(P.it2, h->stream);
     P.d_itb.upload(P.itb, h->stream);
     P.d_zcol.upload(P.zcol, h->stream);
@@ -899,6 +931,7 @@ static void build_plan(hbi_hmat *h, int prec)
                     J.src_rs = f.nc; J.src_cs = 1;
                     P.alg_bytes += (long long)(hi - lo) * f.nc * sz;
                     P.alg_bytes_p1 += (long long)(hi - lo) * f.nc * sz;
+                    P.alg_dense += (long long)(hi - lo) * f.nc * sz;
                 }
                 jobs.push_back(J);
                 elems += (long long)TR * ncol;
@@ -931,7 +964,7 @@ static void build_plan(hbi_hmat *h, int prec)
     P.built = true;
     // ---- work-item sizes: fixed by the caller, or picked by timing the product
     if (h->opts.chunk_kb > 0) {
-        make_items(h, P, h->opts.chunk_kb * 1024, 4, 4, 8);
+        make_items(h, P, h->opts.chunk_kb * 1024, 4, 4, 8, h->opts.dense_launch == 2);
         return;
     }
     const int c2s[] = {16384, 32768, 65536, 131072, 262144};
@@ -947,21 +980,32 @@ static void build_plan(hbi_hmat *h, int prec)
         return ms / reps;
     };
     // dense tile items and their loads in flight, then the loads in flight of
-    // the phase-1 and the low-rank items
+    // the phase-1 and the low-rank items, then the dense blocks in launch B
+    // (h->opts.dense_launch pins the last choice)
+    const int dl = h->opts.dense_launch;
     int b2 = c2s[2], bu1 = 4, bu2 = 4, bu3 = 8;
+    bool bdb = dl == 2;
     float best = 1e30f;
     for (int u : {4, 8})
         for (int c2 : c2s) {
-            make_items(h, P, c2, bu1, u, bu3);
+            make_items(h, P, c2, bu1, u, bu3, bdb);
             const float ms = time_it(5);
             if (ms < best) { best = ms; b2 = c2; bu2 = u; }
+            if (bdb) break;   // no dense items in launch A: chunk2 and unroll2 do not matter
         }
-    make_items(h, P, b2, 8, bu2, bu3);
+    make_items(h, P, b2, 8, bu2, bu3, bdb);
     float ms = time_it(5);
     if (ms < best) { best = ms; bu1 = 8; }
-    make_items(h, P, b2, bu1, bu2, 16);
-    if (time_it(5) < best) bu3 = 16;
-    make_items(h, P, b2, bu1, bu2, bu3);
+    make_items(h, P, b2, bu1, bu2, 12, bdb);
+    ms = time_it(5);
+    if (ms < best) { best = ms; bu3 = 12; }
+    if (dl == 0)
+        for (int u : {8, 12}) {
+            make_items(h, P, b2, bu1, bu2, u, true);
+            ms = time_it(5);
+            if (ms < best) { best = ms; bu3 = u; bdb = true; }
+        }
+    make_items(h, P, b2, bu1, bu2, bu3, bdb);
 }
 
 size_t hmat_z_size(hbi_hmat *h, int prec) { return z_size(h); }
@@ -989,17 +1033,17 @@ static void launch_a(const Plan &P, const double *x, double *z, double *ypart, h
 }
 
 template <int TR>
-static void launch_b(const hbi_hmat *h, const Plan &P, const double *z, const double *ypart, double *y,
-                     hipStream_t s, const int *gate)
+static void launch_b(const hbi_hmat *h, const Plan &P, const double *x, const double *z, const double *ypart,
+                     double *y, hipStream_t s, const int *gate)
 {
     const int n = (int)P.itb.size();
     if (n == 0) return;
-    if (P.unroll3 == 16)
+    if (P.unroll3 == 12)
         lowrank_kernel<TR, 12><<<n, kMvThreads, 0, s>>>(P.d_itb.p, P.d_zcol.p, P.buf.p, z, ypart, P.d_tile_part0.p,
-                                                        y, h->n, gate);
+                                                        y, h->n, x, gate);
     else
         lowrank_kernel<TR, 8><<<n, kMvThreads, 0, s>>>(P.d_itb.p, P.d_zcol.p, P.buf.p, z, ypart, P.d_tile_part0.p,
-                                                       y, h->n, gate);
+                                                       y, h->n, x, gate);
 }
 
 void hmat_phase1(hbi_hmat *h, int prec, const double *x, double *z, double *ypart, hipStream_t s, const int *gate)
@@ -1009,12 +1053,12 @@ void hmat_phase1(hbi_hmat *h, int prec, const double *x, double *z, double *ypar
     else launch_a<32>(P, x, z, ypart, s, gate);
 }
 
-void hmat_phase2(hbi_hmat *h, int prec, const double *z, const double *ypart, double *y, hipStream_t s,
-                 const int *gate)
+void hmat_phase2(hbi_hmat *h, int prec, const double *x, const double *z, const double *ypart, double *y,
+                 hipStream_t s, const int *gate)
 {
     const Plan &P = h->plan[prec];
-    if (h->tr == 64) launch_b<64>(h, P, z, ypart, y, s, gate);
-    else launch_b<32>(h, P, z, ypart, y, s, gate);
+    if (h->tr == 64) launch_b<64>(h, P, x, z, ypart, y, s, gate);
+    else launch_b<32>(h, P, x, z, ypart, y, s, gate);
 }
 
 size_t hmat_ypart_size(hbi_hmat *h, int prec) { return (size_t)std::max(hmat_plan(h, prec).nparts, 1) * h->tr; }
@@ -1022,7 +1066,7 @@ size_t hmat_ypart_size(hbi_hmat *h, int prec) { return (size_t)std::max(hmat_pla
 void hmat_apply(hbi_hmat *h, int prec, const double *x, double *y)
 {
     hmat_phase1(h, prec, x, h->d_z.p, h->d_ypart.p, h->stream, nullptr);
-    hmat_phase2(h, prec, h->d_z.p, h->d_ypart.p, y, h->stream, nullptr);
+    hmat_phase2(h, prec, x, h->d_z.p, h->d_ypart.p, y, h->stream, nullptr);
 }
 
 // ------------------------------------------------------------- geometry --
@@ -1091,6 +1135,7 @@ static void build(hbi_hmat *h, const hbi_geom *g, const hbi_hmat_opts *o)
     if (h->opts.eta <= 0) h->opts.eta = 2.0;
     if (h->opts.kmax <= 0) h->opts.kmax = 1000;
     HBI_REQUIRE(h->opts.eps_aca > 0, "eps_aca must be > 0");
+    HBI_REQUIRE(h->opts.dense_launch >= 0 && h->opts.dense_launch <= 2, "dense_launch must be 0, 1 or 2");
     h->n = g->n;
     h->tr = h->opts.tile_rows > 0 ? h->opts.tile_rows : (h->opts.leaf <= 32 ? 32 : 64);
     HBI_REQUIRE(h->tr == 32 || h->tr == 64, "tile_rows must be 32 or 64");
@@ -1307,6 +1352,7 @@ static void build(hbi_hmat *h, const hbi_geom *g, const hbi_hmat_opts *o)
         S.alg_bytes_p2[p] = h->plan[p].alg_bytes_p2;
         S.chunk_p1[p] = h->plan[p].chunk1;
         S.chunk_p2[p] = h->plan[p].chunk2;
+        S.dense_launch[p] = h->plan[p].dense_b ? 2 : 1;
     }
     S.nbf16_blocks = h->plan[HBI_PREC_MIXED].nbf16;
 }
@@ -1493,7 +1539,7 @@ int hbi_hmat_matvec_noalloc(hbi_hmat *h, const double *x, double *y, int prec)
                                             " (use hbi_hmat_matvec)"};
         double *sp = scratch_for_current_device();
         hbi::hmat_phase1(h, prec, x, sp, sp + zn, h->stream, nullptr);
-        hbi::hmat_phase2(h, prec, sp, sp + zn, y, h->stream, nullptr);
+        hbi::hmat_phase2(h, prec, x, sp, sp + zn, y, h->stream, nullptr);
         HBI_HIP(hipGetLastError());
         HBI_HIP(hipStreamSynchronize(h->stream));
     });
@@ -1512,7 +1558,7 @@ int hbi_hmat_time_matvec(hbi_hmat *h, int prec, int iters, double *ms)
             HBI_HIP(hipEventRecord(h->ev[0], h->stream));
             hbi::hmat_phase1(h, prec, h->d_x.p, h->d_z.p, h->d_ypart.p, h->stream, nullptr);
             HBI_HIP(hipEventRecord(h->ev[1], h->stream));
-            hbi::hmat_phase2(h, prec, h->d_z.p, h->d_ypart.p, h->d_y.p, h->stream, nullptr);
+            hbi::hmat_phase2(h, prec, h->d_x.p, h->d_z.p, h->d_ypart.p, h->d_y.p, h->stream, nullptr);
             HBI_HIP(hipEventRecord(h->ev[2], h->stream));
             HBI_HIP(hipEventSynchronize(h->ev[2]));
             float a, b, c;

@@ -25,7 +25,9 @@ struct Item2 {
 };
 // Low-rank work item of the single-vector product: the low-rank segments of
 // one row tile, for each dtype d ncol[d] contiguous columns from byte offset
-// off[d]; their z indices are zcol[zc0 ..], dtype by dtype.
+// off[d]; their z indices are zcol[zc0 ..], dtype by dtype.  In plans with
+// the dense blocks in this launch (dense_b) each run starts with the tile's
+// dense columns of its dtype, whose entries are x indices with bit 31 set.
 struct ItemB {
     uint64_t off[3];
     int ncol[3], tile, zc0, pad;
@@ -65,6 +67,8 @@ struct Plan {
     std::vector<SegGroup> groups;
     int chunk1 = 0, chunk2 = 0;   // work-item sizes in bytes (phase 2 autotuned)
     int unroll1 = 4, unroll2 = 4, unroll3 = 8;   // loads in flight per lane: phase 1, dense, low-rank (autotuned)
+    bool dense_b = false;         // dense blocks streamed by launch B's row-tile items, not by launch A (autotuned)
+    long long alg_dense = 0;      // dense factor bytes (in alg_bytes_p1, or p2 when dense_b)
     DevBuf<uint8_t> buf;
     size_t bytes = 0;
     std::vector<Seg> segs;
@@ -117,7 +121,7 @@ void hmat_apply(hbi_hmat *h, int prec, const double *x, double *y);
 size_t hmat_z_size(hbi_hmat *h, int prec);
 size_t hmat_ypart_size(hbi_hmat *h, int prec);
 void hmat_phase1(hbi_hmat *h, int prec, const double *x, double *z, double *ypart, hipStream_t s, const int *gate);
-void hmat_phase2(hbi_hmat *h, int prec, const double *z, const double *ypart, double *y, hipStream_t s,
+void hmat_phase2(hbi_hmat *h, int prec, const double *x, const double *z, const double *ypart, double *y, hipStream_t s,
                  const int *gate);
 const Plan &hmat_plan(hbi_hmat *h, int prec);
 // multi-RHS product (RR = 64 columns, row-major N x 64)

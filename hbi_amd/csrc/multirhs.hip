@@ -24,6 +24,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <vector>
 
 namespace hbi {
 
@@ -721,6 +723,9 @@ __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restr
         it1 = items1[item];
         ncol = it1.tn;
     }
+    // phase 1: a wave whose 16 rows lie past the item's rows only stages B
+    // (wave-uniform, so its A loads and MFMAs are skipped, not masked)
+    const bool live = P2 || 16 * w < it1.kt;
     const int nsl = (ncol + 31) / 32;
     const float *zf = (const float *)g_zero_row;
     // input row of item column col (past the item's end: zeros)
@@ -752,7 +757,7 @@ __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restr
             const float *ap = s < nsl ? a16 + it2.off16 + ((size_t)(s * NW + w) * 64 + lane) * 8 : zf;
             S.a0 = *(const fvec4 *)ap;
             S.a1 = *(const fvec4 *)(ap + 4);
-        } else {
+        } else if (live) {
             // FP32 rows: 8 values in two 16-byte loads; BF16 rows: one 16-byte load (read twice)
             const int row = 16 * w + c, col = it1.t0 + 32 * s + 8 * q;
             const bool ok = row < it1.kt && col < it1.t0 + it1.tn;
@@ -853,7 +858,7 @@ __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restr
         for (int b = 0; b < NS; b++) {
             const int sl = i + b;
             load(st[(b + NS - 1) % NS], sl + NS - 1);
-            if (sl < nsl) multiply(NB == 2 ? (sl & 1) : 0, st[b]);
+            if (sl < nsl && live) multiply(NB == 2 ? (sl & 1) : 0, st[b]);
             if constexpr (NB == 1) __syncthreads();   // one buffer: every wave has read it
             bstore(st[(b + 1) % NS], NB == 2 ? ((sl + 1) & 1) : 0);
             __syncthreads();
@@ -911,6 +916,7 @@ __global__ __launch_bounds__(256) void mr_pack_bf_kernel(const MPack *__restrict
 struct MrPlan {
     bool built = false;
     int kc1 = 512, cols2 = 256;    // phase-1 column chunk, phase-2 columns per item (autotuned)
+    bool xcd1 = false;             // BF16 phase 1: items of one X chunk on one XCD (autotuned)
     bool f32 = false;              // FP32 / BF16 factors only: the FP32-core kernels, X32 and Z in FP32
     int p1rows = kP1Rows;          // a1^T rows per phase-1 item (kP1RowsBf on the BF16 cores)
     int r2[3][2] = {};             // phase-2 item range per dtype
@@ -952,6 +958,56 @@ std::vector<MrCache *> g_mr;
 }  // namespace
 
 namespace hbi {
+
+// Workgroup b of a launch runs on XCD b % 8 (round-robin dispatch) and each
+// XCD has its own L2.  Phase-1 items that read the same X rows (one column
+// chunk of one column cluster: the row groups of its stacked a1^T rows) are
+// placed on one XCD at consecutive positions, so the chunk is fetched into
+// that L2 once; the chunks are dealt largest first to the XCD with the least
+// MFMA work so far.  Positions left at the end of the shorter XCD lists hold
+// empty items (no rows, no columns).
+static void xcd_order_items1(MrPlan &M)
+{
+    constexpr int kX = 8;
+    auto work = [](const MItem1 &x) { return (long long)((x.kt + 15) / 16) * ((x.tn + 31) / 32); };
+    std::map<std::pair<int, int>, int> gid;
+    std::vector<std::vector<MItem1>> grp;
+    std::vector<long long> gw;
+    for (const MItem1 &it : M.it1) {
+        const auto key = std::make_pair(it.xoff + it.t0, it.tn);
+        auto f = gid.find(key);
+        int g;
+        if (f == gid.end()) {
+            g = (int)grp.size();
+            gid.emplace(key, g);
+            grp.emplace_back();
+            gw.push_back(0);
+        } else {
+            g = f->second;
+        }
+        grp[g].push_back(it);
+        gw[g] += work(it);
+    }
+    std::vector<int> ord(grp.size());
+    for (size_t g = 0; g < ord.size(); g++) ord[g] = (int)g;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return gw[a] > gw[b]; });
+    std::vector<MItem1> lane[kX];
+    long long lw[kX] = {};
+    for (int g : ord) {
+        int x = 0;
+        for (int k = 1; k < kX; k++)
+            if (lw[k] < lw[x]) x = k;
+        lane[x].insert(lane[x].end(), grp[g].begin(), grp[g].end());
+        lw[x] += gw[g];
+    }
+    size_t len = 0;
+    for (int x = 0; x < kX; x++) len = std::max(len, lane[x].size());
+    MItem1 empty{};
+    empty.part = -1;
+    M.it1.assign(len * kX, empty);
+    for (int x = 0; x < kX; x++)
+        for (size_t k = 0; k < lane[x].size(); k++) M.it1[k * kX + x] = lane[x][k];
+}
 
 // Phase-1 items.  Runs of blocks with the same column cluster and dtype are
 // adjacent in the packed layout (same ldt): whole blocks are packed into row
@@ -1018,10 +1074,9 @@ static void mr_items1(const Plan &P, MrPlan &M, int kc1)
     std::stable_sort(M.it1.begin(), M.it1.end(), [](const MItem1 &x, const MItem1 &y) {
         return (x.kt + 15) / 16 * ((x.tn + 31) / 32) > (y.kt + 15) / 16 * ((y.tn + 31) / 32);
     });
-    if (M.bf) {   // BF16 cores: items of up to 64 rows on 4 waves, the rest on 8
-        auto mid = std::stable_partition(M.it1.begin(), M.it1.end(), [](const MItem1 &x) { return x.kt <= 64; });
-        M.r1b[0] = (int)(mid - M.it1.begin());
-        M.r1b[1] = (int)M.it1.size();
+    if (M.bf) {   // BF16 cores: every item fits 4 waves (kP1RowsBf rows)
+        if (M.xcd1) xcd_order_items1(M);
+        M.r1b[0] = M.r1b[1] = (int)M.it1.size();
     }
     M.kc1 = kc1;
 }
@@ -1332,11 +1387,15 @@ static void mr_tune(hbi_hmat *h, int prec, MrPlan &M)
         if (ms < best) { best = ms; b2 = c; }
     }
     best = 1e30f;
-    for (int c : kc1s) {
-        mr_items1(P, M, c);
-        const float ms = time_it(1);
-        if (ms < best) { best = ms; b1 = c; }
-    }
+    bool bx = false;
+    for (int x = 0; x < (M.bf ? 2 : 1); x++)
+        for (int c : kc1s) {
+            M.xcd1 = x == 1;
+            mr_items1(P, M, c);
+            const float ms = time_it(1);
+            if (ms < best) { best = ms; b1 = c; bx = M.xcd1; }
+        }
+    M.xcd1 = bx;
     mr_items1(P, M, b1);
     mr_items2(h, P, M, b2);
     mr_upload(h, P, M);
@@ -1383,6 +1442,7 @@ extern "C" int hbi_hmat_matmul_plan(hbi_hmat *h, int prec, int *info)
         info[3] = (int)M.it2.size();
         info[4] = M.nzpart;
         info[5] = M.bf ? 2 : (M.f32 ? 1 : 0);
+        info[6] = M.xcd1 ? 1 : 0;
     });
 }
 

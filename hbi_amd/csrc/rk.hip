@@ -557,16 +557,16 @@ static void launch_stages(hbi_rk *r, int nstages, int mode)
     rk_stage0_kernel<<<g, kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A, 0);
     for (int s = 0; s < nstages; s++) {
         hmat_phase1(r->hs, r->opts.prec, r->x.p, r->z.p, r->ypart.p, r->stream, gate);
-        hmat_phase2(r->hs, r->opts.prec, r->z.p, r->ypart.p, r->hx.p, r->stream, gate);
+        hmat_phase2(r->hs, r->opts.prec, r->x.p, r->z.p, r->ypart.p, r->hx.p, r->stream, gate);
         if (r->hn) {
             hmat_phase1(r->hn, r->opts.prec, r->x.p, r->zn.p, r->ypart_n.p, r->stream, gate);
-            hmat_phase2(r->hn, r->opts.prec, r->zn.p, r->ypart_n.p, r->hx_n.p, r->stream, gate);
+            hmat_phase2(r->hn, r->opts.prec, r->x.p, r->zn.p, r->ypart_n.p, r->hx_n.p, r->stream, gate);
         }
         if (r->P.has_open) {
             hmat_phase1(r->hs2, r->opts.prec, r->xn.p, r->zo.p, r->ypart_o.p, r->stream, gate);
-            hmat_phase2(r->hs2, r->opts.prec, r->zo.p, r->ypart_o.p, r->hxo.p, r->stream, gate);
+            hmat_phase2(r->hs2, r->opts.prec, r->xn.p, r->zo.p, r->ypart_o.p, r->hxo.p, r->stream, gate);
             hmat_phase1(r->hn2, r->opts.prec, r->xn.p, r->zo_n.p, r->ypart_on.p, r->stream, gate);
-            hmat_phase2(r->hn2, r->opts.prec, r->zo_n.p, r->ypart_on.p, r->hxo_n.p, r->stream, gate);
+            hmat_phase2(r->hn2, r->opts.prec, r->xn.p, r->zo_n.p, r->ypart_on.p, r->hxo_n.p, r->stream, gate);
         }
         rk_stage_kernel<<<g, kRkThreads, 0, r->stream>>>(r->P, r->ctl.p, A, s, mode);
     }

@@ -22,11 +22,11 @@ program lhbiem_fv_gpu
   implicit none
   real(8), parameter :: pi = 4 * datan(1.d0), rigid = 32.04d0, yr = 365 * 24 * 3600d0
   character(len=256) :: input_file, param, pvalue, fname, problem, setting, bcl, bcr, evlaw, precision
-  integer :: ios, number, ncellg, nstep, interval, leaf, outbatch, i, k, r, nb, nrow, kdone, u
+  integer :: ios, number, ncellg, nstep, interval, leaf, outbatch, i, k, r, nb, nrow, kdone, u, nout
   real(8) :: ds0, tmax, tinj, kpmax, kpmin, eta, beta, phi0, q0, a0, b0, dc0, f00, tauinit, sigmainit
   real(8) :: velinit, pfinit, muinit, pbcl, pbcr, pinj, kL, kT, s0, eps_r, eps_h, vpl, dtout, velmin, velmax
   real(8) :: vref, Bs, dtinit, tout, meandisp0, moment, onset_time, kp0, lf
-  logical :: backslip, pressuredependent, pfconst, outpertime, slipping, outfield
+  logical :: backslip, pressuredependent, pfconst, outpertime, slipping, outfield, stopped
   integer :: eventcount, hypoloc, lrtrn
   integer(c_int) :: rc
   type(st_HACApK_calc_entry) :: st_bemv
@@ -298,10 +298,17 @@ program lhbiem_fv_gpu
   do while (kdone < nstep)
     nb = min(outbatch, nstep - kdone)
     if (interval > 0) nb = min(nb, interval - mod(kdone, interval))
+    ! the call also ends at the first step past tout (field output, main_fv.f90:1160-1163)
+    rc = hbi_fv_set_break(fv, tout)
     rc = hbi_fv_step(fv, int(nb, c_int), hist, info)
     call system_clock(c1)
     nrow = int(info%accepted) - kdone
-    do r = 1, nrow
+    ! a stop control ends main_fv's loop before the outputs of that step
+    ! (main_fv.f90:1136-1152 come before 1156-1239): no rows for it
+    stopped = info%status == 1 .or. info%status == 3 .or. info%status == 4
+    nout = nrow
+    if (stopped) nout = max(nrow - 1, 0)
+    do r = 1, nout
       k = kdone + r
       ! output_monitor, main_fv.f90:1307-1311 (xcol indexed by the cluster
       ! index of max |V|, as there)
@@ -326,11 +333,11 @@ program lhbiem_fv_gpu
     if (nprobe > 0) call write_local()
     kdone = int(info%accepted)
     outfield = interval > 0 .and. mod(kdone, max(interval, 1)) == 0 .and. nrow > 0
-    if (info%t > tout) then
+    if (info%t > tout .and. .not. stopped) then
       outfield = .true.
       tout = info%t + dtout * 365 * 24 * 60 * 60
     end if
-    if (outfield) call write_fields()
+    if (outfield .and. .not. stopped) call write_fields()
     if (info%status /= 0) then
       select case (info%status)
       case (1); write(*, *) 'slip rate outside [velmin, velmax] at time step ', kdone
@@ -362,7 +369,7 @@ contains
   subroutine write_local()
     integer :: j, rr, o
     rc = hbi_fv_get_probes(fv, nprow, prow)
-    do rr = 1, int(nprow)
+    do rr = 1, min(int(nprow), nout)
       do j = 1, nprobe
         o = 9 * (nprobe * (rr - 1) + j - 1)
         write(punit(j), '(i7,f19.4,9e16.6)') kdone + rr, hist(10 * rr - 9), log10(prow(o + 1)), prow(o + 2), &

@@ -32,6 +32,7 @@ module m_hbi_abi
     real(c_double) :: bf16_margin = 0d0
     integer(c_int) :: tile_rows = 0
     integer(c_int) :: chunk_kb = 0
+    integer(c_int) :: dense_launch = 0
   end type hbi_hmat_opts
 
   type, bind(C) :: hbi_rk_opts
@@ -317,6 +318,11 @@ module m_hbi_abi
       type(c_ptr), value :: f
       integer(c_long_long), intent(out) :: nrows
       real(c_double), intent(out) :: rows(*)
+    end function
+    integer(c_int) function hbi_fv_set_break(f, t) bind(C, name="hbi_fv_set_break")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: f
+      real(c_double), value :: t
     end function
     integer(c_int) function hbi_fv_step(f, nsteps, hist, info) bind(C, name="hbi_fv_step")
       import :: c_int, c_ptr, c_double, hbi_fv_info

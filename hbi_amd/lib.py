@@ -32,7 +32,8 @@ class Geom(C.Structure):
 
 class HmatOpts(C.Structure):
     _fields_ = [("leaf", C.c_int), ("eta", C.c_double), ("eps_aca", C.c_double), ("kmax", C.c_int),
-                ("bf16_margin", C.c_double), ("tile_rows", C.c_int), ("chunk_kb", C.c_int)]
+                ("bf16_margin", C.c_double), ("tile_rows", C.c_int), ("chunk_kb", C.c_int),
+                ("dense_launch", C.c_int)]
 
 
 class HmatStats(C.Structure):
@@ -41,7 +42,7 @@ class HmatStats(C.Structure):
                 ("lr_entries", C.c_longlong), ("lr_dims", C.c_longlong), ("stored_bytes", C.c_longlong * 3),
                 ("alg_bytes", C.c_longlong * 3), ("alg_bytes_p1", C.c_longlong * 3),
                 ("alg_bytes_p2", C.c_longlong * 3), ("chunk_p1", C.c_int * 3), ("chunk_p2", C.c_int * 3),
-                ("nbf16_blocks", C.c_int), ("ncapped", C.c_int),
+                ("dense_launch", C.c_int * 3), ("nbf16_blocks", C.c_int), ("ncapped", C.c_int),
                 ("t_partition", C.c_double), ("t_dense", C.c_double), ("t_aca", C.c_double), ("t_pack", C.c_double)]
 
     def as_dict(self):
@@ -122,7 +123,8 @@ SYMBOLS = ("hbi_version", "hbi_last_error", "hbi_device_count", "hbi_entries", "
            "hbi_ens_set_trace", "hbi_ens_get_trace", "hbi_ens_set_probes", "hbi_ens_set_step_grid", "hbi_rk_step_many", "hbi_rk_set_output", "hbi_rk_get_output",
            "hbi_rk_set_opening", "hbi_rk_get_opening", "hbi_rk_set_probes", "hbi_rk_get_probes",
            "hbi_fv_create", "hbi_fv_destroy", "hbi_fv_set_state", "hbi_fv_get_state", "hbi_fv_derivs",
-           "hbi_fv_diffuse", "hbi_fv_step", "hbi_fv_last_ms", "hbi_fv_set_probes", "hbi_fv_get_probes", "hbi_fv_set_wells")
+           "hbi_fv_diffuse", "hbi_fv_step", "hbi_fv_last_ms", "hbi_fv_set_probes", "hbi_fv_get_probes", "hbi_fv_set_wells",
+           "hbi_fv_set_break")
 
 
 def lib():
@@ -195,6 +197,7 @@ def lib():
         L.hbi_fv_step.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(FvInfo)]
         L.hbi_fv_last_ms.argtypes = [C.c_void_p, _dp]
         L.hbi_fv_set_probes.argtypes = [C.c_void_p, C.c_int, _ip]
+        L.hbi_fv_set_break.argtypes = [C.c_void_p, C.c_double]
         L.hbi_fv_get_probes.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), _dp]
         L.hbi_fv_set_wells.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp]
         L.hbi_rk_get_output.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), _dp, _dp,
@@ -254,10 +257,11 @@ def entries(g, ii, jj) -> np.ndarray:
 class HMatrix:
     """Device lattice H-matrix (HACApK_generate + HACApK_construct_LH)."""
 
-    def __init__(self, g, leaf=15, eta=2.0, eps=1e-4, kmax=1000, bf16_margin=0.0, tile_rows=0, chunk_kb=0):
+    def __init__(self, g, leaf=15, eta=2.0, eps=1e-4, kmax=1000, bf16_margin=0.0, tile_rows=0, chunk_kb=0,
+                 dense_launch=0):
         G = _GeomHold(g)
         o = HmatOpts(leaf=leaf, eta=eta, eps_aca=eps, kmax=kmax, bf16_margin=bf16_margin, tile_rows=tile_rows,
-                     chunk_kb=chunk_kb)
+                     chunk_kb=chunk_kb, dense_launch=dense_launch)
         h = C.c_void_p()
         _check(lib().hbi_hmat_build(C.byref(G.s), C.byref(o), C.byref(h)))
         self.h = h
@@ -323,10 +327,11 @@ class HMatrix:
 
     def matmul_plan(self, prec=PREC_F64) -> dict:
         """The autotuned 64-column plan: chunk sizes, item counts, matrix cores."""
-        info = np.zeros(6, dtype=np.int32)
+        info = np.zeros(7, dtype=np.int32)
         _check(lib().hbi_hmat_matmul_plan(self.h, int(prec), info.ctypes.data_as(_ip)))
         return {"kc1": int(info[0]), "cols2": int(info[1]), "items1": int(info[2]), "items2": int(info[3]),
-                "zpart_slots": int(info[4]), "cores": ("fp64", "fp32", "bf16")[int(info[5])]}
+                "zpart_slots": int(info[4]), "cores": ("fp64", "fp32", "bf16")[int(info[5])],
+                "xcd_phase1": bool(info[6])}
 
 
 def mfma_f64_rate(waves_per_simd=2, iters=4000) -> float:
@@ -756,6 +761,10 @@ class FV:
         if history:
             hist = hist[:max(0, min(nsteps, done))]
         return d, hist
+
+    def set_break(self, t=np.inf):
+        """Later step calls also end after the first accepted step with time > t."""
+        _check(lib().hbi_fv_set_break(self.f, float(t)))
 
     def last_ms(self) -> float:
         ms = C.c_double(0)

@@ -86,6 +86,9 @@ typedef struct hbi_hmat_opts {
                             0 disables BF16 */
     int tile_rows;       /* matvec row tile (32 or 64); 0 = from leaf size */
     int chunk_kb;        /* matvec work-item size target in KiB; 0 = default */
+    int dense_launch;    /* single-vector product: the launch that streams the dense blocks,
+                            1 = the first (with the a1^T tiles), 2 = the second (with the
+                            a2 columns, per row tile); 0 = picked by the autotuner */
 } hbi_hmat_opts;
 
 typedef struct hbi_hmat_stats {
@@ -99,6 +102,7 @@ typedef struct hbi_hmat_stats {
     long long alg_bytes_p1[3];   /* first launch: a1 and dense factors + x read + z written */
     long long alg_bytes_p2[3];   /* second launch: a2 factors + z read + y written */
     int chunk_p1[3], chunk_p2[3];   /* work-item bytes per precision chosen by the autotuner */
+    int dense_launch[3];         /* launch (1 or 2) that streams the dense blocks, per precision */
     int nbf16_blocks;            /* low-rank blocks stored as BF16 in HBI_PREC_MIXED */
     int ncapped;                 /* low-rank blocks whose ACA reached kmax */
     double t_partition, t_dense, t_aca, t_pack;   /* build phase times, seconds */
@@ -158,7 +162,8 @@ int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms);
    info[0] phase-1 column chunk, [1] phase-2 columns per item, [2] phase-1
    items, [3] phase-2 items (= FP64 partial row tiles written), [4] phase-1
    partial slots, [5] the matrix cores used: 0 FP64, 1 FP32, 2 BF16 (split
-   FP32 operands). */
+   FP32 operands), [6] 1 if the phase-1 items that read the same input rows
+   are placed on one XCD (BF16 cores). */
 int hbi_hmat_matmul_plan(hbi_hmat *h, int prec, int *info);
 /* Measured FP64 matrix-core rate (TFLOP/s) of v_mfma_f64_16x16x4f64, the
    instruction of both 64-column kernels: waves_per_simd (1..8) waves per SIMD
@@ -503,6 +508,11 @@ int hbi_fv_diffuse(hbi_fv *f, const double *pf, const double *y, const double *s
    cluster index of the first max |V|, errmax, dt, CG iterations (the
    output_monitor columns of main_fv.f90:1307-1311). */
 int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info);
+/* The following hbi_fv_step calls also end (status 0) after the first
+   accepted step whose time exceeds t: main_fv's time-based field output
+   (x > tout, main_fv.f90:1160-1163) then falls on the last step of a call.
+   HUGE_VAL (the default) disables it. */
+int hbi_fv_set_break(hbi_fv *f, double t);
 int hbi_fv_last_ms(const hbi_fv *f, double *ms);
 /* Local output (main_fv.f90:862-868, output_local at 1312-1316): up to 16
    cluster-order elements; after every accepted step of the following step

@@ -33,7 +33,7 @@ def test_version_and_argument_errors_without_gpu():
     rc = L.hbi_hmat_build(None, None, C.byref(out))
     assert rc == -1 and b"NULL" in L.hbi_last_error()
     g = H.Geom()
-    g.kind, g.n = 11, 10
+    g.kind, g.n = 14, 10
     o = H.HmatOpts(leaf=15, eta=2.0, eps_aca=1e-4, kmax=1000)
     assert L.hbi_hmat_build(C.byref(g), C.byref(o), C.byref(out)) == -1
     assert b"kind" in L.hbi_last_error()

@@ -315,3 +315,74 @@ def test_gpu_fluid_driver_steadystate_setting(tmp_path):
     xcol = (np.arange(n) + 0.5) * ds0
     assert np.allclose(pfs[0], pbcl + (0.0 - pbcl) * xcol / lf, rtol=1e-12, atol=1e-12 * pbcl)
     assert np.abs(pfs[-1] - pfs[0]).max() > 0
+
+
+def test_gpu_fluid_break_time_ends_the_call_not_the_run():
+    """hbi_fv_set_break: the call ends (status 0) at the first accepted step
+    past the break time, and continuing from there is the unbroken run, row
+    for row."""
+    n = 256
+    g = P.mesh_2dp(n, 0.01)
+    Hs = H.HMatrix(g, leaf=32, eps=1e-4)
+    opts, arr, y, pf = P.fluid_bp6a(g, Hs.lod)
+    a = H.FV(Hs, opts, arr)
+    a.set_state(y, pf, t=0.0, dt=1.0)
+    _, ha = a.step(60, history=True)
+    b = H.FV(Hs, opts, arr)
+    b.set_state(y, pf, t=0.0, dt=1.0)
+    tb = 0.5 * (ha[19, 0] + ha[20, 0])
+    b.set_break(tb)
+    info, hb = b.step(60, history=True)
+    print("break", tb, info["accepted"], flush=True)
+    assert info["status"] == 0 and info["accepted"] == 21 and len(hb) == 21
+    assert hb[-2, 0] <= tb < hb[-1, 0]
+    b.set_break()
+    info, hb2 = b.step(39, history=True)
+    assert info["accepted"] == 60
+    assert np.array_equal(np.concatenate([hb, hb2]), ha)
+
+
+def test_gpu_fluid_driver_stop_step_and_time_based_outputs(tmp_path):
+    """lhbiem_fv_gpu: a stop control (t > tmax) ends the run before the
+    stopping step's outputs, as main_fv.f90:1136-1152 precede 1156-1239 (no
+    monitor, local or field row for it); field records by time (dtout) fall on
+    the first step past each tout (main_fv.f90:1160-1163), not at the end of
+    an output batch.  Steps and times from the library run of the same bp6a
+    set-up."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    n, nstep = 1000, 300
+    g = P.mesh_2dp(n, 0.01)
+    Hs = H.HMatrix(g, leaf=32, eps=1e-4)
+    opts, arr, y, pf = P.fluid_bp6a(g, Hs.lod)
+    fv = H.FV(Hs, opts, arr)
+    fv.set_state(y, pf, t=0.0, dt=1.0)
+    _, hist = fv.step(80, history=True)
+    t = hist[:, 0]
+    tmax_s = 0.5 * (t[59] + t[60])              # the run stops at step 61
+    dtout_s = 0.5 * (t[14] + t[15])             # first field record at step 16
+    expect, tout = [0], dtout_s
+    for k in range(1, 61):                      # step 61 stops before its outputs
+        if t[k - 1] > tout:
+            expect.append(k)
+            tout = t[k - 1] + dtout_s
+    assert len(expect) >= 3
+    keys = {"problem": "2dp", "ncellg": n, "filenumber": 5, "interval": 100000, "nstep": nstep,
+            "tmax": tmax_s / P.YEAR_S, "dtout": dtout_s / P.YEAR_S, "outbatch": 64,
+            "tinj": 0.27397, "ds": 0.01, "kpmax": 1e-13, "kpmin": 1e-13, "eta": 1e-3, "beta": 1e-8, "phi0": 0.1,
+            "q0": 1.25e-6, "a": 0.007, "b": 0.005, "dc": 0.004, "f0": 0.6, "tauinit": 29.2, "sigmainit": 50.0,
+            "velinit": "1d-12"}
+    P.write_in_file(str(tmp_path / "bp6.in"), keys)
+    r = subprocess.run([os.path.join(root, "hbi_amd", "lhbiem_fv_gpu"), "bp6.in"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0 and "tmax" in r.stdout
+    out = tmp_path / "output"
+    mon = np.loadtxt(out / "monitor5.dat")
+    assert np.array_equal(mon[:, 0], np.arange(61))
+    loc = np.loadtxt(out / ("local5-%d.dat" % int(0.5 / 5 * n)))
+    assert np.array_equal(loc[:, 0], np.arange(1, 61))
+    tr = np.loadtxt(out / "time5.dat", ndmin=2)
+    assert list(tr[:, 0].astype(int)) == expect
+    assert np.fromfile(out / "pf5.dat").size == len(expect) * n

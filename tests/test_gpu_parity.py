@@ -117,6 +117,30 @@ def test_gpu_lowrank_blocks_match_oracle_factors():
         assert np.linalg.norm(Bg - Bo) <= 1e-8 * np.linalg.norm(Bo) + 1e-12
 
 
+@pytest.mark.parametrize("leaf,chunk_kb", [(32, 0), (64, 0), (64, 32)])
+def test_gpu_dense_blocks_in_either_launch(leaf, chunk_kb):
+    """The single-vector product with its dense blocks streamed by the first
+    launch (partial slots per tile) or by the second (inside the row-tile
+    item): the same sum up to its order, against the oracle's product."""
+    g = P.mesh_3dph(64, 32, 1.0)
+    Ho = O.HMatrix(g, leaf=leaf, eps=1e-5, nthreads=8)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(g.n)
+    yo = Ho.matvec(x)
+    ys = {}
+    for dl in (1, 2):
+        Hg = H.HMatrix(g, leaf=leaf, eps=1e-5, bf16_margin=1.0, chunk_kb=chunk_kb, dense_launch=dl)
+        for prec in (H.PREC_F64, H.PREC_F32, H.PREC_MIXED):
+            print("leaf", leaf, "chunk_kb", chunk_kb, "dense_launch", dl, "prec", prec, flush=True)
+            assert Hg.stats["dense_launch"][prec] == dl
+            ys[dl, prec] = Hg.matvec(x, prec)
+        Hg.close()
+    for prec in (H.PREC_F64, H.PREC_F32, H.PREC_MIXED):
+        assert _relerr(ys[1, prec], ys[2, prec]) <= 1e-13
+    assert _relerr(ys[2, H.PREC_F64], yo) <= 1e-5
+    assert _relerr(ys[2, H.PREC_F32], yo) <= 1e-5
+
+
 def test_gpu_dense_leaves_match_oracle_entries():
     g = P.mesh_3dph(32, 16, 1.0)
     Hg = H.HMatrix(g, leaf=32, eps=1e-5)
