@@ -119,9 +119,14 @@ __device__ __forceinline__ EPair e_combine(EPair l, EPair r)
     return EPair{l.g, fmax(l.v, r.v)};
 }
 
-template <int s>
+template <int s, bool SC>
 __global__ __launch_bounds__(256) void ens_stage_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A)
 {
+    // SC: sigdot is zero for every element and realisation and there is no
+    // normal-stress product, so every sigma slope is exactly 0 (deriv's
+    // dsigdt = gn = 0 + 0) and the stage states keep sigma = y bit for bit:
+    // the sigma slopes are neither stored nor read back (a third of the
+    // slope traffic of a stage)
 #pragma clang fp contract(off)
     __shared__ int sg[4][kR];
     __shared__ double sv[4][kR];
@@ -148,7 +153,7 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(EnsPar P, const EnsCtl *
             const size_t e = e0 + cc * kR;
             yv[cc] = A.y[e];
 #pragma unroll
-            for (int j = 0; j < s; j++) kv[cc][j] = A.k[j * m3 + e];
+            for (int j = 0; j < s; j++) kv[cc][j] = SC && cc == 2 ? 0.0 : A.k[j * m3 + e];
             ys[cc] = s == 0 ? yv[cc] : stage_state(P, s, h, yv[cc], kv[cc]);
         }
         double kk[3];
@@ -158,7 +163,7 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(EnsPar P, const EnsCtl *
         double *ks = A.k + s * m3;
 #pragma unroll
         for (int cc = 0; cc < 3; cc++) {
-            ks[e0 + cc * kR] = kk[cc];
+            if (!(SC && cc == 2)) ks[e0 + cc * kR] = kk[cc];
             kv[cc][s] = kk[cc];
         }
         if constexpr (s < 5) {
@@ -480,6 +485,7 @@ struct hbi_ens {
     hbi::EnsPar P{};
     int n = 0, nreal = 0, nblk = 0;
     int shared = 0;   // arrays kept once per element (bit k: EnsArrays order)
+    int sig_const = 0;   // every sigdot is 0: sigma slopes are zero and not stored
     hbi::DevBuf<double> a, b, dc, f0, vpl, taudot, sigdot;
     hbi::DevBuf<double> y, yout, k, vel, x, slip, velc, errpart, monpart, z, zpart, ypart;
     hbi::DevBuf<float> x32;      // FP32 copy of x for the FP32-core product
@@ -525,12 +531,15 @@ static void e_attempt(hbi_ens *E)
     const int *gate = &E->ctl.p->done;
     ens_stage0_kernel<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
     using StageFn = void (*)(EnsPar, const EnsCtl *, EnsArrays);
-    static const StageFn stage[6] = {ens_stage_kernel<0>, ens_stage_kernel<1>, ens_stage_kernel<2>,
-                                     ens_stage_kernel<3>, ens_stage_kernel<4>, ens_stage_kernel<5>};
+    static const StageFn stage[2][6] = {
+        {ens_stage_kernel<0, false>, ens_stage_kernel<1, false>, ens_stage_kernel<2, false>,
+         ens_stage_kernel<3, false>, ens_stage_kernel<4, false>, ens_stage_kernel<5, false>},
+        {ens_stage_kernel<0, true>, ens_stage_kernel<1, true>, ens_stage_kernel<2, true>,
+         ens_stage_kernel<3, true>, ens_stage_kernel<4, true>, ens_stage_kernel<5, true>}};
     for (int s = 0; s < 6; s++) {
         mr_apply_parts(E->hs, E->opts.prec, E->x32_direct ? nullptr : E->x.p, E->x32.p, E->zpart.p, E->z.p,
                        E->ypart.p, E->stream, gate);
-        stage[s]<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
+        stage[E->sig_const][s]<<<g, 256, 0, E->stream>>>(E->P, E->ctl.p, A);
     }
     ens_fold_kernel<<<kFold, 1024, 0, E->stream>>>(E->ctl.p, E->errpart.p, g, E->fold.p, 0);
     ens_decide_kernel<<<1, kDecThreads, 0, E->stream>>>(E->P, E->ctl.p, E->fold.p, kFold);
@@ -602,6 +611,8 @@ int hbi_ens_create(hbi_hmat *hs, const hbi_rk_opts *opts, int nreal, const doubl
             const double *src[hbi::kNPar] = {a, b, dc, f0, vpl, taudot, sigdot};
             hbi::DevBuf<double> *dst[hbi::kNPar] = {&E->a, &E->b, &E->dc, &E->f0, &E->vpl, &E->taudot, &E->sigdot};
             E->shared = 0;
+            E->sig_const = 1;
+            for (size_t q = 0; q < (size_t)nreal * n && E->sig_const; q++) E->sig_const = sigdot[q] == 0.0;
             for (int k = 0; k < hbi::kNPar; k++) {
                 bool same = true;
                 for (int r = 1; r < nreal && same; r++)
