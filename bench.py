@@ -15,8 +15,8 @@ realisations of the fault's friction (seeds 1..64) advanced in lockstep with
 one 64-column H product per RK stage, FP32 factors split into three BF16
 parts on the BF16 matrix cores (FP64 state), each
 realisation's next step proposed by the step-size MLP trained on this fault
-(hbi_amd/data/step_mlp_c2.npz; the embedded error check judges every step)
--- as accepted steps/s summed over the realisations, over K lockstep steps
+(hbi_amd/data/step_mlp_c2.npz; within [1/8, 1] x the rkqs proposal; the
+embedded error check judges every step) -- as accepted steps/s summed over the realisations, over K lockstep steps
 after W warm-up steps.  The same ensemble under the reference's controller
 (rkqs) is measured beside it.
 Alongside: single-run FP64 (the plain GPU form) and mixed, the single-vector
@@ -185,7 +185,7 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
               file=sys.stderr, flush=True)
         E = H.Ensemble(Hm, frs, lod, hdiv=DS0, eps_r=EPS_R, prec=H.PREC_MIXED)
         if learned:
-            E.set_controller(1, mlp, 0.9)
+            E.set_controller(1, mlp, **L.C2_BOUNDS)
         E.set_state(ys, 0.0, 1.0)
         hw = E.step(warmup, history=True)[1] if warmup else np.zeros((NREAL, 0, 5))
         t0 = time.perf_counter()
@@ -322,6 +322,7 @@ def controllers_by_phase(H, Hm, frs, lod, ys, mlp, warmup, steps):
     event at 3000); from that state (per-realisation t and next step) each
     controller takes `warmup` untimed and `steps` timed lockstep steps:
     accepted steps/s, lockstep attempts and simulated time per wall-second."""
+    from hbi_amd import learned as L
     S = H.Ensemble(Hm, frs, lod, hdiv=DS0, eps_r=EPS_R, prec=H.PREC_MIXED)
     S.set_state(ys, 0.0, 1.0)
     done, rows = 0, []
@@ -336,7 +337,7 @@ def controllers_by_phase(H, Hm, frs, lod, ys, mlp, warmup, steps):
         for key, learned in (("rkqs", False), ("learned", True)):
             E = H.Ensemble(Hm, frs, lod, hdiv=DS0, eps_r=EPS_R, prec=H.PREC_MIXED)
             if learned:
-                E.set_controller(1, mlp, 0.9)
+                E.set_controller(1, mlp, **L.C2_BOUNDS)
             E.set_state(st["y"], 0.0, 1.0)
             E.set_clock(t, h)
             if warmup:

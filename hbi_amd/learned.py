@@ -20,6 +20,16 @@ from . import problems as P
 
 N_FEAT = 12
 
+# Proposal bounds of the 3-D ensemble controller: safety 0.9 and the learned
+# step within [1/8, 1] x the rkqs proposal, i.e. the network may shorten the
+# reference's proposal (to step short of a predicted rejection) but never
+# lengthen it.  On the 32768-element fault (64 realisations, mixed precision)
+# this never loses to rkqs: equal attempts in the 2x-growth start-up from
+# t = 0, where rkqs never rejects, and fewer rejections from every spun-up
+# state; an upper bound of 8 x rkqs buys longer steps that are rejected more
+# often (start-up: 25 lockstep attempts for 20 steps against 20).
+C2_BOUNDS = dict(safety=0.9, ratio_lo=0.125, ratio_hi=1.0)
+
 
 def c1_realisation(g, seed: int):
     """Friction of the 2-D antiplane fault with lognormal perturbations of Dc
