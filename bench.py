@@ -41,15 +41,18 @@ HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
 FP32_MFMA_PEAK_TFS = 157.3
 BF16_MFMA_PEAK_TFS = 2500.0
-# HBM bytes per launch of the two roofline kernels (mixed precision), from
+# HBM bytes per launch of the roofline kernels (mixed precision), from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE passes over
 # hbi_amd/profile_kernels.py, scaled by the counters of a 256 MiB device copy
-# (fetch x 2.00, write x 1.00): mr_bf16_kernel<4,true,2,2> (137317.2 KiB
+# (fetch x 2.00, write x 1.00): mr_bf16_kernel<4,true,2,2> (140776.1 KiB
 # fetched, 1016 x 32 KiB partial tiles written at the tuner's usual 1016 items
-# of up to 1024 columns) and stream_kernel (110083.2 KiB fetched, 2887.5 KiB
-# written)
-TRAFFIC_MR_BF16_MIXED = 314.5e6
-TRAFFIC_STREAM_MIXED = 228.4e6
+# of up to 1024 columns); the single-vector product's second launch with the
+# dense blocks in it (lowrank_kernel, --dense-launch 2: 110648.8 KiB fetched,
+# 256 KiB written) and its first launch when it holds them (stream_kernel,
+# --dense-launch 1: 109129.8 KiB fetched, 2887.5 KiB written)
+TRAFFIC_MR_BF16_MIXED = 321.6e6
+TRAFFIC_LOWRANK_DENSE_MIXED = 226.9e6
+TRAFFIC_STREAM_DENSE_MIXED = 226.5e6
 # rocprofv3 --pmc MfmaUtil over the same program: fraction of cycles the
 # matrix cores of mr_bf16_kernel<4,true,2,2> are busy
 MFMA_UTIL_MR_BF16 = 0.231
@@ -83,7 +86,7 @@ def gpu_part(steps: int, warmup: int, single_steps: int):
     p2_entries = int((lv[dense, 2] * lv[dense, 4]).sum() + (lv[lr, 2] * lv[lr, 5]).sum())
     out = {"build_s": round(t_build, 3),
            "hmat": {k: st[k] for k in ("nlr", "ndense", "kt_mean", "kt_max", "dense_entries", "lr_entries",
-                                       "nbf16_blocks")}}
+                                       "nbf16_blocks", "dense_launch")}}
 
     # single-vector products (HIP events on the library stream)
     mv = {}
@@ -475,6 +478,14 @@ def main():
     ens0 = r["ensemble_rkqs"]
     mmx = r["matmul64"]["mixed"]
     mvx = r["matvec"]["mixed"]
+    # the single-vector product's longer launch: the second when it streams the dense blocks too
+    if r["hmat"]["dense_launch"][2] == 2:
+        mv_kernel = ("lowrank_kernel<64> (single-vector product, second launch: per row tile the dense and a2 "
+                     "columns as one run, one x/z gather, mixed)")
+        mv_gbps, mv_traffic = mvx["b_GBps"], TRAFFIC_LOWRANK_DENSE_MIXED
+    else:
+        mv_kernel = "stream_kernel<64> (single-vector product, first launch: stacked a1^T tiles + dense tiles, mixed)"
+        mv_gbps, mv_traffic = mvx["a_GBps"], TRAFFIC_STREAM_DENSE_MIXED
     f64 = r["single"]["f64"]["steps_per_s"]
     line = {
         "metric": "RK steps/s & H-matvec GB/s (%HBM peak), mixed+learned vs FP64, 1e-5 rel err",
@@ -506,10 +517,9 @@ def main():
                      "mfma_util_counter": MFMA_UTIL_MR_BF16,
                      "note": "arithmetic intensity 2*64 flop per 4-byte factor (32 flop/B) is below the BF16 ridge "
                              "(312 flop/B): HBM-bound; six BF16 MFMA terms per FP32 product"},
-        "roofline_matvec": {"bound": "hbm", "kernel": "stream_kernel<64> (single-vector product, first launch: "
-                                                      "stacked a1^T tiles + dense tiles, mixed)",
-                            "achieved": round(mvx["a_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(mvx["a_GBps"] / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_STREAM_MIXED,
+        "roofline_matvec": {"bound": "hbm", "kernel": mv_kernel,
+                            "achieved": round(mv_gbps, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(mv_gbps / HBM_PEAK_GBS, 4), "traffic": mv_traffic,
                             "whole_product_frac": {k: round(v["GBps"] / HBM_PEAK_GBS, 4)
                                                    for k, v in r["matvec"].items()}},
         "fp64_single_steps_per_s": round(f64, 2),

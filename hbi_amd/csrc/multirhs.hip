@@ -1386,16 +1386,20 @@ static void mr_tune(hbi_hmat *h, int prec, MrPlan &M)
         const float ms = time_it(2);
         if (ms < best) { best = ms; b2 = c; }
     }
-    best = 1e30f;
-    bool bx = false;
+    // phase 1: column chunk for each item order; the XCD placement fetches
+    // about half the bytes from beyond L2 (the X chunks once per XCD) and is
+    // kept unless it is more than 2% slower
+    float bt[2] = {1e30f, 1e30f};
+    int bc[2] = {512, 512};
     for (int x = 0; x < (M.bf ? 2 : 1); x++)
         for (int c : kc1s) {
             M.xcd1 = x == 1;
             mr_items1(P, M, c);
             const float ms = time_it(1);
-            if (ms < best) { best = ms; b1 = c; bx = M.xcd1; }
+            if (ms < bt[x]) { bt[x] = ms; bc[x] = c; }
         }
-    M.xcd1 = bx;
+    M.xcd1 = M.bf && bt[1] <= 1.02f * bt[0];
+    b1 = bc[M.xcd1 ? 1 : 0];
     mr_items1(P, M, b1);
     mr_items2(h, P, M, b2);
     mr_upload(h, P, M);

@@ -12,7 +12,8 @@
 ! per-step monitor values), the vel / slip / sigma / tau / ks / pf / qflow<n>.dat
 ! FP64 streams of every output step (1155-1198), xyz / ind<n>.dat
 ! (811-824) and the five local<n>-<id>.dat files (862-868, 1312-1316).  Output steps: every interval-th step exactly (a batch ends
-! there); t > tout is checked at the end of each batch of `outbatch` steps.
+! there) and the first step past each tout (hbi_fv_set_break ends the batch there); the step a stop
+! control ends gets no rows (main_fv exits at 1136-1152, before its outputs).
 ! Keys beyond the reference's: vref (main_fv leaves it undeclared; 1e-6),
 ! leaf, precision, outbatch.
 program lhbiem_fv_gpu

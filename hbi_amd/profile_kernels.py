@@ -1,7 +1,7 @@
 """Kernel timings of the H products on the 32768-element planar fault, for
 profilers (rocprofv3 --kernel-trace / --pmc) and for the roofline figures.
 
-    python -m hbi_amd.profile_kernels [--iters N] [--copy-mb M]
+    python -m hbi_amd.profile_kernels [--iters N] [--copy-mb M] [--dense-launch 0|1|2]
 
 Builds the 256 x 128 half-space fault's lattice H-matrix (leaf 64, ACA eps
 1e-5) and times, with HIP events on the library stream, the single-vector
@@ -37,6 +37,8 @@ def main():
     ap.add_argument("--ensemble", type=int, default=0)
     ap.add_argument("--fluid", type=int, default=0)
     ap.add_argument("--fluid3dp", type=int, default=0)
+    ap.add_argument("--dense-launch", type=int, default=0,
+                    help="single-vector product layout: 0 autotuned, 1 dense blocks in the first launch, 2 in the second")
     a = ap.parse_args()
     if a.fluid > 0:
         print(json.dumps(fluid(a.fluid)), flush=True)
@@ -45,7 +47,7 @@ def main():
         print(json.dumps(fluid3dp(a.fluid3dp)), flush=True)
         return
     g = P.mesh_3dph(IMAX, JMAX, DS0, 90.0)
-    Hm = H.HMatrix(g, leaf=LEAF, eps=EPS_H, bf16_margin=1.0)
+    Hm = H.HMatrix(g, leaf=LEAF, eps=EPS_H, bf16_margin=1.0, dense_launch=a.dense_launch)
     if a.ensemble > 0:
         print(json.dumps(ensemble(g, Hm, a.ensemble)), flush=True)
         return
@@ -55,7 +57,8 @@ def main():
     out = {"n": g.n, "iters": a.iters,
            "p1_entries": int((lv[lr, 4] * lv[lr, 5]).sum()),
            "p2_entries": int((lv[dense, 2] * lv[dense, 4]).sum() + (lv[lr, 2] * lv[lr, 5]).sum()),
-           "alg_bytes_p1": st["alg_bytes_p1"], "alg_bytes_p2": st["alg_bytes_p2"], "matvec": {}, "matmul64": {}}
+           "alg_bytes_p1": st["alg_bytes_p1"], "alg_bytes_p2": st["alg_bytes_p2"], "dense_launch": st["dense_launch"],
+           "matvec": {}, "matmul64": {}}
     for prec, name in ((H.PREC_F64, "f64"), (H.PREC_F32, "f32"), (H.PREC_MIXED, "mixed")):
         out["matvec"][name] = Hm.time_matvec(prec, a.iters)
     for prec, name in ((H.PREC_F64, "f64"), (H.PREC_F32, "f32"), (H.PREC_MIXED, "mixed")):
@@ -66,7 +69,7 @@ def main():
 
 
 def ensemble(g, Hm, steps: int) -> dict:
-    """The bench's learned-step ensemble: 20 warm-up, then `steps` lockstep steps."""
+    """The bench's learned-step ensemble (its proposal bounds): 20 warm-up, then `steps` lockstep steps."""
     import os
     from . import learned as L
     lod = Hm.lod
@@ -77,7 +80,7 @@ def ensemble(g, Hm, steps: int) -> dict:
         P.perturb_state(y, g, lod, amp=0.02, depth=15.0, width=3.0)
         ys.append(y)
     E = H.Ensemble(Hm, frs, lod, hdiv=DS0, eps_r=1e-6, prec=H.PREC_MIXED)
-    E.set_controller(1, L.load(os.path.join(os.path.dirname(__file__), "data", "step_mlp_c2.npz")), 0.9)
+    E.set_controller(1, L.load(os.path.join(os.path.dirname(__file__), "data", "step_mlp_c2.npz")), **L.C2_BOUNDS)
     E.set_state(np.array(ys), 0.0, 1.0)
     E.step(20)
     att = E.step(steps)
