@@ -361,15 +361,25 @@ __global__ __launch_bounds__(256) void ens_hist_kernel(EnsPar P, const EnsCtl *_
     const int i0 = m * per + ((int)threadIdx.x / kHistR) * run;
     const int i1 = min(min(P.n, (m + 1) * per), i0 + run);
     if (r < P.nreal && c->acc[r]) {
+        // kHistLd values loaded together (indices clamped into the run), then
+        // binned in element order
+        constexpr int kHistLd = 8;
         int cur_b = -1, cur_n = 0;
-        for (int i = i0; i < i1; i++) {
-            const int b = vbin(fabs(A.velc[(size_t)i * kR + r]));
-            if (b == cur_b) {
-                cur_n++;
-            } else {
-                if (cur_n) atomicAdd(&hl[rl][cur_b], cur_n);
-                cur_b = b;
-                cur_n = 1;
+        for (int i = i0; i < i1; i += kHistLd) {
+            double v[kHistLd];
+#pragma unroll
+            for (int k = 0; k < kHistLd; k++) v[k] = A.velc[(size_t)min(i + k, i1 - 1) * kR + r];
+#pragma unroll
+            for (int k = 0; k < kHistLd; k++) {
+                if (i + k >= i1) break;
+                const int b = vbin(fabs(v[k]));
+                if (b == cur_b) {
+                    cur_n++;
+                } else {
+                    if (cur_n) atomicAdd(&hl[rl][cur_b], cur_n);
+                    cur_b = b;
+                    cur_n = 1;
+                }
             }
         }
         if (cur_n) atomicAdd(&hl[rl][cur_b], cur_n);

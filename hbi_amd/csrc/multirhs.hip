@@ -344,16 +344,19 @@ __global__ void mr_narrow_kernel(size_t n4, const double *__restrict__ X, float 
     ((fvec4 *)X32)[q] = fvec4{(float)a[0], (float)a[1], (float)b[0], (float)b[1]};
 }
 
+// Z rows of the blocks whose phase-1 rows were split into column chunks:
+// the partial slots summed in slot order.  Workgroup q takes rows
+// r0 .. r0 + 3 of block blk (zq[q] = (blk, r0), rows < kt only), one
+// right-hand side per thread.
 template <typename OT>
-__global__ void mr_zreduce_kernel(const MBlk1 *__restrict__ blks, int nblk, int prow, const double *__restrict__ zpart,
-                                  OT *__restrict__ Z, const int *__restrict__ gate)
+__global__ __launch_bounds__(4 * kRR) void mr_zreduce_kernel(const MBlk1 *__restrict__ blks, const int2 *__restrict__ zq,
+                                                            int prow, const double *__restrict__ zpart,
+                                                            OT *__restrict__ Z, const int *__restrict__ gate)
 {
-    // workgroup (b, k): row k of block b, one right-hand side per thread (64
-    // threads); prow: rows per partial slot
     if (gate && *gate) return;
-    const int b = blockIdx.x / prow, k = blockIdx.x % prow, rr = threadIdx.x;
-    if (b >= nblk) return;
-    const MBlk1 B = blks[b];
+    const int2 q = zq[blockIdx.x];
+    const MBlk1 B = blks[q.x];
+    const int k = q.y + (int)threadIdx.x / kRR, rr = (int)threadIdx.x % kRR;
     if (k >= B.kt) return;
     double s = 0.0;
     for (int p = B.p0; p < B.p1; p++) s += zpart[((size_t)p * prow + B.roff + k) * kRR + rr];
@@ -917,6 +920,8 @@ struct MrPlan {
     bool built = false;
     int kc1 = 512, cols2 = 256;    // phase-1 column chunk, phase-2 columns per item (autotuned)
     bool xcd1 = false;             // BF16 phase 1: items of one X chunk on one XCD (autotuned)
+    int p1ns = kBfStages;          // BF16 phase 1: register stages of the load ring (autotuned)
+    int p2ns = kBfStages;          // BF16 phase 2 (64-row tiles): the same (autotuned)
     bool f32 = false;              // FP32 / BF16 factors only: the FP32-core kernels, X32 and Z in FP32
     int p1rows = kP1Rows;          // a1^T rows per phase-1 item (kP1RowsBf on the BF16 cores)
     int r2[3][2] = {};             // phase-2 item range per dtype
@@ -928,6 +933,8 @@ struct MrPlan {
     std::vector<int> part0;        // ntiles + 1: partial slots of each row tile
     DevBuf<MItem1> d_it1;
     DevBuf<MBlk1> d_blk1;
+    std::vector<int2> zq;          // (block of blk1, first of 4 rows) per workgroup of the Z reduction
+    DevBuf<int2> d_zq;
     DevBuf<Item2> d_it2;
     DevBuf<MItem2> d_it2f;
     DevBuf<int> d_cmap;
@@ -1179,6 +1186,10 @@ static void mr_upload(hbi_hmat *h, const Plan &P, MrPlan &M)
 {
     M.d_it1.upload(M.it1, h->stream);
     M.d_blk1.upload(M.blk1, h->stream);
+    M.zq.clear();
+    for (size_t b = 0; b < M.blk1.size(); b++)
+        for (int r = 0; r < M.blk1[b].kt; r += 4) M.zq.push_back(int2{(int)b, r});
+    M.d_zq.upload(M.zq, h->stream);
     M.d_it2.upload(M.it2, h->stream);
     M.d_it2f.upload(M.it2f, h->stream);
     M.d_cmap.upload(M.cmap.empty() ? std::vector<int>{0} : M.cmap, h->stream);
@@ -1250,24 +1261,27 @@ static void phase1(hbi_hmat *h, const Plan &P, const MrPlan &M, const double *X,
     if (M.it1.empty()) return;
     if (!M.f32) {
         mr_lrvt_kernel<<<(int)M.it1.size(), 64, 0, s>>>(M.d_it1.p, P.buf.p, X, zpart, P.d_zrow.p, Z, gate);
-        if (!M.blk1.empty())
-            mr_zreduce_kernel<double><<<(int)M.blk1.size() * M.p1rows, kRR, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(),
+        if (!M.zq.empty())
+            mr_zreduce_kernel<double><<<(int)M.zq.size(), 4 * kRR, 0, s>>>(M.d_blk1.p, M.d_zq.p,
                                                                                    M.p1rows, zpart, Z, gate);
     } else if (M.bf) {
         float *Z32 = (float *)Z;
-        // every item fits 4 waves (kP1RowsBf rows)
-        mr_bf16_kernel<4, false, kBfStages, kBfBufs><<<(int)M.it1.size(), 256, 0, s>>>(nullptr, M.d_it1.p, nullptr, nullptr,
-                                                                            P.buf.p, X32, nullptr, zpart, P.d_zrow.p,
-                                                                            Z32, M.p1rows, gate);
-        if (!M.blk1.empty())
-            mr_zreduce_kernel<float><<<(int)M.blk1.size() * M.p1rows, kRR, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(),
+        // every item fits 4 waves (kP1RowsBf rows); 2 or 3 register stages (autotuned)
+        if (M.p1ns == 3)
+            mr_bf16_kernel<4, false, 3, kBfBufs><<<(int)M.it1.size(), 256, 0, s>>>(
+                nullptr, M.d_it1.p, nullptr, nullptr, P.buf.p, X32, nullptr, zpart, P.d_zrow.p, Z32, M.p1rows, gate);
+        else
+            mr_bf16_kernel<4, false, kBfStages, kBfBufs><<<(int)M.it1.size(), 256, 0, s>>>(
+                nullptr, M.d_it1.p, nullptr, nullptr, P.buf.p, X32, nullptr, zpart, P.d_zrow.p, Z32, M.p1rows, gate);
+        if (!M.zq.empty())
+            mr_zreduce_kernel<float><<<(int)M.zq.size(), 4 * kRR, 0, s>>>(M.d_blk1.p, M.d_zq.p,
                                                                                   M.p1rows, zpart, Z32, gate);
     } else {
         float *Z32 = (float *)Z;
         mr_lrvt32_kernel<<<(int)M.it1.size(), 2 * kP1Rows32, 0, s>>>(M.d_it1.p, P.buf.p, X32, zpart, P.d_zrow.p, Z32,
                                                                      gate);
-        if (!M.blk1.empty())
-            mr_zreduce_kernel<float><<<(int)M.blk1.size() * M.p1rows, kRR, 0, s>>>(M.d_blk1.p, (int)M.blk1.size(),
+        if (!M.zq.empty())
+            mr_zreduce_kernel<float><<<(int)M.zq.size(), 4 * kRR, 0, s>>>(M.d_blk1.p, M.d_zq.p,
                                                                                   M.p1rows, zpart, Z32, gate);
     }
 }
@@ -1281,7 +1295,10 @@ static void phase2(const hbi_hmat *h, const Plan &P, const MrPlan &M, const doub
         const int n = (int)M.it2b.size();
         if (n == 0) return;
         const size_t cmb = (size_t)M.ncol2max * sizeof(int);
-        if (h->tr == 64)
+        if (h->tr == 64 && M.p2ns == 3)
+            mr_bf16_kernel<4, true, 3, kBfBufs><<<n, 256, cmb, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr,
+                                                                   X32, Z32, ypart, nullptr, nullptr, 0, gate);
+        else if (h->tr == 64)
             mr_bf16_kernel<4, true, kBfStages, kBfBufs><<<n, 256, cmb, s>>>(M.d_it2b.p, nullptr, M.d_cmap.p, M.a16.p, nullptr, X32,
                                                                  Z32, ypart, nullptr, nullptr, 0, gate);
         else
@@ -1379,13 +1396,18 @@ static void mr_tune(hbi_hmat *h, int prec, MrPlan &M)
         HBI_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
         return ms;
     };
-    int b1 = 512, b2 = 256;
+    int b1 = 512, b2 = 256, bn = kBfStages;
     float best = 1e30f;
-    for (int c : c2s) {
-        mr_items2(h, P, M, c);
-        const float ms = time_it(2);
-        if (ms < best) { best = ms; b2 = c; }
+    for (int ns : {kBfStages, 3}) {
+        if (ns == 3 && !(M.bf && h->tr == 64)) break;
+        M.p2ns = ns;
+        for (int c : c2s) {
+            mr_items2(h, P, M, c);
+            const float ms = time_it(2);
+            if (ms < best) { best = ms; b2 = c; bn = ns; }
+        }
     }
+    M.p2ns = bn;
     // phase 1: column chunk for each item order; the XCD placement fetches
     // about half the bytes from beyond L2 (the X chunks once per XCD) and is
     // kept unless it is more than 2% slower
@@ -1401,6 +1423,11 @@ static void mr_tune(hbi_hmat *h, int prec, MrPlan &M)
     M.xcd1 = M.bf && bt[1] <= 1.02f * bt[0];
     b1 = bc[M.xcd1 ? 1 : 0];
     mr_items1(P, M, b1);
+    if (M.bf) {   // a deeper load ring for phase 1 at the chosen items
+        const float t2 = time_it(1);
+        M.p1ns = 3;
+        if (time_it(1) >= t2) M.p1ns = kBfStages;
+    }
     mr_items2(h, P, M, b2);
     mr_upload(h, P, M);
 }
@@ -1446,7 +1473,7 @@ extern "C" int hbi_hmat_matmul_plan(hbi_hmat *h, int prec, int *info)
         info[3] = (int)M.it2.size();
         info[4] = M.nzpart;
         info[5] = M.bf ? 2 : (M.f32 ? 1 : 0);
-        info[6] = M.xcd1 ? 1 : 0;
+        info[6] = (M.xcd1 ? 1 : 0) | (M.p1ns << 1) | (M.p2ns << 4);
     });
 }
 

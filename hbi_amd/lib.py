@@ -331,7 +331,8 @@ class HMatrix:
         _check(lib().hbi_hmat_matmul_plan(self.h, int(prec), info.ctypes.data_as(_ip)))
         return {"kc1": int(info[0]), "cols2": int(info[1]), "items1": int(info[2]), "items2": int(info[3]),
                 "zpart_slots": int(info[4]), "cores": ("fp64", "fp32", "bf16")[int(info[5])],
-                "xcd_phase1": bool(info[6])}
+                "xcd_phase1": bool(info[6] & 1), "stages_phase1": int((info[6] >> 1) & 7),
+                "stages_phase2": int((info[6] >> 4) & 7)}
 
 
 def mfma_f64_rate(waves_per_simd=2, iters=4000) -> float:

@@ -162,8 +162,9 @@ int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms);
    info[0] phase-1 column chunk, [1] phase-2 columns per item, [2] phase-1
    items, [3] phase-2 items (= FP64 partial row tiles written), [4] phase-1
    partial slots, [5] the matrix cores used: 0 FP64, 1 FP32, 2 BF16 (split
-   FP32 operands), [6] 1 if the phase-1 items that read the same input rows
-   are placed on one XCD (BF16 cores). */
+   FP32 operands), [6] bit 0: the phase-1 items that read the same input rows
+   are placed on one XCD, bits 1-3 and 4-6: register stages of the phase-1
+   and phase-2 load rings (BF16 cores). */
 int hbi_hmat_matmul_plan(hbi_hmat *h, int prec, int *info);
 /* Measured FP64 matrix-core rate (TFLOP/s) of v_mfma_f64_16x16x4f64, the
    instruction of both 64-column kernels: waves_per_simd (1..8) waves per SIMD
