@@ -42,7 +42,8 @@ struct FvPar {
 
 struct FvCtl {
     double h, t, hdid, errmax, vmax;
-    double tbreak;   // the call ends after the first accepted step with t > tbreak
+    double tbreak, vbreak_hi, vbreak_lo;   // the call ends after the first accepted step with t > tbreak,
+                                           // max |V| > vbreak_hi or max |V| < vbreak_lo
     long long attempts, accepted, rejected, target, hist_base, hist_cap;
     int done, status, acc, niter;
     int sw, nn;   // grid mode: the well schedule's switch flag and 1-based position (implicitsolver2)
@@ -100,10 +101,12 @@ __device__ __forceinline__ void fv_deriv(const FvPar &P, double gs, double gn, d
     o[3] = dksdt;
 }
 
-__global__ void fv_arm_kernel(FvCtl *c, long long nsteps, double tbreak)
+__global__ void fv_arm_kernel(FvCtl *c, long long nsteps, double tbreak, double vhi, double vlo)
 {
     if (c->status == 0) c->done = 0;
     c->tbreak = tbreak;
+    c->vbreak_hi = vhi;
+    c->vbreak_lo = vlo;
     c->target = c->accepted + nsteps;
     c->hist_base = c->accepted;
     c->hist_cap = nsteps;
@@ -1059,7 +1062,8 @@ __global__ void fv_monitor_kernel(FvPar P, FvCtl *c, FvArrays A, int nblk)
     if (vm > P.velmax || vm < P.velmin) c->status = 1;
     else if (c->t > P.tmax) c->status = 3;
     else if (b[4] < (double)0.2f) c->status = 4;
-    if (c->status || c->accepted >= c->target || c->t > c->tbreak) c->done = 1;
+    if (c->status || c->accepted >= c->target || c->t > c->tbreak || vm > c->vbreak_hi || vm < c->vbreak_lo)
+        c->done = 1;
 }
 
 }  // namespace hbi
@@ -1092,7 +1096,7 @@ struct hbi_fv {
     bool dirty = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0.0;
-    double tbreak = HUGE_VAL;   // hbi_fv_set_break
+    double tbreak = HUGE_VAL, vbreak_hi = HUGE_VAL, vbreak_lo = 0.0;   // hbi_fv_set_break
 };
 
 namespace hbi {
@@ -1533,7 +1537,7 @@ int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info)
         if (f->dirty) recapture = true;
         f->dirty = false;
         if (recapture) hbi::fv_capture(f);
-        hbi::fv_arm_kernel<<<1, 1, 0, f->stream>>>(f->ctl.p, nsteps, f->tbreak);
+        hbi::fv_arm_kernel<<<1, 1, 0, f->stream>>>(f->ctl.p, nsteps, f->tbreak, f->vbreak_hi, f->vbreak_lo);
         HBI_HIP(hipEventRecord(f->e0, f->stream));
         hbi::FvCtl c;
         long long launched = 0;
@@ -1568,11 +1572,13 @@ int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info)
     });
 }
 
-int hbi_fv_set_break(hbi_fv *f, double t)
+int hbi_fv_set_break(hbi_fv *f, double t, double vmax_above, double vmax_below)
 {
     return hbi::guarded([&] {
-        HBI_REQUIRE(f && !std::isnan(t), "bad arguments");
+        HBI_REQUIRE(f && !std::isnan(t) && !std::isnan(vmax_above) && !std::isnan(vmax_below), "bad arguments");
         f->tbreak = t;
+        f->vbreak_hi = vmax_above;
+        f->vbreak_lo = vmax_below;
     });
 }
 

@@ -300,7 +300,7 @@ program lhbiem_fv_gpu
     nb = min(outbatch, nstep - kdone)
     if (interval > 0) nb = min(nb, interval - mod(kdone, interval))
     ! the call also ends at the first step past tout (field output, main_fv.f90:1160-1163)
-    rc = hbi_fv_set_break(fv, tout)
+    rc = hbi_fv_set_break(fv, tout, huge(1d0), 0d0)
     rc = hbi_fv_step(fv, int(nb, c_int), hist, info)
     call system_clock(c1)
     nrow = int(info%accepted) - kdone

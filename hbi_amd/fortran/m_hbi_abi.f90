@@ -319,10 +319,10 @@ module m_hbi_abi
       integer(c_long_long), intent(out) :: nrows
       real(c_double), intent(out) :: rows(*)
     end function
-    integer(c_int) function hbi_fv_set_break(f, t) bind(C, name="hbi_fv_set_break")
+    integer(c_int) function hbi_fv_set_break(f, t, vmax_above, vmax_below) bind(C, name="hbi_fv_set_break")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: f
-      real(c_double), value :: t
+      real(c_double), value :: t, vmax_above, vmax_below
     end function
     integer(c_int) function hbi_fv_step(f, nsteps, hist, info) bind(C, name="hbi_fv_step")
       import :: c_int, c_ptr, c_double, hbi_fv_info

@@ -197,7 +197,7 @@ def lib():
         L.hbi_fv_step.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(FvInfo)]
         L.hbi_fv_last_ms.argtypes = [C.c_void_p, _dp]
         L.hbi_fv_set_probes.argtypes = [C.c_void_p, C.c_int, _ip]
-        L.hbi_fv_set_break.argtypes = [C.c_void_p, C.c_double]
+        L.hbi_fv_set_break.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.hbi_fv_get_probes.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), _dp]
         L.hbi_fv_set_wells.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp]
         L.hbi_rk_get_output.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), _dp, _dp,
@@ -763,9 +763,10 @@ class FV:
             hist = hist[:max(0, min(nsteps, done))]
         return d, hist
 
-    def set_break(self, t=np.inf):
-        """Later step calls also end after the first accepted step with time > t."""
-        _check(lib().hbi_fv_set_break(self.f, float(t)))
+    def set_break(self, t=np.inf, vmax_above=np.inf, vmax_below=0.0):
+        """Later step calls also end after the first accepted step with time > t,
+        max |V| > vmax_above or max |V| < vmax_below."""
+        _check(lib().hbi_fv_set_break(self.f, float(t), float(vmax_above), float(vmax_below)))
 
     def last_ms(self) -> float:
         ms = C.c_double(0)

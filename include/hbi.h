@@ -510,10 +510,13 @@ int hbi_fv_diffuse(hbi_fv *f, const double *pf, const double *y, const double *s
    output_monitor columns of main_fv.f90:1307-1311). */
 int hbi_fv_step(hbi_fv *f, int nsteps, double *hist, hbi_fv_info *info);
 /* The following hbi_fv_step calls also end (status 0) after the first
-   accepted step whose time exceeds t: main_fv's time-based field output
-   (x > tout, main_fv.f90:1160-1163) then falls on the last step of a call.
-   HUGE_VAL (the default) disables it. */
-int hbi_fv_set_break(hbi_fv *f, double t);
+   accepted step whose time exceeds t, or whose max |V| exceeds vmax_above or
+   falls below vmax_below: main_fv's time-based field output (x > tout,
+   main_fv.f90:1160-1163) and the event onset / end of main_fv3dp
+   (max V > 1e-2, < 1e-3, main_fv3dp.f90:871-905, whose coseismic slip needs
+   the fields of both steps) then fall on the last step of a call.  HUGE_VAL,
+   HUGE_VAL, 0 (the defaults) disable them. */
+int hbi_fv_set_break(hbi_fv *f, double t, double vmax_above, double vmax_below);
 int hbi_fv_last_ms(const hbi_fv *f, double *ms);
 /* Local output (main_fv.f90:862-868, output_local at 1312-1316): up to 16
    cluster-order elements; after every accepted step of the following step
