@@ -16,6 +16,7 @@ FSRC = os.path.join(HERE, "fortran")
 LIB = os.path.join(HERE, "libhbi.so")
 DRIVER = os.path.join(HERE, "lhbiem_gpu")
 DRIVER_FV = os.path.join(HERE, "lhbiem_fv_gpu")
+DRIVER_FV3 = os.path.join(HERE, "lhbiem_fv3dp_gpu")
 
 SOURCES = ["partition.cpp", "hmat.hip", "rk.hip", "multirhs.hip", "ens.hip", "learn.hip", "fv.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -63,9 +64,9 @@ def build_lib(jobs: int = 4) -> str:
 
 
 def build_driver() -> str:
-    """Fortran hosts (amdflang) linked against libhbi.so: lhbiem_gpu (main_LH)
-    and lhbiem_fv_gpu (main_fv)."""
-    srcs = ["m_hbi_abi.f90", "m_hacapk_gpu.f90", "lhbiem_gpu.f90", "lhbiem_fv_gpu.f90"]
+    """Fortran hosts (amdflang) linked against libhbi.so: lhbiem_gpu (main_LH),
+    lhbiem_fv_gpu (main_fv) and lhbiem_fv3dp_gpu (main_fv3dp)."""
+    srcs = ["m_hbi_abi.f90", "m_hacapk_gpu.f90", "lhbiem_gpu.f90", "lhbiem_fv_gpu.f90", "lhbiem_fv3dp_gpu.f90"]
     paths = [os.path.join(FSRC, s) for s in srcs]
     if not all(os.path.exists(p) for p in paths) or not shutil.which(FC) and not os.path.exists(FC):
         return ""
@@ -80,6 +81,7 @@ def build_driver() -> str:
     link = ["-L" + HERE, "-lhbi", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + HERE]
     _run([FC, "-o", DRIVER] + common + [objs["lhbiem_gpu.f90"]] + link)
     _run([FC, "-o", DRIVER_FV] + common + [objs["lhbiem_fv_gpu.f90"]] + link)
+    _run([FC, "-o", DRIVER_FV3] + common + [objs["lhbiem_fv3dp_gpu.f90"]] + link)
     return DRIVER
 
 

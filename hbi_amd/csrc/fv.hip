@@ -18,8 +18,8 @@
 //   monitor                 history row, stop criteria                1083-1152
 // Grid mode (main_fv3dp.f90, imax x jmax 3dp fault): the same sequence with
 // sigma held (not rebuilt from sigma_e + pf), the pressure solved on the 2-D
-// grid by one cooperative launch over all CUs (grid-wide reductions between
-// the conjugate-gradient steps), hnext <= dtmax, the minsig rule and the
+// grid by one launch of at most 40 resident workgroups (grid-wide barriers
+// and reductions between the conjugate-gradient steps), hnext <= dtmax, the minsig rule and the
 // monitor values of main_fv3dp.f90:725-815.
 // Every kernel first reads the device flag and returns when the batch target
 // is reached or the run stopped (rejected attempts skip commit .. monitor).
@@ -571,7 +571,9 @@ constexpr int kFv2Failed = 5;   // run status: a barrier of the 2-D solve timed 
 constexpr int kFv2MaxBlocks = 40;     // workgroups of the 2-D solve at most: 40 of 256 threads measured fastest on
                                       // the 200 x 200 grid (barrier cost against cells per thread)
 
-// Grid-wide barrier of a cooperative launch (every workgroup resident),
+// Grid-wide barrier of a launch whose workgroups are all resident (at most
+// kFv2MaxBlocks, or one per CU with 16 cells per thread, each workgroup on a
+// CU of its own once the stream's earlier kernels have ended),
 // without a contended counter: workgroup b publishes generation g in its own
 // 128-byte slot of W.bar, workgroup 0 waits (one thread per slot) until every
 // slot shows g and then publishes g as the release word W.bar[0]; the others
@@ -737,7 +739,7 @@ __device__ __forceinline__ double fv2_well_rate(const FvArrays &A, int w, double
 }
 
 // implicitsolver2 / implicitsolver of main_fv3dp (1314-1424) with Beuler
-// (1426-1649) on the imax x jmax grid: cooperative launch, kFv2Threads
+// (1426-1649) on the imax x jmax grid: one launch of resident workgroups, kFv2Threads
 // threads per workgroup, NPT cells per thread (cell l = g + q G kFv2Threads,
 // g the global thread index); x and r of its cells in registers, p and the
 // coefficients in global memory (neighbours).  Gated like the 1-D kernel.
@@ -1091,7 +1093,6 @@ struct hbi_fv {
     hbi::DevBuf<double> wq, wt;
     int g2_blocks = 0, g2_npt = 0;
     hipGraphExec_t graph = nullptr;
-    hipGraphExec_t graph2 = nullptr;   // grid mode: fields and monitor (after the cooperative pressure launch)
     double *graph_hist = nullptr, *graph_probe = nullptr;
     bool dirty = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1129,8 +1130,8 @@ static Fv2Work fv2_work(hbi_fv *f)
                    bar, bar ? (int *)(bar + kFv2Slot * (1 + (size_t)f->g2_blocks)) : nullptr};
 }
 
-// cells per thread and workgroups of the cooperative 2-D solve: the fewest
-// cells per thread whose grid fits on the device at once
+// cells per thread and workgroups of the 2-D solve: the fewest cells per
+// thread whose grid fits on the device at once
 static void fv2_shape(hbi_fv *f)
 {
     int dev = 0, cus = 0;
@@ -1150,7 +1151,7 @@ static void fv2_shape(hbi_fv *f)
             return;
         }
     }
-    throw Error{HBI_E_ARG, "pore-pressure grid too large for one cooperative launch"};
+    throw Error{HBI_E_ARG, "pore-pressure grid too large for one launch of resident workgroups"};
 }
 
 static void fv2_launch(hbi_fv *f, FvArrays A, FvSolveIO io)
@@ -1163,7 +1164,11 @@ static void fv2_launch(hbi_fv *f, FvArrays A, FvSolveIO io)
                    : f->g2_npt == 4 ? (const void *)fv2_pressure_kernel<4>
                    : f->g2_npt == 8 ? (const void *)fv2_pressure_kernel<8>
                                     : (const void *)fv2_pressure_kernel<16>;
-    HBI_HIP(hipLaunchCooperativeKernel(fn, dim3(f->g2_blocks), dim3(kFv2Threads), args, 0, f->stream));
+    // a plain launch: the barriers need every workgroup resident, which at
+    // most one workgroup per CU gives; a cooperative launch would also
+    // guarantee it, but takes the device's global wave-sync resource, which
+    // one process at a time can hold (a second process's launch then fails)
+    HBI_HIP(hipLaunchKernel(fn, dim3(f->g2_blocks), dim3(kFv2Threads), args, 0, f->stream));
 }
 
 static void fv_launch_pressure(hbi_fv *f, FvArrays A, const FvSolveIO &io)
@@ -1232,7 +1237,7 @@ static void fv_launch_part2(hbi_fv *f)
     fv_monitor_kernel<<<1, 64, 0, f->stream>>>(f->P, c, A, g);
 }
 
-static void fv_capture_one(hbi_fv *f, hipGraphExec_t *out, int part)
+static void fv_capture_one(hbi_fv *f, hipGraphExec_t *out)
 {
     if (*out) {
         (void)hipGraphExecDestroy(*out);
@@ -1240,26 +1245,19 @@ static void fv_capture_one(hbi_fv *f, hipGraphExec_t *out, int part)
     }
     hipGraph_t gr;
     HBI_HIP(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
-    if (part != 2) fv_launch_part1(f);
-    if (part == 0) fv_launch_gated_pressure(f);
-    if (part != 1) fv_launch_part2(f);
+    fv_launch_part1(f);
+    fv_launch_gated_pressure(f);
+    fv_launch_part2(f);
     HBI_HIP(hipStreamEndCapture(f->stream, &gr));
     HBI_HIP(hipGraphInstantiate(out, gr, nullptr, nullptr, 0));
     (void)hipGraphDestroy(gr);
 }
 
-// 1-D: the whole attempt as one graph.  Grid mode: the cooperative pressure
-// launch stays outside the graphs (launched between them on the same
-// stream, gated on the device like the graphs' kernels), so no graph node
-// needs the cooperative launch mode.
+// the whole attempt (stages, decide, commit, the gated pressure step,
+// fields, monitor) as one graph, in both modes
 static void fv_capture(hbi_fv *f)
 {
-    if (f->P.grid) {
-        fv_capture_one(f, &f->graph, 1);
-        fv_capture_one(f, &f->graph2, 2);
-    } else {
-        fv_capture_one(f, &f->graph, 0);
-    }
+    fv_capture_one(f, &f->graph);
     f->graph_hist = f->hist.p;
     f->graph_probe = f->probe.p;
 }
@@ -1267,10 +1265,6 @@ static void fv_capture(hbi_fv *f)
 static void fv_launch_graphs(hbi_fv *f)
 {
     HBI_HIP(hipGraphLaunch(f->graph, f->stream));
-    if (f->P.grid) {
-        fv_launch_gated_pressure(f);
-        HBI_HIP(hipGraphLaunch(f->graph2, f->stream));
-    }
 }
 
 }  // namespace hbi
@@ -1388,7 +1382,6 @@ int hbi_fv_destroy(hbi_fv *f)
     if (!f) return HBI_OK;
     if (f->stream) (void)hipStreamSynchronize(f->stream);
     if (f->graph) (void)hipGraphExecDestroy(f->graph);
-    if (f->graph2) (void)hipGraphExecDestroy(f->graph2);
     if (f->e0) (void)hipEventDestroy(f->e0);
     if (f->e1) (void)hipEventDestroy(f->e1);
     hipStream_t s = f->stream;

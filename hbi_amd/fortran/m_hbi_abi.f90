@@ -8,6 +8,8 @@ module m_hbi_abi
     & HBI_KIND_TRI_HS = 5, HBI_KIND_2DP = 6, HBI_KIND_3DP = 7, HBI_KIND_2DPH = 8, HBI_KIND_2DN = 9, &
     & HBI_KIND_25D = 10, HBI_KIND_2DPFZ = 11, HBI_KIND_2DNV = 12, HBI_KIND_2DNA = 13
   integer(c_int), parameter :: HBI_PREC_F64 = 0, HBI_PREC_F32 = 1, HBI_PREC_MIXED = 2
+  integer(c_int), parameter :: HBI_FV_INJECTIONQ = 0, HBI_FV_INJECTIONP = 1, HBI_FV_STEADYSTATE = 2, &
+    & HBI_FV_THRUST = 3, HBI_FV_OTHER = 4
 
   type, bind(C) :: hbi_geom
     integer(c_int) :: kind = 0, n = 0, vcomp = 0, md = 0

@@ -423,7 +423,7 @@ int hbi_ens_last_ms(const hbi_ens *e, double *ms);
    main_fv3dp.f90:1426-1649 (finite-volume operators along j and i with their
    own boundaries, the penalties of the 'ss' / 'thrust' settings, the
    'injection' source at the centre or at wells on rate schedules), solved by
-   conjugate gradients on all CUs (one cooperative launch, grid-wide
+   conjugate gradients on up to 40 CUs (one launch of resident workgroups, grid-wide
    reductions) to |r|^2 < tol^2 imax jmax (tol = 1e-6, at most 1000
    iterations); implicitsolver (1314-1366) / implicitsolver2 (1368-1424) with
    dpth 0.1 on max |dpf| and the schedule's step bound; the RK step bounded

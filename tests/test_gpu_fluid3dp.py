@@ -1,5 +1,5 @@
 """GPU: main_fv3dp's fluid-driven cycle on a 3dp fault -- the pore pressure
-on the fault's imax x jmax grid, solved by the cooperative multi-workgroup
+on the fault's imax x jmax grid, solved by the multi-workgroup
 conjugate-gradient kernel -- against the oracle (oracle/fluid3dp.py):
 
 * the pressure step for each setting and boundary set, both solvers, the
@@ -134,3 +134,176 @@ def test_gpu_fv3dp_example_grid_holds_the_injected_volume():
     assert abs((w * pfg).sum() - vol) <= 40 * 2 * TOL * np.sqrt(g.n)
     assert np.unravel_index(np.argmax(pfg), pfg.shape) == (im // 2 - 1, jm // 2 - 1)
     assert hist[:, 9].max() > 0
+
+
+def _run_fv3dp_driver(tmp_path, keys, extra_files=None):
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for name, text in (extra_files or {}).items():
+        (tmp_path / name).write_text(text)
+    P.write_in_file(str(tmp_path / "run.in"), keys)
+    r = subprocess.run([os.path.join(root, "hbi_amd", "lhbiem_fv3dp_gpu"), "run.in"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=180)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    return r.stdout, tmp_path / "output"
+
+
+# a velocity-weakening 32 x 24 fault with a strong centre source: two events,
+# then sigma_e falls below 0.2 MPa near the source and the run stops
+_EV = dict(a=0.010, b=0.015, dc=1e-4, q0=3e-3, muinit=0.55, velinit=1e-12, sigmainit=30.0, kpmin=1e-14,
+           kpmax=3e-14, beta=5e-8, eta=0.89e-3, phi0=0.2, kT=1e15, kL=1e-3, s0=50.0, dtmax=1e6, tmax=5.0)
+
+
+def _ev_keys(number, **kw):
+    keys = {"problem": "3dp", "imax": 32, "jmax": 24, "ds": 0.02, "filenumber": number, "backslip": True,
+            "vpl": 0.0, "bc": "Dirichlet", "pbc": 0.0, "f0": 0.6}
+    keys.update({k: v for k, v in _EV.items()})
+    keys.update(kw)
+    return keys
+
+
+def test_gpu_fv3dp_driver_matches_the_library_run(tmp_path):
+    """lhbiem_fv3dp_gpu (the main_fv3dp host) on a 32 x 24 fault with two
+    events and a stop control: its monitor rows (output_monitor,
+    main_fv3dp.f90:951-955) are the library run's history rows; the streams
+    (mesh order) hold the initial state and every interval-th and
+    time-based output step, the fields of the library run at step 100;
+    event rows and EQslip records (871-905) follow the monitor's slip rate
+    with the coseismic slip of the local files; the stop step gets no rows."""
+    im, jm, n = 32, 24, 768
+    yr = 365 * 24 * 3600.0
+    dtout = 2e-4
+    out_stdout, out = _run_fv3dp_driver(tmp_path, _ev_keys(7, interval=100, nstep=3000, dtout=dtout))
+    assert "normal stress below 0.2MPa" in out_stdout
+    mon = np.loadtxt(out / "monitor7.dat")
+    K = int(mon[-1, 0])
+    assert mon.shape == (K + 1, 11) and np.array_equal(mon[:, 0], np.arange(K + 1))
+    assert np.allclose(mon[0, 1:9], [0, -12, 0, 0, 30, 30, 0, 0])
+    # the library run from the same initial condition
+    g0, *_ = P.fluid3dp_injection(im, jm, 0.02, **_EV)
+    Hs = H.HMatrix(g0, leaf=32, eps=1e-4)
+    g, opts, arr, y, pf = P.fluid3dp_injection(im, jm, 0.02, lod=Hs.lod, **_EV)
+    fv = H.FV(Hs, opts, arr)
+    fv.set_state(y, pf, t=0.0, dt=1.0)
+    info1, hist1 = fv.step(100, history=True)
+    s100 = fv.get_state()
+    info, hist2 = fv.step(5000, history=True)
+    hist = np.concatenate([hist1, hist2])
+    print("library", info, "driver stop step", K + 1, flush=True)
+    assert info["status"] == 4
+    k = 100
+    assert np.allclose(mon[1:k + 1, 1], hist[:k, 0], rtol=1e-9, atol=6e-5)          # f19.4
+    for c_mon, c_hist in ((2, 1), (3, 2), (4, 3), (5, 4), (6, 5), (7, 7), (8, 8)):   # e16.5
+        assert np.allclose(mon[1:k + 1, c_mon], hist[:k, c_hist], rtol=6e-5, atol=1e-300), c_mon
+    assert np.array_equal(mon[1:k + 1, 10], hist[:k, 9])
+    # the two processes may tune the product differently (another summation
+    # order): the stop step agrees to the run's own variability
+    assert abs((K + 1) - len(hist)) <= max(3, len(hist) // 50)
+    # output steps: every 100th and the first past each tout
+    outs, tout = [], dtout * yr
+    for r in mon[1:]:
+        kk, t = int(r[0]), r[1]
+        hit = kk % 100 == 0
+        if t > tout:
+            hit, tout = True, t + dtout * yr
+        if hit:
+            outs.append(kk)
+    tr = np.loadtxt(out / "time7.dat", ndmin=2)
+    assert list(tr[:, 0].astype(int)) == [0] + outs
+    assert len(outs) > K // 100
+    nrec = 1 + len(outs)
+    fld = {name: np.fromfile(out / ("%s7.dat" % name)).reshape(nrec, n)
+           for name in ("vel", "slip", "sigma", "tau", "ks", "pf", "qflow")}
+    assert np.all(fld["qflow"] == 3e-3)       # main_fv3dp keeps qflow = q0
+    cell = np.empty(n, dtype=np.int64)         # mesh index of each cluster position
+    cell[:] = Hs.lod
+    r100 = 1 + outs.index(100)
+    for name, key in (("pf", "pf"), ("slip", "disp"), ("vel", "vel")):
+        ref = np.empty(n)
+        ref[cell] = s100[key]
+        assert np.abs(fld[name][r100] - ref).max() <= 1e-9 * np.abs(ref).max(), name
+    ref = np.empty(n)
+    ref[cell] = s100["y"][2::4]
+    assert np.abs(fld["sigma"][r100] - ref).max() <= 1e-9 * np.abs(ref).max()
+    # the centre source at grid cell (im/2, jm/2) (1-based): pf peaks there in mesh order
+    assert int(np.argmax(fld["pf"][r100])) == (im // 2 - 1) * jm + jm // 2 - 1
+    # local files at find_locid's five points (mesh order)
+    locid = [(im // 2 - 1) * jm + jm // 2, (im // 4 - 1) * jm + jm // 2, (im * 3 // 4 - 1) * jm + jm // 2,
+             (im // 2 - 1) * jm + jm // 4, (im // 2 - 1) * jm + jm * 3 // 4]
+    loc = []
+    for i, lid in enumerate(locid):
+        L = np.loadtxt(out / ("local7-%d.dat" % (i + 1)))
+        assert L.shape == (K, 10) and np.array_equal(L[:, 0], np.arange(1, K + 1))
+        rr = L[99]
+        for c, ref in ((2, np.log10(fld["vel"][r100, lid - 1])), (3, fld["slip"][r100, lid - 1]),
+                       (4, fld["sigma"][r100, lid - 1]), (5, fld["tau"][r100, lid - 1]),
+                       (6, fld["pf"][r100, lid - 1]), (9, fld["ks"][r100, lid - 1])):
+            assert abs(rr[c] - ref) <= 6e-6 * abs(ref) + 1e-300, (lid, c)
+        loc.append(L)
+    # events from the monitor's slip rate (main_fv3dp.f90:871-905)
+    ev, slipping = [], False
+    for r in mon[1:]:
+        if not slipping and 10 ** r[2] > 1e-2:
+            slipping, on = True, r
+        elif slipping and 10 ** r[2] < 1e-3:
+            slipping = False
+            ev.append((int(on[0]), int(r[0]), on[1], r[3] - on[3]))
+    assert len(ev) >= 2
+    rows = np.loadtxt(out / "event7.dat", ndmin=2)
+    eq = np.fromfile(out / "EQslip7.dat").reshape(-1, n)
+    assert rows.shape == (len(ev), 6) and eq.shape == (len(ev), n)
+    sg = 0.02 * 0.02 * n
+    for q, (kon, kend, ton, mom) in enumerate(ev):
+        assert int(rows[q, 0]) == q + 1 and int(rows[q, 1]) == kend
+        assert abs(rows[q, 2] - ton) <= 0.01
+        # the monitor's mean slip is e16.5
+        assert abs(rows[q, 4] - mom) <= 1e-5 * (abs(mom) + abs(mon[kend, 3]) + abs(mon[kon, 3]))
+        assert abs(rows[q, 5] - (np.log10(rows[q, 4] * 32.04 * sg) + 5.9) / 1.5) <= 1e-4
+        assert abs(eq[q].mean() - rows[q, 4]) <= 1e-5 * abs(rows[q, 4])
+        assert 1 <= int(rows[q, 3]) <= n
+        for i, lid in enumerate(locid):
+            d = loc[i][kend - 1, 3] - loc[i][kon - 1, 3]
+            assert abs(eq[q, lid - 1] - d) <= 1e-5 * max(abs(loc[i][kend - 1, 3]), 1e-300), (q, lid)
+    # the first event's hypocentre: the library's first max of V at the onset step, in mesh order
+    kon = ev[0][0]
+    assert int(rows[0, 3]) == int(cell[int(hist[kon - 1, 6]) - 1]) + 1
+
+
+def test_gpu_fv3dp_driver_wells_from_file(tmp_path):
+    """lhbiem_fv3dp_gpu with injectionfromfile (input_well,
+    main_fv3dp.f90:1101-1118): the wells file of the keys drives the run as
+    hbi_fv_set_wells drives the library's; the first 40 monitor rows agree,
+    and pf peaks at the strongest well."""
+    im, jm = 40, 30
+    wells = [(10, 15, [3e-4, 3e-4, 9e-4, 9e-4, 0.0, 0.0], [0.0, 2e4, 2e4, 6e4, 6e4, 99e7]),
+             (30, 12, [6e-4, 6e-4, 0.0, 0.0, 3e-4, 3e-4], [0.0, 2e4, 2e4, 6e4, 6e4, 99e7])]
+    text = "%d\n" % len(wells) + "".join(
+        "%d %d %d\n%s\n%s\n" % (i, j, len(q), " ".join("%.6e" % v for v in q), " ".join("%.6e" % v for v in t))
+        for i, j, q, t in wells)
+    kw = dict(a=0.013, b=0.0, dc=1e-4, muinit=0.55, velinit=1e-12, sigmainit=30.0, kpmin=1e-14, kpmax=3e-14,
+              beta=5e-8, eta=0.89e-3, phi0=0.2, kT=1e15, kL=1e-3, s0=50.0, dtmax=1e6, tmax=5.0)
+    keys = {"problem": "3dp", "imax": im, "jmax": jm, "ds": 0.02, "filenumber": 8, "backslip": True, "vpl": 0.0,
+            "bc": "Dirichlet", "pbc": 0.0, "f0": 0.6, "injectionfromfile": True, "injection_file": "wells.txt",
+            "interval": 40, "nstep": 40, "dtout": 1e3}
+    keys.update(kw)
+    _, out = _run_fv3dp_driver(tmp_path, keys, {"wells.txt": text})
+    mon = np.loadtxt(out / "monitor8.dat")
+    assert mon.shape == (41, 11)
+    g0, *_ = P.fluid3dp_injection(im, jm, 0.02, **kw)
+    Hs = H.HMatrix(g0, leaf=32, eps=1e-4)
+    g, opts, arr, y, pf = P.fluid3dp_injection(im, jm, 0.02, lod=Hs.lod, **kw)
+    fv = H.FV(Hs, opts, arr)
+    fv.set_wells(wells)
+    fv.set_state(y, pf, t=0.0, dt=1.0)
+    info, hist = fv.step(40, history=True)
+    assert np.allclose(mon[1:, 1], hist[:, 0], rtol=1e-9, atol=6e-5)
+    for c_mon, c_hist in ((2, 1), (4, 3), (6, 5), (8, 8)):
+        assert np.allclose(mon[1:, c_mon], hist[:, c_hist], rtol=6e-5, atol=1e-300), c_mon
+    assert np.array_equal(mon[1:, 10], hist[:, 9])
+    pfs = np.fromfile(out / "pf8.dat").reshape(2, g.n)
+    assert int(np.argmax(pfs[1])) == (30 - 1) * jm + 12 - 1 or int(np.argmax(pfs[1])) == (10 - 1) * jm + 15 - 1
+    ref = np.empty(g.n)
+    ref[Hs.lod] = fv.get_state()["pf"]
+    assert np.abs(pfs[1] - ref).max() <= 1e-9 * np.abs(ref).max()
