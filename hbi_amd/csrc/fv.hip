@@ -572,8 +572,8 @@ constexpr int kFv2MaxBlocks = 40;     // workgroups of the 2-D solve at most: 40
                                       // the 200 x 200 grid (barrier cost against cells per thread)
 
 // Grid-wide barrier of a launch whose workgroups are all resident (at most
-// kFv2MaxBlocks, or one per CU with 16 cells per thread, each workgroup on a
-// CU of its own once the stream's earlier kernels have ended),
+// kFv2MaxBlocks, or with 16 cells per thread at most one per CU: a fraction of
+// what the CUs hold at once, resident once the stream's earlier kernels end),
 // without a contended counter: workgroup b publishes generation g in its own
 // 128-byte slot of W.bar, workgroup 0 waits (one thread per slot) until every
 // slot shows g and then publishes g as the release word W.bar[0]; the others
@@ -1145,7 +1145,7 @@ static void fv2_shape(hbi_fv *f)
         int per = 0;
         HBI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn[k], kFv2Threads, 0));
         const long long need = ((long long)f->n + (long long)kFv2Threads * npt - 1) / ((long long)kFv2Threads * npt);
-        if (need <= (long long)per * cus && (need <= kFv2MaxBlocks || npt == 16)) {
+        if (need <= (long long)std::min(per, 1) * cus && (need <= kFv2MaxBlocks || npt == 16)) {
             f->g2_npt = npt;
             f->g2_blocks = (int)need;
             return;

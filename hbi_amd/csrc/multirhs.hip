@@ -690,7 +690,7 @@ __device__ __forceinline__ void bf_split3(float a, float b, uint32_t &h, uint32_
 }
 __device__ __forceinline__ bf16x8 as_bf8(uvec4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-template <int NW, bool P2, int NS, int NB>
+template <int NW, bool P2, int NS, int NB, bool A16 = false>
 __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restrict__ items2,
                                                           const MItem1 *__restrict__ items1,
                                                           const int *__restrict__ cmap,
@@ -761,13 +761,12 @@ __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restr
             S.a0 = *(const fvec4 *)ap;
             S.a1 = *(const fvec4 *)(ap + 4);
         } else if (live) {
-            // FP32 rows: 8 values in two 16-byte loads; BF16 rows: one 16-byte load (read twice)
+            // FP32 rows: 8 values in two 16-byte loads; BF16 rows (A16): one 16-byte load (read twice)
             const int row = 16 * w + c, col = it1.t0 + 32 * s + 8 * q;
             const bool ok = row < it1.kt && col < it1.t0 + it1.tn;
-            const bool f32 = it1.dtype == kF32;
-            const uint8_t *ap = ok ? buf + it1.off + ((size_t)row * it1.ldt + col) * (f32 ? 4 : 2)
+            const uint8_t *ap = ok ? buf + it1.off + ((size_t)row * it1.ldt + col) * (A16 ? 2 : 4)
                                    : (const uint8_t *)zf;
-            const fvec4 v0 = *(const fvec4 *)ap, v1 = *(const fvec4 *)(ap + (f32 ? 16 : 0));
+            const fvec4 v0 = *(const fvec4 *)ap, v1 = *(const fvec4 *)(ap + (A16 ? 0 : 16));
             S.a0 = v0;
             S.a1 = v1;
         }
@@ -775,14 +774,12 @@ __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restr
     auto widen_a = [&](const Stage &S, fvec4 &a0, fvec4 &a1) {
         a0 = S.a0;
         a1 = S.a1;
-        if constexpr (!P2) {
-            if (it1.dtype != kF32) {   // BF16 factors: the 16 bytes hold 8 values
-                const uvec4 v = __builtin_bit_cast(uvec4, S.a0);
-                a0 = fvec4{__uint_as_float(v[0] << 16), __uint_as_float(v[0] & 0xffff0000u),
-                           __uint_as_float(v[1] << 16), __uint_as_float(v[1] & 0xffff0000u)};
-                a1 = fvec4{__uint_as_float(v[2] << 16), __uint_as_float(v[2] & 0xffff0000u),
-                           __uint_as_float(v[3] << 16), __uint_as_float(v[3] & 0xffff0000u)};
-            }
+        if constexpr (!P2 && A16) {   // BF16 factors: the 16 bytes hold 8 values
+            const uvec4 v = __builtin_bit_cast(uvec4, S.a0);
+            a0 = fvec4{__uint_as_float(v[0] << 16), __uint_as_float(v[0] & 0xffff0000u),
+                       __uint_as_float(v[1] << 16), __uint_as_float(v[1] & 0xffff0000u)};
+            a1 = fvec4{__uint_as_float(v[2] << 16), __uint_as_float(v[2] & 0xffff0000u),
+                       __uint_as_float(v[3] << 16), __uint_as_float(v[3] & 0xffff0000u)};
         }
     };
     auto bstore = [&](const Stage &S, int b) {
@@ -821,26 +818,32 @@ __global__ __launch_bounds__(64 * NW) void mr_bf16_kernel(const MItem2b *__restr
         bf_split3(a1[2], a1[3], ah[3], am[3], al[3]);
         const bf16x8 Ah = as_bf8(uvec4{ah[0], ah[1], ah[2], ah[3]}), Am = as_bf8(uvec4{am[0], am[1], am[2], am[3]}),
                      Al = as_bf8(uvec4{al[0], al[1], al[2], al[3]});
-        bf16x8 Bh[4], Bm[4], Bl[4];
+        // two 16-column blocks of right-hand sides at a time: their B parts
+        // (24 registers) are live, not all four blocks' (48)
 #pragma unroll
-        for (int jb = 0; jb < 4; jb++) {
-            const int o = (16 * jb + c) * kBfRW + 4 * (q ^ (jb & 3));
-            Bh[jb] = as_bf8(*(const uvec4 *)&bs[b][0][o]);
-            Bm[jb] = as_bf8(*(const uvec4 *)&bs[b][1][o]);
-            Bl[jb] = as_bf8(*(const uvec4 *)&bs[b][2][o]);
+        for (int jp = 0; jp < 4; jp += 2) {
+            bf16x8 Bh[2], Bm[2], Bl[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int jb = jp + u;
+                const int o = (16 * jb + c) * kBfRW + 4 * (q ^ (jb & 3));
+                Bh[u] = as_bf8(*(const uvec4 *)&bs[b][0][o]);
+                Bm[u] = as_bf8(*(const uvec4 *)&bs[b][1][o]);
+                Bl[u] = as_bf8(*(const uvec4 *)&bs[b][2][o]);
+            }
+#pragma unroll
+            for (int u = 0; u < 2; u++) acc[jp + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh[u], acc[jp + u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < 2; u++) acc[jp + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bl[u], acc[jp + u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < 2; u++) acc[jp + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am, Bm[u], acc[jp + u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < 2; u++) acc[jp + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am, Bh[u], acc[jp + u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < 2; u++) acc[jp + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bm[u], acc[jp + u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < 2; u++) acc[jp + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bh[u], acc[jp + u], 0, 0, 0);
         }
-#pragma unroll
-        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh[jb], acc[jb], 0, 0, 0);
-#pragma unroll
-        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bl[jb], acc[jb], 0, 0, 0);
-#pragma unroll
-        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am, Bm[jb], acc[jb], 0, 0, 0);
-#pragma unroll
-        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am, Bh[jb], acc[jb], 0, 0, 0);
-#pragma unroll
-        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bm[jb], acc[jb], 0, 0, 0);
-#pragma unroll
-        for (int jb = 0; jb < 4; jb++) acc[jb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bh[jb], acc[jb], 0, 0, 0);
     };
     // FP32 sums of at most two slices (64 columns) are added into the FP64 sums
     auto flush = [&] {
@@ -942,7 +945,7 @@ struct MrPlan {
     int nzpart = 0, nparts = 0;
     // BF16 matrix cores (FP32 / BF16 factors, split operands)
     bool bf = false;
-    int r1b[2] = {0, 0};           // phase-1 items [0, r1b[0]) fit 4 waves (64 rows), [r1b[0], r1b[1]) 8 waves
+    int n1f = 0;                   // BF16 cores: phase-1 items [0, n1f) have FP32 rows, the rest BF16 rows
     std::vector<MItem2b> it2b;     // phase-2 items with their lane-order factor copies
     int ncol2max = 0;              // columns of the widest phase-2 item (its column map is staged in LDS)
     std::vector<MPack> pack;       // (item, slice) blocks of the copy
@@ -973,14 +976,15 @@ namespace hbi {
 // that L2 once; the chunks are dealt largest first to the XCD with the least
 // MFMA work so far.  Positions left at the end of the shorter XCD lists hold
 // empty items (no rows, no columns).
-static void xcd_order_items1(MrPlan &M)
+static void xcd_order_items1(std::vector<MItem1> &items)
 {
+    if (items.empty()) return;
     constexpr int kX = 8;
     auto work = [](const MItem1 &x) { return (long long)((x.kt + 15) / 16) * ((x.tn + 31) / 32); };
     std::map<std::pair<int, int>, int> gid;
     std::vector<std::vector<MItem1>> grp;
     std::vector<long long> gw;
-    for (const MItem1 &it : M.it1) {
+    for (const MItem1 &it : items) {
         const auto key = std::make_pair(it.xoff + it.t0, it.tn);
         auto f = gid.find(key);
         int g;
@@ -1011,9 +1015,9 @@ static void xcd_order_items1(MrPlan &M)
     for (int x = 0; x < kX; x++) len = std::max(len, lane[x].size());
     MItem1 empty{};
     empty.part = -1;
-    M.it1.assign(len * kX, empty);
+    items.assign(len * kX, empty);
     for (int x = 0; x < kX; x++)
-        for (size_t k = 0; k < lane[x].size(); k++) M.it1[k * kX + x] = lane[x][k];
+        for (size_t k = 0; k < lane[x].size(); k++) items[k * kX + x] = lane[x][k];
 }
 
 // Phase-1 items.  Runs of blocks with the same column cluster and dtype are
@@ -1081,9 +1085,19 @@ static void mr_items1(const Plan &P, MrPlan &M, int kc1)
     std::stable_sort(M.it1.begin(), M.it1.end(), [](const MItem1 &x, const MItem1 &y) {
         return (x.kt + 15) / 16 * ((x.tn + 31) / 32) > (y.kt + 15) / 16 * ((y.tn + 31) / 32);
     });
-    if (M.bf) {   // BF16 cores: every item fits 4 waves (kP1RowsBf rows)
-        if (M.xcd1) xcd_order_items1(M);
-        M.r1b[0] = M.r1b[1] = (int)M.it1.size();
+    if (M.bf) {
+        // BF16 cores: every item fits 4 waves (kP1RowsBf rows); the items of
+        // FP32 rows and those of BF16 rows are two launches (a kernel for
+        // each row type uses fewer registers), each in its own XCD order
+        std::vector<MItem1> f32, b16;
+        for (const MItem1 &it : M.it1) (it.dtype == kF32 ? f32 : b16).push_back(it);
+        if (M.xcd1) {
+            xcd_order_items1(f32);
+            xcd_order_items1(b16);
+        }
+        M.n1f = (int)f32.size();
+        M.it1 = f32;
+        M.it1.insert(M.it1.end(), b16.begin(), b16.end());
     }
     M.kc1 = kc1;
 }
@@ -1266,13 +1280,25 @@ static void phase1(hbi_hmat *h, const Plan &P, const MrPlan &M, const double *X,
                                                                                    M.p1rows, zpart, Z, gate);
     } else if (M.bf) {
         float *Z32 = (float *)Z;
-        // every item fits 4 waves (kP1RowsBf rows); 2 or 3 register stages (autotuned)
-        if (M.p1ns == 3)
-            mr_bf16_kernel<4, false, 3, kBfBufs><<<(int)M.it1.size(), 256, 0, s>>>(
-                nullptr, M.d_it1.p, nullptr, nullptr, P.buf.p, X32, nullptr, zpart, P.d_zrow.p, Z32, M.p1rows, gate);
-        else
-            mr_bf16_kernel<4, false, kBfStages, kBfBufs><<<(int)M.it1.size(), 256, 0, s>>>(
-                nullptr, M.d_it1.p, nullptr, nullptr, P.buf.p, X32, nullptr, zpart, P.d_zrow.p, Z32, M.p1rows, gate);
+        // every item fits 4 waves (kP1RowsBf rows); 2 or 3 register stages (autotuned); items of FP32
+        // rows, then those of BF16 rows
+        const int nf = M.n1f, nb = (int)M.it1.size() - M.n1f;
+        const MItem1 *i16 = M.d_it1.p + nf;
+        if (M.p1ns == 3) {
+            if (nf)
+                mr_bf16_kernel<4, false, 3, kBfBufs><<<nf, 256, 0, s>>>(
+                    nullptr, M.d_it1.p, nullptr, nullptr, P.buf.p, X32, nullptr, zpart, P.d_zrow.p, Z32, M.p1rows, gate);
+            if (nb)
+                mr_bf16_kernel<4, false, 3, kBfBufs, true><<<nb, 256, 0, s>>>(
+                    nullptr, i16, nullptr, nullptr, P.buf.p, X32, nullptr, zpart, P.d_zrow.p, Z32, M.p1rows, gate);
+        } else {
+            if (nf)
+                mr_bf16_kernel<4, false, kBfStages, kBfBufs><<<nf, 256, 0, s>>>(
+                    nullptr, M.d_it1.p, nullptr, nullptr, P.buf.p, X32, nullptr, zpart, P.d_zrow.p, Z32, M.p1rows, gate);
+            if (nb)
+                mr_bf16_kernel<4, false, kBfStages, kBfBufs, true><<<nb, 256, 0, s>>>(
+                    nullptr, i16, nullptr, nullptr, P.buf.p, X32, nullptr, zpart, P.d_zrow.p, Z32, M.p1rows, gate);
+        }
         if (!M.zq.empty())
             mr_zreduce_kernel<float><<<(int)M.zq.size(), 4 * kRR, 0, s>>>(M.d_blk1.p, M.d_zq.p,
                                                                                   M.p1rows, zpart, Z32, gate);

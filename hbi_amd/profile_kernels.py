@@ -63,6 +63,7 @@ def main():
         out["matvec"][name] = Hm.time_matvec(prec, a.iters)
     for prec, name in ((H.PREC_F64, "f64"), (H.PREC_F32, "f32"), (H.PREC_MIXED, "mixed")):
         out["matmul64"][name] = Hm.time_matmul(prec, a.iters)
+        out["matmul64"][name]["plan"] = Hm.matmul_plan(prec)
     if a.copy_mb > 0:
         out["copy"] = device_copy(a.copy_mb * (1 << 20), a.iters)
     print(json.dumps(out), flush=True)
@@ -84,7 +85,7 @@ def ensemble(g, Hm, steps: int) -> dict:
     E.set_state(np.array(ys), 0.0, 1.0)
     E.step(20)
     att = E.step(steps)
-    out = {"steps": steps, "attempts": int(att), "device_ms": E.last_ms()}
+    out = {"steps": steps, "attempts": int(att), "device_ms": E.last_ms(), "plan": Hm.matmul_plan(H.PREC_MIXED)}
     E.close()
     return out
 

@@ -19,16 +19,21 @@ from hbi_amd import problems as P
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("label,make,leaf,eps", [
-    ("2dvs", lambda: P.mesh_2dvs(2048, 40 / 2048), 32, 1e-6),
-    ("3dph", lambda: P.mesh_3dph(48, 24, 1.0), 32, 1e-5),
-    ("3dph-highrank", lambda: P.mesh_3dph(64, 32, 1.0), 64, 1e-10),
+@pytest.mark.parametrize("label,make,leaf,eps,margin", [
+    ("2dvs", lambda: P.mesh_2dvs(2048, 40 / 2048), 32, 1e-6, 1.0),
+    ("3dph", lambda: P.mesh_3dph(48, 24, 1.0), 32, 1e-5, 1.0),
+    ("3dph-highrank", lambda: P.mesh_3dph(64, 32, 1.0), 64, 1e-10, 1.0),
+    # a loose BF16 bound: blocks of the mixed plan hold BF16 factors (the
+    # BF16-row kernel of phase 1 and BF16 segments of phase 2)
+    ("3dph-bf16", lambda: P.mesh_3dph(48, 24, 1.0), 32, 1e-5, 1e6),
 ])
-def test_gpu_matmul_matches_matvec_columns(label, make, leaf, eps):
+def test_gpu_matmul_matches_matvec_columns(label, make, leaf, eps, margin):
     g = make()
-    Hg = H.HMatrix(g, leaf=leaf, eps=eps, bf16_margin=1.0)
+    Hg = H.HMatrix(g, leaf=leaf, eps=eps, bf16_margin=margin)
     st = Hg.stats
-    print(label, g.n, "kt_max", st["kt_max"], flush=True)
+    print(label, g.n, "kt_max", st["kt_max"], "BF16 blocks", st["nbf16_blocks"], "of", st["nlr"], flush=True)
+    if label == "3dph-bf16":
+        assert st["nbf16_blocks"] >= 16
     rng = np.random.default_rng(3)
     X = rng.standard_normal((g.n, 70))      # one full 64-column pass and a partial one
     lod = Hg.lod
