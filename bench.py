@@ -44,18 +44,18 @@ BF16_MFMA_PEAK_TFS = 2500.0
 # HBM bytes per launch of the roofline kernels (mixed precision), from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE passes over
 # hbi_amd/profile_kernels.py, scaled by the counters of a 256 MiB device copy
-# (fetch x 2.00, write x 1.00): mr_bf16_kernel<4,true,2,2> (140776.1 KiB
+# (fetch x 2.00, write x 1.00): mr_bf16_kernel<4,true,2,2> (141292.7 KiB
 # fetched, 1016 x 32 KiB partial tiles written at the tuner's usual 1016 items
 # of up to 1024 columns); the single-vector product's second launch with the
 # dense blocks in it (lowrank_kernel, --dense-launch 2: 110648.8 KiB fetched,
 # 256 KiB written) and its first launch when it holds them (stream_kernel,
 # --dense-launch 1: 109129.8 KiB fetched, 2887.5 KiB written)
-TRAFFIC_MR_BF16_MIXED = 321.6e6
+TRAFFIC_MR_BF16_MIXED = 322.6e6
 TRAFFIC_LOWRANK_DENSE_MIXED = 226.9e6
 TRAFFIC_STREAM_DENSE_MIXED = 226.5e6
 # rocprofv3 --pmc MfmaUtil over the same program: fraction of cycles the
 # matrix cores of mr_bf16_kernel<4,true,2,2> are busy
-MFMA_UTIL_MR_BF16 = 0.231
+MFMA_UTIL_MR_BF16 = 0.235
 IMAX, JMAX, DS0, LEAF, EPS_H, EPS_R = 256, 128, 0.390625, 64, 1e-5, 1e-6
 NREAL = 64
 FLUID_STEPS = 500      # timed accepted steps of the bp6a fluid-driven cycle

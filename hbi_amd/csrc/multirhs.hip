@@ -1515,26 +1515,50 @@ extern "C" int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms
         HBI_HIP(hipMemsetAsync(dX.p, 0, dX.n * 8, h->stream));
         HBI_HIP(hipMemsetAsync(dZ.p, 0, dZ.n * 8, h->stream));
         for (int w = 0; w < 2; w++) hbi::mr_apply_parts(h, prec, dX.p, x32.p, zp.p, dZ.p, yp.p, h->stream, nullptr);
-        auto timed = [&](int which) {
-            HBI_HIP(hipEventRecord(h->ev[0], h->stream));
-            for (int it = 0; it < iters; it++) {
-                if (which == 0) hbi::mr_phase1(h, prec, dX.p, x32.p, zp.p, dZ.p, h->stream, nullptr);
-                else if (which == 1) hbi::mr_phase2(h, prec, dX.p, x32.p, dZ.p, yp.p, h->stream, nullptr);
-                else {
-                    hbi::mr_apply_parts(h, prec, dX.p, x32.p, zp.p, dZ.p, yp.p, h->stream, nullptr);
-                    hbi::mr_reduce(h, prec, yp.p, dY.p, h->stream);
-                }
+        // the phases as they run inside whole products (phase 2 after phase 1,
+        // so neither finds its own data of the previous call in the caches):
+        // events between them, summed over the calls
+        std::vector<hipEvent_t> ev(3 * (size_t)iters, nullptr);
+        struct Events {
+            std::vector<hipEvent_t> &v;
+            ~Events()
+            {
+                for (hipEvent_t e : v)
+                    if (e) (void)hipEventDestroy(e);
             }
-            HBI_HIP(hipEventRecord(h->ev[1], h->stream));
-            HBI_HIP(hipEventSynchronize(h->ev[1]));
-            float t = 0;
-            HBI_HIP(hipEventElapsedTime(&t, h->ev[0], h->ev[1]));
-            return (double)t / iters;
-        };
+        } guard{ev};
+        for (auto &e : ev) HBI_HIP(hipEventCreate(&e));
+        for (int it = 0; it < iters; it++) {
+            HBI_HIP(hipEventRecord(ev[3 * it], h->stream));
+            hbi::mr_phase1(h, prec, dX.p, x32.p, zp.p, dZ.p, h->stream, nullptr);
+            HBI_HIP(hipEventRecord(ev[3 * it + 1], h->stream));
+            hbi::mr_phase2(h, prec, dX.p, x32.p, dZ.p, yp.p, h->stream, nullptr);
+            HBI_HIP(hipEventRecord(ev[3 * it + 2], h->stream));
+            hbi::mr_reduce(h, prec, yp.p, dY.p, h->stream);
+        }
+        HBI_HIP(hipStreamSynchronize(h->stream));
+        double t1 = 0.0, t2 = 0.0;
+        for (int it = 0; it < iters; it++) {
+            float a = 0, b = 0;
+            HBI_HIP(hipEventElapsedTime(&a, ev[3 * it], ev[3 * it + 1]));
+            HBI_HIP(hipEventElapsedTime(&b, ev[3 * it + 1], ev[3 * it + 2]));
+            t1 += a;
+            t2 += b;
+        }
+        ms[0] = t1 / iters;
+        ms[1] = t2 / iters;
+        // whole products back to back
+        HBI_HIP(hipEventRecord(h->ev[0], h->stream));
+        for (int it = 0; it < iters; it++) {
+            hbi::mr_apply_parts(h, prec, dX.p, x32.p, zp.p, dZ.p, yp.p, h->stream, nullptr);
+            hbi::mr_reduce(h, prec, yp.p, dY.p, h->stream);
+        }
+        HBI_HIP(hipEventRecord(h->ev[1], h->stream));
+        HBI_HIP(hipEventSynchronize(h->ev[1]));
+        float t = 0;
+        HBI_HIP(hipEventElapsedTime(&t, h->ev[0], h->ev[1]));
+        ms[2] = (double)t / iters;
         (void)P;
-        ms[0] = timed(0);
-        ms[1] = timed(1);
-        ms[2] = timed(2);
     });
 }
 

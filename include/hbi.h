@@ -156,7 +156,10 @@ int hbi_hmat_matvec_noalloc(hbi_hmat *h, const double *x, double *y, int prec);
 int hbi_hmat_matmul(hbi_hmat *h, const double *X, double *Y, int nrhs, int prec);
 /* Average device time per call over iters 64-column products on device-resident
    data, HIP events on the library stream: ms[0] = phase 1 (a1^T X of the
-   low-rank blocks), ms[1] = phase 2 (row tiles), ms[2] = whole product. */
+   low-rank blocks, with the reduction of its split rows), ms[1] = phase 2
+   (row tiles), both as they run inside whole products (events between the
+   phases), ms[2] = whole product (with the reduction of the row-tile
+   partials), products back to back. */
 int hbi_hmat_time_matmul(hbi_hmat *h, int prec, int iters, double *ms);
 /* The autotuned 64-column plan of a precision (built on first use):
    info[0] phase-1 column chunk, [1] phase-2 columns per item, [2] phase-1
