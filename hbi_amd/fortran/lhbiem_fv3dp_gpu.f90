@@ -29,7 +29,8 @@ program lhbiem_fv3dp_gpu
   use m_hbi_abi
   use m_HACApK_gpu
   implicit none
-  real(8), parameter :: pi = 4 * datan(1.d0), rigid = 32.04d0, yr = 365 * 24 * 3600d0
+  real(8), parameter :: pi = 4 * datan(1.d0), rigid = 32.04d0, yr = 365 * 24 * 3600d0, &
+    grav = real(9.80, 8)   ! the reference's g: a default-real literal
   character(len=256) :: input_file, param, pvalue, fname, problem, setting, bc, bcl, bcr, bct, bcb, evlaw
   character(len=256) :: precision, parameter_file, injection_file
   integer :: ios, number, imax, jmax, ncellg, nstep, interval, leaf, outbatch, i, j, k, r, nb, nrow, kdone
@@ -250,9 +251,9 @@ program lhbiem_fv3dp_gpu
     tau = sige * (f0 + (a - b) * log(vel / vref)) * 1.001
     mu = tau / sige
     fo%setting = HBI_FV_STEADYSTATE
-  case ('thrust')      ! initcond_thrust, main_fv3dp.f90:979-1018
-    sigma = 9.80d0 * 2.6d3 * (zcol + 0.2) * 1d-3
-    pfhyd = 9.80d0 * 1.0d3 * (zcol + 0.2) * 1d-3
+  case ('thrust')      ! initcond_thrust, main_fv3dp.f90:979-1018 (its g is the default-real 9.80)
+    sigma = grav * 2.6d3 * (zcol + 0.2) * 1d-3
+    pfhyd = grav * 1.0d3 * (zcol + 0.2) * 1d-3
     allocate(sigmaed(jmax))
     sigmaed(1) = sigma(1) - pfhyd(1)
     pbcl = pfhyd(1)
@@ -260,7 +261,7 @@ program lhbiem_fv3dp_gpu
     kpmaxv = kp0 * (1 + kL / kT / vpl) - kpmin * kL / kT / vpl
     ks = kp0
     do j = 2, jmax
-      sigmaed(j) = sigmaed(j - 1) + ds0 * (sin(dipangle * pi / 180d0) * (2.6d3 - 1.0d3) * 9.80d0 * 1d-3 &
+      sigmaed(j) = sigmaed(j - 1) + ds0 * (sin(dipangle * pi / 180d0) * (2.6d3 - 1.0d3) * grav * 1d-3 &
                    - q0 * eta / kp0 * 1d-3 * exp(sigmaed(j - 1) / s0))
     end do
     do l = 1, ncellg

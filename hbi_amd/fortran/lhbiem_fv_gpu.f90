@@ -2,10 +2,12 @@
 ! `lhbiem_fv_gpu <file.in>`, the counterpart of main_fv.f90.
 !
 ! Reads main_fv's `key value` input (main_fv.f90:166-316, defaults 130-162),
-! meshes a 2dp fault (coordinate2dp, 1327-1344), builds the lattice H-matrix
-! through the HACApK call sites (m_HACApK_gpu), sets the initial condition
-! of the setting (injectionq 1596-1643, injectionp 1548-1594, steadystate
-! 1452-1498, swarm 1645-1675), advances with hbi_fv_step (rkqs with the 4-component derivs,
+! meshes a 2dp fault (coordinate2dp, 1327-1344) or a 2dph fault dipping at
+! dipangle (coordinate2dph, 1346-1364), builds the lattice H-matrix through
+! the HACApK call sites (m_HACApK_gpu; with the normal-stress matrix unless
+! sigmaconst, 571-589), sets the initial condition of the setting
+! (injectionq 1596-1643, injectionp 1548-1594, steadystate 1452-1498, thrust
+! 1378-1450 with its models B and D, swarm 1645-1675), advances with hbi_fv_step (rkqs with the 4-component derivs,
 ! the implicit pore-pressure step, field update and stop controls on the
 ! device) and writes the reference's files: monitor<n>.dat (output_monitor,
 ! 1307-1311), time<n>.dat, event<n>.dat (1197-1239, decided from the
@@ -21,22 +23,27 @@ program lhbiem_fv_gpu
   use m_hbi_abi
   use m_HACApK_gpu
   implicit none
-  real(8), parameter :: pi = 4 * datan(1.d0), rigid = 32.04d0, yr = 365 * 24 * 3600d0
+  real(8), parameter :: pi = 4 * datan(1.d0), rigid = 32.04d0, yr = 365 * 24 * 3600d0, &
+    grav = real(9.80, 8)   ! the reference's g: a default-real literal
   character(len=256) :: input_file, param, pvalue, fname, problem, setting, bcl, bcr, evlaw, precision
   integer :: ios, number, ncellg, nstep, interval, leaf, outbatch, i, k, r, nb, nrow, kdone, u, nout
   real(8) :: ds0, tmax, tinj, kpmax, kpmin, eta, beta, phi0, q0, a0, b0, dc0, f00, tauinit, sigmainit
   real(8) :: velinit, pfinit, muinit, pbcl, pbcr, pinj, kL, kT, s0, eps_r, eps_h, vpl, dtout, velmin, velmax
-  real(8) :: vref, Bs, dtinit, tout, meandisp0, moment, onset_time, kp0, lf
-  logical :: backslip, pressuredependent, pfconst, outpertime, slipping, outfield, stopped
+  real(8) :: vref, Bs, dtinit, tout, meandisp0, moment, onset_time, kp0, lf, dipangle, qsum
+  logical :: backslip, pressuredependent, pfconst, outpertime, slipping, outfield, stopped, sigmaconst
+  character(len=256) :: model
   integer :: eventcount, hypoloc, lrtrn
   integer(c_int) :: rc
   type(st_HACApK_calc_entry) :: st_bemv
   type(st_HACApK_lcontrol) :: st_ctl
-  type(st_HACApK_leafmtxp) :: st_leafmtxp_s
-  type(st_HACApK_LHp) :: st_LHp_s
+  type(st_HACApK_leafmtxp) :: st_leafmtxp_s, st_leafmtxp_n
+  type(st_HACApK_LHp) :: st_LHp_s, st_LHp_n
+  type(c_ptr) :: hn
+  real(8), allocatable, target :: yel(:), yer(:), ang(:)
+  real(8), allocatable :: qcum(:)
   type(st_HACApK_latticevec) :: st_sum
-  real(8), allocatable :: coord(:, :), ds(:), hist(:), y(:), pf(:), disp(:), vel(:), qflow(:)
-  real(8), allocatable, target :: xcol(:), ycol(:), zcol(:), rake(:), xel(:), xer(:)
+  real(8), allocatable :: coord(:, :), hist(:), y(:), pf(:), disp(:), vel(:), qflow(:)
+  real(8), allocatable, target :: xcol(:), ycol(:), zcol(:), rake(:), xel(:), xer(:), ds(:)
   real(8), allocatable :: a(:), b(:), dc(:), f0(:), vplv(:), kpmaxv(:), kTv(:), phi(:), pfhyd(:), qin(:), area(:)
   real(8), allocatable :: sigma(:), sige(:), tau(:), ks(:), psi(:)
   type(hbi_fv_opts) :: fo
@@ -64,6 +71,7 @@ program lhbiem_fv_gpu
   dc0 = 0d0; f00 = 0d0; tauinit = 0d0; sigmainit = 0d0; velinit = 0d0; pfinit = 0d0; muinit = 0d0
   pbcl = 0d0; pbcr = 0d0; pinj = 0d0; s0 = 0d0; vpl = 0d0; kp0 = 0d0
   vref = 1d-6; leaf = 32; precision = 'f64'; outbatch = 200
+  model = ' '; dipangle = 0d0; sigmaconst = .false.
   do
     read(33, *, iostat=ios) param, pvalue
     if (ios /= 0) exit
@@ -117,32 +125,49 @@ program lhbiem_fv_gpu
     case ('leaf'); read(pvalue, *) leaf
     case ('precision'); read(pvalue, '(a)') precision
     case ('outbatch'); read(pvalue, *) outbatch
+    case ('model'); read(pvalue, '(a)') model
+    case ('dipangle'); read(pvalue, *) dipangle
+    case ('sigmaconst'); read(pvalue, *) sigmaconst
     case default
       write(*, *) 'WARNING: ', trim(param), ' is not read by this driver'
     end select
   end do
   close(33)
-  if (trim(problem) /= '2dp') then
-    write(*, *) 'ERROR: this driver meshes 2dp faults (the problem of the fluid examples)'
+  if (trim(problem) /= '2dp' .and. trim(problem) /= '2dph') then
+    write(*, *) 'ERROR: this driver meshes 2dp and 2dph faults'
     stop 1
   end if
+  if (trim(problem) == '2dp') sigmaconst = .true.   ! main_fv.f90:573-576
   if (ncellg < 4 .or. ds0 <= 0d0) then
     write(*, *) 'ERROR: ncellg and ds must be set'
     stop 1
   end if
 
   ! ---------------------------------------------------------------- mesh --
+  ! coordinate2dp (main_fv.f90:1327-1344) or coordinate2dph (1346-1364): a
+  ! segment dipping at dipangle from the surface
   allocate(xcol(ncellg), ycol(ncellg), zcol(ncellg), rake(ncellg), xel(ncellg), xer(ncellg), ds(ncellg))
+  allocate(yel(ncellg), yer(ncellg), ang(ncellg))
   ycol = 0d0; zcol = 0d0; rake = 0d0
   do i = 1, ncellg
     ds(i) = ds0
-    xel(i) = (i - 1) * ds0
-    xer(i) = i * ds0
+    if (trim(problem) == '2dph') then
+      xel(i) = (i - 1) * ds0 * cos(dipangle * pi / 180)
+      xer(i) = i * ds0 * cos(dipangle * pi / 180)
+      yel(i) = (i - 1) * ds0 * sin(dipangle * pi / 180)
+      yer(i) = i * ds0 * sin(dipangle * pi / 180)
+      ycol(i) = 0.5d0 * (yel(i) + yer(i))
+      ang(i) = datan2(yer(i) - yel(i), xer(i) - xel(i))
+    else
+      xel(i) = (i - 1) * ds0
+      xer(i) = i * ds0
+    end if
     xcol(i) = 0.5d0 * (xel(i) + xer(i))
   end do
 
   ! ------------------------------------------------------------ H-matrix --
-  ! main_fv.f90:573-589: shear matrix only (sigma constant for 2dp)
+  ! main_fv.f90:571-589: the shear matrix, and the normal-stress one unless
+  ! sigma is constant (always for 2dp)
   lrtrn = HACApK_init(ncellg, st_ctl, st_bemv)
   st_ctl%param(21) = leaf
   select case (trim(precision))
@@ -153,6 +178,9 @@ program lhbiem_fv_gpu
   st_bemv%problem = problem
   st_bemv%xcol => xcol; st_bemv%ycol => ycol; st_bemv%zcol => zcol; st_bemv%rake => rake
   st_bemv%xel => xel; st_bemv%xer => xer
+  if (trim(problem) == '2dph') then
+    st_bemv%yel => yel; st_bemv%yer => yer; st_bemv%ang => ang; st_bemv%ds => ds
+  end if
   st_bemv%md = 's'; st_bemv%v = 's'
   allocate(coord(ncellg, 3))
   coord(:, 1) = xcol; coord(:, 2) = ycol; coord(:, 3) = zcol
@@ -160,6 +188,14 @@ program lhbiem_fv_gpu
   if (lrtrn /= 0) stop 2
   lrtrn = HACApK_construct_LH(st_LHp_s, st_leafmtxp_s, st_bemv, st_ctl, coord, eps_h)
   lrtrn = HACApK_gen_lattice_vector(st_sum, st_leafmtxp_s, st_ctl)
+  hn = c_null_ptr
+  if (.not. sigmaconst) then
+    st_bemv%v = 'n'
+    lrtrn = HACApK_generate(st_leafmtxp_n, st_bemv, st_ctl, coord, eps_h)
+    if (lrtrn /= 0) stop 2
+    lrtrn = HACApK_construct_LH(st_LHp_n, st_leafmtxp_n, st_bemv, st_ctl, coord, eps_h)
+    hn = st_LHp_n%h
+  end if
 
   ! ------------------------------------------------- initial condition --
   allocate(a(ncellg), b(ncellg), dc(ncellg), f0(ncellg), vplv(ncellg), kpmaxv(ncellg), kTv(ncellg))
@@ -201,6 +237,42 @@ program lhbiem_fv_gpu
     sige = sigma - pf
     tau = sige * (f00 + (a - b) * log(velinit / vref)) * 1.001
     fo%setting = 2
+  case ('thrust')       ! initcond_thrust, main_fv.f90:1378-1450 (its g is the default-real 9.80);
+    ! indices as there: i is the element along the fault, qin(1950), qin(1600) and, for model D,
+    ! qin(1200) are the sources; the library reads qin(2000) for the right boundary
+    if (ncellg < 2000) then
+      write(*, *) 'ERROR: setting thrust places sources at elements 1200-2000: ncellg >= 2000'
+      stop 1
+    end if
+    do i = 1, ncellg
+      a(i) = max(a0, a0 + (abs(i - 500) - 300) * 2d-5)
+      if (trim(model) == 'B') a(i) = max(a0, a0 + (abs(i - 700) - 500) * 2d-5)
+      a(i) = min(a(i), b0 + 0.002)
+    end do
+    qin(1950) = q0
+    if (trim(model) == 'D') qin(1200) = -19 * q0
+    qin(1600) = 19 * q0
+    allocate(qcum(ncellg))
+    qsum = sum(qin)
+    do i = 1, ncellg
+      qcum(i) = qsum - qin(i)
+      qsum = qcum(i)
+      kTv(i) = min(1e0 * exp(1d4 / (300d0 + 12d0 * ycol(i))), 1d12)
+      ks(i) = (kpmax * vpl / kL + kpmin / kTv(i)) / (vpl / kL + 1 / kTv(i))
+    end do
+    sigma = grav * 2.6d3 * (ycol + 0.2) * 1d-3
+    pfhyd = grav * 1.0d3 * (ycol + 0.2) * 1d-3
+    pf(1) = pfhyd(1)
+    sige(1) = sigma(1) - pf(1)
+    pbcl = pfhyd(1)
+    do i = 2, ncellg
+      sige(i) = sige(i - 1) + ds0 * (sin(dipangle * pi / 180d0) * (2.6d3 - 1.0d3) * grav * 1d-3 &
+                - qcum(i) * eta / ks(i) * 1d-3 * exp(sige(i - 1) / s0))
+      pf(i) = sigma(i) - sige(i)
+    end do
+    pbcr = pf(ncellg)
+    tau = sige * (f0 + (a - b) * log(velinit / vref)) * 1.001
+    fo%setting = 3
   case ('swarm')        ! main_fv.f90:1645-1675
     kTv = 1e12
     ks = kpmin
@@ -235,7 +307,7 @@ program lhbiem_fv_gpu
   fo%tinj_s = tinj * 365 * 24 * 3600; fo%dtout_s = dtout * 365 * 24 * 3600
   fo%velmin = velmin; fo%velmax = velmax; fo%tmax_s = tmax * 365 * 24 * 3600
   ! the state arrays go in cluster order (np = 1: lodc is the cluster permutation)
-  rc = hbi_fv_create(st_LHp_s%h, c_null_ptr, fo, perm(a), perm(b), perm(dc), perm(f0), perm(vplv), &
+  rc = hbi_fv_create(st_LHp_s%h, hn, fo, perm(a), perm(b), perm(dc), perm(f0), perm(vplv), &
                      perm(kpmaxv), perm(kTv), perm(phi), perm(pfhyd), perm(qin), perm(area), fv)
   if (rc /= 0) then
     write(*, *) 'ERROR: ', trim(hbi_error_text())
@@ -361,6 +433,7 @@ program lhbiem_fv_gpu
   end do
   rc = hbi_fv_destroy(fv)
   lrtrn = HACApK_free_leafmtxp(st_leafmtxp_s)
+  if (.not. sigmaconst) lrtrn = HACApK_free_leafmtxp(st_leafmtxp_n)
   lrtrn = HACApK_finalize(st_ctl)
 
 contains

@@ -517,6 +517,71 @@ def fluid_injection(g: Geometry, lod, setting: str = "injectionq", a=0.007, b=0.
     return opts, arrays, y, pf
 
 
+def fluid_thrust(g: Geometry, lod, dipangle: float, model: str = "", a0=0.01, b0=0.015, dc=0.02, f0=0.6,
+                 vref=1e-6, vpl=1e-9, velinit=1e-9, phi0=0.1, beta=1e-8, eta=1e-3, kpmin=1e-16, kpmax=1e-13,
+                 kL=1e-3, s0=50.0, q0=1e-12, law=0, pressure_dependent=False, eps_r=1e-4, backslip=True, prec=0,
+                 velmin=1e-16, velmax=1e7, tmax=1e12, Bs=0.0, ds0=None):
+    """Options, per-element arrays (cluster order) and the initial state of
+    main_fv's 'thrust' setting on a 2dph fault of at least 2000 elements
+    (initcond_thrust, main_fv.f90:1378-1450): a ramps from a0 away from
+    element 500 (700 for model B) and is capped at b0 + 0.002; sources q0 at
+    element 1950 and 19 q0 at 1600 (and -19 q0 at 1200 for model D), their
+    flux q(i) = the sum of the sources past element i; the healing time
+    kT = min(exp(1e4 / (300 + 12 depth)), 1e12) and the steady permeability
+    ks between kpmin and kpmax; lithostatic sigma and hydrostatic pfhyd
+    (g = the reference's default-real 9.80), sigma_e integrated down dip with
+    the flux's pressure gradient; Dirichlet pressures pbcl = pfhyd(1), pbcr =
+    pf(N).  The element index is the position along the fault (mesh order).
+    ds0: the element length of the pressure grid (default: the mesh's)."""
+    n = g.n
+    ds0 = float(g.ds0 if ds0 is None else ds0)
+    assert n >= 2000, "thrust places sources at elements 1200-2000"
+    lod = np.asarray(lod)
+    grav = float(np.float32(9.80))
+    ycol = np.asarray(g.ycol, dtype=np.float64)
+    i = np.arange(1, n + 1)
+    a = np.maximum(a0, a0 + (np.abs(i - 500) - 300) * 2e-5)
+    if model == "B":
+        a = np.maximum(a0, a0 + (np.abs(i - 700) - 500) * 2e-5)
+    a = np.minimum(a, b0 + float(np.float32(0.002)))
+    qin = np.zeros(n)
+    qin[1949] = q0
+    if model == "D":
+        qin[1199] = -19 * q0
+    qin[1599] = 19 * q0
+    q = np.empty(n)
+    qs = float(np.cumsum(qin)[-1])
+    for k in range(n):
+        q[k] = qs - qin[k]
+        qs = q[k]
+    kT = np.minimum(np.exp(1e4 / (300.0 + 12.0 * ycol)), 1e12)
+    ks = (kpmax * vpl / kL + kpmin / kT) / (vpl / kL + 1 / kT)
+    c02 = float(np.float32(0.2))
+    sigma = grav * 2.6e3 * (ycol + c02) * 1e-3
+    pfhyd = grav * 1.0e3 * (ycol + c02) * 1e-3
+    pf = np.empty(n)
+    sige = np.empty(n)
+    pf[0] = pfhyd[0]
+    sige[0] = sigma[0] - pf[0]
+    sd = math.sin(dipangle * PI / 180.0) * (2.6e3 - 1.0e3) * grav * 1e-3
+    for k in range(1, n):
+        sige[k] = sige[k - 1] + ds0 * (sd - q[k] * eta / ks[k] * 1e-3 * math.exp(sige[k - 1] / s0))
+        pf[k] = sigma[k] - sige[k]
+    tau = sige * (f0 + (a - b0) * np.log(velinit / vref)) * float(np.float32(1.001))
+    psi = a * np.log(2 * vref / velinit * np.sinh(tau / sige / a))
+    one = np.ones(n)
+    arrays = dict(a=a[lod], b=b0 * one, dc=dc * one, f0=f0 * one, vpl=vpl * one, kpmaxv=kpmax * one, kTv=kT[lod],
+                  phi=phi0 * one, pfhyd=pfhyd[lod], qin=qin[lod], area=np.asarray(g.ds, dtype=np.float64)[lod])
+    y = np.empty(4 * n)
+    y[0::4], y[1::4], y[2::4], y[3::4] = psi[lod], tau[lod], sige[lod], ks[lod]
+    opts = dict(law=law, backslip=int(backslip), prec=prec, pressure_dependent=int(pressure_dependent), pfconst=0,
+                bcl=0, bcr=0, setting=3, outpertime=0, vref=vref, eps_r=eps_r, hdiv=0.0, ds0=ds0, kL=kL,
+                kpmin=kpmin, s0=s0, eta=eta, beta=beta, Bs=Bs, pbcl=float(pfhyd[0]), pbcr=float(pf[-1]), q0=q0,
+                pinj=0.0, tinj_s=0.0, dtout_s=1e3 * 365 * 24 * 3600, velmin=velmin, velmax=velmax,
+                tmax_s=tmax * 365 * 24 * 3600)
+    return opts, arrays, y, pf[lod]
+
+
 BP6A_N, BP6A_DS0 = 4000, 0.01
 
 

@@ -386,3 +386,48 @@ def test_gpu_fluid_driver_stop_step_and_time_based_outputs(tmp_path):
     tr = np.loadtxt(out / "time5.dat", ndmin=2)
     assert list(tr[:, 0].astype(int)) == expect
     assert np.fromfile(out / "pf5.dat").size == len(expect) * n
+
+
+def test_gpu_fluid_driver_thrust_setting(tmp_path):
+    """lhbiem_fv_gpu with setting thrust on a 2dph fault dipping 30 degrees
+    (initcond_thrust, main_fv.f90:1378-1450; the normal-stress matrix as
+    sigmaconst is F): the initial streams hold problems.fluid_thrust's state
+    (sigma_e, pf, tau, ks in cluster order), and the monitor rows are those of
+    the library run from that state with both matrices."""
+    import dataclasses
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    n, nstep, dip = 2000, 100, 30.0
+    kw = dict(a0=0.01, b0=0.015, dc=0.02, f0=0.6, vpl=1e-9, velinit=1e-9, phi0=0.1, beta=1e-8, eta=1e-3,
+              kpmin=1e-16, kpmax=1e-13, kL=1e-3, s0=50.0, q0=1e-12)
+    keys = {"problem": "2dph", "ncellg": n, "ds": 0.01, "dipangle": dip, "setting": "thrust", "filenumber": 5,
+            "interval": 50, "nstep": nstep, "dtout": 1e3, "backslip": True, "a": kw["a0"], "b": kw["b0"],
+            "dc": kw["dc"], "f0": kw["f0"], "vpl": kw["vpl"], "velinit": kw["velinit"], "phi0": kw["phi0"],
+            "beta": kw["beta"], "eta": kw["eta"], "kpmin": kw["kpmin"], "kpmax": kw["kpmax"], "kL": kw["kL"],
+            "s0": kw["s0"], "q0": kw["q0"]}
+    P.write_in_file(str(tmp_path / "thrust.in"), keys)
+    r = subprocess.run([os.path.join(root, "hbi_amd", "lhbiem_fv_gpu"), "thrust.in"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    out = tmp_path / "output"
+    g = P.mesh_2dph(n, 0.01, dip)
+    Hs = H.HMatrix(g, leaf=32, eps=1e-4)
+    Hn = H.HMatrix(dataclasses.replace(g, vcomp=1), leaf=32, eps=1e-4)
+    opts, arr, y, pf = P.fluid_thrust(g, Hs.lod, dip, ds0=0.01, **kw)
+    rec = {name: np.fromfile(out / ("%s5.dat" % name)).reshape(-1, n) for name in ("sigma", "pf", "tau", "ks")}
+    for name, ref in (("sigma", y[2::4]), ("pf", pf), ("tau", y[1::4]), ("ks", y[3::4])):
+        assert np.abs(rec[name][0] - ref).max() <= 1e-12 * np.abs(ref).max(), name
+    assert rec["sigma"][0].min() > 3.0 and rec["pf"][0].max() > 90.0
+    mon = np.loadtxt(out / "monitor5.dat")
+    fv = H.FV(Hs, opts, arr, hn=Hn)
+    fv.set_state(y, pf, t=0.0, dt=1.0)
+    info, hist = fv.step(nstep, history=True)
+    print("library", info, flush=True)
+    k = min(len(hist), len(mon) - 1)
+    assert k >= 50
+    assert np.allclose(mon[1:k + 1, 1], hist[:k, 0], rtol=1e-9, atol=6e-5)
+    for c_mon, c_hist in ((2, 1), (3, 2), (4, 3), (5, 4), (6, 5), (8, 7), (9, 8)):
+        assert np.allclose(mon[1:k + 1, c_mon], hist[:k, c_hist], rtol=6e-5, atol=1e-300), c_mon
+    assert np.array_equal(mon[1:k + 1, 11], hist[:k, 9])
