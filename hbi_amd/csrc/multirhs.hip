@@ -347,7 +347,8 @@ __global__ void mr_narrow_kernel(size_t n4, const double *__restrict__ X, float 
 // Z rows of the blocks whose phase-1 rows were split into column chunks:
 // the partial slots summed in slot order.  Workgroup q takes rows
 // r0 .. r0 + 3 of block blk (zq[q] = (blk, r0), rows < kt only), one
-// right-hand side per thread.
+// right-hand side per thread; the slots are loaded eight at a time (one
+// memory latency per eight instead of one per slot), then added in order.
 template <typename OT>
 __global__ __launch_bounds__(4 * kRR) void mr_zreduce_kernel(const MBlk1 *__restrict__ blks, const int2 *__restrict__ zq,
                                                             int prow, const double *__restrict__ zpart,
@@ -358,8 +359,19 @@ __global__ __launch_bounds__(4 * kRR) void mr_zreduce_kernel(const MBlk1 *__rest
     const MBlk1 B = blks[q.x];
     const int k = q.y + (int)threadIdx.x / kRR, rr = (int)threadIdx.x % kRR;
     if (k >= B.kt) return;
+    constexpr int kLd = 8;
+    const size_t stride = (size_t)prow * kRR;
+    const double *src = zpart + ((size_t)B.p0 * prow + B.roff + k) * kRR + rr;
+    const int np = B.p1 - B.p0;
     double s = 0.0;
-    for (int p = B.p0; p < B.p1; p++) s += zpart[((size_t)p * prow + B.roff + k) * kRR + rr];
+    for (int p = 0; p < np; p += kLd) {
+        double v[kLd];
+#pragma unroll
+        for (int u = 0; u < kLd; u++) v[u] = src[(size_t)min(p + u, np - 1) * stride];
+#pragma unroll
+        for (int u = 0; u < kLd; u++)
+            if (p + u < np) s += v[u];
+    }
     Z[(size_t)(B.zoff + k) * kRR + rr] = (OT)s;
 }
 
