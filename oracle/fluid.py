@@ -29,8 +29,12 @@ driver; tests/test_fluid_oracle.py):
     the centre to pinj; the conjugate-gradient result agrees with a direct
     banded solve of the same system to the residual bound;
   * implicitsolver: the Picard result is a fixed point to the 1e-3 test.
-rkqs's controller and the post-step update are parity unpinned beyond the
-shared tableau and the device-vs-oracle tests.
+  * rkqs: the controller of main_fv.f90:2208-2248 by hand-worked values
+    (rejection to max(h/2, 0.9f h e^-0.25), a tiny errmax halving h, growth
+    min(2h, 0.9f h e^-0.2f), the outpertime cap, underflow);
+  * the post-step update on a scripted step: sigma held through the
+    pressure step, the slip's trapezoid of V before and after it, Darcy's
+    qflow of the new pf (one-sided at the ends).
 """
 from __future__ import annotations
 

@@ -36,7 +36,11 @@ pressure sum and a centre source adds exactly its volume; a uniform pressure
 is a fixed point without sources; the 'ss' setting's linear profile is
 steady; the conjugate-gradient result agrees with a direct solve of the
 assembled system to the residual bound; mirror symmetry of a centred
-source on a uniform grid; the rate schedule's interpolation by hand.
+source on a uniform grid; the rate schedule's interpolation by hand; the
+'thrust' penalties add exactly their hand-computed volumes (top rows from
+pbcl, bottom flux from q0); the minsig rule and the post-step order on a
+scripted step (sigma held, sigma_e = minsig where the RK's falls below it,
+the slip's trapezoid of V before and after the pressure step, plain means).
 """
 from __future__ import annotations
 

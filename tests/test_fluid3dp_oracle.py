@@ -22,6 +22,7 @@ import types
 import numpy as np
 import pytest
 
+from hbi_amd import problems as P
 from oracle import fluid as F
 from oracle import fluid3dp as F3
 
@@ -206,3 +207,75 @@ def test_ss_setting_keeps_the_linear_background_flow():
         x = np.linalg.solve(_dense(G, Amx, Amy), b.ravel()).reshape(im, jm)
         sat = 2 * h * q0 * 1e-9 / (beta * phi0 * DS0)
         assert np.abs(x - pf).max() <= 3e-8 * sat
+
+
+@pytest.mark.parametrize("plant", [False, True])
+def test_thrust_penalties_add_their_volumes(plant):
+    """'thrust' (main_fv3dp.f90:1531-1545) by hand: with Neumann boundaries
+    the operator keeps the half-weighted sum of pf - pfhyd, so a step adds
+    exactly the penalties' volumes: at the top, 1.0 x (j = 1) and 0.5 x
+    (j = 2) of 2 h c(i, 1) / ds0^2 (pbcl - pfhyd(i, 1)), i.e. 2 h c (pbcl -
+    pfhyd) / ds0^2 per unit weight of column i; at the bottom the flux
+    2 h q0 / (beta phi0) 1e-9 / ds0 on row jmax (1e-9 the single-precision
+    literal), i.e. h q0 / (beta phi0) 1e-9 / ds0 per unit weight.  A planted
+    mistake (the top penalty from c(i, 2)) breaks it."""
+    rng = np.random.default_rng(11)
+    q0, beta, phi0, pbcl, h = 2e-4, 5e-8, 0.2, 1.5, 2e3
+    G = F3.Grid2D(IM, JM, _opts(setting=F3.THRUST, q0=q0, beta=beta, phi0=phi0, pbcl=pbcl))
+    c = 1e-7 * (1 + rng.random((IM, JM)))
+    hyd = 0.3 * rng.random((IM, JM))
+    pf = hyd + 1 + rng.standard_normal((IM, JM))
+    cc = c.copy()
+    if plant:
+        cc[:, 0] = c[:, 1]
+    Amx, Amy, b, x0 = G.system(pf, cc, h, 0.0, hyd)
+    x = np.linalg.solve(_dense(G, Amx, Amy), b.ravel()).reshape(IM, JM)
+    w = _weights(IM, JM)
+    wi = w[:, 1]
+    top = (wi * 2 * h * c[:, 0] / DS0 ** 2 * (pbcl - hyd[:, 0])).sum()
+    bottom = wi.sum() * h * q0 / (beta * phi0) * float(np.float32(1e-9)) / DS0
+    got = (w * (x - (pf - hyd))).sum()
+    ok = abs(got - (top + bottom)) <= 1e-10 * (abs(top) + abs(bottom))
+    assert ok != plant
+
+
+def test_minsig_rule_and_post_step_order():
+    """main_fv3dp.f90:761-787 by hand on a scripted step: the pressure step
+    acts on the held sigma; where the RK's sigma_e is below minsig, sigma =
+    pf(new) + minsig (so sigma_e = minsig there), elsewhere sigma_e = sigma -
+    pf(new); the slip grows by the trapezoid of V from the RK state and V
+    from the new sigma_e; the monitor takes the plain means of N cells."""
+    im, jm = 6, 5
+    g = P.mesh_3dp(im, jm, DS0)
+    hs = types.SimpleNamespace(n=g.n)
+    _, opts, arr, y, pf = P.fluid3dp_injection(im, jm, DS0, minsig=20.0)
+    y = y.copy()
+    low = np.zeros(g.n, dtype=bool)
+    low[[3, 11, 17]] = True
+    y[2::4][low] = 15.0                      # the RK's sigma_e below minsig there
+    dpf = np.linspace(0.1, 0.9, g.n)
+
+    class Fixed(F3.Fv3Oracle):
+        def rkqs(self, yy, t, htry):
+            return yy.copy(), t + htry, htry, 2 * htry, 0.5, 0
+
+        def diffuse(self, pf_, yy, sigma, h, t, dtnxt):
+            return pf_ + dpf, 4, dtnxt
+
+    S = Fixed(hs, opts, arr, np.arange(g.n))
+    sigma = y[2::4] + pf + 5.0               # the held total normal stress
+    h = 1e3
+    st = dict(y=y.copy(), pf=pf.copy(), sigma=sigma.copy(), disp=np.zeros(g.n), t=0.0, h=h)
+    row = S.step(st)
+    p1 = pf + dpf
+    sig1 = sigma.copy()
+    sig1[low] = p1[low] + 20.0
+    sige1 = sig1 - p1
+    assert np.array_equal(st["sigma"], sig1) and np.allclose(st["y"][2::4], sige1, rtol=1e-15)
+    assert np.all(st["y"][2::4][low] == sig1[low] - p1[low])
+    psi, tau = y[0::4], y[1::4]
+    v0 = S.vel(psi, tau, y[2::4])
+    v1 = S.vel(psi, tau, sige1)
+    assert np.allclose(st["disp"], v0 * h * 0.5 + v1 * h * 0.5, rtol=1e-14)
+    assert np.isclose(row["minnorm"], sige1.min()) and np.isclose(row["meandisp"], st["disp"].mean(), rtol=1e-14)
+    assert np.isclose(row["meanmu"], (tau / sige1).mean(), rtol=1e-14)

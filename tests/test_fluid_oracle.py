@@ -183,3 +183,93 @@ def test_fluid_oracle_run_holds_the_injected_volume(hs):
     # (main_fv.f90:1908-1910): the first, short steps leave pf as it is
     assert np.all(np.diff(pk) >= 0) and pk[-1] > pk[10] > 0
     assert abs(np.dot(w, st["pf"]) - vol) <= 30 * tol * N
+
+
+class _Scripted(F.FvOracle):
+    """The oracle's rkqs driven by scripted error ratios: the k-th rkck call
+    returns yerr / yout = errs[k] * eps_r on every component."""
+
+    def __init__(self, hs, errs, **opts):
+        g, H = hs
+        o, arr, _, _ = P.fluid_injection(g, H.lod)
+        o.update(opts)
+        super().__init__(H, o, arr)
+        self.errs = list(errs)
+        self.calls = []
+
+    def rkck(self, y, h):
+        self.calls.append(h)
+        e = self.errs[len(self.calls) - 1]
+        return np.ones_like(y), np.full_like(y, e * self.o["eps_r"])
+
+
+def test_fluid_rkqs_controller_known_answers(hs):
+    """main_fv.f90:2208-2248 by hand: SAFETY = 0.9 and PGROW = -0.2 are
+    default-real literals, PSHRNK = -0.25; a rejection (errmax >= 1) shrinks
+    h to max(h / 2, 0.9 h errmax^-0.25); errmax <= 1e-15 is a rejection too
+    (h / 2), unlike main_LH; an accepted step proposes min(2 h, 0.9 h
+    errmax^-0.2), then min(., dtout) with outpertime."""
+    y0 = np.ones(4 * N)
+    # errmax 2 then 0.5: one rejection to 0.9f * 2^-0.25 = 0.75680675 h, then growth 0.9f * 0.5^-0.2f
+    S = _Scripted(hs, [2.0, 0.5])
+    _, t1, hdid, hnext, errmax, nrej = S.rkqs(y0, 0.0, 100.0)
+    assert nrej == 1 and S.calls == [100.0, hdid]
+    assert abs(hdid - 75.680675367981) <= 1e-9 and t1 == hdid
+    assert abs(hnext / hdid - 1.033828494245849) <= 1e-14 and errmax == 0.5
+    # errmax 1e-16: h / 2 (not a growth step), then 0.001: growth capped at 2 h
+    S = _Scripted(hs, [1e-16, 0.001])
+    _, _, hdid, hnext, _, nrej = S.rkqs(y0, 0.0, 64.0)
+    assert nrej == 1 and hdid == 32.0 and hnext == 64.0
+    # errmax 0.5 with outpertime: the proposal is capped at dtout
+    S = _Scripted(hs, [0.5], outpertime=1, dtout_s=10.0)
+    _, _, hdid, hnext, _, nrej = S.rkqs(y0, 0.0, 100.0)
+    assert nrej == 0 and hdid == 100.0 and hnext == 10.0
+    # a rejection from errmax 1.5: max(h / 2, 0.9f * 1.5^-0.25) = 0.81324178 h
+    S = _Scripted(hs, [1.5, 0.9])
+    _, _, hdid, _, _, nrej = S.rkqs(y0, 0.0, 1.0)
+    assert nrej == 1 and abs(hdid - 0.8132417817053098) <= 1e-15
+    # step-size underflow at large t
+    S = _Scripted(hs, [2.0] * 8)
+    with pytest.raises(FloatingPointError):
+        S.rkqs(y0, 1e12, 1e-5)
+
+
+def test_fluid_post_step_update_order(hs):
+    """main_fv.f90:1045-1082 by hand on a scripted step: the pore-pressure
+    update acts on sigma = sigma_e(RK) + pf(old) held fixed, so sigma_e(new)
+    = sigma - pf(new); the slip grows by the trapezoid of V before (RK state)
+    and after (new sigma_e) the pressure update; qflow is Darcy's flux of
+    the new pf in excess of pfhyd (one-sided at the ends, centred inside).
+    A planted order mistake (V after the update from the RK sigma_e) moves
+    the slip."""
+    g, H = hs
+    o, arr, y, pf = P.fluid_injection(g, H.lod, tauinit=29.0, muinit=0.0, velinit=1e-9)
+    O_ = F.FvOracle(H, o, arr)
+    dpf = np.linspace(0.5, 1.5, N)
+
+    class Fixed(F.FvOracle):
+        def rkqs(self, yy, t, htry):
+            return yy.copy(), t + htry, htry, 2 * htry, 0.5, 0
+
+        def diffuse(self, pf_, yy, sigma, h, t, dtnxt):
+            return pf_ + dpf, 3, dtnxt
+
+    S = Fixed(H, o, arr)
+    h = 1e4
+    st = dict(y=y.copy(), pf=pf.copy(), disp=np.zeros(N), t=0.0, h=h)
+    S.step(st)
+    psi, tau, sige0, ks = y[0::4], y[1::4], y[2::4], y[3::4]
+    sige1 = sige0 + pf - (pf + dpf)
+    v0 = O_.vel(psi, tau, sige0)
+    v1 = O_.vel(psi, tau, sige1)
+    assert np.allclose(st["y"][2::4], sige1, rtol=1e-15)
+    assert np.allclose(st["disp"], (v0 * h * 0.5) + v1 * h * 0.5, rtol=1e-14)
+    assert not np.allclose(st["disp"], v0 * h, rtol=1e-6)          # the planted order mistake differs
+    assert np.allclose(st["vel"], v1, rtol=1e-15)
+    p1 = pf + dpf
+    dx = o["ds0"]
+    k = ks / o["eta"] * 1e3
+    assert np.isclose(st["qflow"][0], -k[0] * (p1[1] - p1[0]) / dx, rtol=1e-13)
+    assert np.isclose(st["qflow"][5], -k[5] * (p1[6] - p1[4]) / 2 / dx, rtol=1e-13)
+    assert np.isclose(st["qflow"][-1], -k[-1] * (p1[-1] - p1[-2]) / dx, rtol=1e-13)
+    assert st["h"] == 2 * h and st["t"] == h
