@@ -206,11 +206,14 @@ __global__ __launch_bounds__(kFvThreads) void fv_stage_kernel(FvPar P, const int
 __global__ void fv_decide_kernel(FvPar P, FvCtl *c, const double *__restrict__ errpart, int nblk)
 {
 #pragma clang fp contract(off)
-    if (c->done || threadIdx.x != 0) return;
+    if (c->done) return;
     const double SAFETY = (double)0.9f, PGROW = (double)-0.2f, PSHRNK = (double)-0.25f;
+    // the largest block error over the wave's lanes (a maximum: any order gives the same bits)
     double errmax = 0.0;
-    for (int b = 0; b < nblk; b++)
+    for (int b = threadIdx.x; b < nblk; b += 64)
         if (errpart[b] > errmax) errmax = errpart[b];
+    for (int o = 32; o > 0; o >>= 1) errmax = fmax(errmax, __shfl_xor(errmax, o, 64));
+    if (threadIdx.x != 0) return;
     const double h = c->h;
     c->attempts++;
     c->errmax = errmax;
@@ -1032,17 +1035,46 @@ __global__ __launch_bounds__(kFvThreads) void fv_fields_kernel(FvPar P, const Fv
 __global__ void fv_monitor_kernel(FvPar P, FvCtl *c, FvArrays A, int nblk)
 {
 #pragma clang fp contract(off)
-    if (c->done || !c->acc || threadIdx.x != 0) return;
-    double b[kFvMon];
-    for (int k = 0; k < kFvMon; k++) b[k] = A.monpart[k];
-    for (int q = 1; q < nblk; q++) {
+    if (c->done || !c->acc) return;
+    // max |V| with its first element, max and min sigma_e over the wave's
+    // lanes (exact in any order; ties of max |V| go to the first block), then
+    // the two sums in block order by lane 0 from LDS, kMonChunk blocks at a time
+    const int lane = threadIdx.x;
+    double vmx = -HUGE_VAL, vloc = 0.0, smx = -HUGE_VAL, smn = HUGE_VAL;
+    int vblk = nblk;
+    for (int q = lane; q < nblk; q += 64) {
         const double *mp = A.monpart + kFvMon * (size_t)q;
-        if (mp[0] > b[0]) { b[0] = mp[0]; b[5] = mp[5]; }
-        b[1] = b[1] + mp[1];
-        b[2] = b[2] + mp[2];
-        b[3] = fmax(b[3], mp[3]);
-        b[4] = fmin(b[4], mp[4]);
+        if (mp[0] > vmx) { vmx = mp[0]; vloc = mp[5]; vblk = q; }
+        smx = fmax(smx, mp[3]);
+        smn = fmin(smn, mp[4]);
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(vmx, o, 64), l2 = __shfl_xor(vloc, o, 64);
+        const int b2 = __shfl_xor(vblk, o, 64);
+        if (v2 > vmx || (v2 == vmx && b2 < vblk)) { vmx = v2; vloc = l2; vblk = b2; }
+        smx = fmax(smx, __shfl_xor(smx, o, 64));
+        smn = fmin(smn, __shfl_xor(smn, o, 64));
+    }
+    constexpr int kMonChunk = 512;
+    __shared__ double s1[kMonChunk], s2[kMonChunk];
+    double sum1 = 0.0, sum2 = 0.0;
+    for (int q0 = 0; q0 < nblk; q0 += kMonChunk) {
+        const int m = min(kMonChunk, nblk - q0);
+        __syncthreads();
+        for (int q = lane; q < m; q += 64) {
+            s1[q] = A.monpart[kFvMon * (size_t)(q0 + q) + 1];
+            s2[q] = A.monpart[kFvMon * (size_t)(q0 + q) + 2];
+        }
+        __syncthreads();
+        if (lane == 0)
+            for (int q = 0; q < m; q++) {
+                sum1 = q0 + q == 0 ? s1[q] : sum1 + s1[q];
+                sum2 = q0 + q == 0 ? s2[q] : sum2 + s2[q];
+            }
+    }
+    if (lane != 0) return;
+    double b[kFvMon];
+    b[0] = vmx; b[1] = sum1; b[2] = sum2; b[3] = smx; b[4] = smn; b[5] = vloc;
     const double vm = b[0];
     c->vmax = vm;
     const long long row = c->accepted - c->hist_base - 1;
