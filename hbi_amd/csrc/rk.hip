@@ -222,11 +222,21 @@ __global__ __launch_bounds__(kRkThreads) void rk_stage_kernel(RkPar P, const RkC
 __global__ void rk_decide_kernel(RkPar P, RkCtl *c, const double *__restrict__ errpart, int nblk)
 {
 #pragma clang fp contract(off)
-    if (c->done || threadIdx.x != 0) return;
-    c->out_pending = 0;
+    if (c->done) return;
     const double SAFETY = (double)0.9f, PGROW = (double)-0.2f, PSHRNK = (double)-0.25f, tiny = (double)1e-20f;
+    // the block pairs combined in block order: lane l takes a contiguous run of
+    // blocks, then the lanes in order (err_combine is associative)
+    const int lane = threadIdx.x, per = (nblk + 63) / 64;
     ErrPair e{0, 0.0};
-    for (int b = 0; b < nblk; b++) e = err_combine(e, ErrPair{(int)errpart[2 * b], errpart[2 * b + 1]});
+    for (int b = lane * per; b < min(nblk, (lane + 1) * per); b++)
+        e = err_combine(e, ErrPair{(int)errpart[2 * b], errpart[2 * b + 1]});
+    for (int o = 1; o < 64; o <<= 1) {
+        const int g2 = __shfl_down(e.g, o, 64);
+        const double v2 = __shfl_down(e.v, o, 64);
+        if ((lane & (2 * o - 1)) == 0 && lane + o < 64) e = err_combine(e, ErrPair{g2, v2});
+    }
+    if (lane != 0) return;
+    c->out_pending = 0;
     const double errmax = e.v;
     const double h = c->h;
     c->attempts++;
@@ -369,16 +379,53 @@ __global__ void rk_monitor_kernel(RkPar P, RkCtl *c, RkArrays A, int nblk)
 #pragma clang fp contract(off)
     if (c->done || !c->acc) return;
     __shared__ double vm, msum, mmu, smax, smin;
-    if (threadIdx.x == 0) {
-        double a = -1.0, b = 0.0, u = 0.0, hi = -HUGE_VAL, lo = HUGE_VAL, loc = -1.0;
-        for (int q = 0; q < nblk; q++) {
-            const double *mp = A.monpart + kMon * (size_t)q;
-            if (mp[0] > a) { a = mp[0]; loc = mp[5]; }
-            b = b + mp[1];
-            u = u + mp[2];
-            hi = fmax(hi, mp[3]);
-            lo = fmin(lo, mp[4]);
+    // max |V| with its first element and the sigma extremes across the
+    // workgroup (exact in any order; ties go to the first block), the two
+    // sums in block order by thread 0 from LDS, kMonChunk blocks at a time
+    constexpr int kMonChunk = 512, kMW = 256 / kWave;
+    __shared__ double s1[kMonChunk], s2[kMonChunk], wv[kMW], wl[kMW], wh[kMW], wo[kMW];
+    __shared__ int wb[kMW];
+    const int tid = threadIdx.x, lane = tid % kWave, wv_i = tid / kWave;
+    double pa = -1.0, ploc = -1.0, phi_ = -HUGE_VAL, plo = HUGE_VAL;
+    int pblk = nblk;
+    for (int q = tid; q < nblk; q += blockDim.x) {
+        const double *mp = A.monpart + kMon * (size_t)q;
+        if (mp[0] > pa) { pa = mp[0]; ploc = mp[5]; pblk = q; }
+        phi_ = fmax(phi_, mp[3]);
+        plo = fmin(plo, mp[4]);
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double a2 = __shfl_xor(pa, o, kWave), l2 = __shfl_xor(ploc, o, kWave);
+        const int b2 = __shfl_xor(pblk, o, kWave);
+        if (a2 > pa || (a2 == pa && b2 < pblk)) { pa = a2; ploc = l2; pblk = b2; }
+        phi_ = fmax(phi_, __shfl_xor(phi_, o, kWave));
+        plo = fmin(plo, __shfl_xor(plo, o, kWave));
+    }
+    if (lane == 0) { wv[wv_i] = pa; wl[wv_i] = ploc; wb[wv_i] = pblk; wh[wv_i] = phi_; wo[wv_i] = plo; }
+    double bsum = 0.0, usum = 0.0;
+    for (int q0 = 0; q0 < nblk; q0 += kMonChunk) {
+        const int m = min(kMonChunk, nblk - q0);
+        __syncthreads();
+        for (int q = tid; q < m; q += blockDim.x) {
+            s1[q] = A.monpart[kMon * (size_t)(q0 + q) + 1];
+            s2[q] = A.monpart[kMon * (size_t)(q0 + q) + 2];
         }
+        __syncthreads();
+        if (tid == 0)
+            for (int q = 0; q < m; q++) {
+                bsum = bsum + s1[q];
+                usum = usum + s2[q];
+            }
+    }
+    if (threadIdx.x == 0) {
+        double a = wv[0], loc = wl[0], hi = wh[0], lo = wo[0];
+        int ab = wb[0];
+        for (int w = 1; w < kMW; w++) {
+            if (wv[w] > a || (wv[w] == a && wb[w] < ab)) { a = wv[w]; loc = wl[w]; ab = wb[w]; }
+            hi = fmax(hi, wh[w]);
+            lo = fmin(lo, wo[w]);
+        }
+        const double b = bsum, u = usum;
         vm = a;
         msum = b / P.sg;
         mmu = u / P.sg;
