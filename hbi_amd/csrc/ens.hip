@@ -299,9 +299,7 @@ __global__ __launch_bounds__(kDecThreads) void ens_decide_kernel(EnsPar P, EnsCt
 }
 
 // Each wave commits kCommitRun consecutive elements for its 64 realisations
-// (lane = realisation); the slip-rate histogram of the learned step size is
-// filled with run-length aggregated atomics (neighbouring elements mostly
-// share a bin).
+// (lane = realisation).
 constexpr int kCommitRun = 4, kCommitPerBlock = 4 * kCommitRun;
 
 __global__ __launch_bounds__(256) void ens_commit_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A)
@@ -346,7 +344,9 @@ __global__ __launch_bounds__(256) void ens_commit_kernel(EnsPar P, const EnsCtl 
 // of one realisation, counted as runs of equal bins (most elements of a fault
 // share a few bins) -- then adds its nonzero bins to the global histograms (at
 // most kHistChunks adds per address)
-constexpr int kHistR = 4, kHistChunks = 16;
+// 64 chunks: 1024 workgroups (4 waves per SIMD), each thread at most 8
+// elements of one load batch for the 32768-element fault
+constexpr int kHistR = 4, kHistChunks = 64;
 
 __global__ __launch_bounds__(256) void ens_hist_kernel(EnsPar P, const EnsCtl *__restrict__ c, EnsArrays A)
 {
