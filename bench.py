@@ -359,7 +359,7 @@ def controllers_by_phase(H, Hm, frs, lod, ys, mlp, warmup, steps):
     mean = {k: float(np.mean([r[k]["steps_per_s"] for r in rows])) for k in ("rkqs", "learned")}
     return {"phases": rows, "mean_steps_per_s": mean,
             "note": "both controllers from the same spun-up states (rkqs spin-up from t = 0); the headline window "
-                    "starts at t = 0, where rkqs grows h by 2x per step without rejections"}
+                    "starts at t = 0 (the ensemble and ensemble_rkqs entries)"}
 
 
 def ensemble_accuracy(H, Hm, frs, lod, ys, g, steps=1500):
